@@ -1,0 +1,48 @@
+/* gfsph_oracle.h -- interface of the serial FP64 CPU restatement (oracle).
+ * TEST INFRASTRUCTURE ONLY; see gfsph_oracle.c. */
+#ifndef GFSPH_ORACLE_H
+#define GFSPH_ORACLE_H
+#include <stdint.h>
+
+typedef struct {
+  int32_t integrator;         /* 4 = RK4 (RK45, 7:746); 2 = RK2 Heun (extension) */
+  int32_t pa_sph, skf, sle;   /* 5:85-102 */
+  int32_t summation_density;  /* must be 1 */
+  int32_t average_velocity;   /* XSPH, 7:987 */
+  int32_t nor_density;        /* must be 0 */
+  int32_t icpwp;              /* 0, 1 or 3 (6:1418) */
+  int32_t npoin0;             /* particles [0,npoin0) skip XSPH */
+  int32_t pad_;
+  double dt;                  /* dt_sph */
+  double h_inf;               /* h_inf_SW */
+  double pwp0rel;             /* initial pwp ratio (6:1811-1852) */
+  double cst[111];            /* const(1..110); index 0 unused; REAL(4) values */
+} gfo_params;
+
+typedef struct {
+  int32_t nx, ny;             /* npoigx, npoigy */
+  int32_t xmax_set, pad_;     /* 1: use xmax/ymax below (Xmaxg/Ymaxg as read) */
+  double xmin, ymin, dx, dy;  /* Xming, Yming, deltxg, deltyg */
+  double xmax, ymax;
+  const double *topol;        /* [nx*ny][9]: Z Zx Zy top4..top9 */
+  const int32_t *in_model;    /* [nx*ny] or NULL (all 1) */
+  const double *coorg;        /* [nx*ny][2] or NULL (regular grid) */
+} gfo_dem;
+
+void gfo_kernel(int skf, double r, double dx, double dy, double hsml, double *w, double *dwx,
+                double *dwy);
+int64_t gfo_pairs(int n, int skf, const double *x, const double *hsml, const int32_t *flag,
+                  int brute, int64_t cap, int32_t *ij, double *w, double *dwdx);
+void gfo_topo_lookup(const gfo_dem *d, int n, const double *xy, double *out9);
+void gfo_fdiff_derivs(int nx, int ny, double dxg, double dyg, const int32_t *in_model,
+                      double *topol);
+int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, double *h,
+             const double *mass, const double *hsml, double *u, double *uaux, int32_t *flag);
+int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,
+            const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
+            int nsteps);
+void gfo_raster_idw(const gfo_dem *d, int n, const double *x, const double *v, const double *rho,
+                    const double *u, const int32_t *flag, double *out, double *wsum);
+void gfo_mc_pixel_accum(const gfo_dem *d, int n, const double *x, const double *v,
+                        const double *rho, const int32_t *flag, double *acc, int32_t *cnt);
+#endif

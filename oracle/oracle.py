@@ -1,0 +1,176 @@
+"""ctypes binding of the oracle library (liboracle_gfsph.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker / CPU baseline.  The product path
+(the HIP library) never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_gfsph.so")
+
+
+class GfoParams(C.Structure):
+    _fields_ = [("integrator", C.c_int32), ("pa_sph", C.c_int32), ("skf", C.c_int32), ("sle", C.c_int32),
+                ("summation_density", C.c_int32), ("average_velocity", C.c_int32), ("nor_density", C.c_int32),
+                ("icpwp", C.c_int32), ("npoin0", C.c_int32), ("pad_", C.c_int32),
+                ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double), ("cst", C.c_double * 111)]
+
+
+class GfoDem(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("xmax_set", C.c_int32), ("pad_", C.c_int32),
+                ("xmin", C.c_double), ("ymin", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("xmax", C.c_double), ("ymax", C.c_double),
+                ("topol", C.POINTER(C.c_double)), ("in_model", C.POINTER(C.c_int32)),
+                ("coorg", C.POINTER(C.c_double))]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle_gfsph.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.gfo_kernel.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]
+        L.gfo_pairs.argtypes = [C.c_int, C.c_int, dp, dp, ip, C.c_int, C.c_int64, ip, dp, dp]
+        L.gfo_pairs.restype = C.c_int64
+        L.gfo_topo_lookup.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp]
+        L.gfo_fdiff_derivs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp]
+        L.gfo_init.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, ip]
+        L.gfo_run.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, dp, ip,
+                              C.c_int]
+        L.gfo_raster_idw.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, dp, dp]
+        L.gfo_mc_pixel_accum.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, dp, ip]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def make_params(prm) -> GfoParams:
+    p = GfoParams()
+    p.integrator, p.pa_sph, p.skf, p.sle = prm.integrator, prm.pa_sph, prm.skf, prm.sle
+    p.summation_density, p.average_velocity, p.nor_density = prm.summation_density, prm.average_velocity, prm.nor_density
+    p.icpwp, p.npoin0 = prm.icpwp, prm.npoin0
+    p.dt, p.h_inf, p.pwp0rel = prm.dt, prm.h_inf, prm.pwp0rel
+    c = prm.cst()
+    for k in range(111):
+        p.cst[k] = c[k]
+    return p
+
+
+class DemHandle:
+    """Keeps the numpy buffers alive while the C struct points at them."""
+
+    def __init__(self, dem, topol=None):
+        self.topol = np.ascontiguousarray(dem.topol() if topol is None else topol, dtype=np.float64)
+        self.in_model = None if dem.in_model is None else np.ascontiguousarray(dem.in_model, dtype=np.int32)
+        d = GfoDem()
+        d.nx, d.ny, d.xmax_set = dem.nx, dem.ny, 1
+        d.xmin, d.ymin, d.dx, d.dy = dem.xmin, dem.ymin, dem.dx, dem.dy
+        d.xmax, d.ymax = dem.xmax, dem.ymax
+        d.topol = _dp(self.topol)
+        d.in_model = _ip(self.in_model) if self.in_model is not None else C.POINTER(C.c_int32)()
+        d.coorg = C.POINTER(C.c_double)()
+        self.s = d
+
+
+def kernel(skf, r, dx, dy, h):
+    w, a, b = C.c_double(), C.c_double(), C.c_double()
+    lib().gfo_kernel(skf, r, dx, dy, h, C.byref(w), C.byref(a), C.byref(b))
+    return w.value, a.value, b.value
+
+
+def pairs(x, hsml, flag=None, skf=1, brute=False):
+    """Neighbour set: returns (ij[np,2] int32 with i<j, w[np], dwdx[np,2])."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    hsml = np.ascontiguousarray(hsml, dtype=np.float64)
+    n = x.shape[0]
+    flag = np.zeros(n, np.int32) if flag is None else np.ascontiguousarray(flag, dtype=np.int32)
+    L = lib()
+    npairs = L.gfo_pairs(n, skf, _dp(x), _dp(hsml), _ip(flag), int(brute), 0, None, None, None)
+    ij = np.zeros((max(npairs, 1), 2), np.int32)
+    w = np.zeros(max(npairs, 1))
+    dw = np.zeros((max(npairs, 1), 2))
+    L.gfo_pairs(n, skf, _dp(x), _dp(hsml), _ip(flag), int(brute), npairs, _ip(ij), _dp(w), _dp(dw))
+    return ij[:npairs], w[:npairs], dw[:npairs]
+
+
+def topo_lookup(dem, xy):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    out = np.zeros((xy.shape[0], 9))
+    h = DemHandle(dem)
+    lib().gfo_topo_lookup(C.byref(h.s), xy.shape[0], _dp(xy), _dp(out))
+    return out
+
+
+class State:
+    def __init__(self, parts, n_uaux=True):
+        self.x = np.ascontiguousarray(parts.x, dtype=np.float64).copy()
+        self.v = np.ascontiguousarray(parts.v, dtype=np.float64).copy()
+        self.h = np.ascontiguousarray(parts.h, dtype=np.float64).copy()
+        self.mass = np.ascontiguousarray(parts.mass, dtype=np.float64).copy()
+        self.hsml = np.ascontiguousarray(parts.hsml, dtype=np.float64).copy()
+        self.u = np.ascontiguousarray(parts.u, dtype=np.float64).copy()
+        self.uaux = np.zeros_like(self.h)
+        self.flag = np.zeros(self.h.shape[0], np.int32)
+
+
+def init(case, state=None):
+    """Input_SW initialisation (re-summed h, pwp) -> State."""
+    st = State(case.particles) if state is None else state
+    p = make_params(case.params)
+    dh = DemHandle(case.dem)
+    lib().gfo_init(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.h), _dp(st.mass), _dp(st.hsml),
+                   _dp(st.u), _dp(st.uaux), _ip(st.flag))
+    return st
+
+
+def run(case, st, nsteps):
+    p = make_params(case.params)
+    dh = DemHandle(case.dem)
+    rc = lib().gfo_run(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _dp(st.mass),
+                       _dp(st.hsml), _dp(st.u), _dp(st.uaux), _ip(st.flag), nsteps)
+    if rc != 0:
+        raise RuntimeError(f"gfo_run failed: {rc}")
+    return st
+
+
+def raster_idw(dem, st):
+    np_ = dem.nx * dem.ny
+    out = np.zeros((np_, 4))
+    ws = np.zeros(np_)
+    dh = DemHandle(dem)
+    lib().gfo_raster_idw(C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _dp(st.u), _ip(st.flag),
+                         _dp(out), _dp(ws))
+    return out, ws
+
+
+def mc_pixel_accum(dem, st, acc=None, cnt=None):
+    np_ = dem.nx * dem.ny
+    acc = np.zeros((np_, 4)) if acc is None else acc
+    cnt = np.zeros(np_, np.int32) if cnt is None else cnt
+    dh = DemHandle(dem)
+    lib().gfo_mc_pixel_accum(C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _ip(st.flag),
+                             _dp(acc), _ip(cnt))
+    return acc, cnt

@@ -1,0 +1,165 @@
+!  Test-infrastructure driver for the reference Geoflow-SPH modules.
+!
+!  It replaces PROGRAM SPH_FEM_Driver_2019 ("8 SPH_FEM_Driver_2019.f90":6)
+!  for one plain SPH run (ic_cases_mat=0): it sets the driver-level variables
+!  that Driver_Input (8:165-470) would read from <name>.master.dat, calls
+!  the reference's own Init_sph (7:74) -- which reads <name>.dat, the .top
+!  named on stdin and restart.dat exactly as the reference does -- and then
+!  repeats the Driver_Flow step (8:1580-1620): adaptive_dt_sph,
+!  itimestep_sph += 1, time_integration_sph, time_sph += dt_sph.
+!
+!  Control file harness.ctl (cwd):
+!     line 1: problem name (the .dat stem)
+!     line 2: nsteps  dt_sph
+!     line 3: ndump   followed by ndump step numbers (0 = after Init_sph)
+!
+!  At each listed step it writes dump_<step>.bin (stream, native endian):
+!     int32  magic(=1195725648) version(=1) npoin ndimn has_uaux npairs
+!     real64 time_sph
+!     real64 x(ndimn,npoin) vx(ndimn,npoin) rho mass hsml u  [uaux1]
+!     int32  If_Out_Domain(npoin)
+!     real64 x0(ndimn,npoin)        positions the pair list was built from
+!     int32  pair_i(npairs) pair_j(npairs)
+!     real64 w(npairs) dwdx(ndimn,npairs)
+!  It is built only by oracle/build_ref.sh and never ships with the product.
+
+program gfsph_ref_harness
+
+use SPH_FEM_variable_types_2019
+use SPH_FEM_Driver_time_vars_2019
+use SPH_MAIN_2019, only: Init_SPH, time_integration_sph, adaptive_dt_sph
+use SPH_Main_Vars_2019
+use SPH_SW_Vars_2019
+
+implicit none
+
+character(16)  name
+integer        nsteps, ndump, istep, k, ictl
+integer, allocatable :: dump_steps(:)
+real(8)        dtin
+
+open (newunit=ictl, file='harness.ctl', status='old')
+read (ictl,'(a16)') name
+read (ictl,*) nsteps, dtin
+read (ictl,*) ndump
+allocate (dump_steps(max(ndump,1)))
+if (ndump.gt.0) then
+   backspace (ictl)
+   read (ictl,*) ndump, (dump_steps(k), k=1,ndump)
+endif
+close (ictl)
+
+!      ------  what Driver_Input would have read (8:229-470)
+
+problem_name_Global = name
+SPH_SW_problem_name = name
+if_Read_TopoDat  = 0
+if_sph = 1; if_gfl = 0; if_tgf = 0
+ic_cases_mat     = 0
+SPH_ProblemType  = 1
+SPH_t_Integ_Alg  = 4
+dt               = dtin
+initial_dt       = dtin
+time_end         = 1.e30
+maxtimestep      = nsteps
+print_step = 1000000000; save_step = 1000000000; plot_step = 1000000000
+nsearch_step     = 1
+isearch_step     = 0
+dt_sph           = dtin
+initial_dt_sph   = dtin
+ic_adapt_dt_sph  = 0
+dt_gfl           = -999.
+ntcurves = 0; mptstcurves = 0
+time_print = print_step*dt; time_save = save_step*dt; time_plot = plot_step*dt
+t_print_reset = 0.; t_save_reset = 0.; t_plot_reset = 0.
+dt_limit_inf     = 1.e-7*dt
+ic_Trigger = 0; ic_Prob_Propag = 0; ic_pixel_Hazard = 0
+ic_MCarlo = 0; ic_FOSM = 0; N_Gzones = 1; npixel_HVars = 10
+ic_cases_win     = 0
+nproblems        = 1
+nproblems_index  = 1
+time_sph_restart = 0.0
+
+call Init_sph
+
+!      ------  Driver_Flow (8:1565-1620) for one SPH substep per step
+
+time = 0.0; time_sph = 0.0; itimestep_sph = 0
+if (time_sph_restart.gt.0.0) then
+   time     = time_sph_restart
+   time_sph = time_sph_restart
+endif
+
+call maybe_dump (0)
+do istep = 1, nsteps
+   t_print_reset = t_print_reset + dt
+   call adaptive_dt_sph
+   itimestep_sph = itimestep_sph + 1
+   call time_integration_sph
+   time_sph = time_sph + dt_sph
+   time     = time + dt
+   call maybe_dump (istep)
+enddo
+
+contains
+
+subroutine maybe_dump (is)
+   integer, intent(in) :: is
+   integer k2
+   do k2 = 1, ndump
+      if (dump_steps(k2).eq.is) call write_dump (is)
+   enddo
+end subroutine maybe_dump
+
+subroutine write_dump (is)
+   integer, intent(in) :: is
+   integer iu, np, has_uaux, ipair, ip
+   character(64) fname
+   type(pairs), pointer :: node
+   integer(4), allocatable :: pi_(:), pj_(:)
+   real(8),    allocatable :: pw(:), pdw(:,:)
+
+   np = 0
+   node => last
+   do while (associated(node))
+      np = np + 1
+      node => node%next
+   enddo
+   allocate (pi_(max(np,1)), pj_(max(np,1)), pw(max(np,1)), pdw(ndimn,max(np,1)))
+   ipair = 0
+   node => last
+   do while (associated(node))
+      ipair = ipair + 1
+      pi_(ipair) = node%pair_i
+      pj_(ipair) = node%pair_j
+      pw(ipair)  = node%w
+      pdw(:,ipair) = node%dwdx(:)
+      node => node%next
+   enddo
+
+   has_uaux = 0
+   if (allocated(Uaux)) has_uaux = 1
+   write (fname,'(a,i0,a)') 'dump_', is, '.bin'
+   open (newunit=iu, file=trim(fname), access='stream', form='unformatted', status='replace')
+   write (iu) int(1195725648,4), int(1,4), int(npoin,4), int(ndimn,4), int(has_uaux,4), int(np,4)
+   write (iu) real(time_sph,8)
+   write (iu) (real(x(:,ip),8), ip=1,npoin)
+   write (iu) (real(vx(:,ip),8), ip=1,npoin)
+   write (iu) (real(rho(ip),8), ip=1,npoin)
+   write (iu) (real(mass(ip),8), ip=1,npoin)
+   write (iu) (real(hsml(ip),8), ip=1,npoin)
+   write (iu) (real(u(ip),8), ip=1,npoin)
+   if (has_uaux.eq.1) write (iu) (real(Uaux(1,ip),8), ip=1,npoin)
+   write (iu) (int(If_Out_Domain(ip),4), ip=1,npoin)
+   write (iu) (real(x0(:,ip),8), ip=1,npoin)
+   if (np.gt.0) then
+      write (iu) pi_(1:np)
+      write (iu) pj_(1:np)
+      write (iu) pw(1:np)
+      write (iu) pdw(:,1:np)
+   endif
+   close (iu)
+   deallocate (pi_, pj_, pw, pdw)
+end subroutine write_dump
+
+end program gfsph_ref_harness

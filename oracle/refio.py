@@ -1,0 +1,151 @@
+"""Drive the reference Fortran build (oracle/_ref/gfsph_ref) on a case.
+
+Writes the reference's own positional ASCII inputs -- <name>.dat
+("6 SPH_SW_2019.f90":812-1570), an explicit-grid <topo>.top with ictop=100
+("2 SPH_FEM_topo_2019.f90":208-252, finite-difference slopes) and an exact
+restart.dat particle set (icunk_s=200, "6 SPH_SW_2019.f90":3270-3340, 17
+significant digits) -- runs the harness and reads its binary dumps.
+
+TEST INFRASTRUCTURE ONLY (golden-fixture generation and the optional
+reference CPU baseline).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_EXE = os.path.join(HERE, "_ref", "gfsph_ref")
+
+
+def available() -> bool:
+    return os.path.exists(REF_EXE)
+
+
+def _r(v: float) -> str:
+    return repr(float(v))
+
+
+def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
+    prm, dem, P = case.params, case.dem, case.particles
+    c = prm.cst()
+    if any(c[k] != 0.0 for k in range(16, 111)):
+        raise ValueError("only const(1:15) are written by this helper")
+    L = []
+    L += ["1", "synthetic case written for the reference oracle"]
+    L += ["ic_SW_Alg", "0"]
+    L += ["nhist", "0"]
+    L += ["ndimn", "2"]
+    L += ["ic_soil ic_water ic_vps ic_abs", "1 0 0 0"]
+    L += ["Soil unkno  h_inf_SW", f"200 {_r(prm.h_inf)}"]
+    L += ["restart name", name]
+    L += ["pa_sph nnps sle skf", f"{prm.pa_sph} {prm.nnps} {prm.sle} {prm.skf}"]
+    tf = lambda b: "T" if b else "F"
+    L += ["sum_den av_vel virt_part nor_dens",
+          f"{tf(prm.summation_density)} {tf(prm.average_velocity)} F {tf(prm.nor_density)}"]
+    L += ["const 1..15", " ".join(_r(c[k]) for k in range(1, 16))]
+    nfric = int(c[5] + 0.01)
+    if (5 <= nfric <= 10) or nfric in (26, 27):
+        L += ["K0 activated?", "0"]
+    L += ["icpwp", str(prm.icpwp)]
+    if prm.icpwp in (1, 3):
+        L += ["pwp0rel", _r(prm.pwp0rel)]
+    elif prm.icpwp != 0:
+        raise ValueError("icpwp must be 0, 1 or 3")
+    L += ["coarse mesh saving utility?", "0"]
+    L += ["control points?", "0"]
+    L += ["GID filter", "1 1 1 1 0 1 0 0 0 1 0 0"]
+    L += ["T_change_to_W", "1.e+12"]
+    with open(os.path.join(d, name + ".dat"), "w") as f:
+        f.write("\n".join(L) + "\n")
+
+    nx, ny = dem.nx, dem.ny
+    xs = dem.xmin + np.arange(nx) * dem.dx
+    ys = dem.ymin + np.arange(ny) * dem.dy
+    Z = np.asarray(dem.Z, dtype=np.float64)
+    with open(os.path.join(d, topo + ".top"), "w") as f:
+        f.write("explicit grid, finite-difference slopes\n100\nnpoigx npoigy\n")
+        f.write(f"{nx} {ny}\nipoig x y z\n")
+        k = 1
+        rows = []
+        for iy in range(ny):
+            for ix in range(nx):
+                rows.append(f"{k} {_r(xs[ix])} {_r(ys[iy])} {_r(Z[iy, ix])}")
+                k += 1
+        f.write("\n".join(rows) + "\n")
+        f.write("topo props (zones) 0 = none\n0\n")
+
+    with open(os.path.join(d, "restart.dat"), "w") as f:
+        f.write("restart written for the reference oracle\n")
+        f.write(f"{P.n} 0.0\n")
+        f.write("i itype x y vx vy h mass hsml u\n")
+        rows = []
+        for i in range(P.n):
+            rows.append(" ".join([str(i + 1), str(int(P.itype[i])), _r(P.x[i, 0]), _r(P.x[i, 1]), _r(P.v[i, 0]),
+                                  _r(P.v[i, 1]), _r(P.h[i]), _r(P.mass[i]), _r(P.hsml[i]), _r(P.u[i])]))
+        f.write("\n".join(rows) + "\n")
+
+
+def read_dump(path: str) -> dict:
+    raw = open(path, "rb").read()
+    hdr = np.frombuffer(raw, dtype=np.int32, count=6)
+    magic, ver, n, ndimn, has_uaux, npairs = (int(v) for v in hdr)
+    assert magic == 1195725648 and ver == 1 and ndimn == 2
+    off = 24
+    out = {}
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=off)
+        off += a.nbytes
+        return a.copy()
+
+    out["time"] = float(take(np.float64, 1)[0])
+    out["x"] = take(np.float64, 2 * n).reshape(n, 2)
+    out["v"] = take(np.float64, 2 * n).reshape(n, 2)
+    out["h"] = take(np.float64, n)
+    out["mass"] = take(np.float64, n)
+    out["hsml"] = take(np.float64, n)
+    out["u"] = take(np.float64, n)
+    out["uaux"] = take(np.float64, n) if has_uaux else None
+    out["flag"] = take(np.int32, n)
+    out["x0"] = take(np.float64, 2 * n).reshape(n, 2)
+    if npairs > 0:
+        pi = take(np.int32, npairs)
+        pj = take(np.int32, npairs)
+        out["pair_w"] = take(np.float64, npairs)
+        out["pair_dwdx"] = take(np.float64, 2 * npairs).reshape(npairs, 2)
+        out["pairs"] = np.stack([pi - 1, pj - 1], axis=1)
+    else:
+        out["pairs"] = np.zeros((0, 2), np.int32)
+        out["pair_w"] = np.zeros(0)
+        out["pair_dwdx"] = np.zeros((0, 2))
+    return out
+
+
+def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, timeout: float = 3600.0,
+                  keep: bool = False) -> dict:
+    """Run the reference on ``case`` for ``nsteps`` RK4 steps; return
+    {step: dump}.  Raises if the reference build is absent."""
+    if not available():
+        raise FileNotFoundError(REF_EXE)
+    if case.params.integrator != 4:
+        raise ValueError("the reference integrates with RK4 only")
+    tmp = workdir or tempfile.mkdtemp(prefix="gfsph_ref_")
+    write_inputs(tmp, case)
+    steps = sorted(set(int(s) for s in dump_steps))
+    with open(os.path.join(tmp, "harness.ctl"), "w") as f:
+        f.write("case\n")
+        f.write(f"{nsteps} {_r(case.params.dt)}\n")
+        f.write(f"{len(steps)} " + " ".join(str(s) for s in steps) + "\n")
+    r = subprocess.run([REF_EXE], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
+                       timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    out = {s: read_dump(os.path.join(tmp, f"dump_{s}.bin")) for s in steps}
+    if not keep and workdir is None:
+        subprocess.run(["rm", "-rf", tmp])
+    return out
