@@ -1,0 +1,1990 @@
+// gfsph.hip -- MI355X (gfx950) implementation of the Geoflow-SPH depth
+// integrated SPH propagation step behind the C ABI of include/gfsph.h.
+//
+// One step (frozen pair geometry, "7 SPH_MAIN_2019.f90":784-799):
+//   k_setup_clear  grid of the active bounding box (cell = scale_k*max hsml)
+//   k_keys         cell key per particle + atomic count per cell
+//   k_scan_*       exclusive scan of the cell counts -> cell starts
+//   k_scatter      counting-sort scatter
+//   k_cellsort     order each cell by previous slot (deterministic order)
+//   k_reorder      gather the particle state into cell order
+//   k_density      summed depth h_i = W(0)m_i + sum m_j w_ij  (5a:1763)
+//   k_force_rk     pressure-force gather (6:6669) fused with every RK stage:
+//                  DEM bilinear lookup (2:2395), gravity, basal rheology and
+//                  limiter (6:9355-9901), stage update (7:808-922), pore
+//                  pressure (6:11176-11310), domain flag (7:1764), h update
+//                  (7:2866), next-step bounding box
+//   k_xsph         XSPH velocity correction (7:3042-3099, 7:987-993)
+// All arithmetic is FP64.  The file is compiled with -ffp-contract=off so
+// expressions round as the reference's FP64 build writes them.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gfsph.h"
+
+#define GF_BLOCK 256
+
+// ---------------------------------------------------------------------------
+// Device-side parameter blocks
+// ---------------------------------------------------------------------------
+
+struct DevParams {
+  int nfric, if_frictional, if_V2byR, icpwp, nuaux, pa_sph, skf, scale_k, sle, av_vel, npoin0;
+  int nstages;
+  double dt, h_inf, cgra, cgra05, dens, xi_voellmy, constMu_CF, K_Bagnold;
+  double tauy0, constK, visco, tanfi8, hfrict0, Bfact, comp, alphaS, hrelpw2, c73;
+  double fact_dens, fact_pwp_max, fact_pwp_min, fpwp_max, fpwp_min;
+  double c3, c6, c7, c8, c47;
+  double f1[4], f2[4], fin;
+  double pi;
+};
+
+struct DevDem {
+  int nx, ny;
+  double xmin, ymin, dx, dy, xmax, ymax;
+  const double2 *zxy;  // [npoig] (Zx, Zy)
+  const double *top;   // [npoig][6] top4..top9, or null
+};
+
+struct GridInfo {
+  double xmin, ymin, cell;
+  int ncx, ncy, ncell, pad;
+};
+
+// bounding-box accumulator of active particles (ordered-integer encoding)
+struct BBoxAcc {
+  unsigned long long xmin, xmax, ymin, ymax, hmax;
+  unsigned int nactive, pad;
+};
+
+// particle state in cell order
+struct SortedSet {
+  double2 *pos, *vel;
+  double *h, *mass, *hsml, *u, *uaux;
+  int *flag, *id;
+};
+
+// step outputs in the same (cell) order as the sorted set that produced them
+struct NewSet {
+  double2 *pos, *vel;
+  double *h, *hsml, *u, *uaux;
+  int *flag;
+};
+
+// The state after a step: positions, depths, smoothing lengths, pore
+// pressures and flags are in the step's output set; velocities (after XSPH),
+// masses and caller ids are in the sorted set, both in the same slot order.
+struct CurView {
+  const double2 *pos, *vel;
+  const double *h, *mass, *hsml, *u, *uaux;
+  const int *flag, *id;
+};
+
+__device__ __forceinline__ unsigned long long ord_of(double v) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double val_of(unsigned long long o) {
+  unsigned long long b = (o & 0x8000000000000000ull) ? (o & 0x7fffffffffffffffull) : ~o;
+  double d;
+  memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// a2. kernel_TOOLS ("5a My_SPH_Tools_2019.f90":2062-2180), ndimn = 2
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void sph_kernel(int skf, double pi, double r, double dx, double dy,
+                                           double hsml, double &w, double &gx, double &gy) {
+  double q = r / hsml, factor, a;
+  w = 0.0;
+  gx = 0.0;
+  gy = 0.0;
+  if (skf == 1) {
+    factor = 15.e0 / (7.e0 * pi * hsml * hsml);
+    if (q >= 0 && q <= 1.e0) {
+      w = factor * (2. / 3. - q * q + q * q * q / 2.);
+      a = factor * (-2. + 3. / 2. * q) / (hsml * hsml);
+      gx = a * dx;
+      gy = a * dy;
+    } else if (q > 1.e0 && q <= 2) {
+      double t = 2. - q;
+      w = factor * 1.e0 / 6.e0 * (t * t * t);
+      a = -(factor * 1.e0 / 6.e0 * 3. * (t * t) / hsml);
+      gx = a * (dx / r);
+      gy = a * (dy / r);
+    }
+  } else if (skf == 2) {
+    factor = 1.e0 / (hsml * hsml * pi);
+    if (q >= 0 && q <= 3) {
+      w = factor * exp(-q * q);
+      gx = w * (-2. * dx / hsml / hsml);
+      gy = w * (-2. * dy / hsml / hsml);
+    }
+  } else if (skf == 3) {
+    factor = 7.e0 / (478.e0 * pi * hsml * hsml);
+    double t3 = 3 - q, t2 = 2 - q, t1 = 1 - q;
+    double t3_4 = t3 * t3 * t3 * t3, t2_4 = t2 * t2 * t2 * t2;
+    if (q >= 0 && q <= 1) {
+      w = factor * (t3_4 * t3 - 6 * t2_4 * t2 + 15 * (t1 * t1 * t1 * t1 * t1));
+      a = factor * ((-120 + 120 * q - 50 * q * q) / (hsml * hsml));
+      gx = a * dx;
+      gy = a * dy;
+    } else if (q > 1 && q <= 2) {
+      w = factor * (t3_4 * t3 - 6 * t2_4 * t2);
+      a = factor * (-5 * t3_4 + 30 * t2_4) / hsml;
+      gx = a * (dx / r);
+      gy = a * (dy / r);
+    } else if (q > 2 && q <= 3) {
+      w = factor * (t3_4 * t3);
+      a = factor * (-5 * t3_4) / hsml;
+      gx = a * (dx / r);
+      gy = a * (dy / r);
+    }
+  } else {  // skf == 4, Wendland
+    factor = 7.e0 / (4.e0 * pi * hsml * hsml);
+    if (q >= 0 && q <= 2) {
+      double b = 1 - q / 2;
+      w = factor * ((b * b * b * b) * (2 * q + 1));
+      a = factor * (-5 * (b * b * b)) * ((1 / hsml / hsml));
+      gx = a * dx;
+      gy = a * dy;
+    }
+  }
+}
+
+// kernel value at r = 0 (self contribution in sum_density_Tools, 5a:1818-1822)
+__device__ __forceinline__ double sph_self_w(int skf, double pi, double hsml) {
+  double w, a, b;
+  sph_kernel(skf, pi, 0.0, 0.0, 0.0, hsml, w, a, b);
+  return w;
+}
+
+// a1. pair predicate of grid_find_NEW_tools3 (5a:1010-1019), rounded exactly
+// as the reference writes it: dr = dx*dx + dy*dy without contraction.
+__device__ __forceinline__ bool pair_pred(int scale_k, double2 pi_, double2 pj, double hi, double hj,
+                                          double &dx, double &dy, double &r, double &mh) {
+  dx = __dsub_rn(pi_.x, pj.x);
+  dy = __dsub_rn(pi_.y, pj.y);
+  double dr = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+  r = sqrt(dr);
+  mh = __dadd_rn(hi, hj) / 2.;
+  return r < scale_k * mh;
+}
+
+// ---------------------------------------------------------------------------
+// a7. Get_Z_Topo ("2 SPH_FEM_topo_2019.f90":2395-2491)
+// ---------------------------------------------------------------------------
+struct TopoPt {
+  double Zx, Zy, t4, t5, t6, t7, t8, t9;
+};
+
+__device__ __forceinline__ void get_z_topo(const DevDem &d, double xx, double yy, bool want_top,
+                                           TopoPt &o) {
+  int ipx = (int)((xx - d.xmin) / d.dx + 1);
+  int ipy = (int)((yy - d.ymin) / d.dy + 1);
+  if (ipx == d.nx && fabs(d.xmax - xx) <= 1.e-3) ipx = ipx - 1;
+  if (ipy == d.ny && fabs(d.ymax - yy) <= 1.e-3) ipy = ipy - 1;
+  // in-bounds guard: the reference only warns here (2:2421-2435)
+  ipx = ipx < 1 ? 1 : (ipx > d.nx - 1 ? d.nx - 1 : ipx);
+  ipy = ipy < 1 ? 1 : (ipy > d.ny - 1 ? d.ny - 1 : ipy);
+  long n1 = (long)(ipy - 1) * d.nx + (ipx - 1);
+  long n2 = n1 + 1, n4 = n1 + d.nx, n3 = n4 + 1;
+  double xref = d.xmin + (ipx - 1) * d.dx;
+  double yref = d.ymin + (ipy - 1) * d.dy;
+  double xi = (xx - xref) / d.dx, eta = (yy - yref) / d.dy;
+  double sh1 = (1. - xi) * (1. - eta), sh2 = xi * (1. - eta), sh3 = xi * eta, sh4 = (1. - xi) * eta;
+  double2 a = d.zxy[n1], b = d.zxy[n2], c = d.zxy[n3], e = d.zxy[n4];
+  o.Zx = sh1 * a.x + sh2 * b.x + sh3 * c.x + sh4 * e.x;
+  o.Zy = sh1 * a.y + sh2 * b.y + sh3 * c.y + sh4 * e.y;
+  if (want_top) {
+    const double *T1 = d.top + 6 * n1, *T2 = d.top + 6 * n2, *T3 = d.top + 6 * n3, *T4 = d.top + 6 * n4;
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = sh1 * T1[k] + sh2 * T2[k] + sh3 * T3[k] + sh4 * T4[k];
+    o.t4 = v[0];
+    o.t5 = v[1];
+    o.t6 = v[2];
+    o.t7 = v[3];
+    o.t8 = v[4];
+    o.t9 = v[5];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a6 + a8. ExtForces_SW(0) body for one particle (6:8532-8657) with
+// Get_Grav_Ext_SW (6:9241-9349) and basal_friction_Ext_SW (6:9355-9901).
+// ---------------------------------------------------------------------------
+// Bingham intermediate regime (6:9609-9618): returns true and Tau0 when
+// 0.01 <= a_hat <= 50.
+__device__ __forceinline__ bool bingham_tau0(const DevParams &P, double hvmod, double hfrict, double &Tau0) {
+  double a_hat = (6. * P.constK * hvmod) / (P.tauy0 * pow(hfrict, 2.));
+  if (a_hat > 50 || a_hat < 0.01) return false;
+  double a2 = 48., b2 = -32. * a_hat - 114., c2 = 65.;
+  double xh = (-b2 - pow(b2 * b2 - 4. * a2 * c2, 0.5)) / (2. * a2);
+  Tau0 = P.tauy0 / xh;
+  return true;
+}
+
+// candidate Tau0 of one particle for the last-valid scan (NaN if the
+// particle is not in the intermediate regime); same arithmetic as ext_force
+__device__ __forceinline__ double bingham_candidate(const DevParams &P, double rho_i, double vx1, double vy1) {
+  double h = rho_i, hvmod;
+  if (h < P.comp) {
+    h = P.comp;
+    hvmod = 0;
+  } else {
+    double hu = h * vx1, hv = h * vy1;
+    hvmod = sqrt(hu * hu + hv * hv);
+  }
+  double Tau0;
+  if (bingham_tau0(P, hvmod, fmax(h, P.hfrict0), Tau0)) return Tau0;
+  return __longlong_as_double(0x7ff8000000000000ll);
+}
+
+// tau_in: for nfric = 4, the Tau0 resolved by the last-valid scan (the
+// particle's own value when it is in the intermediate regime)
+__device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem, double xx, double yy,
+                                          double vx1, double vy1, double rho_i, double &u_i,
+                                          double uaux1, double inx, double iny, double tau_in, double &exo,
+                                          double &eyo, double &duo) {
+  const double dt = P.dt;
+  double h, hu, hv, hinv, vmod, hvmod, hvmodinv;
+  double vxx, vyy;
+  const double gravz = P.cgra;
+  double frictx = 0, fricty = 0, TotFrict = 0, cf5 = 0, hfrict, pwp;
+  double xi_voellmy = P.xi_voellmy;
+  double exx = 0.0, exy = 0.0, du = 0.0;
+
+  h = rho_i;
+  hu = h * vx1;
+  hv = h * vy1;
+  if (h < P.comp) {
+    h = P.comp;
+    hu = 0;
+    hv = 0;
+    hinv = 0;
+    vmod = 0;
+    hvmod = 0;
+    hvmodinv = 0;
+  } else {
+    hinv = 1 / h;
+    hvmod = sqrt(hu * hu + hv * hv);
+    vmod = hvmod * hinv;
+    hvmodinv = (hvmod >= 1.e-5) ? 1. / hvmod : 0.0;
+  }
+
+  TopoPt tp;
+  get_z_topo(dem, xx, yy, P.if_frictional != 0, tp);
+  exx = exx + (-P.cgra * tp.Zx);
+  exy = exy + (-P.cgra * tp.Zy);
+
+  if (P.icpwp == 1) pwp = u_i;
+  else if (P.icpwp == 3) pwp = uaux1;
+  else pwp = 0.0;
+  hfrict = fmax(h, P.hfrict0);
+  const int nfric = P.nfric;
+
+  if (nfric == 1) {
+    double cf1 = P.cgra * xi_voellmy * xi_voellmy;
+    double hp = pow(hfrict, P.c73);
+    frictx = cf1 * hu * hvmod / hp;
+    fricty = cf1 * hv * hvmod / hp;
+    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+  } else if (nfric == 2) {
+    double cf2a = P.constK / (8. * P.dens);
+    double h2 = pow(hfrict, 2.);
+    frictx = cf2a * hu / h2;
+    fricty = cf2a * hv / h2;
+    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+  } else if (nfric == 4 || nfric == 24 || nfric == 25) {
+    const double Tauy0 = P.tauy0, constK = P.constK, m_vexp = P.visco;
+    if (nfric == 4) {
+      // every regime ends with cf4b = Tau0/dens (6:9619-9622); Tau0 is this
+      // particle's own in the intermediate regime, else the carried one
+      double Tau0 = tau_in;
+      bingham_tau0(P, hvmod, hfrict, Tau0);
+      double cf4b = Tau0 / P.dens;
+      frictx = cf4b * hu * hvmodinv;
+      fricty = cf4b * hv * hvmodinv;
+      (void)constK;
+    } else {
+      double Tau0 = P.constK * pow(vmod / hfrict, m_vexp);
+      double Cm = (nfric == 24) ? 1. : pow((2 * m_vexp + 1) / m_vexp, m_vexp);
+      Tau0 = Tauy0 + Tau0 * Cm;
+      double cf4b = Tau0 / P.dens;
+      frictx = cf4b * hu * hvmodinv;
+      fricty = cf4b * hv * hvmodinv;
+    }
+    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+  } else if (P.if_frictional == 1) {
+    double cv2byR = 0.0, tanfi;
+    if (P.if_V2byR == 1) {
+      vxx = hu * hinv;
+      vyy = hv * hinv;
+      vmod = pow(vxx * vxx + vyy * vyy, 0.5);
+      if (vmod >= 1e-5) {
+        double cosa = vxx / vmod, sina = vyy / vmod;
+        double cos2 = cosa * cosa, sin2 = sina * sina, cbs = sina * cosa;
+        double xnum = tp.t7 * cos2 + tp.t8 * cbs + tp.t9 * sin2;
+        double xden = tp.t4 * cos2 + tp.t5 * cbs + tp.t6 * sin2;
+        cv2byR = vmod * vmod * (xnum / xden);
+      }
+    }
+    tanfi = P.tanfi8;
+    if (nfric == 8 || nfric == 26 || nfric == 27) {
+      xi_voellmy = 0.0;
+      double Froude = vmod / pow(gravz * h, 0.5);
+      if (nfric == 8 || nfric == 26) {
+        double ff = (1 + P.c3 * pow(Froude * P.c6 / h, P.c7));
+        tanfi = tanfi * ff;
+      } else {
+        double Inert = Froude * P.c6 / h;
+        tanfi = tanfi + (P.c3 - tanfi) / (1.0 + (P.c7 / Inert));
+      }
+    }
+    double hhh = rho_i;
+    if (pwp > P.fpwp_max * hhh) pwp = P.fpwp_max * hhh;
+    if (pwp < P.fpwp_min * hhh) pwp = P.fpwp_min * hhh;
+    u_i = pwp;  // 6:9675
+    if (P.icpwp == 1 || P.icpwp == 3) {
+      double factpw = 1. - pwp * hinv;
+      if (factpw < P.fact_pwp_min) factpw = P.fact_pwp_min;
+      if (factpw > P.fact_pwp_max) factpw = P.fact_pwp_max;
+      tanfi = tanfi * factpw;
+    }
+    double gtanfi = (gravz + cv2byR) * tanfi * P.fact_dens;
+    if (gtanfi < 0.0) gtanfi = 0.0;
+    cf5 = gtanfi;
+    frictx = cf5 * hu * hvmodinv * h;
+    fricty = cf5 * hv * hvmodinv * h;
+    if (xi_voellmy > 0.0) {
+      vxx = vx1;
+      vyy = vy1;
+      vmod = vxx * vxx + vyy * vyy;
+      vmod = (vmod > 0.0) ? sqrt(vmod) : 0.0;
+      frictx = frictx + P.cgra * vmod * vxx / xi_voellmy;
+      fricty = fricty + P.cgra * vmod * vyy / xi_voellmy;
+    } else if (xi_voellmy < 0.) {
+      vxx = vx1;
+      vyy = vy1;
+      vmod = vxx * vxx + vyy * vyy;
+      vmod = (vmod > 0.0) ? sqrt(vmod) : 0.0;
+      frictx = frictx + P.constMu_CF * vmod * vxx / (hfrict * hfrict);
+      fricty = fricty + P.constMu_CF * vmod * vyy / (hfrict * hfrict);
+    }
+    if (nfric == 9 || nfric == 10) {
+      double C67 = P.c6 + P.c7;
+      double frict_abs = pow(frictx * frictx + fricty * fricty, 0.5);
+      double fact9 = pow(2. * P.c8 * vmod / hfrict, 1. / C67) * pow(frict_abs, -P.c6 / C67);
+      if (P.c47 > 1.e-6) fact9 = fact9 * ((1 - P.c47) / (1 + P.c47));
+      frictx = frictx * (1 + fact9);
+      fricty = fricty * (1 + fact9);
+    }
+    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+  } else if (nfric == 11) {
+    vxx = hu * hinv;
+    vyy = hv * hinv;
+    frictx = P.K_Bagnold * vxx * vmod / (hfrict * hfrict);
+    fricty = P.K_Bagnold * vyy * vmod / (hfrict * hfrict);
+    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+  }
+
+  // limiter (6:9836-9895)
+  double dcx = -frictx * hinv, dcy = -fricty * hinv;
+  double ax = exx + inx, ay = exy + iny;
+  if (vmod > 1.e-5) {
+    double vsx = vx1 + dt * (ax + dcx), vsy = vy1 + dt * (ay + dcy);
+    double vdot = vsx * vx1 + vsy * vy1;
+    if (vdot < 0) {
+      double tx = vx1 / vmod, ty = vy1 / vmod;
+      double accelT = ax * tx + ay * ty;
+      double deccelT = dcx * tx + dcy * ty;
+      if (accelT * deccelT < 0) {
+        dcx = -vx1 / dt - ax;
+        dcy = -vy1 / dt - ay;
+        frictx = -dcx * h;
+        fricty = -dcy * h;
+      }
+    }
+  } else {
+    double accelT = pow(ax * ax + ay * ay, 0.5);
+    if (accelT > 1.e-5) {
+      double ctf = TotFrict;
+      if (P.if_frictional == 1) ctf = cf5;
+      frictx = ctf * ax / accelT;
+      fricty = ctf * ay / accelT;
+      double fricT = pow(frictx * frictx + fricty * fricty, 0.5);
+      if (accelT <= fricT) {
+        frictx = ax;
+        fricty = ay;
+      }
+    } else {
+      frictx = 0.0;
+      fricty = 0.0;
+    }
+    frictx = frictx * h;
+    fricty = fricty * h;
+  }
+  if (P.icpwp == 1) {
+    double betaw = P.Bfact * hinv * hinv / P.hrelpw2;
+    du = -(u_i / dt) * (1 - exp(-betaw * dt));
+  }
+  exo = exx - frictx / h;
+  eyo = exy - fricty / h;
+  duo = du;
+}
+
+// a13. Check_Out_Domain (7:1764-1788)
+__device__ __forceinline__ bool out_of_domain(const DevDem &d, double x, double y) {
+  return ((x - d.xmin) * (x - d.xmax) > 0.0) || ((y - d.ymin) * (y - d.ymax) > 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// wave / block reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t = __shfl_xor(v, o, 64);
+    v = t < v ? t : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t = __shfl_xor(v, o, 64);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Accumulates the bounding box of active particles; one atomic per wave.
+__device__ __forceinline__ void bbox_accumulate(BBoxAcc *acc, bool active, double2 p, double hs) {
+  const unsigned long long BIG = ~0ull, SMALL = 0ull;
+  unsigned long long oxmin = active ? ord_of(p.x) : BIG, oxmax = active ? ord_of(p.x) : SMALL;
+  unsigned long long oymin = active ? ord_of(p.y) : BIG, oymax = active ? ord_of(p.y) : SMALL;
+  unsigned long long ohmax = active ? ord_of(hs) : SMALL;
+  unsigned cnt = active ? 1u : 0u;
+  oxmin = wave_min_u64(oxmin);
+  oxmax = wave_max_u64(oxmax);
+  oymin = wave_min_u64(oymin);
+  oymax = wave_max_u64(oymax);
+  ohmax = wave_max_u64(ohmax);
+  cnt = wave_sum_u32(cnt);
+  if ((threadIdx.x & 63) == 0 && cnt > 0) {
+    atomicMin(&acc->xmin, oxmin);
+    atomicMax(&acc->xmax, oxmax);
+    atomicMin(&acc->ymin, oymin);
+    atomicMax(&acc->ymax, oymax);
+    atomicMax(&acc->hmax, ohmax);
+    atomicAdd(&acc->nactive, cnt);
+  }
+}
+
+__device__ __forceinline__ void bbox_reset(BBoxAcc *acc) {
+  acc->xmin = ~0ull;
+  acc->ymin = ~0ull;
+  acc->xmax = 0ull;
+  acc->ymax = 0ull;
+  acc->hmax = 0ull;
+  acc->nactive = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// neighbour build
+// ---------------------------------------------------------------------------
+
+__global__ void k_bbox(int n, const double2 *pos, const double *hsml, const int *flag, BBoxAcc *acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = false;
+  double2 p = make_double2(0, 0);
+  double hs = 0;
+  if (i < n) {
+    act = flag[i] == 0;
+    p = pos[i];
+    hs = hsml[i];
+  }
+  bbox_accumulate(acc, act, p, hs);
+}
+
+__global__ void k_bbox_reset(BBoxAcc *acc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) bbox_reset(acc);
+}
+
+// Grid of the active bounding box.  Cell edge = scale_k*max(hsml)*(1+1e-7),
+// so every pair passing the predicate lies in adjacent cells; grown if the
+// grid would exceed cap_cells.  Every block derives the same grid; block 0
+// stores it.  Then all blocks clear the counts of ncell+1 buckets.
+__device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int cap_cells) {
+  GridInfo g;
+  if (a.nactive == 0) {
+    g.xmin = 0;
+    g.ymin = 0;
+    g.cell = 1;
+    g.ncx = 1;
+    g.ncy = 1;
+    g.ncell = 1;
+    g.pad = 0;
+    return g;
+  }
+  double xmin = val_of(a.xmin), xmax = val_of(a.xmax), ymin = val_of(a.ymin), ymax = val_of(a.ymax);
+  double hmax = val_of(a.hmax);
+  double cell = scale_k * hmax * (1.0 + 1.0e-7);
+  if (!(cell > 0)) cell = 1.0;
+  double lx = xmax - xmin, ly = ymax - ymin;
+  double ncx = floor(lx / cell) + 1, ncy = floor(ly / cell) + 1;
+  if (ncx * ncy > (double)cap_cells) {
+    double f = sqrt(ncx * ncy / (double)cap_cells) * 1.01;
+    cell *= f;
+    ncx = floor(lx / cell) + 1;
+    ncy = floor(ly / cell) + 1;
+  }
+  g.xmin = xmin;
+  g.ymin = ymin;
+  g.cell = cell;
+  g.ncx = (int)ncx;
+  g.ncy = (int)ncy;
+  g.ncell = g.ncx * g.ncy;
+  g.pad = 0;
+  return g;
+}
+
+__global__ void k_setup_clear(const BBoxAcc *acc, GridInfo *gout, int *cnt, int scale_k, int cap_cells) {
+  GridInfo g = make_grid(*acc, scale_k, cap_cells);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *gout = g;
+  int nb = g.ncell + 1;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += gridDim.x * blockDim.x) cnt[c] = 0;
+}
+
+__device__ __forceinline__ void cell_of(const GridInfo &g, double2 p, int &cx, int &cy) {
+  cx = (int)((p.x - g.xmin) / g.cell);
+  cy = (int)((p.y - g.ymin) / g.cell);
+  cx = cx < 0 ? 0 : (cx >= g.ncx ? g.ncx - 1 : cx);
+  cy = cy < 0 ? 0 : (cy >= g.ncy ? g.ncy - 1 : cy);
+}
+
+__global__ void k_keys(int n, const double2 *pos, const int *flag, const GridInfo *gp, int *key, int *rank,
+                       int *cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  GridInfo g = *gp;
+  int k;
+  if (flag[i]) {
+    k = g.ncell;
+  } else {
+    int cx, cy;
+    cell_of(g, pos[i], cx, cy);
+    k = cy * g.ncx + cx;
+  }
+  key[i] = k;
+  rank[i] = atomicAdd(&cnt[k], 1);
+}
+
+// exclusive scan of cnt[0..ncell] -> start[0..ncell+1]; 2048 items per block
+#define SCAN_ITEMS 8
+#define SCAN_TILE (GF_BLOCK * SCAN_ITEMS)
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total) {
+  // wave inclusive scan
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(x, o, 64);
+    if (lane >= o) x += t;
+  }
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < GF_BLOCK / 64; w++) {
+      int t = lds[w];
+      lds[w] = s;
+      s += t;
+    }
+    lds[GF_BLOCK / 64] = s;
+  }
+  __syncthreads();
+  int res = x - v + lds[wid];
+  total = lds[GF_BLOCK / 64];
+  __syncthreads();
+  return res;
+}
+
+__global__ void k_scan_reduce(const GridInfo *gp, const int *cnt, int *bsum) {
+  __shared__ int lds[GF_BLOCK / 64 + 1];
+  int nb = gp->ncell + 1;
+  int base = blockIdx.x * SCAN_TILE;
+  if (base >= nb) {
+    if (threadIdx.x == 0) bsum[blockIdx.x] = 0;
+    return;
+  }
+  int s = 0;
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    int c = base + threadIdx.x * SCAN_ITEMS + k;
+    if (c < nb) s += cnt[c];
+  }
+  int total;
+  block_exclusive_scan(s, lds, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void k_scan_top(int nblocks, int *bsum) {
+  __shared__ int lds[GF_BLOCK / 64 + 1];
+  int carry = 0;
+  for (int base = 0; base < nblocks; base += GF_BLOCK) {
+    int i = base + threadIdx.x;
+    int v = i < nblocks ? bsum[i] : 0;
+    int total;
+    int ex = block_exclusive_scan(v, lds, total);
+    if (i < nblocks) bsum[i] = ex + carry;
+    carry += total;
+  }
+}
+
+__global__ void k_scan_down(const GridInfo *gp, const int *cnt, const int *bsum, int *start, int n) {
+  __shared__ int lds[GF_BLOCK / 64 + 1];
+  int nb = gp->ncell + 1;
+  int base = blockIdx.x * SCAN_TILE;
+  if (base >= nb) return;
+  int v[SCAN_ITEMS];
+  int s = 0;
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    int c = base + threadIdx.x * SCAN_ITEMS + k;
+    v[k] = (c < nb) ? cnt[c] : 0;
+    s += v[k];
+  }
+  int total;
+  int ex = block_exclusive_scan(s, lds, total) + bsum[blockIdx.x];
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    int c = base + threadIdx.x * SCAN_ITEMS + k;
+    if (c < nb) start[c] = ex;
+    ex += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) start[nb] = n;
+}
+
+__global__ void k_scatter(int n, const int *key, const int *rank, const int *start, int *perm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  perm[start[key[i]] + rank[i]] = i;
+}
+
+// Each cell ordered by previous slot so that runs are deterministic (the
+// out-of-domain bucket is left unordered: its particles never interact).
+__global__ void k_cellsort(const GridInfo *gp, const int *start, int *perm) {
+  int nb = gp->ncell;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += gridDim.x * blockDim.x) {
+    int s = start[c], e = start[c + 1];
+    for (int a = s + 1; a < e; a++) {
+      int v = perm[a];
+      int b = a - 1;
+      while (b >= s && perm[b] > v) {
+        perm[b + 1] = perm[b];
+        b--;
+      }
+      perm[b + 1] = v;
+    }
+  }
+}
+
+// Gathers the state into cell order.  Sources: positions/depth/smoothing
+// length/pore pressure/flags of the previous step's outputs (src_new) and
+// mass/id/velocity of the previous sorted set (src_old).
+__global__ void k_reorder(int n, const int *perm, NewSet src_new, SortedSet src_old, SortedSet dst,
+                          BBoxAcc *acc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) bbox_reset(acc);
+  if (s >= n) return;
+  int i = perm[s];
+  dst.pos[s] = src_new.pos[i];
+  dst.vel[s] = src_old.vel[i];
+  dst.h[s] = src_new.h[i];
+  dst.mass[s] = src_old.mass[i];
+  dst.hsml[s] = src_new.hsml[i];
+  dst.u[s] = src_new.u[i];
+  dst.uaux[s] = src_new.uaux[i];
+  dst.flag[s] = src_new.flag[i];
+  dst.id[s] = src_old.id[i];
+}
+
+// ---------------------------------------------------------------------------
+// neighbour iteration helper: calls f(j) for every candidate j in the 3x3
+// block of cells around particle slot i (3 contiguous slot ranges).
+// ---------------------------------------------------------------------------
+template <typename F>
+__device__ __forceinline__ void for_candidates(const GridInfo &g, const int *start, double2 p, F &&f) {
+  int cx, cy;
+  cell_of(g, p, cx, cy);
+  int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
+  int r0 = cy > 0 ? cy - 1 : 0, r1 = cy < g.ncy - 1 ? cy + 1 : g.ncy - 1;
+  for (int r = r0; r <= r1; r++) {
+    int b = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
+    for (int j = b; j < e; j++) f(j);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a4. density: S_i = W(0,hsml_i) m_i + sum_j m_j w_ij (5a:1763-1859);
+// also P_i = p_i / rhoF_i^2 with rhoF = max(S, h_inf), p = g/2 rhoF^2
+// (IntForces_SW, 6:6769, 6:6797, pa_sph = 2), or p_i for pa_sph 1/4.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const GridInfo *gp, const int *start,
+                                                       SortedSet s, double *S, double *Pout) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (s.flag[i]) {
+    S[i] = s.h[i];
+    Pout[i] = 0.0;
+    return;
+  }
+  const GridInfo g = *gp;
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  double acc = sph_self_w(P.skf, P.pi, hi) * s.mass[i];
+  for_candidates(g, start, pi_, [&](int j) {
+    if (j == i) return;
+    double2 pj = s.pos[j];
+    double hj = s.hsml[j];
+    double dx, dy, r, mh;
+    if (pair_pred(P.scale_k, pi_, pj, hi, hj, dx, dy, r, mh)) {
+      double w, gx, gy;
+      sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
+      acc = acc + s.mass[j] * w;
+    }
+  });
+  S[i] = acc;
+  double rf = acc < P.h_inf ? P.h_inf : acc;
+  double p = P.cgra05 * rf * rf;
+  if (P.pa_sph == 2) Pout[i] = p / (rf * rf);
+  else Pout[i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// a5. pressure force gather (IntForces_SW pair sum, 6:6892-6941)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void pressure_force(const DevParams &P, const GridInfo &g, const int *start,
+                                               const SortedSet &s, const double *S, const double *Pp, int i,
+                                               double &inx, double &iny) {
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  const double Si = S[i];
+  const double rfi = Si < P.h_inf ? P.h_inf : Si;
+  const double Pi = Pp[i];
+  inx = 0.0;
+  iny = 0.0;
+  for_candidates(g, start, pi_, [&](int j) {
+    if (j == i) return;
+    double2 pj = s.pos[j];
+    double hj = s.hsml[j];
+    double dx, dy, r, mh;
+    if (pair_pred(P.scale_k, pi_, pj, hi, hj, dx, dy, r, mh)) {
+      double w, gx, gy;
+      sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
+      double mj = s.mass[j];
+      double Pj = Pp[j];
+      if (P.pa_sph == 2) {
+        double c = -(Pi + Pj);
+        inx = inx + mj * (c * gx);
+        iny = iny + mj * (c * gy);
+      } else if (P.pa_sph == 1) {
+        double Sj = S[j];
+        double rfj = Sj < P.h_inf ? P.h_inf : Sj;
+        double rhoij = 1.e0 / (rfi * rfj);
+        inx = inx + mj * ((-(Pi + Pj) * gx) * rhoij);
+        iny = iny + mj * ((-(Pi + Pj) * gy) * rhoij);
+      } else {  // pa_sph == 4
+        double Sj = S[j];
+        double rfj = Sj < P.h_inf ? P.h_inf : Sj;
+        inx = inx - ((Pj - Pi) * gx * mj / rfj);
+        iny = iny - ((Pj - Pi) * gy * mj / rfj);
+      }
+    }
+  });
+}
+
+// frozen out-of-domain particle: IntForces' h_inf floor (6:6769) and
+// Get_Pwp_SW's Uaux = 0 (6:11184) still apply
+__device__ __forceinline__ void write_frozen(const DevParams &P, const SortedSet &s, int i, NewSet &o) {
+  double hh = s.h[i];
+  o.pos[i] = s.pos[i];
+  o.vel[i] = s.vel[i];
+  o.h[i] = hh < P.h_inf ? P.h_inf : hh;
+  o.hsml[i] = s.hsml[i];
+  if (P.nuaux) {
+    o.uaux[i] = 0.0;
+    o.u[i] = 0.0;
+  } else {
+    o.uaux[i] = s.uaux[i];
+    o.u[i] = s.u[i];
+  }
+  o.flag[i] = 1;
+}
+
+// After the stages: final update (7:908-922), final depth (7:926-931), domain
+// flag, pore pressure npwp = 1 (6:11176-11310), u clamp (7:971-980),
+// h_upgrade sle = 1 (7:2925-2931).  Writes the particle's outputs.
+__device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem &dem, const SortedSet &s, int i,
+                                                double Si, double rfi, double2 x, double2 v, double u, double Dx,
+                                                double Dy, double Dvx, double Dvy, double Du, bool flagged,
+                                                NewSet &o, bool &active_out, double2 &xo, double &hso) {
+  const double dt = P.dt;
+  const double2 x0 = s.pos[i], v0 = s.vel[i];
+  const double u0 = s.u[i], h0 = s.h[i], uaux1 = s.uaux[i], mi = s.mass[i], hi = s.hsml[i];
+  double hnew;
+  if (!flagged) {
+    x.x = x0.x + P.fin * Dx;
+    x.y = x0.y + P.fin * Dy;
+    v.x = v0.x + P.fin * Dvx;
+    v.y = v0.y + P.fin * Dvy;
+    u = u0 + P.fin * Du;
+    hnew = Si;
+  } else {
+    hnew = rfi;  // frozen mid-step: keeps its clamped stage depth
+  }
+  if (out_of_domain(dem, x.x, x.y)) flagged = true;
+  double uaux_new = uaux1;
+  if (P.nuaux) {
+    if (flagged) {
+      uaux_new = 0.0;
+    } else {
+      double hh = hnew, dhh = hnew - h0, hinv;
+      if (hh < P.comp) {
+        hh = P.comp;
+        hinv = 1.0 / P.comp;
+      } else {
+        hinv = 1 / hh;
+      }
+      double betaw = P.Bfact * hinv * hinv / P.hrelpw2;
+      double e = exp(-betaw * dt);
+      uaux_new = uaux1 * e + (dhh / dt) * (1. - e) / betaw;
+      if (uaux_new > hh) uaux_new = hh;
+      if (uaux_new < hh * P.alphaS * P.fpwp_min) uaux_new = hh * P.alphaS * P.fpwp_min;
+    }
+    u = uaux_new;
+  } else if (!flagged) {
+    if (u > hnew) u = hnew;
+    if (u < 0.0) u = 0.0;
+  }
+  double hs = hi;
+  if (P.sle == 1 && !flagged) hs = 2.0 * pow(mi / hnew, 1. / 2);
+  o.pos[i] = x;
+  o.vel[i] = v;
+  o.h[i] = hnew;
+  o.hsml[i] = hs;
+  o.u[i] = u;
+  o.uaux[i] = uaux_new;
+  o.flag[i] = flagged ? 1 : 0;
+  active_out = !flagged;
+  xo = x;
+  hso = hs;
+}
+
+// stage update of position, velocity and pore pressure (7:811-833)
+__device__ __forceinline__ void stage_update(const DevParams &P, int st, double2 x0, double2 v0, double u0, double h0,
+                                             double rfi, double ex, double ey, double du, double inx, double iny,
+                                             double2 &x, double2 &v, double &u) {
+  const double f1 = P.f1[st], dt = P.dt;
+  x.x = x0.x + f1 * dt * v.x;
+  x.y = x0.y + f1 * dt * v.y;
+  v.x = v0.x + f1 * dt * (ex + inx);
+  v.y = v0.y + f1 * dt * (ey + iny);
+  u = u0 + f1 * dt * du;
+  if (P.nuaux == 0) {
+    double rr = (st == 0) ? h0 : rfi;
+    if (u > rr) u = rr;
+    else if (u < 0.0) u = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a5 + a6..a10, a12, a13: pressure force gather fused with the RK stages
+// (every basal law except Bingham, whose stages couple particles)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GF_BLOCK) void k_force_rk(int n, DevParams P, DevDem dem, const GridInfo *gp,
+                                                        const int *start, SortedSet s, const double *S,
+                                                        const double *Pp, NewSet o, BBoxAcc *acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active_out = false;
+  double2 xo = make_double2(0, 0);
+  double hso = 0;
+  if (i < n) {
+    if (s.flag[i]) {
+      write_frozen(P, s, i, o);
+    } else {
+      const GridInfo g = *gp;
+      double inx, iny;
+      pressure_force(P, g, start, s, S, Pp, i, inx, iny);
+      const double Si = S[i];
+      const double rfi = Si < P.h_inf ? P.h_inf : Si;
+      const double2 x0 = s.pos[i], v0 = s.vel[i];
+      const double u0 = s.u[i], h0 = s.h[i], uaux1 = s.uaux[i];
+      double2 x = x0, v = v0;
+      double u = u0, ex = 0.0, ey = 0.0, du = 0.0;
+      double Dx = 0, Dy = 0, Dvx = 0, Dvy = 0, Du = 0;
+      bool flagged = false;
+      double tau_unused = 0.0;
+      for (int st = 0; st < P.nstages; st++) {
+        stage_update(P, st, x0, v0, u0, h0, rfi, ex, ey, du, inx, iny, x, v, u);
+        if (out_of_domain(dem, x.x, x.y)) {
+          flagged = true;
+          break;
+        }
+        ext_force(P, dem, x.x, x.y, v.x, v.y, rfi, u, uaux1, inx, iny, tau_unused, ex, ey, du);
+        const double f2 = P.f2[st];
+        Dvx += f2 * (ex + inx);
+        Dvy += f2 * (ey + iny);
+        Dx += f2 * v.x;
+        Dy += f2 * v.y;
+        Du += f2 * du;
+      }
+      finish_particle(P, dem, s, i, Si, rfi, x, v, u, Dx, Dy, Dvx, Dvy, Du, flagged, o, active_out, xo, hso);
+    }
+  }
+  bbox_accumulate(acc, active_out, xo, hso);
+}
+
+// ---------------------------------------------------------------------------
+// Bingham (nfric = 4): ExtForces_SW applies cf4b = Tau0/dens after every
+// branch (6:9619-9622) but sets Tau0 only in the intermediate regime
+// (6:9616), so a particle in the viscous or cohesive regime uses the Tau0 of
+// the last particle, in particle index order, that was intermediate in the
+// same ExtForces call -- or in an earlier call.  Each stage therefore runs as
+//   k_bingham_stage(phase 0): stage update, regime, candidate Tau0 per id
+//   last-valid scan over particle ids (with the carry of earlier calls)
+//   k_bingham_stage(phase 1): basal friction with the resolved Tau0
+// ---------------------------------------------------------------------------
+struct BinghamState {
+  double2 *in;   // pressure force (stage invariant)
+  double2 *x, *v, *ex;
+  double *u, *du;
+  double *D;     // [n][5] Dx Dy Dvx Dvy Du
+  int *fl;       // flagged in mid-step
+  double *cand;  // [n] by caller id: Tau0 or NaN
+  double *tau;   // [n] by caller id: resolved Tau0
+  double *carry; // [1]
+  double *bsum;  // per scan block: last valid value (NaN if none)
+};
+
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_force(int n, DevParams P, const GridInfo *gp, const int *start,
+                                                             SortedSet s, const double *S, const double *Pp,
+                                                             BinghamState b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double inx = 0.0, iny = 0.0;
+  if (!s.flag[i]) {
+    const GridInfo g = *gp;
+    pressure_force(P, g, start, s, S, Pp, i, inx, iny);
+  }
+  b.in[i] = make_double2(inx, iny);
+  b.x[i] = s.pos[i];
+  b.v[i] = s.vel[i];
+  b.ex[i] = make_double2(0.0, 0.0);
+  b.u[i] = s.u[i];
+  b.du[i] = 0.0;
+  for (int k = 0; k < 5; k++) b.D[5 * i + k] = 0.0;
+  b.fl[i] = 0;
+}
+
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, int st, DevParams P, DevDem dem,
+                                                             SortedSet s, const double *S, BinghamState b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int id = s.id[i];
+  if (s.flag[i] || b.fl[i]) {
+    if (phase == 0) b.cand[id] = __longlong_as_double(0x7ff8000000000000ll);
+    return;
+  }
+  const double Si = S[i];
+  const double rfi = Si < P.h_inf ? P.h_inf : Si;
+  const double2 in = b.in[i];
+  if (phase == 0) {
+    double2 x = b.x[i], v = b.v[i], ex = b.ex[i];
+    double u = b.u[i];
+    stage_update(P, st, s.pos[i], s.vel[i], s.u[i], s.h[i], rfi, ex.x, ex.y, b.du[i], in.x, in.y, x, v, u);
+    b.x[i] = x;
+    b.v[i] = v;
+    b.u[i] = u;
+    if (out_of_domain(dem, x.x, x.y)) {
+      b.fl[i] = 1;
+      b.cand[id] = __longlong_as_double(0x7ff8000000000000ll);
+      return;
+    }
+    b.cand[id] = bingham_candidate(P, rfi, v.x, v.y);
+  } else {
+    double2 x = b.x[i], v = b.v[i];
+    double u = b.u[i], ex, ey, du;
+    double tau = b.tau[id];
+    ext_force(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, tau, ex, ey, du);
+    b.u[i] = u;
+    b.ex[i] = make_double2(ex, ey);
+    b.du[i] = du;
+    const double f2 = P.f2[st];
+    double *D = b.D + 5 * i;
+    D[2] += f2 * (ex + in.x);
+    D[3] += f2 * (ey + in.y);
+    D[0] += f2 * v.x;
+    D[1] += f2 * v.y;
+    D[4] += f2 * du;
+  }
+}
+
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P, DevDem dem, SortedSet s,
+                                                              const double *S, BinghamState b, NewSet o,
+                                                              BBoxAcc *acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active_out = false;
+  double2 xo = make_double2(0, 0);
+  double hso = 0;
+  if (i < n) {
+    if (s.flag[i]) {
+      write_frozen(P, s, i, o);
+    } else {
+      const double Si = S[i];
+      const double rfi = Si < P.h_inf ? P.h_inf : Si;
+      const double *D = b.D + 5 * i;
+      finish_particle(P, dem, s, i, Si, rfi, b.x[i], b.v[i], b.u[i], D[0], D[1], D[2], D[3], D[4], b.fl[i] != 0,
+                      o, active_out, xo, hso);
+    }
+  }
+  bbox_accumulate(acc, active_out, xo, hso);
+}
+
+// last-valid inclusive scan over particle ids: tau[k] = cand of the largest
+// k' <= k with cand[k'] a number, else the carry of earlier calls
+#define LV_ITEMS 8
+#define LV_TILE (GF_BLOCK * LV_ITEMS)
+
+__device__ __forceinline__ bool is_valid(double v) { return v == v; }
+
+__global__ void k_lv_reduce(int n, const double *cand, double *bsum) {
+  __shared__ double lds[GF_BLOCK];
+  int base = blockIdx.x * LV_TILE;
+  double last = __longlong_as_double(0x7ff8000000000000ll);
+  for (int k = 0; k < LV_ITEMS; k++) {
+    int idx = base + threadIdx.x * LV_ITEMS + k;
+    if (idx < n && is_valid(cand[idx])) last = cand[idx];
+  }
+  lds[threadIdx.x] = last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = __longlong_as_double(0x7ff8000000000000ll);
+    for (int t = 0; t < GF_BLOCK; t++)
+      if (is_valid(lds[t])) r = lds[t];
+    bsum[blockIdx.x] = r;
+  }
+}
+
+// sequential over blocks (one thread): bsum becomes the value valid before
+// each block; the carry becomes the last valid value of this call
+__global__ void k_lv_top(int nblocks, double *bsum, double *carry) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double c = *carry;
+  for (int b = 0; b < nblocks; b++) {
+    double t = bsum[b];
+    bsum[b] = c;
+    if (is_valid(t)) c = t;
+  }
+  *carry = c;
+}
+
+__global__ void k_lv_down(int n, const double *cand, const double *bsum, double *tau) {
+  __shared__ double lds[GF_BLOCK];
+  int base = blockIdx.x * LV_TILE;
+  double last = __longlong_as_double(0x7ff8000000000000ll);
+  for (int k = 0; k < LV_ITEMS; k++) {
+    int idx = base + threadIdx.x * LV_ITEMS + k;
+    if (idx < n && is_valid(cand[idx])) last = cand[idx];
+  }
+  lds[threadIdx.x] = last;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of "last valid" over the block's threads
+  for (int off = 1; off < GF_BLOCK; off <<= 1) {
+    double mine = lds[threadIdx.x];
+    double other = threadIdx.x >= off ? lds[threadIdx.x - off] : __longlong_as_double(0x7ff8000000000000ll);
+    __syncthreads();
+    if (!is_valid(mine)) lds[threadIdx.x] = other;
+    __syncthreads();
+  }
+  double c = threadIdx.x > 0 ? lds[threadIdx.x - 1] : __longlong_as_double(0x7ff8000000000000ll);
+  if (!is_valid(c)) c = bsum[blockIdx.x];
+  for (int k = 0; k < LV_ITEMS; k++) {
+    int idx = base + threadIdx.x * LV_ITEMS + k;
+    if (idx < n) {
+      if (is_valid(cand[idx])) c = cand[idx];
+      tau[idx] = c;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a11. XSPH av_vel (7:3042-3099) with the frozen step-start geometry and the
+// post-RK velocities and depths; v += 0.3*av for particles still active.
+// Writes the final velocity into vout (the sorted set's velocity array).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GF_BLOCK) void k_xsph(int n, DevParams P, const GridInfo *gp, const int *start,
+                                                    SortedSet s, NewSet o, double2 *vout) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 vi = o.vel[i];
+  if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0) {
+    vout[i] = vi;
+    return;
+  }
+  const GridInfo g = *gp;
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  const double rhoi = o.h[i];
+  double ax = 0.0, ay = 0.0;
+  for_candidates(g, start, pi_, [&](int j) {
+    if (j == i) return;
+    if (s.id[j] < P.npoin0) return;
+    double2 pj = s.pos[j];
+    double hj = s.hsml[j];
+    double dx, dy, r, mh;
+    if (pair_pred(P.scale_k, pi_, pj, hi, hj, dx, dy, r, mh)) {
+      double w, gx, gy;
+      sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
+      double mj = s.mass[j];
+      double2 vj = o.vel[j];
+      double den = rhoi + o.h[j];
+      ax = ax - 2 * mj * (vi.x - vj.x) / den * w;
+      ay = ay - 2 * mj * (vi.y - vj.y) / den * w;
+    }
+  });
+  vi.x = vi.x + 0.3 * ax;
+  vi.y = vi.y + 0.3 * ay;
+  vout[i] = vi;
+}
+
+// ---------------------------------------------------------------------------
+// initialisation helpers
+// ---------------------------------------------------------------------------
+__global__ void k_domain_flag(int n, DevDem dem, const double2 *pos, int *flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 p = pos[i];
+  if (out_of_domain(dem, p.x, p.y)) flag[i] = 1;
+}
+
+// Input_SW: h = summed depth (6:1440), then pwp initialisation (6:1705-1852)
+__global__ void k_init_pwp(int n, DevParams P, double alfs, double pwp0rel, int resum, const double *S,
+                           SortedSet s) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (resum && !s.flag[i]) s.h[i] = S[i];
+  double hh = s.h[i] * alfs;
+  if (P.icpwp == 1) {
+    s.u[i] = pwp0rel * hh;
+  } else if (P.icpwp == 3) {
+    s.uaux[i] = pwp0rel * hh;
+    s.u[i] = s.uaux[i];
+  }
+}
+
+// new-set view of a sorted set (used to re-sort the initial state)
+__global__ void k_copy_to_new(int n, SortedSet s, NewSet o) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  o.pos[i] = s.pos[i];
+  o.h[i] = s.h[i];
+  o.hsml[i] = s.hsml[i];
+  o.u[i] = s.u[i];
+  o.uaux[i] = s.uaux[i];
+  o.flag[i] = s.flag[i];
+}
+
+// FD nodal derivatives (Fdiff_derivs, 2:2812-2877)
+__global__ void k_fdiff(int nx, int ny, double dxg, double dyg, const double *Z, const int *in_model, double2 *zxy,
+                        double *top) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long np = (long)nx * ny;
+  if (k >= np) return;
+  int ix = (int)(k % nx), iy = (int)(k / nx);
+  double zx = 0, zy = 0, t[6] = {0, 0, 0, 0, 0, 0};
+  bool inner = ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2;
+  if (inner && (in_model == nullptr || in_model[k])) {
+    double del2x = 2. * dxg, del2y = 2. * dyg, delx2 = dxg * dxg, dely2 = dyg * dyg, delxy = dxg * dyg;
+    double zA = Z[k], zL = Z[k - 1], zR = Z[k + 1], zT = Z[k + nx], zB = Z[k - nx];
+    double zTL = Z[k + nx - 1], zTR = Z[k + nx + 1], zBL = Z[k - nx - 1], zBR = Z[k - nx + 1];
+    zx = (zR - zL) / del2x;
+    zy = (zT - zB) / del2y;
+    double rt = pow(1. + zx * zx + zy * zy, 0.5);
+    double zxx = (zL - 2. * zA + zR) / delx2;
+    double zyy = (zB - 2. * zA + zT) / dely2;
+    double zxy2 = (zTR - zBR - zTL + zBL) / (4. * delxy);
+    t[0] = 1. + zx * zx;
+    t[1] = zx * zy;
+    t[2] = 1. + zy * zy;
+    t[3] = zxx / rt;
+    t[4] = zxy2 / rt;
+    t[5] = zyy / rt;
+  }
+  zxy[k] = make_double2(zx, zy);
+  if (top)
+    for (int c = 0; c < 6; c++) top[6 * k + c] = t[c];
+}
+
+__global__ void k_topol_split(long np, const double *topol9, double2 *zxy, double *top) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  zxy[k] = make_double2(topol9[9 * k + 1], topol9[9 * k + 2]);
+  if (top)
+    for (int c = 0; c < 6; c++) top[6 * k + c] = topol9[9 * k + 3 + c];
+}
+
+// un-permute to caller order
+__global__ void k_unsort(int n, CurView s, double2 *x, double2 *v, double *h, double *u, double *uaux,
+                         double *hsml, int *flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int d = s.id[i];
+  x[d] = s.pos[i];
+  v[d] = s.vel[i];
+  h[d] = s.h[i];
+  u[d] = s.u[i];
+  uaux[d] = s.uaux[i];
+  hsml[d] = s.hsml[i];
+  flag[d] = s.flag[i];
+}
+
+__global__ void k_nonfinite(int n, CurView s, int *bad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 p = s.pos[i], v = s.vel[i];
+  double q = p.x + p.y + v.x + v.y + s.h[i] + s.u[i];
+  if (!isfinite(q)) atomicOr(bad, 1);
+}
+
+// pair enumeration (i<j by caller index) for the neighbour-set export
+__global__ void k_pairs(int n, int skf, int scale_k, const GridInfo *gp, const int *start, SortedSet s,
+                        long long cap, unsigned long long *count, int *ij) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || s.flag[i]) return;
+  const GridInfo g = *gp;
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  const int idi = s.id[i];
+  for_candidates(g, start, pi_, [&](int j) {
+    if (j == i) return;
+    int idj = s.id[j];
+    if (idj <= idi) return;
+    double2 pj = s.pos[j];
+    double dx, dy, r, mh;
+    if (pair_pred(scale_k, pi_, pj, hi, s.hsml[j], dx, dy, r, mh)) {
+      unsigned long long k = atomicAdd(count, 1ull);
+      if ((long long)k < cap) {
+        ij[2 * k] = idi;
+        ij[2 * k + 1] = idj;
+      }
+    }
+  });
+}
+
+// a14 (i): IDW particle -> node scatter (6:14312-14440), soil particles
+__global__ void k_raster_idw(int n, DevDem dem, CurView s, double *acc5) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || s.flag[i]) return;
+  double2 p = s.pos[i];
+  double2 v = s.vel[i];
+  double h = s.h[i], uu = s.u[i];
+  double distg = pow(dem.dx * dem.dx + dem.dy * dem.dy, 0.5);
+  int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
+  int inx0 = ipx - 1 > 1 ? ipx - 1 : 1, iny0 = ipy - 1 > 1 ? ipy - 1 : 1;
+  int inx1 = ipx + 2 < dem.nx ? ipx + 2 : dem.nx, iny1 = ipy + 2 < dem.ny ? ipy + 2 : dem.ny;
+  for (int inx = inx0; inx <= inx1; inx++)
+    for (int iny = iny0; iny <= iny1; iny++) {
+      long gidx = (long)(iny - 1) * dem.nx + inx - 1;
+      double gx = dem.xmin + (inx - 1) * dem.dx, gy = dem.ymin + (iny - 1) * dem.dy;
+      double a = p.x - gx, b = p.y - gy;
+      a = a * a;
+      b = b * b;
+      double dist = fmax(1.e-6, pow(a + b, 0.5));
+      dist = dist / distg;
+      atomicAdd(&acc5[5 * gidx + 0], h / dist);
+      atomicAdd(&acc5[5 * gidx + 1], v.x / dist);
+      atomicAdd(&acc5[5 * gidx + 2], v.y / dist);
+      atomicAdd(&acc5[5 * gidx + 3], (uu / h) / dist);
+      atomicAdd(&acc5[5 * gidx + 4], 1.00 / dist);
+    }
+}
+
+__global__ void k_raster_norm(long np, const double *acc5, double *out, double *wsum) {
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= np) return;
+  double ws = acc5[5 * g + 4];
+  wsum[g] = ws;
+  for (int k = 0; k < 4; k++) out[4 * g + k] = ws >= 1.e-6 ? acc5[5 * g + k] / ws : acc5[5 * g + k];
+}
+
+// a14 (ii): MC pixel accumulation (6:20717-20833)
+__device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
+  // values are >= 0: the IEEE bit pattern orders like the unsigned integer
+  atomicMax((unsigned long long *)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void k_mc_pixel(int n, DevDem dem, CurView s, double *acc4, int *cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || s.flag[i]) return;
+  double2 p = s.pos[i];
+  double2 v = s.vel[i];
+  int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
+  if (ipx == dem.nx && fabs(dem.xmax - p.x) <= 1.e-3) ipx--;
+  if (ipy == dem.ny && fabs(dem.ymax - p.y) <= 1.e-3) ipy--;
+  if (ipx < 1 || ipy < 1 || ipx > dem.nx - 1 || ipy > dem.ny - 1) return;
+  long g1 = (long)(ipy - 1) * dem.nx + ipx - 1;
+  long nodes[4] = {g1, g1 + 1, g1 + 1 + dem.nx, g1 + dem.nx};
+  double hh = s.h[i];
+  double vv = pow(v.x * v.x + v.y * v.y, 0.5);
+  for (int k = 0; k < 4; k++) {
+    long g = nodes[k];
+    atomicAdd(&cnt[g], 1);
+    atomicAdd(&acc4[4 * g + 0], hh);
+    atomicAdd(&acc4[4 * g + 2], vv);
+    atomic_max_pos(&acc4[4 * g + 1], hh);
+    atomic_max_pos(&acc4[4 * g + 3], vv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct gfsph_ctx {
+  int n = 0;
+  DevParams P{};
+  DevDem dem{};
+  int check_finite = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // device buffers
+  SortedSet set[2]{};
+  NewSet nw{};
+  double *S = nullptr, *Pp = nullptr;
+  int *key = nullptr, *rank = nullptr, *perm = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
+  int cap_cells = 0, scan_blocks = 0;
+  GridInfo *grid = nullptr;
+  BBoxAcc *acc = nullptr;
+  int *bad = nullptr;
+  double2 *zxy = nullptr;
+  double *top = nullptr;
+  double *raster5 = nullptr, *mc_acc = nullptr;
+  int *mc_cnt = nullptr;
+  BinghamState bs{};
+  int lv_blocks = 0;
+  int cur = 0;
+  std::vector<void *> allocs;
+};
+
+static CurView cur_view(const gfsph_ctx *c) {
+  const SortedSet &s = c->set[c->cur];
+  CurView v;
+  v.pos = c->nw.pos;
+  v.vel = s.vel;
+  v.h = c->nw.h;
+  v.mass = s.mass;
+  v.hsml = c->nw.hsml;
+  v.u = c->nw.u;
+  v.uaux = c->nw.uaux;
+  v.flag = c->nw.flag;
+  v.id = s.id;
+  return v;
+}
+
+static int fail(gfsph_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(ctx, GFSPH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+template <typename T>
+static int dalloc(gfsph_ctx *c, T **p, size_t count) {
+  void *q = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(c, GFSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return GFSPH_OK;
+}
+
+static inline int nblk(long n) { return (int)((n + GF_BLOCK - 1) / GF_BLOCK); }
+
+static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) {
+  const double *c = p->cst;
+  memset(&P, 0, sizeof(P));
+  P.nfric = (int)(c[5] + 0.01);
+  P.if_frictional = 0;
+  P.if_V2byR = 0;
+  if ((P.nfric >= 5 && P.nfric <= 10) || P.nfric == 26 || P.nfric == 27) {
+    P.if_frictional = 1;
+    if (P.nfric == 7 || P.nfric == 8 || P.nfric == 10 || P.nfric == 26 || P.nfric == 27) P.if_V2byR = 1;
+  }
+  const int ok_nfric[] = {1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 24, 25, 26, 27};
+  bool ok = false;
+  for (int v : ok_nfric) ok = ok || (v == P.nfric);
+  if (!ok) {
+    err = "basal law nfric=" + std::to_string(P.nfric) + " is outside the implemented set";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if ((int)(c[16] + 0.01) != 0) {
+    err = "ic_cosTheta (const(16)) must be 0";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (c[20] != 0.0 || c[34] != 0.0) {
+    err = "only ic_fct_dens = 0 and ic_hrelSat = 0 are implemented";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (c[21] > 1e-9 || c[4] > 1e-4 || c[36] > 0.01 || c[43] > 0) {
+    err = "erosion, crushing and end-solid laws are outside the implemented path";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (p->summation_density != 1 || p->nor_density != 0) {
+    err = "summation density without normalisation is required";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (!(p->pa_sph == 1 || p->pa_sph == 2 || p->pa_sph == 4)) {
+    err = "pa_sph must be 1, 2 or 4";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (p->skf < 1 || p->skf > 4) {
+    err = "skf must be 1..4";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (!(p->sle == 0 || p->sle == 1)) {
+    err = "sle must be 0 or 1";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (!(p->icpwp == 0 || p->icpwp == 1 || p->icpwp == 3)) {
+    err = "icpwp must be 0, 1 or 3";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (!(p->integrator == 4 || p->integrator == 2)) {
+    err = "integrator must be 4 (RK4) or 2 (RK2)";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (!(p->dt > 0) || c[15] <= 1e-8 || c[10] <= 1e-8) {
+    err = "dt, comp (const(15)) and hfrict0 (const(10)) must be positive";
+    return GFSPH_ERR_ARG;
+  }
+  if (p->icpwp == 3 && !(c[13] > 0)) {
+    err = "icpwp = 3 needs a positive dissipation rate Bfact (const(13))";
+    return GFSPH_ERR_ARG;
+  }
+  if (P.nfric == 4 && c[6] < 1.e-32) {
+    err = "Bingham yield stress const(6) must exceed 1e-32";
+    return GFSPH_ERR_ARG;
+  }
+  P.icpwp = p->icpwp;
+  P.nuaux = (p->icpwp == 3) ? 1 : 0;
+  P.pa_sph = p->pa_sph;
+  P.skf = p->skf;
+  P.scale_k = (p->skf == 1 || p->skf == 4) ? 2 : 3;
+  P.sle = p->sle;
+  P.av_vel = p->average_velocity;
+  P.npoin0 = p->npoin0;
+  P.dt = p->dt;
+  P.h_inf = p->h_inf;
+  P.cgra = c[1];
+  P.cgra05 = c[1] * 0.5;
+  P.dens = c[2];
+  P.xi_voellmy = c[3];
+  P.constMu_CF = (c[3] < 0) ? -(25. * c[3] / 4.) : 0.0;
+  P.K_Bagnold = c[3];
+  P.tauy0 = c[6];
+  P.constK = c[7];
+  P.visco = c[8];
+  P.tanfi8 = c[9];
+  P.hfrict0 = c[10];
+  P.Bfact = c[13];
+  P.comp = c[15];
+  P.alphaS = c[14];
+  P.hrelpw2 = (double)((float)c[14] * (float)c[14]);
+  P.c73 = 7. / 3.;
+  P.fact_dens = 1.0;
+  P.fact_pwp_max = 1.0;
+  P.fact_pwp_min = 0.0;
+  P.fpwp_max = 1.0;
+  P.fpwp_min = 0.0;
+  P.c3 = c[3];
+  P.c6 = c[6];
+  P.c7 = c[7];
+  P.c8 = c[8];
+  P.c47 = c[47];
+  P.pi = 4.0 * atan(1.0);
+  if (p->integrator == 4) {
+    const double f1[4] = {0., 0.5, 0.5, 1.0}, f2[4] = {1., 2., 2., 1.0};
+    P.nstages = 4;
+    for (int k = 0; k < 4; k++) {
+      P.f1[k] = f1[k];
+      P.f2[k] = f2[k];
+    }
+    P.fin = p->dt / 6.;
+  } else {
+    P.nstages = 2;
+    P.f1[0] = 0.;
+    P.f1[1] = 1.;
+    P.f2[0] = 1.;
+    P.f2[1] = 1.;
+    P.fin = p->dt / 2.;
+  }
+  return GFSPH_OK;
+}
+
+static int alloc_sets(gfsph_ctx *c, int n) {
+  int rc;
+  for (int k = 0; k < 2; k++) {
+    SortedSet &s = c->set[k];
+    if ((rc = dalloc(c, &s.pos, n)) || (rc = dalloc(c, &s.vel, n)) || (rc = dalloc(c, &s.h, n)) ||
+        (rc = dalloc(c, &s.mass, n)) || (rc = dalloc(c, &s.hsml, n)) || (rc = dalloc(c, &s.u, n)) ||
+        (rc = dalloc(c, &s.uaux, n)) || (rc = dalloc(c, &s.flag, n)) || (rc = dalloc(c, &s.id, n)))
+      return rc;
+  }
+  NewSet &o = c->nw;
+  if ((rc = dalloc(c, &o.pos, n)) || (rc = dalloc(c, &o.vel, n)) || (rc = dalloc(c, &o.h, n)) ||
+      (rc = dalloc(c, &o.hsml, n)) || (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) ||
+      (rc = dalloc(c, &o.flag, n)))
+    return rc;
+  if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
+      (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
+    return rc;
+  c->cap_cells = 16 * n + 4096;
+  c->scan_blocks = (c->cap_cells + 2 + SCAN_TILE - 1) / SCAN_TILE;
+  if ((rc = dalloc(c, &c->cnt, (size_t)c->cap_cells + 2)) || (rc = dalloc(c, &c->start, (size_t)c->cap_cells + 2)) ||
+      (rc = dalloc(c, &c->bsum, (size_t)c->scan_blocks)) || (rc = dalloc(c, &c->grid, 1)) ||
+      (rc = dalloc(c, &c->acc, 1)) || (rc = dalloc(c, &c->bad, 1)))
+    return rc;
+  return GFSPH_OK;
+}
+
+static int alloc_bingham(gfsph_ctx *c, int n) {
+  int rc;
+  BinghamState &b = c->bs;
+  c->lv_blocks = (n + LV_TILE - 1) / LV_TILE;
+  if ((rc = dalloc(c, &b.in, n)) || (rc = dalloc(c, &b.x, n)) || (rc = dalloc(c, &b.v, n)) ||
+      (rc = dalloc(c, &b.ex, n)) || (rc = dalloc(c, &b.u, n)) || (rc = dalloc(c, &b.du, n)) ||
+      (rc = dalloc(c, &b.D, 5 * (size_t)n)) || (rc = dalloc(c, &b.fl, n)) || (rc = dalloc(c, &b.cand, n)) ||
+      (rc = dalloc(c, &b.tau, n)) || (rc = dalloc(c, &b.carry, 1)) || (rc = dalloc(c, &b.bsum, c->lv_blocks)))
+    return rc;
+  double zero = 0.0;  // Tau0 before any intermediate-regime particle
+  HIPCHK(c, hipMemcpy(b.carry, &zero, sizeof(double), hipMemcpyHostToDevice));
+  return GFSPH_OK;
+}
+
+// Pressure force and RK stages of one step on the sorted set s.
+static void launch_forces(gfsph_ctx *c, SortedSet &s) {
+  const int n = c->n;
+  hipStream_t st = c->stream;
+  const int nb = nblk(n);
+  if (c->P.nfric != 4) {
+    k_force_rk<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->dem, c->grid, c->start, s, c->S, c->Pp, c->nw, c->acc);
+    return;
+  }
+  BinghamState &b = c->bs;
+  k_bingham_force<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->S, c->Pp, b);
+  for (int k = 0; k < c->P.nstages; k++) {
+    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 0, k, c->P, c->dem, s, c->S, b);
+    k_lv_reduce<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum);
+    k_lv_top<<<1, 64, 0, st>>>(c->lv_blocks, b.bsum, b.carry);
+    k_lv_down<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum, b.tau);
+    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->P, c->dem, s, c->S, b);
+  }
+  k_bingham_finish<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->dem, s, c->S, b, c->nw, c->acc);
+}
+
+// Neighbour build into set[dst] from (nw, set[src]); acc must hold the
+// bounding box of the active particles of nw.
+static int neighbour_build(gfsph_ctx *c, int src, int dst) {
+  const int n = c->n;
+  hipStream_t st = c->stream;
+  const int nb = nblk(n);
+  int clear_blocks = 1024;
+  k_setup_clear<<<clear_blocks, GF_BLOCK, 0, st>>>(c->acc, c->grid, c->cnt, c->P.scale_k, c->cap_cells);
+  k_keys<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.flag, c->grid, c->key, c->rank, c->cnt);
+  k_scan_reduce<<<c->scan_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt, c->bsum);
+  k_scan_top<<<1, GF_BLOCK, 0, st>>>(c->scan_blocks, c->bsum);
+  k_scan_down<<<c->scan_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt, c->bsum, c->start, n);
+  k_scatter<<<nb, GF_BLOCK, 0, st>>>(n, c->key, c->rank, c->start, c->perm);
+  k_cellsort<<<clear_blocks, GF_BLOCK, 0, st>>>(c->grid, c->start, c->perm);
+  k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, c->perm, c->nw, c->set[src], c->set[dst], c->acc);
+  HIPCHK(c, hipGetLastError());
+  return GFSPH_OK;
+}
+
+static int one_step(gfsph_ctx *c) {
+  const int n = c->n;
+  hipStream_t st = c->stream;
+  const int nb = nblk(n);
+  int src = c->cur, dst = 1 - c->cur;
+  int rc = neighbour_build(c, src, dst);
+  if (rc) return rc;
+  SortedSet &s = c->set[dst];
+  k_density<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->S, c->Pp);
+  launch_forces(c, s);
+  if (c->P.av_vel) {
+    k_xsph<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->nw, s.vel);
+  } else {
+    HIPCHK(c, hipMemcpyAsync(s.vel, c->nw.vel, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(c, hipGetLastError());
+  c->cur = dst;
+  return GFSPH_OK;
+}
+
+extern "C" {
+
+int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const double *x, const double *v,
+                 const double *h, const double *mass, const double *hsml, const double *u, const int32_t *itype,
+                 gfsph_ctx **out) {
+  if (!out || !p || !dem || n <= 0 || !x || !v || !h || !mass || !hsml || !dem->Z) return GFSPH_ERR_ARG;
+  *out = nullptr;
+  gfsph_ctx *c = new gfsph_ctx();
+  std::string err;
+  int rc = derive_params(p, c->P, err);
+  if (rc) {
+    c->err = err;
+    *out = c;
+    return rc;
+  }
+  if (dem->nx < 2 || dem->ny < 2 || !(dem->dx > 0) || !(dem->dy > 0)) {
+    *out = c;
+    return fail(c, GFSPH_ERR_ARG, "DEM needs at least 2x2 nodes and positive spacing");
+  }
+  if (itype)
+    for (int i = 0; i < n; i++)
+      if (itype[i] != 1) {
+        *out = c;
+        return fail(c, GFSPH_ERR_UNSUPPORTED, "only single-phase soil particles (itype = 1) are implemented");
+      }
+  c->n = n;
+  c->check_finite = p->check_finite;
+  *out = c;
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if ((rc = alloc_sets(c, n))) return rc;
+  if (c->P.nfric == 4 && (rc = alloc_bingham(c, n))) return rc;
+  hipStream_t st = c->stream;
+
+  // DEM
+  long np = (long)dem->nx * dem->ny;
+  c->dem.nx = dem->nx;
+  c->dem.ny = dem->ny;
+  c->dem.xmin = dem->xmin;
+  c->dem.ymin = dem->ymin;
+  c->dem.dx = dem->dx;
+  c->dem.dy = dem->dy;
+  c->dem.xmax = dem->xmax;
+  c->dem.ymax = dem->ymax;
+  if ((rc = dalloc(c, &c->zxy, np))) return rc;
+  if (c->P.if_frictional)
+    if ((rc = dalloc(c, &c->top, 6 * np))) return rc;
+  if (dem->topol) {
+    double *t9;
+    if ((rc = dalloc(c, &t9, 9 * np))) return rc;
+    HIPCHK(c, hipMemcpyAsync(t9, dem->topol, sizeof(double) * 9 * np, hipMemcpyHostToDevice, st));
+    k_topol_split<<<nblk(np), GF_BLOCK, 0, st>>>(np, t9, c->zxy, c->top);
+  } else {
+    double *Zd;
+    int *im = nullptr;
+    if ((rc = dalloc(c, &Zd, np))) return rc;
+    HIPCHK(c, hipMemcpyAsync(Zd, dem->Z, sizeof(double) * np, hipMemcpyHostToDevice, st));
+    if (dem->in_model) {
+      if ((rc = dalloc(c, &im, np))) return rc;
+      HIPCHK(c, hipMemcpyAsync(im, dem->in_model, sizeof(int) * np, hipMemcpyHostToDevice, st));
+    }
+    k_fdiff<<<nblk(np), GF_BLOCK, 0, st>>>(dem->nx, dem->ny, dem->dx, dem->dy, Zd, im, c->zxy, c->top);
+  }
+  c->dem.zxy = c->zxy;
+  c->dem.top = c->top;
+
+  // particles -> set[0] in caller order; the first neighbour build sorts them
+  SortedSet &s0 = c->set[0];
+  std::vector<int> ids(n), zeros(n, 0);
+  for (int i = 0; i < n; i++) ids[i] = i;
+  std::vector<double> zd(n, 0.0);
+  HIPCHK(c, hipMemcpyAsync(s0.pos, x, sizeof(double2) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.vel, v, sizeof(double2) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.h, h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.mass, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.hsml, hsml, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.u, u ? u : zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.uaux, zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.flag, zeros.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(s0.id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  const int nb = nblk(n);
+  k_domain_flag<<<nb, GF_BLOCK, 0, st>>>(n, c->dem, s0.pos, s0.flag);
+  k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw);
+  k_bbox_reset<<<1, 64, 0, st>>>(c->acc);
+  k_bbox<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->acc);
+  if ((rc = neighbour_build(c, 0, 1))) return rc;
+  c->cur = 1;
+  SortedSet &s1 = c->set[1];
+  int resum = (p->init_input && p->icpwp != 0 && p->summation_density) ? 1 : 0;
+  if (resum) k_density<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s1, c->S, c->Pp);
+  if (p->init_input) k_init_pwp<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->P.alphaS, p->pwp0rel, resum, c->S, s1);
+  // the step begins by re-sorting from the "new" set: publish the state there
+  k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s1, c->nw);
+  k_bbox<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->acc);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  return GFSPH_OK;
+}
+
+int gfsph_step(gfsph_ctx *c, int32_t nsteps) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  if (nsteps < 0) return fail(c, GFSPH_ERR_ARG, "nsteps < 0");
+  for (int k = 0; k < nsteps; k++) {
+    int rc = one_step(c);
+    if (rc) return rc;
+  }
+  if (c->check_finite) {
+    HIPCHK(c, hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+    k_nonfinite<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, cur_view(c), c->bad);
+    int bad = 0;
+    HIPCHK(c, hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bad) return fail(c, GFSPH_ERR_NONFINITE, "non-finite particle state");
+  }
+  return GFSPH_OK;
+}
+
+int gfsph_sync(gfsph_ctx *c) {
+  if (!c) return GFSPH_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFSPH_OK;
+}
+
+int gfsph_get_state(gfsph_ctx *c, double *x, double *v, double *h, double *u, double *uaux, double *hsml,
+                    int32_t *flag) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  const int n = c->n;
+  double2 *dx, *dv;
+  double *dh, *du, *dua, *dhs;
+  int *df;
+  // scratch: reuse the "new" set arrays after a full sync
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<void *> tmp;
+  auto A = [&](void **p, size_t b) -> hipError_t {
+    hipError_t e = hipMalloc(p, b);
+    if (e == hipSuccess) tmp.push_back(*p);
+    return e;
+  };
+  hipError_t e = hipSuccess;
+  e = A((void **)&dx, sizeof(double2) * n);
+  if (!e) e = A((void **)&dv, sizeof(double2) * n);
+  if (!e) e = A((void **)&dh, sizeof(double) * n);
+  if (!e) e = A((void **)&du, sizeof(double) * n);
+  if (!e) e = A((void **)&dua, sizeof(double) * n);
+  if (!e) e = A((void **)&dhs, sizeof(double) * n);
+  if (!e) e = A((void **)&df, sizeof(int) * n);
+  if (!e) {
+    k_unsort<<<nblk(n), GF_BLOCK, 0, c->stream>>>(n, cur_view(c), dx, dv, dh, du, dua, dhs, df);
+    e = hipGetLastError();
+  }
+  if (!e && x) e = hipMemcpyAsync(x, dx, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e && v) e = hipMemcpyAsync(v, dv, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e && h) e = hipMemcpyAsync(h, dh, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e && u) e = hipMemcpyAsync(u, du, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e && uaux) e = hipMemcpyAsync(uaux, dua, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e && hsml) e = hipMemcpyAsync(hsml, dhs, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e && flag) e = hipMemcpyAsync(flag, df, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  for (void *q : tmp) (void)hipFree(q);
+  if (e) return fail(c, GFSPH_ERR_HIP, std::string("get_state: ") + hipGetErrorString(e));
+  return GFSPH_OK;
+}
+
+int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double *hsml, const int32_t *flag,
+                          int64_t cap, int64_t *npairs, int32_t *ij) {
+  if (n <= 0 || !x || !hsml || !npairs || skf < 1 || skf > 4 || cap < 0 || (cap > 0 && !ij)) return GFSPH_ERR_ARG;
+  gfsph_ctx *c = new gfsph_ctx();
+  c->n = n;
+  c->P.skf = skf;
+  c->P.scale_k = (skf == 1 || skf == 4) ? 2 : 3;
+  int rc = GFSPH_OK;
+  std::vector<int> ids(n), fl(n, 0);
+  for (int i = 0; i < n; i++) {
+    ids[i] = i;
+    if (flag) fl[i] = flag[i] ? 1 : 0;
+  }
+  std::vector<double> zd(n, 0.0);
+  unsigned long long *dcount = nullptr;
+  int *dij = nullptr;
+  unsigned long long count = 0;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e) rc = GFSPH_ERR_HIP;
+  if (!rc) rc = alloc_sets(c, n);
+  if (!rc) rc = dalloc(c, &dcount, 1);
+  if (!rc) rc = dalloc(c, &dij, 2 * (cap > 0 ? cap : 1));
+  if (!rc) {
+    hipStream_t st = c->stream;
+    SortedSet &s0 = c->set[0];
+    e = hipMemcpyAsync(s0.pos, x, sizeof(double2) * n, hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemcpyAsync(s0.hsml, hsml, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemcpyAsync(s0.flag, fl.data(), sizeof(int) * n, hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemcpyAsync(s0.id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st);
+    for (auto arr : {s0.h, s0.mass, s0.u, s0.uaux})
+      if (!e) e = hipMemcpyAsync(arr, zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemcpyAsync(s0.vel, zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemsetAsync(dcount, 0, sizeof(unsigned long long), st);
+    if (e) rc = GFSPH_ERR_HIP;
+    if (!rc) {
+      const int nb = nblk(n);
+      k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw);
+      k_bbox_reset<<<1, 64, 0, st>>>(c->acc);
+      k_bbox<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->acc);
+      rc = neighbour_build(c, 0, 1);
+    }
+    if (!rc) {
+      k_pairs<<<nblk(n), GF_BLOCK, 0, st>>>(n, skf, c->P.scale_k, c->grid, c->start, c->set[1], cap, dcount, dij);
+      e = hipMemcpyAsync(&count, dcount, sizeof(count), hipMemcpyDeviceToHost, st);
+      if (!e) e = hipStreamSynchronize(st);
+      if (e) rc = GFSPH_ERR_HIP;
+    }
+    if (!rc) {
+      *npairs = (int64_t)count;
+      if ((int64_t)count > cap) {
+        rc = GFSPH_ERR_CAPACITY;
+      } else if (count > 0) {
+        e = hipMemcpy(ij, dij, sizeof(int) * 2 * count, hipMemcpyDeviceToHost);
+        if (e) rc = GFSPH_ERR_HIP;
+      }
+    }
+  }
+  gfsph_destroy(c);
+  return rc;
+}
+
+int gfsph_raster_idw(gfsph_ctx *c, double *out, double *wsum) {
+  if (!c || !out || !wsum) return GFSPH_ERR_ARG;
+  long np = (long)c->dem.nx * c->dem.ny;
+  int rc;
+  if (!c->raster5 && (rc = dalloc(c, &c->raster5, 5 * np + 5 * np))) return rc;
+  double *acc5 = c->raster5, *nrm = c->raster5 + 5 * np;
+  HIPCHK(c, hipMemsetAsync(acc5, 0, sizeof(double) * 5 * np, c->stream));
+  k_raster_idw<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), acc5);
+  k_raster_norm<<<nblk(np), GF_BLOCK, 0, c->stream>>>(np, acc5, nrm, nrm + 4 * np);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, nrm, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(wsum, nrm + 4 * np, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFSPH_OK;
+}
+
+int gfsph_mc_pixel_accumulate(gfsph_ctx *c) {
+  if (!c) return GFSPH_ERR_ARG;
+  long np = (long)c->dem.nx * c->dem.ny;
+  int rc;
+  if (!c->mc_acc) {
+    if ((rc = dalloc(c, &c->mc_acc, 4 * np)) || (rc = dalloc(c, &c->mc_cnt, np))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->mc_acc, 0, sizeof(double) * 4 * np, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mc_cnt, 0, sizeof(int) * np, c->stream));
+  }
+  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), c->mc_acc, c->mc_cnt);
+  HIPCHK(c, hipGetLastError());
+  return GFSPH_OK;
+}
+
+int gfsph_mc_pixel_read(gfsph_ctx *c, double *acc, int32_t *cnt, int32_t reset) {
+  if (!c || !acc || !cnt) return GFSPH_ERR_ARG;
+  long np = (long)c->dem.nx * c->dem.ny;
+  if (!c->mc_acc) {
+    memset(acc, 0, sizeof(double) * 4 * np);
+    memset(cnt, 0, sizeof(int) * np);
+    return GFSPH_OK;
+  }
+  HIPCHK(c, hipMemcpyAsync(acc, c->mc_acc, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cnt, c->mc_cnt, sizeof(int) * np, hipMemcpyDeviceToHost, c->stream));
+  if (reset) {
+    HIPCHK(c, hipMemsetAsync(c->mc_acc, 0, sizeof(double) * 4 * np, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mc_cnt, 0, sizeof(int) * np, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFSPH_OK;
+}
+
+int gfsph_profile_step(gfsph_ctx *c, int32_t nrep, int32_t cap, int32_t *nkernels, const char **names, double *ms) {
+  if (!c || nrep <= 0 || !nkernels) return GFSPH_ERR_ARG;
+  static const char *kn[] = {"neighbour_build", "k_density", "k_force_rk", "k_xsph", "step_total"};
+  const int K = 5;
+  *nkernels = K;
+  if (cap < K || !names || !ms) return GFSPH_ERR_CAPACITY;
+  hipEvent_t ev[6];
+  for (int k = 0; k < 6; k++) HIPCHK(c, hipEventCreate(&ev[k]));
+  double acc[5] = {0, 0, 0, 0, 0};
+  const int n = c->n;
+  const int nb = nblk(n);
+  for (int r = 0; r < nrep; r++) {
+    hipStream_t st = c->stream;
+    int src = c->cur, dst = 1 - c->cur;
+    HIPCHK(c, hipEventRecord(ev[0], st));
+    int rc = neighbour_build(c, src, dst);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(ev[1], st));
+    SortedSet &s = c->set[dst];
+    k_density<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->S, c->Pp);
+    HIPCHK(c, hipEventRecord(ev[2], st));
+    launch_forces(c, s);
+    HIPCHK(c, hipEventRecord(ev[3], st));
+    if (c->P.av_vel) k_xsph<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->nw, s.vel);
+    else HIPCHK(c, hipMemcpyAsync(s.vel, c->nw.vel, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(c, hipEventRecord(ev[4], st));
+    HIPCHK(c, hipEventRecord(ev[5], st));
+    c->cur = dst;
+    HIPCHK(c, hipEventSynchronize(ev[5]));
+    float t;
+    for (int k = 0; k < 4; k++) {
+      HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      acc[k] += t;
+    }
+    HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[5]));
+    acc[4] += t;
+  }
+  for (int k = 0; k < 6; k++) (void)hipEventDestroy(ev[k]);
+  for (int k = 0; k < K; k++) {
+    names[k] = kn[k];
+    ms[k] = acc[k] / nrep;
+  }
+  return GFSPH_OK;
+}
+
+int gfsph_counts(gfsph_ctx *c, int32_t *n, int32_t *nactive) {
+  if (!c) return GFSPH_ERR_ARG;
+  if (n) *n = c->n;
+  if (nactive) {
+    BBoxAcc a;
+    HIPCHK(c, hipMemcpyAsync(&a, c->acc, sizeof(a), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *nactive = (int32_t)a.nactive;
+  }
+  return GFSPH_OK;
+}
+
+const char *gfsph_last_error(gfsph_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void gfsph_destroy(gfsph_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,11 @@
+"""gfsph: MI355X-native Geoflow-SPH propagation step (host side).
+
+The compute path is the HIP library libgfsph.so (csrc/gfsph.hip) behind the
+C ABI declared in include/gfsph.h; this package builds it, mirrors the
+reference's Init_sph / time_integration_sph verbs over it (api.SPHModel),
+and generates the synthetic DEMs and particle releases of the workloads
+(dem, cases).
+"""
+from . import cases, dem  # noqa: F401
+from ._build import LIB, build  # noqa: F401
+from .api import GfsphError, SPHModel, load_library, neighbour_pairs  # noqa: F401

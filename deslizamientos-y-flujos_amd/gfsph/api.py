@@ -1,0 +1,225 @@
+"""Host-side mirror of the reference's SPH_MAIN_2019 interface over the C ABI.
+
+The reference drives one simulation through module procedures with shared
+module state ("7 SPH_MAIN_2019.f90":47-61): ``Init_sph`` reads inputs and
+allocates, ``time_integration_sph`` advances one step, ``Out_plot_sph`` and
+``Manage_pixel_Hazard_SPH`` produce rasters.  ``SPHModel`` keeps the same
+verbs over one ``gfsph_ctx`` (include/gfsph.h); all computation runs in the
+HIP library, and a missing library is an error, never a CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._build import LIB
+
+__all__ = ["GfsphError", "SPHModel", "neighbour_pairs", "load_library", "EXPORTED_SYMBOLS"]
+
+EXPORTED_SYMBOLS = (
+    "gfsph_create", "gfsph_step", "gfsph_sync", "gfsph_get_state", "gfsph_neighbour_pairs",
+    "gfsph_raster_idw", "gfsph_mc_pixel_accumulate", "gfsph_mc_pixel_read", "gfsph_profile_step",
+    "gfsph_counts", "gfsph_last_error", "gfsph_destroy",
+)
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
+          -4: "non-finite state", -5: "capacity exceeded"}
+
+
+class GfsphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gfsph status {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class CParams(C.Structure):
+    _fields_ = [("integrator", C.c_int32), ("pa_sph", C.c_int32), ("nnps", C.c_int32), ("sle", C.c_int32),
+                ("skf", C.c_int32), ("summation_density", C.c_int32), ("average_velocity", C.c_int32),
+                ("nor_density", C.c_int32), ("icpwp", C.c_int32), ("npoin0", C.c_int32),
+                ("init_input", C.c_int32), ("check_finite", C.c_int32),
+                ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double), ("cst", C.c_double * 111)]
+
+
+class CDem(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("xmin", C.c_double), ("ymin", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("xmax", C.c_double), ("ymax", C.c_double),
+                ("Z", C.POINTER(C.c_double)), ("in_model", C.POINTER(C.c_int32)),
+                ("topol", C.POINTER(C.c_double))]
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Loads libgfsph.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(p)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_void_p
+    L.gfsph_create.argtypes = [C.POINTER(CParams), C.POINTER(CDem), C.c_int32, dp, dp, dp, dp, dp, dp, ip,
+                               C.POINTER(vp)]
+    L.gfsph_step.argtypes = [vp, C.c_int32]
+    L.gfsph_sync.argtypes = [vp]
+    L.gfsph_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp, ip]
+    L.gfsph_neighbour_pairs.argtypes = [C.c_int32, C.c_int32, dp, dp, ip, C.c_int64, C.POINTER(C.c_int64), ip]
+    L.gfsph_raster_idw.argtypes = [vp, dp, dp]
+    L.gfsph_mc_pixel_accumulate.argtypes = [vp]
+    L.gfsph_mc_pixel_read.argtypes = [vp, dp, ip, C.c_int32]
+    L.gfsph_profile_step.argtypes = [vp, C.c_int32, C.c_int32, ip, C.POINTER(C.c_char_p), dp]
+    L.gfsph_counts.argtypes = [vp, ip, ip]
+    L.gfsph_last_error.argtypes = [vp]
+    L.gfsph_last_error.restype = C.c_char_p
+    L.gfsph_destroy.argtypes = [vp]
+    L.gfsph_destroy.restype = None
+    if path is None:
+        _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def make_cparams(prm, init_input: bool = True, check_finite: bool = False) -> CParams:
+    p = CParams()
+    p.integrator, p.pa_sph, p.nnps, p.sle, p.skf = prm.integrator, prm.pa_sph, prm.nnps, prm.sle, prm.skf
+    p.summation_density, p.average_velocity, p.nor_density = (prm.summation_density, prm.average_velocity,
+                                                              prm.nor_density)
+    p.icpwp, p.npoin0 = prm.icpwp, prm.npoin0
+    p.init_input, p.check_finite = int(init_input), int(check_finite)
+    p.dt, p.h_inf, p.pwp0rel = prm.dt, prm.h_inf, prm.pwp0rel
+    c = prm.cst()
+    for k in range(111):
+        p.cst[k] = c[k]
+    return p
+
+
+class SPHModel:
+    """One propagation run on the GPU (Init_sph + time_integration_sph)."""
+
+    def __init__(self, case, init_input: bool = True, check_finite: bool = False):
+        self.lib = load_library()
+        self.case = case
+        P = case.particles
+        self.n = P.n
+        dem = case.dem
+        self._Z = np.ascontiguousarray(dem.Z, dtype=np.float64)
+        self._im = None if dem.in_model is None else np.ascontiguousarray(dem.in_model, dtype=np.int32)
+        d = CDem()
+        d.nx, d.ny = dem.nx, dem.ny
+        d.xmin, d.ymin, d.dx, d.dy, d.xmax, d.ymax = dem.xmin, dem.ymin, dem.dx, dem.dy, dem.xmax, dem.ymax
+        d.Z = _dp(self._Z)
+        d.in_model = _ip(self._im) if self._im is not None else C.POINTER(C.c_int32)()
+        d.topol = C.POINTER(C.c_double)()
+        p = make_cparams(case.params, init_input, check_finite)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (P.x, P.v, P.h, P.mass, P.hsml, P.u)]
+        it = np.ascontiguousarray(P.itype, dtype=np.int32)
+        ctx = C.c_void_p()
+        rc = self.lib.gfsph_create(C.byref(p), C.byref(d), self.n, *(_dp(a) for a in arrs), _ip(it), C.byref(ctx))
+        self.ctx = ctx
+        if rc != 0:
+            msg = self.lib.gfsph_last_error(ctx).decode() if ctx.value else ""
+            self.close()
+            raise GfsphError(rc, msg)
+        self.nsteps_done = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GfsphError(rc, self.lib.gfsph_last_error(self.ctx).decode())
+
+    # --- reference verbs ---------------------------------------------
+    def time_integration_sph(self, nsteps: int = 1) -> None:
+        self._check(self.lib.gfsph_step(self.ctx, int(nsteps)))
+        self.nsteps_done += int(nsteps)
+
+    def sync(self) -> None:
+        self._check(self.lib.gfsph_sync(self.ctx))
+
+    def state(self) -> dict:
+        n = self.n
+        out = {"x": np.zeros((n, 2)), "v": np.zeros((n, 2)), "h": np.zeros(n), "u": np.zeros(n),
+               "uaux": np.zeros(n), "hsml": np.zeros(n), "flag": np.zeros(n, np.int32)}
+        self._check(self.lib.gfsph_get_state(self.ctx, _dp(out["x"]), _dp(out["v"]), _dp(out["h"]), _dp(out["u"]),
+                                             _dp(out["uaux"]), _dp(out["hsml"]), _ip(out["flag"])))
+        return out
+
+    def raster_idw(self):
+        npg = self.case.dem.nx * self.case.dem.ny
+        out = np.zeros((npg, 4))
+        ws = np.zeros(npg)
+        self._check(self.lib.gfsph_raster_idw(self.ctx, _dp(out), _dp(ws)))
+        return out, ws
+
+    def mc_pixel_accumulate(self) -> None:
+        self._check(self.lib.gfsph_mc_pixel_accumulate(self.ctx))
+
+    def mc_pixel_read(self, reset: bool = False):
+        npg = self.case.dem.nx * self.case.dem.ny
+        acc = np.zeros((npg, 4))
+        cnt = np.zeros(npg, np.int32)
+        self._check(self.lib.gfsph_mc_pixel_read(self.ctx, _dp(acc), _ip(cnt), int(reset)))
+        return acc, cnt
+
+    def profile_step(self, nrep: int = 10) -> dict:
+        cap = 16
+        nk = C.c_int32()
+        names = (C.c_char_p * cap)()
+        ms = np.zeros(cap)
+        self._check(self.lib.gfsph_profile_step(self.ctx, int(nrep), cap, C.byref(nk), names, _dp(ms)))
+        self.nsteps_done += int(nrep)
+        return {names[k].decode(): float(ms[k]) for k in range(nk.value)}
+
+    def counts(self):
+        n, na = C.c_int32(), C.c_int32()
+        self._check(self.lib.gfsph_counts(self.ctx, C.byref(n), C.byref(na)))
+        return n.value, na.value
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.lib.gfsph_destroy(self.ctx)
+        self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def neighbour_pairs(x, hsml, flag=None, skf: int = 1) -> np.ndarray:
+    """Neighbour set (i<j, 0-based, sorted) computed on the GPU."""
+    L = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    hsml = np.ascontiguousarray(hsml, dtype=np.float64)
+    n = x.shape[0]
+    fl = np.zeros(n, np.int32) if flag is None else np.ascontiguousarray(flag, dtype=np.int32)
+    cap = max(16, 40 * n)
+    for _ in range(3):
+        ij = np.zeros((cap, 2), np.int32)
+        npairs = C.c_int64()
+        rc = L.gfsph_neighbour_pairs(skf, n, _dp(x), _dp(hsml), _ip(fl), cap, C.byref(npairs), _ip(ij))
+        if rc == -5:
+            cap = int(npairs.value) + 16
+            continue
+        if rc != 0:
+            raise GfsphError(rc, "gfsph_neighbour_pairs")
+        ij = ij[: npairs.value]
+        order = np.lexsort((ij[:, 1], ij[:, 0]))
+        return ij[order]
+    raise GfsphError(-5, "pair capacity")

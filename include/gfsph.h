@@ -1,0 +1,132 @@
+/*
+ * gfsph.h -- C ABI of the MI355X-native Geoflow-SPH propagation step.
+ *
+ * One FP64 depth-integrated SPH time step on one GPU, as the reference's
+ * time_integration_sph / RK45 perform it ("7 SPH_MAIN_2019.f90":220-283,
+ * 746-1005).  Plain pointers and sizes only: a Fortran driver binds it with
+ * ISO_C_BINDING, Python with ctypes.
+ *
+ * Array conventions.  Particle arrays are in the caller's particle order.
+ * Two-component arrays are [n][2] (x then y), which is the memory layout of
+ * the reference's Fortran x(ndimn, npoin) / vx(ndimn, npoin) arrays
+ * ("5 SPH_Main_Vars_2019.f90":29-35), so they pass unchanged.  Pair indices
+ * are 0-based and i < j.
+ *
+ * Every entry point returns GFSPH_OK (0) or a negative status; the message of
+ * the last failure on a context is returned by gfsph_last_error().  A context
+ * is used by one host thread at a time.
+ */
+#ifndef GFSPH_H
+#define GFSPH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFSPH_OK 0
+#define GFSPH_ERR_ARG (-1)         /* invalid argument or inconsistent sizes      */
+#define GFSPH_ERR_UNSUPPORTED (-2) /* option outside the implemented path         */
+#define GFSPH_ERR_HIP (-3)         /* HIP runtime failure                         */
+#define GFSPH_ERR_NONFINITE (-4)   /* NaN/Inf detected in the particle state      */
+#define GFSPH_ERR_CAPACITY (-5)    /* output buffer too small (size reported)     */
+
+typedef struct gfsph_ctx gfsph_ctx;
+
+/* Solver switches and material constants.  Replaces the module state that
+ * the reference reads from SPH_Main_Vars_2019 / SPH_SW_Vars_2019 / the time
+ * variables ("5 SPH_Main_Vars_2019.f90":85-122, "4 SPH_SW_Vars_2019.f90":27,
+ * "3 SPH_FEM_Driver_time_vars_2019.f90":26-37).  cst[k] is const(k),
+ * k = 1..110, holding the REAL(4) value the reference reads (widened). */
+typedef struct gfsph_params {
+  int32_t integrator;        /* 4: RK4 (SPH_t_Integ_Alg=4); 2: RK2 (Heun)     */
+  int32_t pa_sph;            /* 1, 2 or 4                                      */
+  int32_t nnps;              /* accepted: 1..4 (all give the same pair set)    */
+  int32_t sle;               /* 0 or 1                                         */
+  int32_t skf;               /* 1 cubic spline, 2 Gauss, 3 quintic, 4 Wendland */
+  int32_t summation_density; /* must be 1                                      */
+  int32_t average_velocity;  /* XSPH on/off                                    */
+  int32_t nor_density;       /* must be 0                                      */
+  int32_t icpwp;             /* 0, 1 or 3                                      */
+  int32_t npoin0;            /* particles [0, npoin0) are skipped by XSPH      */
+  int32_t init_input;        /* 1: apply Input_SW initialisation at create     */
+  int32_t check_finite;      /* 1: gfsph_step fails with GFSPH_ERR_NONFINITE   */
+  double dt;                 /* dt_sph                                         */
+  double h_inf;              /* h_inf_SW                                       */
+  double pwp0rel;            /* initial pore-pressure ratio                    */
+  double cst[111];           /* const(1..110), index 0 unused                  */
+} gfsph_params;
+
+/* Regular DEM grid ("2 SPH_FEM_topo_2019.f90":94-110, 223-252).  Z is
+ * [ny][nx] (node ipoig = iy*nx+ix).  If topol is NULL the nodal slopes and
+ * curvature terms are derived on the device by central differences
+ * (Fdiff_derivs, 2:2812-2877); otherwise topol is [ny*nx][9] holding
+ * Z, Zx, Zy, 1+Zx^2, ZxZy, 1+Zy^2, Zxx/rt, Zxy/rt, Zyy/rt. */
+typedef struct gfsph_dem {
+  int32_t nx, ny;
+  double xmin, ymin, dx, dy; /* Xming, Yming, deltxg, deltyg                   */
+  double xmax, ymax;         /* Xmaxg, Ymaxg as read                           */
+  const double *Z;           /* [ny*nx]                                        */
+  const int32_t *in_model;   /* [ny*nx] 0/1 or NULL (all in model)             */
+  const double *topol;       /* [ny*nx][9] or NULL                             */
+} gfsph_dem;
+
+/* Uploads the DEM and the particle set and, if p->init_input, performs the
+ * Input_SW initialisation: summed depth from the initial positions and pore
+ * pressure u = pwp0rel*h*const(14) ("6 SPH_SW_2019.f90":1428-1452,
+ * 1705-1852).  Replaces Init_sph ("7 SPH_MAIN_2019.f90":74) and
+ * Setup_Global_Arrays_SW ("6 SPH_SW_2019.f90":6223).  itype must be all 1
+ * (single-phase soil). */
+int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const double *x,
+                 const double *v, const double *h, const double *mass, const double *hsml,
+                 const double *u, const int32_t *itype, gfsph_ctx **out);
+
+/* nsteps calls of time_integration_sph ("7 SPH_MAIN_2019.f90":220-283),
+ * i.e. the Driver_Flow step loop body ("8 SPH_FEM_Driver_2019.f90":1614-1620)
+ * with fixed dt.  Asynchronous unless check_finite is set. */
+int gfsph_step(gfsph_ctx *ctx, int32_t nsteps);
+
+/* Waits for all work queued on the context's stream. */
+int gfsph_sync(gfsph_ctx *ctx);
+
+/* Copies the state back in the caller's particle order (any pointer may be
+ * NULL).  h is the depth ("rho"), uaux the bottom pore pressure Uaux(1,:). */
+int gfsph_get_state(gfsph_ctx *ctx, double *x, double *v, double *h, double *u, double *uaux,
+                    double *hsml, int32_t *out_flag);
+
+/* Neighbour set of grid_find_NEW_tools3 ("5 a My_SPH_Tools_2019.f90":802-1104)
+ * for the given positions: pairs (i<j) with r < scale_k*(hsml_i+hsml_j)/2.
+ * Stateless; used by the parity tests and the Pint_Update export.  On
+ * GFSPH_ERR_CAPACITY *npairs holds the required size. */
+int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double *hsml,
+                          const int32_t *flag, int64_t cap, int64_t *npairs, int32_t *ij);
+
+/* Particle-to-DEM-node inverse-distance raster of OutputRes_WIR
+ * ("6 SPH_SW_2019.f90":14312-14466): out[ny*nx][4] = h, vx, vy, u/h,
+ * wsum[ny*nx] = sum of 1/d. */
+int gfsph_raster_idw(gfsph_ctx *ctx, double *out, double *wsum);
+
+/* Monte Carlo pixel accumulation of Get_MC_pixel_Vars_SW
+ * ("6 SPH_SW_2019.f90":20717-20833) into the context's device rasters:
+ * sum h, max h, sum |v|, max |v| and the visit count of the 4 nodes of each
+ * particle's DEM cell.  gfsph_mc_pixel_read copies them out and, if reset,
+ * clears them. */
+int gfsph_mc_pixel_accumulate(gfsph_ctx *ctx);
+int gfsph_mc_pixel_read(gfsph_ctx *ctx, double *acc /*[npoig][4]*/, int32_t *cnt, int32_t reset);
+
+/* Average device time per launch of each step kernel over nrep steps, in
+ * milliseconds, measured with HIP events on the context's stream.
+ * names/ms are arrays of *nkernels entries (at most cap). */
+int gfsph_profile_step(gfsph_ctx *ctx, int32_t nrep, int32_t cap, int32_t *nkernels,
+                       const char **names, double *ms);
+
+/* Number of particles and of active (in-domain) particles. */
+int gfsph_counts(gfsph_ctx *ctx, int32_t *n, int32_t *nactive);
+
+const char *gfsph_last_error(gfsph_ctx *ctx);
+void gfsph_destroy(gfsph_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

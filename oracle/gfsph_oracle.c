@@ -407,6 +407,8 @@ static double self_w(int skf, double hsml) {
   return w;
 }
 
+#define PAIR_K(kk) (p->rev_order ? pv->n - 1 - (kk) : (kk))
+
 static void sum_density(const gfo_params *p, int n, const pairvec *pv, const double *pw,
                         const double *mass, const double *hsml, const int *flag, double *rho) {
   double *wi = NULL;
@@ -414,7 +416,8 @@ static void sum_density(const gfo_params *p, int n, const pairvec *pv, const dou
     wi = (double *)calloc(n, sizeof(double));
     for (int i = 0; i < n; i++)
       if (!flag[i]) wi[i] = self_w(p->skf, hsml[i]) * mass[i] / rho[i];
-    for (int64_t k = 0; k < pv->n; k++) {
+    for (int64_t kk = 0; kk < pv->n; kk++) {
+      int64_t k = PAIR_K(kk);
       int i = pv->i[k], j = pv->j[k];
       wi[i] = wi[i] + mass[j] / rho[j] * pw[k];
       wi[j] = wi[j] + mass[i] / rho[i] * pw[k];
@@ -422,7 +425,8 @@ static void sum_density(const gfo_params *p, int n, const pairvec *pv, const dou
   }
   for (int i = 0; i < n; i++)
     if (!flag[i]) rho[i] = self_w(p->skf, hsml[i]) * mass[i];
-  for (int64_t k = 0; k < pv->n; k++) {
+  for (int64_t kk = 0; kk < pv->n; kk++) {
+    int64_t k = PAIR_K(kk);
     int i = pv->i[k], j = pv->j[k];
     rho[i] = rho[i] + mass[j] * pw[k];
     rho[j] = rho[j] + mass[i] * pw[k];
@@ -444,9 +448,15 @@ typedef struct {
   double ex, ey, du;
 } extout;
 
+/* tau0 carries the Bingham Tau0 between particles: ExtForces_SW applies
+   cf4b = Tau0/dens after every nfric=4 branch (6:9619-9622) but sets Tau0
+   only in the intermediate regime (6:9616), so the viscous and cohesive
+   regimes use the Tau0 of the last particle, in particle order, that was in
+   the intermediate regime.  The compiled reference behaves this way (checked
+   against its dumps); the value carries across calls, starting from 0. */
 static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem *dem, double xx,
                           double yy, double vx1, double vy1, double rho_i, double *u_i,
-                          double uaux1, double inx, double iny, extout *o) {
+                          double uaux1, double inx, double iny, double *tau0, extout *o) {
   const double dt = p->dt;
   double h, hu, hv, hinv, vmod, hvmod, hvmodinv;
   double vxx = vx1, vyy = vy1;
@@ -499,30 +509,20 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
     fricty = cf2a * hv / pow(hfrict, 2.);
     TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
   } else if (nfric == 4 || nfric == 24 || nfric == 25) { /* 6:9538-9623 */
-    double Tauy0 = D->tauy0, constK = D->constK, m_vexp = D->visco, Tau0 = 0.0;
+    double Tauy0 = D->tauy0, m_vexp = D->visco, Tau0 = *tau0;
     double constMu_CF = D->constK;
     if (nfric == 4) {
+      double constK = D->constK;
       double a_hat = (6. * constK * hvmod) / (Tauy0 * pow(hfrict, 2.));
-      if (a_hat > 50) {
-        double cf4b = 3 * constK / D->dens;
-        frictx = cf4b * hu / (hfrict * hfrict);
-        fricty = cf4b * hv / (hfrict * hfrict);
-      } else if (a_hat < 0.01) {
-        double cf4b = Tauy0 / D->dens;
-        frictx = cf4b * hu * hvmodinv;
-        fricty = cf4b * hv * hvmodinv;
-      } else {
+      if (!(a_hat > 50) && !(a_hat < 0.01)) { /* intermediate regime */
         double a2 = 48., b2 = -32. * a_hat - 114., c2 = 65.;
         double xh = (-b2 - pow(b2 * b2 - 4. * a2 * c2, 0.5)) / (2. * a2);
         Tau0 = Tauy0 / xh;
-        double cf4b = Tau0 / D->dens;
-        frictx = cf4b * hu * hvmodinv;
-        fricty = cf4b * hv * hvmodinv;
+        *tau0 = Tau0;
       }
-      /* The reference then re-applies Tau0/dens*hu*hvmodinv unconditionally
-         (6:9619-9622), reading an unset Tau0 in the viscous/cohesive limits.
-         The branch results above are kept for those limits; in the
-         intermediate regime the two are identical. */
+      double cf4b = Tau0 / D->dens; /* 6:9619-9622, every regime */
+      frictx = cf4b * hu * hvmodinv;
+      fricty = cf4b * hv * hvmodinv;
     } else {
       Tau0 = constMu_CF * pow(vmod / hfrict, m_vexp);
       double Cm = (nfric == 24) ? 1. : pow((2 * m_vexp + 1) / m_vexp, m_vexp);
@@ -711,7 +711,7 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
 /* ------------------------------------------------------------------ */
 int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,
             const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
-            int nsteps) {
+            int nsteps, double *tau0_io) {
   derived_t D;
   derive(p, &D);
   const double dt = p->dt;
@@ -733,6 +733,7 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
   pairvec pv = {0, 0, NULL, NULL};
   double *pw = NULL, *pdw = NULL;
   int64_t pcap = 0;
+  double tau0_carry = tau0_io ? *tau0_io : 0.0;
 
   for (int step = 0; step < nsteps; step++) {
     /* time_integration_sph: Check_Out_Domain before RK45 (7:232) */
@@ -777,7 +778,8 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
     double cgra05 = D.cgra * 0.5;
     for (int i = 0; i < n; i++)
       if (!flag[i]) p_[i] = cgra05 * rhoF[i] * rhoF[i];
-    for (int64_t k = 0; k < pv.n; k++) {
+    for (int64_t kk = 0; kk < pv.n; kk++) {
+      int64_t k = p->rev_order ? pv.n - 1 - kk : kk;
       int i = pv.i[k], j = pv.j[k];
       double mI = mass[i], mJ = mass[j];
       for (int d = 0; d < 2; d++) {
@@ -823,7 +825,7 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
         if (flag[i]) continue;
         extout o;
         ext_force_one(p, &D, dem, x[2 * i], x[2 * i + 1], v[2 * i], v[2 * i + 1], rho[i], &u[i],
-                      nuaux ? uaux[i] : 0.0, inf[2 * i], inf[2 * i + 1], &o);
+                      nuaux ? uaux[i] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, &o);
         ex[2 * i] = o.ex;
         ex[2 * i + 1] = o.ey;
         du[i] = o.du;
@@ -885,7 +887,8 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
     }
     if (p->average_velocity) {
       memset(av, 0, sizeof(double) * 2 * n);
-      for (int64_t k = 0; k < pv.n; k++) {
+      for (int64_t kk = 0; kk < pv.n; kk++) {
+        int64_t k = p->rev_order ? pv.n - 1 - kk : kk;
         int i = pv.i[k], j = pv.j[k];
         if (!(i >= p->npoin0 && j >= p->npoin0)) continue;
         for (int d = 0; d < 2; d++) {
@@ -922,6 +925,7 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
   free(pdw);
   free(pv.i);
   free(pv.j);
+  if (tau0_io) *tau0_io = tau0_carry;
   return 0;
 }
 

@@ -12,7 +12,7 @@ typedef struct {
   int32_t nor_density;        /* must be 0 */
   int32_t icpwp;              /* 0, 1 or 3 (6:1418) */
   int32_t npoin0;             /* particles [0,npoin0) skip XSPH */
-  int32_t pad_;
+  int32_t rev_order;          /* 1: every pair sum in reverse order (conditioning probe) */
   double dt;                  /* dt_sph */
   double h_inf;               /* h_inf_SW */
   double pwp0rel;             /* initial pwp ratio (6:1811-1852) */
@@ -40,7 +40,7 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
              const double *mass, const double *hsml, double *u, double *uaux, int32_t *flag);
 int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,
             const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
-            int nsteps);
+            int nsteps, double *tau0_io /* Bingham Tau0 carry, in/out; NULL = 0 */);
 void gfo_raster_idw(const gfo_dem *d, int n, const double *x, const double *v, const double *rho,
                     const double *u, const int32_t *flag, double *out, double *wsum);
 void gfo_mc_pixel_accum(const gfo_dem *d, int n, const double *x, const double *v,

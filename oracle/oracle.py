@@ -19,8 +19,9 @@ LIB = os.path.join(HERE, "liboracle_gfsph.so")
 class GfoParams(C.Structure):
     _fields_ = [("integrator", C.c_int32), ("pa_sph", C.c_int32), ("skf", C.c_int32), ("sle", C.c_int32),
                 ("summation_density", C.c_int32), ("average_velocity", C.c_int32), ("nor_density", C.c_int32),
-                ("icpwp", C.c_int32), ("npoin0", C.c_int32), ("pad_", C.c_int32),
-                ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double), ("cst", C.c_double * 111)]
+                ("icpwp", C.c_int32), ("npoin0", C.c_int32), ("rev_order", C.c_int32),
+                ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double),
+                ("cst", C.c_double * 111)]
 
 
 class GfoDem(C.Structure):
@@ -52,7 +53,7 @@ def lib():
         L.gfo_fdiff_derivs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp]
         L.gfo_init.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, ip]
         L.gfo_run.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, dp, ip,
-                              C.c_int]
+                              C.c_int, dp]
         L.gfo_raster_idw.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, dp, dp]
         L.gfo_mc_pixel_accum.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, dp, ip]
         _lib = L
@@ -125,6 +126,12 @@ def topo_lookup(dem, xy):
 
 
 class State:
+    def copy(self):
+        c = State.__new__(State)
+        for k, v in self.__dict__.items():
+            setattr(c, k, v.copy())
+        return c
+
     def __init__(self, parts, n_uaux=True):
         self.x = np.ascontiguousarray(parts.x, dtype=np.float64).copy()
         self.v = np.ascontiguousarray(parts.v, dtype=np.float64).copy()
@@ -134,6 +141,7 @@ class State:
         self.u = np.ascontiguousarray(parts.u, dtype=np.float64).copy()
         self.uaux = np.zeros_like(self.h)
         self.flag = np.zeros(self.h.shape[0], np.int32)
+        self.tau0 = np.zeros(1)  # Bingham Tau0 carried between calls
 
 
 def init(case, state=None):
@@ -146,11 +154,15 @@ def init(case, state=None):
     return st
 
 
-def run(case, st, nsteps):
+def run(case, st, nsteps, rev_order=False):
+    """Advances ``st`` by ``nsteps``.  rev_order=True sums every pair loop in
+    reverse order: the spread between the two orders measures how strongly a
+    case amplifies rounding (its conditioning)."""
     p = make_params(case.params)
+    p.rev_order = int(rev_order)
     dh = DemHandle(case.dem)
     rc = lib().gfo_run(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _dp(st.mass),
-                       _dp(st.hsml), _dp(st.u), _dp(st.uaux), _ip(st.flag), nsteps)
+                       _dp(st.hsml), _dp(st.u), _dp(st.uaux), _ip(st.flag), nsteps, _dp(st.tau0))
     if rc != 0:
         raise RuntimeError(f"gfo_run failed: {rc}")
     return st

@@ -1,0 +1,207 @@
+"""Generates the golden fixtures from the reference Fortran build.
+
+Run in the build container (needs oracle/_ref/gfsph_ref, built from the
+reference sources by oracle/build_ref.sh):
+
+    python tests/golden/make_golden.py            # all cases
+    python tests/golden/make_golden.py g_bingham  # one case
+
+Each case writes tests/golden/<name>.npz holding the inputs (DEM, particle
+set, parameters) and the reference's state after the listed steps, plus the
+neighbour pair set the reference built for those steps (positions x0 they
+were built from included).  Fixtures are data only: inputs and the
+reference's outputs.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "deslizamientos-y-flujos_amd"))
+
+from gfsph import cases as C  # noqa: E402
+from gfsph.dem import DEM, fractal_dem, plane_runout_dem  # noqa: E402
+from oracle import refio  # noqa: E402
+
+
+def small_plane():
+    return plane_runout_dem(nx=81, ny=41, d=5.0, x_break=250.0, blend=50.0)
+
+
+def small_fractal(seed=7):
+    return fractal_dem(65, 2.0, seed, relief=40.0)
+
+
+def scar_on(dem, n, seed, spacing=1.0, h_range=(2.0, 8.0)):
+    L = (dem.nx - 1) * dem.dx
+    return C.elliptical_scar(0.3 * L, 0.5 * L, n, spacing, 2.0, seed, h_range=h_range)
+
+
+def case_defs():
+    defs = {}
+    # Voellmy on the inclined plane (cfg1 physics, downscaled), XSPH on
+    defs["g_voellmy_plane"] = dict(
+        case=C.Case("g_voellmy_plane", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=C.voellmy(0.2, 500.0)),
+                    small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
+        dumps=[1, 10, 100])
+    # frictional 28 deg + pore pressure icpwp=3 on fractal terrain (cfg2 physics)
+    dem = small_fractal()
+    defs["g_frictional_pwp"] = dict(
+        case=C.Case("g_frictional_pwp", C.Params(integrator=4, dt=0.02, h_inf=1e-3, icpwp=3, pwp0rel=0.6,
+                                                 const=C.frictional(28.0, cv=0.01)),
+                    dem, scar_on(dem, 300, 11), 100), dumps=[1, 10, 100])
+    # Bingham (cfg5 physics)
+    defs["g_bingham"] = dict(
+        case=C.Case("g_bingham", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=C.bingham(1000.0, 50.0)),
+                    dem, scar_on(dem, 300, 12), 100), dumps=[1, 10, 100])
+    # curvature term V^2/R (nfric 7) + Voellmy
+    defs["g_curvature7"] = dict(
+        case=C.Case("g_curvature7", C.Params(integrator=4, dt=0.02, h_inf=1e-3,
+                                             const=C.voellmy(0.25, 400.0, nfric=7)),
+                    dem, scar_on(dem, 300, 13), 100), dumps=[1, 100])
+    # mu(I) rheology nfric 8: tan(phi) * (1 + b (Fr r / h)^n)
+    c8 = C.voellmy(0.3, 0.0, nfric=8)
+    c8.update({3: 0.5, 6: 0.8, 7: 1.5})
+    defs["g_mui8"] = dict(case=C.Case("g_mui8", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=c8),
+                                      dem, scar_on(dem, 300, 14), 100), dumps=[1, 100])
+    # pore pressure as an RK variable (icpwp = 1), sle = 1
+    defs["g_icpwp1_sle1"] = dict(
+        case=C.Case("g_icpwp1_sle1", C.Params(integrator=4, dt=0.02, h_inf=1e-3, icpwp=1, pwp0rel=0.5, sle=1,
+                                              const=C.frictional(30.0, cv=0.05)),
+                    dem, scar_on(dem, 300, 15), 100), dumps=[1, 100])
+    # Manning (nfric 1), XSPH off, pa_sph 1
+    cm = {1: 9.81, 2: 1000.0, 3: 0.05, 5: 1, 10: 1e-2, 14: 1.0, 15: 1e-2}
+    defs["g_manning1"] = dict(
+        case=C.Case("g_manning1", C.Params(integrator=4, dt=0.02, h_inf=1e-3, pa_sph=1, average_velocity=0,
+                                           const=cm), dem, scar_on(dem, 300, 16), 100), dumps=[1, 100])
+    # Perzyna viscoplastic frictional (nfric 9)
+    c9 = C.voellmy(0.35, 0.0, nfric=9)
+    c9.update({6: 1.0, 7: 1.0, 8: 20.0})
+    defs["g_perzyna9"] = dict(case=C.Case("g_perzyna9", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=c9),
+                                          dem, scar_on(dem, 300, 17), 100), dumps=[1, 100])
+    # Herschel-Bulkley approximation (nfric 25), Bagnold (nfric 11)
+    c25 = {1: 9.81, 2: 1800.0, 5: 25, 6: 400.0, 7: 30.0, 8: 0.5, 10: 1e-2, 14: 1.0, 15: 1e-2}
+    defs["g_hb25"] = dict(case=C.Case("g_hb25", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=c25),
+                                      dem, scar_on(dem, 300, 18), 100), dumps=[1, 100])
+    c11 = {1: 9.81, 2: 1800.0, 3: 0.8, 5: 11, 10: 1e-2, 14: 1.0, 15: 1e-2}
+    defs["g_bagnold11"] = dict(case=C.Case("g_bagnold11", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=c11),
+                                           dem, scar_on(dem, 300, 19), 100), dumps=[1, 100])
+    # other kernels: Gauss (scale_k 3), quintic, Wendland
+    for skf in (2, 3, 4):
+        defs[f"g_kernel_skf{skf}"] = dict(
+            case=C.Case(f"g_kernel_skf{skf}", C.Params(integrator=4, dt=0.02, h_inf=1e-3, skf=skf,
+                                                       const=C.voellmy(0.2, 500.0)),
+                        dem, scar_on(dem, 200, 20 + skf), 40), dumps=[1, 40])
+    # particles sliding off the DEM edge: isolated particles on a steep plane
+    pl = plane_runout_dem(nx=21, ny=11, d=5.0, slope_deg=40.0, x_break=1000.0, blend=10.0)
+    pe = C.lattice_block(70.0, 10.0, 20.0, 20.0, 2, 2, 2.0, 1.0)
+    # one particle per row, 10 m apart: no pairs, so every particle's exit is
+    # exact (a neighbour of a particle that exits in mid-step sees the
+    # reference's un-reset depth of that particle in the remaining stages)
+    pe.x[:, 0] = np.array([10.0, 30.0, 60.0, 90.0])
+    pe.x[:, 1] = np.array([8.0, 18.0, 28.0, 38.0])
+    pe.v[:, 0] = -np.array([5.0, 10.0, 1.0, 20.0])
+    defs["g_domain_exit"] = dict(
+        case=C.Case("g_domain_exit", C.Params(integrator=4, dt=0.05, h_inf=1e-3, average_velocity=0,
+                                              const=C.voellmy(0.1, 1000.0)), pl, pe, 80), dumps=[1, 20, 40, 80])
+    # random cloud with varied smoothing lengths: neighbour set only
+    rng = np.random.Generator(np.random.Philox(99))
+    n = 600
+    x = np.stack([rng.uniform(40, 80, n), rng.uniform(40, 80, n)], axis=1)
+    hs = rng.uniform(0.5, 2.5, n)
+    pr = C.Particles(x=x, v=np.zeros((n, 2)), h=np.full(n, 1.0), mass=np.full(n, 1.0), hsml=hs, u=np.zeros(n),
+                     itype=np.ones(n, np.int32))
+    defs["g_random_neighbours"] = dict(
+        case=C.Case("g_random_neighbours", C.Params(integrator=4, dt=0.001, h_inf=1e-3, average_velocity=0,
+                                                    const=C.voellmy(0.6, 500.0)),
+                    small_fractal(3), pr, 1), dumps=[1])
+    return defs
+
+
+def cfg1_full_def(nsteps=100):
+    case = C.cfg_plane_voellmy(nsteps=nsteps)
+    return case
+
+
+def save_case(name, d, out_dir=HERE):
+    case, dumps = d["case"], d["dumps"]
+    # each dumped step s also dumps s-1: the step-start flags and smoothing
+    # lengths that built step s's pair list
+    dumps = sorted(set(list(dumps) + [s - 1 for s in dumps if s > 1]))
+    ref = refio.run_reference(case, case.nsteps, [0] + list(dumps))
+    P = case.particles
+    prm = case.params
+    arr = {
+        "params_json": np.frombuffer(json.dumps({k: v for k, v in prm.__dict__.items() if k != "const"}).encode(),
+                                     dtype=np.uint8),
+        "const": prm.cst(),
+        "dem_Z": case.dem.Z, "dem_geom": np.array([case.dem.xmin, case.dem.ymin, case.dem.dx, case.dem.dy]),
+        "p_x": P.x, "p_v": P.v, "p_h": P.h, "p_mass": P.mass, "p_hsml": P.hsml, "p_u": P.u,
+        "nsteps": np.array(case.nsteps), "dumps": np.array(sorted(set([0] + list(dumps)))),
+    }
+    for s, dd in ref.items():
+        arr[f"s{s}_x"] = dd["x"]
+        arr[f"s{s}_v"] = dd["v"]
+        arr[f"s{s}_h"] = dd["h"]
+        arr[f"s{s}_u"] = dd["u"]
+        arr[f"s{s}_hsml"] = dd["hsml"]
+        arr[f"s{s}_flag"] = dd["flag"]
+        if dd["uaux"] is not None:
+            arr[f"s{s}_uaux"] = dd["uaux"]
+        if s > 0:
+            pr = np.sort(dd["pairs"], axis=1)
+            order = np.lexsort((pr[:, 1], pr[:, 0]))
+            arr[f"s{s}_x0"] = dd["x0"]
+            arr[f"s{s}_pairs"] = pr[order].astype(np.int32)
+            # kernel values of the reference, in the same (i<j) order with the
+            # gradient oriented as dx = x_i - x_j
+            sign = np.where(dd["pairs"][:, 0] <= dd["pairs"][:, 1], 1.0, -1.0)
+            arr[f"s{s}_pair_w"] = dd["pair_w"][order]
+            arr[f"s{s}_pair_dwdx"] = (dd["pair_dwdx"] * sign[:, None])[order]
+    np.savez_compressed(os.path.join(out_dir, name + ".npz"), **arr)
+    return arr
+
+
+def save_cfg1(out_dir=HERE, nsteps=100, sample_every=37):
+    """Full-size cfg1 (10,011 particles): reference state after 1 and
+    ``nsteps`` steps, stored as global sums, a particle sample and the
+    SHA-256 of the sorted pair set."""
+    case = cfg1_full_def(nsteps)
+    ref = refio.run_reference(case, nsteps, [1, nsteps])
+    arr = {}
+    for s, dd in ref.items():
+        idx = np.arange(0, case.particles.n, sample_every)
+        arr[f"s{s}_idx"] = idx
+        for k in ("x", "v", "h", "u"):
+            arr[f"s{s}_{k}"] = dd[k][idx]
+            arr[f"s{s}_{k}_sum"] = np.array(np.sum(dd[k], axis=0))
+        pr = np.sort(dd["pairs"], axis=1)
+        pr = pr[np.lexsort((pr[:, 1], pr[:, 0]))].astype(np.int32)
+        arr[f"s{s}_npairs"] = np.array(pr.shape[0])
+        arr[f"s{s}_pairs_sha256"] = np.frombuffer(hashlib.sha256(pr.tobytes()).hexdigest().encode(), dtype=np.uint8)
+        arr[f"s{s}_x0_sample"] = dd["x0"][idx]
+    arr["nsteps"] = np.array(nsteps)
+    np.savez_compressed(os.path.join(out_dir, "g_cfg1_full.npz"), **arr)
+
+
+def main(argv):
+    defs = case_defs()
+    names = argv[1:] or list(defs) + ["g_cfg1_full"]
+    for nm in names:
+        if nm == "g_cfg1_full":
+            save_cfg1()
+        else:
+            save_case(nm, defs[nm])
+        print("wrote", nm, flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv)
