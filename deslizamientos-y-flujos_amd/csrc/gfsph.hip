@@ -2,7 +2,7 @@
 // integrated SPH propagation step behind the C ABI of include/gfsph.h.
 //
 // One step (frozen pair geometry, "7 SPH_MAIN_2019.f90":784-799):
-//   k_setup_clear  grid of the active bounding box (cell = scale_k*max hsml)
+//   k_grid/k_clear grid of the active bounding box (cell = scale_k*max hsml)
 //   k_keys         cell key per particle + atomic count per cell
 //   k_scan_*       exclusive scan of the cell counts -> cell starts
 //   k_scatter      counting-sort scatter
@@ -170,6 +170,10 @@ __device__ __forceinline__ double sph_self_w(int skf, double pi, double hsml) {
 
 // a1. pair predicate of grid_find_NEW_tools3 (5a:1010-1019), rounded exactly
 // as the reference writes it: dr = dx*dx + dy*dy without contraction.
+// pair_pred_fast first rejects, without the square root, every candidate
+// with dr > t*t*(1+2^-46), t = scale_k*mh: then sqrt(dr) > t even after the
+// roundings of t*t and of the square root, so the exact test would fail too;
+// all other candidates take the exact test.
 __device__ __forceinline__ bool pair_pred(int scale_k, double2 pi_, double2 pj, double hi, double hj,
                                           double &dx, double &dy, double &r, double &mh) {
   dx = __dsub_rn(pi_.x, pj.x);
@@ -178,6 +182,16 @@ __device__ __forceinline__ bool pair_pred(int scale_k, double2 pi_, double2 pj, 
   r = sqrt(dr);
   mh = __dadd_rn(hi, hj) / 2.;
   return r < scale_k * mh;
+}
+
+__device__ __forceinline__ bool pair_pred_fast(int scale_k, double2 pi_, double2 pj, double hi, double hj) {
+  double dx = __dsub_rn(pi_.x, pj.x);
+  double dy = __dsub_rn(pi_.y, pj.y);
+  double dr = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+  double mh = __dadd_rn(hi, hj) / 2.;
+  double t = scale_k * mh;
+  if (dr > __dmul_rn(__dmul_rn(t, t), 1.0 + 0x1p-46)) return false;
+  return sqrt(dr) < t;
 }
 
 // ---------------------------------------------------------------------------
@@ -470,8 +484,10 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
   return v;
 }
 
-// Accumulates the bounding box of active particles; one atomic per wave.
-__device__ __forceinline__ void bbox_accumulate(BBoxAcc *acc, bool active, double2 p, double hs) {
+// Bounding box of active particles: every block of the producing kernel
+// writes one partial (no atomics); k_grid reduces the partials of all blocks.
+__device__ __forceinline__ void bbox_block(BBoxAcc *part, bool active, double2 p, double hs) {
+  __shared__ BBoxAcc wp[GF_BLOCK / 64];
   const unsigned long long BIG = ~0ull, SMALL = 0ull;
   unsigned long long oxmin = active ? ord_of(p.x) : BIG, oxmax = active ? ord_of(p.x) : SMALL;
   unsigned long long oymin = active ? ord_of(p.y) : BIG, oymax = active ? ord_of(p.y) : SMALL;
@@ -483,31 +499,47 @@ __device__ __forceinline__ void bbox_accumulate(BBoxAcc *acc, bool active, doubl
   oymax = wave_max_u64(oymax);
   ohmax = wave_max_u64(ohmax);
   cnt = wave_sum_u32(cnt);
-  if ((threadIdx.x & 63) == 0 && cnt > 0) {
-    atomicMin(&acc->xmin, oxmin);
-    atomicMax(&acc->xmax, oxmax);
-    atomicMin(&acc->ymin, oymin);
-    atomicMax(&acc->ymax, oymax);
-    atomicMax(&acc->hmax, ohmax);
-    atomicAdd(&acc->nactive, cnt);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    wp[wid].xmin = oxmin;
+    wp[wid].xmax = oxmax;
+    wp[wid].ymin = oymin;
+    wp[wid].ymax = oymax;
+    wp[wid].hmax = ohmax;
+    wp[wid].nactive = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    BBoxAcc a = wp[0];
+    for (int w = 1; w < GF_BLOCK / 64; w++) {
+      a.xmin = wp[w].xmin < a.xmin ? wp[w].xmin : a.xmin;
+      a.xmax = wp[w].xmax > a.xmax ? wp[w].xmax : a.xmax;
+      a.ymin = wp[w].ymin < a.ymin ? wp[w].ymin : a.ymin;
+      a.ymax = wp[w].ymax > a.ymax ? wp[w].ymax : a.ymax;
+      a.hmax = wp[w].hmax > a.hmax ? wp[w].hmax : a.hmax;
+      a.nactive += wp[w].nactive;
+    }
+    part[blockIdx.x] = a;
   }
 }
 
-__device__ __forceinline__ void bbox_reset(BBoxAcc *acc) {
-  acc->xmin = ~0ull;
-  acc->ymin = ~0ull;
-  acc->xmax = 0ull;
-  acc->ymax = 0ull;
-  acc->hmax = 0ull;
-  acc->nactive = 0u;
+// XCD-aware block order: the dispatcher deals consecutive workgroups round
+// robin to the 8 XCDs, each with its own L2.  Launched on a grid that is a
+// multiple of 8, logical block (b % 8) * (grid/8) + b / 8 gives every XCD one
+// contiguous range of cell-ordered particles, so neighbour gathers of
+// adjacent workgroups hit the same L2.
+__device__ __forceinline__ int xcd_block() {
+  const int per = gridDim.x >> 3;
+  const int b = blockIdx.x;
+  return (b & 7) * per + (b >> 3);
 }
 
 // ---------------------------------------------------------------------------
 // neighbour build
 // ---------------------------------------------------------------------------
 
-__global__ void k_bbox(int n, const double2 *pos, const double *hsml, const int *flag, BBoxAcc *acc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_bbox(int n, const double2 *pos, const double *hsml, const int *flag, BBoxAcc *part) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   bool act = false;
   double2 p = make_double2(0, 0);
   double hs = 0;
@@ -516,11 +548,7 @@ __global__ void k_bbox(int n, const double2 *pos, const double *hsml, const int 
     p = pos[i];
     hs = hsml[i];
   }
-  bbox_accumulate(acc, act, p, hs);
-}
-
-__global__ void k_bbox_reset(BBoxAcc *acc) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) bbox_reset(acc);
+  bbox_block(part, act, p, hs);
 }
 
 // Grid of the active bounding box.  Cell edge = scale_k*max(hsml)*(1+1e-7),
@@ -561,10 +589,49 @@ __device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int
   return g;
 }
 
-__global__ void k_setup_clear(const BBoxAcc *acc, GridInfo *gout, int *cnt, int scale_k, int cap_cells) {
-  GridInfo g = make_grid(*acc, scale_k, cap_cells);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *gout = g;
-  int nb = g.ncell + 1;
+// One block: reduces the bounding-box partials, stores the total and the grid.
+__global__ void k_grid(int nparts, const BBoxAcc *part, BBoxAcc *acc, GridInfo *gout, int scale_k, int cap_cells) {
+  __shared__ BBoxAcc lds[GF_BLOCK];
+  BBoxAcc a;
+  a.xmin = ~0ull;
+  a.ymin = ~0ull;
+  a.xmax = 0ull;
+  a.ymax = 0ull;
+  a.hmax = 0ull;
+  a.nactive = 0u;
+  a.pad = 0u;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    BBoxAcc b = part[k];
+    a.xmin = b.xmin < a.xmin ? b.xmin : a.xmin;
+    a.xmax = b.xmax > a.xmax ? b.xmax : a.xmax;
+    a.ymin = b.ymin < a.ymin ? b.ymin : a.ymin;
+    a.ymax = b.ymax > a.ymax ? b.ymax : a.ymax;
+    a.hmax = b.hmax > a.hmax ? b.hmax : a.hmax;
+    a.nactive += b.nactive;
+  }
+  lds[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = GF_BLOCK / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      BBoxAcc b = lds[threadIdx.x + off];
+      a.xmin = b.xmin < a.xmin ? b.xmin : a.xmin;
+      a.xmax = b.xmax > a.xmax ? b.xmax : a.xmax;
+      a.ymin = b.ymin < a.ymin ? b.ymin : a.ymin;
+      a.ymax = b.ymax > a.ymax ? b.ymax : a.ymax;
+      a.hmax = b.hmax > a.hmax ? b.hmax : a.hmax;
+      a.nactive += b.nactive;
+      lds[threadIdx.x] = a;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *acc = a;
+    *gout = make_grid(a, scale_k, cap_cells);
+  }
+}
+
+__global__ void k_clear(const GridInfo *gp, int *cnt) {
+  int nb = gp->ncell + 1;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += gridDim.x * blockDim.x) cnt[c] = 0;
 }
 
@@ -681,42 +748,34 @@ __global__ void k_scatter(int n, const int *key, const int *rank, const int *sta
   perm[start[key[i]] + rank[i]] = i;
 }
 
-// Each cell ordered by previous slot so that runs are deterministic (the
-// out-of-domain bucket is left unordered: its particles never interact).
-__global__ void k_cellsort(const GridInfo *gp, const int *start, int *perm) {
-  int nb = gp->ncell;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += gridDim.x * blockDim.x) {
-    int s = start[c], e = start[c + 1];
-    for (int a = s + 1; a < e; a++) {
-      int v = perm[a];
-      int b = a - 1;
-      while (b >= s && perm[b] > v) {
-        perm[b + 1] = perm[b];
-        b--;
-      }
-      perm[b + 1] = v;
-    }
+// Gathers the state into cell order.  perm is the counting-sort scatter
+// (slot order inside a cell follows the atomics); each cell is reordered by
+// previous slot -- a particle's rank among its cell's particles -- so that
+// runs are deterministic (the out-of-domain bucket keeps the atomic order:
+// its particles never interact).  Sources: positions / depth / smoothing
+// length / pore pressure / flags from the previous step's outputs (src_new),
+// mass / id / velocity from the previous sorted set (src_old).
+__global__ void k_reorder(int n, const GridInfo *gp, const int *perm, const int *key, const int *start,
+                          NewSet src_new, SortedSet src_old, SortedSet dst) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int i = perm[t];
+  int c = key[i];
+  int d = t;
+  if (c < gp->ncell) {
+    int b = start[c], e = start[c + 1], rank = 0;
+    for (int u = b; u < e; u++) rank += perm[u] < i;
+    d = b + rank;
   }
-}
-
-// Gathers the state into cell order.  Sources: positions/depth/smoothing
-// length/pore pressure/flags of the previous step's outputs (src_new) and
-// mass/id/velocity of the previous sorted set (src_old).
-__global__ void k_reorder(int n, const int *perm, NewSet src_new, SortedSet src_old, SortedSet dst,
-                          BBoxAcc *acc) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) bbox_reset(acc);
-  if (s >= n) return;
-  int i = perm[s];
-  dst.pos[s] = src_new.pos[i];
-  dst.vel[s] = src_old.vel[i];
-  dst.h[s] = src_new.h[i];
-  dst.mass[s] = src_old.mass[i];
-  dst.hsml[s] = src_new.hsml[i];
-  dst.u[s] = src_new.u[i];
-  dst.uaux[s] = src_new.uaux[i];
-  dst.flag[s] = src_new.flag[i];
-  dst.id[s] = src_old.id[i];
+  dst.pos[d] = src_new.pos[i];
+  dst.vel[d] = src_old.vel[i];
+  dst.h[d] = src_new.h[i];
+  dst.mass[d] = src_old.mass[i];
+  dst.hsml[d] = src_new.hsml[i];
+  dst.u[d] = src_new.u[i];
+  dst.uaux[d] = src_new.uaux[i];
+  dst.flag[d] = src_new.flag[i];
+  dst.id[d] = src_old.id[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -735,14 +794,93 @@ __device__ __forceinline__ void for_candidates(const GridInfo &g, const int *sta
   }
 }
 
+// Neighbour masks: a mask pass records, for each particle, which candidates
+// of each of its (up to) 3 cell-row ranges pass the predicate: 128 bits per
+// row (row cy-1+k in words 2k, 2k+1).  A particle with a row range of more
+// than 128 candidates is marked full (all words set) and re-tests the
+// predicate.  The density, force and XSPH passes then visit exactly the
+// neighbours, in ascending slot order within each row.
+#define NB_WORDS 6
+#define NB_FULL 0xffffffffffffffffull
+
+template <typename F>
+__device__ __forceinline__ void for_neighbours(const GridInfo &g, const int *start, const unsigned long long *mask,
+                                               int scale_k, int i, double2 pi_, double hi, const double2 *pos,
+                                               const double *hsml, F &&f) {
+  int cx, cy;
+  cell_of(g, pi_, cx, cy);
+  int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
+  int r0 = cy > 0 ? cy - 1 : 0, r1 = cy < g.ncy - 1 ? cy + 1 : g.ncy - 1;
+  const unsigned long long *m = mask + NB_WORDS * (long)i;
+  bool full = true;
+#pragma unroll
+  for (int w = 0; w < NB_WORDS; w++) full = full && (m[w] == NB_FULL);
+  for (int r = r0; r <= r1; r++) {
+    int b = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
+    if (full) {
+      for (int j = b; j < e; j++) {
+        if (j == i) continue;
+        double dx, dy, rr, mh;
+        if (pair_pred(scale_k, pi_, pos[j], hi, hsml[j], dx, dy, rr, mh)) f(j, dx, dy, rr, mh);
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < 2; w++) {
+        unsigned long long bits = m[2 * (r - cy + 1) + w];
+        while (bits) {
+          int k = __ffsll((long long)bits) - 1;
+          bits &= bits - 1;
+          int j = b + 64 * w + k;
+          double dx, dy, rr, mh;
+          pair_pred(scale_k, pi_, pos[j], hi, hsml[j], dx, dy, rr, mh);
+          f(j, dx, dy, rr, mh);
+        }
+      }
+    }
+  }
+}
+
+// Predicate pass: builds the neighbour masks of particle i.
+__device__ __forceinline__ void build_mask(const GridInfo &g, const int *start, int scale_k, int i, double2 pi_,
+                                           double hi, const double2 *pos, const double *hsml,
+                                           unsigned long long *mo) {
+  int cx, cy;
+  cell_of(g, pi_, cx, cy);
+  int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
+  int r0 = cy > 0 ? cy - 1 : 0, r1 = cy < g.ncy - 1 ? cy + 1 : g.ncy - 1;
+  unsigned long long m[NB_WORDS] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+  bool full = false;
+  for (int r = r0; r <= r1; r++) {
+    int b = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
+    if (e - b > 128) {
+      full = true;
+      break;
+    }
+    unsigned long long w0 = 0ull, w1 = 0ull;
+    for (int j = b; j < e; j++) {
+      if (j == i) continue;
+      if (pair_pred_fast(scale_k, pi_, pos[j], hi, hsml[j])) {
+        int k = j - b;
+        if (k < 64) w0 |= 1ull << k;
+        else w1 |= 1ull << (k - 64);
+      }
+    }
+    m[2 * (r - cy + 1)] = w0;
+    m[2 * (r - cy + 1) + 1] = w1;
+  }
+#pragma unroll
+  for (int w = 0; w < NB_WORDS; w++) mo[w] = full ? NB_FULL : m[w];
+}
+
 // ---------------------------------------------------------------------------
 // a4. density: S_i = W(0,hsml_i) m_i + sum_j m_j w_ij (5a:1763-1859);
 // also P_i = p_i / rhoF_i^2 with rhoF = max(S, h_inf), p = g/2 rhoF^2
 // (IntForces_SW, 6:6769, 6:6797, pa_sph = 2), or p_i for pa_sph 1/4.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                       SortedSet s, double *S, double *Pout) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                       SortedSet s, double *S, double *Pout,
+                                                       unsigned long long *mask) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (s.flag[i]) {
     S[i] = s.h[i];
@@ -752,18 +890,14 @@ __global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const 
   const GridInfo g = *gp;
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
+  build_mask(g, start, P.scale_k, i, pi_, hi, s.pos, s.hsml, mask + NB_WORDS * (long)i);
   double acc = sph_self_w(P.skf, P.pi, hi) * s.mass[i];
-  for_candidates(g, start, pi_, [&](int j) {
-    if (j == i) return;
-    double2 pj = s.pos[j];
-    double hj = s.hsml[j];
-    double dx, dy, r, mh;
-    if (pair_pred(P.scale_k, pi_, pj, hi, hj, dx, dy, r, mh)) {
-      double w, gx, gy;
-      sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
-      acc = acc + s.mass[j] * w;
-    }
-  });
+  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.pos, s.hsml,
+                 [&](int j, double dx, double dy, double r, double mh) {
+                   double w, gx, gy;
+                   sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
+                   acc = acc + s.mass[j] * w;
+                 });
   S[i] = acc;
   double rf = acc < P.h_inf ? P.h_inf : acc;
   double p = P.cgra05 * rf * rf;
@@ -775,8 +909,9 @@ __global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const 
 // a5. pressure force gather (IntForces_SW pair sum, 6:6892-6941)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void pressure_force(const DevParams &P, const GridInfo &g, const int *start,
-                                               const SortedSet &s, const double *S, const double *Pp, int i,
-                                               double &inx, double &iny) {
+                                               const unsigned long long *mask, const SortedSet &s,
+                                               const double *S, const double *Pp, int i, double &inx,
+                                               double &iny) {
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   const double Si = S[i];
@@ -784,34 +919,29 @@ __device__ __forceinline__ void pressure_force(const DevParams &P, const GridInf
   const double Pi = Pp[i];
   inx = 0.0;
   iny = 0.0;
-  for_candidates(g, start, pi_, [&](int j) {
-    if (j == i) return;
-    double2 pj = s.pos[j];
-    double hj = s.hsml[j];
-    double dx, dy, r, mh;
-    if (pair_pred(P.scale_k, pi_, pj, hi, hj, dx, dy, r, mh)) {
-      double w, gx, gy;
-      sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
-      double mj = s.mass[j];
-      double Pj = Pp[j];
-      if (P.pa_sph == 2) {
-        double c = -(Pi + Pj);
-        inx = inx + mj * (c * gx);
-        iny = iny + mj * (c * gy);
-      } else if (P.pa_sph == 1) {
-        double Sj = S[j];
-        double rfj = Sj < P.h_inf ? P.h_inf : Sj;
-        double rhoij = 1.e0 / (rfi * rfj);
-        inx = inx + mj * ((-(Pi + Pj) * gx) * rhoij);
-        iny = iny + mj * ((-(Pi + Pj) * gy) * rhoij);
-      } else {  // pa_sph == 4
-        double Sj = S[j];
-        double rfj = Sj < P.h_inf ? P.h_inf : Sj;
-        inx = inx - ((Pj - Pi) * gx * mj / rfj);
-        iny = iny - ((Pj - Pi) * gy * mj / rfj);
-      }
-    }
-  });
+  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.pos, s.hsml,
+                 [&](int j, double dx, double dy, double r, double mh) {
+                   double w, gx, gy;
+                   sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
+                   double mj = s.mass[j];
+                   double Pj = Pp[j];
+                   if (P.pa_sph == 2) {
+                     double c = -(Pi + Pj);
+                     inx = inx + mj * (c * gx);
+                     iny = iny + mj * (c * gy);
+                   } else if (P.pa_sph == 1) {
+                     double Sj = S[j];
+                     double rfj = Sj < P.h_inf ? P.h_inf : Sj;
+                     double rhoij = 1.e0 / (rfi * rfj);
+                     inx = inx + mj * ((-(Pi + Pj) * gx) * rhoij);
+                     iny = iny + mj * ((-(Pi + Pj) * gy) * rhoij);
+                   } else {  // pa_sph == 4
+                     double Sj = S[j];
+                     double rfj = Sj < P.h_inf ? P.h_inf : Sj;
+                     inx = inx - ((Pj - Pi) * gx * mj / rfj);
+                     iny = iny - ((Pj - Pi) * gy * mj / rfj);
+                   }
+                 });
 }
 
 // frozen out-of-domain particle: IntForces' h_inf floor (6:6769) and
@@ -913,9 +1043,10 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 // (every basal law except Bingham, whose stages couple particles)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(GF_BLOCK) void k_force_rk(int n, DevParams P, DevDem dem, const GridInfo *gp,
-                                                        const int *start, SortedSet s, const double *S,
-                                                        const double *Pp, NewSet o, BBoxAcc *acc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                        const int *start, const unsigned long long *mask,
+                                                        SortedSet s, const double *S, const double *Pp, NewSet o,
+                                                        BBoxAcc *part) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
   double hso = 0;
@@ -925,7 +1056,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_force_rk(int n, DevParams P, DevDe
     } else {
       const GridInfo g = *gp;
       double inx, iny;
-      pressure_force(P, g, start, s, S, Pp, i, inx, iny);
+      pressure_force(P, g, start, mask, s, S, Pp, i, inx, iny);
       const double Si = S[i];
       const double rfi = Si < P.h_inf ? P.h_inf : Si;
       const double2 x0 = s.pos[i], v0 = s.vel[i];
@@ -952,7 +1083,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_force_rk(int n, DevParams P, DevDe
       finish_particle(P, dem, s, i, Si, rfi, x, v, u, Dx, Dy, Dvx, Dvy, Du, flagged, o, active_out, xo, hso);
     }
   }
-  bbox_accumulate(acc, active_out, xo, hso);
+  bbox_block(part, active_out, xo, hso);
 }
 
 // ---------------------------------------------------------------------------
@@ -978,14 +1109,14 @@ struct BinghamState {
 };
 
 __global__ __launch_bounds__(GF_BLOCK) void k_bingham_force(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                             SortedSet s, const double *S, const double *Pp,
-                                                             BinghamState b) {
+                                                             const unsigned long long *mask, SortedSet s,
+                                                             const double *S, const double *Pp, BinghamState b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double inx = 0.0, iny = 0.0;
   if (!s.flag[i]) {
     const GridInfo g = *gp;
-    pressure_force(P, g, start, s, S, Pp, i, inx, iny);
+    pressure_force(P, g, start, mask, s, S, Pp, i, inx, iny);
   }
   b.in[i] = make_double2(inx, iny);
   b.x[i] = s.pos[i];
@@ -1042,8 +1173,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, in
 
 __global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P, DevDem dem, SortedSet s,
                                                               const double *S, BinghamState b, NewSet o,
-                                                              BBoxAcc *acc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                              BBoxAcc *part) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
   double hso = 0;
@@ -1058,7 +1189,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P,
                       o, active_out, xo, hso);
     }
   }
-  bbox_accumulate(acc, active_out, xo, hso);
+  bbox_block(part, active_out, xo, hso);
 }
 
 // last-valid inclusive scan over particle ids: tau[k] = cand of the largest
@@ -1134,8 +1265,9 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 // Writes the final velocity into vout (the sorted set's velocity array).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(GF_BLOCK) void k_xsph(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                    SortedSet s, NewSet o, double2 *vout) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                    const unsigned long long *mask, SortedSet s, NewSet o,
+                                                    double2 *vout) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double2 vi = o.vel[i];
   if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0) {
@@ -1147,22 +1279,17 @@ __global__ __launch_bounds__(GF_BLOCK) void k_xsph(int n, DevParams P, const Gri
   const double hi = s.hsml[i];
   const double rhoi = o.h[i];
   double ax = 0.0, ay = 0.0;
-  for_candidates(g, start, pi_, [&](int j) {
-    if (j == i) return;
-    if (s.id[j] < P.npoin0) return;
-    double2 pj = s.pos[j];
-    double hj = s.hsml[j];
-    double dx, dy, r, mh;
-    if (pair_pred(P.scale_k, pi_, pj, hi, hj, dx, dy, r, mh)) {
-      double w, gx, gy;
-      sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
-      double mj = s.mass[j];
-      double2 vj = o.vel[j];
-      double den = rhoi + o.h[j];
-      ax = ax - 2 * mj * (vi.x - vj.x) / den * w;
-      ay = ay - 2 * mj * (vi.y - vj.y) / den * w;
-    }
-  });
+  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.pos, s.hsml,
+                 [&](int j, double dx, double dy, double r, double mh) {
+                   if (s.id[j] < P.npoin0) return;
+                   double w, gx, gy;
+                   sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
+                   double mj = s.mass[j];
+                   double2 vj = o.vel[j];
+                   double den = rhoi + o.h[j];
+                   ax = ax - 2 * mj * (vi.x - vj.x) / den * w;
+                   ay = ay - 2 * mj * (vi.y - vj.y) / den * w;
+                 });
   vi.x = vi.x + 0.3 * ax;
   vi.y = vi.y + 0.3 * ay;
   vout[i] = vi;
@@ -1372,10 +1499,13 @@ struct gfsph_ctx {
   SortedSet set[2]{};
   NewSet nw{};
   double *S = nullptr, *Pp = nullptr;
+  unsigned long long *mask = nullptr;
   int *key = nullptr, *rank = nullptr, *perm = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
   int cap_cells = 0, scan_blocks = 0;
   GridInfo *grid = nullptr;
-  BBoxAcc *acc = nullptr;
+  BBoxAcc *acc = nullptr, *part = nullptr;
+  int gridN = 0;  // blocks of the per-particle kernels (multiple of 8)
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one step from set[k]
   int *bad = nullptr;
   double2 *zxy = nullptr;
   double *top = nullptr;
@@ -1563,6 +1693,7 @@ static int alloc_sets(gfsph_ctx *c, int n) {
       (rc = dalloc(c, &o.hsml, n)) || (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) ||
       (rc = dalloc(c, &o.flag, n)))
     return rc;
+  if ((rc = dalloc(c, &c->mask, NB_WORDS * (size_t)n))) return rc;
   if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
       (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
     return rc;
@@ -1572,6 +1703,8 @@ static int alloc_sets(gfsph_ctx *c, int n) {
       (rc = dalloc(c, &c->bsum, (size_t)c->scan_blocks)) || (rc = dalloc(c, &c->grid, 1)) ||
       (rc = dalloc(c, &c->acc, 1)) || (rc = dalloc(c, &c->bad, 1)))
     return rc;
+  c->gridN = ((nblk(n) + 7) / 8) * 8;
+  if ((rc = dalloc(c, &c->part, c->gridN))) return rc;
   return GFSPH_OK;
 }
 
@@ -1595,11 +1728,12 @@ static void launch_forces(gfsph_ctx *c, SortedSet &s) {
   hipStream_t st = c->stream;
   const int nb = nblk(n);
   if (c->P.nfric != 4) {
-    k_force_rk<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->dem, c->grid, c->start, s, c->S, c->Pp, c->nw, c->acc);
+    k_force_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->dem, c->grid, c->start, c->mask, s, c->S, c->Pp, c->nw,
+                                              c->part);
     return;
   }
   BinghamState &b = c->bs;
-  k_bingham_force<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->S, c->Pp, b);
+  k_bingham_force<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, c->mask, s, c->S, c->Pp, b);
   for (int k = 0; k < c->P.nstages; k++) {
     k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 0, k, c->P, c->dem, s, c->S, b);
     k_lv_reduce<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum);
@@ -1607,7 +1741,7 @@ static void launch_forces(gfsph_ctx *c, SortedSet &s) {
     k_lv_down<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum, b.tau);
     k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->P, c->dem, s, c->S, b);
   }
-  k_bingham_finish<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->dem, s, c->S, b, c->nw, c->acc);
+  k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->dem, s, c->S, b, c->nw, c->part);
 }
 
 // Neighbour build into set[dst] from (nw, set[src]); acc must hold the
@@ -1617,34 +1751,70 @@ static int neighbour_build(gfsph_ctx *c, int src, int dst) {
   hipStream_t st = c->stream;
   const int nb = nblk(n);
   int clear_blocks = 1024;
-  k_setup_clear<<<clear_blocks, GF_BLOCK, 0, st>>>(c->acc, c->grid, c->cnt, c->P.scale_k, c->cap_cells);
+  k_grid<<<1, GF_BLOCK, 0, st>>>(c->gridN, c->part, c->acc, c->grid, c->P.scale_k, c->cap_cells);
+  k_clear<<<clear_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt);
   k_keys<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.flag, c->grid, c->key, c->rank, c->cnt);
   k_scan_reduce<<<c->scan_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt, c->bsum);
   k_scan_top<<<1, GF_BLOCK, 0, st>>>(c->scan_blocks, c->bsum);
   k_scan_down<<<c->scan_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt, c->bsum, c->start, n);
   k_scatter<<<nb, GF_BLOCK, 0, st>>>(n, c->key, c->rank, c->start, c->perm);
-  k_cellsort<<<clear_blocks, GF_BLOCK, 0, st>>>(c->grid, c->start, c->perm);
-  k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, c->perm, c->nw, c->set[src], c->set[dst], c->acc);
+  k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, c->grid, c->perm, c->key, c->start, c->nw, c->set[src], c->set[dst]);
   HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
 }
 
-static int one_step(gfsph_ctx *c) {
-  const int n = c->n;
-  hipStream_t st = c->stream;
-  const int nb = nblk(n);
-  int src = c->cur, dst = 1 - c->cur;
+__global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+static void launch_density(gfsph_ctx *c, SortedSet &s) {
+  k_density<<<c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, s, c->S, c->Pp, c->mask);
+}
+
+// XSPH (or the plain copy of the post-RK velocity) into the sorted set
+static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
+  if (c->P.av_vel)
+    k_xsph<<<c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s, c->nw, s.vel);
+  else
+    k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
+}
+
+// Enqueues one whole step reading set[src] + nw and producing set[dst] + nw.
+static int enqueue_step(gfsph_ctx *c, int src, int dst) {
   int rc = neighbour_build(c, src, dst);
   if (rc) return rc;
   SortedSet &s = c->set[dst];
-  k_density<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->S, c->Pp);
+  launch_density(c, s);
   launch_forces(c, s);
-  if (c->P.av_vel) {
-    k_xsph<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->nw, s.vel);
+  launch_velocity(c, s);
+  return GFSPH_OK;
+}
+
+// One step.  The kernel sequence of a step from set[src] is captured once in
+// a HIP graph (the launch gaps of ~15 separate launches dominate at 10^5
+// particles) and replayed; GFSPH_NO_GRAPH=1 launches the kernels directly.
+static int one_step(gfsph_ctx *c) {
+  int src = c->cur, dst = 1 - c->cur;
+  static const bool no_graph = getenv("GFSPH_NO_GRAPH") && getenv("GFSPH_NO_GRAPH")[0] == '1';
+  if (no_graph) {
+    int rc = enqueue_step(c, src, dst);
+    if (rc) return rc;
+    HIPCHK(c, hipGetLastError());
   } else {
-    HIPCHK(c, hipMemcpyAsync(s.vel, c->nw.vel, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+    if (!c->graph[src]) {
+      hipGraph_t g;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step(c, src, dst);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(c, GFSPH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->graph[src], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(c, GFSPH_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    HIPCHK(c, hipGraphLaunch(c->graph[src], c->stream));
   }
-  HIPCHK(c, hipGetLastError());
   c->cur = dst;
   return GFSPH_OK;
 }
@@ -1731,17 +1901,16 @@ int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const d
   const int nb = nblk(n);
   k_domain_flag<<<nb, GF_BLOCK, 0, st>>>(n, c->dem, s0.pos, s0.flag);
   k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw);
-  k_bbox_reset<<<1, 64, 0, st>>>(c->acc);
-  k_bbox<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->acc);
+  k_bbox<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
   if ((rc = neighbour_build(c, 0, 1))) return rc;
   c->cur = 1;
   SortedSet &s1 = c->set[1];
   int resum = (p->init_input && p->icpwp != 0 && p->summation_density) ? 1 : 0;
-  if (resum) k_density<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s1, c->S, c->Pp);
+  if (resum) launch_density(c, s1);
   if (p->init_input) k_init_pwp<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->P.alphaS, p->pwp0rel, resum, c->S, s1);
   // the step begins by re-sorting from the "new" set: publish the state there
   k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s1, c->nw);
-  k_bbox<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->acc);
+  k_bbox<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(st));
   return GFSPH_OK;
@@ -1848,8 +2017,7 @@ int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double 
     if (!rc) {
       const int nb = nblk(n);
       k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw);
-      k_bbox_reset<<<1, 64, 0, st>>>(c->acc);
-      k_bbox<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->acc);
+      k_bbox<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
       rc = neighbour_build(c, 0, 1);
     }
     if (!rc) {
@@ -1939,12 +2107,11 @@ int gfsph_profile_step(gfsph_ctx *c, int32_t nrep, int32_t cap, int32_t *nkernel
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(ev[1], st));
     SortedSet &s = c->set[dst];
-    k_density<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->S, c->Pp);
+    launch_density(c, s);
     HIPCHK(c, hipEventRecord(ev[2], st));
     launch_forces(c, s);
     HIPCHK(c, hipEventRecord(ev[3], st));
-    if (c->P.av_vel) k_xsph<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, s, c->nw, s.vel);
-    else HIPCHK(c, hipMemcpyAsync(s.vel, c->nw.vel, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+    launch_velocity(c, s);
     HIPCHK(c, hipEventRecord(ev[4], st));
     HIPCHK(c, hipEventRecord(ev[5], st));
     c->cur = dst;
@@ -1970,6 +2137,8 @@ int gfsph_counts(gfsph_ctx *c, int32_t *n, int32_t *nactive) {
   if (n) *n = c->n;
   if (nactive) {
     BBoxAcc a;
+    // the partials of the last step's force kernel describe the current state
+    k_grid<<<1, GF_BLOCK, 0, c->stream>>>(c->gridN, c->part, c->acc, c->grid, c->P.scale_k, c->cap_cells);
     HIPCHK(c, hipMemcpyAsync(&a, c->acc, sizeof(a), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *nactive = (int32_t)a.nactive;
@@ -1982,6 +2151,8 @@ const char *gfsph_last_error(gfsph_ctx *c) { return c ? c->err.c_str() : "null c
 void gfsph_destroy(gfsph_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 2; k++)
+    if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
