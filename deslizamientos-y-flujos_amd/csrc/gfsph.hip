@@ -9,11 +9,12 @@
 //   k_cellsort     order each cell by previous slot (deterministic order)
 //   k_reorder      gather the particle state into cell order
 //   k_density      summed depth h_i = W(0)m_i + sum m_j w_ij  (5a:1763)
-//   k_force_rk     pressure-force gather (6:6669) fused with every RK stage:
-//                  DEM bilinear lookup (2:2395), gravity, basal rheology and
-//                  limiter (6:9355-9901), stage update (7:808-922), pore
-//                  pressure (6:11176-11310), domain flag (7:1764), h update
-//                  (7:2866), next-step bounding box
+//   k_pforce       pressure-force gather (IntForces_SW, 6:6669)
+//   k_rk           every RK stage per particle: DEM bilinear lookup
+//                  (2:2395), gravity, basal rheology and limiter
+//                  (6:9355-9901), stage update (7:808-922), pore pressure
+//                  (6:11176-11310), domain flag (7:1764), h update (7:2866),
+//                  next-step bounding box (Bingham: per-stage kernels)
 //   k_xsph         XSPH velocity correction (7:3042-3099, 7:987-993)
 // All arithmetic is FP64.  The file is compiled with -ffp-contract=off so
 // expressions round as the reference's FP64 build writes them.
@@ -65,8 +66,10 @@ struct BBoxAcc {
   unsigned int nactive, pad;
 };
 
-// particle state in cell order
+// particle state in cell order; nb packs what neighbour gathers read
+// (x, y, hsml, mass) into one 32-byte record
 struct SortedSet {
+  double4 *nb;
   double2 *pos, *vel;
   double *h, *mass, *hsml, *u, *uaux;
   int *flag, *id;
@@ -158,6 +161,55 @@ __device__ __forceinline__ void sph_kernel(int skf, double pi, double r, double 
       gx = a * dx;
       gy = a * dy;
     }
+  }
+}
+
+// Kernel value and gradient for a compile-time kernel type, arranged around
+// one reciprocal of the smoothing length (and one of r where the gradient
+// needs dx/r) instead of the reference's six divisions; values agree with
+// kernel_TOOLS to a few ulps.  Only the neighbour predicate must round
+// exactly as the reference, and it is evaluated separately.
+template <int SKF>
+__device__ __forceinline__ void sph_kernel_t(double pi, double r, double dx, double dy, double hsml, double &w,
+                                             double &gx, double &gy) {
+  if (SKF == 1) {
+    const double ih = 1.0 / hsml;
+    const double q = r * ih;
+    const double factor = (15.0 / 7.0) / pi * (ih * ih);
+    if (q <= 1.0) {
+      w = factor * (2. / 3. - q * q + 0.5 * (q * q * q));
+      const double a = factor * (-2. + 1.5 * q) * (ih * ih);
+      gx = a * dx;
+      gy = a * dy;
+    } else if (q <= 2.0) {
+      const double t = 2. - q;
+      w = factor * (1.0 / 6.0) * (t * t * t);
+      const double a = -(factor * 0.5 * (t * t) * ih) / r;
+      gx = a * dx;
+      gy = a * dy;
+    } else {
+      w = 0.0;
+      gx = 0.0;
+      gy = 0.0;
+    }
+  } else if (SKF == 4) {
+    const double ih = 1.0 / hsml;
+    const double q = r * ih;
+    const double factor = (7.0 / 4.0) / pi * (ih * ih);
+    if (q <= 2.0) {
+      const double b = 1 - 0.5 * q;
+      const double b3 = b * b * b;
+      w = factor * ((b3 * b) * (2 * q + 1));
+      const double a = factor * (-5 * b3) * (ih * ih);
+      gx = a * dx;
+      gy = a * dy;
+    } else {
+      w = 0.0;
+      gx = 0.0;
+      gy = 0.0;
+    }
+  } else {
+    sph_kernel(SKF, pi, r, dx, dy, hsml, w, gx, gy);
   }
 }
 
@@ -297,7 +349,9 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   }
 
   TopoPt tp;
-  get_z_topo(dem, xx, yy, P.if_frictional != 0, tp);
+  // the curvature terms are interpolated only when the law uses them (V^2/R);
+  // the reference interpolates them for every frictional law (6:9257-9259)
+  get_z_topo(dem, xx, yy, P.if_V2byR != 0, tp);
   exx = exx + (-P.cgra * tp.Zx);
   exy = exy + (-P.cgra * tp.Zy);
 
@@ -767,11 +821,14 @@ __global__ void k_reorder(int n, const GridInfo *gp, const int *perm, const int 
     for (int u = b; u < e; u++) rank += perm[u] < i;
     d = b + rank;
   }
-  dst.pos[d] = src_new.pos[i];
+  const double2 p = src_new.pos[i];
+  const double hs = src_new.hsml[i], m = src_old.mass[i];
+  dst.nb[d] = make_double4(p.x, p.y, hs, m);
+  dst.pos[d] = p;
   dst.vel[d] = src_old.vel[i];
   dst.h[d] = src_new.h[i];
-  dst.mass[d] = src_old.mass[i];
-  dst.hsml[d] = src_new.hsml[i];
+  dst.mass[d] = m;
+  dst.hsml[d] = hs;
   dst.u[d] = src_new.u[i];
   dst.uaux[d] = src_new.uaux[i];
   dst.flag[d] = src_new.flag[i];
@@ -805,8 +862,8 @@ __device__ __forceinline__ void for_candidates(const GridInfo &g, const int *sta
 
 template <typename F>
 __device__ __forceinline__ void for_neighbours(const GridInfo &g, const int *start, const unsigned long long *mask,
-                                               int scale_k, int i, double2 pi_, double hi, const double2 *pos,
-                                               const double *hsml, F &&f) {
+                                               int scale_k, int i, double2 pi_, double hi, const double4 *nb,
+                                               F &&f) {
   int cx, cy;
   cell_of(g, pi_, cx, cy);
   int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
@@ -820,8 +877,9 @@ __device__ __forceinline__ void for_neighbours(const GridInfo &g, const int *sta
     if (full) {
       for (int j = b; j < e; j++) {
         if (j == i) continue;
+        const double4 q = nb[j];
         double dx, dy, rr, mh;
-        if (pair_pred(scale_k, pi_, pos[j], hi, hsml[j], dx, dy, rr, mh)) f(j, dx, dy, rr, mh);
+        if (pair_pred(scale_k, pi_, make_double2(q.x, q.y), hi, q.z, dx, dy, rr, mh)) f(j, q, dx, dy, rr, mh);
       }
     } else {
 #pragma unroll
@@ -831,9 +889,10 @@ __device__ __forceinline__ void for_neighbours(const GridInfo &g, const int *sta
           int k = __ffsll((long long)bits) - 1;
           bits &= bits - 1;
           int j = b + 64 * w + k;
+          const double4 q = nb[j];
           double dx, dy, rr, mh;
-          pair_pred(scale_k, pi_, pos[j], hi, hsml[j], dx, dy, rr, mh);
-          f(j, dx, dy, rr, mh);
+          pair_pred(scale_k, pi_, make_double2(q.x, q.y), hi, q.z, dx, dy, rr, mh);
+          f(j, q, dx, dy, rr, mh);
         }
       }
     }
@@ -842,8 +901,7 @@ __device__ __forceinline__ void for_neighbours(const GridInfo &g, const int *sta
 
 // Predicate pass: builds the neighbour masks of particle i.
 __device__ __forceinline__ void build_mask(const GridInfo &g, const int *start, int scale_k, int i, double2 pi_,
-                                           double hi, const double2 *pos, const double *hsml,
-                                           unsigned long long *mo) {
+                                           double hi, const double4 *nb, unsigned long long *mo) {
   int cx, cy;
   cell_of(g, pi_, cx, cy);
   int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
@@ -859,7 +917,8 @@ __device__ __forceinline__ void build_mask(const GridInfo &g, const int *start, 
     unsigned long long w0 = 0ull, w1 = 0ull;
     for (int j = b; j < e; j++) {
       if (j == i) continue;
-      if (pair_pred_fast(scale_k, pi_, pos[j], hi, hsml[j])) {
+      const double4 q = nb[j];
+      if (pair_pred_fast(scale_k, pi_, make_double2(q.x, q.y), hi, q.z)) {
         int k = j - b;
         if (k < 64) w0 |= 1ull << k;
         else w1 |= 1ull << (k - 64);
@@ -877,6 +936,7 @@ __device__ __forceinline__ void build_mask(const GridInfo &g, const int *start, 
 // also P_i = p_i / rhoF_i^2 with rhoF = max(S, h_inf), p = g/2 rhoF^2
 // (IntForces_SW, 6:6769, 6:6797, pa_sph = 2), or p_i for pa_sph 1/4.
 // ---------------------------------------------------------------------------
+template <int SKF>
 __global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const GridInfo *gp, const int *start,
                                                        SortedSet s, double *S, double *Pout,
                                                        unsigned long long *mask) {
@@ -890,13 +950,13 @@ __global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const 
   const GridInfo g = *gp;
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
-  build_mask(g, start, P.scale_k, i, pi_, hi, s.pos, s.hsml, mask + NB_WORDS * (long)i);
-  double acc = sph_self_w(P.skf, P.pi, hi) * s.mass[i];
-  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.pos, s.hsml,
-                 [&](int j, double dx, double dy, double r, double mh) {
+  build_mask(g, start, P.scale_k, i, pi_, hi, s.nb, mask + NB_WORDS * (long)i);
+  double acc = sph_self_w(SKF, P.pi, hi) * s.mass[i];
+  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.nb,
+                 [&](int j, const double4 &q, double dx, double dy, double r, double mh) {
                    double w, gx, gy;
-                   sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
-                   acc = acc + s.mass[j] * w;
+                   sph_kernel_t<SKF>(P.pi, r, dx, dy, mh, w, gx, gy);
+                   acc = acc + q.w * w;
                  });
   S[i] = acc;
   double rf = acc < P.h_inf ? P.h_inf : acc;
@@ -908,6 +968,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const 
 // ---------------------------------------------------------------------------
 // a5. pressure force gather (IntForces_SW pair sum, 6:6892-6941)
 // ---------------------------------------------------------------------------
+template <int SKF>
 __device__ __forceinline__ void pressure_force(const DevParams &P, const GridInfo &g, const int *start,
                                                const unsigned long long *mask, const SortedSet &s,
                                                const double *S, const double *Pp, int i, double &inx,
@@ -919,11 +980,11 @@ __device__ __forceinline__ void pressure_force(const DevParams &P, const GridInf
   const double Pi = Pp[i];
   inx = 0.0;
   iny = 0.0;
-  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.pos, s.hsml,
-                 [&](int j, double dx, double dy, double r, double mh) {
+  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.nb,
+                 [&](int j, const double4 &q, double dx, double dy, double r, double mh) {
                    double w, gx, gy;
-                   sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
-                   double mj = s.mass[j];
+                   sph_kernel_t<SKF>(P.pi, r, dx, dy, mh, w, gx, gy);
+                   double mj = q.w;
                    double Pj = Pp[j];
                    if (P.pa_sph == 2) {
                      double c = -(Pi + Pj);
@@ -1039,13 +1100,31 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 }
 
 // ---------------------------------------------------------------------------
-// a5 + a6..a10, a12, a13: pressure force gather fused with the RK stages
-// (every basal law except Bingham, whose stages couple particles)
+// a5: pressure force gather -> ain[i]; stage invariant under the frozen
+// geometry (IntForces_SW gives the same indvxdt in every RK stage when
+// ic_K0 = 0 and ic_cosTheta = 0)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GF_BLOCK) void k_force_rk(int n, DevParams P, DevDem dem, const GridInfo *gp,
-                                                        const int *start, const unsigned long long *mask,
-                                                        SortedSet s, const double *S, const double *Pp, NewSet o,
-                                                        BBoxAcc *part) {
+template <int SKF>
+__global__ __launch_bounds__(GF_BLOCK) void k_pforce(int n, DevParams P, const GridInfo *gp, const int *start,
+                                                      const unsigned long long *mask, SortedSet s, const double *S,
+                                                      const double *Pp, double2 *ain) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double inx = 0.0, iny = 0.0;
+  if (!s.flag[i]) {
+    const GridInfo g = *gp;
+    pressure_force<SKF>(P, g, start, mask, s, S, Pp, i, inx, iny);
+  }
+  ain[i] = make_double2(inx, iny);
+}
+
+// ---------------------------------------------------------------------------
+// a6..a10, a12, a13: the RK stages per particle (every basal law except
+// Bingham, whose stages couple particles): DEM lookup, gravity, rheology,
+// limiter, stage update, then the end-of-step updates
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, DevParams P, DevDem dem, SortedSet s, const double *S,
+                                                  const double2 *ain, NewSet o, BBoxAcc *part) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
@@ -1054,9 +1133,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_force_rk(int n, DevParams P, DevDe
     if (s.flag[i]) {
       write_frozen(P, s, i, o);
     } else {
-      const GridInfo g = *gp;
-      double inx, iny;
-      pressure_force(P, g, start, mask, s, S, Pp, i, inx, iny);
+      const double2 in = ain[i];
+      const double inx = in.x, iny = in.y;
       const double Si = S[i];
       const double rfi = Si < P.h_inf ? P.h_inf : Si;
       const double2 x0 = s.pos[i], v0 = s.vel[i];
@@ -1108,17 +1186,10 @@ struct BinghamState {
   double *bsum;  // per scan block: last valid value (NaN if none)
 };
 
-__global__ __launch_bounds__(GF_BLOCK) void k_bingham_force(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                             const unsigned long long *mask, SortedSet s,
-                                                             const double *S, const double *Pp, BinghamState b) {
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_init(int n, SortedSet s, const double2 *ain, BinghamState b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double inx = 0.0, iny = 0.0;
-  if (!s.flag[i]) {
-    const GridInfo g = *gp;
-    pressure_force(P, g, start, mask, s, S, Pp, i, inx, iny);
-  }
-  b.in[i] = make_double2(inx, iny);
+  b.in[i] = ain[i];
   b.x[i] = s.pos[i];
   b.v[i] = s.vel[i];
   b.ex[i] = make_double2(0.0, 0.0);
@@ -1264,6 +1335,7 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 // post-RK velocities and depths; v += 0.3*av for particles still active.
 // Writes the final velocity into vout (the sorted set's velocity array).
 // ---------------------------------------------------------------------------
+template <int SKF>
 __global__ __launch_bounds__(GF_BLOCK) void k_xsph(int n, DevParams P, const GridInfo *gp, const int *start,
                                                     const unsigned long long *mask, SortedSet s, NewSet o,
                                                     double2 *vout) {
@@ -1279,16 +1351,17 @@ __global__ __launch_bounds__(GF_BLOCK) void k_xsph(int n, DevParams P, const Gri
   const double hi = s.hsml[i];
   const double rhoi = o.h[i];
   double ax = 0.0, ay = 0.0;
-  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.pos, s.hsml,
-                 [&](int j, double dx, double dy, double r, double mh) {
-                   if (s.id[j] < P.npoin0) return;
+  const bool skip0 = P.npoin0 > 0;
+  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.nb,
+                 [&](int j, const double4 &q, double dx, double dy, double r, double mh) {
+                   if (skip0 && s.id[j] < P.npoin0) return;
                    double w, gx, gy;
-                   sph_kernel(P.skf, P.pi, r, dx, dy, mh, w, gx, gy);
-                   double mj = s.mass[j];
+                   sph_kernel_t<SKF>(P.pi, r, dx, dy, mh, w, gx, gy);
+                   double mj = q.w;
                    double2 vj = o.vel[j];
-                   double den = rhoi + o.h[j];
-                   ax = ax - 2 * mj * (vi.x - vj.x) / den * w;
-                   ay = ay - 2 * mj * (vi.y - vj.y) / den * w;
+                   double c = (2 * mj) * w / (rhoi + o.h[j]);
+                   ax = ax - c * (vi.x - vj.x);
+                   ay = ay - c * (vi.y - vj.y);
                  });
   vi.x = vi.x + 0.3 * ax;
   vi.y = vi.y + 0.3 * ay;
@@ -1500,6 +1573,7 @@ struct gfsph_ctx {
   NewSet nw{};
   double *S = nullptr, *Pp = nullptr;
   unsigned long long *mask = nullptr;
+  double2 *ain = nullptr;  // pressure force per particle
   int *key = nullptr, *rank = nullptr, *perm = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
   int cap_cells = 0, scan_blocks = 0;
   GridInfo *grid = nullptr;
@@ -1683,7 +1757,7 @@ static int alloc_sets(gfsph_ctx *c, int n) {
   int rc;
   for (int k = 0; k < 2; k++) {
     SortedSet &s = c->set[k];
-    if ((rc = dalloc(c, &s.pos, n)) || (rc = dalloc(c, &s.vel, n)) || (rc = dalloc(c, &s.h, n)) ||
+    if ((rc = dalloc(c, &s.nb, n)) || (rc = dalloc(c, &s.pos, n)) || (rc = dalloc(c, &s.vel, n)) || (rc = dalloc(c, &s.h, n)) ||
         (rc = dalloc(c, &s.mass, n)) || (rc = dalloc(c, &s.hsml, n)) || (rc = dalloc(c, &s.u, n)) ||
         (rc = dalloc(c, &s.uaux, n)) || (rc = dalloc(c, &s.flag, n)) || (rc = dalloc(c, &s.id, n)))
       return rc;
@@ -1693,7 +1767,7 @@ static int alloc_sets(gfsph_ctx *c, int n) {
       (rc = dalloc(c, &o.hsml, n)) || (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) ||
       (rc = dalloc(c, &o.flag, n)))
     return rc;
-  if ((rc = dalloc(c, &c->mask, NB_WORDS * (size_t)n))) return rc;
+  if ((rc = dalloc(c, &c->mask, NB_WORDS * (size_t)n)) || (rc = dalloc(c, &c->ain, n))) return rc;
   if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
       (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
     return rc;
@@ -1722,18 +1796,32 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
   return GFSPH_OK;
 }
 
-// Pressure force and RK stages of one step on the sorted set s.
-static void launch_forces(gfsph_ctx *c, SortedSet &s) {
+#define SKF_DISPATCH(skf, KERNEL, ...)                 \
+  do {                                                 \
+    switch (skf) {                                     \
+      case 1: KERNEL<1><<<__VA_ARGS__; break;           \
+      case 2: KERNEL<2><<<__VA_ARGS__; break;           \
+      case 3: KERNEL<3><<<__VA_ARGS__; break;           \
+      default: KERNEL<4><<<__VA_ARGS__; break;          \
+    }                                                  \
+  } while (0)
+
+static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
+  SKF_DISPATCH(c->P.skf, k_pforce, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s,
+                                                                        c->S, c->Pp, c->ain));
+}
+
+// RK stages of one step on the sorted set s (pressure force in c->ain).
+static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   const int n = c->n;
   hipStream_t st = c->stream;
   const int nb = nblk(n);
   if (c->P.nfric != 4) {
-    k_force_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->dem, c->grid, c->start, c->mask, s, c->S, c->Pp, c->nw,
-                                              c->part);
+    k_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->dem, s, c->S, c->ain, c->nw, c->part);
     return;
   }
   BinghamState &b = c->bs;
-  k_bingham_force<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->grid, c->start, c->mask, s, c->S, c->Pp, b);
+  k_bingham_init<<<nb, GF_BLOCK, 0, st>>>(n, s, c->ain, b);
   for (int k = 0; k < c->P.nstages; k++) {
     k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 0, k, c->P, c->dem, s, c->S, b);
     k_lv_reduce<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum);
@@ -1769,13 +1857,15 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 }
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
-  k_density<<<c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, s, c->S, c->Pp, c->mask);
+  SKF_DISPATCH(c->P.skf, k_density, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, s, c->S,
+                                                                          c->Pp, c->mask));
 }
 
 // XSPH (or the plain copy of the post-RK velocity) into the sorted set
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   if (c->P.av_vel)
-    k_xsph<<<c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s, c->nw, s.vel);
+    SKF_DISPATCH(c->P.skf, k_xsph, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s,
+                                                                       c->nw, s.vel));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
@@ -1786,7 +1876,8 @@ static int enqueue_step(gfsph_ctx *c, int src, int dst) {
   if (rc) return rc;
   SortedSet &s = c->set[dst];
   launch_density(c, s);
-  launch_forces(c, s);
+  launch_pforce(c, s);
+  launch_stages(c, s);
   launch_velocity(c, s);
   return GFSPH_OK;
 }
@@ -2090,15 +2181,13 @@ int gfsph_mc_pixel_read(gfsph_ctx *c, double *acc, int32_t *cnt, int32_t reset) 
 
 int gfsph_profile_step(gfsph_ctx *c, int32_t nrep, int32_t cap, int32_t *nkernels, const char **names, double *ms) {
   if (!c || nrep <= 0 || !nkernels) return GFSPH_ERR_ARG;
-  static const char *kn[] = {"neighbour_build", "k_density", "k_force_rk", "k_xsph", "step_total"};
-  const int K = 5;
+  static const char *kn[] = {"neighbour_build", "k_density", "k_pforce", "k_rk", "k_xsph", "step_total"};
+  const int K = 6;
   *nkernels = K;
   if (cap < K || !names || !ms) return GFSPH_ERR_CAPACITY;
   hipEvent_t ev[6];
   for (int k = 0; k < 6; k++) HIPCHK(c, hipEventCreate(&ev[k]));
-  double acc[5] = {0, 0, 0, 0, 0};
-  const int n = c->n;
-  const int nb = nblk(n);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int r = 0; r < nrep; r++) {
     hipStream_t st = c->stream;
     int src = c->cur, dst = 1 - c->cur;
@@ -2109,26 +2198,28 @@ int gfsph_profile_step(gfsph_ctx *c, int32_t nrep, int32_t cap, int32_t *nkernel
     SortedSet &s = c->set[dst];
     launch_density(c, s);
     HIPCHK(c, hipEventRecord(ev[2], st));
-    launch_forces(c, s);
+    launch_pforce(c, s);
     HIPCHK(c, hipEventRecord(ev[3], st));
-    launch_velocity(c, s);
+    launch_stages(c, s);
     HIPCHK(c, hipEventRecord(ev[4], st));
+    launch_velocity(c, s);
     HIPCHK(c, hipEventRecord(ev[5], st));
     c->cur = dst;
     HIPCHK(c, hipEventSynchronize(ev[5]));
     float t;
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 5; k++) {
       HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
       acc[k] += t;
     }
     HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[5]));
-    acc[4] += t;
+    acc[5] += t;
   }
   for (int k = 0; k < 6; k++) (void)hipEventDestroy(ev[k]);
   for (int k = 0; k < K; k++) {
     names[k] = kn[k];
     ms[k] = acc[k] / nrep;
   }
+  HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
 }
 
