@@ -38,13 +38,15 @@ PEAK_HBM_GBS = 8000.0
 # on chip):
 #   neighbour_build  pass A  read x, hsml; write key, id; radix passes     64
 #   k_density        pass B  read x, m, hsml, id; write h                  44
-#   k_force_rk       pass C  read x, m, hsml, h; write a             56
-#                    pass D  read x, v, h, h_prev, U, a, flag;
-#                            write x, v, U, h_prev, flag            128  = 184
+#   k_pforce         pass C  read x, m, hsml, h; write a                   56
+#   k_rk             pass D  read x, v, h, h_prev, U, a, flag;
+#                            write x, v, U, h_prev, flag                 128
 #   k_xsph           pass E  read x^n, v, m, h, hsml; write v              72
-# k_force_rk fuses C and D (a stays in registers); x^n needs no extra copy
-# (pass D', 16 B) because the step writes its outputs to a second buffer.
-B_ALG = {"neighbour_build": 64.0, "k_density": 44.0, "k_force_rk": 184.0, "k_xsph": 72.0}
+# The pair geometry and h are frozen over the RK stages, so the internal
+# force is gathered once per step and k_rk runs every stage in registers;
+# x^n needs no extra copy (pass D', 16 B) because the step writes its outputs
+# to a second buffer.  Total 364 B per particle-step with XSPH.
+B_ALG = {"neighbour_build": 64.0, "k_density": 44.0, "k_pforce": 56.0, "k_rk": 128.0, "k_xsph": 72.0}
 
 
 def cpu_model() -> str:

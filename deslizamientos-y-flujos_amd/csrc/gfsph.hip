@@ -46,6 +46,7 @@ struct DevParams {
   double c3, c6, c7, c8, c47;
   double f1[4], f2[4], fin;
   double pi;
+  double kc;  // kernel normalisation: 15/(7 pi) for skf 1, 7/(4 pi) for skf 4
 };
 
 struct DevDem {
@@ -57,12 +58,13 @@ struct DevDem {
 
 struct GridInfo {
   double xmin, ymin, cell;
-  int ncx, ncy, ncell, pad;
+  int ncx, ncy, ncell;
+  int uniform_h;  // every active particle has the same smoothing length
 };
 
 // bounding-box accumulator of active particles (ordered-integer encoding)
 struct BBoxAcc {
-  unsigned long long xmin, xmax, ymin, ymax, hmax;
+  unsigned long long xmin, xmax, ymin, ymax, hmax, hmin;
   unsigned int nactive, pad;
 };
 
@@ -164,52 +166,77 @@ __device__ __forceinline__ void sph_kernel(int skf, double pi, double r, double 
   }
 }
 
-// Kernel value and gradient for a compile-time kernel type, arranged around
-// one reciprocal of the smoothing length (and one of r where the gradient
-// needs dx/r) instead of the reference's six divisions; values agree with
-// kernel_TOOLS to a few ulps.  Only the neighbour predicate must round
+// Reciprocal for kernel evaluation: the hardware estimate refined by two
+// Newton-Raphson steps (relative error a few 2^-53 for normal x).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// r = sqrt(dr) and ir = 1/sqrt(dr) for kernel evaluation: the hardware
+// reciprocal-root estimate refined by two Newton-Raphson steps and one
+// correction of the root; dr = 0 gives r = ir = 0.
+__device__ __forceinline__ void root_nr(double dr, double &r, double &ir) {
+  double y = __builtin_amdgcn_rsq(dr);
+  const double hx = 0.5 * dr;
+  double e = fma(-(hx * y), y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-(hx * y), y, 0.5);
+  y = fma(y, e, y);
+  double g = dr * y;
+  const double d = fma(-g, g, dr);
+  g = fma(d, 0.5 * y, g);
+  const bool pos = dr > 0.0;
+  r = pos ? g : 0.0;
+  ir = pos ? y : 0.0;
+}
+
+// Kernel value and gradient for a compile-time kernel type from the squared
+// separation dr, arranged around one reciprocal of the smoothing length (and
+// one reciprocal root where the gradient needs dx/r) instead of the
+// reference's six divisions; kc is the normalisation 15/(7 pi) (cubic
+// spline) or 7/(4 pi) (Wendland) computed once.  Values agree with
+// kernel_TOOLS to a few ulps; only the neighbour predicate must round
 // exactly as the reference, and it is evaluated separately.
 template <int SKF>
-__device__ __forceinline__ void sph_kernel_t(double pi, double r, double dx, double dy, double hsml, double &w,
-                                             double &gx, double &gy) {
+__device__ __forceinline__ void sph_kernel_t(double kc, double pi, double dr, double dx, double dy, double hsml,
+                                             double &w, double &gx, double &gy) {
   if (SKF == 1) {
-    const double ih = 1.0 / hsml;
+    double r, ir;
+    root_nr(dr, r, ir);
+    const double ih = rcp_nr(hsml);
+    const double ih2 = ih * ih;
     const double q = r * ih;
-    const double factor = (15.0 / 7.0) / pi * (ih * ih);
-    if (q <= 1.0) {
-      w = factor * (2. / 3. - q * q + 0.5 * (q * q * q));
-      const double a = factor * (-2. + 1.5 * q) * (ih * ih);
-      gx = a * dx;
-      gy = a * dy;
-    } else if (q <= 2.0) {
-      const double t = 2. - q;
-      w = factor * (1.0 / 6.0) * (t * t * t);
-      const double a = -(factor * 0.5 * (t * t) * ih) / r;
-      gx = a * dx;
-      gy = a * dy;
-    } else {
-      w = 0.0;
-      gx = 0.0;
-      gy = 0.0;
-    }
+    const double factor = kc * ih2;
+    const double w1 = factor * (2. / 3. - q * q + 0.5 * (q * q * q));
+    const double a1 = factor * (-2. + 1.5 * q) * ih2;
+    const double t = 2. - q;
+    const double w2 = factor * (1.0 / 6.0) * (t * t * t);
+    const double a2 = -(factor * 0.5 * (t * t) * ih) * ir;
+    const bool in1 = q <= 1.0, in2 = q <= 2.0;
+    w = in1 ? w1 : (in2 ? w2 : 0.0);
+    const double a = in1 ? a1 : (in2 ? a2 : 0.0);
+    gx = a * dx;
+    gy = a * dy;
   } else if (SKF == 4) {
-    const double ih = 1.0 / hsml;
+    double r, ir;
+    root_nr(dr, r, ir);
+    const double ih = rcp_nr(hsml);
+    const double ih2 = ih * ih;
     const double q = r * ih;
-    const double factor = (7.0 / 4.0) / pi * (ih * ih);
-    if (q <= 2.0) {
-      const double b = 1 - 0.5 * q;
-      const double b3 = b * b * b;
-      w = factor * ((b3 * b) * (2 * q + 1));
-      const double a = factor * (-5 * b3) * (ih * ih);
-      gx = a * dx;
-      gy = a * dy;
-    } else {
-      w = 0.0;
-      gx = 0.0;
-      gy = 0.0;
-    }
+    const double factor = kc * ih2;
+    const double b = 1 - 0.5 * q;
+    const double b3 = b * b * b;
+    const bool in = q <= 2.0;
+    w = in ? factor * ((b3 * b) * (2 * q + 1)) : 0.0;
+    const double a = in ? factor * (-5 * b3) * ih2 : 0.0;
+    gx = a * dx;
+    gy = a * dy;
   } else {
-    sph_kernel(SKF, pi, r, dx, dy, hsml, w, gx, gy);
+    sph_kernel(SKF, pi, sqrt(dr), dx, dy, hsml, w, gx, gy);
   }
 }
 
@@ -545,13 +572,14 @@ __device__ __forceinline__ void bbox_block(BBoxAcc *part, bool active, double2 p
   const unsigned long long BIG = ~0ull, SMALL = 0ull;
   unsigned long long oxmin = active ? ord_of(p.x) : BIG, oxmax = active ? ord_of(p.x) : SMALL;
   unsigned long long oymin = active ? ord_of(p.y) : BIG, oymax = active ? ord_of(p.y) : SMALL;
-  unsigned long long ohmax = active ? ord_of(hs) : SMALL;
+  unsigned long long ohmax = active ? ord_of(hs) : SMALL, ohmin = active ? ord_of(hs) : BIG;
   unsigned cnt = active ? 1u : 0u;
   oxmin = wave_min_u64(oxmin);
   oxmax = wave_max_u64(oxmax);
   oymin = wave_min_u64(oymin);
   oymax = wave_max_u64(oymax);
   ohmax = wave_max_u64(ohmax);
+  ohmin = wave_min_u64(ohmin);
   cnt = wave_sum_u32(cnt);
   const int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
@@ -560,6 +588,7 @@ __device__ __forceinline__ void bbox_block(BBoxAcc *part, bool active, double2 p
     wp[wid].ymin = oymin;
     wp[wid].ymax = oymax;
     wp[wid].hmax = ohmax;
+    wp[wid].hmin = ohmin;
     wp[wid].nactive = cnt;
   }
   __syncthreads();
@@ -571,6 +600,7 @@ __device__ __forceinline__ void bbox_block(BBoxAcc *part, bool active, double2 p
       a.ymin = wp[w].ymin < a.ymin ? wp[w].ymin : a.ymin;
       a.ymax = wp[w].ymax > a.ymax ? wp[w].ymax : a.ymax;
       a.hmax = wp[w].hmax > a.hmax ? wp[w].hmax : a.hmax;
+      a.hmin = wp[w].hmin < a.hmin ? wp[w].hmin : a.hmin;
       a.nactive += wp[w].nactive;
     }
     part[blockIdx.x] = a;
@@ -618,7 +648,7 @@ __device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int
     g.ncx = 1;
     g.ncy = 1;
     g.ncell = 1;
-    g.pad = 0;
+    g.uniform_h = 0;
     return g;
   }
   double xmin = val_of(a.xmin), xmax = val_of(a.xmax), ymin = val_of(a.ymin), ymax = val_of(a.ymax);
@@ -639,7 +669,7 @@ __device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int
   g.ncx = (int)ncx;
   g.ncy = (int)ncy;
   g.ncell = g.ncx * g.ncy;
-  g.pad = 0;
+  g.uniform_h = a.hmin == a.hmax ? 1 : 0;
   return g;
 }
 
@@ -652,6 +682,7 @@ __global__ void k_grid(int nparts, const BBoxAcc *part, BBoxAcc *acc, GridInfo *
   a.xmax = 0ull;
   a.ymax = 0ull;
   a.hmax = 0ull;
+  a.hmin = ~0ull;
   a.nactive = 0u;
   a.pad = 0u;
   for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
@@ -661,6 +692,7 @@ __global__ void k_grid(int nparts, const BBoxAcc *part, BBoxAcc *acc, GridInfo *
     a.ymin = b.ymin < a.ymin ? b.ymin : a.ymin;
     a.ymax = b.ymax > a.ymax ? b.ymax : a.ymax;
     a.hmax = b.hmax > a.hmax ? b.hmax : a.hmax;
+    a.hmin = b.hmin < a.hmin ? b.hmin : a.hmin;
     a.nactive += b.nactive;
   }
   lds[threadIdx.x] = a;
@@ -673,6 +705,7 @@ __global__ void k_grid(int nparts, const BBoxAcc *part, BBoxAcc *acc, GridInfo *
       a.ymin = b.ymin < a.ymin ? b.ymin : a.ymin;
       a.ymax = b.ymax > a.ymax ? b.ymax : a.ymax;
       a.hmax = b.hmax > a.hmax ? b.hmax : a.hmax;
+      a.hmin = b.hmin < a.hmin ? b.hmin : a.hmin;
       a.nactive += b.nactive;
       lds[threadIdx.x] = a;
     }
@@ -851,159 +884,14 @@ __device__ __forceinline__ void for_candidates(const GridInfo &g, const int *sta
   }
 }
 
-// Neighbour masks: a mask pass records, for each particle, which candidates
-// of each of its (up to) 3 cell-row ranges pass the predicate: 128 bits per
-// row (row cy-1+k in words 2k, 2k+1).  A particle with a row range of more
-// than 128 candidates is marked full (all words set) and re-tests the
-// predicate.  The density, force and XSPH passes then visit exactly the
+// Neighbour masks: the density pass records, for each particle, which
+// candidates of each of its (up to) 3 cell-row ranges pass the predicate:
+// 128 bits per row (row cy-1+k in words 2k, 2k+1).  A particle with a row
+// range of more than 128 candidates is marked full (all words set) and
+// re-tests the predicate.  The force and XSPH passes then visit exactly the
 // neighbours, in ascending slot order within each row.
 #define NB_WORDS 6
 #define NB_FULL 0xffffffffffffffffull
-
-template <typename F>
-__device__ __forceinline__ void for_neighbours(const GridInfo &g, const int *start, const unsigned long long *mask,
-                                               int scale_k, int i, double2 pi_, double hi, const double4 *nb,
-                                               F &&f) {
-  int cx, cy;
-  cell_of(g, pi_, cx, cy);
-  int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
-  int r0 = cy > 0 ? cy - 1 : 0, r1 = cy < g.ncy - 1 ? cy + 1 : g.ncy - 1;
-  const unsigned long long *m = mask + NB_WORDS * (long)i;
-  bool full = true;
-#pragma unroll
-  for (int w = 0; w < NB_WORDS; w++) full = full && (m[w] == NB_FULL);
-  for (int r = r0; r <= r1; r++) {
-    int b = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
-    if (full) {
-      for (int j = b; j < e; j++) {
-        if (j == i) continue;
-        const double4 q = nb[j];
-        double dx, dy, rr, mh;
-        if (pair_pred(scale_k, pi_, make_double2(q.x, q.y), hi, q.z, dx, dy, rr, mh)) f(j, q, dx, dy, rr, mh);
-      }
-    } else {
-#pragma unroll
-      for (int w = 0; w < 2; w++) {
-        unsigned long long bits = m[2 * (r - cy + 1) + w];
-        while (bits) {
-          int k = __ffsll((long long)bits) - 1;
-          bits &= bits - 1;
-          int j = b + 64 * w + k;
-          const double4 q = nb[j];
-          double dx, dy, rr, mh;
-          pair_pred(scale_k, pi_, make_double2(q.x, q.y), hi, q.z, dx, dy, rr, mh);
-          f(j, q, dx, dy, rr, mh);
-        }
-      }
-    }
-  }
-}
-
-// Predicate pass: builds the neighbour masks of particle i.
-__device__ __forceinline__ void build_mask(const GridInfo &g, const int *start, int scale_k, int i, double2 pi_,
-                                           double hi, const double4 *nb, unsigned long long *mo) {
-  int cx, cy;
-  cell_of(g, pi_, cx, cy);
-  int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
-  int r0 = cy > 0 ? cy - 1 : 0, r1 = cy < g.ncy - 1 ? cy + 1 : g.ncy - 1;
-  unsigned long long m[NB_WORDS] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-  bool full = false;
-  for (int r = r0; r <= r1; r++) {
-    int b = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
-    if (e - b > 128) {
-      full = true;
-      break;
-    }
-    unsigned long long w0 = 0ull, w1 = 0ull;
-    for (int j = b; j < e; j++) {
-      if (j == i) continue;
-      const double4 q = nb[j];
-      if (pair_pred_fast(scale_k, pi_, make_double2(q.x, q.y), hi, q.z)) {
-        int k = j - b;
-        if (k < 64) w0 |= 1ull << k;
-        else w1 |= 1ull << (k - 64);
-      }
-    }
-    m[2 * (r - cy + 1)] = w0;
-    m[2 * (r - cy + 1) + 1] = w1;
-  }
-#pragma unroll
-  for (int w = 0; w < NB_WORDS; w++) mo[w] = full ? NB_FULL : m[w];
-}
-
-// ---------------------------------------------------------------------------
-// a4. density: S_i = W(0,hsml_i) m_i + sum_j m_j w_ij (5a:1763-1859);
-// also P_i = p_i / rhoF_i^2 with rhoF = max(S, h_inf), p = g/2 rhoF^2
-// (IntForces_SW, 6:6769, 6:6797, pa_sph = 2), or p_i for pa_sph 1/4.
-// ---------------------------------------------------------------------------
-template <int SKF>
-__global__ __launch_bounds__(GF_BLOCK) void k_density(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                       SortedSet s, double *S, double *Pout,
-                                                       unsigned long long *mask) {
-  int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (s.flag[i]) {
-    S[i] = s.h[i];
-    Pout[i] = 0.0;
-    return;
-  }
-  const GridInfo g = *gp;
-  const double2 pi_ = s.pos[i];
-  const double hi = s.hsml[i];
-  build_mask(g, start, P.scale_k, i, pi_, hi, s.nb, mask + NB_WORDS * (long)i);
-  double acc = sph_self_w(SKF, P.pi, hi) * s.mass[i];
-  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.nb,
-                 [&](int j, const double4 &q, double dx, double dy, double r, double mh) {
-                   double w, gx, gy;
-                   sph_kernel_t<SKF>(P.pi, r, dx, dy, mh, w, gx, gy);
-                   acc = acc + q.w * w;
-                 });
-  S[i] = acc;
-  double rf = acc < P.h_inf ? P.h_inf : acc;
-  double p = P.cgra05 * rf * rf;
-  if (P.pa_sph == 2) Pout[i] = p / (rf * rf);
-  else Pout[i] = p;
-}
-
-// ---------------------------------------------------------------------------
-// a5. pressure force gather (IntForces_SW pair sum, 6:6892-6941)
-// ---------------------------------------------------------------------------
-template <int SKF>
-__device__ __forceinline__ void pressure_force(const DevParams &P, const GridInfo &g, const int *start,
-                                               const unsigned long long *mask, const SortedSet &s,
-                                               const double *S, const double *Pp, int i, double &inx,
-                                               double &iny) {
-  const double2 pi_ = s.pos[i];
-  const double hi = s.hsml[i];
-  const double Si = S[i];
-  const double rfi = Si < P.h_inf ? P.h_inf : Si;
-  const double Pi = Pp[i];
-  inx = 0.0;
-  iny = 0.0;
-  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.nb,
-                 [&](int j, const double4 &q, double dx, double dy, double r, double mh) {
-                   double w, gx, gy;
-                   sph_kernel_t<SKF>(P.pi, r, dx, dy, mh, w, gx, gy);
-                   double mj = q.w;
-                   double Pj = Pp[j];
-                   if (P.pa_sph == 2) {
-                     double c = -(Pi + Pj);
-                     inx = inx + mj * (c * gx);
-                     iny = iny + mj * (c * gy);
-                   } else if (P.pa_sph == 1) {
-                     double Sj = S[j];
-                     double rfj = Sj < P.h_inf ? P.h_inf : Sj;
-                     double rhoij = 1.e0 / (rfi * rfj);
-                     inx = inx + mj * ((-(Pi + Pj) * gx) * rhoij);
-                     iny = iny + mj * ((-(Pi + Pj) * gy) * rhoij);
-                   } else {  // pa_sph == 4
-                     double Sj = S[j];
-                     double rfj = Sj < P.h_inf ? P.h_inf : Sj;
-                     inx = inx - ((Pj - Pi) * gx * mj / rfj);
-                     iny = iny - ((Pj - Pi) * gy * mj / rfj);
-                   }
-                 });
-}
 
 // frozen out-of-domain particle: IntForces' h_inf floor (6:6769) and
 // Get_Pwp_SW's Uaux = 0 (6:11184) still apply
@@ -1097,25 +985,6 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
     if (u > rr) u = rr;
     else if (u < 0.0) u = 0.0;
   }
-}
-
-// ---------------------------------------------------------------------------
-// a5: pressure force gather -> ain[i]; stage invariant under the frozen
-// geometry (IntForces_SW gives the same indvxdt in every RK stage when
-// ic_K0 = 0 and ic_cosTheta = 0)
-// ---------------------------------------------------------------------------
-template <int SKF>
-__global__ __launch_bounds__(GF_BLOCK) void k_pforce(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                      const unsigned long long *mask, SortedSet s, const double *S,
-                                                      const double *Pp, double2 *ain) {
-  int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double inx = 0.0, iny = 0.0;
-  if (!s.flag[i]) {
-    const GridInfo g = *gp;
-    pressure_force<SKF>(P, g, start, mask, s, S, Pp, i, inx, iny);
-  }
-  ain[i] = make_double2(inx, iny);
 }
 
 // ---------------------------------------------------------------------------
@@ -1331,15 +1200,306 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 }
 
 // ---------------------------------------------------------------------------
-// a11. XSPH av_vel (7:3042-3099) with the frozen step-start geometry and the
-// post-RK velocities and depths; v += 0.3*av for particles still active.
-// Writes the final velocity into vout (the sorted set's velocity array).
+// Neighbour gathers: one lane per particle.  The particle's 3 stencil rows
+// are contiguous ranges of the cell-sorted set.  Loads are issued in batches
+// of NB_BATCH independent candidates before any of them is used, and the
+// batch bookkeeping (mask bits, slot extraction) is branch-free, so a wave's
+// instruction stream is mostly FP64 arithmetic.  Sums run in slot order
+// (row, then slot within the row), so results are deterministic.
 // ---------------------------------------------------------------------------
+#define NB_BATCH 4
+
+struct RowSpan {
+  int b[3], cnt[3];  // first slot and candidate count of each stencil row (0: row outside the grid)
+  bool full;         // some row holds more than 128 candidates: masks are not kept
+};
+
+__device__ __forceinline__ RowSpan row_spans(const GridInfo &g, const int *start, double2 p) {
+  int cx, cy;
+  cell_of(g, p, cx, cy);
+  const int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
+  RowSpan rs;
+  rs.full = false;
+#pragma unroll
+  for (int rr = 0; rr < 3; rr++) {
+    const int r = cy - 1 + rr;
+    const bool in = r >= 0 && r < g.ncy;
+    const int rc = in ? r : cy;
+    const int b = start[rc * g.ncx + c0], e = start[rc * g.ncx + c1 + 1];
+    rs.b[rr] = b;
+    rs.cnt[rr] = in ? e - b : 0;
+    rs.full = rs.full || rs.cnt[rr] > 128;
+  }
+  return rs;
+}
+
+// Separation and mean smoothing length of a pair, rounded as in pair_pred
+// (dr = dx*dx + dy*dy without contraction).
+__device__ __forceinline__ void pair_geom(double2 pi_, double hi, const double4 &q, double &dx, double &dy,
+                                          double &dr, double &mh) {
+  dx = __dsub_rn(pi_.x, q.x);
+  dy = __dsub_rn(pi_.y, q.y);
+  dr = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+  mh = __dadd_rn(hi, q.z) / 2.;
+}
+
+// Thresholds of the square-root-free predicate for support radius t: with
+// lo = t*t*(1-2^-46) and hi = t*t*(1+2^-46) (each rounded), dr < lo implies
+// sqrt(dr) < t and dr > hi implies sqrt(dr) > t after every rounding
+// involved; only dr inside [lo, hi] needs the exact test of pair_pred.
+struct PredBand {
+  double t, lo, hi;
+};
+
+__device__ __forceinline__ PredBand pred_band(double t) {
+  const double tt = __dmul_rn(t, t);
+  PredBand pb;
+  pb.t = t;
+  pb.lo = __dmul_rn(tt, 1.0 - 0x1p-46);
+  pb.hi = __dmul_rn(tt, 1.0 + 0x1p-46);
+  return pb;
+}
+
+// Neighbour mask of particle i: bit k of words (2*rr, 2*rr+1) is slot k of
+// stencil row rr.  UNIFORM: all smoothing lengths are equal, so the support
+// radius scale_k*(hi+hj)/2 = scale_k*hi is the same for every candidate.
+template <bool UNIFORM>
+__device__ __forceinline__ void build_mask(const RowSpan &rs, int scale_k, int i, double2 pi_, double hi,
+                                           const double4 *nb, unsigned long long (&m)[NB_WORDS]) {
+  const PredBand pu = pred_band(scale_k * hi);
+#pragma unroll
+  for (int rr = 0; rr < 3; rr++) {
+    const int b = rs.b[rr], cnt = rs.cnt[rr];
+    unsigned long long w0 = 0ull, w1 = 0ull;
+    for (int k0 = 0; k0 < cnt; k0 += NB_BATCH) {
+      double4 q[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) q[u] = nb[b + min(k0 + u, cnt - 1)];
+      double dr[NB_BATCH];
+      PredBand pb[NB_BATCH];
+      bool in[NB_BATCH], band[NB_BATCH];
+      bool anyband = false;
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        const double dx = __dsub_rn(pi_.x, q[u].x), dy = __dsub_rn(pi_.y, q[u].y);
+        dr[u] = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+        pb[u] = UNIFORM ? pu : pred_band(scale_k * (__dadd_rn(hi, q[u].z) / 2.));
+        in[u] = dr[u] < pb[u].lo;
+        band[u] = !in[u] && !(dr[u] > pb[u].hi);
+        anyband = anyband || band[u];
+      }
+      if (__any(anyband)) {
+#pragma unroll
+        for (int u = 0; u < NB_BATCH; u++)
+          if (band[u]) in[u] = sqrt(dr[u]) < pb[u].t;
+      }
+      unsigned nib = 0u;
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        const int k = k0 + u;
+        nib |= (in[u] && k < cnt && b + k != i) ? (1u << u) : 0u;
+      }
+      const unsigned long long wn = (unsigned long long)nib;
+      if (k0 < 64) w0 |= wn << k0;
+      else w1 |= wn << (k0 - 64);
+    }
+    m[2 * rr] = w0;
+    m[2 * rr + 1] = w1;
+  }
+}
+
+// Calls f(js, ok) for batches of NB_BATCH neighbours of particle i taken
+// from its mask in slot order; unused entries of a row's last batch have
+// ok = false and js = i, so f may read them unconditionally.
+template <typename F>
+__device__ __forceinline__ void masked_batches(const RowSpan &rs, const unsigned long long (&m)[NB_WORDS], int i,
+                                               F &&f) {
+#pragma unroll
+  for (int rr = 0; rr < 3; rr++) {
+    unsigned long long lo = m[2 * rr], hi = m[2 * rr + 1];
+    const int b = rs.b[rr];
+    while (lo | hi) {
+      int js[NB_BATCH];
+      bool ok[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        const bool inlo = lo != 0ull;
+        const unsigned long long cur = inlo ? lo : hi;
+        ok[u] = cur != 0ull;
+        const int k = __ffsll((long long)cur) - 1 + (inlo ? 0 : 64);
+        const unsigned long long nxt = cur & (cur - 1ull);
+        lo = inlo ? nxt : lo;
+        hi = inlo ? hi : nxt;
+        js[u] = ok[u] ? b + k : i;
+      }
+      f(js, ok);
+    }
+  }
+}
+
+// Every neighbour of i, with the exact predicate, for particles whose
+// stencil rows are too dense for masks (not batched: rare).
+template <typename F>
+__device__ __forceinline__ void dense_neighbours(const RowSpan &rs, int scale_k, int i, double2 pi_, double hi,
+                                                 const double4 *nb, F &&f) {
+  for (int rr = 0; rr < 3; rr++) {
+    for (int k = 0; k < rs.cnt[rr]; k++) {
+      const int j = rs.b[rr] + k;
+      if (j == i) continue;
+      const double4 q = nb[j];
+      double dx, dy, d, mh;
+      if (pair_pred(scale_k, pi_, make_double2(q.x, q.y), hi, q.z, dx, dy, d, mh))
+        f(j, q, dx, dy, __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), mh);
+    }
+  }
+}
+
+__device__ __forceinline__ bool load_mask(const unsigned long long *mask, int i, unsigned long long (&m)[NB_WORDS]) {
+  const ulonglong2 *mv = reinterpret_cast<const ulonglong2 *>(mask + NB_WORDS * (long)i);
+  bool full = true;
+#pragma unroll
+  for (int w = 0; w < NB_WORDS / 2; w++) {
+    const ulonglong2 v = mv[w];
+    m[2 * w] = v.x;
+    m[2 * w + 1] = v.y;
+    full = full && v.x == NB_FULL && v.y == NB_FULL;
+  }
+  return full;
+}
+
+// a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
 template <int SKF>
-__global__ __launch_bounds__(GF_BLOCK) void k_xsph(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                    const unsigned long long *mask, SortedSet s, NewSet o,
-                                                    double2 *vout) {
-  int i = xcd_block() * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(GF_BLOCK) void k_density_m(int n, DevParams P, const GridInfo *gp, const int *start,
+                                                         SortedSet s, double *S, double *Pout,
+                                                         unsigned long long *mask) {
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= n) return;
+  if (s.flag[i]) {
+    S[i] = s.h[i];
+    Pout[i] = 0.0;
+    return;
+  }
+  const GridInfo g = *gp;
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  const RowSpan rs = row_spans(g, start, pi_);
+  double acc = 0.0;
+  ulonglong2 *mv = reinterpret_cast<ulonglong2 *>(mask + NB_WORDS * (long)i);
+  if (rs.full) {
+#pragma unroll
+    for (int w = 0; w < NB_WORDS / 2; w++) mv[w] = make_ulonglong2(NB_FULL, NB_FULL);
+    dense_neighbours(rs, P.scale_k, i, pi_, hi, s.nb,
+                     [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                       double w, gx, gy;
+                       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                       acc = acc + q.w * w;
+                     });
+  } else {
+    unsigned long long m[NB_WORDS];
+    if (g.uniform_h) build_mask<true>(rs, P.scale_k, i, pi_, hi, s.nb, m);
+    else build_mask<false>(rs, P.scale_k, i, pi_, hi, s.nb, m);
+#pragma unroll
+    for (int w = 0; w < NB_WORDS / 2; w++) mv[w] = make_ulonglong2(m[2 * w], m[2 * w + 1]);
+    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+      double4 q[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) q[u] = s.nb[js[u]];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh, w, gx, gy;
+        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+        sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+        const double t = acc + q[u].w * w;
+        acc = ok[u] ? t : acc;
+      }
+    });
+  }
+  const double Si = sph_self_w(SKF, P.pi, hi) * s.mass[i] + acc;
+  S[i] = Si;
+  const double rf = Si < P.h_inf ? P.h_inf : Si;
+  const double p = P.cgra05 * rf * rf;
+  Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
+}
+
+// pressure-force contribution of neighbour j (IntForces_SW pair term)
+template <int PA>
+__device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, double Sj, double h_inf, double mj,
+                                            double gx, double gy, double &inx, double &iny) {
+  if (PA == 2) {
+    const double c = -(Pi + Pj);
+    inx = inx + mj * (c * gx);
+    iny = iny + mj * (c * gy);
+  } else if (PA == 1) {
+    const double rfj = Sj < h_inf ? h_inf : Sj;
+    const double rhoij = 1.e0 / (rfi * rfj);
+    inx = inx + mj * ((-(Pi + Pj) * gx) * rhoij);
+    iny = iny + mj * ((-(Pi + Pj) * gy) * rhoij);
+  } else {  // pa_sph == 4
+    const double rfj = Sj < h_inf ? h_inf : Sj;
+    inx = inx - ((Pj - Pi) * gx * mj / rfj);
+    iny = iny - ((Pj - Pi) * gy * mj / rfj);
+  }
+}
+
+// a5 pressure force (IntForces_SW pair sum, 6:6892-6941) -> ain[i]
+template <int SKF, int PA>
+__global__ __launch_bounds__(GF_BLOCK) void k_pforce_m(int n, DevParams P, const GridInfo *gp, const int *start,
+                                                        const unsigned long long *mask, SortedSet s, const double *S,
+                                                        const double *Pp, double2 *ain) {
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= n) return;
+  if (s.flag[i]) {
+    ain[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  const GridInfo g = *gp;
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  unsigned long long m[NB_WORDS];
+  const bool full = load_mask(mask, i, m);
+  const RowSpan rs = row_spans(g, start, pi_);
+  const double Si = S[i];
+  const double rfi = Si < P.h_inf ? P.h_inf : Si;
+  const double Pi = Pp[i];
+  double inx = 0.0, iny = 0.0;
+  if (full) {
+    dense_neighbours(rs, P.scale_k, i, pi_, hi, s.nb,
+                     [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                       double w, gx, gy;
+                       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                       pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
+                     });
+  } else {
+    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+      double4 q[NB_BATCH];
+      double pj[NB_BATCH], sj[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        q[u] = s.nb[js[u]];
+        pj[u] = Pp[js[u]];
+        sj[u] = PA == 2 ? 0.0 : S[js[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh, w, gx, gy;
+        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+        sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+        double tx = inx, ty = iny;
+        pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
+        inx = ok[u] ? tx : inx;
+        iny = ok[u] ? ty : iny;
+      }
+    });
+  }
+  ain[i] = make_double2(inx, iny);
+}
+
+// a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
+template <int SKF>
+__global__ __launch_bounds__(GF_BLOCK) void k_xsph_m(int n, DevParams P, const GridInfo *gp, const int *start,
+                                                      const unsigned long long *mask, SortedSet s, NewSet o,
+                                                      double2 *vout) {
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
   double2 vi = o.vel[i];
   if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0) {
@@ -1349,20 +1509,48 @@ __global__ __launch_bounds__(GF_BLOCK) void k_xsph(int n, DevParams P, const Gri
   const GridInfo g = *gp;
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
+  unsigned long long m[NB_WORDS];
+  const bool full = load_mask(mask, i, m);
+  const RowSpan rs = row_spans(g, start, pi_);
   const double rhoi = o.h[i];
-  double ax = 0.0, ay = 0.0;
   const bool skip0 = P.npoin0 > 0;
-  for_neighbours(g, start, mask, P.scale_k, i, pi_, hi, s.nb,
-                 [&](int j, const double4 &q, double dx, double dy, double r, double mh) {
-                   if (skip0 && s.id[j] < P.npoin0) return;
-                   double w, gx, gy;
-                   sph_kernel_t<SKF>(P.pi, r, dx, dy, mh, w, gx, gy);
-                   double mj = q.w;
-                   double2 vj = o.vel[j];
-                   double c = (2 * mj) * w / (rhoi + o.h[j]);
-                   ax = ax - c * (vi.x - vj.x);
-                   ay = ay - c * (vi.y - vj.y);
-                 });
+  double ax = 0.0, ay = 0.0;
+  if (full) {
+    dense_neighbours(rs, P.scale_k, i, pi_, hi, s.nb,
+                     [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                       if (skip0 && s.id[j] < P.npoin0) return;
+                       double w, gx, gy;
+                       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                       const double2 vj = o.vel[j];
+                       const double c = (2 * q.w) * w / (rhoi + o.h[j]);
+                       ax = ax - c * (vi.x - vj.x);
+                       ay = ay - c * (vi.y - vj.y);
+                     });
+  } else {
+    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+      double4 q[NB_BATCH];
+      double2 vj[NB_BATCH];
+      double hj[NB_BATCH];
+      bool use[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        q[u] = s.nb[js[u]];
+        vj[u] = o.vel[js[u]];
+        hj[u] = o.h[js[u]];
+        use[u] = ok[u] && !(skip0 && s.id[js[u]] < P.npoin0);
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh, w, gx, gy;
+        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+        sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+        const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
+        const double tx = ax - c * (vi.x - vj[u].x), ty = ay - c * (vi.y - vj[u].y);
+        ax = use[u] ? tx : ax;
+        ay = use[u] ? ty : ay;
+      }
+    });
+  }
   vi.x = vi.x + 0.3 * ax;
   vi.y = vi.y + 0.3 * ay;
   vout[i] = vi;
@@ -1734,6 +1922,7 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
   P.c8 = c[8];
   P.c47 = c[47];
   P.pi = 4.0 * atan(1.0);
+  P.kc = p->skf == 1 ? (15.0 / 7.0) / P.pi : (7.0 / 4.0) / P.pi;
   if (p->integrator == 4) {
     const double f1[4] = {0., 0.5, 0.5, 1.0}, f2[4] = {1., 2., 2., 1.0};
     P.nstages = 4;
@@ -1796,19 +1985,32 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
   return GFSPH_OK;
 }
 
-#define SKF_DISPATCH(skf, KERNEL, ...)                 \
-  do {                                                 \
-    switch (skf) {                                     \
-      case 1: KERNEL<1><<<__VA_ARGS__; break;           \
-      case 2: KERNEL<2><<<__VA_ARGS__; break;           \
-      case 3: KERNEL<3><<<__VA_ARGS__; break;           \
-      default: KERNEL<4><<<__VA_ARGS__; break;          \
-    }                                                  \
+#define SKF_DISPATCH(KERNEL, ...)              \
+  do {                                         \
+    switch (c->P.skf) {                        \
+      case 1: KERNEL<1><<<__VA_ARGS__; break;  \
+      case 2: KERNEL<2><<<__VA_ARGS__; break;  \
+      case 3: KERNEL<3><<<__VA_ARGS__; break;  \
+      default: KERNEL<4><<<__VA_ARGS__; break; \
+    }                                          \
   } while (0)
 
+#define PFORCE_CASE(SKFV)                                                                                  \
+  switch (c->P.pa_sph) {                                                                                   \
+    case 1: k_pforce_m<SKFV, 1><<<c->gridN, GF_BLOCK, 0, c->stream>>>(PFORCE_ARGS); break;                 \
+    case 2: k_pforce_m<SKFV, 2><<<c->gridN, GF_BLOCK, 0, c->stream>>>(PFORCE_ARGS); break;                 \
+    default: k_pforce_m<SKFV, 4><<<c->gridN, GF_BLOCK, 0, c->stream>>>(PFORCE_ARGS); break;                \
+  }
+
 static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
-  SKF_DISPATCH(c->P.skf, k_pforce, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s,
-                                                                        c->S, c->Pp, c->ain));
+#define PFORCE_ARGS c->n, c->P, c->grid, c->start, c->mask, s, c->S, c->Pp, c->ain
+  switch (c->P.skf) {
+    case 1: PFORCE_CASE(1) break;
+    case 2: PFORCE_CASE(2) break;
+    case 3: PFORCE_CASE(3) break;
+    default: PFORCE_CASE(4) break;
+  }
+#undef PFORCE_ARGS
 }
 
 // RK stages of one step on the sorted set s (pressure force in c->ain).
@@ -1857,15 +2059,15 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 }
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
-  SKF_DISPATCH(c->P.skf, k_density, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, s, c->S,
-                                                                          c->Pp, c->mask));
+  SKF_DISPATCH(k_density_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, s, c->S, c->Pp,
+                                                                 c->mask));
 }
 
 // XSPH (or the plain copy of the post-RK velocity) into the sorted set
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   if (c->P.av_vel)
-    SKF_DISPATCH(c->P.skf, k_xsph, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s,
-                                                                       c->nw, s.vel));
+    SKF_DISPATCH(k_xsph_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s, c->nw,
+                                                                s.vel));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
@@ -1986,7 +2188,10 @@ int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const d
   HIPCHK(c, hipMemcpyAsync(s0.mass, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.hsml, hsml, sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.u, u ? u : zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(s0.uaux, zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  // a state that is already initialised (init_input = 0) carries its bottom
+  // pore pressure in u: the reference keeps u = Uaux(1,:) (7:956-958)
+  const double *ua0 = (!p->init_input && p->icpwp == 3 && u) ? u : zd.data();
+  HIPCHK(c, hipMemcpyAsync(s0.uaux, ua0, sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.flag, zeros.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   const int nb = nblk(n);

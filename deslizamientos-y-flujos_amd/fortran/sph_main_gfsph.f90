@@ -1,0 +1,142 @@
+!  SPH_MAIN_GFSPH -- drop-in replacement of the reference's
+!  time_integration_sph ("7 SPH_MAIN_2019.f90":220-283) that advances the
+!  Geoflow-SPH module state on the GPU through the C ABI (gfsph_c).
+!
+!  It is compiled together with the reference's modules: it reads the
+!  particle arrays of SPH_Main_Vars_2019 ("5 SPH_Main_Vars_2019.f90":29-80),
+!  the material vector const(:) and pore-pressure switches of
+!  SPH_SW_Vars_2019 ("4 SPH_SW_Vars_2019.f90":17-27, 196), the DEM of
+!  SPH_FEM_topo_2019 ("2 SPH_FEM_topo_2019.f90":94-110) and dt_sph of the
+!  driver time variables, exactly as the reference step does, so the
+!  driver's call site keeps its no-argument form:
+!
+!      call time_integration_sph_gfsph          ! instead of time_integration_sph
+!      time_sph = time_sph + dt_sph
+!
+!  The first call uploads the state (already initialised by Init_sph, so
+!  init_input = 0) and the DEM with the reference's own nodal slopes
+!  topol(1:9,:); every call advances one step and copies the particle state
+!  back into the module arrays, so the reference's output routines
+!  (Out_Plot_SPH, Out_Save_SPH, Manage_pixel_Hazard_SPH) see the current
+!  state.  The build must use the same real kind as the reference modules
+!  (irk = double): -fdefault-real-8.
+
+module SPH_MAIN_GFSPH
+
+use, intrinsic :: iso_c_binding
+use gfsph_c
+use SPH_FEM_variable_types_2019
+use SPH_FEM_Driver_time_vars_2019
+use SPH_Main_Vars_2019
+use SPH_SW_Vars_2019
+use SPH_FEM_topo_2019
+
+implicit none
+private
+
+type(c_ptr), save :: ctx = c_null_ptr
+integer, save, public :: gfsph_integrator = 4     ! 4: RK4 as RK45 (7:746); 2: RK2
+integer, save, public :: gfsph_status = 0
+
+public :: time_integration_sph_gfsph, gfsph_release
+
+contains
+
+subroutine time_integration_sph_gfsph
+
+   integer(c_int) rc
+
+   if (.not. c_associated(ctx)) call create_context
+   rc = gfsph_step(ctx, 1_c_int32_t)
+   call check(rc, 'gfsph_step')
+   call pull_state
+
+end subroutine time_integration_sph_gfsph
+
+
+subroutine create_context
+
+   type(gfsph_params_t) :: p
+   type(gfsph_dem_t)    :: d
+   real(c_double),  allocatable, target :: t9(:,:), zz(:)
+   integer(c_int32_t), allocatable, target :: inm(:), ity(:)
+   integer(c_int) rc
+   integer k
+
+   if (ntotal .ne. npoin) then
+      write(*,*) ' gfsph: virtual, absorbing or vn0 particles are not supported'
+      stop
+   endif
+
+   p%integrator        = gfsph_integrator
+   p%pa_sph            = pa_sph
+   p%nnps              = nnps
+   p%sle               = sle
+   p%skf               = skf
+   p%summation_density = merge(1, 0, summation_density)
+   p%average_velocity  = merge(1, 0, average_velocity)
+   p%nor_density       = merge(1, 0, nor_density)
+   p%icpwp             = icpwp
+   p%npoin0            = npoin0
+   p%init_input        = 0          ! Init_sph has already run (6:1428-1452)
+   p%check_finite      = 0
+   p%dt                = dt_sph
+   p%h_inf             = h_inf_SW
+   p%pwp0rel           = pwp0rel
+   p%cst               = 0.0_c_double
+   do k = 1, min(110, size(const))
+      p%cst(k) = real(const(k), c_double)      ! REAL(4) values, widened
+   enddo
+
+   allocate (t9(9, npoig), zz(npoig), inm(npoig), ity(npoin))
+   t9(1:9, :) = topol(1:9, 1:npoig)
+   zz(:)      = topol(1, 1:npoig)
+   inm(:)     = poig_in_model(1:npoig)
+   ity(:)     = itype(1:npoin)
+   d%nx = npoigx;  d%ny = npoigy
+   d%xmin = xming; d%ymin = yming; d%dx = deltxg; d%dy = deltyg
+   d%xmax = xmaxg; d%ymax = ymaxg
+   d%Z        = c_loc(zz)
+   d%in_model = c_loc(inm)
+   d%topol    = c_loc(t9)
+
+   rc = gfsph_create(p, d, int(npoin, c_int32_t), x, vx, rho, mass, hsml, u, ity, ctx)
+   call check(rc, 'gfsph_create')
+   deallocate (t9, zz, inm, ity)
+
+end subroutine create_context
+
+
+subroutine pull_state
+
+   real(c_double), allocatable :: ua(:)
+   integer(c_int32_t), allocatable :: fl(:)
+   integer(c_int) rc
+
+   allocate (ua(npoin), fl(npoin))
+   rc = gfsph_get_state(ctx, x, vx, rho, u, ua, hsml, fl)
+   call check(rc, 'gfsph_get_state')
+   if (allocated(Uaux)) Uaux(1, 1:npoin) = ua(:)
+   If_Out_Domain(1:npoin) = fl(:)
+   deallocate (ua, fl)
+
+end subroutine pull_state
+
+
+subroutine gfsph_release
+   if (c_associated(ctx)) call gfsph_destroy(ctx)
+   ctx = c_null_ptr
+end subroutine gfsph_release
+
+
+subroutine check(rc, what)
+   integer(c_int), intent(in) :: rc
+   character(*), intent(in)   :: what
+   gfsph_status = rc
+   if (rc .ne. GFSPH_OK) then
+      write(*,*) ' gfsph: ', what, ' failed with status ', rc, ': ', gfsph_last_error_f(ctx)
+      stop 2
+   endif
+end subroutine check
+
+end module SPH_MAIN_GFSPH

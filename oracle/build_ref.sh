@@ -76,9 +76,24 @@ rm -f "$BLD/r7_edit.f90"
 
 compile_asis "7b GFL_MAIN_2019" r7b
 
-"$FC" -c $FFLAGS -module-dir "$BLD" "$HERE/ref_harness.f90" -o "$BLD/harness.o"
+"$FC" -c $FFLAGS -cpp -module-dir "$BLD" "$HERE/ref_harness.f90" -o "$BLD/harness.o"
 "$FC" $FFLAGS -o "$OUT/gfsph_ref" \
   "$BLD"/r1.o "$BLD"/r3.o "$BLD"/r2.o "$BLD"/r4.o "$BLD"/r5.o "$BLD"/r5a.o \
   "$BLD"/r5b.o "$BLD"/r5c.o "$BLD"/r5d.o "$BLD"/r5e.o "$BLD"/r5f.o \
   "$BLD"/r6.o "$BLD"/r7.o "$BLD"/r7b.o "$BLD"/harness.o
 echo "build_ref.sh: built $OUT/gfsph_ref"
+
+# The same harness driving the GPU step through the product's ISO_C_BINDING
+# layer (SPH_MAIN_GFSPH), linked against the in-tree libgfsph.so.
+PKG="$HERE/../deslizamientos-y-flujos_amd"
+if [ -f "$PKG/gfsph/libgfsph.so" ]; then
+  "$FC" -c $FFLAGS -module-dir "$BLD" "$PKG/fortran/gfsph_c.f90" -o "$BLD/gfsph_c.o"
+  "$FC" -c $FFLAGS -module-dir "$BLD" "$PKG/fortran/sph_main_gfsph.f90" -o "$BLD/sph_main_gfsph.o"
+  "$FC" -c $FFLAGS -cpp -DGFSPH_GPU -module-dir "$BLD" "$HERE/ref_harness.f90" -o "$BLD/harness_gpu.o"
+  "$FC" $FFLAGS -o "$OUT/gfsph_ref_gpu" \
+    "$BLD"/r1.o "$BLD"/r3.o "$BLD"/r2.o "$BLD"/r4.o "$BLD"/r5.o "$BLD"/r5a.o \
+    "$BLD"/r5b.o "$BLD"/r5c.o "$BLD"/r5d.o "$BLD"/r5e.o "$BLD"/r5f.o \
+    "$BLD"/r6.o "$BLD"/r7.o "$BLD"/r7b.o "$BLD"/gfsph_c.o "$BLD"/sph_main_gfsph.o "$BLD"/harness_gpu.o \
+    -L"$PKG/gfsph" -lgfsph -Wl,-rpath,'$ORIGIN/../../deslizamientos-y-flujos_amd/gfsph'
+  echo "build_ref.sh: built $OUT/gfsph_ref_gpu"
+fi

@@ -22,6 +22,9 @@
 !     int32  pair_i(npairs) pair_j(npairs)
 !     real64 w(npairs) dwdx(ndimn,npairs)
 !  It is built only by oracle/build_ref.sh and never ships with the product.
+!  Built with -DGFSPH_GPU it calls time_integration_sph_gfsph (module
+!  SPH_MAIN_GFSPH of the product's Fortran layer) instead, i.e. the
+!  reference's own inputs and Init_sph driving the GPU step.
 
 program gfsph_ref_harness
 
@@ -30,6 +33,9 @@ use SPH_FEM_Driver_time_vars_2019
 use SPH_MAIN_2019, only: Init_SPH, time_integration_sph, adaptive_dt_sph
 use SPH_Main_Vars_2019
 use SPH_SW_Vars_2019
+#ifdef GFSPH_GPU
+use SPH_MAIN_GFSPH, only: time_integration_sph_gfsph, gfsph_release
+#endif
 
 implicit none
 
@@ -95,11 +101,18 @@ do istep = 1, nsteps
    t_print_reset = t_print_reset + dt
    call adaptive_dt_sph
    itimestep_sph = itimestep_sph + 1
+#ifdef GFSPH_GPU
+   call time_integration_sph_gfsph
+#else
    call time_integration_sph
+#endif
    time_sph = time_sph + dt_sph
    time     = time + dt
    call maybe_dump (istep)
 enddo
+#ifdef GFSPH_GPU
+call gfsph_release
+#endif
 
 contains
 

@@ -19,6 +19,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_EXE = os.path.join(HERE, "_ref", "gfsph_ref")
+# the same harness with time_integration_sph replaced by the product's
+# ISO_C_BINDING drop-in (SPH_MAIN_GFSPH -> libgfsph.so)
+REF_GPU_EXE = os.path.join(HERE, "_ref", "gfsph_ref_gpu")
 
 
 def available() -> bool:
@@ -127,11 +130,11 @@ def read_dump(path: str) -> dict:
 
 
 def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, timeout: float = 3600.0,
-                  keep: bool = False) -> dict:
+                  keep: bool = False, exe: str = REF_EXE) -> dict:
     """Run the reference on ``case`` for ``nsteps`` RK4 steps; return
     {step: dump}.  Raises if the reference build is absent."""
-    if not available():
-        raise FileNotFoundError(REF_EXE)
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
     if case.params.integrator != 4:
         raise ValueError("the reference integrates with RK4 only")
     tmp = workdir or tempfile.mkdtemp(prefix="gfsph_ref_")
@@ -141,7 +144,7 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
         f.write("case\n")
         f.write(f"{nsteps} {_r(case.params.dt)}\n")
         f.write(f"{len(steps)} " + " ".join(str(s) for s in steps) + "\n")
-    r = subprocess.run([REF_EXE], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
+    r = subprocess.run([exe], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
                        timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")

@@ -1,0 +1,31 @@
+"""The drop-in boundary end to end: the reference's own Fortran input path
+(Init_sph reading <name>.dat, the .top grid and restart.dat) with
+time_integration_sph replaced by the product's ISO_C_BINDING layer
+(deslizamientos-y-flujos_amd/fortran/sph_main_gfsph.f90 -> libgfsph.so),
+against the reference's CPU run on the same inputs (golden fixtures)."""
+import os
+
+import numpy as np
+import pytest
+
+import goldens
+from oracle import refio
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_icpwp1_sle1", "g_curvature7"]
+
+
+@pytest.mark.skipif(not os.path.exists(refio.REF_GPU_EXE),
+                    reason="oracle/_ref/gfsph_ref_gpu is built only where the reference sources are present")
+@pytest.mark.parametrize("name", CASES)
+def test_fortran_driver_with_gpu_step_matches_reference(name, tmp_path):
+    case, f = goldens.case_of(name)
+    steps = [int(s) for s in f["dumps"] if s > 0]
+    out = refio.run_reference(case, case.nsteps, steps, workdir=str(tmp_path), exe=refio.REF_GPU_EXE)
+    for s in steps:
+        tol = 1e-10 if s >= 100 else 1e-8
+        for k in ("x", "v", "h", "u", "hsml"):
+            err = goldens.rel_err(out[s][k], f[f"s{s}_{k}"])
+            assert err <= tol, (name, s, k, err)
+        assert np.array_equal(out[s]["flag"], f[f"s{s}_flag"])
