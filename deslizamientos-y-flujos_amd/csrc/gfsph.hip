@@ -75,6 +75,17 @@ struct SortedSet {
   double2 *pos, *vel;
   double *h, *mass, *hsml, *u, *uaux;
   int *flag, *id;
+  int *rid;  // realisation of the particle (0 for a single run)
+};
+
+// Cell index of one or several independent realisations: realisation r has
+// its own grid (bounding box of its particles) and owns the cell keys
+// [coff[r], coff[r+1]); out-of-domain particles take key coff[R].
+struct Cells {
+  const GridInfo *grid;  // [R]
+  const int *coff;       // [R+1]
+  const int *ntot;       // coff + R: number of cells of all realisations
+  const int *start;      // [ntot+2] first slot of every key
 };
 
 // step outputs in the same (cell) order as the sorted set that produced them
@@ -90,7 +101,7 @@ struct NewSet {
 struct CurView {
   const double2 *pos, *vel;
   const double *h, *mass, *hsml, *u, *uaux;
-  const int *flag, *id;
+  const int *flag, *id, *rid;
 };
 
 __device__ __forceinline__ unsigned long long ord_of(double v) {
@@ -674,7 +685,8 @@ __device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int
 }
 
 // One block: reduces the bounding-box partials, stores the total and the grid.
-__global__ void k_grid(int nparts, const BBoxAcc *part, BBoxAcc *acc, GridInfo *gout, int scale_k, int cap_cells) {
+__global__ void k_grid(int nparts, const BBoxAcc *part, BBoxAcc *acc, GridInfo *gout, int *coff, int scale_k,
+                       int cap_cells) {
   __shared__ BBoxAcc lds[GF_BLOCK];
   BBoxAcc a;
   a.xmin = ~0ull;
@@ -713,12 +725,78 @@ __global__ void k_grid(int nparts, const BBoxAcc *part, BBoxAcc *acc, GridInfo *
   }
   if (threadIdx.x == 0) {
     *acc = a;
-    *gout = make_grid(a, scale_k, cap_cells);
+    const GridInfo g = make_grid(a, scale_k, cap_cells);
+    *gout = g;
+    coff[0] = 0;
+    coff[1] = g.ncell;
   }
 }
 
-__global__ void k_clear(const GridInfo *gp, int *cnt) {
-  int nb = gp->ncell + 1;
+// Several realisations: slot range of every realisation in the order of the
+// last sort (its cells are contiguous keys), read before the grids change.
+__global__ void k_rng(int R, const int *coff, const int *start, int2 *rng) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) rng[r] = make_int2(start[coff[r]], start[coff[r + 1]]);
+}
+
+// One block per realisation: bounding box of its active particles (slot
+// range rng[r] of the step's output set) and its grid, capped at
+// 16 cells per particle + 64.
+__global__ void k_bbox_real(const int2 *rng, const double2 *pos, const double *hsml, const int *flag, int scale_k,
+                            GridInfo *grid) {
+  __shared__ BBoxAcc wp[GF_BLOCK / 64];
+  const int r = blockIdx.x;
+  const int2 rg = rng[r];
+  const unsigned long long BIG = ~0ull, SMALL = 0ull;
+  unsigned long long oxmin = BIG, oxmax = SMALL, oymin = BIG, oymax = SMALL, ohmax = SMALL, ohmin = BIG;
+  unsigned cnt = 0u;
+  for (int i = rg.x + (int)threadIdx.x; i < rg.y; i += GF_BLOCK) {
+    if (flag[i]) continue;
+    const double2 p = pos[i];
+    const unsigned long long ox = ord_of(p.x), oy = ord_of(p.y), oh = ord_of(hsml[i]);
+    oxmin = ox < oxmin ? ox : oxmin;
+    oxmax = ox > oxmax ? ox : oxmax;
+    oymin = oy < oymin ? oy : oymin;
+    oymax = oy > oymax ? oy : oymax;
+    ohmax = oh > ohmax ? oh : ohmax;
+    ohmin = oh < ohmin ? oh : ohmin;
+    cnt++;
+  }
+  oxmin = wave_min_u64(oxmin);
+  oxmax = wave_max_u64(oxmax);
+  oymin = wave_min_u64(oymin);
+  oymax = wave_max_u64(oymax);
+  ohmax = wave_max_u64(ohmax);
+  ohmin = wave_min_u64(ohmin);
+  cnt = wave_sum_u32(cnt);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    wp[wid].xmin = oxmin;
+    wp[wid].xmax = oxmax;
+    wp[wid].ymin = oymin;
+    wp[wid].ymax = oymax;
+    wp[wid].hmax = ohmax;
+    wp[wid].hmin = ohmin;
+    wp[wid].nactive = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    BBoxAcc a = wp[0];
+    for (int w = 1; w < GF_BLOCK / 64; w++) {
+      a.xmin = wp[w].xmin < a.xmin ? wp[w].xmin : a.xmin;
+      a.xmax = wp[w].xmax > a.xmax ? wp[w].xmax : a.xmax;
+      a.ymin = wp[w].ymin < a.ymin ? wp[w].ymin : a.ymin;
+      a.ymax = wp[w].ymax > a.ymax ? wp[w].ymax : a.ymax;
+      a.hmax = wp[w].hmax > a.hmax ? wp[w].hmax : a.hmax;
+      a.hmin = wp[w].hmin < a.hmin ? wp[w].hmin : a.hmin;
+      a.nactive += wp[w].nactive;
+    }
+    grid[r] = make_grid(a, scale_k, 16 * (rg.y - rg.x) + 64);
+  }
+}
+
+__global__ void k_clear(const int *ntot, int *cnt) {
+  int nb = *ntot + 1;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += gridDim.x * blockDim.x) cnt[c] = 0;
 }
 
@@ -729,18 +807,19 @@ __device__ __forceinline__ void cell_of(const GridInfo &g, double2 p, int &cx, i
   cy = cy < 0 ? 0 : (cy >= g.ncy ? g.ncy - 1 : cy);
 }
 
-__global__ void k_keys(int n, const double2 *pos, const int *flag, const GridInfo *gp, int *key, int *rank,
-                       int *cnt) {
+__global__ void k_keys(int n, const double2 *pos, const int *flag, const int *rid, const GridInfo *grid,
+                       const int *coff, const int *ntot, int *key, int *rank, int *cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  GridInfo g = *gp;
   int k;
   if (flag[i]) {
-    k = g.ncell;
+    k = *ntot;
   } else {
+    const int r = rid[i];
+    const GridInfo g = grid[r];
     int cx, cy;
     cell_of(g, pos[i], cx, cy);
-    k = cy * g.ncx + cx;
+    k = coff[r] + cy * g.ncx + cx;
   }
   key[i] = k;
   rank[i] = atomicAdd(&cnt[k], 1);
@@ -776,9 +855,9 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
   return res;
 }
 
-__global__ void k_scan_reduce(const GridInfo *gp, const int *cnt, int *bsum) {
+__global__ void k_scan_reduce(const int *ntot, const int *cnt, int *bsum) {
   __shared__ int lds[GF_BLOCK / 64 + 1];
-  int nb = gp->ncell + 1;
+  int nb = *ntot + 1;
   int base = blockIdx.x * SCAN_TILE;
   if (base >= nb) {
     if (threadIdx.x == 0) bsum[blockIdx.x] = 0;
@@ -807,9 +886,9 @@ __global__ void k_scan_top(int nblocks, int *bsum) {
   }
 }
 
-__global__ void k_scan_down(const GridInfo *gp, const int *cnt, const int *bsum, int *start, int n) {
+__global__ void k_scan_down(const int *ntot, const int *cnt, const int *bsum, int *start, int n) {
   __shared__ int lds[GF_BLOCK / 64 + 1];
-  int nb = gp->ncell + 1;
+  int nb = *ntot + 1;
   int base = blockIdx.x * SCAN_TILE;
   if (base >= nb) return;
   int v[SCAN_ITEMS];
@@ -829,6 +908,21 @@ __global__ void k_scan_down(const GridInfo *gp, const int *cnt, const int *bsum,
   if (blockIdx.x == 0 && threadIdx.x == 0) start[nb] = n;
 }
 
+// exclusive scan of the realisations' cell counts -> coff[0..R]
+__global__ void k_coff(int R, const GridInfo *grid, int *coff) {
+  __shared__ int lds[GF_BLOCK / 64 + 1];
+  int carry = 0;
+  for (int base = 0; base < R; base += GF_BLOCK) {
+    const int r = base + (int)threadIdx.x;
+    const int v = r < R ? grid[r].ncell : 0;
+    int total;
+    const int ex = block_exclusive_scan(v, lds, total);
+    if (r < R) coff[r] = ex + carry;
+    carry += total;
+  }
+  if (threadIdx.x == 0) coff[R] = carry;
+}
+
 __global__ void k_scatter(int n, const int *key, const int *rank, const int *start, int *perm) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -842,14 +936,14 @@ __global__ void k_scatter(int n, const int *key, const int *rank, const int *sta
 // its particles never interact).  Sources: positions / depth / smoothing
 // length / pore pressure / flags from the previous step's outputs (src_new),
 // mass / id / velocity from the previous sorted set (src_old).
-__global__ void k_reorder(int n, const GridInfo *gp, const int *perm, const int *key, const int *start,
+__global__ void k_reorder(int n, const int *ntot, const int *perm, const int *key, const int *start,
                           NewSet src_new, SortedSet src_old, SortedSet dst) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int i = perm[t];
   int c = key[i];
   int d = t;
-  if (c < gp->ncell) {
+  if (c < *ntot) {
     int b = start[c], e = start[c + 1], rank = 0;
     for (int u = b; u < e; u++) rank += perm[u] < i;
     d = b + rank;
@@ -866,6 +960,7 @@ __global__ void k_reorder(int n, const GridInfo *gp, const int *perm, const int 
   dst.uaux[d] = src_new.uaux[i];
   dst.flag[d] = src_new.flag[i];
   dst.id[d] = src_old.id[i];
+  dst.rid[d] = src_old.rid[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -992,13 +1087,22 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 // Bingham, whose stages couple particles): DEM lookup, gravity, rheology,
 // limiter, stage update, then the end-of-step updates
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, DevParams P, DevDem dem, SortedSet s, const double *S,
-                                                  const double2 *ain, NewSet o, BBoxAcc *part) {
+// Pr holds one parameter block per realisation; the lanes of a wave are
+// served one realisation at a time (normally one pass), so every parameter
+// read is a scalar load.  part = nullptr: no bounding-box partials (several
+// realisations: k_bbox_real computes their boxes).
+__global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
+                                                  const double *S, const double2 *ain, NewSet o, BBoxAcc *part) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
   double hso = 0;
-  if (i < n) {
+  const int myr = i < n ? s.rid[i] : 0;
+  for (bool todo = i < n; todo;) {
+    const int r0 = __builtin_amdgcn_readfirstlane(myr);
+    if (myr != r0) continue;
+    todo = false;
+    const DevParams &P = Pr[r0];
     if (s.flag[i]) {
       write_frozen(P, s, i, o);
     } else {
@@ -1030,7 +1134,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, DevParams P, DevDem dem,
       finish_particle(P, dem, s, i, Si, rfi, x, v, u, Dx, Dy, Dvx, Dvy, Du, flagged, o, active_out, xo, hso);
     }
   }
-  bbox_block(part, active_out, xo, hso);
+  if (part) bbox_block(part, active_out, xo, hso);
 }
 
 // ---------------------------------------------------------------------------
@@ -1369,8 +1473,7 @@ __device__ __forceinline__ bool load_mask(const unsigned long long *mask, int i,
 
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
 template <int SKF>
-__global__ __launch_bounds__(GF_BLOCK) void k_density_m(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                         SortedSet s, double *S, double *Pout,
+__global__ __launch_bounds__(GF_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S, double *Pout,
                                                          unsigned long long *mask) {
   const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
@@ -1379,7 +1482,9 @@ __global__ __launch_bounds__(GF_BLOCK) void k_density_m(int n, DevParams P, cons
     Pout[i] = 0.0;
     return;
   }
-  const GridInfo g = *gp;
+  const int rr_ = s.rid[i];
+  const GridInfo g = cl.grid[rr_];
+  const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   const RowSpan rs = row_spans(g, start, pi_);
@@ -1443,8 +1548,7 @@ __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, do
 
 // a5 pressure force (IntForces_SW pair sum, 6:6892-6941) -> ain[i]
 template <int SKF, int PA>
-__global__ __launch_bounds__(GF_BLOCK) void k_pforce_m(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                        const unsigned long long *mask, SortedSet s, const double *S,
+__global__ __launch_bounds__(GF_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, const double *S,
                                                         const double *Pp, double2 *ain) {
   const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
@@ -1452,7 +1556,9 @@ __global__ __launch_bounds__(GF_BLOCK) void k_pforce_m(int n, DevParams P, const
     ain[i] = make_double2(0.0, 0.0);
     return;
   }
-  const GridInfo g = *gp;
+  const int rr_ = s.rid[i];
+  const GridInfo g = cl.grid[rr_];
+  const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   unsigned long long m[NB_WORDS];
@@ -1496,8 +1602,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_pforce_m(int n, DevParams P, const
 
 // a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
 template <int SKF>
-__global__ __launch_bounds__(GF_BLOCK) void k_xsph_m(int n, DevParams P, const GridInfo *gp, const int *start,
-                                                      const unsigned long long *mask, SortedSet s, NewSet o,
+__global__ __launch_bounds__(GF_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, NewSet o,
                                                       double2 *vout) {
   const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
@@ -1506,7 +1611,9 @@ __global__ __launch_bounds__(GF_BLOCK) void k_xsph_m(int n, DevParams P, const G
     vout[i] = vi;
     return;
   }
-  const GridInfo g = *gp;
+  const int rr_ = s.rid[i];
+  const GridInfo g = cl.grid[rr_];
+  const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   unsigned long long m[NB_WORDS];
@@ -1655,12 +1762,21 @@ __global__ void k_nonfinite(int n, CurView s, int *bad) {
   if (!isfinite(q)) atomicOr(bad, 1);
 }
 
+__global__ void k_count_active(int n, const int *flag, int *count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned a = (i < n && flag[i] == 0) ? 1u : 0u;
+  a = wave_sum_u32(a);
+  if ((threadIdx.x & 63) == 0 && a) atomicAdd(count, (int)a);
+}
+
 // pair enumeration (i<j by caller index) for the neighbour-set export
-__global__ void k_pairs(int n, int skf, int scale_k, const GridInfo *gp, const int *start, SortedSet s,
-                        long long cap, unsigned long long *count, int *ij) {
+__global__ void k_pairs(int n, int skf, int scale_k, Cells cl, SortedSet s, long long cap,
+                        unsigned long long *count, int *ij) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || s.flag[i]) return;
-  const GridInfo g = *gp;
+  const int rr_ = s.rid[i];
+  const GridInfo g = cl.grid[rr_];
+  const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   const int idi = s.id[i];
@@ -1722,16 +1838,18 @@ __device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
   atomicMax((unsigned long long *)addr, (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void k_mc_pixel(int n, DevDem dem, CurView s, double *acc4, int *cnt) {
+// stride > 0: per-realisation accumulators, realisation r at offset r*stride nodes
+__global__ void k_mc_pixel(int n, DevDem dem, CurView s, long stride, double *acc4, int *cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || s.flag[i]) return;
+  const long base = stride * s.rid[i];
   double2 p = s.pos[i];
   double2 v = s.vel[i];
   int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
   if (ipx == dem.nx && fabs(dem.xmax - p.x) <= 1.e-3) ipx--;
   if (ipy == dem.ny && fabs(dem.ymax - p.y) <= 1.e-3) ipy--;
   if (ipx < 1 || ipy < 1 || ipx > dem.nx - 1 || ipy > dem.ny - 1) return;
-  long g1 = (long)(ipy - 1) * dem.nx + ipx - 1;
+  long g1 = base + (long)(ipy - 1) * dem.nx + ipx - 1;
   long nodes[4] = {g1, g1 + 1, g1 + 1 + dem.nx, g1 + dem.nx};
   double hh = s.h[i];
   double vv = pow(v.x * v.x + v.y * v.y, 0.5);
@@ -1743,6 +1861,49 @@ __global__ void k_mc_pixel(int n, DevDem dem, CurView s, double *acc4, int *cnt)
     atomic_max_pos(&acc4[4 * g + 1], hh);
     atomic_max_pos(&acc4[4 * g + 3], vv);
   }
+}
+
+// Monte Carlo statistics per node over the realisations (Norm_MC_pixel_Vars_SW,
+// 6:20838-20908): each realisation that reached the node contributes its
+// per-node averages (sums over plot times / visits) and maxima; the mean and
+// standard deviation (variance below 1e-6 taken as 0) are over those
+// realisations.  Also the maxima over all realisations and the fraction of
+// realisations whose maximum depth exceeds hthr.
+__global__ void k_mc_hazard(long np, int R, const double *acc4, const int *cnt, double hthr, double *stats,
+                            int *nprob, double *maxs, double *pexc) {
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= np) return;
+  double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0}, mh = 0.0, mv = 0.0;
+  int np_r = 0, nexc = 0;
+  for (int r = 0; r < R; r++) {
+    const long k = (long)r * np + g;
+    const int c = cnt[k];
+    if (c == 0) continue;
+    np_r++;
+    const double4 a = reinterpret_cast<const double4 *>(acc4)[k];
+    const double o[4] = {a.x / c, a.y, a.z / c, a.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      s1[q] += o[q];
+      s2[q] += o[q] * o[q];
+    }
+    mh = fmax(mh, a.y);
+    mv = fmax(mv, a.w);
+    nexc += a.y > hthr ? 1 : 0;
+  }
+  const int nd = np_r > 0 ? np_r : 1;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const double mean = s1[q] / nd, sig2 = s2[q] / nd;
+    double t = sig2 - mean * mean;
+    if (t < 1.e-6) t = 0.0;
+    stats[8 * g + 2 * q] = mean;
+    stats[8 * g + 2 * q + 1] = sqrt(t);
+  }
+  nprob[g] = np_r;
+  maxs[2 * g] = mh;
+  maxs[2 * g + 1] = mv;
+  pexc[g] = (double)nexc / R;
 }
 
 // ---------------------------------------------------------------------------
@@ -1776,8 +1937,27 @@ struct gfsph_ctx {
   BinghamState bs{};
   int lv_blocks = 0;
   int cur = 0;
+  // realisations (1 for a single run): per-realisation grids, cell offsets,
+  // slot ranges and parameter blocks
+  int R = 1;
+  int *coff = nullptr;
+  int2 *rng = nullptr;
+  DevParams *Pr = nullptr;
+  bool first_build = true;
+  // per-realisation Monte Carlo pixel accumulators [R][npoig] (lazily allocated)
+  double *mcr_acc = nullptr;
+  int *mcr_cnt = nullptr;
   std::vector<void *> allocs;
 };
+
+static Cells cells_of(const gfsph_ctx *c) {
+  Cells cl;
+  cl.grid = c->grid;
+  cl.coff = c->coff;
+  cl.ntot = c->coff + c->R;
+  cl.start = c->start;
+  return cl;
+}
 
 static CurView cur_view(const gfsph_ctx *c) {
   const SortedSet &s = c->set[c->cur];
@@ -1791,6 +1971,7 @@ static CurView cur_view(const gfsph_ctx *c) {
   v.uaux = c->nw.uaux;
   v.flag = c->nw.flag;
   v.id = s.id;
+  v.rid = s.rid;
   return v;
 }
 
@@ -1948,7 +2129,8 @@ static int alloc_sets(gfsph_ctx *c, int n) {
     SortedSet &s = c->set[k];
     if ((rc = dalloc(c, &s.nb, n)) || (rc = dalloc(c, &s.pos, n)) || (rc = dalloc(c, &s.vel, n)) || (rc = dalloc(c, &s.h, n)) ||
         (rc = dalloc(c, &s.mass, n)) || (rc = dalloc(c, &s.hsml, n)) || (rc = dalloc(c, &s.u, n)) ||
-        (rc = dalloc(c, &s.uaux, n)) || (rc = dalloc(c, &s.flag, n)) || (rc = dalloc(c, &s.id, n)))
+        (rc = dalloc(c, &s.uaux, n)) || (rc = dalloc(c, &s.flag, n)) || (rc = dalloc(c, &s.id, n)) ||
+        (rc = dalloc(c, &s.rid, n)))
       return rc;
   }
   NewSet &o = c->nw;
@@ -1960,10 +2142,13 @@ static int alloc_sets(gfsph_ctx *c, int n) {
   if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
       (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
     return rc;
-  c->cap_cells = 16 * n + 4096;
+  // a single run caps its grid at 16n + 4096 cells, realisation r of a batch
+  // at 16 n_r + 64
+  c->cap_cells = 16 * n + 64 * c->R + 4096;
   c->scan_blocks = (c->cap_cells + 2 + SCAN_TILE - 1) / SCAN_TILE;
   if ((rc = dalloc(c, &c->cnt, (size_t)c->cap_cells + 2)) || (rc = dalloc(c, &c->start, (size_t)c->cap_cells + 2)) ||
-      (rc = dalloc(c, &c->bsum, (size_t)c->scan_blocks)) || (rc = dalloc(c, &c->grid, 1)) ||
+      (rc = dalloc(c, &c->bsum, (size_t)c->scan_blocks)) || (rc = dalloc(c, &c->grid, c->R)) ||
+      (rc = dalloc(c, &c->coff, c->R + 1)) || (rc = dalloc(c, &c->rng, c->R)) || (rc = dalloc(c, &c->Pr, c->R)) ||
       (rc = dalloc(c, &c->acc, 1)) || (rc = dalloc(c, &c->bad, 1)))
     return rc;
   c->gridN = ((nblk(n) + 7) / 8) * 8;
@@ -2003,7 +2188,7 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
   }
 
 static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
-#define PFORCE_ARGS c->n, c->P, c->grid, c->start, c->mask, s, c->S, c->Pp, c->ain
+#define PFORCE_ARGS c->n, c->P, cells_of(c), c->mask, s, c->S, c->Pp, c->ain
   switch (c->P.skf) {
     case 1: PFORCE_CASE(1) break;
     case 2: PFORCE_CASE(2) break;
@@ -2019,7 +2204,7 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   hipStream_t st = c->stream;
   const int nb = nblk(n);
   if (c->P.nfric != 4) {
-    k_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->dem, s, c->S, c->ain, c->nw, c->part);
+    k_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw, c->R == 1 ? c->part : nullptr);
     return;
   }
   BinghamState &b = c->bs;
@@ -2034,21 +2219,32 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->dem, s, c->S, b, c->nw, c->part);
 }
 
-// Neighbour build into set[dst] from (nw, set[src]); acc must hold the
-// bounding box of the active particles of nw.
+// Neighbour build into set[dst] from (nw, set[src]).  A single run takes its
+// grid from the bounding-box partials of the step that wrote nw; a batch
+// computes every realisation's box over its slot range (from the previous
+// sort, or from the caller's layout at the first build).
 static int neighbour_build(gfsph_ctx *c, int src, int dst) {
-  const int n = c->n;
+  const int n = c->n, R = c->R;
   hipStream_t st = c->stream;
   const int nb = nblk(n);
   int clear_blocks = 1024;
-  k_grid<<<1, GF_BLOCK, 0, st>>>(c->gridN, c->part, c->acc, c->grid, c->P.scale_k, c->cap_cells);
-  k_clear<<<clear_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt);
-  k_keys<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.flag, c->grid, c->key, c->rank, c->cnt);
-  k_scan_reduce<<<c->scan_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt, c->bsum);
+  const int *ntot = c->coff + R;
+  if (R == 1) {
+    k_grid<<<1, GF_BLOCK, 0, st>>>(c->gridN, c->part, c->acc, c->grid, c->coff, c->P.scale_k, c->cap_cells);
+  } else {
+    if (!c->first_build) k_rng<<<(R + GF_BLOCK - 1) / GF_BLOCK, GF_BLOCK, 0, st>>>(R, c->coff, c->start, c->rng);
+    k_bbox_real<<<R, GF_BLOCK, 0, st>>>(c->rng, c->nw.pos, c->nw.hsml, c->nw.flag, c->P.scale_k, c->grid);
+    k_coff<<<1, GF_BLOCK, 0, st>>>(R, c->grid, c->coff);
+  }
+  c->first_build = false;
+  k_clear<<<clear_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt);
+  k_keys<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.flag, c->set[src].rid, c->grid, c->coff, ntot, c->key,
+                                  c->rank, c->cnt);
+  k_scan_reduce<<<c->scan_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt, c->bsum);
   k_scan_top<<<1, GF_BLOCK, 0, st>>>(c->scan_blocks, c->bsum);
-  k_scan_down<<<c->scan_blocks, GF_BLOCK, 0, st>>>(c->grid, c->cnt, c->bsum, c->start, n);
+  k_scan_down<<<c->scan_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt, c->bsum, c->start, n);
   k_scatter<<<nb, GF_BLOCK, 0, st>>>(n, c->key, c->rank, c->start, c->perm);
-  k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, c->grid, c->perm, c->key, c->start, c->nw, c->set[src], c->set[dst]);
+  k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, ntot, c->perm, c->key, c->start, c->nw, c->set[src], c->set[dst]);
   HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
 }
@@ -2059,15 +2255,13 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 }
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
-  SKF_DISPATCH(k_density_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, s, c->S, c->Pp,
-                                                                 c->mask));
+  SKF_DISPATCH(k_density_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
 }
 
 // XSPH (or the plain copy of the post-RK velocity) into the sorted set
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   if (c->P.av_vel)
-    SKF_DISPATCH(k_xsph_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, c->grid, c->start, c->mask, s, c->nw,
-                                                                s.vel));
+    SKF_DISPATCH(k_xsph_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
@@ -2114,19 +2308,57 @@ static int one_step(gfsph_ctx *c) {
 
 extern "C" {
 
-int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const double *x, const double *v,
-                 const double *h, const double *mass, const double *hsml, const double *u, const int32_t *itype,
-                 gfsph_ctx **out) {
-  if (!out || !p || !dem || n <= 0 || !x || !v || !h || !mass || !hsml || !dem->Z) return GFSPH_ERR_ARG;
+// Shared by gfsph_create (one realisation) and gfsph_batch_create: R
+// realisations of counts[r] particles each, stored one after another; cstR
+// ([R][111]) gives each realisation's const(:) (NULL: p->cst for all).
+static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, const int32_t *counts,
+                       const double *cstR, int32_t n, const double *x, const double *v, const double *h,
+                       const double *mass, const double *hsml, const double *u, const int32_t *itype,
+                       gfsph_ctx **out) {
+  if (!out || !p || !dem || n <= 0 || !x || !v || !h || !mass || !hsml || !dem->Z || R < 1) return GFSPH_ERR_ARG;
   *out = nullptr;
   gfsph_ctx *c = new gfsph_ctx();
   std::string err;
-  int rc = derive_params(p, c->P, err);
+  std::vector<DevParams> Ph(R);
+  int rc = GFSPH_OK;
+  bool any_frictional = false;
+  for (int r = 0; r < R && !rc; r++) {
+    gfsph_params pr = *p;
+    if (cstR) memcpy(pr.cst, cstR + 111 * (size_t)r, sizeof(pr.cst));
+    rc = derive_params(&pr, Ph[r], err);
+    if (!rc && R > 1) {
+      if (Ph[r].nfric == 4) {
+        err = "Bingham (nfric = 4) couples the particles of a run; it is not batched";
+        rc = GFSPH_ERR_UNSUPPORTED;
+      } else if (Ph[r].cgra != Ph[0].cgra || Ph[r].alphaS != Ph[0].alphaS) {
+        err = "the realisations of a batch must share const(1) and const(14)";
+        rc = GFSPH_ERR_ARG;
+      }
+    }
+    if (!rc) any_frictional = any_frictional || Ph[r].if_frictional;
+  }
+  if (!rc && R > 1) {
+    long tot = 0;
+    for (int r = 0; r < R; r++) {
+      if (counts[r] <= 0) {
+        err = "every realisation needs at least one particle";
+        rc = GFSPH_ERR_ARG;
+        break;
+      }
+      tot += counts[r];
+    }
+    if (!rc && tot != n) {
+      err = "the realisation counts do not add up to n";
+      rc = GFSPH_ERR_ARG;
+    }
+  }
   if (rc) {
     c->err = err;
     *out = c;
     return rc;
   }
+  c->P = Ph[0];
+  c->R = R;
   if (dem->nx < 2 || dem->ny < 2 || !(dem->dx > 0) || !(dem->dy > 0)) {
     *out = c;
     return fail(c, GFSPH_ERR_ARG, "DEM needs at least 2x2 nodes and positive spacing");
@@ -2156,7 +2388,7 @@ int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const d
   c->dem.xmax = dem->xmax;
   c->dem.ymax = dem->ymax;
   if ((rc = dalloc(c, &c->zxy, np))) return rc;
-  if (c->P.if_frictional)
+  if (any_frictional)
     if ((rc = dalloc(c, &c->top, 6 * np))) return rc;
   if (dem->topol) {
     double *t9;
@@ -2194,6 +2426,22 @@ int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const d
   HIPCHK(c, hipMemcpyAsync(s0.uaux, ua0, sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.flag, zeros.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  // realisation of every particle and the realisations' slot ranges in the caller's layout
+  std::vector<int> rid(n, 0);
+  std::vector<int2> rg(R);
+  if (R == 1) {
+    rg[0] = make_int2(0, n);
+  } else {
+    int o = 0;
+    for (int r = 0; r < R; r++) {
+      rg[r] = make_int2(o, o + counts[r]);
+      for (int k = 0; k < counts[r]; k++) rid[o + k] = r;
+      o += counts[r];
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(s0.rid, rid.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->rng, rg.data(), sizeof(int2) * R, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->Pr, Ph.data(), sizeof(DevParams) * R, hipMemcpyHostToDevice, st));
   const int nb = nblk(n);
   k_domain_flag<<<nb, GF_BLOCK, 0, st>>>(n, c->dem, s0.pos, s0.flag);
   k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw);
@@ -2210,6 +2458,23 @@ int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const d
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(st));
   return GFSPH_OK;
+}
+
+int gfsph_create(const gfsph_params *p, const gfsph_dem *dem, int32_t n, const double *x, const double *v,
+                 const double *h, const double *mass, const double *hsml, const double *u, const int32_t *itype,
+                 gfsph_ctx **out) {
+  const int32_t one[1] = {n};
+  return create_impl(p, dem, 1, one, nullptr, n, x, v, h, mass, hsml, u, itype, out);
+}
+
+int gfsph_batch_create(const gfsph_params *p, const gfsph_dem *dem, int32_t nreal, const int32_t *counts,
+                       const double *cst, const double *x, const double *v, const double *h, const double *mass,
+                       const double *hsml, const double *u, gfsph_ctx **out) {
+  if (!out || nreal < 1 || !counts) return GFSPH_ERR_ARG;
+  long n = 0;
+  for (int r = 0; r < nreal; r++) n += counts[r] > 0 ? counts[r] : 0;
+  if (n <= 0 || n > 0x7fffffffL) return GFSPH_ERR_ARG;
+  return create_impl(p, dem, nreal, counts, cst, (int32_t)n, x, v, h, mass, hsml, u, nullptr, out);
 }
 
 int gfsph_step(gfsph_ctx *c, int32_t nsteps) {
@@ -2305,6 +2570,7 @@ int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double 
     if (!e) e = hipMemcpyAsync(s0.hsml, hsml, sizeof(double) * n, hipMemcpyHostToDevice, st);
     if (!e) e = hipMemcpyAsync(s0.flag, fl.data(), sizeof(int) * n, hipMemcpyHostToDevice, st);
     if (!e) e = hipMemcpyAsync(s0.id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemsetAsync(s0.rid, 0, sizeof(int) * n, st);
     for (auto arr : {s0.h, s0.mass, s0.u, s0.uaux})
       if (!e) e = hipMemcpyAsync(arr, zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st);
     if (!e) e = hipMemcpyAsync(s0.vel, zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st);
@@ -2317,7 +2583,7 @@ int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double 
       rc = neighbour_build(c, 0, 1);
     }
     if (!rc) {
-      k_pairs<<<nblk(n), GF_BLOCK, 0, st>>>(n, skf, c->P.scale_k, c->grid, c->start, c->set[1], cap, dcount, dij);
+      k_pairs<<<nblk(n), GF_BLOCK, 0, st>>>(n, skf, c->P.scale_k, cells_of(c), c->set[1], cap, dcount, dij);
       e = hipMemcpyAsync(&count, dcount, sizeof(count), hipMemcpyDeviceToHost, st);
       if (!e) e = hipStreamSynchronize(st);
       if (e) rc = GFSPH_ERR_HIP;
@@ -2361,7 +2627,7 @@ int gfsph_mc_pixel_accumulate(gfsph_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->mc_acc, 0, sizeof(double) * 4 * np, c->stream));
     HIPCHK(c, hipMemsetAsync(c->mc_cnt, 0, sizeof(int) * np, c->stream));
   }
-  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), c->mc_acc, c->mc_cnt);
+  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), 0, c->mc_acc, c->mc_cnt);
   HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
 }
@@ -2428,16 +2694,72 @@ int gfsph_profile_step(gfsph_ctx *c, int32_t nrep, int32_t cap, int32_t *nkernel
   return GFSPH_OK;
 }
 
+static int ensure_mcr(gfsph_ctx *c) {
+  if (c->mcr_acc) return GFSPH_OK;
+  const long np = (long)c->dem.nx * c->dem.ny;
+  int rc;
+  if ((rc = dalloc(c, &c->mcr_acc, 4 * np * c->R)) || (rc = dalloc(c, &c->mcr_cnt, np * c->R))) return rc;
+  HIPCHK(c, hipMemsetAsync(c->mcr_acc, 0, sizeof(double) * 4 * np * c->R, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->mcr_cnt, 0, sizeof(int) * np * c->R, c->stream));
+  return GFSPH_OK;
+}
+
+int gfsph_batch_pixel_accumulate(gfsph_ctx *c) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  int rc = ensure_mcr(c);
+  if (rc) return rc;
+  const long np = (long)c->dem.nx * c->dem.ny;
+  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), np, c->mcr_acc, c->mcr_cnt);
+  HIPCHK(c, hipGetLastError());
+  return GFSPH_OK;
+}
+
+int gfsph_batch_pixel_read(gfsph_ctx *c, int32_t r, double *acc, int32_t *cnt) {
+  if (!c || r < 0 || r >= c->R) return GFSPH_ERR_ARG;
+  int rc = ensure_mcr(c);
+  if (rc) return rc;
+  const long np = (long)c->dem.nx * c->dem.ny;
+  if (acc)
+    HIPCHK(c, hipMemcpyAsync(acc, c->mcr_acc + 4 * np * r, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream));
+  if (cnt) HIPCHK(c, hipMemcpyAsync(cnt, c->mcr_cnt + np * r, sizeof(int) * np, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFSPH_OK;
+}
+
+int gfsph_batch_hazard(gfsph_ctx *c, double hthr, double *stats, int32_t *nprob, double *maxs, double *pexc) {
+  if (!c || !stats || !nprob || !maxs || !pexc) return GFSPH_ERR_ARG;
+  int rc = ensure_mcr(c);
+  if (rc) return rc;
+  const long np = (long)c->dem.nx * c->dem.ny;
+  // one temporary block: stats [np][8], maxs [np][2], pexc [np], nprob [np]
+  void *tmp = nullptr;
+  const size_t bytes = sizeof(double) * 11 * np + sizeof(int) * np;
+  HIPCHK(c, hipMalloc(&tmp, bytes));
+  double *d_stats = (double *)tmp, *d_maxs = d_stats + 8 * np, *d_pexc = d_maxs + 2 * np;
+  int *d_nprob = (int *)(d_pexc + np);
+  k_mc_hazard<<<nblk(np), GF_BLOCK, 0, c->stream>>>(np, c->R, c->mcr_acc, c->mcr_cnt, hthr, d_stats, d_nprob, d_maxs,
+                                                     d_pexc);
+  hipError_t e = hipGetLastError();
+  if (!e) e = hipMemcpyAsync(stats, d_stats, sizeof(double) * 8 * np, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipMemcpyAsync(nprob, d_nprob, sizeof(int) * np, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipMemcpyAsync(maxs, d_maxs, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipMemcpyAsync(pexc, d_pexc, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e) return fail(c, GFSPH_ERR_HIP, std::string("batch hazard: ") + hipGetErrorString(e));
+  return GFSPH_OK;
+}
+
 int gfsph_counts(gfsph_ctx *c, int32_t *n, int32_t *nactive) {
   if (!c) return GFSPH_ERR_ARG;
   if (n) *n = c->n;
   if (nactive) {
-    BBoxAcc a;
-    // the partials of the last step's force kernel describe the current state
-    k_grid<<<1, GF_BLOCK, 0, c->stream>>>(c->gridN, c->part, c->acc, c->grid, c->P.scale_k, c->cap_cells);
-    HIPCHK(c, hipMemcpyAsync(&a, c->acc, sizeof(a), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+    k_count_active<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.flag, c->bad);
+    int na = 0;
+    HIPCHK(c, hipMemcpyAsync(&na, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *nactive = (int32_t)a.nactive;
+    *nactive = (int32_t)na;
   }
   return GFSPH_OK;
 }

@@ -16,12 +16,13 @@ import numpy as np
 
 from ._build import LIB
 
-__all__ = ["GfsphError", "SPHModel", "neighbour_pairs", "load_library", "EXPORTED_SYMBOLS"]
+__all__ = ["GfsphError", "SPHModel", "BatchModel", "neighbour_pairs", "load_library", "EXPORTED_SYMBOLS"]
 
 EXPORTED_SYMBOLS = (
     "gfsph_create", "gfsph_step", "gfsph_sync", "gfsph_get_state", "gfsph_neighbour_pairs",
     "gfsph_raster_idw", "gfsph_mc_pixel_accumulate", "gfsph_mc_pixel_read", "gfsph_profile_step",
-    "gfsph_counts", "gfsph_last_error", "gfsph_destroy",
+    "gfsph_counts", "gfsph_last_error", "gfsph_destroy", "gfsph_batch_create", "gfsph_batch_pixel_accumulate",
+    "gfsph_batch_pixel_read", "gfsph_batch_hazard",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -77,6 +78,11 @@ def load_library(path: str | None = None):
     L.gfsph_last_error.restype = C.c_char_p
     L.gfsph_destroy.argtypes = [vp]
     L.gfsph_destroy.restype = None
+    L.gfsph_batch_create.argtypes = [C.POINTER(CParams), C.POINTER(CDem), C.c_int32, ip, dp, dp, dp, dp, dp, dp, dp,
+                                     C.POINTER(vp)]
+    L.gfsph_batch_pixel_accumulate.argtypes = [vp]
+    L.gfsph_batch_pixel_read.argtypes = [vp, C.c_int32, dp, ip]
+    L.gfsph_batch_hazard.argtypes = [vp, C.c_double, dp, ip, dp, dp]
     if path is None:
         _lib = L
     return L
@@ -104,6 +110,21 @@ def make_cparams(prm, init_input: bool = True, check_finite: bool = False) -> CP
     return p
 
 
+class _DemArgs:
+    """C view of a DEM; keeps the numpy buffers alive."""
+
+    def __init__(self, dem):
+        self._Z = np.ascontiguousarray(dem.Z, dtype=np.float64)
+        self._im = None if dem.in_model is None else np.ascontiguousarray(dem.in_model, dtype=np.int32)
+        d = CDem()
+        d.nx, d.ny = dem.nx, dem.ny
+        d.xmin, d.ymin, d.dx, d.dy, d.xmax, d.ymax = dem.xmin, dem.ymin, dem.dx, dem.dy, dem.xmax, dem.ymax
+        d.Z = _dp(self._Z)
+        d.in_model = _ip(self._im) if self._im is not None else C.POINTER(C.c_int32)()
+        d.topol = C.POINTER(C.c_double)()
+        self.c = d
+
+
 class SPHModel:
     """One propagation run on the GPU (Init_sph + time_integration_sph)."""
 
@@ -113,14 +134,8 @@ class SPHModel:
         P = case.particles
         self.n = P.n
         dem = case.dem
-        self._Z = np.ascontiguousarray(dem.Z, dtype=np.float64)
-        self._im = None if dem.in_model is None else np.ascontiguousarray(dem.in_model, dtype=np.int32)
-        d = CDem()
-        d.nx, d.ny = dem.nx, dem.ny
-        d.xmin, d.ymin, d.dx, d.dy, d.xmax, d.ymax = dem.xmin, dem.ymin, dem.dx, dem.dy, dem.xmax, dem.ymax
-        d.Z = _dp(self._Z)
-        d.in_model = _ip(self._im) if self._im is not None else C.POINTER(C.c_int32)()
-        d.topol = C.POINTER(C.c_double)()
+        self._dem = _DemArgs(dem)
+        d = self._dem.c
         p = make_cparams(case.params, init_input, check_finite)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (P.x, P.v, P.h, P.mass, P.hsml, P.u)]
         it = np.ascontiguousarray(P.itype, dtype=np.int32)
@@ -200,6 +215,60 @@ class SPHModel:
 
     def __exit__(self, *a):
         self.close()
+
+
+class BatchModel(SPHModel):
+    """Many independent realisations on one DEM advanced together (the
+    reference's Monte Carlo loop run in one launch sequence).  ``cases``
+    share the DEM and the solver switches; each brings its particles and its
+    const(:) vector.  ``state()`` returns all particles stacked in case
+    order; ``split(state)`` cuts it per realisation."""
+
+    def __init__(self, cases, init_input: bool = True, check_finite: bool = False):
+        self.lib = load_library()
+        self.cases = list(cases)
+        self.case = self.cases[0]
+        self.R = len(self.cases)
+        self.counts = np.array([c.particles.n for c in self.cases], dtype=np.int32)
+        self.n = int(self.counts.sum())
+        self._dem = _DemArgs(self.case.dem)
+        p = make_cparams(self.case.params, init_input, check_finite)
+        cst = np.ascontiguousarray(np.stack([c.params.cst() for c in self.cases]), dtype=np.float64)
+        arrs = [np.ascontiguousarray(np.concatenate([getattr(c.particles, k) for c in self.cases]), dtype=np.float64)
+                for k in ("x", "v", "h", "mass", "hsml", "u")]
+        ctx = C.c_void_p()
+        rc = self.lib.gfsph_batch_create(C.byref(p), C.byref(self._dem.c), self.R, _ip(self.counts), _dp(cst),
+                                         *(_dp(a) for a in arrs), C.byref(ctx))
+        self.ctx = ctx
+        if rc != 0:
+            msg = self.lib.gfsph_last_error(ctx).decode() if ctx.value else ""
+            self.close()
+            raise GfsphError(rc, msg)
+        self.nsteps_done = 0
+
+    def split(self, st: dict) -> list:
+        edges = np.concatenate([[0], np.cumsum(self.counts)])
+        return [{k: v[edges[r]:edges[r + 1]] for k, v in st.items()} for r in range(self.R)]
+
+    def pixel_accumulate(self) -> None:
+        self._check(self.lib.gfsph_batch_pixel_accumulate(self.ctx))
+
+    def pixel_read(self, r: int):
+        npg = self.case.dem.nx * self.case.dem.ny
+        acc = np.zeros((npg, 4))
+        cnt = np.zeros(npg, np.int32)
+        self._check(self.lib.gfsph_batch_pixel_read(self.ctx, int(r), _dp(acc), _ip(cnt)))
+        return acc, cnt
+
+    def hazard(self, hthr: float = 0.5) -> dict:
+        npg = self.case.dem.nx * self.case.dem.ny
+        stats = np.zeros((npg, 4, 2))
+        nprob = np.zeros(npg, np.int32)
+        maxs = np.zeros((npg, 2))
+        pexc = np.zeros(npg)
+        self._check(self.lib.gfsph_batch_hazard(self.ctx, float(hthr), _dp(stats), _ip(nprob), _dp(maxs), _dp(pexc)))
+        return {"mean": stats[:, :, 0], "std": stats[:, :, 1], "nprob": nprob, "max_h": maxs[:, 0],
+                "max_v": maxs[:, 1], "p_exceed": pexc}
 
 
 def neighbour_pairs(x, hsml, flag=None, skf: int = 1) -> np.ndarray:

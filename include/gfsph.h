@@ -114,6 +114,35 @@ int gfsph_raster_idw(gfsph_ctx *ctx, double *out, double *wsum);
 int gfsph_mc_pixel_accumulate(gfsph_ctx *ctx);
 int gfsph_mc_pixel_read(gfsph_ctx *ctx, double *acc /*[npoig][4]*/, int32_t *cnt, int32_t reset);
 
+/* Batched Monte Carlo: nreal independent realisations (the ic_MCarlo loop of
+ * Driver_Input / Driver_Flow, "8 SPH_FEM_Driver_2019.f90":1282-1439 and
+ * 1554-1710, run together instead of one after another) on one DEM, stepped
+ * by gfsph_step in one launch sequence.  Realisation r has counts[r]
+ * particles, stored one realisation after another in x, v, h, mass, hsml, u
+ * (n = sum of counts), and its own const(:) in cst[r*111 .. r*111+110]
+ * (cst == NULL: p->cst for all; Update_Const_SW, 6:20035).  The switches of
+ * p are shared, and so must be const(1) and const(14).  Realisations never
+ * interact; each advances exactly as gfsph_create + gfsph_step would advance
+ * it alone.  Bingham (nfric = 4) is not batched (GFSPH_ERR_UNSUPPORTED).
+ * gfsph_get_state returns all particles in the same stacked order. */
+int gfsph_batch_create(const gfsph_params *p, const gfsph_dem *dem, int32_t nreal, const int32_t *counts,
+                       const double *cst, const double *x, const double *v, const double *h,
+                       const double *mass, const double *hsml, const double *u, gfsph_ctx **out);
+/* Get_MC_pixel_Vars_SW ("6 SPH_SW_2019.f90":20717-20833) for every
+ * realisation at the current (plot) time, into per-realisation node
+ * accumulators [nreal][npoig] (sum h, max h, sum |v|, max |v|, visits),
+ * allocated on first use.  gfsph_batch_pixel_read copies realisation r's
+ * accumulators out. */
+int gfsph_batch_pixel_accumulate(gfsph_ctx *ctx);
+int gfsph_batch_pixel_read(gfsph_ctx *ctx, int32_t r, double *acc /*[npoig][4]*/, int32_t *cnt /*[npoig]*/);
+/* Per-node statistics over the realisations (Norm_MC_pixel_Vars_SW,
+ * "6 SPH_SW_2019.f90":20838-20908): stats[npoig][4][2] = mean and standard
+ * deviation, over the nprob[node] realisations that reached the node, of
+ * (average h, max h, average |v|, max |v|); maxs[npoig][2] = max h and max |v|
+ * over all realisations; pexc[npoig] = fraction of the realisations whose
+ * max h exceeds hthr (the exceedance-probability raster, a build extension
+ * with no reference counterpart). */
+int gfsph_batch_hazard(gfsph_ctx *ctx, double hthr, double *stats, int32_t *nprob, double *maxs, double *pexc);
 /* Average device time per launch of each step kernel over nrep steps, in
  * milliseconds, measured with HIP events on the context's stream.
  * names/ms are arrays of *nkernels entries (at most cap). */

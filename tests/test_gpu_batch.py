@@ -1,0 +1,96 @@
+"""Batched Monte Carlo realisations against the same realisations run one
+by one through the plain drop-in (the batch must not change any
+realisation's result), and the hazard statistics against their definition
+(Norm_MC_pixel_Vars_SW, "6 SPH_SW_2019.f90":20838-20908) evaluated from the
+per-realisation accumulators."""
+import numpy as np
+import pytest
+
+from gfsph import BatchModel, SPHModel
+from gfsph import cases as C
+from gfsph.dem import fractal_dem
+
+pytestmark = pytest.mark.gpu
+
+
+def realisations(R=5, seed=11, integrator=2):
+    dem = fractal_dem(129, 2.0, 31, relief=60.0)
+    rng = np.random.default_rng(seed)
+    out = []
+    sizes = [300, 900, 150, 1200, 500, 60, 700][:R]
+    for r in range(R):
+        mu = float(rng.uniform(0.08, 0.35))
+        xi = float(np.exp(rng.normal(np.log(400.0), 0.5)))
+        cx = 60.0 + rng.uniform(-20, 20)
+        cy = 128.0 + rng.uniform(-30, 30)
+        parts = C.elliptical_scar(cx, cy, sizes[r], 1.0, 2.0, seed + r, h_range=(1.0, 4.0))
+        prm = C.Params(integrator=integrator, dt=0.02, h_inf=1e-3, const=C.voellmy(mu, xi))
+        out.append(C.Case(f"r{r}", prm, dem, parts, 0))
+    return out
+
+
+@pytest.mark.parametrize("integrator", [2, 4])
+def test_batch_equals_single_runs(integrator):
+    cases = realisations(integrator=integrator)
+    nsteps = 80
+    with BatchModel(cases) as b:
+        b.time_integration_sph(nsteps)
+        per = b.split(b.state())
+    for r, case in enumerate(cases):
+        with SPHModel(case) as m:
+            m.time_integration_sph(nsteps)
+            st = m.state()
+        for k in ("x", "v", "h", "u", "hsml"):
+            np.testing.assert_array_equal(per[r][k], st[k], err_msg=f"realisation {r} field {k}")
+        np.testing.assert_array_equal(per[r]["flag"], st["flag"])
+
+
+def test_batch_pixels_and_hazard():
+    cases = realisations(R=4, seed=5)
+    plots, every = 6, 15
+    with BatchModel(cases) as b:
+        for _ in range(plots):
+            b.time_integration_sph(every)
+            b.pixel_accumulate()
+        accs = [b.pixel_read(r) for r in range(b.R)]
+        hz = b.hazard(0.5)
+    for r, case in enumerate(cases):
+        with SPHModel(case) as m:
+            for _ in range(plots):
+                m.time_integration_sph(every)
+                m.mc_pixel_accumulate()
+            acc, cnt = m.mc_pixel_read()
+        np.testing.assert_array_equal(accs[r][1], cnt)
+        np.testing.assert_allclose(accs[r][0], acc, rtol=1e-12, atol=0)
+    # hazard statistics from their definition
+    A = np.stack([a for a, _ in accs])            # [R, np, 4]
+    N = np.stack([c for _, c in accs])            # [R, np]
+    hit = N > 0
+    nprob = hit.sum(0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        o = np.stack([A[:, :, 0] / N, A[:, :, 1], A[:, :, 2] / N, A[:, :, 3]], axis=2)
+    o = np.where(hit[:, :, None], o, 0.0)
+    nd = np.maximum(nprob, 1)[:, None]
+    mean = o.sum(0) / nd
+    var = (o * o).sum(0) / nd - mean * mean
+    std = np.sqrt(np.where(var < 1e-6, 0.0, var))
+    np.testing.assert_array_equal(hz["nprob"], nprob)
+    np.testing.assert_allclose(hz["mean"], mean, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(hz["std"], std, rtol=1e-9, atol=1e-9)
+    np.testing.assert_array_equal(hz["max_h"], A[:, :, 1].max(0))
+    np.testing.assert_array_equal(hz["max_v"], A[:, :, 3].max(0))
+    np.testing.assert_allclose(hz["p_exceed"], (A[:, :, 1] > 0.5).sum(0) / len(cases), rtol=0, atol=1e-15)
+    assert hz["p_exceed"].max() > 0
+
+
+def test_batch_rejects_bingham_and_mismatched_gravity():
+    cases = realisations(R=2)
+    bing = C.Case("b", C.Params(integrator=2, dt=0.02, h_inf=1e-3, const=C.bingham(1000.0, 50.0)), cases[0].dem,
+                  cases[0].particles, 0)
+    with pytest.raises(Exception, match="unsupported|Bingham"):
+        BatchModel([cases[0], bing])
+    g2 = dict(C.voellmy(0.2, 400.0))
+    g2[1] = 9.7
+    other = C.Case("g", C.Params(integrator=2, dt=0.02, h_inf=1e-3, const=g2), cases[0].dem, cases[1].particles, 0)
+    with pytest.raises(Exception, match="const\\(1\\)"):
+        BatchModel([cases[0], other])
