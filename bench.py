@@ -73,6 +73,11 @@ def main() -> int:
     ap.add_argument("--integrator", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-steps", type=int, default=40)
+    ap.add_argument("--no-mc", action="store_true", help="skip the batched Monte Carlo measurement")
+    ap.add_argument("--mc-real", type=int, default=256)
+    ap.add_argument("--mc-steps", type=int, default=3000)
+    ap.add_argument("--mc-seq-sample", type=int, default=8,
+                    help="realisations timed one by one to estimate the sequential cost")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -126,6 +131,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(case, args.cpu_sample_steps)
+    mc = None if args.no_mc else mc_hazard(args.mc_real, args.mc_steps, args.mc_seq_sample)
 
     out = {
         "metric": "particle-steps/s at 200k",
@@ -145,6 +151,7 @@ def main() -> int:
                    "integrator": "RK2" if args.integrator == 2 else "RK4", "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mc": mc,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -163,6 +170,56 @@ def calibrate_copy() -> int:
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
     return 0
+
+
+def mc_hazard(R, nsteps, nseq, plot_every=20):
+    """Batched Monte Carlo hazard workload (fractal DEM 1024 x 1024 at 5 m,
+    R realisations of 5k-80k particles, Voellmy mu ~ U(0.08, 0.35), xi ~
+    LogNormal(ln 400, 0.5), RK2 dt 0.05 s, pixel statistics every
+    ``plot_every`` steps, then the hazard rasters): the whole batch is timed
+    from context creation to the rasters on the host; the sequential cost is
+    the mean of ``nseq`` randomly chosen realisations run one by one through
+    the plain drop-in (same steps and pixel accumulation) times R."""
+    from gfsph import BatchModel, SPHModel, cases
+
+    cs = cases.cfg_mc_hazard(R=R, nsteps=nsteps)
+    n = sum(c.particles.n for c in cs)
+    print(f"# MC: {R} realisations, {n} particles, {nsteps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = BatchModel(cs)
+    b.time_integration_sph(1)
+    blocks = b.autotune(reps=2)
+    done = 1
+    while done < nsteps:
+        k = min(plot_every - done % plot_every, nsteps - done)
+        b.time_integration_sph(k)
+        done += k
+        if done % plot_every == 0:
+            b.pixel_accumulate()
+    hz = b.hazard(0.5)
+    t_batch = time.perf_counter() - t0
+    b.close()
+    print(f"# MC batch {t_batch:.2f}s (blocks {blocks})", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(1)
+    ts = []
+    for r in rng.choice(R, size=min(nseq, R), replace=False):
+        t1 = time.perf_counter()
+        m = SPHModel(cs[r])
+        for _ in range(nsteps // plot_every):
+            m.time_integration_sph(plot_every)
+            m.mc_pixel_accumulate()
+        m.time_integration_sph(nsteps % plot_every)
+        m.mc_pixel_read()
+        m.close()
+        ts.append(time.perf_counter() - t1)
+    t_seq = float(np.mean(ts)) * R
+    print(f"# MC sequential estimate {t_seq:.2f}s from {len(ts)} runs", file=sys.stderr, flush=True)
+    return {"metric": "MC realisations/hr", "value": round(R * 3600.0 / t_batch, 1), "unit": "realisations/hr",
+            "realisations": R, "steps": nsteps, "particles": n, "batch_s": round(t_batch, 3),
+            "particle_steps_per_s": round(n * nsteps / t_batch, 1),
+            "sequential_s_estimate": round(t_seq, 3), "sequential_sample": len(ts),
+            "speedup_vs_sequential": round(t_seq / t_batch, 2), "autotuned_blocks": list(blocks),
+            "max_h": round(float(hz["max_h"].max()), 3), "exceed_nodes": int((hz["p_exceed"] > 0).sum())}
 
 
 def cpu_baseline(case, nsteps):

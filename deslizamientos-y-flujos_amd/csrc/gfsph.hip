@@ -1312,6 +1312,7 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 // (row, then slot within the row), so results are deterministic.
 // ---------------------------------------------------------------------------
 #define NB_BATCH 4
+#define GATHER_MAX_BLOCK 512
 
 struct RowSpan {
   int b[3], cnt[3];  // first slot and candidate count of each stencil row (0: row outside the grid)
@@ -1473,7 +1474,7 @@ __device__ __forceinline__ bool load_mask(const unsigned long long *mask, int i,
 
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
 template <int SKF>
-__global__ __launch_bounds__(GF_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S, double *Pout,
+__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S, double *Pout,
                                                          unsigned long long *mask) {
   const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
@@ -1548,7 +1549,7 @@ __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, do
 
 // a5 pressure force (IntForces_SW pair sum, 6:6892-6941) -> ain[i]
 template <int SKF, int PA>
-__global__ __launch_bounds__(GF_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, const double *S,
+__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, const double *S,
                                                         const double *Pp, double2 *ain) {
   const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
@@ -1602,7 +1603,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_pforce_m(int n, DevParams P, Cells
 
 // a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
 template <int SKF>
-__global__ __launch_bounds__(GF_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, NewSet o,
+__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, NewSet o,
                                                       double2 *vout) {
   const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
@@ -1928,7 +1929,11 @@ struct gfsph_ctx {
   GridInfo *grid = nullptr;
   BBoxAcc *acc = nullptr, *part = nullptr;
   int gridN = 0;  // blocks of the per-particle kernels (multiple of 8)
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one step from set[k]
+  hipGraphExec_t graph[2] = {nullptr, nullptr};   // one step from set[k]
+  hipGraphExec_t graphG[2] = {nullptr, nullptr};  // GRAPH_STEPS steps from set[k]
+  // work-group size of each gather kernel (density, force, XSPH), chosen by
+  // gfsph_autotune; all sizes give identical results
+  int gbs[3] = {GF_BLOCK, GF_BLOCK, GF_BLOCK};
   int *bad = nullptr;
   double2 *zxy = nullptr;
   double *top = nullptr;
@@ -2180,14 +2185,18 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
     }                                          \
   } while (0)
 
-#define PFORCE_CASE(SKFV)                                                                                  \
-  switch (c->P.pa_sph) {                                                                                   \
-    case 1: k_pforce_m<SKFV, 1><<<c->gridN, GF_BLOCK, 0, c->stream>>>(PFORCE_ARGS); break;                 \
-    case 2: k_pforce_m<SKFV, 2><<<c->gridN, GF_BLOCK, 0, c->stream>>>(PFORCE_ARGS); break;                 \
-    default: k_pforce_m<SKFV, 4><<<c->gridN, GF_BLOCK, 0, c->stream>>>(PFORCE_ARGS); break;                \
+#define PFORCE_CASE(SKFV)                                                                    \
+  switch (c->P.pa_sph) {                                                                     \
+    case 1: k_pforce_m<SKFV, 1><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS); break;  \
+    case 2: k_pforce_m<SKFV, 2><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS); break;  \
+    default: k_pforce_m<SKFV, 4><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS); break; \
   }
 
+// blocks of bsz threads covering n particles, rounded up to a multiple of 8 (xcd_block)
+static inline int xcd_grid(long n, int bsz) { return (int)((((n + bsz - 1) / bsz) + 7) / 8 * 8); }
+
 static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
+  const int bsz = c->gbs[1];
 #define PFORCE_ARGS c->n, c->P, cells_of(c), c->mask, s, c->S, c->Pp, c->ain
   switch (c->P.skf) {
     case 1: PFORCE_CASE(1) break;
@@ -2255,13 +2264,17 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 }
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
-  SKF_DISPATCH(k_density_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
+  const int bsz = c->gbs[0];
+  SKF_DISPATCH(k_density_m, xcd_grid(c->n, bsz), bsz, 0,
+               c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
 }
 
 // XSPH (or the plain copy of the post-RK velocity) into the sorted set
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
+  const int bsz = c->gbs[2];
   if (c->P.av_vel)
-    SKF_DISPATCH(k_xsph_m, c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel));
+    SKF_DISPATCH(k_xsph_m, xcd_grid(c->n, bsz), bsz, 0,
+                 c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
@@ -2278,31 +2291,59 @@ static int enqueue_step(gfsph_ctx *c, int src, int dst) {
   return GFSPH_OK;
 }
 
-// One step.  The kernel sequence of a step from set[src] is captured once in
-// a HIP graph (the launch gaps of ~15 separate launches dominate at 10^5
-// particles) and replayed; GFSPH_NO_GRAPH=1 launches the kernels directly.
-static int one_step(gfsph_ctx *c) {
-  int src = c->cur, dst = 1 - c->cur;
-  static const bool no_graph = getenv("GFSPH_NO_GRAPH") && getenv("GFSPH_NO_GRAPH")[0] == '1';
-  if (no_graph) {
-    int rc = enqueue_step(c, src, dst);
-    if (rc) return rc;
-    HIPCHK(c, hipGetLastError());
-  } else {
-    if (!c->graph[src]) {
-      hipGraph_t g;
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_step(c, src, dst);
-      hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (rc) return rc;
-      if (e != hipSuccess) return fail(c, GFSPH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&c->graph[src], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(c, GFSPH_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    }
-    HIPCHK(c, hipGraphLaunch(c->graph[src], c->stream));
+// Steps.  The kernel sequence of GRAPH_STEPS steps (an even number, so the
+// sequence ends on the parity it started from) and of a single step from
+// either parity are captured once in HIP graphs and replayed: launch gaps of
+// ~15 separate launches per step dominate small runs.  GFSPH_NO_GRAPH=1
+// launches the kernels directly.
+#define GRAPH_STEPS 8
+
+static int capture(gfsph_ctx *c, int nsteps, hipGraphExec_t *out) {
+  hipGraph_t g;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = GFSPH_OK, cur = c->cur;
+  for (int k = 0; k < nsteps && !rc; k++) {
+    rc = enqueue_step(c, cur, 1 - cur);
+    cur = 1 - cur;
   }
-  c->cur = dst;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(c, GFSPH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(c, GFSPH_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  return GFSPH_OK;
+}
+
+static void drop_graphs(gfsph_ctx *c) {
+  for (int k = 0; k < 2; k++) {
+    if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+    if (c->graphG[k]) (void)hipGraphExecDestroy(c->graphG[k]);
+    c->graph[k] = c->graphG[k] = nullptr;
+  }
+}
+
+static int run_steps(gfsph_ctx *c, int nsteps) {
+  static const bool no_graph = getenv("GFSPH_NO_GRAPH") && getenv("GFSPH_NO_GRAPH")[0] == '1';
+  int rc;
+  if (no_graph) {
+    for (int k = 0; k < nsteps; k++) {
+      if ((rc = enqueue_step(c, c->cur, 1 - c->cur))) return rc;
+      c->cur = 1 - c->cur;
+    }
+    HIPCHK(c, hipGetLastError());
+    return GFSPH_OK;
+  }
+  while (nsteps >= GRAPH_STEPS) {
+    if (!c->graphG[c->cur] && (rc = capture(c, GRAPH_STEPS, &c->graphG[c->cur]))) return rc;
+    HIPCHK(c, hipGraphLaunch(c->graphG[c->cur], c->stream));
+    nsteps -= GRAPH_STEPS;
+  }
+  for (; nsteps > 0; nsteps--) {
+    if (!c->graph[c->cur] && (rc = capture(c, 1, &c->graph[c->cur]))) return rc;
+    HIPCHK(c, hipGraphLaunch(c->graph[c->cur], c->stream));
+    c->cur = 1 - c->cur;
+  }
   return GFSPH_OK;
 }
 
@@ -2480,10 +2521,8 @@ int gfsph_batch_create(const gfsph_params *p, const gfsph_dem *dem, int32_t nrea
 int gfsph_step(gfsph_ctx *c, int32_t nsteps) {
   if (!c || c->n <= 0) return GFSPH_ERR_ARG;
   if (nsteps < 0) return fail(c, GFSPH_ERR_ARG, "nsteps < 0");
-  for (int k = 0; k < nsteps; k++) {
-    int rc = one_step(c);
-    if (rc) return rc;
-  }
+  int rc = run_steps(c, nsteps);
+  if (rc) return rc;
   if (c->check_finite) {
     HIPCHK(c, hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
     k_nonfinite<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, cur_view(c), c->bad);
@@ -2750,6 +2789,52 @@ int gfsph_batch_hazard(gfsph_ctx *c, double hthr, double *stats, int32_t *nprob,
   return GFSPH_OK;
 }
 
+// Work-group size of each gather kernel, timed on the context's current
+// sorted set: the three gathers are idempotent there (they recompute the same
+// depths, masks, forces and XSPH velocities), so tuning does not advance or
+// change the state.  Needs at least one completed step.
+int gfsph_autotune(gfsph_ctx *c, int32_t reps, int32_t *chosen) {
+  if (!c || c->n <= 0 || reps <= 0) return GFSPH_ERR_ARG;
+  static const int sizes[] = {64, 128, 256, 512};
+  SortedSet &s = c->set[c->cur];
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int best[3];
+  for (int k = 0; k < 3; k++) {
+    float tbest = 1e30f;
+    best[k] = c->gbs[k];
+    for (int bsz : sizes) {
+      if (k == 2 && !c->P.av_vel) break;
+      c->gbs[k] = bsz;
+      for (int w = 0; w < 2; w++) {  // warm-up, then timed
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; r++) {
+          if (k == 0) launch_density(c, s);
+          else if (k == 1) launch_pforce(c, s);
+          else launch_velocity(c, s);
+        }
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+      }
+      HIPCHK(c, hipEventSynchronize(e1));
+      float t;
+      HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+      if (t < tbest) {
+        tbest = t;
+        best[k] = bsz;
+      }
+    }
+    c->gbs[k] = best[k];
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(c, hipGetLastError());
+  drop_graphs(c);  // recaptured with the chosen sizes
+  if (chosen)
+    for (int k = 0; k < 3; k++) chosen[k] = best[k];
+  return GFSPH_OK;
+}
+
 int gfsph_counts(gfsph_ctx *c, int32_t *n, int32_t *nactive) {
   if (!c) return GFSPH_ERR_ARG;
   if (n) *n = c->n;
@@ -2769,8 +2854,7 @@ const char *gfsph_last_error(gfsph_ctx *c) { return c ? c->err.c_str() : "null c
 void gfsph_destroy(gfsph_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < 2; k++)
-    if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+  drop_graphs(c);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -22,7 +22,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_create", "gfsph_step", "gfsph_sync", "gfsph_get_state", "gfsph_neighbour_pairs",
     "gfsph_raster_idw", "gfsph_mc_pixel_accumulate", "gfsph_mc_pixel_read", "gfsph_profile_step",
     "gfsph_counts", "gfsph_last_error", "gfsph_destroy", "gfsph_batch_create", "gfsph_batch_pixel_accumulate",
-    "gfsph_batch_pixel_read", "gfsph_batch_hazard",
+    "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -83,6 +83,7 @@ def load_library(path: str | None = None):
     L.gfsph_batch_pixel_accumulate.argtypes = [vp]
     L.gfsph_batch_pixel_read.argtypes = [vp, C.c_int32, dp, ip]
     L.gfsph_batch_hazard.argtypes = [vp, C.c_double, dp, ip, dp, dp]
+    L.gfsph_autotune.argtypes = [vp, C.c_int32, ip]
     if path is None:
         _lib = L
     return L
@@ -194,6 +195,13 @@ class SPHModel:
         self.nsteps_done += int(nrep)
         return {names[k].decode(): float(ms[k]) for k in range(nk.value)}
 
+    def autotune(self, reps: int = 3) -> tuple:
+        """Work-group sizes of the density, force and XSPH gathers chosen by
+        timing on the current state (needs one completed step)."""
+        ch = np.zeros(3, np.int32)
+        self._check(self.lib.gfsph_autotune(self.ctx, int(reps), _ip(ch)))
+        return tuple(int(v) for v in ch)
+
     def counts(self):
         n, na = C.c_int32(), C.c_int32()
         self._check(self.lib.gfsph_counts(self.ctx, C.byref(n), C.byref(na)))
@@ -229,15 +237,15 @@ class BatchModel(SPHModel):
         self.cases = list(cases)
         self.case = self.cases[0]
         self.R = len(self.cases)
-        self.counts = np.array([c.particles.n for c in self.cases], dtype=np.int32)
-        self.n = int(self.counts.sum())
+        self.sizes = np.array([c.particles.n for c in self.cases], dtype=np.int32)
+        self.n = int(self.sizes.sum())
         self._dem = _DemArgs(self.case.dem)
         p = make_cparams(self.case.params, init_input, check_finite)
         cst = np.ascontiguousarray(np.stack([c.params.cst() for c in self.cases]), dtype=np.float64)
         arrs = [np.ascontiguousarray(np.concatenate([getattr(c.particles, k) for c in self.cases]), dtype=np.float64)
                 for k in ("x", "v", "h", "mass", "hsml", "u")]
         ctx = C.c_void_p()
-        rc = self.lib.gfsph_batch_create(C.byref(p), C.byref(self._dem.c), self.R, _ip(self.counts), _dp(cst),
+        rc = self.lib.gfsph_batch_create(C.byref(p), C.byref(self._dem.c), self.R, _ip(self.sizes), _dp(cst),
                                          *(_dp(a) for a in arrs), C.byref(ctx))
         self.ctx = ctx
         if rc != 0:
@@ -247,7 +255,7 @@ class BatchModel(SPHModel):
         self.nsteps_done = 0
 
     def split(self, st: dict) -> list:
-        edges = np.concatenate([[0], np.cumsum(self.counts)])
+        edges = np.concatenate([[0], np.cumsum(self.sizes)])
         return [{k: v[edges[r]:edges[r + 1]] for k, v in st.items()} for r in range(self.R)]
 
     def pixel_accumulate(self) -> None:

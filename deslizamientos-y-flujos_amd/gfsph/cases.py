@@ -25,7 +25,7 @@ import numpy as np
 from .dem import DEM, fractal_dem, plane_runout_dem
 
 __all__ = ["Params", "Particles", "Case", "f32", "voellmy", "frictional", "bingham",
-           "lattice_block", "elliptical_scar", "cfg_plane_voellmy", "cfg_fractal_frictional"]
+           "lattice_block", "elliptical_scar", "cfg_plane_voellmy", "cfg_fractal_frictional", "cfg_mc_hazard"]
 
 
 def f32(v: float) -> float:
@@ -185,3 +185,34 @@ def cfg_fractal_frictional(n: int = 200_000, dem_n: int = 2048, d: float = 2.0, 
     prm = Params(integrator=integrator, dt=0.02, h_inf=1.0e-3, icpwp=3, pwp0rel=0.6,
                  const=frictional(28.0, cv=0.01))
     return Case("fractal_frictional", prm, dem, parts, nsteps)
+
+
+def cfg_mc_hazard(R: int = 256, dem_n: int = 1024, d: float = 5.0, n_range=(5_000, 80_000), nsteps: int = 3000,
+                  seed: int = 3003, spacing: float = 2.0, jitter: float = 50.0, integrator: int = 2,
+                  dt: float = 0.05, dem: DEM | None = None) -> list:
+    """Monte Carlo hazard batch: fractal DEM 1024 x 1024 at 5 m (H 0.8, 400 m
+    relief, carved valley); R realisations with Voellmy mu ~ U(0.08, 0.35),
+    xi ~ LogNormal(ln 400, 0.5) m/s2, a release of N ~ logU(5k, 80k)
+    particles (elliptical scar, spacing 2 m, hsml 3 m, thickness ~ U(2, 8) m)
+    on the valley axis, its centre jittered by U(-50, 50) m in x and y; RK2,
+    dt = 0.05 s.  All draws come from one Philox stream seeded with ``seed``
+    (the reference's sampler, "8 SPH_FEM_Driver_2019.f90":1282-1369, is not
+    reproducible and has no log-uniform law)."""
+    dem = fractal_dem(dem_n, d, seed) if dem is None else dem
+    L = (dem.nx - 1) * dem.dx
+    cx0 = 0.14 * L
+    ix = int(round(cx0 / dem.dx))
+    col = dem.Z[:, ix]
+    iy = int(np.argmin(np.abs(np.arange(dem.ny) * dem.dy - 0.5 * L) + 1e-9 * col))
+    rng = np.random.Generator(np.random.Philox(seed))
+    mus = rng.uniform(0.08, 0.35, size=R)
+    xis = np.exp(rng.normal(math.log(400.0), 0.5, size=R))
+    ns = np.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1]), size=R)).astype(int)
+    jx = rng.uniform(-jitter, jitter, size=R)
+    jy = rng.uniform(-jitter, jitter, size=R)
+    out = []
+    for r in range(R):
+        parts = elliptical_scar(cx0 + jx[r], iy * dem.dy + jy[r], int(ns[r]), spacing, 2.0, seed * 1000 + r)
+        prm = Params(integrator=integrator, dt=dt, h_inf=1.0e-3, const=voellmy(float(mus[r]), float(xis[r])))
+        out.append(Case(f"mc{r}", prm, dem, parts, nsteps))
+    return out

@@ -143,6 +143,13 @@ int gfsph_batch_pixel_read(gfsph_ctx *ctx, int32_t r, double *acc /*[npoig][4]*/
  * max h exceeds hthr (the exceedance-probability raster, a build extension
  * with no reference counterpart). */
 int gfsph_batch_hazard(gfsph_ctx *ctx, double hthr, double *stats, int32_t *nprob, double *maxs, double *pexc);
+/* Picks the work-group size (64..512) of each neighbour-gather kernel
+ * (density, pressure force, XSPH) by timing reps launches of each on the
+ * context's current state; no reference counterpart (the reference is
+ * serial).  Results do not depend on the choice, and the state is not
+ * advanced.  Needs one completed gfsph_step.  chosen[3] (may be NULL)
+ * receives the sizes. */
+int gfsph_autotune(gfsph_ctx *ctx, int32_t reps, int32_t *chosen);
 /* Average device time per launch of each step kernel over nrep steps, in
  * milliseconds, measured with HIP events on the context's stream.
  * names/ms are arrays of *nkernels entries (at most cap). */

@@ -94,3 +94,17 @@ def test_batch_rejects_bingham_and_mismatched_gravity():
     other = C.Case("g", C.Params(integrator=2, dt=0.02, h_inf=1e-3, const=g2), cases[0].dem, cases[1].particles, 0)
     with pytest.raises(Exception, match="const\\(1\\)"):
         BatchModel([cases[0], other])
+
+
+def test_autotune_keeps_results():
+    cases = realisations(R=3, seed=21)
+    with BatchModel(cases) as a, BatchModel(cases) as b:
+        a.time_integration_sph(9)
+        b.time_integration_sph(9)
+        chosen = b.autotune(reps=2)
+        assert all(c in (64, 128, 256, 512) for c in chosen)
+        a.time_integration_sph(17)
+        b.time_integration_sph(17)
+        sa, sb = a.state(), b.state()
+    for k in ("x", "v", "h", "u"):
+        np.testing.assert_array_equal(sa[k], sb[k])
