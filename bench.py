@@ -78,6 +78,8 @@ def main() -> int:
     ap.add_argument("--mc-steps", type=int, default=3000)
     ap.add_argument("--mc-seq-sample", type=int, default=8,
                     help="realisations timed one by one to estimate the sequential cost")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--adj-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -132,6 +134,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(case, args.cpu_sample_steps)
     mc = None if args.no_mc else mc_hazard(args.mc_real, args.mc_steps, args.mc_seq_sample)
+    adj = None if args.no_adjoint else adjoint_cost(args.adj_steps)
 
     out = {
         "metric": "particle-steps/s at 200k",
@@ -152,6 +155,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mc": mc,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -220,6 +224,76 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
             "sequential_s_estimate": round(t_seq, 3), "sequential_sample": len(ts),
             "speedup_vs_sequential": round(t_seq / t_batch, 2), "autotuned_blocks": list(blocks),
             "max_h": round(float(hz["max_h"].max()), 3), "exceed_nodes": int((hz["p_exceed"] > 0).sum())}
+
+
+def adjoint_cost(nsteps, ckpt=50, fd_steps=50):
+    """Adjoint back-analysis workload (fractal DEM 512 x 512 at 5 m, 50,000
+    particles, Voellmy, RK2 dt 0.05 s): the time of the forward pass with
+    checkpoints + deposit misfit + reverse sweep over the time of one plain
+    forward pass of the same steps; then dL/dmu and dL/dxi against central
+    differences (eps = 1e-6, relative, applied to the REAL(4) constants) over
+    ``fd_steps`` steps, a horizon on which the difference quotient has
+    converged (over long horizons the misfit of the windowed IDW raster
+    jumps when a particle crosses a DEM cell, so no difference quotient
+    converges there)."""
+    from gfsph import AdjointModel, SPHModel, cases
+
+    def target(n):
+        with SPHModel(cases.cfg_adjoint(nsteps=n, mu=0.17, xi=650.0)) as m:
+            m.time_integration_sph(n)
+            return m.raster_idw()[0][:, 0].copy()
+
+    case = cases.cfg_adjoint(nsteps=nsteps)
+    T = target(nsteps)
+    with SPHModel(case) as w:
+        w.time_integration_sph(16)
+        w.sync()
+    m = SPHModel(case)
+    m.sync()
+    t0 = time.perf_counter()
+    m.time_integration_sph(nsteps)
+    m.sync()
+    t_fwd = time.perf_counter() - t0
+    m.close()
+    a = AdjointModel(case)
+    a.sync()
+    t0 = time.perf_counter()
+    a.forward(nsteps, ckpt)
+    L, lx, lh = a.misfit(T)
+    g = a.backward(lam_x=lx, lam_h=lh)
+    t_adj = time.perf_counter() - t0
+    a.close()
+    # difference check on the short horizon
+    Ts = target(fd_steps)
+    short = cases.cfg_adjoint(nsteps=fd_steps)
+    with AdjointModel(short) as a2:
+        a2.forward(fd_steps, ckpt)
+        _, lx2, lh2 = a2.misfit(Ts)
+        g2 = a2.backward(lam_x=lx2, lam_h=lh2)
+
+    def loss(c):
+        with SPHModel(c) as mm:
+            mm.time_integration_sph(fd_steps)
+            out = mm.raster_idw()[0][:, 0]
+        return float(((out - Ts) ** 2).sum())
+
+    fd = {}
+    for k, name in ((9, "mu"), (3, "xi")):
+        v = short.params.const[k]
+        hi, lo = cases.f32(v * (1 + 1e-6)), cases.f32(v * (1 - 1e-6))
+        vals = []
+        for val in (hi, lo):
+            cst = dict(short.params.const)
+            cst[k] = val
+            prm = cases.Params(**{**short.params.__dict__, "const": cst})
+            vals.append(loss(cases.Case(short.name, prm, short.dem, short.particles, fd_steps)))
+        q = (vals[0] - vals[1]) / (hi - lo)
+        fd[name] = {"adjoint": g2[name], "fd": q, "rel": abs(g2[name] - q) / abs(q)}
+    print(f"# adjoint: forward {t_fwd:.3f}s, forward+adjoint {t_adj:.3f}s", file=sys.stderr, flush=True)
+    return {"metric": "adjoint/forward cost ratio", "value": round(t_adj / t_fwd, 3), "unit": "x",
+            "higher_is_better": False, "steps": nsteps, "ckpt_every": ckpt, "particles": case.particles.n,
+            "forward_s": round(t_fwd, 4), "forward_plus_adjoint_s": round(t_adj, 4), "loss": L,
+            "grad_mu": g["mu"], "grad_xi": g["xi"], "fd_check_steps": fd_steps, "fd_check": fd}
 
 
 def cpu_baseline(case, nsteps):

@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "gfsph.h"
@@ -1665,6 +1666,561 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint (reverse mode) of one step, for the frictional / Voellmy laws
+// (nfric 5, 6) with pa_sph = 2, skf = 1, icpwp = 0, sle = 0.  With the
+// neighbour set of the step fixed, one step is
+//   S     = W(0) m_i + sum_j m_j W_ij                      (density)
+//   a_i   = -g sum_j m_j grad W_ij                         (pressure force, P = g/2)
+//   (x', v')_i = stages(x_i, v_i, max(S_i, h_inf), a_i; mu, xi)   (per particle)
+//   v''_i = v'_i + 0.3 sum_j 2 m_j W_ij / (S_i + S_j) (v'_j - v'_i)  (XSPH)
+// and its vector-Jacobian product runs backwards through these four maps:
+// the pair maps by gathers over the same neighbour masks (each particle
+// collects both sides of its pairs, so no atomics), the per-particle map by
+// forward-mode dual numbers over its 9 inputs.
+// ---------------------------------------------------------------------------
+#define AD_N 9  // x0 (2), v0 (2), rf, a (2), mu, xi
+
+struct Dual {
+  double v;
+  double d[AD_N];
+};
+
+__device__ __forceinline__ Dual dconst(double v) {
+  Dual r;
+  r.v = v;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = 0.0;
+  return r;
+}
+__device__ __forceinline__ Dual dvar(double v, int k) {
+  Dual r = dconst(v);
+  r.d[k] = 1.0;
+  return r;
+}
+__device__ __forceinline__ Dual operator+(const Dual &a, const Dual &b) {
+  Dual r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator-(const Dual &a, const Dual &b) {
+  Dual r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator-(const Dual &a) {
+  Dual r;
+  r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = -a.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator*(const Dual &a, const Dual &b) {
+  Dual r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator*(double a, const Dual &b) {
+  Dual r;
+  r.v = a * b.v;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = a * b.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator*(const Dual &b, double a) { return a * b; }
+__device__ __forceinline__ Dual operator/(const Dual &a, const Dual &b) {
+  Dual r;
+  r.v = a.v / b.v;
+  const double ib = 1.0 / b.v;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+  return r;
+}
+__device__ __forceinline__ Dual operator/(const Dual &a, double b) {
+  Dual r;
+  r.v = a.v / b;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = a.d[k] / b;
+  return r;
+}
+__device__ __forceinline__ Dual operator/(double a, const Dual &b) { return dconst(a) / b; }
+__device__ __forceinline__ Dual operator+(const Dual &a, double b) { return a + dconst(b); }
+__device__ __forceinline__ Dual operator+(double b, const Dual &a) { return a + dconst(b); }
+__device__ __forceinline__ Dual operator-(const Dual &a, double b) { return a - dconst(b); }
+__device__ __forceinline__ Dual operator-(double b, const Dual &a) { return dconst(b) - a; }
+__device__ __forceinline__ Dual dsqrt(const Dual &a) {
+  Dual r;
+  r.v = sqrt(a.v);
+  const double h = r.v > 0.0 ? 0.5 / r.v : 0.0;
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) r.d[k] = a.d[k] * h;
+  return r;
+}
+__device__ __forceinline__ double val(double a) { return a; }
+__device__ __forceinline__ double val(const Dual &a) { return a.v; }
+__device__ __forceinline__ double tsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ Dual tsqrt(const Dual &a) { return dsqrt(a); }
+template <typename T>
+__device__ __forceinline__ T tconst(double v);
+template <>
+__device__ __forceinline__ double tconst<double>(double v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ Dual tconst<Dual>(double v) {
+  return dconst(v);
+}
+
+// bilinear nodal slopes of Get_Z_Topo (same arithmetic as get_z_topo)
+template <typename T>
+__device__ __forceinline__ void slope_at(const DevDem &d, const T &xx, const T &yy, T &Zx, T &Zy) {
+  int ipx = (int)((val(xx) - d.xmin) / d.dx + 1);
+  int ipy = (int)((val(yy) - d.ymin) / d.dy + 1);
+  if (ipx == d.nx && fabs(d.xmax - val(xx)) <= 1.e-3) ipx = ipx - 1;
+  if (ipy == d.ny && fabs(d.ymax - val(yy)) <= 1.e-3) ipy = ipy - 1;
+  ipx = ipx < 1 ? 1 : (ipx > d.nx - 1 ? d.nx - 1 : ipx);
+  ipy = ipy < 1 ? 1 : (ipy > d.ny - 1 ? d.ny - 1 : ipy);
+  long n1 = (long)(ipy - 1) * d.nx + (ipx - 1);
+  long n2 = n1 + 1, n4 = n1 + d.nx, n3 = n4 + 1;
+  double xref = d.xmin + (ipx - 1) * d.dx;
+  double yref = d.ymin + (ipy - 1) * d.dy;
+  T xi = (xx - xref) / d.dx, eta = (yy - yref) / d.dy;
+  T sh1 = (1. - xi) * (1. - eta), sh2 = xi * (1. - eta), sh3 = xi * eta, sh4 = (1. - xi) * eta;
+  double2 a = d.zxy[n1], b = d.zxy[n2], c = d.zxy[n3], e = d.zxy[n4];
+  Zx = sh1 * a.x + sh2 * b.x + sh3 * c.x + sh4 * e.x;
+  Zy = sh1 * a.y + sh2 * b.y + sh3 * c.y + sh4 * e.y;
+}
+
+// ext_force for nfric 5/6 without curvature or pore pressure, with mu
+// (const(9)) and xi (const(3)) as arguments; T = double reproduces ext_force.
+template <typename T>
+__device__ __forceinline__ void ext_frictional(const DevParams &P, const DevDem &dem, const T &xx, const T &yy,
+                                               const T &vx1, const T &vy1, const T &rho_i, const T &inx,
+                                               const T &iny, const T &mu, const T &xi_voellmy, T &exo, T &eyo) {
+  const double dt = P.dt;
+  T h, hu, hv, hinv, vmod, hvmod, hvmodinv;
+  const double gravz = P.cgra;
+  T frictx = tconst<T>(0), fricty = tconst<T>(0), cf5 = tconst<T>(0);
+  h = rho_i;
+  hu = h * vx1;
+  hv = h * vy1;
+  if (val(h) < P.comp) {
+    h = tconst<T>(P.comp);
+    hu = tconst<T>(0);
+    hv = tconst<T>(0);
+    hinv = tconst<T>(0);
+    vmod = tconst<T>(0);
+    hvmod = tconst<T>(0);
+    hvmodinv = tconst<T>(0);
+  } else {
+    hinv = 1 / h;
+    hvmod = tsqrt(hu * hu + hv * hv);
+    vmod = hvmod * hinv;
+    hvmodinv = (val(hvmod) >= 1.e-5) ? 1. / hvmod : tconst<T>(0);
+  }
+  T Zx, Zy;
+  slope_at(dem, xx, yy, Zx, Zy);
+  T exx = tconst<T>(0) + (-P.cgra) * Zx;
+  T exy = tconst<T>(0) + (-P.cgra) * Zy;
+  T gtanfi = (gravz + 0.0) * mu * P.fact_dens;
+  if (val(gtanfi) < 0.0) gtanfi = tconst<T>(0);
+  cf5 = gtanfi;
+  frictx = cf5 * hu * hvmodinv * h;
+  fricty = cf5 * hv * hvmodinv * h;
+  if (val(xi_voellmy) > 0.0) {
+    T vm2 = vx1 * vx1 + vy1 * vy1;
+    vmod = (val(vm2) > 0.0) ? tsqrt(vm2) : tconst<T>(0);
+    frictx = frictx + P.cgra * vmod * vx1 / xi_voellmy;
+    fricty = fricty + P.cgra * vmod * vy1 / xi_voellmy;
+  }
+  T dcx = -frictx * hinv, dcy = -fricty * hinv;
+  T ax = exx + inx, ay = exy + iny;
+  if (val(vmod) > 1.e-5) {
+    T vsx = vx1 + dt * (ax + dcx), vsy = vy1 + dt * (ay + dcy);
+    T vdot = vsx * vx1 + vsy * vy1;
+    if (val(vdot) < 0) {
+      T tx = vx1 / vmod, ty = vy1 / vmod;
+      T accelT = ax * tx + ay * ty;
+      T deccelT = dcx * tx + dcy * ty;
+      if (val(accelT) * val(deccelT) < 0) {
+        dcx = -vx1 / dt - ax;
+        dcy = -vy1 / dt - ay;
+        frictx = -dcx * h;
+        fricty = -dcy * h;
+      }
+    }
+  } else {
+    T accelT = tsqrt(ax * ax + ay * ay);
+    if (val(accelT) > 1.e-5) {
+      T ctf = cf5;
+      frictx = ctf * ax / accelT;
+      fricty = ctf * ay / accelT;
+      T fricT = tsqrt(frictx * frictx + fricty * fricty);
+      if (val(accelT) <= val(fricT)) {
+        frictx = ax;
+        fricty = ay;
+      }
+    } else {
+      frictx = tconst<T>(0);
+      fricty = tconst<T>(0);
+    }
+    frictx = frictx * h;
+    fricty = fricty * h;
+  }
+  exo = exx - frictx / h;
+  eyo = exy - fricty / h;
+}
+
+// all RK stages of one particle and the final update (k_rk for these laws);
+// mid = true if the particle left the domain during the stages
+template <typename T>
+__device__ __forceinline__ void stage_map(const DevParams &P, const DevDem &dem, const T &x0x, const T &x0y,
+                                          const T &v0x, const T &v0y, const T &rf, const T &inx, const T &iny,
+                                          const T &mu, const T &xi, T &xox, T &xoy, T &vox, T &voy, bool &mid) {
+  const double dt = P.dt;
+  T xx = x0x, xy = x0y, vx = v0x, vy = v0y;
+  T ex = tconst<T>(0), ey = tconst<T>(0);
+  T Dx = tconst<T>(0), Dy = tconst<T>(0), Dvx = tconst<T>(0), Dvy = tconst<T>(0);
+  mid = false;
+  for (int st = 0; st < P.nstages; st++) {
+    const double f1 = P.f1[st];
+    xx = x0x + f1 * dt * vx;
+    xy = x0y + f1 * dt * vy;
+    vx = v0x + f1 * dt * (ex + inx);
+    vy = v0y + f1 * dt * (ey + iny);
+    if (out_of_domain(dem, val(xx), val(xy))) {
+      mid = true;
+      break;
+    }
+    ext_frictional(P, dem, xx, xy, vx, vy, rf, inx, iny, mu, xi, ex, ey);
+    const double f2 = P.f2[st];
+    Dvx = Dvx + f2 * (ex + inx);
+    Dvy = Dvy + f2 * (ey + iny);
+    Dx = Dx + f2 * vx;
+    Dy = Dy + f2 * vy;
+  }
+  xox = x0x + P.fin * Dx;
+  xoy = x0y + P.fin * Dy;
+  vox = v0x + P.fin * Dvx;
+  voy = v0y + P.fin * Dvy;
+}
+
+// cubic-spline kernel with its gradient factor a (grad W = a d) and Hessian
+// H = W'' dd^T/r^2 + (W'/r)(I - dd^T/r^2), from the squared separation dr
+__device__ __forceinline__ void kernel_hess(double kc, double dr, double dx, double dy, double hsml, double &w,
+                                            double &a, double &hxx, double &hxy, double &hyy) {
+  double r, ir;
+  root_nr(dr, r, ir);
+  const double ih = rcp_nr(hsml), ih2 = ih * ih;
+  const double q = r * ih, f = kc * ih2;
+  double w2 = 0.0, a1 = 0.0;
+  if (q <= 1.0) {
+    w = f * (2. / 3. - q * q + 0.5 * (q * q * q));
+    a = f * (-2. + 1.5 * q) * ih2;
+    w2 = f * (-2. + 3. * q) * ih2;
+  } else if (q <= 2.0) {
+    const double t = 2. - q;
+    w = f * (1.0 / 6.0) * (t * t * t);
+    a = -(f * 0.5 * (t * t) * ih) * ir;
+    w2 = f * t * ih2;
+  } else {
+    w = a = 0.0;
+  }
+  a1 = a;
+  // unit vector; at r = 0 the Hessian is isotropic with W''(0) = W'/r(0)
+  const double ux = r > 0 ? dx * ir : 0.0, uy = r > 0 ? dy * ir : 0.0;
+  hxx = w2 * ux * ux + a1 * (1.0 - ux * ux);
+  hxy = (w2 - a1) * ux * uy;
+  hyy = w2 * uy * uy + a1 * (1.0 - uy * uy);
+  if (!(r > 0)) {
+    hxx = hyy = a1;
+    hxy = 0.0;
+  }
+}
+
+// per-step adjoint work arrays, in the step's sorted order
+struct AdjWork {
+  double2 *lxp, *lvpp, *lvp, *la, *lx, *lxn, *lvn;  // lambda of x', v'', v', a, x (pairs), x^n, v^n
+  double *lS, *lm;
+  int *act;  // takes part in XSPH
+};
+
+// recorded forward step (sorted order of that step)
+struct TapeRec {
+  double4 *nb;
+  int *id, *sflag, *oflag;
+  double *S;
+  double2 *ain, *vp, *vn;  // vn: velocity at step start, caller order
+  unsigned long long *mask;
+  int *start;
+  GridInfo *grid;
+  int *coff;
+};
+
+__device__ __forceinline__ Cells tape_cells(const TapeRec &t) {
+  Cells cl;
+  cl.grid = t.grid;
+  cl.coff = t.coff;
+  cl.ntot = t.coff + 1;
+  cl.start = t.start;
+  return cl;
+}
+
+// K1: cotangents of the step outputs into sorted order
+__global__ void k_adj_load(int n, int npoin0, TapeRec t, const double2 *LX, const double2 *LV, const double *LH,
+                           AdjWork w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = t.id[i];
+  w.lxp[i] = LX[k];
+  w.lvpp[i] = LV[k];
+  w.lS[i] = LH ? LH[k] : 0.0;
+  w.lm[i] = 0.0;
+  w.lx[i] = make_double2(0.0, 0.0);
+  w.act[i] = (!t.sflag[i] && !t.oflag[i] && k >= npoin0) ? 1 : 0;
+}
+
+// K2: XSPH adjoint (gather over the step's neighbour masks)
+__global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeRec t, AdjWork w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (t.sflag[i]) {
+    w.lvp[i] = w.lvpp[i];
+    return;
+  }
+  const Cells cl = tape_cells(t);
+  const GridInfo g = cl.grid[0];
+  const double4 qi = t.nb[i];
+  const double2 pi_ = make_double2(qi.x, qi.y);
+  const double hi = qi.z, mi = qi.w;
+  const RowSpan rs = row_spans(g, cl.start, pi_);
+  unsigned long long m[NB_WORDS];
+  const bool full = load_mask(t.mask, i, m);
+  const double2 vi = t.vp[i], li = w.lvpp[i];
+  const double Si = t.S[i];
+  const int acti = w.act[i];
+  double csum = 0.0, lSi = 0.0, lmi = 0.0, gxa = 0.0, gya = 0.0, lvx = 0.0, lvy = 0.0;
+  auto term = [&](int j, const double4 &q) {
+    double dx, dy, dr, mh, wv, gx, gy;
+    pair_geom(pi_, hi, q, dx, dy, dr, mh);
+    sph_kernel_t<1>(P.kc, P.pi, dr, dx, dy, mh, wv, gx, gy);
+    const double2 vj = t.vp[j], lj = w.lvpp[j];
+    const double Ssum = Si + t.S[j];
+    const double cij = (2 * q.w) * wv / Ssum, cji = (2 * mi) * wv / Ssum;
+    const double dvx = vj.x - vi.x, dvy = vj.y - vi.y;
+    const double pi_side = acti ? li.x * dvx + li.y * dvy : 0.0;        // lambda_i . (v_j - v_i)
+    const double pj_side = w.act[j] ? -(lj.x * dvx + lj.y * dvy) : 0.0;  // lambda_j . (v_i - v_j)
+    if (acti) csum += cij;
+    if (w.act[j]) {
+      lvx += cji * lj.x;
+      lvy += cji * lj.y;
+    }
+    lSi += -(pi_side * cij + pj_side * cji) / Ssum;
+    lmi += pj_side * 2.0 * wv / Ssum;
+    const double f = pi_side * 2.0 * q.w / Ssum + pj_side * 2.0 * mi / Ssum;
+    gxa += f * gx;
+    gya += f * gy;
+  };
+  if (full) {
+    dense_neighbours(rs, P.scale_k, i, pi_, hi, t.nb,
+                     [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
+  } else {
+    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++)
+        if (ok[u]) term(js[u], t.nb[js[u]]);
+    });
+  }
+  const double keep = acti ? 1.0 - 0.3 * csum : 1.0;
+  w.lvp[i] = make_double2(li.x * keep + 0.3 * lvx, li.y * keep + 0.3 * lvy);
+  w.lS[i] += 0.3 * lSi;
+  w.lm[i] += 0.3 * lmi;
+  const double2 lx = w.lx[i];
+  w.lx[i] = make_double2(lx.x + 0.3 * gxa, lx.y + 0.3 * gya);
+}
+
+// K3: per-particle stage adjoint; mu / xi cotangents accumulate per block
+// (block b always adds to gmu[b], gxi[b]: a fixed summation order)
+__global__ __launch_bounds__(GF_BLOCK) void k_adj_stage(int n, DevParams P, DevDem dem, TapeRec t, AdjWork w,
+                                                        double *gmu, double *gxi) {
+  __shared__ double red[2][GF_BLOCK / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dmu = 0.0, dxi = 0.0;
+  if (i < n) {
+    const double2 lx = w.lxp[i], lv = w.lvp[i];
+    if (t.sflag[i]) {
+      w.lxn[i] = lx;
+      w.lvn[i] = lv;
+      w.la[i] = make_double2(0.0, 0.0);
+    } else {
+      const double4 q = t.nb[i];
+      const double2 v0 = t.vn[t.id[i]], a = t.ain[i];
+      const double Si = t.S[i];
+      const double rf = Si < P.h_inf ? P.h_inf : Si;
+      Dual X0 = dvar(q.x, 0), Y0 = dvar(q.y, 1), VX = dvar(v0.x, 2), VY = dvar(v0.y, 3), RF = dvar(rf, 4);
+      Dual AX = dvar(a.x, 5), AY = dvar(a.y, 6), MU = dvar(P.tanfi8, 7), XI = dvar(P.xi_voellmy, 8);
+      Dual xo, yo, vxo, vyo;
+      bool mid;
+      stage_map(P, dem, X0, Y0, VX, VY, RF, AX, AY, MU, XI, xo, yo, vxo, vyo, mid);
+      double g[AD_N];
+#pragma unroll
+      for (int k = 0; k < AD_N; k++)
+        g[k] = mid ? 0.0 : lx.x * xo.d[k] + lx.y * yo.d[k] + lv.x * vxo.d[k] + lv.y * vyo.d[k];
+      w.lxn[i] = make_double2(g[0], g[1]);
+      w.lvn[i] = make_double2(g[2], g[3]);
+      if (Si >= P.h_inf) w.lS[i] += g[4];
+      w.la[i] = make_double2(g[5], g[6]);
+      dmu = g[7];
+      dxi = g[8];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dmu += __shfl_xor(dmu, o, 64);
+    dxi += __shfl_xor(dxi, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = dmu;
+    red[1][threadIdx.x >> 6] = dxi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < GF_BLOCK / 64; k++) {
+      s0 += red[0][k];
+      s1 += red[1][k];
+    }
+    gmu[blockIdx.x] += s0;
+    gxi[blockIdx.x] += s1;
+  }
+}
+
+// K4: pressure-force and density adjoints (gathers), then the cotangents of
+// x^n, v^n and the mass back to caller order
+__global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, TapeRec t, AdjWork w, double2 *LX,
+                                                        double2 *LV, double *LM) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = t.id[i];
+  if (t.sflag[i]) {
+    LX[k] = w.lxn[i];
+    LV[k] = w.lvn[i];
+    return;
+  }
+  const Cells cl = tape_cells(t);
+  const GridInfo g = cl.grid[0];
+  const double4 qi = t.nb[i];
+  const double2 pi_ = make_double2(qi.x, qi.y);
+  const double hi = qi.z, mi = qi.w;
+  const RowSpan rs = row_spans(g, cl.start, pi_);
+  unsigned long long m[NB_WORDS];
+  const bool full = load_mask(t.mask, i, m);
+  const double2 lai = w.la[i];
+  const double lSi = w.lS[i];
+  const double grav = 2.0 * P.cgra05;
+  double lxx = 0.0, lxy = 0.0, lmi = 0.0;
+  auto term = [&](int j, const double4 &q) {
+    double dx, dy, dr, mh, wv, a, hxx, hxy, hyy;
+    pair_geom(pi_, hi, q, dx, dy, dr, mh);
+    kernel_hess(P.kc, dr, dx, dy, mh, wv, a, hxx, hxy, hyy);
+    const double2 laj = w.la[j];
+    // pressure force: -g sum_j H_ij (m_j la_i - m_i la_j); mass: sum_j la_j . (g grad W_ij)
+    const double bx = q.w * lai.x - mi * laj.x, by = q.w * lai.y - mi * laj.y;
+    lxx += -grav * (hxx * bx + hxy * by);
+    lxy += -grav * (hxy * bx + hyy * by);
+    lmi += grav * a * (laj.x * dx + laj.y * dy);
+    // density: sum_j (lS_i m_j + lS_j m_i) grad W_ij; mass: sum_j lS_j W_ij
+    const double lSj = w.lS[j];
+    const double c = (lSi * q.w + lSj * mi) * a;
+    lxx += c * dx;
+    lxy += c * dy;
+    lmi += lSj * wv;
+  };
+  if (full) {
+    dense_neighbours(rs, P.scale_k, i, pi_, hi, t.nb,
+                     [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
+  } else {
+    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++)
+        if (ok[u]) term(js[u], t.nb[js[u]]);
+    });
+  }
+  lmi += lSi * sph_self_w(1, P.pi, hi);
+  const double2 lxn = w.lxn[i], lxp = w.lx[i];
+  LX[k] = make_double2(lxn.x + lxp.x + lxx, lxn.y + lxp.y + lxy);
+  LV[k] = w.lvn[i];
+  LM[k] += w.lm[i] + lmi;
+}
+
+// Deposit misfit L = sum_nodes (R - T)^2 of the IDW depth raster R
+// (OutputRes_WIR) and its cotangents with respect to the particles' x and h
+__global__ void k_misfit_nodes(long np, const double *acc5, const double *target, double *res, double *partial) {
+  __shared__ double red[GF_BLOCK / 64];
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (g < np) {
+    const double ws = acc5[5 * g + 4];
+    const double R = ws >= 1.e-6 ? acc5[5 * g] / ws : acc5[5 * g];
+    const double e = R - target[g];
+    res[2 * g] = R;
+    res[2 * g + 1] = e;
+    l = e * e;
+  }
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < GF_BLOCK / 64; k++) s += red[k];
+    partial[blockIdx.x] = s;
+  }
+}
+
+__global__ void k_misfit_vjp(int n, DevDem dem, CurView s, const double *acc5, const double *res, double2 *LX,
+                             double *LH) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = s.id[i];
+  if (s.flag[i]) {
+    LX[k] = make_double2(0.0, 0.0);
+    LH[k] = 0.0;
+    return;
+  }
+  const double2 p = s.pos[i];
+  const double h = s.h[i];
+  const double distg = pow(dem.dx * dem.dx + dem.dy * dem.dy, 0.5);
+  int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
+  int inx0 = ipx - 1 > 1 ? ipx - 1 : 1, iny0 = ipy - 1 > 1 ? ipy - 1 : 1;
+  int inx1 = ipx + 2 < dem.nx ? ipx + 2 : dem.nx, iny1 = ipy + 2 < dem.ny ? ipy + 2 : dem.ny;
+  double lh = 0.0, lx = 0.0, ly = 0.0;
+  for (int inx = inx0; inx <= inx1; inx++)
+    for (int iny = iny0; iny <= iny1; iny++) {
+      long g = (long)(iny - 1) * dem.nx + inx - 1;
+      const double ws = acc5[5 * g + 4];
+      if (!(ws >= 1.e-6)) continue;
+      const double R = res[2 * g], e2 = 2.0 * res[2 * g + 1];
+      double gx = dem.xmin + (inx - 1) * dem.dx, gy = dem.ymin + (iny - 1) * dem.dy;
+      double a = p.x - gx, b = p.y - gy;
+      const double rho = pow(a * a + b * b, 0.5);
+      const double dist = fmax(1.e-6, rho) / distg;
+      lh += e2 / (dist * ws);
+      if (rho > 1.e-6) {
+        // d(1/dist)/dx = -distg (x - node) / rho^3
+        const double c = e2 * (h - R) / ws * (-distg / (rho * rho * rho));
+        lx += c * a;
+        ly += c * b;
+      }
+    }
+  LX[k] = make_double2(lx, ly);
+  LH[k] = lh;
+}
+
+// ---------------------------------------------------------------------------
 // initialisation helpers
 // ---------------------------------------------------------------------------
 __global__ void k_domain_flag(int n, DevDem dem, const double2 *pos, int *flag) {
@@ -1952,6 +2508,19 @@ struct gfsph_ctx {
   // per-realisation Monte Carlo pixel accumulators [R][npoig] (lazily allocated)
   double *mcr_acc = nullptr;
   int *mcr_cnt = nullptr;
+  // adjoint: checkpoints of the whole device state every ckpt steps, the
+  // records of one recomputed segment, cotangents in caller order
+  struct Snap {
+    SortedSet s;
+    NewSet o;
+    int cur;
+  };
+  int ad_ckpt = 0, ad_nsteps = 0;
+  std::vector<Snap> snaps;  // [nseg + 1]: segment starts and the final state
+  std::vector<TapeRec> tape;
+  AdjWork aw{};
+  double2 *LX = nullptr, *LV = nullptr;
+  double *LM = nullptr, *LH = nullptr, *gmu = nullptr, *gxi = nullptr;
   std::vector<void *> allocs;
 };
 
@@ -2832,6 +3401,240 @@ int gfsph_autotune(gfsph_ctx *c, int32_t reps, int32_t *chosen) {
   drop_graphs(c);  // recaptured with the chosen sizes
   if (chosen)
     for (int k = 0; k < 3; k++) chosen[k] = best[k];
+  return GFSPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint host side
+// ---------------------------------------------------------------------------
+static int alloc_snap(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
+  const int n = c->n;
+  int rc;
+  SortedSet &s = sn.s;
+  NewSet &o = sn.o;
+  if ((rc = dalloc(c, &s.nb, n)) || (rc = dalloc(c, &s.pos, n)) || (rc = dalloc(c, &s.vel, n)) ||
+      (rc = dalloc(c, &s.h, n)) || (rc = dalloc(c, &s.mass, n)) || (rc = dalloc(c, &s.hsml, n)) ||
+      (rc = dalloc(c, &s.u, n)) || (rc = dalloc(c, &s.uaux, n)) || (rc = dalloc(c, &s.flag, n)) ||
+      (rc = dalloc(c, &s.id, n)) || (rc = dalloc(c, &s.rid, n)) || (rc = dalloc(c, &o.pos, n)) ||
+      (rc = dalloc(c, &o.vel, n)) || (rc = dalloc(c, &o.h, n)) || (rc = dalloc(c, &o.hsml, n)) ||
+      (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) || (rc = dalloc(c, &o.flag, n)))
+    return rc;
+  return GFSPH_OK;
+}
+
+#define DCOPY(dst, src, count) \
+  HIPCHK(c, hipMemcpyAsync((dst), (src), sizeof(*(src)) * (size_t)(count), hipMemcpyDeviceToDevice, c->stream))
+
+// copies the step state (the current sorted set, the new set, the parity)
+static int copy_state(gfsph_ctx *c, SortedSet &sd, NewSet &od, const SortedSet &ss, const NewSet &os) {
+  const int n = c->n;
+  DCOPY(sd.nb, ss.nb, n);
+  DCOPY(sd.pos, ss.pos, n);
+  DCOPY(sd.vel, ss.vel, n);
+  DCOPY(sd.h, ss.h, n);
+  DCOPY(sd.mass, ss.mass, n);
+  DCOPY(sd.hsml, ss.hsml, n);
+  DCOPY(sd.u, ss.u, n);
+  DCOPY(sd.uaux, ss.uaux, n);
+  DCOPY(sd.flag, ss.flag, n);
+  DCOPY(sd.id, ss.id, n);
+  DCOPY(sd.rid, ss.rid, n);
+  DCOPY(od.pos, os.pos, n);
+  DCOPY(od.vel, os.vel, n);
+  DCOPY(od.h, os.h, n);
+  DCOPY(od.hsml, os.hsml, n);
+  DCOPY(od.u, os.u, n);
+  DCOPY(od.uaux, os.uaux, n);
+  DCOPY(od.flag, os.flag, n);
+  return GFSPH_OK;
+}
+
+static int snap_save(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
+  sn.cur = c->cur;
+  return copy_state(c, sn.s, sn.o, c->set[c->cur], c->nw);
+}
+
+static int snap_restore(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
+  c->cur = sn.cur;
+  return copy_state(c, c->set[c->cur], c->nw, sn.s, sn.o);
+}
+
+__global__ void k_scatter_vel(int n, const int *id, const double2 *vel, double2 *vn) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) vn[id[i]] = vel[i];
+}
+
+static int alloc_tape(gfsph_ctx *c, TapeRec &t) {
+  const int n = c->n;
+  int rc;
+  if ((rc = dalloc(c, &t.nb, n)) || (rc = dalloc(c, &t.id, n)) || (rc = dalloc(c, &t.sflag, n)) ||
+      (rc = dalloc(c, &t.oflag, n)) || (rc = dalloc(c, &t.S, n)) || (rc = dalloc(c, &t.ain, n)) ||
+      (rc = dalloc(c, &t.vp, n)) || (rc = dalloc(c, &t.vn, n)) || (rc = dalloc(c, &t.mask, NB_WORDS * (size_t)n)) ||
+      (rc = dalloc(c, &t.start, (size_t)c->cap_cells + 2)) || (rc = dalloc(c, &t.grid, 1)) ||
+      (rc = dalloc(c, &t.coff, 2)))
+    return rc;
+  return GFSPH_OK;
+}
+
+// one recomputed step with its record
+static int taped_step(gfsph_ctx *c, TapeRec &t) {
+  const int n = c->n;
+  const SortedSet &s0 = c->set[c->cur];
+  k_scatter_vel<<<nblk(n), GF_BLOCK, 0, c->stream>>>(n, s0.id, s0.vel, t.vn);
+  int rc = run_steps(c, 1);
+  if (rc) return rc;
+  const SortedSet &s = c->set[c->cur];
+  DCOPY(t.nb, s.nb, n);
+  DCOPY(t.id, s.id, n);
+  DCOPY(t.sflag, s.flag, n);
+  DCOPY(t.oflag, c->nw.flag, n);
+  DCOPY(t.S, c->S, n);
+  DCOPY(t.ain, c->ain, n);
+  DCOPY(t.vp, c->nw.vel, n);
+  DCOPY(t.mask, c->mask, NB_WORDS * (size_t)n);
+  DCOPY(t.start, c->start, (size_t)c->cap_cells + 2);
+  DCOPY(t.grid, c->grid, 1);
+  DCOPY(t.coff, c->coff, 2);
+  return GFSPH_OK;
+}
+
+static int adjoint_supported(gfsph_ctx *c) {
+  const DevParams &P = c->P;
+  if (c->R != 1) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs on a single realisation");
+  if (P.nfric != 5 || P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 ||
+      P.xi_voellmy < 0.0)
+    return fail(c, GFSPH_ERR_UNSUPPORTED,
+                "the adjoint covers nfric = 5 (frictional / Voellmy, xi >= 0) with pa_sph = 2, skf = 1, "
+                "icpwp = 0, sle = 0");
+  return GFSPH_OK;
+}
+
+int gfsph_adjoint_forward(gfsph_ctx *c, int32_t nsteps, int32_t ckpt_every) {
+  if (!c || c->n <= 0 || nsteps <= 0 || ckpt_every <= 0) return GFSPH_ERR_ARG;
+  int rc = adjoint_supported(c);
+  if (rc) return rc;
+  const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+  if (c->ad_ckpt != ckpt_every || (int)c->snaps.size() < nseg + 1) {
+    if ((int)c->snaps.size() < nseg + 1) {
+      const size_t old = c->snaps.size();
+      c->snaps.resize(nseg + 1);
+      for (size_t k = old; k < c->snaps.size(); k++)
+        if ((rc = alloc_snap(c, c->snaps[k]))) return rc;
+    }
+    if ((int)c->tape.size() < ckpt_every) {
+      const size_t old = c->tape.size();
+      c->tape.resize(ckpt_every);
+      for (size_t k = old; k < c->tape.size(); k++)
+        if ((rc = alloc_tape(c, c->tape[k]))) return rc;
+    }
+  }
+  c->ad_ckpt = ckpt_every;
+  c->ad_nsteps = nsteps;
+  for (int seg = 0; seg < nseg; seg++) {
+    if ((rc = snap_save(c, c->snaps[seg]))) return rc;
+    const int L = std::min(ckpt_every, nsteps - seg * ckpt_every);
+    if ((rc = run_steps(c, L))) return rc;
+  }
+  if ((rc = snap_save(c, c->snaps[nseg]))) return rc;
+  HIPCHK(c, hipGetLastError());
+  return GFSPH_OK;
+}
+
+int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_v, const double *lam_h,
+                           double *g_mu, double *g_xi, double *g_mass, double *g_x0, double *g_v0) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  if (c->ad_nsteps <= 0) return fail(c, GFSPH_ERR_ARG, "gfsph_adjoint_forward has not run");
+  const int n = c->n, K = c->ad_ckpt, N = c->ad_nsteps, nseg = (N + K - 1) / K;
+  hipStream_t st = c->stream;
+  int rc;
+  AdjWork &w = c->aw;
+  if (!c->LX) {
+    if ((rc = dalloc(c, &c->LX, n)) || (rc = dalloc(c, &c->LV, n)) || (rc = dalloc(c, &c->LM, n)) ||
+        (rc = dalloc(c, &c->LH, n)) || (rc = dalloc(c, &c->gmu, nblk(n))) || (rc = dalloc(c, &c->gxi, nblk(n))) ||
+        (rc = dalloc(c, &w.lxp, n)) || (rc = dalloc(c, &w.lvpp, n)) || (rc = dalloc(c, &w.lvp, n)) ||
+        (rc = dalloc(c, &w.la, n)) || (rc = dalloc(c, &w.lx, n)) || (rc = dalloc(c, &w.lxn, n)) ||
+        (rc = dalloc(c, &w.lvn, n)) || (rc = dalloc(c, &w.lS, n)) || (rc = dalloc(c, &w.lm, n)) ||
+        (rc = dalloc(c, &w.act, n)))
+      return rc;
+  }
+  std::vector<double> zero(2 * (size_t)n, 0.0);
+  HIPCHK(c, hipMemcpyAsync(c->LX, lam_x ? lam_x : zero.data(), sizeof(double2) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->LV, lam_v ? lam_v : zero.data(), sizeof(double2) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->LH, lam_h ? lam_h : zero.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemsetAsync(c->LM, 0, sizeof(double) * n, st));
+  HIPCHK(c, hipMemsetAsync(c->gmu, 0, sizeof(double) * nblk(n), st));
+  HIPCHK(c, hipMemsetAsync(c->gxi, 0, sizeof(double) * nblk(n), st));
+  const int nb = nblk(n);
+  for (int seg = nseg - 1; seg >= 0; seg--) {
+    if ((rc = snap_restore(c, c->snaps[seg]))) return rc;
+    const int L = std::min(K, N - seg * K);
+    for (int t = 0; t < L; t++)
+      if ((rc = taped_step(c, c->tape[t]))) return rc;
+    for (int t = L - 1; t >= 0; t--) {
+      TapeRec &tr = c->tape[t];
+      const bool last = (seg == nseg - 1) && (t == L - 1);
+      k_adj_load<<<nb, GF_BLOCK, 0, st>>>(n, c->P.npoin0, tr, c->LX, c->LV, last ? c->LH : nullptr, w);
+      if (c->P.av_vel) {
+        k_adj_xsph<<<nb, GF_BLOCK, 0, st>>>(n, c->P, tr, w);
+      } else {
+        HIPCHK(c, hipMemcpyAsync(w.lvp, w.lvpp, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+      }
+      k_adj_stage<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->dem, tr, w, c->gmu, c->gxi);
+      k_adj_pairs<<<nb, GF_BLOCK, 0, st>>>(n, c->P, tr, w, c->LX, c->LV, c->LM);
+    }
+    HIPCHK(c, hipGetLastError());
+  }
+  // back to the state the forward pass ended in
+  if ((rc = snap_restore(c, c->snaps[nseg]))) return rc;
+  std::vector<double> gm(nb), gx(nb);
+  HIPCHK(c, hipMemcpyAsync(gm.data(), c->gmu, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(gx.data(), c->gxi, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+  if (g_mass) HIPCHK(c, hipMemcpyAsync(g_mass, c->LM, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  if (g_x0) HIPCHK(c, hipMemcpyAsync(g_x0, c->LX, sizeof(double2) * n, hipMemcpyDeviceToHost, st));
+  if (g_v0) HIPCHK(c, hipMemcpyAsync(g_v0, c->LV, sizeof(double2) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  double smu = 0.0, sxi = 0.0;
+  for (int b = 0; b < nb; b++) {
+    smu += gm[b];
+    sxi += gx[b];
+  }
+  if (g_mu) *g_mu = smu;
+  if (g_xi) *g_xi = sxi;
+  return GFSPH_OK;
+}
+
+int gfsph_raster_misfit(gfsph_ctx *c, const double *target, double *loss, double *lam_x, double *lam_h) {
+  if (!c || c->n <= 0 || !target || !loss) return GFSPH_ERR_ARG;
+  const long np = (long)c->dem.nx * c->dem.ny;
+  const int n = c->n;
+  hipStream_t st = c->stream;
+  int rc;
+  if (!c->raster5 && (rc = dalloc(c, &c->raster5, 5 * np + 5 * np))) return rc;
+  const int nbn = nblk(np);
+  void *tmp = nullptr;
+  const size_t bytes = sizeof(double) * (np + 2 * np + nbn) + sizeof(double2) * n + sizeof(double) * n;
+  HIPCHK(c, hipMalloc(&tmp, bytes));
+  double *d_t = (double *)tmp, *d_res = d_t + np, *d_part = d_res + 2 * np;
+  double2 *d_lx = (double2 *)(d_part + nbn);
+  double *d_lh = (double *)(d_lx + n);
+  std::vector<double> part(nbn);
+  hipError_t e = hipMemcpyAsync(d_t, target, sizeof(double) * np, hipMemcpyHostToDevice, st);
+  if (!e) e = hipMemsetAsync(c->raster5, 0, sizeof(double) * 5 * np, st);
+  if (!e) {
+    k_raster_idw<<<nblk(n), GF_BLOCK, 0, st>>>(n, c->dem, cur_view(c), c->raster5);
+    k_misfit_nodes<<<nbn, GF_BLOCK, 0, st>>>(np, c->raster5, d_t, d_res, d_part);
+    k_misfit_vjp<<<nblk(n), GF_BLOCK, 0, st>>>(n, c->dem, cur_view(c), c->raster5, d_res, d_lx, d_lh);
+    e = hipGetLastError();
+  }
+  if (!e) e = hipMemcpyAsync(part.data(), d_part, sizeof(double) * nbn, hipMemcpyDeviceToHost, st);
+  if (!e && lam_x) e = hipMemcpyAsync(lam_x, d_lx, sizeof(double2) * n, hipMemcpyDeviceToHost, st);
+  if (!e && lam_h) e = hipMemcpyAsync(lam_h, d_lh, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+  if (!e) e = hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (e) return fail(c, GFSPH_ERR_HIP, std::string("raster misfit: ") + hipGetErrorString(e));
+  double L = 0.0;
+  for (double v : part) L += v;
+  *loss = L;
   return GFSPH_OK;
 }
 

@@ -16,13 +16,15 @@ import numpy as np
 
 from ._build import LIB
 
-__all__ = ["GfsphError", "SPHModel", "BatchModel", "neighbour_pairs", "load_library", "EXPORTED_SYMBOLS"]
+__all__ = ["GfsphError", "SPHModel", "BatchModel", "AdjointModel", "neighbour_pairs", "load_library",
+           "EXPORTED_SYMBOLS"]
 
 EXPORTED_SYMBOLS = (
     "gfsph_create", "gfsph_step", "gfsph_sync", "gfsph_get_state", "gfsph_neighbour_pairs",
     "gfsph_raster_idw", "gfsph_mc_pixel_accumulate", "gfsph_mc_pixel_read", "gfsph_profile_step",
     "gfsph_counts", "gfsph_last_error", "gfsph_destroy", "gfsph_batch_create", "gfsph_batch_pixel_accumulate",
-    "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune",
+    "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune", "gfsph_adjoint_forward",
+    "gfsph_adjoint_backward", "gfsph_raster_misfit",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -84,6 +86,9 @@ def load_library(path: str | None = None):
     L.gfsph_batch_pixel_read.argtypes = [vp, C.c_int32, dp, ip]
     L.gfsph_batch_hazard.argtypes = [vp, C.c_double, dp, ip, dp, dp]
     L.gfsph_autotune.argtypes = [vp, C.c_int32, ip]
+    L.gfsph_adjoint_forward.argtypes = [vp, C.c_int32, C.c_int32]
+    L.gfsph_adjoint_backward.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.gfsph_raster_misfit.argtypes = [vp, dp, dp, dp, dp]
     if path is None:
         _lib = L
     return L
@@ -277,6 +282,42 @@ class BatchModel(SPHModel):
         self._check(self.lib.gfsph_batch_hazard(self.ctx, float(hthr), _dp(stats), _ip(nprob), _dp(maxs), _dp(pexc)))
         return {"mean": stats[:, :, 0], "std": stats[:, :, 1], "nprob": nprob, "max_h": maxs[:, 0],
                 "max_v": maxs[:, 1], "p_exceed": pexc}
+
+
+class AdjointModel(SPHModel):
+    """A run with its reverse mode: ``forward(nsteps, ckpt)`` advances and
+    keeps checkpoints, ``misfit(target)`` gives the deposit misfit of the
+    current state and its cotangents, ``backward(...)`` the gradients with
+    respect to mu (const(9)), xi (const(3)), the particle masses (volumes)
+    and the initial positions and velocities."""
+
+    def forward(self, nsteps: int, ckpt_every: int = 50) -> None:
+        self._check(self.lib.gfsph_adjoint_forward(self.ctx, int(nsteps), int(ckpt_every)))
+        self.nsteps_done += int(nsteps)
+
+    def misfit(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float64).ravel()
+        lx = np.zeros((self.n, 2))
+        lh = np.zeros(self.n)
+        L = C.c_double()
+        self._check(self.lib.gfsph_raster_misfit(self.ctx, _dp(t), C.byref(L), _dp(lx), _dp(lh)))
+        return L.value, lx, lh
+
+    def backward(self, lam_x=None, lam_v=None, lam_h=None) -> dict:
+        def arg(a, shape):
+            return None if a is None else np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float64)
+
+        lx, lv, lh = arg(lam_x, (self.n, 2)), arg(lam_v, (self.n, 2)), arg(lam_h, (self.n,))
+        gmu, gxi = C.c_double(), C.c_double()
+        gm = np.zeros(self.n)
+        gx = np.zeros((self.n, 2))
+        gv = np.zeros((self.n, 2))
+        nul = C.POINTER(C.c_double)()
+        self._check(self.lib.gfsph_adjoint_backward(self.ctx, _dp(lx) if lx is not None else nul,
+                                                    _dp(lv) if lv is not None else nul,
+                                                    _dp(lh) if lh is not None else nul, C.byref(gmu), C.byref(gxi),
+                                                    _dp(gm), _dp(gx), _dp(gv)))
+        return {"mu": gmu.value, "xi": gxi.value, "mass": gm, "x0": gx, "v0": gv}
 
 
 def neighbour_pairs(x, hsml, flag=None, skf: int = 1) -> np.ndarray:

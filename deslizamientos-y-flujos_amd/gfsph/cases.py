@@ -25,7 +25,8 @@ import numpy as np
 from .dem import DEM, fractal_dem, plane_runout_dem
 
 __all__ = ["Params", "Particles", "Case", "f32", "voellmy", "frictional", "bingham",
-           "lattice_block", "elliptical_scar", "cfg_plane_voellmy", "cfg_fractal_frictional", "cfg_mc_hazard"]
+           "lattice_block", "elliptical_scar", "cfg_plane_voellmy", "cfg_fractal_frictional", "cfg_mc_hazard",
+           "cfg_adjoint"]
 
 
 def f32(v: float) -> float:
@@ -216,3 +217,22 @@ def cfg_mc_hazard(R: int = 256, dem_n: int = 1024, d: float = 5.0, n_range=(5_00
         prm = Params(integrator=integrator, dt=dt, h_inf=1.0e-3, const=voellmy(float(mus[r]), float(xis[r])))
         out.append(Case(f"mc{r}", prm, dem, parts, nsteps))
     return out
+
+
+def cfg_adjoint(n: int = 50_000, dem_n: int = 512, d: float = 5.0, nsteps: int = 1000, mu: float = 0.2,
+                xi: float = 500.0, seed: int = 4004, spacing: float = 2.0, dt: float = 0.05) -> Case:
+    """Adjoint back-analysis: fractal DEM 512 x 512 at 5 m (H 0.8, 400 m
+    relief, carved valley); elliptical release of 50,000 particles (spacing
+    2 m, hsml 3 m, thickness ~ U(2, 8) m) on the valley axis at x = 0.25 L;
+    Voellmy with
+    (mu, xi) = (0.2, 500) as the starting guess (the target deposit comes from
+    mu = 0.17, xi = 650); RK2, dt = 0.05 s, 1,000 steps."""
+    dem = fractal_dem(dem_n, d, seed)
+    L = (dem.nx - 1) * dem.dx
+    cx = 0.25 * L
+    ix = int(round(cx / dem.dx))
+    col = dem.Z[:, ix]
+    iy = int(np.argmin(np.abs(np.arange(dem.ny) * dem.dy - 0.5 * L) + 1e-9 * col))
+    parts = elliptical_scar(cx, iy * dem.dy, n, spacing, 2.0, seed + 1)
+    prm = Params(integrator=2, dt=dt, h_inf=1.0e-3, const=voellmy(mu, xi))
+    return Case("adjoint", prm, dem, parts, nsteps)
