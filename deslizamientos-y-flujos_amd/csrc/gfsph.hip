@@ -59,9 +59,15 @@ struct DevDem {
 
 struct GridInfo {
   double xmin, ymin, cell;
+  double hmax;    // largest smoothing length of the active particles
   int ncx, ncy, ncell;
   int uniform_h;  // every active particle has the same smoothing length
 };
+
+// Cells are half the largest support radius, so a particle's candidates lie
+// in the 5 x 5 cells around its own (rows cy-2 .. cy+2).
+#define CELL_FRAC 0.5
+#define NROWS 5
 
 // bounding-box accumulator of active particles (ordered-integer encoding)
 struct BBoxAcc {
@@ -647,16 +653,16 @@ __global__ void k_bbox(int n, const double2 *pos, const double *hsml, const int 
   bbox_block(part, act, p, hs);
 }
 
-// Grid of the active bounding box.  Cell edge = scale_k*max(hsml)*(1+1e-7),
-// so every pair passing the predicate lies in adjacent cells; grown if the
-// grid would exceed cap_cells.  Every block derives the same grid; block 0
-// stores it.  Then all blocks clear the counts of ncell+1 buckets.
+// Grid of the active bounding box.  Cell edge = CELL_FRAC*scale_k*max(hsml)
+// *(1+1e-7), so every pair passing the predicate lies within 2 cells in x
+// and y; grown if the grid would exceed cap_cells.
 __device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int cap_cells) {
   GridInfo g;
   if (a.nactive == 0) {
     g.xmin = 0;
     g.ymin = 0;
     g.cell = 1;
+    g.hmax = 0;
     g.ncx = 1;
     g.ncy = 1;
     g.ncell = 1;
@@ -665,7 +671,7 @@ __device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int
   }
   double xmin = val_of(a.xmin), xmax = val_of(a.xmax), ymin = val_of(a.ymin), ymax = val_of(a.ymax);
   double hmax = val_of(a.hmax);
-  double cell = scale_k * hmax * (1.0 + 1.0e-7);
+  double cell = CELL_FRAC * scale_k * hmax * (1.0 + 1.0e-7);
   if (!(cell > 0)) cell = 1.0;
   double lx = xmax - xmin, ly = ymax - ymin;
   double ncx = floor(lx / cell) + 1, ncy = floor(ly / cell) + 1;
@@ -678,6 +684,7 @@ __device__ __forceinline__ GridInfo make_grid(const BBoxAcc &a, int scale_k, int
   g.xmin = xmin;
   g.ymin = ymin;
   g.cell = cell;
+  g.hmax = hmax;
   g.ncx = (int)ncx;
   g.ncy = (int)ncy;
   g.ncell = g.ncx * g.ncy;
@@ -972,8 +979,8 @@ template <typename F>
 __device__ __forceinline__ void for_candidates(const GridInfo &g, const int *start, double2 p, F &&f) {
   int cx, cy;
   cell_of(g, p, cx, cy);
-  int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
-  int r0 = cy > 0 ? cy - 1 : 0, r1 = cy < g.ncy - 1 ? cy + 1 : g.ncy - 1;
+  int c0 = cx > 1 ? cx - 2 : 0, c1 = cx < g.ncx - 2 ? cx + 2 : g.ncx - 1;
+  int r0 = cy > 1 ? cy - 2 : 0, r1 = cy < g.ncy - 2 ? cy + 2 : g.ncy - 1;
   for (int r = r0; r <= r1; r++) {
     int b = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
     for (int j = b; j < e; j++) f(j);
@@ -981,12 +988,12 @@ __device__ __forceinline__ void for_candidates(const GridInfo &g, const int *sta
 }
 
 // Neighbour masks: the density pass records, for each particle, which
-// candidates of each of its (up to) 3 cell-row ranges pass the predicate:
-// 128 bits per row (row cy-1+k in words 2k, 2k+1).  A particle with a row
-// range of more than 128 candidates is marked full (all words set) and
-// re-tests the predicate.  The force and XSPH passes then visit exactly the
-// neighbours, in ascending slot order within each row.
-#define NB_WORDS 6
+// candidates of each of its 5 cell-row ranges pass the predicate: 64 bits per
+// row (row cy-2+k in word k).  A particle with a row range of more than 64
+// candidates is marked full (all words set) and re-tests the predicate.  The
+// force and XSPH passes then visit exactly the neighbours, in ascending slot
+// order within each row.
+#define NB_WORDS NROWS
 #define NB_FULL 0xffffffffffffffffull
 
 // frozen out-of-domain particle: IntForces' h_inf floor (6:6769) and
@@ -1305,38 +1312,70 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour gathers: one lane per particle.  The particle's 3 stencil rows
-// are contiguous ranges of the cell-sorted set.  Loads are issued in batches
-// of NB_BATCH independent candidates before any of them is used, and the
-// batch bookkeeping (mask bits, slot extraction) is branch-free, so a wave's
-// instruction stream is mostly FP64 arithmetic.  Sums run in slot order
-// (row, then slot within the row), so results are deterministic.
+// Neighbour gathers.  A particle's candidates lie in the 5 cell rows
+// cy-2 .. cy+2; in each row they are one contiguous slot range of the
+// cell-sorted set, trimmed to the columns the support circle reaches in that
+// row (x within +-w of the particle, w^2 = T^2 - dy^2 with dy the distance to
+// the row band and T = scale_k*(h_i + h_max)/2 an upper bound of every pair's
+// support radius: candidates outside cannot be neighbours).  LANES lanes
+// share a particle: lane t takes rows t, t+LANES, ...  Loads are issued in
+// batches of NB_BATCH independent candidates before any of them is used, the
+// batch bookkeeping is branch-free, and sums run in a fixed order (rows of a
+// lane in order, slots within a row in order, then lane 0 + lane 1).
 // ---------------------------------------------------------------------------
 #define NB_BATCH 4
 #define GATHER_MAX_BLOCK 512
 
 struct RowSpan {
-  int b[3], cnt[3];  // first slot and candidate count of each stencil row (0: row outside the grid)
-  bool full;         // some row holds more than 128 candidates: masks are not kept
+  int b[NROWS], cnt[NROWS];  // first slot and candidate count of each row (0: not this lane's or empty)
+  bool full;                 // some row of this lane holds more than 64 candidates
 };
 
-__device__ __forceinline__ RowSpan row_spans(const GridInfo &g, const int *start, double2 p) {
+template <int LANES>
+__device__ __forceinline__ RowSpan row_spans(const GridInfo &g, const int *start, double2 p, double hi, int scale_k,
+                                             int t) {
   int cx, cy;
   cell_of(g, p, cx, cy);
-  const int c0 = cx > 0 ? cx - 1 : 0, c1 = cx < g.ncx - 1 ? cx + 1 : g.ncx - 1;
+  const double T = scale_k * (hi + g.hmax) * 0.5 * (1.0 + 1.0e-9);
+  const double T2 = T * T;
+  const int cmin = cx > 1 ? cx - 2 : 0, cmax = cx < g.ncx - 2 ? cx + 2 : g.ncx - 1;
   RowSpan rs;
   rs.full = false;
 #pragma unroll
-  for (int rr = 0; rr < 3; rr++) {
-    const int r = cy - 1 + rr;
-    const bool in = r >= 0 && r < g.ncy;
-    const int rc = in ? r : cy;
-    const int b = start[rc * g.ncx + c0], e = start[rc * g.ncx + c1 + 1];
-    rs.b[rr] = b;
-    rs.cnt[rr] = in ? e - b : 0;
-    rs.full = rs.full || rs.cnt[rr] > 128;
+  for (int rr = 0; rr < NROWS; rr++) {
+    rs.b[rr] = 0;
+    rs.cnt[rr] = 0;
+    if (rr % LANES != t) continue;
+    const int r = cy - 2 + rr;
+    if (r < 0 || r >= g.ncy) continue;
+    const double y0 = g.ymin + r * g.cell, y1 = y0 + g.cell;
+    const double dy = p.y < y0 ? y0 - p.y : (p.y > y1 ? p.y - y1 : 0.0);
+    const double w2 = T2 - dy * dy;
+    if (!(w2 > 0.0)) continue;
+    const double w = sqrt(w2);
+    int c0 = (int)floor((p.x - w - g.xmin) / g.cell), c1 = (int)floor((p.x + w - g.xmin) / g.cell);
+    c0 = c0 < cmin ? cmin : c0;
+    c1 = c1 > cmax ? cmax : c1;
+    if (c1 < c0) continue;
+    const int bb = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
+    rs.b[rr] = bb;
+    rs.cnt[rr] = e - bb;
+    rs.full = rs.full || (e - bb) > 64;
   }
   return rs;
+}
+
+template <int LANES>
+__device__ __forceinline__ bool lanes_or(bool v) {
+  if (LANES == 1) return v;
+  return __shfl_xor(v ? 1 : 0, 1, 64) != 0 || v;
+}
+
+template <int LANES>
+__device__ __forceinline__ double lanes_sum(double v) {
+  if (LANES == 1) return v;
+  const double o = __shfl_xor(v, 1, 64);
+  return (threadIdx.x & 1) ? o + v : v + o;
 }
 
 // Separation and mean smoothing length of a pair, rounded as in pair_pred
@@ -1366,17 +1405,17 @@ __device__ __forceinline__ PredBand pred_band(double t) {
   return pb;
 }
 
-// Neighbour mask of particle i: bit k of words (2*rr, 2*rr+1) is slot k of
-// stencil row rr.  UNIFORM: all smoothing lengths are equal, so the support
-// radius scale_k*(hi+hj)/2 = scale_k*hi is the same for every candidate.
+// Neighbour mask words of this lane's rows: bit k of word rr is slot k of
+// row rr.  UNIFORM: all smoothing lengths are equal, so the support radius
+// scale_k*(hi+hj)/2 = scale_k*hi is the same for every candidate.
 template <bool UNIFORM>
 __device__ __forceinline__ void build_mask(const RowSpan &rs, int scale_k, int i, double2 pi_, double hi,
                                            const double4 *nb, unsigned long long (&m)[NB_WORDS]) {
   const PredBand pu = pred_band(scale_k * hi);
 #pragma unroll
-  for (int rr = 0; rr < 3; rr++) {
+  for (int rr = 0; rr < NROWS; rr++) {
     const int b = rs.b[rr], cnt = rs.cnt[rr];
-    unsigned long long w0 = 0ull, w1 = 0ull;
+    unsigned long long w0 = 0ull;
     for (int k0 = 0; k0 < cnt; k0 += NB_BATCH) {
       double4 q[NB_BATCH];
 #pragma unroll
@@ -1405,50 +1444,46 @@ __device__ __forceinline__ void build_mask(const RowSpan &rs, int scale_k, int i
         const int k = k0 + u;
         nib |= (in[u] && k < cnt && b + k != i) ? (1u << u) : 0u;
       }
-      const unsigned long long wn = (unsigned long long)nib;
-      if (k0 < 64) w0 |= wn << k0;
-      else w1 |= wn << (k0 - 64);
+      w0 |= (unsigned long long)nib << k0;
     }
-    m[2 * rr] = w0;
-    m[2 * rr + 1] = w1;
+    m[rr] = w0;
   }
 }
 
-// Calls f(js, ok) for batches of NB_BATCH neighbours of particle i taken
-// from its mask in slot order; unused entries of a row's last batch have
-// ok = false and js = i, so f may read them unconditionally.
-template <typename F>
-__device__ __forceinline__ void masked_batches(const RowSpan &rs, const unsigned long long (&m)[NB_WORDS], int i,
-                                               F &&f) {
+// Calls f(js, ok) for batches of NB_BATCH neighbours taken from this lane's
+// mask words in slot order; js are indices base[rr] + slot.  Unused entries
+// of a row's last batch have ok = false and js = self (a valid index), so f
+// may read them unconditionally.  (Taking rows in pairs to fill the batches
+// across row ends costs more in bit extraction than it saves.)
+template <int LANES, typename F>
+__device__ __forceinline__ void masked_batches(const int (&base)[NROWS], const unsigned long long (&m)[NB_WORDS],
+                                               int self, int t, F &&f) {
 #pragma unroll
-  for (int rr = 0; rr < 3; rr++) {
-    unsigned long long lo = m[2 * rr], hi = m[2 * rr + 1];
-    const int b = rs.b[rr];
-    while (lo | hi) {
+  for (int rr = 0; rr < NROWS; rr++) {
+    if (rr % LANES != t) continue;
+    unsigned long long cur = m[rr];
+    const int b = base[rr];
+    while (cur) {
       int js[NB_BATCH];
       bool ok[NB_BATCH];
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++) {
-        const bool inlo = lo != 0ull;
-        const unsigned long long cur = inlo ? lo : hi;
         ok[u] = cur != 0ull;
-        const int k = __ffsll((long long)cur) - 1 + (inlo ? 0 : 64);
-        const unsigned long long nxt = cur & (cur - 1ull);
-        lo = inlo ? nxt : lo;
-        hi = inlo ? hi : nxt;
-        js[u] = ok[u] ? b + k : i;
+        const int k = __ffsll((long long)cur) - 1;
+        js[u] = ok[u] ? b + k : self;
+        cur &= cur - 1ull;
       }
       f(js, ok);
     }
   }
 }
 
-// Every neighbour of i, with the exact predicate, for particles whose
-// stencil rows are too dense for masks (not batched: rare).
+// Every neighbour in this lane's rows, with the exact predicate, for
+// particles whose rows are too dense for masks (not batched: rare).
 template <typename F>
 __device__ __forceinline__ void dense_neighbours(const RowSpan &rs, int scale_k, int i, double2 pi_, double hi,
                                                  const double4 *nb, F &&f) {
-  for (int rr = 0; rr < 3; rr++) {
+  for (int rr = 0; rr < NROWS; rr++) {
     for (int k = 0; k < rs.cnt[rr]; k++) {
       const int j = rs.b[rr] + k;
       if (j == i) continue;
@@ -1460,28 +1495,38 @@ __device__ __forceinline__ void dense_neighbours(const RowSpan &rs, int scale_k,
   }
 }
 
-__device__ __forceinline__ bool load_mask(const unsigned long long *mask, int i, unsigned long long (&m)[NB_WORDS]) {
-  const ulonglong2 *mv = reinterpret_cast<const ulonglong2 *>(mask + NB_WORDS * (long)i);
+// this lane's mask words; true if the particle is marked full
+template <int LANES>
+__device__ __forceinline__ bool load_mask(const unsigned long long *mask, int i, int t,
+                                          unsigned long long (&m)[NB_WORDS]) {
   bool full = true;
 #pragma unroll
-  for (int w = 0; w < NB_WORDS / 2; w++) {
-    const ulonglong2 v = mv[w];
-    m[2 * w] = v.x;
-    m[2 * w + 1] = v.y;
-    full = full && v.x == NB_FULL && v.y == NB_FULL;
+  for (int rr = 0; rr < NROWS; rr++) {
+    m[rr] = 0ull;
+    if (rr % LANES != t) continue;
+    m[rr] = mask[NB_WORDS * (long)i + rr];
+    full = full && m[rr] == NB_FULL;
   }
   return full;
 }
 
+__device__ __forceinline__ void own_bases(const RowSpan &rs, int (&base)[NROWS]) {
+#pragma unroll
+  for (int rr = 0; rr < NROWS; rr++) base[rr] = rs.b[rr];
+}
+
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
-template <int SKF>
-__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S, double *Pout,
-                                                         unsigned long long *mask) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+template <int SKF, int LANES>
+__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
+                                                                 double *Pout, unsigned long long *mask) {
+  const int gi = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  const int i = gi / LANES, t = gi % LANES;
   if (i >= n) return;
   if (s.flag[i]) {
-    S[i] = s.h[i];
-    Pout[i] = 0.0;
+    if (t == 0) {
+      S[i] = s.h[i];
+      Pout[i] = 0.0;
+    }
     return;
   }
   const int rr_ = s.rid[i];
@@ -1489,12 +1534,14 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
   const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
-  const RowSpan rs = row_spans(g, start, pi_);
+  const RowSpan rs = row_spans<LANES>(g, start, pi_, hi, P.scale_k, t);
+  const bool full = lanes_or<LANES>(rs.full);
   double acc = 0.0;
-  ulonglong2 *mv = reinterpret_cast<ulonglong2 *>(mask + NB_WORDS * (long)i);
-  if (rs.full) {
+  unsigned long long *mrow = mask + NB_WORDS * (long)i;
+  if (full) {
 #pragma unroll
-    for (int w = 0; w < NB_WORDS / 2; w++) mv[w] = make_ulonglong2(NB_FULL, NB_FULL);
+    for (int rr = 0; rr < NROWS; rr++)
+      if (rr % LANES == t) mrow[rr] = NB_FULL;
     dense_neighbours(rs, P.scale_k, i, pi_, hi, s.nb,
                      [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
                        double w, gx, gy;
@@ -1506,8 +1553,11 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
     if (g.uniform_h) build_mask<true>(rs, P.scale_k, i, pi_, hi, s.nb, m);
     else build_mask<false>(rs, P.scale_k, i, pi_, hi, s.nb, m);
 #pragma unroll
-    for (int w = 0; w < NB_WORDS / 2; w++) mv[w] = make_ulonglong2(m[2 * w], m[2 * w + 1]);
-    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    for (int rr = 0; rr < NROWS; rr++)
+      if (rr % LANES == t) mrow[rr] = m[rr];
+    int base[NROWS];
+    own_bases(rs, base);
+    masked_batches<LANES>(base, m, i, t, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
       double4 q[NB_BATCH];
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++) q[u] = s.nb[js[u]];
@@ -1516,16 +1566,19 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
         double dx, dy, dr, mh, w, gx, gy;
         pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
         sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-        const double t = acc + q[u].w * w;
-        acc = ok[u] ? t : acc;
+        const double tt = acc + q[u].w * w;
+        acc = ok[u] ? tt : acc;
       }
     });
   }
-  const double Si = sph_self_w(SKF, P.pi, hi) * s.mass[i] + acc;
-  S[i] = Si;
-  const double rf = Si < P.h_inf ? P.h_inf : Si;
-  const double p = P.cgra05 * rf * rf;
-  Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
+  acc = lanes_sum<LANES>(acc);
+  if (t == 0) {
+    const double Si = sph_self_w(SKF, P.pi, hi) * s.mass[i] + acc;
+    S[i] = Si;
+    const double rf = Si < P.h_inf ? P.h_inf : Si;
+    const double p = P.cgra05 * rf * rf;
+    Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
+  }
 }
 
 // pressure-force contribution of neighbour j (IntForces_SW pair term)
@@ -1549,13 +1602,15 @@ __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, do
 }
 
 // a5 pressure force (IntForces_SW pair sum, 6:6892-6941) -> ain[i]
-template <int SKF, int PA>
-__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, const double *S,
-                                                        const double *Pp, double2 *ain) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+template <int SKF, int PA, int LANES>
+__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl,
+                                                                const unsigned long long *mask, SortedSet s,
+                                                                const double *S, const double *Pp, double2 *ain) {
+  const int gi = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  const int i = gi / LANES, t = gi % LANES;
   if (i >= n) return;
   if (s.flag[i]) {
-    ain[i] = make_double2(0.0, 0.0);
+    if (t == 0) ain[i] = make_double2(0.0, 0.0);
     return;
   }
   const int rr_ = s.rid[i];
@@ -1564,8 +1619,8 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   unsigned long long m[NB_WORDS];
-  const bool full = load_mask(mask, i, m);
-  const RowSpan rs = row_spans(g, start, pi_);
+  const bool full = load_mask<LANES>(mask, i, t, m);
+  const RowSpan rs = row_spans<LANES>(g, start, pi_, hi, P.scale_k, t);
   const double Si = S[i];
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
@@ -1578,7 +1633,9 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
                        pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
                      });
   } else {
-    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    int base[NROWS];
+    own_bases(rs, base);
+    masked_batches<LANES>(base, m, i, t, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
       double4 q[NB_BATCH];
       double pj[NB_BATCH], sj[NB_BATCH];
 #pragma unroll
@@ -1599,18 +1656,22 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
       }
     });
   }
-  ain[i] = make_double2(inx, iny);
+  inx = lanes_sum<LANES>(inx);
+  iny = lanes_sum<LANES>(iny);
+  if (t == 0) ain[i] = make_double2(inx, iny);
 }
 
 // a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
-template <int SKF>
-__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl, const unsigned long long *mask, SortedSet s, NewSet o,
-                                                      double2 *vout) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+template <int SKF, int LANES>
+__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl,
+                                                              const unsigned long long *mask, SortedSet s, NewSet o,
+                                                              double2 *vout) {
+  const int gi = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  const int i = gi / LANES, t = gi % LANES;
   if (i >= n) return;
   double2 vi = o.vel[i];
   if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0) {
-    vout[i] = vi;
+    if (t == 0) vout[i] = vi;
     return;
   }
   const int rr_ = s.rid[i];
@@ -1619,8 +1680,8 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   unsigned long long m[NB_WORDS];
-  const bool full = load_mask(mask, i, m);
-  const RowSpan rs = row_spans(g, start, pi_);
+  const bool full = load_mask<LANES>(mask, i, t, m);
+  const RowSpan rs = row_spans<LANES>(g, start, pi_, hi, P.scale_k, t);
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
   double ax = 0.0, ay = 0.0;
@@ -1636,7 +1697,9 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
                        ay = ay - c * (vi.y - vj.y);
                      });
   } else {
-    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    int base[NROWS];
+    own_bases(rs, base);
+    masked_batches<LANES>(base, m, i, t, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
       double4 q[NB_BATCH];
       double2 vj[NB_BATCH];
       double hj[NB_BATCH];
@@ -1660,9 +1723,13 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
       }
     });
   }
-  vi.x = vi.x + 0.3 * ax;
-  vi.y = vi.y + 0.3 * ay;
-  vout[i] = vi;
+  ax = lanes_sum<LANES>(ax);
+  ay = lanes_sum<LANES>(ay);
+  if (t == 0) {
+    vi.x = vi.x + 0.3 * ax;
+    vi.y = vi.y + 0.3 * ay;
+    vout[i] = vi;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1998,9 +2065,11 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
   const double4 qi = t.nb[i];
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
-  const RowSpan rs = row_spans(g, cl.start, pi_);
+  const RowSpan rs = row_spans<1>(g, cl.start, pi_, hi, P.scale_k, 0);
   unsigned long long m[NB_WORDS];
-  const bool full = load_mask(t.mask, i, m);
+  const bool full = load_mask<1>(t.mask, i, 0, m);
+  int base[NROWS];
+  own_bases(rs, base);
   const double2 vi = t.vp[i], li = w.lvpp[i];
   const double Si = t.S[i];
   const int acti = w.act[i];
@@ -2030,7 +2099,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
     dense_neighbours(rs, P.scale_k, i, pi_, hi, t.nb,
                      [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
   } else {
-    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    masked_batches<1>(base, m, i, 0, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
         if (ok[u]) term(js[u], t.nb[js[u]]);
@@ -2116,9 +2185,11 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
   const double4 qi = t.nb[i];
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
-  const RowSpan rs = row_spans(g, cl.start, pi_);
+  const RowSpan rs = row_spans<1>(g, cl.start, pi_, hi, P.scale_k, 0);
   unsigned long long m[NB_WORDS];
-  const bool full = load_mask(t.mask, i, m);
+  const bool full = load_mask<1>(t.mask, i, 0, m);
+  int base[NROWS];
+  own_bases(rs, base);
   const double2 lai = w.la[i];
   const double lSi = w.lS[i];
   const double grav = 2.0 * P.cgra05;
@@ -2144,7 +2215,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
     dense_neighbours(rs, P.scale_k, i, pi_, hi, t.nb,
                      [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
   } else {
-    masked_batches(rs, m, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    masked_batches<1>(base, m, i, 0, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
         if (ok[u]) term(js[u], t.nb[js[u]]);
@@ -2490,6 +2561,7 @@ struct gfsph_ctx {
   // work-group size of each gather kernel (density, force, XSPH), chosen by
   // gfsph_autotune; all sizes give identical results
   int gbs[3] = {GF_BLOCK, GF_BLOCK, GF_BLOCK};
+  int glanes = 1;  // lanes per particle of the gathers (1 or 2; GFSPH_GATHER_LANES)
   int *bad = nullptr;
   double2 *zxy = nullptr;
   double *top = nullptr;
@@ -2727,6 +2799,10 @@ static int alloc_sets(gfsph_ctx *c, int n) {
     return rc;
   c->gridN = ((nblk(n) + 7) / 8) * 8;
   if ((rc = dalloc(c, &c->part, c->gridN))) return rc;
+  {
+    const char *ev = getenv("GFSPH_GATHER_LANES");
+    if (ev && atoi(ev) == 2) c->glanes = 2;
+  }
   return GFSPH_OK;
 }
 
@@ -2744,21 +2820,33 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
   return GFSPH_OK;
 }
 
-#define SKF_DISPATCH(KERNEL, ...)              \
-  do {                                         \
-    switch (c->P.skf) {                        \
-      case 1: KERNEL<1><<<__VA_ARGS__; break;  \
-      case 2: KERNEL<2><<<__VA_ARGS__; break;  \
-      case 3: KERNEL<3><<<__VA_ARGS__; break;  \
-      default: KERNEL<4><<<__VA_ARGS__; break; \
-    }                                          \
+#define SKF_LANES(KERNEL, SKFV, ...)                                  \
+  do {                                                               \
+    if (c->glanes == 2) KERNEL<SKFV, 2><<<__VA_ARGS__;               \
+    else KERNEL<SKFV, 1><<<__VA_ARGS__;                              \
+  } while (0)
+#define SKF_DISPATCH(KERNEL, ...)                          \
+  do {                                                     \
+    switch (c->P.skf) {                                    \
+      case 1: SKF_LANES(KERNEL, 1, __VA_ARGS__); break;    \
+      case 2: SKF_LANES(KERNEL, 2, __VA_ARGS__); break;    \
+      case 3: SKF_LANES(KERNEL, 3, __VA_ARGS__); break;    \
+      default: SKF_LANES(KERNEL, 4, __VA_ARGS__); break;   \
+    }                                                      \
   } while (0)
 
-#define PFORCE_CASE(SKFV)                                                                    \
-  switch (c->P.pa_sph) {                                                                     \
-    case 1: k_pforce_m<SKFV, 1><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS); break;  \
-    case 2: k_pforce_m<SKFV, 2><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS); break;  \
-    default: k_pforce_m<SKFV, 4><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS); break; \
+#define PFORCE_PA(SKFV, PAV)                                                                          \
+  do {                                                                                                \
+    if (c->glanes == 2)                                                                               \
+      k_pforce_m<SKFV, PAV, 2><<<xcd_grid(2L * c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS);         \
+    else                                                                                              \
+      k_pforce_m<SKFV, PAV, 1><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS);              \
+  } while (0)
+#define PFORCE_CASE(SKFV)                      \
+  switch (c->P.pa_sph) {                       \
+    case 1: PFORCE_PA(SKFV, 1); break;         \
+    case 2: PFORCE_PA(SKFV, 2); break;         \
+    default: PFORCE_PA(SKFV, 4); break;        \
   }
 
 // blocks of bsz threads covering n particles, rounded up to a multiple of 8 (xcd_block)
@@ -2834,7 +2922,7 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[0];
-  SKF_DISPATCH(k_density_m, xcd_grid(c->n, bsz), bsz, 0,
+  SKF_DISPATCH(k_density_m, xcd_grid((long)c->glanes * c->n, bsz), bsz, 0,
                c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
 }
 
@@ -2842,7 +2930,7 @@ static void launch_density(gfsph_ctx *c, SortedSet &s) {
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[2];
   if (c->P.av_vel)
-    SKF_DISPATCH(k_xsph_m, xcd_grid(c->n, bsz), bsz, 0,
+    SKF_DISPATCH(k_xsph_m, xcd_grid((long)c->glanes * c->n, bsz), bsz, 0,
                  c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
