@@ -28,6 +28,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "gfsph.h"
@@ -219,13 +220,15 @@ __device__ __forceinline__ void root_nr(double dr, double &r, double &ir) {
 // spline) or 7/(4 pi) (Wendland) computed once.  Values agree with
 // kernel_TOOLS to a few ulps; only the neighbour predicate must round
 // exactly as the reference, and it is evaluated separately.
-template <int SKF>
+// UH: ihu is 1/hsml already (all smoothing lengths equal, so hsml is the
+// particle's own and its reciprocal is computed once per particle).
+template <int SKF, bool UH = false>
 __device__ __forceinline__ void sph_kernel_t(double kc, double pi, double dr, double dx, double dy, double hsml,
-                                             double &w, double &gx, double &gy) {
+                                             double &w, double &gx, double &gy, double ihu = 0.0) {
   if (SKF == 1) {
     double r, ir;
     root_nr(dr, r, ir);
-    const double ih = rcp_nr(hsml);
+    const double ih = UH ? ihu : rcp_nr(hsml);
     const double ih2 = ih * ih;
     const double q = r * ih;
     const double factor = kc * ih2;
@@ -242,7 +245,7 @@ __device__ __forceinline__ void sph_kernel_t(double kc, double pi, double dr, do
   } else if (SKF == 4) {
     double r, ir;
     root_nr(dr, r, ir);
-    const double ih = rcp_nr(hsml);
+    const double ih = UH ? ihu : rcp_nr(hsml);
     const double ih2 = ih * ih;
     const double q = r * ih;
     const double factor = kc * ih2;
@@ -1557,7 +1560,9 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
       if (rr % LANES == t) mrow[rr] = m[rr];
     int base[NROWS];
     own_bases(rs, base);
-    masked_batches<LANES>(base, m, i, t, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    const double ihi = rcp_nr(hi);
+    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      constexpr bool UH = decltype(uh)::value;
       double4 q[NB_BATCH];
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++) q[u] = s.nb[js[u]];
@@ -1565,11 +1570,19 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
       for (int u = 0; u < NB_BATCH; u++) {
         double dx, dy, dr, mh, w, gx, gy;
         pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-        sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
         const double tt = acc + q[u].w * w;
         acc = ok[u] ? tt : acc;
       }
-    });
+    };
+    if (g.uniform_h)
+      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::true_type{}, js, ok);
+      });
+    else
+      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::false_type{}, js, ok);
+      });
   }
   acc = lanes_sum<LANES>(acc);
   if (t == 0) {
@@ -1635,7 +1648,9 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
   } else {
     int base[NROWS];
     own_bases(rs, base);
-    masked_batches<LANES>(base, m, i, t, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    const double ihi = rcp_nr(hi);
+    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      constexpr bool UH = decltype(uh)::value;
       double4 q[NB_BATCH];
       double pj[NB_BATCH], sj[NB_BATCH];
 #pragma unroll
@@ -1648,13 +1663,21 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
       for (int u = 0; u < NB_BATCH; u++) {
         double dx, dy, dr, mh, w, gx, gy;
         pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-        sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
         double tx = inx, ty = iny;
         pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
         inx = ok[u] ? tx : inx;
         iny = ok[u] ? ty : iny;
       }
-    });
+    };
+    if (g.uniform_h)
+      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::true_type{}, js, ok);
+      });
+    else
+      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::false_type{}, js, ok);
+      });
   }
   inx = lanes_sum<LANES>(inx);
   iny = lanes_sum<LANES>(iny);
@@ -1699,7 +1722,9 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
   } else {
     int base[NROWS];
     own_bases(rs, base);
-    masked_batches<LANES>(base, m, i, t, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    const double ihi = rcp_nr(hi);
+    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      constexpr bool UH = decltype(uh)::value;
       double4 q[NB_BATCH];
       double2 vj[NB_BATCH];
       double hj[NB_BATCH];
@@ -1715,13 +1740,21 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
       for (int u = 0; u < NB_BATCH; u++) {
         double dx, dy, dr, mh, w, gx, gy;
         pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-        sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
         const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
         const double tx = ax - c * (vi.x - vj[u].x), ty = ay - c * (vi.y - vj[u].y);
         ax = use[u] ? tx : ax;
         ay = use[u] ? ty : ay;
       }
-    });
+    };
+    if (g.uniform_h)
+      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::true_type{}, js, ok);
+      });
+    else
+      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::false_type{}, js, ok);
+      });
   }
   ax = lanes_sum<LANES>(ax);
   ay = lanes_sum<LANES>(ay);
