@@ -1183,10 +1183,15 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_init(int n, SortedSet s, c
   b.fl[i] = 0;
 }
 
-__global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, int st, DevParams P, DevDem dem,
-                                                             SortedSet s, const double *S, BinghamState b) {
+// Pd: the device parameter block, whose dt_sph (and RK weight) the adaptive
+// time step updates between steps
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, int st, DevParams P,
+                                                             const DevParams *Pd, DevDem dem, SortedSet s,
+                                                             const double *S, BinghamState b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  P.dt = Pd->dt;
+  P.fin = Pd->fin;
   const int id = s.id[i];
   if (s.flag[i] || b.fl[i]) {
     if (phase == 0) b.cand[id] = __longlong_as_double(0x7ff8000000000000ll);
@@ -1226,10 +1231,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, in
   }
 }
 
-__global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P, DevDem dem, SortedSet s,
-                                                              const double *S, BinghamState b, NewSet o,
-                                                              BBoxAcc *part) {
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P, const DevParams *Pd, DevDem dem,
+                                                              SortedSet s, const double *S, BinghamState b,
+                                                              NewSet o, BBoxAcc *part) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
+  P.dt = Pd->dt;
+  P.fin = Pd->fin;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
   double hso = 0;
@@ -2430,6 +2437,72 @@ __global__ void k_count_active(int n, const int *flag, int *count) {
   if ((threadIdx.x & 63) == 0 && a) atomicAdd(count, (int)a);
 }
 
+// ---------------------------------------------------------------------------
+// a15 adaptive dt_sph: Adaptive_dt_sph_SW (6:6521-6664), called by
+// Driver_Flow before each time_integration_sph (8:1590).  Modes
+// (ic_adapt_dt_sph): 1 hsml/(3c), 2 hsml/(3(c+|v|)), 4 |dx_g|/|v|,
+// 10 min of 1 and 4, with c = sqrt(g h) and h clamped to h_inf in place.
+// The minimum is over the active particles of each realisation; positive
+// doubles order as their bit patterns, so the minimum is an integer
+// atomicMin and its value does not depend on the reduction order.
+// ---------------------------------------------------------------------------
+#define DT_INIT 0x4202a05f20000000ull  // bit pattern of 1.e10, the reference's dtmin start
+
+__global__ __launch_bounds__(GF_BLOCK) void k_dt_min(int n, int mode, double h_inf, double cgra, double dxg,
+                                                     const double2 *vel, double *h, const double *hsml,
+                                                     const int *flag, const int *rid, unsigned long long *dtm) {
+  __shared__ unsigned long long wmin[GF_BLOCK / 64];
+  const int b0 = blockIdx.x * blockDim.x;
+  const int i = b0 + threadIdx.x;
+  unsigned long long key = ~0ull;
+  if (i < n && !flag[i]) {
+    const double2 v = vel[i];
+    const double vv = sqrt(v.x * v.x + v.y * v.y);
+    double dm = 1.e10;
+    if (mode != 4) {
+      double hh = h[i];
+      if (hh < h_inf) {
+        hh = h_inf;
+        h[i] = hh;  // the reference keeps the clamped depth (6:6554)
+      }
+      double cc = fmax(1.e-6, sqrt(cgra * hh));
+      if (mode == 2) cc = cc + vv;
+      dm = fmin(hsml[i] / (3 * cc), dm);
+    }
+    if (mode == 4 || mode == 10) dm = fmin(dxg / vv, dm);
+    key = (unsigned long long)__double_as_longlong(dm);
+  }
+  // slots are grouped by realisation: a block whose first and last slot
+  // share one reduces in LDS and issues one atomic
+  const int rf = rid[b0], rl = rid[min(n, b0 + (int)blockDim.x) - 1];
+  if (rf != rl) {
+    if (key != ~0ull) atomicMin(&dtm[rid[i]], key);
+    return;
+  }
+  key = wave_min_u64(key);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = wmin[w] < m ? wmin[w] : m;
+    if (m != ~0ull) atomicMin(&dtm[rf], m);
+  }
+}
+
+// New dt_sph of every realisation (6:6655-6660), its RK weight, time_sph
+// (8:1617), and the minimum slots reset for the next step.
+__global__ void k_dt_apply(int R, int nstages, DevParams *Pr, unsigned long long *dtm, double *tm) {
+  for (int r = threadIdx.x; r < R; r += blockDim.x) {
+    const double dtmin = __longlong_as_double((long long)dtm[r]);
+    const double dt = Pr[r].dt;
+    const double nd = dtmin < dt ? dtmin : fmin(1.1 * dt, dtmin);
+    Pr[r].dt = nd;
+    Pr[r].fin = nstages == 4 ? nd / 6. : nd / 2.;
+    tm[r] = tm[r] + nd;
+    dtm[r] = DT_INIT;
+  }
+}
+
 // pair enumeration (i<j by caller index) for the neighbour-set export
 __global__ void k_pairs(int n, int skf, int scale_k, Cells cl, SortedSet s, long long cap,
                         unsigned long long *count, int *ij) {
@@ -2626,6 +2699,14 @@ struct gfsph_ctx {
   AdjWork aw{};
   double2 *LX = nullptr, *LV = nullptr;
   double *LM = nullptr, *LH = nullptr, *gmu = nullptr, *gxi = nullptr;
+  // time step: ic_adapt_dt_sph (0: fixed dt_sph); with adaptation dt_sph and
+  // time_sph of every realisation live on the device (Pr[r].dt, tm, dtm the
+  // per-step minima), otherwise time_sph is summed here from dt_h
+  int adapt = 0;
+  double dxg = 0.0;
+  double *tm = nullptr;
+  unsigned long long *dtm = nullptr;
+  std::vector<double> time_h, dt_h;
   std::vector<void *> allocs;
 };
 
@@ -2909,13 +2990,13 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   BinghamState &b = c->bs;
   k_bingham_init<<<nb, GF_BLOCK, 0, st>>>(n, s, c->ain, b);
   for (int k = 0; k < c->P.nstages; k++) {
-    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 0, k, c->P, c->dem, s, c->S, b);
+    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 0, k, c->P, c->Pr, c->dem, s, c->S, b);
     k_lv_reduce<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum);
     k_lv_top<<<1, 64, 0, st>>>(c->lv_blocks, b.bsum, b.carry);
     k_lv_down<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum, b.tau);
-    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->P, c->dem, s, c->S, b);
+    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->P, c->Pr, c->dem, s, c->S, b);
   }
-  k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->dem, s, c->S, b, c->nw, c->part);
+  k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->Pr, c->dem, s, c->S, b, c->nw, c->part);
 }
 
 // Neighbour build into set[dst] from (nw, set[src]).  A single run takes its
@@ -2969,8 +3050,25 @@ static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
 
+// adaptive_dt_sph before a step from set[src] (nothing with a fixed dt_sph)
+static void enqueue_dt(gfsph_ctx *c, int src) {
+  if (!c->adapt) return;
+  const SortedSet &o = c->set[src];
+  k_dt_min<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->adapt, c->P.h_inf, c->P.cgra, c->dxg, o.vel, c->nw.h,
+                                                    c->nw.hsml, c->nw.flag, o.rid, c->dtm);
+  k_dt_apply<<<1, 64, 0, c->stream>>>(c->R, c->P.nstages, c->Pr, c->dtm, c->tm);
+}
+
+// time_sph = time_sph + dt_sph after each of nsteps fixed-dt steps (8:1617)
+static void advance_time(gfsph_ctx *c, int nsteps) {
+  if (c->adapt) return;
+  for (int r = 0; r < c->R; r++)
+    for (int k = 0; k < nsteps; k++) c->time_h[r] = c->time_h[r] + c->dt_h[r];
+}
+
 // Enqueues one whole step reading set[src] + nw and producing set[dst] + nw.
 static int enqueue_step(gfsph_ctx *c, int src, int dst) {
+  enqueue_dt(c, src);
   int rc = neighbour_build(c, src, dst);
   if (rc) return rc;
   SortedSet &s = c->set[dst];
@@ -3188,6 +3286,9 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   k_bbox<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(st));
+  c->time_h.assign(R, 0.0);
+  c->dt_h.assign(R, p->dt);
+  c->dxg = pow(dem->dx * dem->dx + dem->dy * dem->dy, 0.5);  // dxg_mod (6:6566)
   return GFSPH_OK;
 }
 
@@ -3213,6 +3314,7 @@ int gfsph_step(gfsph_ctx *c, int32_t nsteps) {
   if (nsteps < 0) return fail(c, GFSPH_ERR_ARG, "nsteps < 0");
   int rc = run_steps(c, nsteps);
   if (rc) return rc;
+  advance_time(c, nsteps);
   if (c->check_finite) {
     HIPCHK(c, hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
     k_nonfinite<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, cur_view(c), c->bad);
@@ -3391,6 +3493,8 @@ int gfsph_profile_step(gfsph_ctx *c, int32_t nrep, int32_t cap, int32_t *nkernel
   for (int r = 0; r < nrep; r++) {
     hipStream_t st = c->stream;
     int src = c->cur, dst = 1 - c->cur;
+    enqueue_dt(c, src);
+    advance_time(c, 1);
     HIPCHK(c, hipEventRecord(ev[0], st));
     int rc = neighbour_build(c, src, dst);
     if (rc) return rc;
@@ -3622,6 +3726,7 @@ static int taped_step(gfsph_ctx *c, TapeRec &t) {
 static int adjoint_supported(gfsph_ctx *c) {
   const DevParams &P = c->P;
   if (c->R != 1) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs on a single realisation");
+  if (c->adapt) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs with a fixed dt_sph (ic_adapt_dt_sph = 0)");
   if (P.nfric != 5 || P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 ||
       P.xi_voellmy < 0.0)
     return fail(c, GFSPH_ERR_UNSUPPORTED,
@@ -3769,6 +3874,51 @@ int gfsph_counts(gfsph_ctx *c, int32_t *n, int32_t *nactive) {
     HIPCHK(c, hipMemcpyAsync(&na, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *nactive = (int32_t)na;
+  }
+  return GFSPH_OK;
+}
+
+int gfsph_set_adaptive_dt(gfsph_ctx *c, int32_t mode) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  if (mode != 0 && mode != 1 && mode != 2 && mode != 4 && mode != 10)
+    return fail(c, GFSPH_ERR_UNSUPPORTED,
+                "ic_adapt_dt_sph must be 0, 1, 2, 4 or 10 (3, pair-based, and 7, one-dimensional, are not provided)");
+  if (mode == c->adapt) return GFSPH_OK;
+  const int R = c->R;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc;
+  if (!c->tm && ((rc = dalloc(c, &c->tm, R)) || (rc = dalloc(c, &c->dtm, R)))) return rc;
+  if (!c->adapt) {
+    std::vector<unsigned long long> init(R, DT_INIT);
+    HIPCHK(c, hipMemcpy(c->tm, c->time_h.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->dtm, init.data(), sizeof(unsigned long long) * R, hipMemcpyHostToDevice));
+  } else if (!mode) {
+    std::vector<DevParams> ph(R);
+    HIPCHK(c, hipMemcpy(c->time_h.data(), c->tm, sizeof(double) * R, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ph.data(), c->Pr, sizeof(DevParams) * R, hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; r++) c->dt_h[r] = ph[r].dt;
+    c->P.dt = ph[0].dt;
+    c->P.fin = ph[0].fin;
+  }
+  drop_graphs(c);
+  c->adapt = mode;
+  return GFSPH_OK;
+}
+
+int gfsph_time(gfsph_ctx *c, double *time, double *dt) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  const int R = c->R;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->adapt) {
+    if (time) memcpy(time, c->time_h.data(), sizeof(double) * R);
+    if (dt) memcpy(dt, c->dt_h.data(), sizeof(double) * R);
+    return GFSPH_OK;
+  }
+  if (time) HIPCHK(c, hipMemcpy(time, c->tm, sizeof(double) * R, hipMemcpyDeviceToHost));
+  if (dt) {
+    std::vector<DevParams> ph(R);
+    HIPCHK(c, hipMemcpy(ph.data(), c->Pr, sizeof(DevParams) * R, hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; r++) dt[r] = ph[r].dt;
   }
   return GFSPH_OK;
 }

@@ -42,6 +42,7 @@ module gfsph_c
 
   public :: gfsph_create, gfsph_step, gfsph_sync, gfsph_get_state, gfsph_destroy, gfsph_last_error_f
   public :: gfsph_raster_idw, gfsph_mc_pixel_accumulate, gfsph_mc_pixel_read, gfsph_counts
+  public :: gfsph_set_adaptive_dt, gfsph_time
 
   interface
      integer(c_int) function gfsph_create(p, dem, n, x, v, h, mass, hsml, u, itype, ctx) bind(C, name="gfsph_create")
@@ -96,6 +97,18 @@ module gfsph_c
        type(c_ptr), value              :: ctx
        integer(c_int32_t), intent(out) :: n, nactive
      end function gfsph_counts
+
+     integer(c_int) function gfsph_set_adaptive_dt(ctx, mode) bind(C, name="gfsph_set_adaptive_dt")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value        :: ctx
+       integer(c_int32_t), value :: mode
+     end function gfsph_set_adaptive_dt
+
+     integer(c_int) function gfsph_time(ctx, time, dt) bind(C, name="gfsph_time")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value          :: ctx
+       real(c_double), intent(out) :: time(*), dt(*)
+     end function gfsph_time
 
      type(c_ptr) function gfsph_last_error(ctx) bind(C, name="gfsph_last_error")
        import :: c_ptr

@@ -20,6 +20,11 @@
 !  (Out_Plot_SPH, Out_Save_SPH, Manage_pixel_Hazard_SPH) see the current
 !  state.  The build must use the same real kind as the reference modules
 !  (irk = double): -fdefault-real-8.
+!
+!  With ic_adapt_dt_sph /= 0 the context performs Adaptive_dt_sph_SW
+!  ("6 SPH_SW_2019.f90":6521-6664) on the device at the start of every step
+!  and each call returns the dt_sph that step used, so the driver drops its
+!  own "call adaptive_dt_sph" (8:1590) and keeps "time_sph = time_sph + dt_sph".
 
 module SPH_MAIN_GFSPH
 
@@ -45,10 +50,16 @@ contains
 subroutine time_integration_sph_gfsph
 
    integer(c_int) rc
+   real(c_double) tnow(1), dtnow(1)
 
    if (.not. c_associated(ctx)) call create_context
    rc = gfsph_step(ctx, 1_c_int32_t)
    call check(rc, 'gfsph_step')
+   if (ic_adapt_dt_sph .ne. 0) then
+      rc = gfsph_time(ctx, tnow, dtnow)
+      call check(rc, 'gfsph_time')
+      dt_sph = dtnow(1)
+   endif
    call pull_state
 
 end subroutine time_integration_sph_gfsph
@@ -102,6 +113,10 @@ subroutine create_context
 
    rc = gfsph_create(p, d, int(npoin, c_int32_t), x, vx, rho, mass, hsml, u, ity, ctx)
    call check(rc, 'gfsph_create')
+   if (ic_adapt_dt_sph .ne. 0) then
+      rc = gfsph_set_adaptive_dt(ctx, int(ic_adapt_dt_sph, c_int32_t))
+      call check(rc, 'gfsph_set_adaptive_dt')
+   endif
    deallocate (t9, zz, inm, ity)
 
 end subroutine create_context

@@ -24,7 +24,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_raster_idw", "gfsph_mc_pixel_accumulate", "gfsph_mc_pixel_read", "gfsph_profile_step",
     "gfsph_counts", "gfsph_last_error", "gfsph_destroy", "gfsph_batch_create", "gfsph_batch_pixel_accumulate",
     "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune", "gfsph_adjoint_forward",
-    "gfsph_adjoint_backward", "gfsph_raster_misfit",
+    "gfsph_adjoint_backward", "gfsph_raster_misfit", "gfsph_set_adaptive_dt", "gfsph_time",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -89,6 +89,8 @@ def load_library(path: str | None = None):
     L.gfsph_adjoint_forward.argtypes = [vp, C.c_int32, C.c_int32]
     L.gfsph_adjoint_backward.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp]
     L.gfsph_raster_misfit.argtypes = [vp, dp, dp, dp, dp]
+    L.gfsph_set_adaptive_dt.argtypes = [vp, C.c_int32]
+    L.gfsph_time.argtypes = [vp, dp, dp]
     if path is None:
         _lib = L
     return L
@@ -148,11 +150,22 @@ class SPHModel:
         ctx = C.c_void_p()
         rc = self.lib.gfsph_create(C.byref(p), C.byref(d), self.n, *(_dp(a) for a in arrs), _ip(it), C.byref(ctx))
         self.ctx = ctx
+        self.R = 1
         if rc != 0:
             msg = self.lib.gfsph_last_error(ctx).decode() if ctx.value else ""
             self.close()
             raise GfsphError(rc, msg)
         self.nsteps_done = 0
+        self._apply_time_control(case.params)
+
+    def _apply_time_control(self, params) -> None:
+        mode = int(getattr(params, "adapt_dt", 0))
+        if mode:
+            try:
+                self.set_adaptive_dt(mode)
+            except GfsphError:
+                self.close()
+                raise
 
     def _check(self, rc):
         if rc != 0:
@@ -207,6 +220,20 @@ class SPHModel:
         self._check(self.lib.gfsph_autotune(self.ctx, int(reps), _ip(ch)))
         return tuple(int(v) for v in ch)
 
+    def set_adaptive_dt(self, mode: int) -> None:
+        """ic_adapt_dt_sph: 0 fixed dt_sph, 1, 2, 4 or 10 (Adaptive_dt_sph_SW)."""
+        self._check(self.lib.gfsph_set_adaptive_dt(self.ctx, int(mode)))
+
+    def time(self):
+        """(time_sph, dt_sph of the last step); arrays over the realisations
+        of a batch."""
+        t = np.zeros(self.R)
+        dt = np.zeros(self.R)
+        self._check(self.lib.gfsph_time(self.ctx, _dp(t), _dp(dt)))
+        if self.R == 1:
+            return float(t[0]), float(dt[0])
+        return t, dt
+
     def counts(self):
         n, na = C.c_int32(), C.c_int32()
         self._check(self.lib.gfsph_counts(self.ctx, C.byref(n), C.byref(na)))
@@ -258,6 +285,7 @@ class BatchModel(SPHModel):
             self.close()
             raise GfsphError(rc, msg)
         self.nsteps_done = 0
+        self._apply_time_control(self.case.params)
 
     def split(self, st: dict) -> list:
         edges = np.concatenate([[0], np.cumsum(self.sizes)])

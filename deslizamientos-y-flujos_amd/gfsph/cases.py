@@ -49,6 +49,7 @@ class Params:
     icpwp: int = 0
     npoin0: int = 0
     dt: float = 0.05
+    adapt_dt: int = 0             # ic_adapt_dt_sph: 0 fixed dt; 1, 2, 4, 10 (6:6521-6664)
     h_inf: float = 1.0e-3
     pwp0rel: float = 0.0
     const: dict = dataclasses.field(default_factory=dict)   # 1-based index -> value

@@ -183,6 +183,21 @@ int gfsph_profile_step(gfsph_ctx *ctx, int32_t nrep, int32_t cap, int32_t *nkern
 /* Number of particles and of active (in-domain) particles. */
 int gfsph_counts(gfsph_ctx *ctx, int32_t *n, int32_t *nactive);
 
+/* Time-step control ic_adapt_dt_sph: replaces adaptive_dt_sph /
+ * Adaptive_dt_sph_SW ("7 SPH_MAIN_2019.f90":289-306, "6 SPH_SW_2019.f90":
+ * 6521-6664), which Driver_Flow calls before each step ("8 SPH_FEM_Driver_2019.f90":
+ * 1590).  mode 0 keeps dt_sph fixed; 1: min hsml/(3c), 2: min hsml/(3(c+|v|)),
+ * 4: min |dx_DEM|/|v|, 10: min of 1 and 4 (c = sqrt(const(1) h), h clamped
+ * to h_inf in place), dt_sph growing by at most 10 % per step.  Every
+ * following step first computes the new dt_sph of each realisation on the
+ * device.  Modes 3 and 7 give GFSPH_ERR_UNSUPPORTED. */
+int gfsph_set_adaptive_dt(gfsph_ctx *ctx, int32_t mode);
+
+/* time_sph (the sum of the dt_sph of the steps taken, "8 SPH_FEM_Driver_2019.f90":
+ * 1617) and the dt_sph of the last step, one value per realisation (either
+ * pointer may be NULL).  Synchronises. */
+int gfsph_time(gfsph_ctx *ctx, double *time, double *dt);
+
 const char *gfsph_last_error(gfsph_ctx *ctx);
 void gfsph_destroy(gfsph_ctx *ctx);
 

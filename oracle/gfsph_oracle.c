@@ -21,6 +21,9 @@
  *   RK4 / RK2            - reference dumps (RK4), ODE order tests (RK2 has no
  *                          reference counterpart: pinned by order + closed form)
  *   XSPH / sle=1         - reference dumps
+ *   adaptive dt          - reference dumps of modes 2 and 10 (states and
+ *                          time_sph after 1, 10, 100 steps); modes 1 and 4
+ *                          share their terms (pinned through mode 10)
  *   rasters              - single-particle closed forms, constant-field
  *                          reproduction (no reference dump: the reference
  *                          writes them only as GiD text)
@@ -709,17 +712,51 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
 /* ------------------------------------------------------------------ */
 /* a10. RK45 (7:746-1005) and the RK2 (Heun) extension                  */
 /* ------------------------------------------------------------------ */
-int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,
+/* Adaptive_dt_sph_SW (6:6521-6664) for ic_adapt_dt_sph = 1, 2, 4, 10: the
+   smallest per-particle bound over the active particles (clamping their depth
+   to h_inf_SW first where the mode reads it), then dt_sph = dtmin if that is
+   smaller, else min(1.1 dt_sph, dtmin). */
+static double adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *v, double *rho,
+                          const double *hsml, const int32_t *flag, double dt) {
+  const int mode = p->adapt_dt;
+  const double cgra = p->cst[1];
+  const double dxg_mod = pow(dem->dx * dem->dx + dem->dy * dem->dy, 0.5);
+  double dtmin = 1.e10;
+  for (int i = 0; i < n; i++) {
+    if (flag[i]) continue;
+    double vv = pow(v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1], 0.5), cc;
+    if (mode == 1 || mode == 2 || mode == 10) {
+      if (rho[i] < p->h_inf) rho[i] = p->h_inf;
+      const double c = sqrt(cgra * rho[i]);
+      cc = fmax(1.e-6, c);
+      if (mode == 2) cc = cc + vv;
+      cc = hsml[i] / (3 * cc);
+      dtmin = fmin(cc, dtmin);
+    }
+    if (mode == 4 || mode == 10) {
+      cc = dxg_mod / vv;
+      dtmin = fmin(cc, dtmin);
+    }
+  }
+  if (dtmin < dt) return dtmin;
+  return fmin(1.1 * dt, dtmin);
+}
+
+int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double *v, double *rho,
             const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
-            int nsteps, double *tau0_io) {
+            int nsteps, double *tau0_io, double *dt_time_io) {
   derived_t D;
-  derive(p, &D);
-  const double dt = p->dt;
+  derive(p_in, &D);
+  double dt = dt_time_io ? dt_time_io[0] : p_in->dt;
+  double time_sph = dt_time_io ? dt_time_io[1] : 0.0;
+  gfo_params p_step = *p_in;
+  const gfo_params *p = &p_step; /* dt_sph of the current step */
+  p_step.dt = dt;
   const int nst = (p->integrator == 2) ? 2 : 4;
   const double f1rk4[4] = {0., 0.5, 0.5, 1.0}, f2rk4[4] = {1., 2., 2., 1.0};
   const double f1rk2[2] = {0., 1.0}, f2rk2[2] = {1., 1.};
   const double *f1 = (nst == 4) ? f1rk4 : f1rk2, *f2 = (nst == 4) ? f2rk4 : f2rk2;
-  const double fin = (nst == 4) ? dt / 6. : dt / 2.;
+  double fin = (nst == 4) ? dt / 6. : dt / 2.;
   const int nuaux = (p->icpwp == 2 || p->icpwp == 3) ? 1 : 0;
   int scale_k = D.scale_k;
 
@@ -736,6 +773,12 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
   double tau0_carry = tau0_io ? *tau0_io : 0.0;
 
   for (int step = 0; step < nsteps; step++) {
+    /* Driver_Flow: adaptive_dt_sph before time_integration_sph (8:1610-1616) */
+    if (p->adapt_dt) {
+      dt = adaptive_dt(p, dem, n, v, rho, hsml, flag, dt);
+      fin = (nst == 4) ? dt / 6. : dt / 2.;
+      p_step.dt = dt;
+    }
     /* time_integration_sph: Check_Out_Domain before RK45 (7:232) */
     for (int i = 0; i < n; i++)
       if (out_of_domain(dem, x[2 * i], x[2 * i + 1])) flag[i] = 1;
@@ -907,6 +950,7 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
     /* time_integration_sph: Check_Out_Domain after RK45 (7:268) */
     for (int i = 0; i < n; i++)
       if (out_of_domain(dem, x[2 * i], x[2 * i + 1])) flag[i] = 1;
+    time_sph = time_sph + dt; /* 8:1617 */
   }
   free(x0);
   free(v0);
@@ -926,6 +970,10 @@ int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v
   free(pv.i);
   free(pv.j);
   if (tau0_io) *tau0_io = tau0_carry;
+  if (dt_time_io) {
+    dt_time_io[0] = dt;
+    dt_time_io[1] = time_sph;
+  }
   return 0;
 }
 

@@ -13,7 +13,9 @@ typedef struct {
   int32_t icpwp;              /* 0, 1 or 3 (6:1418) */
   int32_t npoin0;             /* particles [0,npoin0) skip XSPH */
   int32_t rev_order;          /* 1: every pair sum in reverse order (conditioning probe) */
-  double dt;                  /* dt_sph */
+  int32_t adapt_dt;           /* ic_adapt_dt_sph: 0 fixed dt, or 1, 2, 4, 10 (6:6521-6664) */
+  int32_t pad2_;
+  double dt;                  /* dt_sph (initial value when adapt_dt != 0) */
   double h_inf;               /* h_inf_SW */
   double pwp0rel;             /* initial pwp ratio (6:1811-1852) */
   double cst[111];            /* const(1..110); index 0 unused; REAL(4) values */
@@ -40,7 +42,8 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
              const double *mass, const double *hsml, double *u, double *uaux, int32_t *flag);
 int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,
             const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
-            int nsteps, double *tau0_io /* Bingham Tau0 carry, in/out; NULL = 0 */);
+            int nsteps, double *tau0_io /* Bingham Tau0 carry, in/out; NULL = 0 */,
+            double *dt_time_io /* [dt_sph, time_sph] in/out; NULL: p->dt, 0 */);
 void gfo_raster_idw(const gfo_dem *d, int n, const double *x, const double *v, const double *rho,
                     const double *u, const int32_t *flag, double *out, double *wsum);
 void gfo_mc_pixel_accum(const gfo_dem *d, int n, const double *x, const double *v,

@@ -20,7 +20,7 @@ class GfoParams(C.Structure):
     _fields_ = [("integrator", C.c_int32), ("pa_sph", C.c_int32), ("skf", C.c_int32), ("sle", C.c_int32),
                 ("summation_density", C.c_int32), ("average_velocity", C.c_int32), ("nor_density", C.c_int32),
                 ("icpwp", C.c_int32), ("npoin0", C.c_int32), ("rev_order", C.c_int32),
-                ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double),
+                ("adapt_dt", C.c_int32), ("pad2_", C.c_int32), ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double),
                 ("cst", C.c_double * 111)]
 
 
@@ -53,7 +53,7 @@ def lib():
         L.gfo_fdiff_derivs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp]
         L.gfo_init.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, ip]
         L.gfo_run.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, dp, ip,
-                              C.c_int, dp]
+                              C.c_int, dp, dp]
         L.gfo_raster_idw.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, dp, dp]
         L.gfo_mc_pixel_accum.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, dp, ip]
         _lib = L
@@ -73,6 +73,7 @@ def make_params(prm) -> GfoParams:
     p.integrator, p.pa_sph, p.skf, p.sle = prm.integrator, prm.pa_sph, prm.skf, prm.sle
     p.summation_density, p.average_velocity, p.nor_density = prm.summation_density, prm.average_velocity, prm.nor_density
     p.icpwp, p.npoin0 = prm.icpwp, prm.npoin0
+    p.adapt_dt = getattr(prm, "adapt_dt", 0)
     p.dt, p.h_inf, p.pwp0rel = prm.dt, prm.h_inf, prm.pwp0rel
     c = prm.cst()
     for k in range(111):
@@ -129,7 +130,7 @@ class State:
     def copy(self):
         c = State.__new__(State)
         for k, v in self.__dict__.items():
-            setattr(c, k, v.copy())
+            setattr(c, k, None if v is None else v.copy())
         return c
 
     def __init__(self, parts, n_uaux=True):
@@ -142,6 +143,7 @@ class State:
         self.uaux = np.zeros_like(self.h)
         self.flag = np.zeros(self.h.shape[0], np.int32)
         self.tau0 = np.zeros(1)  # Bingham Tau0 carried between calls
+        self.dt_time = None      # [dt_sph, time_sph] carried between calls
 
 
 def init(case, state=None):
@@ -160,9 +162,12 @@ def run(case, st, nsteps, rev_order=False):
     case amplifies rounding (its conditioning)."""
     p = make_params(case.params)
     p.rev_order = int(rev_order)
+    if st.dt_time is None:
+        st.dt_time = np.array([case.params.dt, 0.0])
     dh = DemHandle(case.dem)
     rc = lib().gfo_run(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _dp(st.mass),
-                       _dp(st.hsml), _dp(st.u), _dp(st.uaux), _ip(st.flag), nsteps, _dp(st.tau0))
+                       _dp(st.hsml), _dp(st.u), _dp(st.uaux), _ip(st.flag), nsteps, _dp(st.tau0),
+                       _dp(st.dt_time))
     if rc != 0:
         raise RuntimeError(f"gfo_run failed: {rc}")
     return st

@@ -10,7 +10,7 @@
 !
 !  Control file harness.ctl (cwd):
 !     line 1: problem name (the .dat stem)
-!     line 2: nsteps  dt_sph
+!     line 2: nsteps  dt_sph  ic_adapt_dt_sph
 !     line 3: ndump   followed by ndump step numbers (0 = after Init_sph)
 !
 !  At each listed step it writes dump_<step>.bin (stream, native endian):
@@ -24,7 +24,8 @@
 !  It is built only by oracle/build_ref.sh and never ships with the product.
 !  Built with -DGFSPH_GPU it calls time_integration_sph_gfsph (module
 !  SPH_MAIN_GFSPH of the product's Fortran layer) instead, i.e. the
-!  reference's own inputs and Init_sph driving the GPU step.
+!  reference's own inputs and Init_sph driving the GPU step, which also
+!  performs adaptive_dt_sph on the device.
 
 program gfsph_ref_harness
 
@@ -40,13 +41,13 @@ use SPH_MAIN_GFSPH, only: time_integration_sph_gfsph, gfsph_release
 implicit none
 
 character(16)  name
-integer        nsteps, ndump, istep, k, ictl
+integer        nsteps, ndump, istep, k, ictl, iadapt
 integer, allocatable :: dump_steps(:)
 real(8)        dtin
 
 open (newunit=ictl, file='harness.ctl', status='old')
 read (ictl,'(a16)') name
-read (ictl,*) nsteps, dtin
+read (ictl,*) nsteps, dtin, iadapt
 read (ictl,*) ndump
 allocate (dump_steps(max(ndump,1)))
 if (ndump.gt.0) then
@@ -73,7 +74,7 @@ nsearch_step     = 1
 isearch_step     = 0
 dt_sph           = dtin
 initial_dt_sph   = dtin
-ic_adapt_dt_sph  = 0
+ic_adapt_dt_sph  = iadapt
 dt_gfl           = -999.
 ntcurves = 0; mptstcurves = 0
 time_print = print_step*dt; time_save = save_step*dt; time_plot = plot_step*dt
@@ -99,10 +100,12 @@ endif
 call maybe_dump (0)
 do istep = 1, nsteps
    t_print_reset = t_print_reset + dt
+#ifndef GFSPH_GPU
    call adaptive_dt_sph
+#endif
    itimestep_sph = itimestep_sph + 1
 #ifdef GFSPH_GPU
-   call time_integration_sph_gfsph
+   call time_integration_sph_gfsph      ! adapts dt_sph on the device
 #else
    call time_integration_sph
 #endif

@@ -142,7 +142,7 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
     steps = sorted(set(int(s) for s in dump_steps))
     with open(os.path.join(tmp, "harness.ctl"), "w") as f:
         f.write("case\n")
-        f.write(f"{nsteps} {_r(case.params.dt)}\n")
+        f.write(f"{nsteps} {_r(case.params.dt)} {int(getattr(case.params, 'adapt_dt', 0))}\n")
         f.write(f"{len(steps)} " + " ".join(str(s) for s in steps) + "\n")
     r = subprocess.run([exe], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
                        timeout=timeout)

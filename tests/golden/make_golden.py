@@ -52,6 +52,14 @@ def case_defs():
         case=C.Case("g_voellmy_plane", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=C.voellmy(0.2, 500.0)),
                     small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
         dumps=[1, 10, 100])
+    # adaptive dt (Adaptive_dt_sph_SW, 6:6521-6664): modes 2 (hsml/(c+v)) and 10
+    # (min(hsml/c, dxg/v)) on the same release
+    for mode in (2, 10):
+        defs[f"g_adaptive_dt{mode}"] = dict(
+            case=C.Case(f"g_adaptive_dt{mode}", C.Params(integrator=4, dt=0.05, adapt_dt=mode, h_inf=1e-3,
+                                                         const=C.voellmy(0.2, 500.0)),
+                        small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
+            dumps=[1, 10, 100])
     # frictional 28 deg + pore pressure icpwp=3 on fractal terrain (cfg2 physics)
     dem = small_fractal()
     defs["g_frictional_pwp"] = dict(
@@ -154,6 +162,7 @@ def save_case(name, d, out_dir=HERE):
         arr[f"s{s}_u"] = dd["u"]
         arr[f"s{s}_hsml"] = dd["hsml"]
         arr[f"s{s}_flag"] = dd["flag"]
+        arr[f"s{s}_time"] = np.array(dd["time"])
         if dd["uaux"] is not None:
             arr[f"s{s}_uaux"] = dd["uaux"]
         if s > 0:

@@ -45,6 +45,28 @@ def test_batch_equals_single_runs(integrator):
         np.testing.assert_array_equal(per[r]["flag"], st["flag"])
 
 
+def test_batch_adaptive_dt_equals_single_runs():
+    """Each realisation adapts its own dt_sph (ic_adapt_dt_sph = 2) exactly as
+    it would alone; the minimum over particles does not depend on order."""
+    cases = realisations(R=4, seed=3, integrator=4)
+    for c in cases:
+        c.params.adapt_dt = 2
+    nsteps = 60
+    with BatchModel(cases) as b:
+        b.time_integration_sph(nsteps)
+        per = b.split(b.state())
+        tb, dtb = b.time()
+    assert len(set(dtb.tolist())) > 1  # the realisations' time steps differ
+    for r, case in enumerate(cases):
+        with SPHModel(case) as m:
+            m.time_integration_sph(nsteps)
+            st = m.state()
+            t, dt = m.time()
+        assert (tb[r], dtb[r]) == (t, dt)
+        for k in ("x", "v", "h", "u", "hsml"):
+            np.testing.assert_array_equal(per[r][k], st[k], err_msg=f"realisation {r} field {k}")
+
+
 def test_batch_pixels_and_hazard():
     cases = realisations(R=4, seed=5)
     plots, every = 6, 15

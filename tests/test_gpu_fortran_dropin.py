@@ -13,7 +13,7 @@ from oracle import refio
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_icpwp1_sle1", "g_curvature7"]
+CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_icpwp1_sle1", "g_curvature7", "g_adaptive_dt10"]
 
 
 @pytest.mark.skipif(not os.path.exists(refio.REF_GPU_EXE),
@@ -29,3 +29,5 @@ def test_fortran_driver_with_gpu_step_matches_reference(name, tmp_path):
             err = goldens.rel_err(out[s][k], f[f"s{s}_{k}"])
             assert err <= tol, (name, s, k, err)
         assert np.array_equal(out[s]["flag"], f[f"s{s}_flag"])
+        if f"s{s}_time" in f:
+            assert out[s]["time"] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)

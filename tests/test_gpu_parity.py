@@ -94,6 +94,8 @@ def test_gpu_state_matches_reference(name):
             assert np.array_equal(st["flag"], f[f"s{s}_flag"]), f"{name} step {s}: flags"
             if f"s{s}_uaux" in f:
                 assert goldens.rel_err(st["uaux"], f[f"s{s}_uaux"]) <= tol_for(s)
+            if f"s{s}_time" in f:
+                assert m.time()[0] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)
 
 
 @pytest.mark.parametrize("name", ["g_voellmy_plane", "g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1"])
@@ -114,6 +116,61 @@ def test_gpu_rk2_matches_oracle(name):
                 spread = goldens.rel_err(getattr(alt, k), getattr(st, k))
                 err = goldens.rel_err(got[k], getattr(st, k))
                 assert err <= max(TOL_EARLY if chunk < 50 else TOL, 10 * spread), (name, chunk, k, err, spread)
+
+
+@pytest.mark.parametrize("name,mode,integrator", [("g_bingham", 1, 4), ("g_frictional_pwp", 4, 2),
+                                                   ("g_voellmy_plane", 2, 2), ("g_curvature7", 10, 4)])
+def test_gpu_adaptive_dt_matches_oracle(name, mode, integrator):
+    """Adaptive dt_sph on paths the reference fixtures do not cover (Bingham,
+    RK2, pore pressure): GPU against the oracle, including dt_sph and time_sph
+    after every chunk."""
+    g = _gpu()
+    case, _ = goldens.case_of(name)
+    case.params.adapt_dt = mode
+    case.params.integrator = integrator
+    st = oracle.init(case)
+    alt = st.copy()
+    with g.SPHModel(case) as m:
+        for chunk in (1, 9, 40):
+            m.time_integration_sph(chunk)
+            oracle.run(case, st, chunk)
+            oracle.run(case, alt, chunk, rev_order=True)
+            got = m.state()
+            for k in ("x", "v", "h", "u"):
+                spread = goldens.rel_err(getattr(alt, k), getattr(st, k))
+                err = goldens.rel_err(got[k], getattr(st, k))
+                assert err <= max(TOL_EARLY if chunk < 40 else TOL, 10 * spread), (name, chunk, k, err, spread)
+            t, dt = m.time()
+            assert dt == pytest.approx(st.dt_time[0], rel=1e-8)
+            assert t == pytest.approx(st.dt_time[1], rel=1e-8)
+
+
+def test_gpu_adaptive_dt_switch():
+    """Fixed steps, then adaptive ones, then fixed again with the last adapted
+    dt_sph: time_sph is the sum over all of them."""
+    g = _gpu()
+    case, _ = goldens.case_of("g_voellmy_plane")
+    st = oracle.init(case)
+    with g.SPHModel(case) as m:
+        m.time_integration_sph(5)
+        oracle.run(case, st, 5)
+        case.params.adapt_dt = 2
+        m.set_adaptive_dt(2)
+        m.time_integration_sph(12)
+        oracle.run(case, st, 12)
+        case.params.adapt_dt = 0
+        m.set_adaptive_dt(0)
+        case.params.dt = float(st.dt_time[0])
+        m.time_integration_sph(3)
+        oracle.run(case, st, 3)
+        t, dt = m.time()
+        assert dt == pytest.approx(st.dt_time[0], rel=1e-12)
+        assert t == pytest.approx(st.dt_time[1], rel=1e-12)
+        got = m.state()
+        for k in ("x", "v", "h"):
+            assert goldens.rel_err(got[k], getattr(st, k)) <= TOL_EARLY, k
+        with pytest.raises(g.GfsphError):
+            m.set_adaptive_dt(3)
 
 
 def test_gpu_cfg1_full_size():

@@ -46,6 +46,8 @@ def test_state_matches_reference(name):
         assert np.array_equal(st.flag, f[f"s{s}_flag"]), f"{name} step {s}: domain flags differ"
         if f"s{s}_uaux" in f:
             assert goldens.rel_err(st.uaux, f[f"s{s}_uaux"]) <= tol_for(s)
+        if f"s{s}_time" in f:  # time_sph: the sum of the (adaptive) dt_sph of the steps
+            assert st.dt_time[1] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)
 
 
 @pytest.mark.parametrize("name", goldens.ALL_CASES)
