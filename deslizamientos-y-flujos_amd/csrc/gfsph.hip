@@ -990,14 +990,12 @@ __device__ __forceinline__ void for_candidates(const GridInfo &g, const int *sta
   }
 }
 
-// Neighbour masks: the density pass records, for each particle, which
-// candidates of each of its 5 cell-row ranges pass the predicate: 64 bits per
-// row (row cy-2+k in word k).  A particle with a row range of more than 64
-// candidates is marked full (all words set) and re-tests the predicate.  The
-// force and XSPH passes then visit exactly the neighbours, in ascending slot
-// order within each row.
+// Neighbour masks: NB_WORDS 64-bit words per particle, one bit per position
+// of the particle's candidate stream (its trimmed cell-row ranges
+// concatenated), set by the density pass for the candidates that pass the
+// predicate.  A stream longer than 64*NB_WORDS is not masked: the passes
+// re-test the predicate for it.
 #define NB_WORDS NROWS
-#define NB_FULL 0xffffffffffffffffull
 
 // frozen out-of-domain particle: IntForces' h_inf floor (6:6769) and
 // Get_Pwp_SW's Uaux = 0 (6:11184) still apply
@@ -1327,65 +1325,78 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 // cell-sorted set, trimmed to the columns the support circle reaches in that
 // row (x within +-w of the particle, w^2 = T^2 - dy^2 with dy the distance to
 // the row band and T = scale_k*(h_i + h_max)/2 an upper bound of every pair's
-// support radius: candidates outside cannot be neighbours).  LANES lanes
-// share a particle: lane t takes rows t, t+LANES, ...  Loads are issued in
+// support radius: candidates outside cannot be neighbours).  The 5 ranges,
+// concatenated in row order, form the particle's candidate stream; stream
+// position p of row rr is slot p + delta[rr].  The density pass tests every
+// candidate and records the neighbours as bits of the stream; the force and
+// XSPH passes walk the set bits.  One stream per particle (rather than one
+// loop per row) keeps the lanes of a wave busy: the wave waits for its
+// longest stream, not for the longest of each row.  Loads are issued in
 // batches of NB_BATCH independent candidates before any of them is used, the
-// batch bookkeeping is branch-free, and sums run in a fixed order (rows of a
-// lane in order, slots within a row in order, then lane 0 + lane 1).
+// batch bookkeeping is branch-free, and every sum runs in stream order (rows
+// in order, slots ascending within a row).
 // ---------------------------------------------------------------------------
+#ifndef NB_BATCH
 #define NB_BATCH 4
+#endif
 #define GATHER_MAX_BLOCK 512
 
-struct RowSpan {
-  int b[NROWS], cnt[NROWS];  // first slot and candidate count of each row (0: not this lane's or empty)
-  bool full;                 // some row of this lane holds more than 64 candidates
+struct CandStream {
+  int off[NROWS];  // first stream position of each row (off[0] = 0)
+  int inc[NROWS];  // slot of stream position p: p + sum of inc[rr] over the rows with off[rr] <= p
+  int total;       // length of the stream
 };
 
-template <int LANES>
-__device__ __forceinline__ RowSpan row_spans(const GridInfo &g, const int *start, double2 p, double hi, int scale_k,
-                                             int t) {
+__device__ __forceinline__ CandStream cand_stream(const GridInfo &g, const int *start, double2 p, double hi,
+                                                  int scale_k) {
   int cx, cy;
   cell_of(g, p, cx, cy);
   const double T = scale_k * (hi + g.hmax) * 0.5 * (1.0 + 1.0e-9);
   const double T2 = T * T;
   const int cmin = cx > 1 ? cx - 2 : 0, cmax = cx < g.ncx - 2 ? cx + 2 : g.ncx - 1;
-  RowSpan rs;
-  rs.full = false;
+  CandStream cs;
+  int tot = 0, dprev = 0;
 #pragma unroll
   for (int rr = 0; rr < NROWS; rr++) {
-    rs.b[rr] = 0;
-    rs.cnt[rr] = 0;
-    if (rr % LANES != t) continue;
+    int bb = 0, cnt = 0;
     const int r = cy - 2 + rr;
-    if (r < 0 || r >= g.ncy) continue;
-    const double y0 = g.ymin + r * g.cell, y1 = y0 + g.cell;
-    const double dy = p.y < y0 ? y0 - p.y : (p.y > y1 ? p.y - y1 : 0.0);
-    const double w2 = T2 - dy * dy;
-    if (!(w2 > 0.0)) continue;
-    const double w = sqrt(w2);
-    int c0 = (int)floor((p.x - w - g.xmin) / g.cell), c1 = (int)floor((p.x + w - g.xmin) / g.cell);
-    c0 = c0 < cmin ? cmin : c0;
-    c1 = c1 > cmax ? cmax : c1;
-    if (c1 < c0) continue;
-    const int bb = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
-    rs.b[rr] = bb;
-    rs.cnt[rr] = e - bb;
-    rs.full = rs.full || (e - bb) > 64;
+    if (r >= 0 && r < g.ncy) {
+      const double y0 = g.ymin + r * g.cell, y1 = y0 + g.cell;
+      const double dy = p.y < y0 ? y0 - p.y : (p.y > y1 ? p.y - y1 : 0.0);
+      const double w2 = T2 - dy * dy;
+      if (w2 > 0.0) {
+        const double w = sqrt(w2);
+        int c0 = (int)floor((p.x - w - g.xmin) / g.cell), c1 = (int)floor((p.x + w - g.xmin) / g.cell);
+        c0 = c0 < cmin ? cmin : c0;
+        c1 = c1 > cmax ? cmax : c1;
+        if (c1 >= c0) {
+          bb = start[r * g.ncx + c0];
+          cnt = start[r * g.ncx + c1 + 1] - bb;
+        }
+      }
+    }
+    cs.off[rr] = tot;
+    cs.inc[rr] = bb - tot - dprev;  // increments of delta[rr] = bb - off[rr]
+    dprev = bb - tot;
+    tot += cnt;
   }
-  return rs;
+  cs.total = tot;
+  return cs;
 }
 
-template <int LANES>
-__device__ __forceinline__ bool lanes_or(bool v) {
-  if (LANES == 1) return v;
-  return __shfl_xor(v ? 1 : 0, 1, 64) != 0 || v;
-}
+// a stream longer than the mask holds: the passes re-test the predicate
+__device__ __forceinline__ bool stream_full(const CandStream &cs) { return cs.total > 64 * NB_WORDS; }
 
-template <int LANES>
-__device__ __forceinline__ double lanes_sum(double v) {
-  if (LANES == 1) return v;
-  const double o = __shfl_xor(v, 1, 64);
-  return (threadIdx.x & 1) ? o + v : v + o;
+// Slot of stream position p: p + delta of the last row with off[rr] <= p
+// (an empty row shares its off with the next one, so its increment is
+// cancelled by the next row's).  Summed as masked increments so that it
+// stays in registers (a select between two of the row values would be
+// turned into an indexed load).
+__device__ __forceinline__ int slot_of(const CandStream &cs, int p) {
+  int d = p + cs.inc[0];
+#pragma unroll
+  for (int rr = 1; rr < NROWS; rr++) d += (p >= cs.off[rr] ? -1 : 0) & cs.inc[rr];
+  return d;
 }
 
 // Separation and mean smoothing length of a pair, rounded as in pair_pred
@@ -1415,128 +1426,126 @@ __device__ __forceinline__ PredBand pred_band(double t) {
   return pb;
 }
 
-// Neighbour mask words of this lane's rows: bit k of word rr is slot k of
-// row rr.  UNIFORM: all smoothing lengths are equal, so the support radius
-// scale_k*(hi+hj)/2 = scale_k*hi is the same for every candidate.
-template <bool UNIFORM>
-__device__ __forceinline__ void build_mask(const RowSpan &rs, int scale_k, int i, double2 pi_, double hi,
-                                           const double4 *nb, unsigned long long (&m)[NB_WORDS]) {
-  const PredBand pu = pred_band(scale_k * hi);
+// The mask words of a particle's stream (words past the stream read as 0).
+__device__ __forceinline__ void load_words(const CandStream &cs, const unsigned long long *mrow,
+                                           unsigned long long (&w)[NB_WORDS]) {
+  const int nw = (cs.total + 63) >> 6;
 #pragma unroll
-  for (int rr = 0; rr < NROWS; rr++) {
-    const int b = rs.b[rr], cnt = rs.cnt[rr];
-    unsigned long long w0 = 0ull;
-    for (int k0 = 0; k0 < cnt; k0 += NB_BATCH) {
-      double4 q[NB_BATCH];
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) q[u] = nb[b + min(k0 + u, cnt - 1)];
-      double dr[NB_BATCH];
-      PredBand pb[NB_BATCH];
-      bool in[NB_BATCH], band[NB_BATCH];
-      bool anyband = false;
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        const double dx = __dsub_rn(pi_.x, q[u].x), dy = __dsub_rn(pi_.y, q[u].y);
-        dr[u] = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
-        pb[u] = UNIFORM ? pu : pred_band(scale_k * (__dadd_rn(hi, q[u].z) / 2.));
-        in[u] = dr[u] < pb[u].lo;
-        band[u] = !in[u] && !(dr[u] > pb[u].hi);
-        anyband = anyband || band[u];
-      }
-      if (__any(anyband)) {
-#pragma unroll
-        for (int u = 0; u < NB_BATCH; u++)
-          if (band[u]) in[u] = sqrt(dr[u]) < pb[u].t;
-      }
-      unsigned nib = 0u;
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        const int k = k0 + u;
-        nib |= (in[u] && k < cnt && b + k != i) ? (1u << u) : 0u;
-      }
-      w0 |= (unsigned long long)nib << k0;
-    }
-    m[rr] = w0;
-  }
+  for (int k = 0; k < NB_WORDS; k++) w[k] = k < nw ? mrow[k] : 0ull;
 }
 
-// Calls f(js, ok) for batches of NB_BATCH neighbours taken from this lane's
-// mask words in slot order; js are indices base[rr] + slot.  Unused entries
-// of a row's last batch have ok = false and js = self (a valid index), so f
-// may read them unconditionally.  (Taking rows in pairs to fill the batches
-// across row ends costs more in bit extraction than it saves.)
-template <int LANES, typename F>
-__device__ __forceinline__ void masked_batches(const int (&base)[NROWS], const unsigned long long (&m)[NB_WORDS],
-                                               int self, int t, F &&f) {
-#pragma unroll
-  for (int rr = 0; rr < NROWS; rr++) {
-    if (rr % LANES != t) continue;
-    unsigned long long cur = m[rr];
-    const int b = base[rr];
-    while (cur) {
-      int js[NB_BATCH];
-      bool ok[NB_BATCH];
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        ok[u] = cur != 0ull;
-        const int k = __ffsll((long long)cur) - 1;
-        js[u] = ok[u] ? b + k : self;
-        cur &= cur - 1ull;
-      }
-      f(js, ok);
-    }
-  }
-}
-
-// Every neighbour in this lane's rows, with the exact predicate, for
-// particles whose rows are too dense for masks (not batched: rare).
+// Calls f(js, ok) for batches of NB_BATCH neighbours taken from the mask
+// words w of a particle in stream order; js are slots.  Unused entries of a
+// batch have ok = false and js = self (a valid index), so f may read them
+// unconditionally.  Emptied words are shifted out at batch boundaries.
 template <typename F>
-__device__ __forceinline__ void dense_neighbours(const RowSpan &rs, int scale_k, int i, double2 pi_, double hi,
-                                                 const double4 *nb, F &&f) {
-  for (int rr = 0; rr < NROWS; rr++) {
-    for (int k = 0; k < rs.cnt[rr]; k++) {
-      const int j = rs.b[rr] + k;
-      if (j == i) continue;
-      const double4 q = nb[j];
-      double dx, dy, d, mh;
-      if (pair_pred(scale_k, pi_, make_double2(q.x, q.y), hi, q.z, dx, dy, d, mh))
-        f(j, q, dx, dy, __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), mh);
+__device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned long long (&w)[NB_WORDS], int self,
+                                               F &&f) {
+  int pbase = 0;
+  for (;;) {
+    unsigned long long rest = 0ull;
+#pragma unroll
+    for (int k = 1; k < NB_WORDS; k++) rest |= w[k];
+    if (!(w[0] | rest)) break;
+    if (!w[0]) {
+#pragma unroll
+      for (int k = 0; k + 1 < NB_WORDS; k++) w[k] = w[k + 1];
+      w[NB_WORDS - 1] = 0ull;
+      pbase += 64;
+    }
+    int js[NB_BATCH];
+    bool ok[NB_BATCH];
+#pragma unroll
+    for (int u = 0; u < NB_BATCH; u++) {
+      ok[u] = w[0] != 0ull;
+      const int sl = slot_of(cs, pbase + __ffsll((long long)w[0]) - 1);
+      js[u] = ok[u] ? sl : self;
+      w[0] &= w[0] - 1ull;
+    }
+    f(js, ok);
+  }
+}
+
+// Every neighbour in the stream, with the exact predicate, for particles
+// whose streams are too long for the mask (not batched: rare).
+template <typename F>
+__device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, int i, double2 pi_, double hi,
+                                             const double4 *nb, F &&f) {
+  for (int p = 0; p < cs.total; p++) {
+    const int j = slot_of(cs, p);
+    if (j == i) continue;
+    const double4 q = nb[j];
+    double dx, dy, d, mh;
+    if (pair_pred(scale_k, pi_, make_double2(q.x, q.y), hi, q.z, dx, dy, d, mh))
+      f(j, q, dx, dy, __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), mh);
+  }
+}
+
+// Density sum over the stream with the neighbour-mask build: every candidate
+// takes the predicate and the kernel (selected into the sum when it is a
+// neighbour); mask word k holds stream positions 64k .. 64k+63.  UNIFORM: all
+// smoothing lengths are equal, so the support radius scale_k*(hi+hj)/2 =
+// scale_k*hi and the kernel's 1/h are the particle's own.
+template <int SKF, bool UNIFORM>
+__device__ __forceinline__ double density_stream(const CandStream &cs, const DevParams &P, int i, double2 pi_,
+                                                 double hi, const double4 *nb, unsigned long long *mrow) {
+  const PredBand pu = pred_band(P.scale_k * hi);
+  const double ihi = rcp_nr(hi);
+  double acc = 0.0;
+  unsigned long long word = 0ull;
+  for (int k0 = 0; k0 < cs.total; k0 += NB_BATCH) {
+    int j[NB_BATCH];
+    double4 q[NB_BATCH];
+#pragma unroll
+    for (int u = 0; u < NB_BATCH; u++) {
+      j[u] = slot_of(cs, min(k0 + u, cs.total - 1));
+      q[u] = nb[j[u]];
+    }
+    double dx[NB_BATCH], dy[NB_BATCH], dr[NB_BATCH], mh[NB_BATCH];
+    PredBand pb[NB_BATCH];
+    bool in[NB_BATCH], band[NB_BATCH];
+    bool anyband = false;
+#pragma unroll
+    for (int u = 0; u < NB_BATCH; u++) {
+      pair_geom(pi_, hi, q[u], dx[u], dy[u], dr[u], mh[u]);
+      pb[u] = UNIFORM ? pu : pred_band(P.scale_k * mh[u]);
+      in[u] = dr[u] < pb[u].lo;
+      band[u] = !in[u] && !(dr[u] > pb[u].hi);
+      anyband = anyband || band[u];
+    }
+    if (__any(anyband)) {
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++)
+        if (band[u]) in[u] = sqrt(dr[u]) < pb[u].t;
+    }
+    unsigned nib = 0u;
+#pragma unroll
+    for (int u = 0; u < NB_BATCH; u++) {
+      const bool nbr = in[u] && k0 + u < cs.total && j[u] != i;
+      nib |= nbr ? (1u << u) : 0u;
+      double w, gx, gy;
+      sph_kernel_t<SKF, UNIFORM>(P.kc, P.pi, dr[u], dx[u], dy[u], mh[u], w, gx, gy, ihi);
+      const double t = acc + q[u].w * w;
+      acc = nbr ? t : acc;
+    }
+    word |= (unsigned long long)nib << (k0 & 63);
+    if (((k0 + NB_BATCH) & 63) == 0 || k0 + NB_BATCH >= cs.total) {
+      mrow[k0 >> 6] = word;
+      word = 0ull;
     }
   }
-}
-
-// this lane's mask words; true if the particle is marked full
-template <int LANES>
-__device__ __forceinline__ bool load_mask(const unsigned long long *mask, int i, int t,
-                                          unsigned long long (&m)[NB_WORDS]) {
-  bool full = true;
-#pragma unroll
-  for (int rr = 0; rr < NROWS; rr++) {
-    m[rr] = 0ull;
-    if (rr % LANES != t) continue;
-    m[rr] = mask[NB_WORDS * (long)i + rr];
-    full = full && m[rr] == NB_FULL;
-  }
-  return full;
-}
-
-__device__ __forceinline__ void own_bases(const RowSpan &rs, int (&base)[NROWS]) {
-#pragma unroll
-  for (int rr = 0; rr < NROWS; rr++) base[rr] = rs.b[rr];
+  return acc;
 }
 
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
-template <int SKF, int LANES>
+template <int SKF>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
                                                                  double *Pout, unsigned long long *mask) {
-  const int gi = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  const int i = gi / LANES, t = gi % LANES;
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
   if (s.flag[i]) {
-    if (t == 0) {
-      S[i] = s.h[i];
-      Pout[i] = 0.0;
-    }
+    S[i] = s.h[i];
+    Pout[i] = 0.0;
     return;
   }
   const int rr_ = s.rid[i];
@@ -1544,61 +1553,24 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
   const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
-  const RowSpan rs = row_spans<LANES>(g, start, pi_, hi, P.scale_k, t);
-  const bool full = lanes_or<LANES>(rs.full);
+  const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
   double acc = 0.0;
-  unsigned long long *mrow = mask + NB_WORDS * (long)i;
-  if (full) {
-#pragma unroll
-    for (int rr = 0; rr < NROWS; rr++)
-      if (rr % LANES == t) mrow[rr] = NB_FULL;
-    dense_neighbours(rs, P.scale_k, i, pi_, hi, s.nb,
-                     [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                       double w, gx, gy;
-                       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                       acc = acc + q.w * w;
-                     });
+  if (stream_full(cs)) {
+    stream_dense(cs, P.scale_k, i, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+      double w, gx, gy;
+      sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+      acc = acc + q.w * w;
+    });
   } else {
-    unsigned long long m[NB_WORDS];
-    if (g.uniform_h) build_mask<true>(rs, P.scale_k, i, pi_, hi, s.nb, m);
-    else build_mask<false>(rs, P.scale_k, i, pi_, hi, s.nb, m);
-#pragma unroll
-    for (int rr = 0; rr < NROWS; rr++)
-      if (rr % LANES == t) mrow[rr] = m[rr];
-    int base[NROWS];
-    own_bases(rs, base);
-    const double ihi = rcp_nr(hi);
-    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-      constexpr bool UH = decltype(uh)::value;
-      double4 q[NB_BATCH];
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) q[u] = s.nb[js[u]];
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        double dx, dy, dr, mh, w, gx, gy;
-        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
-        const double tt = acc + q[u].w * w;
-        acc = ok[u] ? tt : acc;
-      }
-    };
-    if (g.uniform_h)
-      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-        body(std::true_type{}, js, ok);
-      });
-    else
-      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-        body(std::false_type{}, js, ok);
-      });
+    unsigned long long *mrow = mask + NB_WORDS * (long)i;
+    acc = g.uniform_h ? density_stream<SKF, true>(cs, P, i, pi_, hi, s.nb, mrow)
+                      : density_stream<SKF, false>(cs, P, i, pi_, hi, s.nb, mrow);
   }
-  acc = lanes_sum<LANES>(acc);
-  if (t == 0) {
-    const double Si = sph_self_w(SKF, P.pi, hi) * s.mass[i] + acc;
-    S[i] = Si;
-    const double rf = Si < P.h_inf ? P.h_inf : Si;
-    const double p = P.cgra05 * rf * rf;
-    Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
-  }
+  const double Si = sph_self_w(SKF, P.pi, hi) * s.mass[i] + acc;
+  S[i] = Si;
+  const double rf = Si < P.h_inf ? P.h_inf : Si;
+  const double p = P.cgra05 * rf * rf;
+  Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
 }
 
 // pressure-force contribution of neighbour j (IntForces_SW pair term)
@@ -1622,15 +1594,14 @@ __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, do
 }
 
 // a5 pressure force (IntForces_SW pair sum, 6:6892-6941) -> ain[i]
-template <int SKF, int PA, int LANES>
+template <int SKF, int PA>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl,
                                                                 const unsigned long long *mask, SortedSet s,
                                                                 const double *S, const double *Pp, double2 *ain) {
-  const int gi = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  const int i = gi / LANES, t = gi % LANES;
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
   if (s.flag[i]) {
-    if (t == 0) ain[i] = make_double2(0.0, 0.0);
+    ain[i] = make_double2(0.0, 0.0);
     return;
   }
   const int rr_ = s.rid[i];
@@ -1638,23 +1609,20 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
   const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
-  unsigned long long m[NB_WORDS];
-  const bool full = load_mask<LANES>(mask, i, t, m);
-  const RowSpan rs = row_spans<LANES>(g, start, pi_, hi, P.scale_k, t);
+  const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
   const double Si = S[i];
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
   double inx = 0.0, iny = 0.0;
-  if (full) {
-    dense_neighbours(rs, P.scale_k, i, pi_, hi, s.nb,
-                     [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                       double w, gx, gy;
-                       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                       pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
-                     });
+  if (stream_full(cs)) {
+    stream_dense(cs, P.scale_k, i, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+      double w, gx, gy;
+      sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+      pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
+    });
   } else {
-    int base[NROWS];
-    own_bases(rs, base);
+    unsigned long long w[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w);
     const double ihi = rcp_nr(hi);
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
@@ -1678,30 +1646,27 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
       }
     };
     if (g.uniform_h)
-      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::true_type{}, js, ok);
       });
     else
-      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::false_type{}, js, ok);
       });
   }
-  inx = lanes_sum<LANES>(inx);
-  iny = lanes_sum<LANES>(iny);
-  if (t == 0) ain[i] = make_double2(inx, iny);
+  ain[i] = make_double2(inx, iny);
 }
 
 // a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
-template <int SKF, int LANES>
+template <int SKF>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl,
                                                               const unsigned long long *mask, SortedSet s, NewSet o,
                                                               double2 *vout) {
-  const int gi = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  const int i = gi / LANES, t = gi % LANES;
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
   double2 vi = o.vel[i];
   if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0) {
-    if (t == 0) vout[i] = vi;
+    vout[i] = vi;
     return;
   }
   const int rr_ = s.rid[i];
@@ -1709,26 +1674,23 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
   const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
-  unsigned long long m[NB_WORDS];
-  const bool full = load_mask<LANES>(mask, i, t, m);
-  const RowSpan rs = row_spans<LANES>(g, start, pi_, hi, P.scale_k, t);
+  const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
   double ax = 0.0, ay = 0.0;
-  if (full) {
-    dense_neighbours(rs, P.scale_k, i, pi_, hi, s.nb,
-                     [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                       if (skip0 && s.id[j] < P.npoin0) return;
-                       double w, gx, gy;
-                       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                       const double2 vj = o.vel[j];
-                       const double c = (2 * q.w) * w / (rhoi + o.h[j]);
-                       ax = ax - c * (vi.x - vj.x);
-                       ay = ay - c * (vi.y - vj.y);
-                     });
+  if (stream_full(cs)) {
+    stream_dense(cs, P.scale_k, i, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+      if (skip0 && s.id[j] < P.npoin0) return;
+      double w, gx, gy;
+      sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+      const double2 vj = o.vel[j];
+      const double c = (2 * q.w) * w / (rhoi + o.h[j]);
+      ax = ax - c * (vi.x - vj.x);
+      ay = ay - c * (vi.y - vj.y);
+    });
   } else {
-    int base[NROWS];
-    own_bases(rs, base);
+    unsigned long long w[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w);
     const double ihi = rcp_nr(hi);
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
@@ -1755,21 +1717,17 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
       }
     };
     if (g.uniform_h)
-      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::true_type{}, js, ok);
       });
     else
-      masked_batches<LANES>(base, m, i, t, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::false_type{}, js, ok);
       });
   }
-  ax = lanes_sum<LANES>(ax);
-  ay = lanes_sum<LANES>(ay);
-  if (t == 0) {
-    vi.x = vi.x + 0.3 * ax;
-    vi.y = vi.y + 0.3 * ay;
-    vout[i] = vi;
-  }
+  vi.x = vi.x + 0.3 * ax;
+  vi.y = vi.y + 0.3 * ay;
+  vout[i] = vi;
 }
 
 // ---------------------------------------------------------------------------
@@ -2105,11 +2063,9 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
   const double4 qi = t.nb[i];
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
-  const RowSpan rs = row_spans<1>(g, cl.start, pi_, hi, P.scale_k, 0);
-  unsigned long long m[NB_WORDS];
-  const bool full = load_mask<1>(t.mask, i, 0, m);
-  int base[NROWS];
-  own_bases(rs, base);
+  const CandStream cs = cand_stream(g, cl.start, pi_, hi, P.scale_k);
+  const bool full = stream_full(cs);
+  const unsigned long long *mrow = t.mask + NB_WORDS * (long)i;
   const double2 vi = t.vp[i], li = w.lvpp[i];
   const double Si = t.S[i];
   const int acti = w.act[i];
@@ -2136,10 +2092,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
     gya += f * gy;
   };
   if (full) {
-    dense_neighbours(rs, P.scale_k, i, pi_, hi, t.nb,
-                     [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
+    stream_dense(cs, P.scale_k, i, pi_, hi, t.nb,
+                 [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
   } else {
-    masked_batches<1>(base, m, i, 0, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    unsigned long long mw[NB_WORDS];
+    load_words(cs, mrow, mw);
+    masked_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
         if (ok[u]) term(js[u], t.nb[js[u]]);
@@ -2225,11 +2183,9 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
   const double4 qi = t.nb[i];
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
-  const RowSpan rs = row_spans<1>(g, cl.start, pi_, hi, P.scale_k, 0);
-  unsigned long long m[NB_WORDS];
-  const bool full = load_mask<1>(t.mask, i, 0, m);
-  int base[NROWS];
-  own_bases(rs, base);
+  const CandStream cs = cand_stream(g, cl.start, pi_, hi, P.scale_k);
+  const bool full = stream_full(cs);
+  const unsigned long long *mrow = t.mask + NB_WORDS * (long)i;
   const double2 lai = w.la[i];
   const double lSi = w.lS[i];
   const double grav = 2.0 * P.cgra05;
@@ -2252,10 +2208,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
     lmi += lSj * wv;
   };
   if (full) {
-    dense_neighbours(rs, P.scale_k, i, pi_, hi, t.nb,
-                     [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
+    stream_dense(cs, P.scale_k, i, pi_, hi, t.nb,
+                 [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
   } else {
-    masked_batches<1>(base, m, i, 0, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    unsigned long long mw[NB_WORDS];
+    load_words(cs, mrow, mw);
+    masked_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
         if (ok[u]) term(js[u], t.nb[js[u]]);
@@ -2667,7 +2625,6 @@ struct gfsph_ctx {
   // work-group size of each gather kernel (density, force, XSPH), chosen by
   // gfsph_autotune; all sizes give identical results
   int gbs[3] = {GF_BLOCK, GF_BLOCK, GF_BLOCK};
-  int glanes = 1;  // lanes per particle of the gathers (1 or 2; GFSPH_GATHER_LANES)
   int *bad = nullptr;
   double2 *zxy = nullptr;
   double *top = nullptr;
@@ -2913,10 +2870,6 @@ static int alloc_sets(gfsph_ctx *c, int n) {
     return rc;
   c->gridN = ((nblk(n) + 7) / 8) * 8;
   if ((rc = dalloc(c, &c->part, c->gridN))) return rc;
-  {
-    const char *ev = getenv("GFSPH_GATHER_LANES");
-    if (ev && atoi(ev) == 2) c->glanes = 2;
-  }
   return GFSPH_OK;
 }
 
@@ -2934,28 +2887,18 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
   return GFSPH_OK;
 }
 
-#define SKF_LANES(KERNEL, SKFV, ...)                                  \
-  do {                                                               \
-    if (c->glanes == 2) KERNEL<SKFV, 2><<<__VA_ARGS__;               \
-    else KERNEL<SKFV, 1><<<__VA_ARGS__;                              \
-  } while (0)
-#define SKF_DISPATCH(KERNEL, ...)                          \
-  do {                                                     \
-    switch (c->P.skf) {                                    \
-      case 1: SKF_LANES(KERNEL, 1, __VA_ARGS__); break;    \
-      case 2: SKF_LANES(KERNEL, 2, __VA_ARGS__); break;    \
-      case 3: SKF_LANES(KERNEL, 3, __VA_ARGS__); break;    \
-      default: SKF_LANES(KERNEL, 4, __VA_ARGS__); break;   \
-    }                                                      \
+#define SKF_DISPATCH(KERNEL, ...)                   \
+  do {                                              \
+    switch (c->P.skf) {                             \
+      case 1: KERNEL<1><<<__VA_ARGS__; break;       \
+      case 2: KERNEL<2><<<__VA_ARGS__; break;       \
+      case 3: KERNEL<3><<<__VA_ARGS__; break;       \
+      default: KERNEL<4><<<__VA_ARGS__; break;      \
+    }                                               \
   } while (0)
 
-#define PFORCE_PA(SKFV, PAV)                                                                          \
-  do {                                                                                                \
-    if (c->glanes == 2)                                                                               \
-      k_pforce_m<SKFV, PAV, 2><<<xcd_grid(2L * c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS);         \
-    else                                                                                              \
-      k_pforce_m<SKFV, PAV, 1><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS);              \
-  } while (0)
+#define PFORCE_PA(SKFV, PAV) \
+  k_pforce_m<SKFV, PAV><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS)
 #define PFORCE_CASE(SKFV)                      \
   switch (c->P.pa_sph) {                       \
     case 1: PFORCE_PA(SKFV, 1); break;         \
@@ -3036,7 +2979,7 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[0];
-  SKF_DISPATCH(k_density_m, xcd_grid((long)c->glanes * c->n, bsz), bsz, 0,
+  SKF_DISPATCH(k_density_m, xcd_grid(c->n, bsz), bsz, 0,
                c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
 }
 
@@ -3044,7 +2987,7 @@ static void launch_density(gfsph_ctx *c, SortedSet &s) {
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[2];
   if (c->P.av_vel)
-    SKF_DISPATCH(k_xsph_m, xcd_grid((long)c->glanes * c->n, bsz), bsz, 0,
+    SKF_DISPATCH(k_xsph_m, xcd_grid(c->n, bsz), bsz, 0,
                  c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
