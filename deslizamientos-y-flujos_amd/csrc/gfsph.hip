@@ -3231,7 +3231,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   HIPCHK(c, hipStreamSynchronize(st));
   c->time_h.assign(R, 0.0);
   c->dt_h.assign(R, p->dt);
-  c->dxg = pow(dem->dx * dem->dx + dem->dy * dem->dy, 0.5);  // dxg_mod (6:6566)
+  c->dxg = sqrt(dem->dx * dem->dx + dem->dy * dem->dy);  // dxg_mod (6:6566); flang lowers **0.5 to sqrt
   return GFSPH_OK;
 }
 
@@ -3845,6 +3845,14 @@ int gfsph_set_adaptive_dt(gfsph_ctx *c, int32_t mode) {
   }
   drop_graphs(c);
   c->adapt = mode;
+  return GFSPH_OK;
+}
+
+int gfsph_set_time(gfsph_ctx *c, double time) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->time_h.assign(c->R, time);
+  if (c->adapt) HIPCHK(c, hipMemcpy(c->tm, c->time_h.data(), sizeof(double) * c->R, hipMemcpyHostToDevice));
   return GFSPH_OK;
 }
 

@@ -1,0 +1,1237 @@
+// gfsph_inputs.hip -- the reference's input files read on the host and the
+// terrain and release built on the GPU.
+//
+//   <name>.MASTER.dat  Driver_Input run control   ("8 SPH_FEM_Driver_2019.f90":229-470)
+//   <name>.dat         Input_SW switches, const(:) ("6 SPH_SW_2019.f90":812-1570)
+//   <topo>.top         Topo_main terrain           ("2 SPH_FEM_topo_2019.f90":208-620)
+//   <name>.pts         Read_Source_plus_New release (6:4999-5272), or
+//   restart.dat        Get_Restart_Info_SW particles (6:3268-3340)
+//
+// The files are read record by record with Fortran list-directed rules
+// (values may continue over records, the rest of the last record is
+// dropped, '/' ends a read early, r*v repeats).  Scattered terrain points are
+// gridded by Get_DTM's inverse-distance weights (2:1155-1449); nodal slopes
+// and curvatures come from Get_Topol's lumped-mass finite-element projection
+// on the two-triangle-per-cell mesh (2:583-609, 2:1658-1939); release points
+// are gridded by Read_Source_plus_New.  Every accumulation that the
+// reference performs as a scatter in input order is a gather here that
+// visits its contributions in that same order, so sums round identically.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "gfsph.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Fortran list-directed reading
+// ---------------------------------------------------------------------------
+class FReader {
+ public:
+  bool open(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string l;
+    while (std::getline(f, l)) {
+      if (!l.empty() && l.back() == '\r') l.pop_back();
+      lines_.push_back(l);
+    }
+    path_ = path;
+    return true;
+  }
+  // (a60) record
+  bool text(std::string *out = nullptr) {
+    if (next_ >= lines_.size()) return fail("end of file");
+    if (out) *out = lines_[next_];
+    next_++;
+    return true;
+  }
+  // list-directed read of n items as strings (fewer if '/' ends the read)
+  bool items(size_t n, std::vector<std::string> &tok) {
+    tok.clear();
+    bool ended = false;
+    while (tok.size() < n && !ended) {
+      if (next_ >= lines_.size()) return fail("end of file");
+      const std::string &l = lines_[next_++];
+      size_t k = 0;
+      while (k < l.size() && tok.size() < n) {
+        while (k < l.size() && (l[k] == ' ' || l[k] == '\t' || l[k] == ',')) k++;
+        if (k >= l.size()) break;
+        if (l[k] == '/') {
+          ended = true;
+          break;
+        }
+        size_t e = k;
+        if (l[k] == '\'' || l[k] == '"') {
+          const char q = l[k];
+          e = l.find(q, k + 1);
+          if (e == std::string::npos) e = l.size();
+          tok.push_back(l.substr(k + 1, e - k - 1));
+          k = e + 1;
+          continue;
+        }
+        while (e < l.size() && l[e] != ' ' && l[e] != '\t' && l[e] != ',' && l[e] != '/') e++;
+        std::string t = l.substr(k, e - k);
+        const size_t star = t.find('*');
+        if (star != std::string::npos && star > 0 &&
+            std::all_of(t.begin(), t.begin() + star, [](char ch) { return ch >= '0' && ch <= '9'; })) {
+          const long r = atol(t.substr(0, star).c_str());
+          const std::string v = t.substr(star + 1);
+          for (long q = 0; q < r && tok.size() < n; q++) tok.push_back(v);
+        } else {
+          tok.push_back(t);
+        }
+        k = e;
+      }
+    }
+    return true;
+  }
+  bool ints(std::vector<long> &v, size_t n) {
+    std::vector<std::string> t;
+    if (!items(n, t)) return false;
+    v.assign(n, 0);
+    for (size_t k = 0; k < t.size(); k++) {
+      char *end = nullptr;
+      v[k] = strtol(t[k].c_str(), &end, 10);
+      if (end == t[k].c_str() || *end) return fail("integer expected, found '" + t[k] + "'");
+    }
+    return true;
+  }
+  bool reals(std::vector<double> &v, size_t n) {
+    std::vector<std::string> t;
+    if (!items(n, t)) return false;
+    v.assign(n, 0.0);
+    for (size_t k = 0; k < t.size(); k++)
+      if (!real_of(t[k], v[k])) return fail("number expected, found '" + t[k] + "'");
+    return true;
+  }
+  // REAL(4) items: each decimal string rounded once to single precision, as
+  // a list-directed read into a REAL(4) variable does
+  bool reals4(std::vector<double> &v, size_t n) {
+    std::vector<std::string> t;
+    if (!items(n, t)) return false;
+    v.assign(n, 0.0);
+    for (size_t k = 0; k < t.size(); k++) {
+      std::string s = fix_exp(t[k]);
+      char *end = nullptr;
+      const float f = strtof(s.c_str(), &end);
+      if (end == s.c_str() || *end) return fail("number expected, found '" + t[k] + "'");
+      v[k] = (double)f;
+    }
+    return true;
+  }
+  bool logicals(std::vector<int> &v, size_t n) {
+    std::vector<std::string> t;
+    if (!items(n, t)) return false;
+    v.assign(n, 0);
+    for (size_t k = 0; k < t.size(); k++) {
+      std::string s = t[k];
+      size_t a = s[0] == '.' ? 1 : 0;
+      if (a >= s.size()) return fail("logical expected");
+      const char ch = s[a];
+      if (ch == 'T' || ch == 't') v[k] = 1;
+      else if (ch == 'F' || ch == 'f') v[k] = 0;
+      else return fail("logical expected, found '" + s + "'");
+    }
+    return true;
+  }
+  static bool real_of(const std::string &tk, double &out) {
+    std::string s = fix_exp(tk);
+    char *end = nullptr;
+    out = strtod(s.c_str(), &end);
+    return end != s.c_str() && !*end;
+  }
+  const std::string &error() const { return err_; }
+  size_t record() const { return next_; }
+
+ private:
+  static std::string fix_exp(std::string s) {
+    for (char &ch : s)
+      if (ch == 'd' || ch == 'D' || ch == 'q' || ch == 'Q') ch = 'e';
+    return s;
+  }
+  bool fail(const std::string &m) {
+    err_ = path_ + " record " + std::to_string(next_) + ": " + m;
+    return false;
+  }
+  std::vector<std::string> lines_;
+  size_t next_ = 0;
+  std::string path_, err_;
+};
+
+std::string trim(const std::string &s) {
+  size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+// ---------------------------------------------------------------------------
+// GPU: ordered window gathers
+// ---------------------------------------------------------------------------
+#define INP_BLOCK 256
+#define MAX_WINDOW 64
+
+// Point indices binned by grid cell: points of bin b are pts[start[b] ..
+// start[b+1]), in input order.  Bins are (nbx x nby), row-major.
+struct Bins {
+  const int *start, *pts;
+  int nbx, nby;
+};
+
+// Calls f(p) for every point of the bins [bx0, bx1] x [by0, by1] (clamped
+// to the bin grid) in increasing point index: a merge of the bins' sorted
+// lists.
+template <typename F>
+__device__ void ordered_window(const Bins &B, int bx0, int bx1, int by0, int by1, F &&f) {
+  int cur[MAX_WINDOW], end[MAX_WINDOW];
+  int nb = 0;
+  bx0 = max(bx0, 0);
+  by0 = max(by0, 0);
+  bx1 = min(bx1, B.nbx - 1);
+  by1 = min(by1, B.nby - 1);
+  for (int by = by0; by <= by1; by++)
+    for (int bx = bx0; bx <= bx1; bx++) {
+      const int b = by * B.nbx + bx;
+      if (nb < MAX_WINDOW) {
+        cur[nb] = B.start[b];
+        end[nb] = B.start[b + 1];
+        nb++;
+      }
+    }
+  for (;;) {
+    int best = -1, bp = 0x7fffffff;
+    for (int k = 0; k < nb; k++)
+      if (cur[k] < end[k]) {
+        const int p = B.pts[cur[k]];
+        if (p < bp) {
+          bp = p;
+          best = k;
+        }
+      }
+    if (best < 0) break;
+    f(bp);
+    cur[best]++;
+  }
+}
+
+// Get_DTM (2:1363-1404): node (ix, iy) of the output grid gathers every DTM
+// point whose cell (ipoigx, ipoigy) lies in [ix-2, ix+1] x [iy-2, iy+1]
+// (1-based; the reference scatters each point to [ipoigx-1, ipoigx+2] x
+// [ipoigy-1, ipoigy+2]) with weight 1 / (max(1e-6, d) / distg).
+__global__ void k_dtm_idw(int nx, int ny, double xmin, double ymin, double dxg, double dyg, double distg,
+                          const double *px, const double *py, const double *pz, Bins B, double zmaxg, double *Z,
+                          int *in_model) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nx * ny) return;
+  const int ix = node % nx, iy = node / nx;  // 0-based: ipoigx - 1, ipoigy - 1
+  const double xg = xmin + (double)ix * dxg, yg = ymin + (double)iy * dyg;
+  double a1 = 0.0, a2 = 0.0;
+  // bins are 1-based cell indices ipoigx stored at bx = ipoigx - 1
+  ordered_window(B, ix - 2, ix + 1, iy - 2, iy + 1, [&](int p) {
+    double dx2 = px[p] - xg;
+    dx2 = dx2 * dx2;
+    double dy2 = py[p] - yg;
+    dy2 = dy2 * dy2;
+    double d = fmax(1.e-6, sqrt(dx2 + dy2));
+    d = d / distg;
+    a1 = a1 + pz[p] / d;
+    a2 = a2 + 1.00 / d;
+  });
+  if (a2 >= 1.e-6) {
+    Z[node] = a1 / a2;
+    in_model[node] = 1;
+  } else {
+    Z[node] = zmaxg;
+    in_model[node] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GPU: Get_Topol (2:1723-1939) on the mesh of 2:583-609.  Cell c (0-based,
+// row-major over (nx-1) x (ny-1)) with lower-left node p holds elements 2c
+// (p, p+1, p+1+nx) and 2c+1 (p, p+1+nx, p+nx).  A node belongs to at most
+// six elements; in increasing element order they are the first and second
+// of the cell below-left, the second of the cell below, the first of the
+// cell to the left and both of its own cell, which is the order in which
+// the reference's element loops add into it.
+// ---------------------------------------------------------------------------
+struct Mesh {
+  int nx, ny;
+  double xmin, ymin, dx, dy;
+};
+
+__device__ __forceinline__ void elem_nodes(const Mesh &M, int e, int (&n)[3]) {
+  const int c = e >> 1, cx = c % (M.nx - 1), cy = c / (M.nx - 1);
+  const int p = cy * M.nx + cx;
+  n[0] = p;
+  if ((e & 1) == 0) {
+    n[1] = p + 1;
+    n[2] = p + 1 + M.nx;
+  } else {
+    n[1] = p + 1 + M.nx;
+    n[2] = p + M.nx;
+  }
+}
+
+// getgeom2D (2:1658-1720): shape-function derivatives g[0..2] (x), g[3..5]
+// (y) and the Jacobian g[6] of element e
+__device__ __forceinline__ void elem_geom(const Mesh &M, int e, const int (&n)[3], double (&g)[7]) {
+  double x[3], y[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    x[k] = M.xmin + (double)(n[k] % M.nx) * M.dx;
+    y[k] = M.ymin + (double)(n[k] / M.nx) * M.dy;
+  }
+  const double x21 = x[1] - x[0], x31 = x[2] - x[0], y21 = y[1] - y[0], y31 = y[2] - y[0];
+  const double rj = x21 * y31 - x31 * y21;
+  const double rj1 = 1. / rj;
+  const double xix = y31 * rj1, xiy = -(x31 * rj1), etx = -(y21 * rj1), ety = x21 * rj1;
+  const double nxi[3] = {-1.0, 1.0, 0.0}, net[3] = {-1.0, 0.0, 1.0};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    g[k] = xix * nxi[k] + etx * net[k];
+    g[k + 3] = xiy * nxi[k] + ety * net[k];
+  }
+  g[6] = rj;
+}
+
+// elements of node `node` in increasing order; returns their count and the
+// node's local position (0..2) in each
+__device__ __forceinline__ int node_elems(const Mesh &M, int node, int (&es)[6], int (&loc)[6]) {
+  const int ix = node % M.nx, iy = node / M.nx;
+  const int ncx = M.nx - 1;
+  int k = 0;
+  if (ix > 0 && iy > 0) {  // cell below-left: node is p+1+nx
+    const int c = (iy - 1) * ncx + (ix - 1);
+    es[k] = 2 * c;
+    loc[k++] = 2;
+    es[k] = 2 * c + 1;
+    loc[k++] = 1;
+  }
+  if (ix < ncx && iy > 0) {  // cell below: node is p+nx
+    const int c = (iy - 1) * ncx + ix;
+    es[k] = 2 * c + 1;
+    loc[k++] = 2;
+  }
+  if (ix > 0 && iy < M.ny - 1) {  // cell to the left: node is p+1
+    const int c = iy * ncx + (ix - 1);
+    es[k] = 2 * c;
+    loc[k++] = 1;
+  }
+  if (ix < ncx && iy < M.ny - 1) {  // own cell: node is p
+    const int c = iy * ncx + ix;
+    es[k] = 2 * c;
+    loc[k++] = 0;
+    es[k] = 2 * c + 1;
+    loc[k++] = 0;
+  }
+  return k;
+}
+
+// A1 lumped mass (2:1767-1772)
+__global__ void k_topo_mass(Mesh M, double *mml) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= M.nx * M.ny) return;
+  int es[6], loc[6];
+  const int ne = node_elems(M, node, es, loc);
+  double m = 0.0;
+  for (int k = 0; k < ne; k++) {
+    int n[3];
+    double g[7];
+    elem_nodes(M, es[k], n);
+    elem_geom(M, es[k], n, g);
+    m = m + g[6] / 6.0;
+  }
+  mml[node] = m;
+}
+
+// element derivatives of nodal field f (stride fs, offset fo): sum over the
+// element's nodes of f * g[k] and f * g[k+3], accumulated from 0 as the
+// reference's loops do
+__device__ __forceinline__ void elem_grad(const double *f, int fs, int fo, const int (&n)[3], const double (&g)[7],
+                                          double &dx, double &dy) {
+  dx = 0.0;
+  dy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double v = f[(long)n[k] * fs + fo];
+    dx = dx + v * g[k];
+    dy = dy + v * g[k + 3];
+  }
+}
+
+// Right-hand sides of the projections.  stage 0: element gradient of Z
+// (A2-A3, 2:1776-1800) -> rhs[2] (x, y); stage 1: element second derivatives
+// of (Zx, Zy) (2:1841-1869) -> rhs[3] (xx, (xy+yx)/2, yy).  T is [npoig][4]
+// (Z, Zx, Zy, spare) for stage 0 input and the nodal slopes for stage 1.
+__global__ void k_topo_rhs(Mesh M, int stage, const double *T, double *rhs) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= M.nx * M.ny) return;
+  int es[6], loc[6];
+  const int ne = node_elems(M, node, es, loc);
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  for (int k = 0; k < ne; k++) {
+    int n[3];
+    double g[7];
+    elem_nodes(M, es[k], n);
+    elem_geom(M, es[k], n, g);
+    const double coe = g[6] / 6.;
+    if (stage == 0) {
+      double dx, dy;
+      elem_grad(T, 4, 0, n, g, dx, dy);
+      r0 = r0 + dx * coe;
+      r1 = r1 + dy * coe;
+    } else {
+      double xx, xy, yx, yy;
+      elem_grad(T, 4, 1, n, g, xx, xy);
+      elem_grad(T, 4, 2, n, g, yx, yy);
+      const double t5 = (xy + yx) / 2.;
+      r0 = r0 + xx * coe;
+      r1 = r1 + t5 * coe;
+      r2 = r2 + yy * coe;
+    }
+  }
+  rhs[3 * (long)node + 0] = r0;
+  rhs[3 * (long)node + 1] = r1;
+  rhs[3 * (long)node + 2] = r2;
+}
+
+// First estimate v = rhs / m_l (A4 / B4) into v[npoig][3]
+__global__ void k_topo_first(int npoig, int nc, const double *rhs, const double *mml, double *v) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= npoig) return;
+  const double cm = mml[node];
+  for (int d = 0; d < nc; d++) v[3 * (long)node + d] = rhs[3 * (long)node + d] / cm;
+}
+
+// One Jacobi iteration v' = v + (rhs - M v) / m_l (A5 / B5, 2:1806-1834,
+// 2:1880-1904) with the consistent mass (coe*(1 + [i == j])) gathered per
+// node over its elements and, within an element, over its nodes in order.
+__global__ void k_topo_iter(Mesh M, int nc, const double *rhs, const double *mml, const double *v, double *vo) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= M.nx * M.ny) return;
+  int es[6], loc[6];
+  const int ne = node_elems(M, node, es, loc);
+  double h[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < ne; k++) {
+    int n[3];
+    double g[7];
+    elem_nodes(M, es[k], n);
+    elem_geom(M, es[k], n, g);
+    const double coe = g[6] / 24.;
+    for (int j = 0; j < 3; j++) {
+      const double cm = (j == loc[k]) ? 2.0 : 1.0;
+      for (int d = 0; d < nc; d++) h[d] = h[d] + cm * coe * v[3 * (long)n[j] + d];
+    }
+  }
+  const double cm = mml[node];
+  for (int d = 0; d < nc; d++) vo[3 * (long)node + d] = v[3 * (long)node + d] + (rhs[3 * (long)node + d] - h[d]) / cm;
+}
+
+// Final topol(1:9) (2:1912-1930) into top9[npoig][9]; slopes in s[][3]
+// (Zx, Zy, -), curvatures in c[][3]
+__global__ void k_topo_final(int npoig, const double *Z, const double *s, const double *c, double *top9,
+                             double *T4) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= npoig) return;
+  const double zx = s[3 * (long)node], zy = s[3 * (long)node + 1];
+  const double zxx = c[3 * (long)node], zxy = c[3 * (long)node + 1], zyy = c[3 * (long)node + 2];
+  const double rt = sqrt(1. + zx * zx + zy * zy);
+  double *t = top9 + 9 * (long)node;
+  t[0] = Z[node];
+  t[1] = zx;
+  t[2] = zy;
+  t[3] = 1. + zx * zx;
+  t[4] = zx * zy;
+  t[5] = 1. + zy * zy;
+  t[6] = zxx / rt;
+  t[7] = zxy / rt;
+  t[8] = zyy / rt;
+  (void)T4;
+}
+
+// Z and the slopes packed for the second-derivative stage: T[][4] = (Z, Zx, Zy, 0)
+__global__ void k_topo_pack(int npoig, const double *Z, const double *s, double *T) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= npoig) return;
+  T[4 * (long)node + 0] = Z[node];
+  T[4 * (long)node + 1] = s ? s[3 * (long)node] : 0.0;
+  T[4 * (long)node + 2] = s ? s[3 * (long)node + 1] : 0.0;
+  T[4 * (long)node + 3] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// GPU: Read_Source_plus_New gridding (6:5166-5212).  Node (inx, iny) gathers
+// the release points whose cell (ipoi2Dx, ipoi2Dy) satisfies
+// ipoi2Dx - nd_x + 1 <= inx <= ipoi2Dx + nd_x + 1 and
+// ipoi2Dy - nd_y <= iny <= ipoi2Dy + nd_y.
+// ---------------------------------------------------------------------------
+__global__ void k_src_idw(int nx, int ny, double xmin, double ymin, double dxg, double dyg, double dist2D, int ndx,
+                          int ndy, const double *px, const double *py, const double *ph, Bins B, double *aux) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nx * ny) return;
+  const int ix = node % nx, iy = node / nx;  // 0-based inx - 1, iny - 1
+  const double xx0 = xmin + (double)ix * dxg, yy0 = ymin + (double)iy * dyg;
+  double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  ordered_window(B, ix - ndx - 1, ix + ndx - 1, iy - ndy, iy + ndy, [&](int p) {
+    double dx2 = px[p] - xx0;
+    dx2 = dx2 * dx2;
+    double dy2 = py[p] - yy0;
+    dy2 = dy2 * dy2;
+    double d = fmax(1.e-6, sqrt(dx2 + dy2));
+    d = d / dist2D;
+    a1 = a1 + 1;
+    a2 = a2 + 1.0 / d;
+    a3 = a3 + ph[p] / d;
+  });
+  aux[4 * (long)node + 0] = a1;
+  aux[4 * (long)node + 1] = a2;
+  aux[4 * (long)node + 2] = a3;
+}
+
+// aux2D(4): number of the node's quads whose four corners carry weight
+// (6:5195-5212)
+__global__ void k_src_area(int nx, int ny, double *aux) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nx * ny) return;
+  const int ix = node % nx, iy = node / nx;
+  int cnt = 0;
+  for (int qy = iy - 1; qy <= iy; qy++)
+    for (int qx = ix - 1; qx <= ix; qx++) {
+      if (qx < 0 || qy < 0 || qx >= nx - 1 || qy >= ny - 1) continue;
+      const int n1 = qy * nx + qx, n2 = n1 + 1, n3 = n1 + nx, n4 = n3 + 1;
+      if (aux[4 * (long)n1 + 1] >= 1.e-6 && aux[4 * (long)n2 + 1] >= 1.e-6 && aux[4 * (long)n3 + 1] >= 1.e-6 &&
+          aux[4 * (long)n4 + 1] >= 1.e-6)
+        cnt++;
+    }
+  aux[4 * (long)node + 3] = (double)cnt;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the inputs object
+// ---------------------------------------------------------------------------
+struct TopoRaw {
+  int ictop = 0, fdiff = 0;
+  std::vector<double> px, py, pz;  // DTM points (ictop 10-16) or grid nodes (ictop 0)
+  double deltx = 0;
+  double xw0 = 0, xw1 = 0, yw0 = 0, yw1 = 0;
+  double zmax = 0;
+  std::vector<int> start, pts;  // points binned by cell (ipoigx, ipoigy), in input order
+  int nbx = 0, nby = 0;
+};
+
+struct SrcRaw {  // Read_Source_plus_New after reading: shifted points and the release grid
+  std::vector<double> sx, sy, sh;
+  double xmin = 0, ymin = 0, d2x = 0, d2y = 0, dist2D = 0, delt2D = 0, facthsml = 0;
+  int nx = 0, ny = 0, ndx = 0, ndy = 0;
+  std::vector<int> start, pts;
+  int nbx = 0, nby = 0;
+};
+
+struct gfsph_inputs {
+  std::string err;
+  gfsph_params P{};
+  gfsph_run run{};
+  gfsph_grid grid{};
+  TopoRaw topo;
+  SrcRaw src;
+  bool built = false;
+  std::vector<double> Z, topol;  // [npoig], [npoig][9] (empty when slopes are finite differences)
+  std::vector<int32_t> in_model;
+  int n = 0;
+  std::vector<double> x, v, h, mass, hsml, u;
+  std::vector<int32_t> itype;
+  int icunk = 0;
+};
+
+namespace {
+
+int ifail(gfsph_inputs *in, int code, const std::string &m) {
+  in->err = m;
+  return code;
+}
+
+#define RD(call)                                 \
+  do {                                           \
+    if (!(call)) return ifail(in, GFSPH_ERR_ARG, R.error()); \
+  } while (0)
+
+// Driver_Input subset (8:229-340): SPH only, no GFL, no case tables
+int read_master(gfsph_inputs *in, const std::string &path) {
+  FReader R;
+  if (!R.open(path)) return GFSPH_OK;  // optional
+  std::vector<long> iv;
+  std::vector<double> rv;
+  RD(R.ints(iv, 1));
+  for (long k = 0; k < iv[0]; k++) RD(R.text());
+  RD(R.text());
+  RD(R.ints(iv, 3));
+  const long if_sph = iv[0], if_gfl = iv[1], cases = iv[2];
+  if (if_sph != 1 || if_gfl != 0 || cases != 0)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": only if_sph = 1, if_gfl = 0, ic_cases_mat = 0 runs are read");
+  RD(R.text());
+  RD(R.ints(iv, 2));
+  if (iv[0] != 1) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": SPH_ProblemType must be 1 (shallow water)");
+  in->run.t_integ_alg = (int32_t)iv[1];
+  RD(R.text());
+  RD(R.text());  // problem name
+  RD(R.text());
+  RD(R.reals(rv, 3));
+  in->run.dt = rv[0];
+  in->run.time_end = rv[1];
+  in->run.maxtimestep = (int32_t)rv[2];
+  RD(R.text());
+  RD(R.ints(iv, 3));
+  in->run.print_step = (int32_t)std::labs(iv[0]);
+  in->run.save_step = (int32_t)iv[1];
+  in->run.plot_step = (int32_t)iv[2];
+  if (iv[0] < 0) {
+    RD(R.text());
+    RD(R.ints(iv, 1));
+  }
+  RD(R.text());
+  std::vector<std::string> t;
+  RD(R.items(2, t));
+  if (t.size() < 2 || !FReader::real_of(t[0], in->run.dt_sph))
+    return ifail(in, GFSPH_ERR_ARG, path + ": dt_sph, ic_adapt_dt_sph expected");
+  in->run.adapt_dt = atoi(t[1].c_str());
+  in->run.present = 1;
+  return GFSPH_OK;
+}
+
+// Input_SW (6:812-1570) for a single-phase soil release
+int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
+  FReader R;
+  if (!R.open(path)) return ifail(in, GFSPH_ERR_ARG, "cannot open " + path);
+  std::vector<long> iv;
+  std::vector<double> rv;
+  std::vector<int> lv;
+  gfsph_params &P = in->P;
+  RD(R.ints(iv, 1));
+  for (long k = 0; k < iv[0]; k++) RD(R.text());
+  RD(R.text());
+  RD(R.ints(iv, 1));
+  if (iv[0] != 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": ic_SWAlg must be 0");
+  RD(R.text());
+  RD(R.ints(iv, 1));
+  if (iv[0] != 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": histograms (n_hist > 0) are not read");
+  RD(R.text());
+  RD(R.ints(iv, 1));
+  if (iv[0] != 2) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": ndimn must be 2");
+  RD(R.text());
+  RD(R.ints(iv, 4));
+  if (iv[0] != 1 || iv[1] != 0 || iv[2] != 0 || iv[3] != 0)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": only a soil release (1 0 0 0) is read");
+  // Get_soil_points (6:3207-3260)
+  RD(R.text());
+  std::vector<std::string> t;
+  RD(R.items(2, t));
+  if (t.size() < 2) return ifail(in, GFSPH_ERR_ARG, path + ": icunk_s, h_inf_SW expected");
+  in->icunk = atoi(t[0].c_str());
+  if (!FReader::real_of(t[1], P.h_inf)) return ifail(in, GFSPH_ERR_ARG, path + ": h_inf_SW");
+  if (in->icunk != 7 && in->icunk != 200)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": release type icunk_s " + std::to_string(in->icunk) +
+                                                " (7: gridded .pts and 200: restart.dat are read)");
+  RD(R.text());
+  std::string nm;
+  RD(R.text(&nm));
+  src_name = trim(nm);
+  // switches (6:1101-1110)
+  RD(R.text());
+  RD(R.ints(iv, 4));
+  P.pa_sph = (int32_t)iv[0];
+  P.nnps = (int32_t)iv[1];
+  P.sle = (int32_t)iv[2];
+  P.skf = (int32_t)iv[3];
+  RD(R.text());
+  RD(R.logicals(lv, 4));
+  P.summation_density = lv[0];
+  P.average_velocity = lv[1];
+  if (lv[2]) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": virtual particles are not supported");
+  P.nor_density = lv[3];
+  // const(:) (6:1130-1150), REAL(4)
+  RD(R.text());
+  RD(R.reals4(rv, 15));
+  for (int k = 1; k <= 15; k++) P.cst[k] = rv[k - 1];
+  if (P.cst[4] > 1.e-4) {
+    P.cst[21] = 1;
+    P.cst[22] = P.cst[4];
+  }
+  if (P.cst[11] < 0) {
+    RD(R.text());
+    RD(R.reals4(rv, 15));
+    for (int k = 16; k <= 30; k++) P.cst[k] = rv[k - 16];
+  }
+  if (P.cst[30] < 0) {
+    RD(R.text());
+    RD(R.reals4(rv, 15));
+    for (int k = 31; k <= 45; k++) P.cst[k] = rv[k - 31];
+  }
+  if (P.cst[45] < 0) {
+    RD(R.text());
+    RD(R.reals4(rv, 15));
+    for (int k = 46; k <= 60; k++) P.cst[k] = rv[k - 46];
+  }
+  const int nfric = (int)(P.cst[5] + 0.01);
+  const bool frictional = (nfric >= 5 && nfric <= 10) || nfric == 26 || nfric == 27;
+  if (nfric == 201 || nfric == 202 || nfric == 203)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": nfric 201-203 (water inflow) is not supported");
+  if (P.cst[1] < 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": inclined gravity (const(1) < 0)");
+  if (frictional) {
+    RD(R.text());
+    RD(R.ints(iv, 1));
+    if (iv[0] != 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": K0 (ic_K0 = 1) is not supported");
+  }
+  RD(R.text());
+  RD(R.ints(iv, 1));
+  P.icpwp = (int32_t)iv[0];
+  if (P.icpwp == 1 || P.icpwp == 3) {  // Initialize_Pwp (6:1722-1729)
+    RD(R.text());
+    RD(R.reals(rv, 1));
+    P.pwp0rel = rv[0];
+  } else if (P.icpwp != 0) {
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": icpwp must be 0, 1 or 3");
+  }
+  P.integrator = 4;
+  P.init_input = 1;
+  return GFSPH_OK;
+}
+
+// Topo_main (2:208-252, 2:330-460) and Get_DTM's reading (2:1181-1227)
+int read_top(gfsph_inputs *in, const std::string &path, TopoRaw &T) {
+  FReader R;
+  if (!R.open(path)) return ifail(in, GFSPH_ERR_ARG, "cannot open " + path);
+  std::vector<long> iv;
+  std::vector<double> rv;
+  RD(R.text());
+  RD(R.ints(iv, 1));
+  T.ictop = (int)iv[0];
+  if (T.ictop >= 100) {
+    T.fdiff = 1;
+    T.ictop -= 100;
+  }
+  if (T.ictop == 0) {
+    RD(R.text());
+    RD(R.ints(iv, 2));
+    in->grid.nx = (int32_t)iv[0];
+    in->grid.ny = (int32_t)iv[1];
+    RD(R.text());
+    const long np = (long)in->grid.nx * in->grid.ny;
+    if (np <= 0) return ifail(in, GFSPH_ERR_ARG, path + ": npoigx, npoigy");
+    T.px.assign(np, 0.0);
+    T.py.assign(np, 0.0);
+    T.pz.assign(np, 0.0);
+    for (long k = 0; k < np; k++) {
+      RD(R.reals(rv, 4));
+      const long ip = (long)rv[0] - 1;
+      if (ip < 0 || ip >= np) return ifail(in, GFSPH_ERR_ARG, path + ": node index out of range");
+      T.px[ip] = rv[1];
+      T.py[ip] = rv[2];
+      T.pz[ip] = rv[3];
+    }
+  } else if (T.ictop == 10 || T.ictop == 11 || T.ictop == 15 || T.ictop == 16) {
+    RD(R.text());
+    RD(R.reals(rv, 2));
+    const long np = (long)rv[0];
+    T.deltx = rv[1];
+    if (np <= 0 || !(T.deltx > 0)) return ifail(in, GFSPH_ERR_ARG, path + ": np_dtm, deltx");
+    if (T.ictop == 15 || T.ictop == 16) {
+      RD(R.text());
+      RD(R.reals(rv, 4));
+      T.xw0 = rv[0];
+      T.xw1 = rv[1];
+      T.yw0 = rv[2];
+      T.yw1 = rv[3];
+    }
+    RD(R.text());
+    T.px.resize(np);
+    T.py.resize(np);
+    T.pz.resize(np);
+    for (long k = 0; k < np; k++) {
+      RD(R.reals(rv, 3));
+      T.px[k] = rv[0];
+      T.py[k] = rv[1];
+      T.pz[k] = rv[2];
+    }
+  } else {
+    return ifail(in, GFSPH_ERR_UNSUPPORTED,
+                 path + ": ictop " + std::to_string(T.ictop) + " (0, 10, 11, 15, 16 and +100 are read)");
+  }
+  RD(R.text());
+  RD(R.ints(iv, 1));
+  if (iv[0] != 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": terrain properties (ic_topo_props != 0)");
+  return GFSPH_OK;
+}
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipMalloc(&p, sizeof(T) * (n ? n : 1)); }
+  hipError_t up(const T *h, size_t n) { return hipMemcpy(p, h, sizeof(T) * n, hipMemcpyHostToDevice); }
+  hipError_t down(T *h, size_t n) const { return hipMemcpy(h, p, sizeof(T) * n, hipMemcpyDeviceToHost); }
+};
+
+#define HCHK(call)                                                                                   \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess) return ifail(in, GFSPH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline int blocks(long n) { return (int)((n + INP_BLOCK - 1) / INP_BLOCK); }
+
+// CSR of point indices per cell, in point order (cells outside are skipped)
+void bin_points(const std::vector<int> &cell, int ncell, std::vector<int> &start, std::vector<int> &pts) {
+  start.assign(ncell + 1, 0);
+  for (int c : cell)
+    if (c >= 0) start[c + 1]++;
+  for (int c = 0; c < ncell; c++) start[c + 1] += start[c];
+  pts.assign(start[ncell], 0);
+  std::vector<int> fill(start.begin(), start.end() - 1);
+  for (size_t p = 0; p < cell.size(); p++)
+    if (cell[p] >= 0) pts[fill[cell[p]]++] = (int)p;
+}
+
+// Get_DTM's grid (2:1227-1284) and the points' cells (2:1367-1373) for
+// ictop 10, 11, 15, 16; ictop 0 takes the grid as read (2:224-252).  Host.
+int grid_geometry(gfsph_inputs *in) {
+  TopoRaw &T = in->topo;
+  gfsph_grid &G = in->grid;
+  G.fdiff = T.fdiff;
+  if (T.ictop == 0) {
+    double xmin = 1.e9, ymin = 1.e9, xmax = -1.e9, ymax = -1.e9;
+    for (size_t k = 0; k < T.px.size(); k++) {
+      if (T.px[k] <= xmin) xmin = T.px[k];
+      if (T.px[k] >= xmax) xmax = T.px[k];
+      if (T.py[k] <= ymin) ymin = T.py[k];
+      if (T.py[k] >= ymax) ymax = T.py[k];
+    }
+    if (G.nx < 2 || G.ny < 2) return ifail(in, GFSPH_ERR_ARG, "terrain grid has fewer than 2 x 2 nodes");
+    G.xmin = xmin;
+    G.ymin = ymin;
+    G.xmax = xmax;
+    G.ymax = ymax;
+    G.dx = (xmax - xmin) / (G.nx - 1);
+    G.dy = (ymax - ymin) / (G.ny - 1);
+    return GFSPH_OK;
+  }
+  const long np = (long)T.px.size();
+  double xmin = 1.e9, ymin = 1.e9, xmax = -1.e9, ymax = -1.e9, zmax = -1.e9;
+  for (long k = 0; k < np; k++) {
+    if (T.px[k] <= xmin) xmin = T.px[k];
+    if (T.px[k] >= xmax) xmax = T.px[k];
+    if (T.py[k] <= ymin) ymin = T.py[k];
+    if (T.py[k] >= ymax) ymax = T.py[k];
+    if (T.pz[k] >= zmax) zmax = T.pz[k];
+  }
+  T.zmax = zmax;
+  int nx = (int)((xmax - xmin) / T.deltx + 1), ny = (int)((ymax - ymin) / T.deltx + 1);
+  double dxg = (xmax - xmin) / (nx - 1), dyg = (ymax - ymin) / (ny - 1);
+  double xs = 0.0, ys = 0.0;
+  if (T.ictop == 11) {
+    xs = xmin;
+    ys = ymin;
+    for (long k = 0; k < np; k++) {
+      T.px[k] = T.px[k] - xmin;
+      T.py[k] = T.py[k] - ymin;
+    }
+    xmax = xmax - xmin;
+    ymax = ymax - ymin;
+    xmin = 0.0;
+    ymin = 0.0;
+  } else if (T.ictop == 15 || T.ictop == 16) {
+    xs = xmin;
+    ys = ymin;
+    if (T.ictop == 15) {
+      xmin = T.xw0 - xs;
+      xmax = T.xw1 - xs;
+      ymin = T.yw0 - ys;
+      ymax = T.yw1 - ys;
+    } else {
+      xmin = T.xw0;
+      xmax = T.xw1;
+      ymin = T.yw0;
+      ymax = T.yw1;
+    }
+    nx = (int)((xmax - xmin) / T.deltx + 1);
+    ny = (int)((ymax - ymin) / T.deltx + 1);
+    dxg = (xmax - xmin) / (nx - 1);
+    dyg = (ymax - ymin) / (ny - 1);
+    for (long k = 0; k < np; k++) {
+      T.px[k] = T.px[k] - xs;
+      T.py[k] = T.py[k] - ys;
+    }
+  }
+  if (nx < 2 || ny < 2) return ifail(in, GFSPH_ERR_ARG, "terrain grid has fewer than 2 x 2 nodes");
+  G.nx = nx;
+  G.ny = ny;
+  G.xmin = xmin;
+  G.ymin = ymin;
+  G.xmax = xmax;
+  G.ymax = ymax;
+  G.dx = dxg;
+  G.dy = dyg;
+  G.xshift = xs;
+  G.yshift = ys;
+  T.nbx = nx + 1;  // ipoigx runs 1 .. nx (nx + 1 if the division rounds up at the edge)
+  T.nby = ny + 1;
+  std::vector<int> cell(np, -1);
+  for (long k = 0; k < np; k++) {
+    const double xx = T.px[k], yy = T.py[k];
+    const bool discr = (xx >= xmin) && (xx <= xmax) && (yy >= ymin) && (yy <= ymax);
+    if (!discr) continue;
+    const int ipx = (int)((xx - xmin) / dxg + 1), ipy = (int)((yy - ymin) / dyg + 1);
+    const int bx = std::min(std::max(ipx - 1, 0), T.nbx - 1), by = std::min(std::max(ipy - 1, 0), T.nby - 1);
+    cell[k] = by * T.nbx + bx;
+  }
+  bin_points(cell, T.nbx * T.nby, T.start, T.pts);
+  return GFSPH_OK;
+}
+
+// Get_DTM's gridding (2:1360-1404) on the device
+int grid_build(gfsph_inputs *in) {
+  TopoRaw &T = in->topo;
+  const gfsph_grid &G = in->grid;
+  const long npoig = (long)G.nx * G.ny;
+  if (T.ictop == 0) {
+    in->Z = T.pz;
+    in->in_model.assign(npoig, 1);
+    return GFSPH_OK;
+  }
+  const long np = (long)T.px.size();
+  DevBuf<double> dx_, dy_, dz_, dZ;
+  DevBuf<int> dstart, dpts, dmod;
+  HCHK(dx_.alloc(np));
+  HCHK(dy_.alloc(np));
+  HCHK(dz_.alloc(np));
+  HCHK(dstart.alloc(T.start.size()));
+  HCHK(dpts.alloc(T.pts.size()));
+  HCHK(dZ.alloc(npoig));
+  HCHK(dmod.alloc(npoig));
+  HCHK(dx_.up(T.px.data(), np));
+  HCHK(dy_.up(T.py.data(), np));
+  HCHK(dz_.up(T.pz.data(), np));
+  HCHK(dstart.up(T.start.data(), T.start.size()));
+  if (!T.pts.empty()) HCHK(dpts.up(T.pts.data(), T.pts.size()));
+  const double distg = sqrt(G.dx * G.dx + G.dy * G.dy);
+  Bins B{dstart.p, dpts.p, T.nbx, T.nby};
+  k_dtm_idw<<<blocks(npoig), INP_BLOCK>>>(G.nx, G.ny, G.xmin, G.ymin, G.dx, G.dy, distg, dx_.p, dy_.p, dz_.p, B,
+                                          T.zmax, dZ.p, dmod.p);
+  HCHK(hipGetLastError());
+  in->Z.resize(npoig);
+  in->in_model.resize(npoig);
+  HCHK(dZ.down(in->Z.data(), npoig));
+  HCHK(dmod.down(in->in_model.data(), npoig));
+  return GFSPH_OK;
+}
+
+// Get_Topol (2:1723-1939) with niter = 5 on the device
+int topol_build(gfsph_inputs *in) {
+  const gfsph_grid &G = in->grid;
+  if (G.fdiff) return GFSPH_OK;  // finite differences on the device at create (Fdiff_derivs)
+  const long npoig = (long)G.nx * G.ny;
+  Mesh M{G.nx, G.ny, G.xmin, G.ymin, G.dx, G.dy};
+  DevBuf<double> dZ, mml, T4, rhs, v0, v1, s, top9;
+  HCHK(dZ.alloc(npoig));
+  HCHK(mml.alloc(npoig));
+  HCHK(T4.alloc(4 * npoig));
+  HCHK(rhs.alloc(3 * npoig));
+  HCHK(v0.alloc(3 * npoig));
+  HCHK(v1.alloc(3 * npoig));
+  HCHK(s.alloc(3 * npoig));
+  HCHK(top9.alloc(9 * npoig));
+  HCHK(dZ.up(in->Z.data(), npoig));
+  const int nb = blocks(npoig);
+  k_topo_mass<<<nb, INP_BLOCK>>>(M, mml.p);
+  k_topo_pack<<<nb, INP_BLOCK>>>((int)npoig, dZ.p, nullptr, T4.p);
+  // slopes (A2-A5)
+  k_topo_rhs<<<nb, INP_BLOCK>>>(M, 0, T4.p, rhs.p);
+  k_topo_first<<<nb, INP_BLOCK>>>((int)npoig, 2, rhs.p, mml.p, v0.p);
+  double *a = v0.p, *b = v1.p;
+  for (int it = 2; it <= 5; it++) {
+    k_topo_iter<<<nb, INP_BLOCK>>>(M, 2, rhs.p, mml.p, a, b);
+    std::swap(a, b);
+  }
+  HCHK(hipMemcpy(s.p, a, sizeof(double) * 3 * npoig, hipMemcpyDeviceToDevice));
+  // curvatures (B1-B5)
+  k_topo_pack<<<nb, INP_BLOCK>>>((int)npoig, dZ.p, s.p, T4.p);
+  k_topo_rhs<<<nb, INP_BLOCK>>>(M, 1, T4.p, rhs.p);
+  k_topo_first<<<nb, INP_BLOCK>>>((int)npoig, 3, rhs.p, mml.p, v0.p);
+  a = v0.p;
+  b = v1.p;
+  for (int it = 2; it <= 5; it++) {
+    k_topo_iter<<<nb, INP_BLOCK>>>(M, 3, rhs.p, mml.p, a, b);
+    std::swap(a, b);
+  }
+  k_topo_final<<<nb, INP_BLOCK>>>((int)npoig, dZ.p, s.p, a, top9.p, nullptr);
+  HCHK(hipGetLastError());
+  in->topol.resize(9 * npoig);
+  HCHK(top9.down(in->topol.data(), 9 * npoig));
+  return GFSPH_OK;
+}
+
+// Read_Source_plus_New's reading, release grid and cells (6:5063-5168), np_Src > 0.  Host.
+int read_pts7(gfsph_inputs *in, const std::string &path) {
+  FReader R;
+  if (!R.open(path)) return ifail(in, GFSPH_ERR_ARG, "cannot open " + path);
+  SrcRaw &S = in->src;
+  std::vector<double> rv;
+  RD(R.text());
+  RD(R.reals(rv, 5));
+  const long nsrc = (long)rv[0];
+  const double dx_s = rv[1];
+  double d2x = rv[2], d2y = rv[3];
+  S.facthsml = rv[4];
+  S.delt2D = std::max(d2x, d2y);
+  if (nsrc <= 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": np_Src <= 0 (points from topol(15,:))");
+  if (!(d2x > 0) || !(d2y > 0)) return ifail(in, GFSPH_ERR_ARG, path + ": grid spacing");
+  RD(R.text());
+  S.sx.resize(nsrc);
+  S.sy.resize(nsrc);
+  S.sh.resize(nsrc);
+  double xmin = 1.e9, ymin = 1.e9, xmax = -1.e9, ymax = -1.e9;
+  for (long k = 0; k < nsrc; k++) {
+    RD(R.reals(rv, 3));
+    S.sx[k] = rv[0] - in->grid.xshift;
+    S.sy[k] = rv[1] - in->grid.yshift;
+    S.sh[k] = rv[2];
+    if (S.sx[k] <= xmin) xmin = S.sx[k];
+    if (S.sx[k] >= xmax) xmax = S.sx[k];
+    if (S.sy[k] <= ymin) ymin = S.sy[k];
+    if (S.sy[k] >= ymax) ymax = S.sy[k];
+  }
+  S.nx = (int)((xmax - xmin) / d2x + 1);
+  S.ny = (int)((ymax - ymin) / d2y + 1);
+  if (S.nx < 2 || S.ny < 2) return ifail(in, GFSPH_ERR_ARG, path + ": release grid has fewer than 2 x 2 nodes");
+  d2x = (xmax - xmin) / (S.nx - 1);
+  d2y = (ymax - ymin) / (S.ny - 1);
+  S.xmin = xmin;
+  S.ymin = ymin;
+  S.d2x = d2x;
+  S.d2y = d2y;
+  S.dist2D = sqrt(d2x * d2x + d2y * d2y);
+  S.ndx = (int)std::max(2., dx_s / d2x);
+  S.ndy = (int)std::max(2., dx_s / d2y);
+  if ((2 * S.ndx + 1) * (2 * S.ndy + 1) > MAX_WINDOW)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": point spacing too large for the release grid");
+  S.nbx = S.nx + 1;
+  S.nby = S.ny + 1;
+  std::vector<int> cell(nsrc);
+  for (long k = 0; k < nsrc; k++) {
+    const int ipx = (int)((S.sx[k] - xmin) / d2x + 1), ipy = (int)((S.sy[k] - ymin) / d2y + 1);
+    cell[k] = std::min(std::max(ipy - 1, 0), S.nby - 1) * S.nbx + std::min(std::max(ipx - 1, 0), S.nbx - 1);
+  }
+  bin_points(cell, S.nbx * S.nby, S.start, S.pts);
+  return GFSPH_OK;
+}
+
+// Read_Source_plus_New's gridding (6:5170-5247) on the device; nodes with
+// weight become particles in node order
+int release_build(gfsph_inputs *in) {
+  const SrcRaw &S = in->src;
+  const long nsrc = (long)S.sx.size(), np2 = (long)S.nx * S.ny;
+  DevBuf<double> dx_, dy_, dh, aux;
+  DevBuf<int> dstart, dpts;
+  HCHK(dx_.alloc(nsrc));
+  HCHK(dy_.alloc(nsrc));
+  HCHK(dh.alloc(nsrc));
+  HCHK(dstart.alloc(S.start.size()));
+  HCHK(dpts.alloc(S.pts.size()));
+  HCHK(aux.alloc(4 * np2));
+  HCHK(dx_.up(S.sx.data(), nsrc));
+  HCHK(dy_.up(S.sy.data(), nsrc));
+  HCHK(dh.up(S.sh.data(), nsrc));
+  HCHK(dstart.up(S.start.data(), S.start.size()));
+  HCHK(dpts.up(S.pts.data(), S.pts.size()));
+  Bins B{dstart.p, dpts.p, S.nbx, S.nby};
+  k_src_idw<<<blocks(np2), INP_BLOCK>>>(S.nx, S.ny, S.xmin, S.ymin, S.d2x, S.d2y, S.dist2D, S.ndx, S.ndy, dx_.p,
+                                        dy_.p, dh.p, B, aux.p);
+  k_src_area<<<blocks(np2), INP_BLOCK>>>(S.nx, S.ny, aux.p);
+  HCHK(hipGetLastError());
+  std::vector<double> ah(4 * np2);
+  HCHK(aux.down(ah.data(), 4 * np2));
+  const double dArea = S.d2x * S.d2y / 4;
+  in->n = 0;
+  for (long k = 0; k < np2; k++) {
+    if (!(ah[4 * k + 1] >= 1.e-6)) continue;
+    const int ix = (int)(k % S.nx), iy = (int)(k / S.nx);
+    double rho = ah[4 * k + 2] / ah[4 * k + 1];
+    if (rho < in->P.h_inf) rho = in->P.h_inf;
+    in->x.push_back(S.xmin + (double)ix * S.d2x);
+    in->x.push_back(S.ymin + (double)iy * S.d2y);
+    in->v.push_back(0.0);
+    in->v.push_back(0.0);
+    in->h.push_back(rho);
+    in->mass.push_back(dArea * rho * ah[4 * k + 3]);
+    in->hsml.push_back(S.delt2D * S.facthsml);
+    in->u.push_back(0.0);
+    in->itype.push_back(1);
+    in->n++;
+  }
+  return GFSPH_OK;
+}
+
+// Get_Restart_Info_SW (6:3268-3340).  Host.
+int read_restart(gfsph_inputs *in, const std::string &path) {
+  FReader R;
+  if (!R.open(path)) return ifail(in, GFSPH_ERR_ARG, "cannot open " + path);
+  std::vector<double> rv;
+  RD(R.text());
+  RD(R.reals(rv, 2));
+  const long n = (long)rv[0];
+  in->run.time_restart = rv[1];
+  if (n <= 0) return ifail(in, GFSPH_ERR_ARG, path + ": npoin_s");
+  RD(R.text());
+  for (long k = 0; k < n; k++) {
+    RD(R.reals(rv, 10));
+    in->itype.push_back((int32_t)rv[1]);
+    in->x.push_back(rv[2]);
+    in->x.push_back(rv[3]);
+    in->v.push_back(rv[4]);
+    in->v.push_back(rv[5]);
+    in->h.push_back(rv[6]);
+    in->mass.push_back(rv[7]);
+    in->hsml.push_back(rv[8]);
+    in->u.push_back(rv[9]);
+  }
+  in->n = (int)n;
+  return GFSPH_OK;
+}
+
+// everything that needs no device: the files and the grids' geometry
+int parse_all(gfsph_inputs *in, const std::string &dir, const std::string &name, const std::string &topo) {
+  const std::string base = dir.empty() ? std::string() : dir + "/";
+  int rc = read_master(in, base + name + ".MASTER.dat");
+  if (rc) return rc;
+  std::string src;
+  if ((rc = read_dat(in, base + name + ".dat", src))) return rc;
+  if (in->run.present) {
+    if (in->run.t_integ_alg != 4)
+      return ifail(in, GFSPH_ERR_UNSUPPORTED, "SPH_t_Integ_Alg " + std::to_string(in->run.t_integ_alg) +
+                                                  " (4, RK4, is the implemented scheme)");
+    in->P.dt = in->run.dt_sph;
+  }
+  if ((rc = read_top(in, base + topo + ".top", in->topo))) return rc;
+  if ((rc = grid_geometry(in))) return rc;
+  // the release file is <problem name>.pts (6:5053-5055); a restart is restart.dat
+  if (in->icunk == 7) return read_pts7(in, base + name + ".pts");
+  return read_restart(in, base + "restart.dat");
+}
+
+int build_all(gfsph_inputs *in) {
+  int rc;
+  if ((rc = grid_build(in)) || (rc = topol_build(in))) return rc;
+  if (in->icunk == 7 && (rc = release_build(in))) return rc;
+  in->built = true;
+  return GFSPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfsph_inputs_parse(const char *dir, const char *name, const char *topo, gfsph_inputs **out) {
+  if (!out || !name) return GFSPH_ERR_ARG;
+  *out = nullptr;
+  gfsph_inputs *in = new (std::nothrow) gfsph_inputs();
+  if (!in) return GFSPH_ERR_ARG;
+  *out = in;
+  return parse_all(in, dir ? dir : "", name, topo ? topo : name);
+}
+
+int gfsph_inputs_read(const char *dir, const char *name, const char *topo, gfsph_inputs **out) {
+  int rc = gfsph_inputs_parse(dir, name, topo, out);
+  if (rc) return rc;
+  return build_all(*out);
+}
+
+int gfsph_inputs_info(gfsph_inputs *in, gfsph_params *p, gfsph_run *run, gfsph_grid *grid, int32_t *npart) {
+  if (!in) return GFSPH_ERR_ARG;
+  if (p) *p = in->P;
+  if (run) *run = in->run;
+  if (grid) *grid = in->grid;
+  if (npart) *npart = in->n;
+  return GFSPH_OK;
+}
+
+int gfsph_inputs_dem(gfsph_inputs *in, double *Z, int32_t *in_model, double *topol) {
+  if (!in) return GFSPH_ERR_ARG;
+  if (!in->built) return ifail(in, GFSPH_ERR_ARG, "the terrain is built by gfsph_inputs_read");
+  const size_t np = in->Z.size();
+  if (Z) memcpy(Z, in->Z.data(), sizeof(double) * np);
+  if (in_model) memcpy(in_model, in->in_model.data(), sizeof(int32_t) * np);
+  if (topol) {
+    if (in->topol.empty()) return ifail(in, GFSPH_ERR_ARG, "slopes are finite differences (ictop >= 100)");
+    memcpy(topol, in->topol.data(), sizeof(double) * 9 * np);
+  }
+  return GFSPH_OK;
+}
+
+int gfsph_inputs_particles(gfsph_inputs *in, double *x, double *v, double *h, double *mass, double *hsml,
+                           double *u, int32_t *itype) {
+  if (!in) return GFSPH_ERR_ARG;
+  if (in->n <= 0) return ifail(in, GFSPH_ERR_ARG, "no release particles (a gridded release is built by gfsph_inputs_read)");
+  const size_t n = in->n;
+  if (x) memcpy(x, in->x.data(), sizeof(double) * 2 * n);
+  if (v) memcpy(v, in->v.data(), sizeof(double) * 2 * n);
+  if (h) memcpy(h, in->h.data(), sizeof(double) * n);
+  if (mass) memcpy(mass, in->mass.data(), sizeof(double) * n);
+  if (hsml) memcpy(hsml, in->hsml.data(), sizeof(double) * n);
+  if (u) memcpy(u, in->u.data(), sizeof(double) * n);
+  if (itype) memcpy(itype, in->itype.data(), sizeof(int32_t) * n);
+  return GFSPH_OK;
+}
+
+int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out) {
+  if (!in || !out) return GFSPH_ERR_ARG;
+  if (!in->built || in->n <= 0) return ifail(in, GFSPH_ERR_ARG, "inputs not built (gfsph_inputs_read)");
+  *out = nullptr;
+  if (!(in->P.dt > 0)) return ifail(in, GFSPH_ERR_ARG, "dt_sph is not set (no MASTER.dat): set it in the params");
+  gfsph_dem d;
+  d.nx = in->grid.nx;
+  d.ny = in->grid.ny;
+  d.xmin = in->grid.xmin;
+  d.ymin = in->grid.ymin;
+  d.dx = in->grid.dx;
+  d.dy = in->grid.dy;
+  d.xmax = in->grid.xmax;
+  d.ymax = in->grid.ymax;
+  d.Z = in->Z.data();
+  d.in_model = in->in_model.data();
+  d.topol = in->topol.empty() ? nullptr : in->topol.data();
+  int rc = gfsph_create(&in->P, &d, in->n, in->x.data(), in->v.data(), in->h.data(), in->mass.data(),
+                        in->hsml.data(), in->u.data(), in->itype.data(), out);
+  if (rc) {
+    in->err = std::string("gfsph_create: ") + (*out ? gfsph_last_error(*out) : "failed");
+    if (*out) gfsph_destroy(*out);
+    *out = nullptr;
+    return rc;
+  }
+  if (in->run.present && in->run.adapt_dt) {
+    if ((rc = gfsph_set_adaptive_dt(*out, in->run.adapt_dt))) {
+      in->err = gfsph_last_error(*out);
+      gfsph_destroy(*out);
+      *out = nullptr;
+      return rc;
+    }
+  }
+  if (in->run.time_restart != 0.0) gfsph_set_time(*out, in->run.time_restart);
+  return GFSPH_OK;
+}
+
+const char *gfsph_inputs_error(gfsph_inputs *in) { return in ? in->err.c_str() : "null inputs"; }
+
+void gfsph_inputs_destroy(gfsph_inputs *in) { delete in; }
+
+}  // extern "C"

@@ -11,6 +11,8 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libgfsph.so")
+# the propagation step and the reference's input path
+SOURCES = ("gfsph.hip", "gfsph_inputs.hip")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
@@ -29,9 +31,9 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    src = os.path.join(CSRC, "gfsph.hip")
+    srcs = [os.path.join(CSRC, f) for f in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, src]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, *srcs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])

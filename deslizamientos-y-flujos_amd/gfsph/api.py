@@ -24,7 +24,9 @@ EXPORTED_SYMBOLS = (
     "gfsph_raster_idw", "gfsph_mc_pixel_accumulate", "gfsph_mc_pixel_read", "gfsph_profile_step",
     "gfsph_counts", "gfsph_last_error", "gfsph_destroy", "gfsph_batch_create", "gfsph_batch_pixel_accumulate",
     "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune", "gfsph_adjoint_forward",
-    "gfsph_adjoint_backward", "gfsph_raster_misfit", "gfsph_set_adaptive_dt", "gfsph_time",
+    "gfsph_adjoint_backward", "gfsph_raster_misfit", "gfsph_set_adaptive_dt", "gfsph_time", "gfsph_set_time",
+    "gfsph_inputs_parse", "gfsph_inputs_read", "gfsph_inputs_info", "gfsph_inputs_dem", "gfsph_inputs_particles",
+    "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -50,6 +52,19 @@ class CDem(C.Structure):
                 ("dx", C.c_double), ("dy", C.c_double), ("xmax", C.c_double), ("ymax", C.c_double),
                 ("Z", C.POINTER(C.c_double)), ("in_model", C.POINTER(C.c_int32)),
                 ("topol", C.POINTER(C.c_double))]
+
+
+class CRun(C.Structure):
+    _fields_ = [("present", C.c_int32), ("t_integ_alg", C.c_int32), ("adapt_dt", C.c_int32),
+                ("maxtimestep", C.c_int32), ("print_step", C.c_int32), ("save_step", C.c_int32),
+                ("plot_step", C.c_int32), ("pad_", C.c_int32), ("dt", C.c_double), ("time_end", C.c_double),
+                ("dt_sph", C.c_double), ("time_restart", C.c_double)]
+
+
+class CGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("xmin", C.c_double), ("ymin", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("xmax", C.c_double), ("ymax", C.c_double),
+                ("xshift", C.c_double), ("yshift", C.c_double), ("fdiff", C.c_int32), ("pad_", C.c_int32)]
 
 
 _lib = None
@@ -91,6 +106,17 @@ def load_library(path: str | None = None):
     L.gfsph_raster_misfit.argtypes = [vp, dp, dp, dp, dp]
     L.gfsph_set_adaptive_dt.argtypes = [vp, C.c_int32]
     L.gfsph_time.argtypes = [vp, dp, dp]
+    L.gfsph_set_time.argtypes = [vp, C.c_double]
+    for f in (L.gfsph_inputs_parse, L.gfsph_inputs_read):
+        f.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(vp)]
+    L.gfsph_inputs_info.argtypes = [vp, C.POINTER(CParams), C.POINTER(CRun), C.POINTER(CGrid), ip]
+    L.gfsph_inputs_dem.argtypes = [vp, dp, ip, dp]
+    L.gfsph_inputs_particles.argtypes = [vp, dp, dp, dp, dp, dp, dp, ip]
+    L.gfsph_inputs_create.argtypes = [vp, C.POINTER(vp)]
+    L.gfsph_inputs_error.argtypes = [vp]
+    L.gfsph_inputs_error.restype = C.c_char_p
+    L.gfsph_inputs_destroy.argtypes = [vp]
+    L.gfsph_inputs_destroy.restype = None
     if path is None:
         _lib = L
     return L
@@ -129,7 +155,11 @@ class _DemArgs:
         d.xmin, d.ymin, d.dx, d.dy, d.xmax, d.ymax = dem.xmin, dem.ymin, dem.dx, dem.dy, dem.xmax, dem.ymax
         d.Z = _dp(self._Z)
         d.in_model = _ip(self._im) if self._im is not None else C.POINTER(C.c_int32)()
-        d.topol = C.POINTER(C.c_double)()
+        # nodal terrain terms projected from the input files (Get_Topol); else
+        # finite differences on the device
+        t9 = getattr(dem, "topol9", None)
+        self._t9 = None if t9 is None else np.ascontiguousarray(t9, dtype=np.float64)
+        d.topol = _dp(self._t9) if self._t9 is not None else C.POINTER(C.c_double)()
         self.c = d
 
 
@@ -157,6 +187,16 @@ class SPHModel:
             raise GfsphError(rc, msg)
         self.nsteps_done = 0
         self._apply_time_control(case.params)
+
+    @classmethod
+    def adopt(cls, lib, ctx, case):
+        """A model around a context created elsewhere (gfsph_inputs_create)."""
+        m = cls.__new__(cls)
+        m.lib, m.case, m.ctx, m.R = lib, case, ctx, 1
+        m.n = case.particles.n
+        m._dem = None
+        m.nsteps_done = 0
+        return m
 
     def _apply_time_control(self, params) -> None:
         mode = int(getattr(params, "adapt_dt", 0))

@@ -198,6 +198,71 @@ int gfsph_set_adaptive_dt(gfsph_ctx *ctx, int32_t mode);
  * pointer may be NULL).  Synchronises. */
 int gfsph_time(gfsph_ctx *ctx, double *time, double *dt);
 
+/* Sets time_sph of every realisation, e.g. to time_sph_restart
+ * ("6 SPH_SW_2019.f90":3302) when a run continues from restart.dat. */
+int gfsph_set_time(gfsph_ctx *ctx, double time);
+
+/* ---------------------------------------------------------------------------
+ * The reference's input files, replacing what Init_sph reads
+ * ("7 SPH_MAIN_2019.f90":74).  In directory dir:
+ *   <name>.MASTER.dat  run control (Driver_Input, "8 SPH_FEM_Driver_2019.f90":
+ *                      229-340), optional: SPH only, SPH_t_Integ_Alg 4
+ *   <name>.dat         switches, h_inf_SW, const(:), icpwp (Input_SW,
+ *                      "6 SPH_SW_2019.f90":812-1570): a single-phase soil release
+ *   <topo>.top         terrain (Topo_main, "2 SPH_FEM_topo_2019.f90":208-620):
+ *                      ictop 0 (explicit grid) or 10, 11, 15, 16 (scattered DTM
+ *                      gridded as Get_DTM does, 2:1155-1449); +100 selects
+ *                      finite-difference slopes, otherwise Get_Topol's
+ *                      finite-element projection (2:1723-1939) runs on the GPU
+ *   <name>.pts         release points gridded as Read_Source_plus_New does
+ *                      (icunk_s 7, 6:4999-5272), or restart.dat (icunk_s 200,
+ *                      Get_Restart_Info_SW, 6:3268-3340)
+ * topo = NULL reads <name>.top.  Other options of these files are rejected
+ * with GFSPH_ERR_UNSUPPORTED.  On failure *out still holds the object for
+ * gfsph_inputs_error unless it could not be allocated.
+ * ------------------------------------------------------------------------- */
+typedef struct gfsph_inputs gfsph_inputs;
+
+typedef struct gfsph_run {       /* <name>.MASTER.dat (all zero if absent)      */
+  int32_t present;
+  int32_t t_integ_alg;           /* SPH_t_Integ_Alg                             */
+  int32_t adapt_dt;              /* ic_adapt_dt_sph                             */
+  int32_t maxtimestep, print_step, save_step, plot_step;
+  int32_t pad_;
+  double dt, time_end, dt_sph;
+  double time_restart;           /* time_sph_restart of restart.dat (icunk 200) */
+} gfsph_run;
+
+typedef struct gfsph_grid {      /* the terrain grid (Topo_main's module state) */
+  int32_t nx, ny;                /* npoigx, npoigy                              */
+  double xmin, ymin, dx, dy;     /* Xming, Yming, deltxg, deltyg                */
+  double xmax, ymax;             /* Xmaxg, Ymaxg                                */
+  double xshift, yshift;         /* xshiftg, yshiftg (ictop 11, 15, 16)         */
+  int32_t fdiff;                 /* 1: slopes by finite differences at create  */
+  int32_t pad_;
+} gfsph_grid;
+
+int gfsph_inputs_read(const char *dir, const char *name, const char *topo, gfsph_inputs **out);
+/* The host half of gfsph_inputs_read: the files, the grids' geometry and the
+ * binning of the points, with no device work (gfsph_inputs_info then reports
+ * the switches, run control, grid and, for a restart, the particles). */
+int gfsph_inputs_parse(const char *dir, const char *name, const char *topo, gfsph_inputs **out);
+/* The parsed switches (dt = dt_sph of MASTER.dat, else 0; init_input = 1),
+ * run control, grid and number of release particles. */
+int gfsph_inputs_info(gfsph_inputs *in, gfsph_params *p, gfsph_run *run, gfsph_grid *grid, int32_t *npart);
+/* Z, poig_in_model ([ny*nx]) and topol(1:9,:) ([ny*nx][9]; not available with
+ * finite-difference slopes). */
+int gfsph_inputs_dem(gfsph_inputs *in, double *Z, int32_t *in_model, double *topol);
+/* The release as read or gridded, before Input_SW's summed-depth and
+ * pore-pressure initialisation (which gfsph_inputs_create applies). */
+int gfsph_inputs_particles(gfsph_inputs *in, double *x, double *v, double *h, double *mass, double *hsml,
+                           double *u, int32_t *itype);
+/* gfsph_create on the inputs (init_input = 1), then ic_adapt_dt_sph of
+ * MASTER.dat and time_sph_restart. */
+int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out);
+const char *gfsph_inputs_error(gfsph_inputs *in);
+void gfsph_inputs_destroy(gfsph_inputs *in);
+
 const char *gfsph_last_error(gfsph_ctx *ctx);
 void gfsph_destroy(gfsph_ctx *ctx);
 

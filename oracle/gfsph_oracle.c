@@ -720,11 +720,11 @@ static double adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const 
                           const double *hsml, const int32_t *flag, double dt) {
   const int mode = p->adapt_dt;
   const double cgra = p->cst[1];
-  const double dxg_mod = pow(dem->dx * dem->dx + dem->dy * dem->dy, 0.5);
+  const double dxg_mod = sqrt(dem->dx * dem->dx + dem->dy * dem->dy); /* **0.5: a square root */
   double dtmin = 1.e10;
   for (int i = 0; i < n; i++) {
     if (flag[i]) continue;
-    double vv = pow(v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1], 0.5), cc;
+    double vv = sqrt(v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1]), cc;
     if (mode == 1 || mode == 2 || mode == 10) {
       if (rho[i] < p->h_inf) rho[i] = p->h_inf;
       const double c = sqrt(cgra * rho[i]);

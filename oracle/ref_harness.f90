@@ -13,7 +13,8 @@
 !     line 2: nsteps  dt_sph  ic_adapt_dt_sph
 !     line 3: ndump   followed by ndump step numbers (0 = after Init_sph)
 !
-!  At each listed step it writes dump_<step>.bin (stream, native endian):
+!  After Init_sph it writes topo.bin (the terrain, see write_topo) and at
+!  each listed step dump_<step>.bin (stream, native endian):
 !     int32  magic(=1195725648) version(=1) npoin ndimn has_uaux npairs
 !     real64 time_sph
 !     real64 x(ndimn,npoin) vx(ndimn,npoin) rho mass hsml u  [uaux1]
@@ -34,6 +35,8 @@ use SPH_FEM_Driver_time_vars_2019
 use SPH_MAIN_2019, only: Init_SPH, time_integration_sph, adaptive_dt_sph
 use SPH_Main_Vars_2019
 use SPH_SW_Vars_2019
+use SPH_FEM_topo_2019, only: npoigx, npoigy, Xming, Xmaxg, Yming, Ymaxg, deltxg, deltyg, &
+                             xshiftg, yshiftg, topol, poig_in_model
 #ifdef GFSPH_GPU
 use SPH_MAIN_GFSPH, only: time_integration_sph_gfsph, gfsph_release
 #endif
@@ -98,6 +101,7 @@ if (time_sph_restart.gt.0.0) then
 endif
 
 call maybe_dump (0)
+call write_topo
 do istep = 1, nsteps
    t_print_reset = t_print_reset + dt
 #ifndef GFSPH_GPU
@@ -118,6 +122,22 @@ call gfsph_release
 #endif
 
 contains
+
+! topo.bin: the terrain Init_sph built (Topo_main's module state)
+!    int32  magic(=1195725648) version(=2) npoigx npoigy
+!    real64 Xming Xmaxg Yming Ymaxg deltxg deltyg xshiftg yshiftg
+!    real64 topol(1:9, npoig)   int32 poig_in_model(npoig)
+subroutine write_topo
+   integer iu, ip, np9
+   np9 = npoigx*npoigy
+   open (newunit=iu, file='topo.bin', access='stream', form='unformatted', status='replace')
+   write (iu) int(1195725648,4), int(2,4), int(npoigx,4), int(npoigy,4)
+   write (iu) real(Xming,8), real(Xmaxg,8), real(Yming,8), real(Ymaxg,8), real(deltxg,8), real(deltyg,8), &
+              real(xshiftg,8), real(yshiftg,8)
+   write (iu) (real(topol(1:9,ip),8), ip=1,np9)
+   write (iu) (int(poig_in_model(ip),4), ip=1,np9)
+   close (iu)
+end subroutine write_topo
 
 subroutine maybe_dump (is)
    integer, intent(in) :: is

@@ -92,6 +92,21 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
         f.write("\n".join(rows) + "\n")
 
 
+def write_master(d: str, name: str, dt: float, adapt: int = 0) -> None:
+    """<name>.MASTER.dat in Driver_Input's layout ("8 SPH_FEM_Driver_2019.f90":
+    229-340): SPH only, RK4, dt_sph and ic_adapt_dt_sph."""
+    L = ["1", "run control written for the reference's input path",
+         "if_sph if_gfl ic_cases_mat", "1 0 0",
+         "SPH_problem_type SPH_t_integ_Alg", "1 4",
+         "sph problem name", name,
+         "dt time_end maxtimesteps", f"{_r(dt)} 1.e9 1000000",
+         "print_step save_step plot_step", "1000000 1000000 1000000",
+         "dt_sph ic_adapt", f"{_r(dt)} {int(adapt)}",
+         "Ntime curves max pts", "0 0", "win", "0"]
+    with open(os.path.join(d, name + ".MASTER.dat"), "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
 def read_dump(path: str) -> dict:
     raw = open(path, "rb").read()
     hdr = np.frombuffer(raw, dtype=np.int32, count=6)
@@ -127,6 +142,46 @@ def read_dump(path: str) -> dict:
         out["pair_w"] = np.zeros(0)
         out["pair_dwdx"] = np.zeros((0, 2))
     return out
+
+
+def read_topo(path: str) -> dict:
+    """topo.bin written by the harness after Init_sph (the terrain grid,
+    topol(1:9,:) and poig_in_model)."""
+    raw = open(path, "rb").read()
+    magic, ver, nx, ny = (int(v) for v in np.frombuffer(raw, dtype=np.int32, count=4))
+    assert magic == 1195725648 and ver == 2
+    g = np.frombuffer(raw, dtype=np.float64, count=8, offset=16)
+    npg = nx * ny
+    top = np.frombuffer(raw, dtype=np.float64, count=9 * npg, offset=80).reshape(npg, 9).copy()
+    im = np.frombuffer(raw, dtype=np.int32, count=npg, offset=80 + 72 * npg).copy()
+    return {"nx": nx, "ny": ny, "xmin": g[0], "xmax": g[1], "ymin": g[2], "ymax": g[3], "dx": g[4], "dy": g[5],
+            "xshift": g[6], "yshift": g[7], "topol": top, "in_model": im}
+
+
+def run_reference_files(src_dir: str, name: str, files, nsteps: int, dt: float, adapt: int, dump_steps,
+                        topo: str | None = None, timeout: float = 3600.0) -> tuple:
+    """Run the reference harness on input files the reference reads itself
+    (<name>.dat, <topo>.top, <name>.pts or restart.dat) copied from src_dir;
+    returns ({step: dump}, topo dict)."""
+    if not os.path.exists(REF_EXE):
+        raise FileNotFoundError(REF_EXE)
+    tmp = tempfile.mkdtemp(prefix="gfsph_ref_")
+    try:
+        for f in files:
+            with open(os.path.join(src_dir, f), "rb") as a, open(os.path.join(tmp, f), "wb") as b:
+                b.write(a.read())
+        steps = sorted(set(int(s) for s in dump_steps))
+        with open(os.path.join(tmp, "harness.ctl"), "w") as f:
+            f.write(f"{name}\n{nsteps} {_r(dt)} {int(adapt)}\n")
+            f.write(f"{len(steps)} " + " ".join(str(s) for s in steps) + "\n")
+        r = subprocess.run([REF_EXE], input=(topo or name) + "\n" + "\n" * 8, cwd=tmp, capture_output=True,
+                           text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        out = {s: read_dump(os.path.join(tmp, f"dump_{s}.bin")) for s in steps}
+        return out, read_topo(os.path.join(tmp, "topo.bin"))
+    finally:
+        subprocess.run(["rm", "-rf", tmp])
 
 
 def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, timeout: float = 3600.0,
