@@ -1101,7 +1101,8 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 // read is a scalar load.  part = nullptr: no bounding-box partials (several
 // realisations: k_bbox_real computes their boxes).
 __global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
-                                                  const double *S, const double2 *ain, NewSet o, BBoxAcc *part) {
+                                                  const double *S, const double2 *ain, NewSet o, BBoxAcc *part,
+                                                  const int *stopr) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
@@ -1112,7 +1113,18 @@ __global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, Dev
     if (myr != r0) continue;
     todo = false;
     const DevParams &P = Pr[r0];
-    if (s.flag[i]) {
+    if (stopr && stopr[r0]) {  // a realisation whose case has stopped does not change
+      o.pos[i] = s.pos[i];
+      o.vel[i] = s.vel[i];
+      o.h[i] = s.h[i];
+      o.hsml[i] = s.hsml[i];
+      o.u[i] = s.u[i];
+      o.uaux[i] = s.uaux[i];
+      o.flag[i] = s.flag[i];
+      active_out = !s.flag[i];
+      xo = s.pos[i];
+      hso = s.hsml[i];
+    } else if (s.flag[i]) {
       write_frozen(P, s, i, o);
     } else {
       const double2 in = ain[i];
@@ -1661,11 +1673,11 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
 template <int SKF>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl,
                                                               const unsigned long long *mask, SortedSet s, NewSet o,
-                                                              double2 *vout) {
+                                                              double2 *vout, const int *stopr) {
   const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
   double2 vi = o.vel[i];
-  if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0) {
+  if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0 || (stopr && stopr[s.rid[i]])) {
     vout[i] = vi;
     return;
   }
@@ -2408,12 +2420,13 @@ __global__ void k_count_active(int n, const int *flag, int *count) {
 
 __global__ __launch_bounds__(GF_BLOCK) void k_dt_min(int n, int mode, double h_inf, double cgra, double dxg,
                                                      const double2 *vel, double *h, const double *hsml,
-                                                     const int *flag, const int *rid, unsigned long long *dtm) {
+                                                     const int *flag, const int *rid, const int *stopr,
+                                                     unsigned long long *dtm) {
   __shared__ unsigned long long wmin[GF_BLOCK / 64];
   const int b0 = blockIdx.x * blockDim.x;
   const int i = b0 + threadIdx.x;
   unsigned long long key = ~0ull;
-  if (i < n && !flag[i]) {
+  if (i < n && !flag[i] && !(stopr && stopr[rid[i]])) {
     const double2 v = vel[i];
     const double vv = sqrt(v.x * v.x + v.y * v.y);
     double dm = 1.e10;
@@ -2448,17 +2461,124 @@ __global__ __launch_bounds__(GF_BLOCK) void k_dt_min(int n, int mode, double h_i
 }
 
 // New dt_sph of every realisation (6:6655-6660), its RK weight, time_sph
-// (8:1617), and the minimum slots reset for the next step.
-__global__ void k_dt_apply(int R, int nstages, DevParams *Pr, unsigned long long *dtm, double *tm) {
+// (8:1617; with case control k_case_pre adds the step's dt instead), and the
+// minimum slots reset for the next step.  Stopped realisations keep theirs.
+__global__ void k_dt_apply(int R, int nstages, DevParams *Pr, unsigned long long *dtm, double *tm, const int *stopr,
+                           int add_time) {
   for (int r = threadIdx.x; r < R; r += blockDim.x) {
     const double dtmin = __longlong_as_double((long long)dtm[r]);
+    dtm[r] = DT_INIT;
+    if (stopr && stopr[r]) continue;
     const double dt = Pr[r].dt;
     const double nd = dtmin < dt ? dtmin : fmin(1.1 * dt, dtmin);
     Pr[r].dt = nd;
     Pr[r].fin = nstages == 4 ? nd / 6. : nd / 2.;
-    tm[r] = tm[r] + nd;
-    dtm[r] = DT_INIT;
+    if (add_time) tm[r] = tm[r] + nd;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Case control: the step loop of Driver_Flow_23 (8:1444-1548) that the
+// reference runs for every case of a case table or Monte Carlo set, here for
+// every realisation of a batch at once, on the device.  Per step:
+//   k_case_pre   dt_sph cut so that the step ends on the next plot time
+//                (8:1481-1488), t_plot_reset, time_sph and time advanced
+//   (the step; a stopped realisation is copied unchanged)
+//   k_case_due   plot time reached (8:1498): the realisation's pixels are
+//                accumulated (Get_MC_pixel_Vars_SW) by k_mc_pixel
+//   k_case_out, k_case_sum   out-of-domain counts and sum of |vx| of the
+//                active particles, per realisation
+//   k_case_stop  dt_sph restored after a plot (8:1508), Stop_case_SW
+//                (6:20114-20177) and time >= time_end (8:1528)
+// ---------------------------------------------------------------------------
+struct CaseState {
+  double tplot;    // t_plot_reset
+  double tglob;    // time
+  double dt_keep;  // dt_sph before the cut to the plot time
+  double vsum;     // sum of |vx| over the active particles after the step
+  int nsteps;      // itimestep: loop iterations done
+  int pad[3];
+};
+
+struct CaseCtl {
+  double vstop, timestop, dtimestop, dt, time_end, time_plot;
+};
+
+__global__ void k_case_pre(int R, int nstages, DevParams *Pr, double *tm, CaseState *cs, const int *stopr,
+                           double time_plot) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R || stopr[r]) return;
+  double dt = Pr[r].dt;
+  cs[r].dt_keep = dt;
+  if (time_plot > 0.0) {
+    const double tpr = cs[r].tplot + dt;
+    if (tpr >= time_plot) {
+      const int nt = (int)(tpr / time_plot);
+      dt = (double)nt * time_plot - cs[r].tplot;
+    }
+    cs[r].tplot = cs[r].tplot + dt;
+  }
+  Pr[r].dt = dt;
+  Pr[r].fin = nstages == 4 ? dt / 6. : dt / 2.;
+  tm[r] = tm[r] + dt;
+  cs[r].tglob = cs[r].tglob + dt;
+}
+
+__global__ void k_case_due(int R, const CaseState *cs, const int *stopr, double time_plot, int *due, int *outc) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  due[r] = (!stopr[r] && time_plot > 0.0 && cs[r].tplot >= time_plot) ? 1 : 0;
+  outc[r] = 0;
+}
+
+// out-of-domain particles per realisation (integer counts: order-free)
+__global__ void k_case_out(int n, const int *flag, const int *rid, int *outc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) atomicAdd(&outc[rid[i]], 1);
+}
+
+// one block per realisation: sum of |vx| over its active particles (its
+// cells' slot range), in a fixed order
+__global__ __launch_bounds__(GF_BLOCK) void k_case_sum(const int *coff, const int *start, const double2 *vel,
+                                                       const int *flag, const int *stopr, CaseState *cs) {
+  __shared__ double part[GF_BLOCK];
+  const int r = blockIdx.x;
+  if (stopr[r]) return;
+  const int b = start[coff[r]], e = start[coff[r + 1]];
+  double acc = 0.0;
+  for (int i = b + (int)threadIdx.x; i < e; i += GF_BLOCK)
+    if (!flag[i]) acc = acc + fabs(vel[i].x);
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = GF_BLOCK / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cs[r].vsum = part[0];
+}
+
+__global__ void k_case_stop(int R, int nstages, DevParams *Pr, const double *tm, CaseState *cs, int *stopr,
+                            const int *due, const int *outc, const int *cnt_r, CaseCtl ctl) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R || stopr[r]) return;
+  const int it = cs[r].nsteps + 1;
+  cs[r].nsteps = it;
+  if (due[r]) {
+    const double dk = cs[r].dt_keep;
+    Pr[r].dt = dk;
+    Pr[r].fin = nstages == 4 ? dk / 6. : dk / 2.;
+    cs[r].tplot = 0.0;
+  }
+  bool stop = false;
+  if (it > 1) {
+    const double v = cs[r].vsum / (double)cnt_r[r];
+    if (v < ctl.vstop && it > 10) stop = true;
+    if (outc[r] > cnt_r[r] / 5) stop = true;
+    if (tm[r] > ctl.timestop) stop = true;
+    if (ctl.dt < ctl.dtimestop) stop = true;
+  }
+  if (cs[r].tglob >= ctl.time_end) stop = true;
+  if (stop) stopr[r] = 1;
 }
 
 // pair enumeration (i<j by caller index) for the neighbour-set export
@@ -2531,9 +2651,10 @@ __device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
 }
 
 // stride > 0: per-realisation accumulators, realisation r at offset r*stride nodes
-__global__ void k_mc_pixel(int n, DevDem dem, CurView s, long stride, double *acc4, int *cnt) {
+__global__ void k_mc_pixel(int n, DevDem dem, CurView s, long stride, double *acc4, int *cnt, const int *due) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || s.flag[i]) return;
+  if (due && !due[s.rid[i]]) return;
   const long base = stride * s.rid[i];
   double2 p = s.pos[i];
   double2 v = s.vel[i];
@@ -2544,7 +2665,7 @@ __global__ void k_mc_pixel(int n, DevDem dem, CurView s, long stride, double *ac
   long g1 = base + (long)(ipy - 1) * dem.nx + ipx - 1;
   long nodes[4] = {g1, g1 + 1, g1 + 1 + dem.nx, g1 + dem.nx};
   double hh = s.h[i];
-  double vv = pow(v.x * v.x + v.y * v.y, 0.5);
+  double vv = sqrt(v.x * v.x + v.y * v.y);  // **0.5: flang emits a square root
   for (int k = 0; k < 4; k++) {
     long g = nodes[k];
     atomicAdd(&cnt[g], 1);
@@ -2664,6 +2785,13 @@ struct gfsph_ctx {
   double *tm = nullptr;
   unsigned long long *dtm = nullptr;
   std::vector<double> time_h, dt_h;
+  // case control (Driver_Flow_23 per realisation): thresholds, per-realisation
+  // loop state, stop flags, plot flags, out-of-domain counts, sizes
+  int cases = 0, case_pixels = 0;
+  std::vector<int> counts_h;  // particles of each realisation
+  CaseCtl cctl{};
+  CaseState *cs = nullptr;
+  int *stopr = nullptr, *due = nullptr, *outc = nullptr, *cnt_r = nullptr;
   std::vector<void *> allocs;
 };
 
@@ -2927,7 +3055,8 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   hipStream_t st = c->stream;
   const int nb = nblk(n);
   if (c->P.nfric != 4) {
-    k_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw, c->R == 1 ? c->part : nullptr);
+    k_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw, c->R == 1 ? c->part : nullptr,
+                                        c->cases ? c->stopr : nullptr);
     return;
   }
   BinghamState &b = c->bs;
@@ -2988,23 +3117,55 @@ static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[2];
   if (c->P.av_vel)
     SKF_DISPATCH(k_xsph_m, xcd_grid(c->n, bsz), bsz, 0,
-                 c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel));
+                 c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, c->cases ? c->stopr : nullptr));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
 
 // adaptive_dt_sph before a step from set[src] (nothing with a fixed dt_sph)
 static void enqueue_dt(gfsph_ctx *c, int src) {
-  if (!c->adapt) return;
-  const SortedSet &o = c->set[src];
-  k_dt_min<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->adapt, c->P.h_inf, c->P.cgra, c->dxg, o.vel, c->nw.h,
-                                                    c->nw.hsml, c->nw.flag, o.rid, c->dtm);
-  k_dt_apply<<<1, 64, 0, c->stream>>>(c->R, c->P.nstages, c->Pr, c->dtm, c->tm);
+  const int *stopr = c->cases ? c->stopr : nullptr;
+  if (c->adapt) {
+    const SortedSet &o = c->set[src];
+    k_dt_min<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->adapt, c->P.h_inf, c->P.cgra, c->dxg, o.vel, c->nw.h,
+                                                      c->nw.hsml, c->nw.flag, o.rid, stopr, c->dtm);
+    k_dt_apply<<<1, 64, 0, c->stream>>>(c->R, c->P.nstages, c->Pr, c->dtm, c->tm, stopr, c->cases ? 0 : 1);
+  }
+  if (c->cases)
+    k_case_pre<<<(c->R + 63) / 64, 64, 0, c->stream>>>(c->R, c->P.nstages, c->Pr, c->tm, c->cs, c->stopr,
+                                                         c->cctl.time_plot);
+}
+
+// after a step into set[dst]: plot-time pixels, stop conditions
+static void enqueue_case_end(gfsph_ctx *c, int dst) {
+  if (!c->cases) return;
+  const int R = c->R, rb = (R + 63) / 64;
+  hipStream_t st = c->stream;
+  k_case_due<<<rb, 64, 0, st>>>(R, c->cs, c->stopr, c->cctl.time_plot, c->due, c->outc);
+  const SortedSet &s = c->set[dst];
+  if (c->case_pixels) {
+    CurView v;
+    v.pos = c->nw.pos;
+    v.vel = s.vel;
+    v.h = c->nw.h;
+    v.mass = s.mass;
+    v.hsml = c->nw.hsml;
+    v.u = c->nw.u;
+    v.uaux = c->nw.uaux;
+    v.flag = c->nw.flag;
+    v.id = s.id;
+    v.rid = s.rid;
+    const long np = (long)c->dem.nx * c->dem.ny;
+    k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, st>>>(c->n, c->dem, v, np, c->mcr_acc, c->mcr_cnt, c->due);
+  }
+  k_case_out<<<nblk(c->n), GF_BLOCK, 0, st>>>(c->n, c->nw.flag, s.rid, c->outc);
+  k_case_sum<<<R, GF_BLOCK, 0, st>>>(c->coff, c->start, s.vel, c->nw.flag, c->stopr, c->cs);
+  k_case_stop<<<rb, 64, 0, st>>>(R, c->P.nstages, c->Pr, c->tm, c->cs, c->stopr, c->due, c->outc, c->cnt_r, c->cctl);
 }
 
 // time_sph = time_sph + dt_sph after each of nsteps fixed-dt steps (8:1617)
 static void advance_time(gfsph_ctx *c, int nsteps) {
-  if (c->adapt) return;
+  if (c->adapt || c->cases) return;
   for (int r = 0; r < c->R; r++)
     for (int k = 0; k < nsteps; k++) c->time_h[r] = c->time_h[r] + c->dt_h[r];
 }
@@ -3019,6 +3180,7 @@ static int enqueue_step(gfsph_ctx *c, int src, int dst) {
   launch_pforce(c, s);
   launch_stages(c, s);
   launch_velocity(c, s);
+  enqueue_case_end(c, dst);
   return GFSPH_OK;
 }
 
@@ -3231,6 +3393,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   HIPCHK(c, hipStreamSynchronize(st));
   c->time_h.assign(R, 0.0);
   c->dt_h.assign(R, p->dt);
+  c->counts_h.assign(counts, counts + R);
   c->dxg = sqrt(dem->dx * dem->dx + dem->dy * dem->dy);  // dxg_mod (6:6566); flang lowers **0.5 to sqrt
   return GFSPH_OK;
 }
@@ -3401,7 +3564,7 @@ int gfsph_mc_pixel_accumulate(gfsph_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->mc_acc, 0, sizeof(double) * 4 * np, c->stream));
     HIPCHK(c, hipMemsetAsync(c->mc_cnt, 0, sizeof(int) * np, c->stream));
   }
-  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), 0, c->mc_acc, c->mc_cnt);
+  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), 0, c->mc_acc, c->mc_cnt, nullptr);
   HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
 }
@@ -3426,6 +3589,7 @@ int gfsph_mc_pixel_read(gfsph_ctx *c, double *acc, int32_t *cnt, int32_t reset) 
 
 int gfsph_profile_step(gfsph_ctx *c, int32_t nrep, int32_t cap, int32_t *nkernels, const char **names, double *ms) {
   if (!c || nrep <= 0 || !nkernels) return GFSPH_ERR_ARG;
+  if (c->cases) return fail(c, GFSPH_ERR_UNSUPPORTED, "per-kernel profiling runs without case control");
   static const char *kn[] = {"neighbour_build", "k_density", "k_pforce", "k_rk", "k_xsph", "step_total"};
   const int K = 6;
   *nkernels = K;
@@ -3485,7 +3649,8 @@ int gfsph_batch_pixel_accumulate(gfsph_ctx *c) {
   int rc = ensure_mcr(c);
   if (rc) return rc;
   const long np = (long)c->dem.nx * c->dem.ny;
-  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), np, c->mcr_acc, c->mcr_cnt);
+  k_mc_pixel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), np, c->mcr_acc, c->mcr_cnt,
+                                                      nullptr);
   HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
 }
@@ -3670,6 +3835,7 @@ static int adjoint_supported(gfsph_ctx *c) {
   const DevParams &P = c->P;
   if (c->R != 1) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs on a single realisation");
   if (c->adapt) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs with a fixed dt_sph (ic_adapt_dt_sph = 0)");
+  if (c->cases) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without case control");
   if (P.nfric != 5 || P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 ||
       P.xi_voellmy < 0.0)
     return fail(c, GFSPH_ERR_UNSUPPORTED,
@@ -3821,28 +3987,42 @@ int gfsph_counts(gfsph_ctx *c, int32_t *n, int32_t *nactive) {
   return GFSPH_OK;
 }
 
+// time_sph and dt_sph live on the device while the time step adapts or the
+// case control runs (they then change inside the step graphs), else on the host
+static bool device_time(const gfsph_ctx *c) { return c->adapt || c->cases; }
+
+static int time_to_device(gfsph_ctx *c) {
+  const int R = c->R;
+  int rc;
+  if (!c->tm && ((rc = dalloc(c, &c->tm, R)) || (rc = dalloc(c, &c->dtm, R)))) return rc;
+  std::vector<unsigned long long> init(R, DT_INIT);
+  HIPCHK(c, hipMemcpy(c->tm, c->time_h.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->dtm, init.data(), sizeof(unsigned long long) * R, hipMemcpyHostToDevice));
+  return GFSPH_OK;
+}
+
+static int time_to_host(gfsph_ctx *c) {
+  const int R = c->R;
+  std::vector<DevParams> ph(R);
+  HIPCHK(c, hipMemcpy(c->time_h.data(), c->tm, sizeof(double) * R, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ph.data(), c->Pr, sizeof(DevParams) * R, hipMemcpyDeviceToHost));
+  for (int r = 0; r < R; r++) c->dt_h[r] = ph[r].dt;
+  c->P.dt = ph[0].dt;
+  c->P.fin = ph[0].fin;
+  return GFSPH_OK;
+}
+
 int gfsph_set_adaptive_dt(gfsph_ctx *c, int32_t mode) {
   if (!c || c->n <= 0) return GFSPH_ERR_ARG;
   if (mode != 0 && mode != 1 && mode != 2 && mode != 4 && mode != 10)
     return fail(c, GFSPH_ERR_UNSUPPORTED,
                 "ic_adapt_dt_sph must be 0, 1, 2, 4 or 10 (3, pair-based, and 7, one-dimensional, are not provided)");
   if (mode == c->adapt) return GFSPH_OK;
-  const int R = c->R;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int rc;
-  if (!c->tm && ((rc = dalloc(c, &c->tm, R)) || (rc = dalloc(c, &c->dtm, R)))) return rc;
-  if (!c->adapt) {
-    std::vector<unsigned long long> init(R, DT_INIT);
-    HIPCHK(c, hipMemcpy(c->tm, c->time_h.data(), sizeof(double) * R, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->dtm, init.data(), sizeof(unsigned long long) * R, hipMemcpyHostToDevice));
-  } else if (!mode) {
-    std::vector<DevParams> ph(R);
-    HIPCHK(c, hipMemcpy(c->time_h.data(), c->tm, sizeof(double) * R, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(ph.data(), c->Pr, sizeof(DevParams) * R, hipMemcpyDeviceToHost));
-    for (int r = 0; r < R; r++) c->dt_h[r] = ph[r].dt;
-    c->P.dt = ph[0].dt;
-    c->P.fin = ph[0].fin;
-  }
+  const bool before = device_time(c);
+  if (!before && mode && (rc = time_to_device(c))) return rc;
+  if (before && !mode && !c->cases && (rc = time_to_host(c))) return rc;
   drop_graphs(c);
   c->adapt = mode;
   return GFSPH_OK;
@@ -3852,7 +4032,13 @@ int gfsph_set_time(gfsph_ctx *c, double time) {
   if (!c || c->n <= 0) return GFSPH_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->time_h.assign(c->R, time);
-  if (c->adapt) HIPCHK(c, hipMemcpy(c->tm, c->time_h.data(), sizeof(double) * c->R, hipMemcpyHostToDevice));
+  if (device_time(c)) HIPCHK(c, hipMemcpy(c->tm, c->time_h.data(), sizeof(double) * c->R, hipMemcpyHostToDevice));
+  if (c->cases) {
+    std::vector<CaseState> h(c->R);
+    HIPCHK(c, hipMemcpy(h.data(), c->cs, sizeof(CaseState) * c->R, hipMemcpyDeviceToHost));
+    for (auto &q : h) q.tglob = time;
+    HIPCHK(c, hipMemcpy(c->cs, h.data(), sizeof(CaseState) * c->R, hipMemcpyHostToDevice));
+  }
   return GFSPH_OK;
 }
 
@@ -3860,7 +4046,7 @@ int gfsph_time(gfsph_ctx *c, double *time, double *dt) {
   if (!c || c->n <= 0) return GFSPH_ERR_ARG;
   const int R = c->R;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (!c->adapt) {
+  if (!device_time(c)) {
     if (time) memcpy(time, c->time_h.data(), sizeof(double) * R);
     if (dt) memcpy(dt, c->dt_h.data(), sizeof(double) * R);
     return GFSPH_OK;
@@ -3870,6 +4056,62 @@ int gfsph_time(gfsph_ctx *c, double *time, double *dt) {
     std::vector<DevParams> ph(R);
     HIPCHK(c, hipMemcpy(ph.data(), c->Pr, sizeof(DevParams) * R, hipMemcpyDeviceToHost));
     for (int r = 0; r < R; r++) dt[r] = ph[r].dt;
+  }
+  return GFSPH_OK;
+}
+
+int gfsph_batch_cases(gfsph_ctx *c, const gfsph_case_ctl *ctl) {
+  if (!c || c->n <= 0 || !ctl) return GFSPH_ERR_ARG;
+  if (c->P.nfric == 4) return fail(c, GFSPH_ERR_UNSUPPORTED, "case control does not cover Bingham (nfric = 4)");
+  if (!(ctl->time_plot >= 0.0) || !(ctl->time_end > 0.0))
+    return fail(c, GFSPH_ERR_ARG, "time_plot must be >= 0 and time_end > 0");
+  const int R = c->R;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc;
+  if (!c->cs && ((rc = dalloc(c, &c->cs, R)) || (rc = dalloc(c, &c->stopr, R)) || (rc = dalloc(c, &c->due, R)) ||
+                 (rc = dalloc(c, &c->outc, R)) || (rc = dalloc(c, &c->cnt_r, R))))
+    return rc;
+  if (ctl->pixels && (rc = ensure_mcr(c))) return rc;
+  if (!device_time(c) && (rc = time_to_device(c))) return rc;
+  // loop state of a fresh case (8:1459-1462, 8:104-106): dt_max = dt is
+  // not used by Driver_Flow_23; t_plot_reset = 0; time = time_sph
+  std::vector<double> t0(R);
+  HIPCHK(c, hipMemcpy(t0.data(), c->tm, sizeof(double) * R, hipMemcpyDeviceToHost));
+  std::vector<CaseState> h(R);
+  std::vector<int> zero(R, 0);
+  for (int r = 0; r < R; r++) {
+    h[r] = CaseState{};
+    h[r].tglob = t0[r];
+  }
+  HIPCHK(c, hipMemcpy(c->cs, h.data(), sizeof(CaseState) * R, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->stopr, zero.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->due, zero.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->outc, zero.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->cnt_r, c->counts_h.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+  c->cctl = CaseCtl{ctl->vstop, ctl->timestop, ctl->dtimestop, ctl->dt, ctl->time_end, ctl->time_plot};
+  c->case_pixels = ctl->pixels ? 1 : 0;
+  c->cases = 1;
+  drop_graphs(c);
+  return GFSPH_OK;
+}
+
+int gfsph_batch_status(gfsph_ctx *c, int32_t *stopped, int32_t *nsteps, int32_t *nstopped) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  if (!c->cases) return fail(c, GFSPH_ERR_ARG, "case control is not enabled (gfsph_batch_cases)");
+  const int R = c->R;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int> st(R);
+  HIPCHK(c, hipMemcpy(st.data(), c->stopr, sizeof(int) * R, hipMemcpyDeviceToHost));
+  if (stopped) memcpy(stopped, st.data(), sizeof(int) * R);
+  if (nsteps) {
+    std::vector<CaseState> h(R);
+    HIPCHK(c, hipMemcpy(h.data(), c->cs, sizeof(CaseState) * R, hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; r++) nsteps[r] = h[r].nsteps;
+  }
+  if (nstopped) {
+    int k = 0;
+    for (int v : st) k += v ? 1 : 0;
+    *nstopped = k;
   }
   return GFSPH_OK;
 }

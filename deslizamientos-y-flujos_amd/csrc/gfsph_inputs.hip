@@ -537,10 +537,21 @@ struct SrcRaw {  // Read_Source_plus_New after reading: shifted points and the r
   int nbx = 0, nby = 0;
 };
 
+// case table of MASTER.dat (Driver_Input, "8 SPH_FEM_Driver_2019.f90":475-639)
+struct CaseTable {
+  int mode = 0;                              // ic_cases_mat (0: none, 1 lines, 2 full factorial)
+  int ncases = 0, ninfo = 0;
+  std::vector<std::vector<double>> C;        // Ccases_mat rows (10 columns; unread ones 0)
+  std::vector<std::vector<double>> list;     // List_Vals rows of statistic-list variables (law >= 100)
+  double vstop = 0, timestop = 0, dtimestop = 0;
+  std::vector<std::vector<double>> OC;       // OC_mat: [nproblems][3 + ncases + 2]
+};
+
 struct gfsph_inputs {
   std::string err;
   gfsph_params P{};
   gfsph_run run{};
+  CaseTable cases;
   gfsph_grid grid{};
   TopoRaw topo;
   SrcRaw src;
@@ -566,6 +577,104 @@ int ifail(gfsph_inputs *in, int code, const std::string &m) {
   } while (0)
 
 // Driver_Input subset (8:229-340): SPH only, no GFL, no case tables
+// Case tables and the problem matrix OC_mat (8:475-639): mode 1 varies one
+// variable at a time over its values (the others at their defaults), mode 2
+// takes every combination (variable 1 fastest); values are min + (max -
+// min)(i - 1)/(n - 1), on a log scale for law 2 (and, in mode 1, for const 13
+// and 27), or the i-th entry of a statistic list (law >= 100, mode 2).
+int read_case_table(gfsph_inputs *in, FReader &R, const std::string &path) {
+  CaseTable &T = in->cases;
+  std::vector<long> iv;
+  std::vector<double> rv;
+  RD(R.text());
+  RD(R.ints(iv, 2));
+  T.ncases = (int)iv[0];
+  T.ninfo = (int)iv[1];
+  if (T.ncases < 1 || T.ninfo < 5 || T.ninfo > 10)
+    return ifail(in, GFSPH_ERR_ARG, path + ": ncases_mat >= 1 and 5 <= ninfo_mat <= 10 expected");
+  RD(R.text());
+  T.C.assign(T.ncases, std::vector<double>(10, 0.0));
+  for (int c = 0; c < T.ncases; c++) {
+    RD(R.reals(rv, T.ninfo));
+    for (int k = 0; k < T.ninfo; k++) T.C[c][k] = rv[k];
+  }
+  T.list.assign(T.ncases, std::vector<double>());
+  if (T.mode == 2) {
+    bool any = false;
+    for (int c = 0; c < T.ncases; c++) any = any || T.C[c][5] >= 100;
+    if (any) {
+      RD(R.text());
+      for (int c = 0; c < T.ncases; c++) {
+        if (!(T.C[c][5] >= 100)) continue;
+        const long nval = (long)T.C[c][2];
+        RD(R.reals(rv, nval + 1));
+        T.list[c] = rv;
+        RD(R.reals(rv, nval + 1));  // List_Probs: not used by the deterministic table
+      }
+    }
+  }
+  RD(R.text());
+  RD(R.reals(rv, 3));
+  T.vstop = rv[0];
+  T.timestop = rv[1];
+  T.dtimestop = rv[2];
+  const int ncol = 3 + T.ncases + 2;
+  T.OC.clear();
+  auto value = [&](int c, long ival, int col_lo, bool logs) {  // min + dval*(ival-1), log scales
+    double mn = T.C[c][col_lo], mx = T.C[c][col_lo + 1];
+    const long nval = (long)T.C[c][2];
+    if (logs) {
+      mn = log(mn);
+      mx = log(mx);
+    }
+    const double dv = nval == 1 ? 0.0 : (mx - mn) / (nval - 1);
+    double v = mn + dv * (ival - 1);
+    if (logs) v = exp(v);
+    return v;
+  };
+  if (T.mode == 1) {
+    for (int c = 0; c < T.ncases; c++) {
+      const int indx = (int)T.C[c][0];
+      const long nval = (long)T.C[c][2];
+      for (long ival = 1; ival <= nval; ival++) {
+        std::vector<double> row(ncol, 0.0);
+        row[0] = c + 1;
+        row[1] = (double)ival;
+        row[2] = indx;
+        const double v = value(c, ival, 3, indx == 13 || indx == 27);
+        for (int j = 0; j < T.ncases; j++) row[3 + j] = (j == c) ? v : T.C[j][1];
+        T.OC.push_back(row);
+      }
+    }
+  } else {
+    long np = 1;
+    std::vector<long> stride(T.ncases);
+    for (int c = 0; c < T.ncases; c++) {
+      const long nval = (long)T.C[c][2];
+      if (nval < 1) return ifail(in, GFSPH_ERR_ARG, path + ": every case variable needs at least one value");
+      stride[c] = np;
+      np *= nval;
+    }
+    if (np > 1000000) return ifail(in, GFSPH_ERR_ARG, path + ": more than 10^6 problems");
+    for (long ip = 0; ip < np; ip++) {
+      std::vector<double> row(ncol, 0.0);
+      for (int c = 0; c < T.ncases; c++) {
+        const long ival = (ip / stride[c]) % (long)T.C[c][2] + 1;
+        const int law = (int)T.C[c][5];
+        if (law >= 100) {
+          if ((long)T.list[c].size() < ival + 1) return ifail(in, GFSPH_ERR_ARG, path + ": statistic list too short");
+          row[3 + c] = T.list[c][ival];
+        } else {
+          row[3 + c] = value(c, ival, 3, law == 2);
+          if ((int)T.C[c][0] == 521) row[3 + T.ncases + 1] = value(c, ival, 6, false);
+        }
+      }
+      T.OC.push_back(row);
+    }
+  }
+  return GFSPH_OK;
+}
+
 int read_master(gfsph_inputs *in, const std::string &path) {
   FReader R;
   if (!R.open(path)) return GFSPH_OK;  // optional
@@ -576,8 +685,13 @@ int read_master(gfsph_inputs *in, const std::string &path) {
   RD(R.text());
   RD(R.ints(iv, 3));
   const long if_sph = iv[0], if_gfl = iv[1], cases = iv[2];
-  if (if_sph != 1 || if_gfl != 0 || cases != 0)
-    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": only if_sph = 1, if_gfl = 0, ic_cases_mat = 0 runs are read");
+  if (if_sph != 1 || if_gfl != 0)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": only if_sph = 1, if_gfl = 0 runs are read");
+  if (cases != 0 && cases != 1 && cases != 2)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED,
+                 path + ": ic_cases_mat " + std::to_string(cases) +
+                     " (0, 1: one variable at a time, 2: all combinations are read; the sampled sets are not)");
+  in->cases.mode = (int)cases;
   RD(R.text());
   RD(R.ints(iv, 2));
   if (iv[0] != 1) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": SPH_ProblemType must be 1 (shallow water)");
@@ -605,6 +719,19 @@ int read_master(gfsph_inputs *in, const std::string &path) {
     return ifail(in, GFSPH_ERR_ARG, path + ": dt_sph, ic_adapt_dt_sph expected");
   in->run.adapt_dt = atoi(t[1].c_str());
   in->run.present = 1;
+  // time curves (8:335-357)
+  RD(R.text());
+  RD(R.ints(iv, 2));
+  const long ntc = iv[0];
+  for (long k = 0; k < ntc; k++) {
+    RD(R.text());
+    RD(R.ints(iv, 1));
+    const long np = iv[0];
+    RD(R.text());
+    RD(R.reals(rv, np));
+    RD(R.reals(rv, np));
+  }
+  if (cases) return read_case_table(in, R, path);
   return GFSPH_OK;
 }
 
@@ -1227,6 +1354,133 @@ int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out) {
     }
   }
   if (in->run.time_restart != 0.0) gfsph_set_time(*out, in->run.time_restart);
+  return GFSPH_OK;
+}
+
+int gfsph_inputs_cases(gfsph_inputs *in, int32_t *mode, int32_t *ncases, int32_t *nproblems, double *table,
+                       double *stop3) {
+  if (!in) return GFSPH_ERR_ARG;
+  const CaseTable &T = in->cases;
+  if (mode) *mode = T.mode;
+  if (ncases) *ncases = T.ncases;
+  if (nproblems) *nproblems = (int32_t)T.OC.size();
+  if (table)
+    for (size_t p = 0; p < T.OC.size(); p++)
+      memcpy(table + p * T.OC[p].size(), T.OC[p].data(), sizeof(double) * T.OC[p].size());
+  if (stop3) {
+    stop3[0] = T.vstop;
+    stop3[1] = T.timestop;
+    stop3[2] = T.dtimestop;
+  }
+  return GFSPH_OK;
+}
+
+// Every problem of the case table as one realisation of a batch: the release
+// and const(:) changed as Input_SW changes them for the problem
+// (6:1040-1096 particles: 501 depth and mass factor, 511 expansion about the
+// centroid, 521 shift; 6:1208-1232 const(:), index 2 read as a porosity when
+// <= 1.01), ic_adapt_dt_sph, and the case control of the loop the reference
+// runs for each problem (Driver_Flow for mode 1: no plot-time cut;
+// Driver_Flow_23 for mode 2: cut at every plot_step*dt).
+int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out) {
+  if (!in || !out) return GFSPH_ERR_ARG;
+  *out = nullptr;
+  const CaseTable &T = in->cases;
+  if (!in->built || in->n <= 0) return ifail(in, GFSPH_ERR_ARG, "inputs not built (gfsph_inputs_read)");
+  if (T.mode == 0 || T.OC.empty()) return ifail(in, GFSPH_ERR_ARG, "MASTER.dat holds no case table");
+  if (!(in->P.dt > 0)) return ifail(in, GFSPH_ERR_ARG, "dt_sph is not set");
+  const int R = (int)T.OC.size(), n0 = in->n;
+  const long ntot = (long)R * n0;
+  if (ntot > 0x7fffffffL) return ifail(in, GFSPH_ERR_ARG, "too many particles over all problems");
+  std::vector<double> cst((size_t)R * 111), x(2 * ntot), v(2 * ntot), h(ntot), mass(ntot), hsml(ntot), u(ntot);
+  std::vector<int32_t> counts(R, n0);
+  for (int p = 0; p < R; p++) {
+    double *c = &cst[(size_t)p * 111];
+    memcpy(c, in->P.cst, sizeof(double) * 111);
+    double *px = &x[2L * p * n0], *pv = &v[2L * p * n0], *ph = &h[(long)p * n0], *pm = &mass[(long)p * n0];
+    memcpy(px, in->x.data(), sizeof(double) * 2 * n0);
+    memcpy(pv, in->v.data(), sizeof(double) * 2 * n0);
+    memcpy(ph, in->h.data(), sizeof(double) * n0);
+    memcpy(pm, in->mass.data(), sizeof(double) * n0);
+    memcpy(&hsml[(long)p * n0], in->hsml.data(), sizeof(double) * n0);
+    memcpy(&u[(long)p * n0], in->u.data(), sizeof(double) * n0);
+    for (int ic = 0; ic < T.ncases; ic++) {
+      if (T.mode == 1 && ic != (int)T.OC[p][0] - 1) continue;  // mode 1: the varied variable only
+      const int idx = (int)(T.C[ic][0] + 0.01);
+      const double cv = T.OC[p][3 + ic];
+      if (idx == 501) {
+        for (int i = 0; i < n0; i++) {
+          ph[i] = ph[i] * cv;
+          pm[i] = pm[i] * cv;
+        }
+      } else if (idx == 511) {
+        double cg[2] = {0.0, 0.0};
+        for (int i = 0; i < n0; i++) {
+          cg[0] = cg[0] + px[2 * i];
+          cg[1] = cg[1] + px[2 * i + 1];
+          if (T.mode == 1) pm[i] = pm[i] * cv;  // the mode-1 branch scales the masses in both loops (6:1094-1101)
+        }
+        cg[0] = cg[0] / n0;
+        cg[1] = cg[1] / n0;
+        for (int i = 0; i < n0; i++) {
+          px[2 * i] = cg[0] + cv * (px[2 * i] - cg[0]);
+          px[2 * i + 1] = cg[1] + cv * (px[2 * i + 1] - cg[1]);
+          pm[i] = pm[i] * cv;
+        }
+      } else if (idx == 521) {
+        const double dx = cv, dy = T.OC[p][3 + T.ncases + 1];
+        for (int i = 0; i < n0; i++) {
+          px[2 * i] = px[2 * i] + dx;
+          px[2 * i + 1] = px[2 * i + 1] + dy;
+        }
+      } else if (idx == 401) {
+        return ifail(in, GFSPH_ERR_UNSUPPORTED, "case variable 401 (initial pore-pressure factor)");
+      } else if (idx >= 1 && idx <= 110) {
+        if (T.mode == 2 && idx == 2 && cv <= 1.01) {  // porosity given: mixture density (6:1216-1226)
+          if (!(c[17] >= 1.e-4))
+            return ifail(in, GFSPH_ERR_ARG, "a porosity for const(2) needs the solid and water densities const(17, 18)");
+          c[2] = (double)(float)((1. - cv) * c[17] + cv * c[18]);
+        } else {
+          c[idx] = (double)(float)cv;
+        }
+      } else {
+        return ifail(in, GFSPH_ERR_UNSUPPORTED, "case variable index " + std::to_string(idx));
+      }
+    }
+  }
+  gfsph_dem d;
+  d.nx = in->grid.nx;
+  d.ny = in->grid.ny;
+  d.xmin = in->grid.xmin;
+  d.ymin = in->grid.ymin;
+  d.dx = in->grid.dx;
+  d.dy = in->grid.dy;
+  d.xmax = in->grid.xmax;
+  d.ymax = in->grid.ymax;
+  d.Z = in->Z.data();
+  d.in_model = in->in_model.data();
+  d.topol = in->topol.empty() ? nullptr : in->topol.data();
+  int rc = gfsph_batch_create(&in->P, &d, R, counts.data(), cst.data(), x.data(), v.data(), h.data(), mass.data(),
+                              hsml.data(), u.data(), out);
+  auto bail = [&](const char *what) {
+    in->err = std::string(what) + ": " + (*out ? gfsph_last_error(*out) : "failed");
+    if (*out) gfsph_destroy(*out);
+    *out = nullptr;
+    return rc;
+  };
+  if (rc) return bail("gfsph_batch_create");
+  if (in->run.present && in->run.adapt_dt && (rc = gfsph_set_adaptive_dt(*out, in->run.adapt_dt)))
+    return bail("gfsph_set_adaptive_dt");
+  gfsph_case_ctl ctl;
+  ctl.vstop = T.vstop;
+  ctl.timestop = T.timestop;
+  ctl.dtimestop = T.dtimestop;
+  ctl.dt = in->run.dt;
+  ctl.time_end = in->run.time_end > 0 ? in->run.time_end : 1.e30;
+  ctl.time_plot = T.mode == 2 ? in->run.plot_step * in->run.dt : 0.0;
+  ctl.pixels = pixels && T.mode == 2 ? 1 : 0;
+  ctl.pad_ = 0;
+  if ((rc = gfsph_batch_cases(*out, &ctl))) return bail("gfsph_batch_cases");
   return GFSPH_OK;
 }
 

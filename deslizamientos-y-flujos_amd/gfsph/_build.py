@@ -11,8 +11,8 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libgfsph.so")
-# the propagation step and the reference's input path
-SOURCES = ("gfsph.hip", "gfsph_inputs.hip")
+# the propagation step, the reference's input path, raster output
+SOURCES = ("gfsph.hip", "gfsph_inputs.hip", "gfsph_output.hip")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

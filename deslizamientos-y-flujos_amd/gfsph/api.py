@@ -17,7 +17,7 @@ import numpy as np
 from ._build import LIB
 
 __all__ = ["GfsphError", "SPHModel", "BatchModel", "AdjointModel", "neighbour_pairs", "load_library",
-           "EXPORTED_SYMBOLS"]
+           "write_netcdf", "EXPORTED_SYMBOLS"]
 
 EXPORTED_SYMBOLS = (
     "gfsph_create", "gfsph_step", "gfsph_sync", "gfsph_get_state", "gfsph_neighbour_pairs",
@@ -26,7 +26,8 @@ EXPORTED_SYMBOLS = (
     "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune", "gfsph_adjoint_forward",
     "gfsph_adjoint_backward", "gfsph_raster_misfit", "gfsph_set_adaptive_dt", "gfsph_time", "gfsph_set_time",
     "gfsph_inputs_parse", "gfsph_inputs_read", "gfsph_inputs_info", "gfsph_inputs_dem", "gfsph_inputs_particles",
-    "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy",
+    "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy", "gfsph_write_netcdf",
+    "gfsph_batch_cases", "gfsph_batch_status", "gfsph_inputs_cases", "gfsph_inputs_batch_create",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -65,6 +66,11 @@ class CGrid(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("xmin", C.c_double), ("ymin", C.c_double),
                 ("dx", C.c_double), ("dy", C.c_double), ("xmax", C.c_double), ("ymax", C.c_double),
                 ("xshift", C.c_double), ("yshift", C.c_double), ("fdiff", C.c_int32), ("pad_", C.c_int32)]
+
+
+class CCaseCtl(C.Structure):
+    _fields_ = [("vstop", C.c_double), ("timestop", C.c_double), ("dtimestop", C.c_double), ("dt", C.c_double),
+                ("time_end", C.c_double), ("time_plot", C.c_double), ("pixels", C.c_int32), ("pad_", C.c_int32)]
 
 
 _lib = None
@@ -107,6 +113,10 @@ def load_library(path: str | None = None):
     L.gfsph_set_adaptive_dt.argtypes = [vp, C.c_int32]
     L.gfsph_time.argtypes = [vp, dp, dp]
     L.gfsph_set_time.argtypes = [vp, C.c_double]
+    L.gfsph_batch_cases.argtypes = [vp, C.POINTER(CCaseCtl)]
+    L.gfsph_inputs_cases.argtypes = [vp, ip, ip, ip, dp, dp]
+    L.gfsph_inputs_batch_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    L.gfsph_batch_status.argtypes = [vp, ip, ip, ip]
     for f in (L.gfsph_inputs_parse, L.gfsph_inputs_read):
         f.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(vp)]
     L.gfsph_inputs_info.argtypes = [vp, C.POINTER(CParams), C.POINTER(CRun), C.POINTER(CGrid), ip]
@@ -117,9 +127,28 @@ def load_library(path: str | None = None):
     L.gfsph_inputs_error.restype = C.c_char_p
     L.gfsph_inputs_destroy.argtypes = [vp]
     L.gfsph_inputs_destroy.restype = None
+    cpp = C.POINTER(C.c_char_p)
+    L.gfsph_write_netcdf.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int32, cpp, cpp, cpp, C.POINTER(dp), C.c_double, C.c_char_p]
     if path is None:
         _lib = L
     return L
+
+
+def write_netcdf(path: str, dem, rasters: dict, title: str = "", fill: float = float("nan")) -> None:
+    """Rasters on the terrain grid of ``dem`` (xmin, ymin, dx, dy, nx, ny) to
+    a NetCDF classic file.  ``rasters``: name -> (array [ny, nx], units,
+    long_name)."""
+    L = load_library()
+    names = list(rasters)
+    arrs = [np.ascontiguousarray(np.asarray(rasters[k][0], dtype=np.float64).reshape(dem.ny, dem.nx)) for k in names]
+    cs = lambda xs: (C.c_char_p * len(xs))(*[x.encode() for x in xs])  # noqa: E731
+    dps = (C.POINTER(C.c_double) * len(arrs))(*[_dp(a) for a in arrs])
+    rc = L.gfsph_write_netcdf(path.encode(), dem.nx, dem.ny, dem.xmin, dem.ymin, dem.dx, dem.dy, len(names),
+                              cs(names), cs([rasters[k][1] for k in names]), cs([rasters[k][2] for k in names]), dps,
+                              fill, title.encode())
+    if rc != 0:
+        raise GfsphError(rc, f"writing {path}: invalid names or sizes, or the file cannot be written")
 
 
 def _dp(a):
@@ -274,6 +303,41 @@ class SPHModel:
             return float(t[0]), float(dt[0])
         return t, dt
 
+    def pixel_read(self, r: int = 0):
+        """Per-realisation pixel accumulators (batch accumulation or case
+        control plot times) of realisation r."""
+        npg = self.case.dem.nx * self.case.dem.ny
+        acc = np.zeros((npg, 4))
+        cnt = np.zeros(npg, np.int32)
+        self._check(self.lib.gfsph_batch_pixel_read(self.ctx, int(r), _dp(acc), _ip(cnt)))
+        return acc, cnt
+
+    # --- case control (Driver_Flow_23 + Stop_case_SW per realisation) ---
+    def case_control(self, vstop: float, timestop: float, dtimestop: float, dt: float, time_end: float,
+                     time_plot: float, pixels: bool = True) -> None:
+        ctl = CCaseCtl(vstop, timestop, dtimestop, dt, time_end, time_plot, int(pixels), 0)
+        self._check(self.lib.gfsph_batch_cases(self.ctx, C.byref(ctl)))
+
+    def case_status(self):
+        """(stopped[R], itimestep[R], number stopped)"""
+        st = np.zeros(self.R, np.int32)
+        it = np.zeros(self.R, np.int32)
+        k = C.c_int32()
+        self._check(self.lib.gfsph_batch_status(self.ctx, _ip(st), _ip(it), C.byref(k)))
+        return st, it, k.value
+
+    def run_cases(self, max_steps: int, chunk: int = 64) -> int:
+        """Steps until every realisation has stopped or max_steps loop
+        iterations have run; returns the iterations run."""
+        done = 0
+        while done < max_steps:
+            k = min(chunk, max_steps - done)
+            self.time_integration_sph(k)
+            done += k
+            if self.case_status()[2] == self.R:
+                break
+        return done
+
     def counts(self):
         n, na = C.c_int32(), C.c_int32()
         self._check(self.lib.gfsph_counts(self.ctx, C.byref(n), C.byref(na)))
@@ -327,6 +391,19 @@ class BatchModel(SPHModel):
         self.nsteps_done = 0
         self._apply_time_control(self.case.params)
 
+    @classmethod
+    def adopt(cls, lib, ctx, cases):
+        """A batch around a context created elsewhere (gfsph_inputs_batch_create)."""
+        b = cls.__new__(cls)
+        b.lib, b.ctx, b.cases = lib, ctx, list(cases)
+        b.case = b.cases[0]
+        b.R = len(b.cases)
+        b.sizes = np.array([c.particles.n for c in b.cases], dtype=np.int32)
+        b.n = int(b.sizes.sum())
+        b._dem = None
+        b.nsteps_done = 0
+        return b
+
     def split(self, st: dict) -> list:
         edges = np.concatenate([[0], np.cumsum(self.sizes)])
         return [{k: v[edges[r]:edges[r + 1]] for k, v in st.items()} for r in range(self.R)]
@@ -334,12 +411,6 @@ class BatchModel(SPHModel):
     def pixel_accumulate(self) -> None:
         self._check(self.lib.gfsph_batch_pixel_accumulate(self.ctx))
 
-    def pixel_read(self, r: int):
-        npg = self.case.dem.nx * self.case.dem.ny
-        acc = np.zeros((npg, 4))
-        cnt = np.zeros(npg, np.int32)
-        self._check(self.lib.gfsph_batch_pixel_read(self.ctx, int(r), _dp(acc), _ip(cnt)))
-        return acc, cnt
 
     def hazard(self, hthr: float = 0.5) -> dict:
         npg = self.case.dem.nx * self.case.dem.ny
@@ -350,6 +421,23 @@ class BatchModel(SPHModel):
         self._check(self.lib.gfsph_batch_hazard(self.ctx, float(hthr), _dp(stats), _ip(nprob), _dp(maxs), _dp(pexc)))
         return {"mean": stats[:, :, 0], "std": stats[:, :, 1], "nprob": nprob, "max_h": maxs[:, 0],
                 "max_v": maxs[:, 1], "p_exceed": pexc}
+
+    def write_hazard(self, path: str, hthr: float = 0.5, title: str = "") -> dict:
+        """hazard(hthr) as a NetCDF classic file: mean and standard deviation
+        over the realisations of the average depth and speed at each node,
+        maxima, the number of realisations reaching the node and P(max h >
+        hthr)."""
+        hz = self.hazard(hthr)
+        r = {"mean_h": (hz["mean"][:, 0], "m", "mean over realisations of the time-averaged depth"),
+             "std_h": (hz["std"][:, 0], "m", "standard deviation over realisations of the time-averaged depth"),
+             "mean_v": (hz["mean"][:, 2], "m s-1", "mean over realisations of the time-averaged speed"),
+             "std_v": (hz["std"][:, 2], "m s-1", "standard deviation over realisations of the time-averaged speed"),
+             "max_h": (hz["max_h"], "m", "maximum depth over realisations and plot times"),
+             "max_v": (hz["max_v"], "m s-1", "maximum speed over realisations and plot times"),
+             "n_reached": (hz["nprob"].astype(np.float64), "1", "number of realisations reaching the node"),
+             "p_exceed": (hz["p_exceed"], "1", f"probability that the maximum depth exceeds {hthr} m")}
+        write_netcdf(path, self.case.dem, r, title or f"Monte Carlo hazard, {self.R} realisations")
+        return hz
 
 
 class AdjointModel(SPHModel):

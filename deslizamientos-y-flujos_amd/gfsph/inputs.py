@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import cases as Cs
-from .api import CGrid, CParams, CRun, GfsphError, SPHModel, _dp, _ip, load_library
+from .api import BatchModel, CGrid, CParams, CRun, GfsphError, SPHModel, _dp, _ip, load_library
 from .dem import DEM
 
 __all__ = ["Inputs"]
@@ -103,6 +103,26 @@ class Inputs:
         ctx = C.c_void_p()
         self._check(self.lib.gfsph_inputs_create(self.h, C.byref(ctx)))
         return SPHModel.adopt(self.lib, ctx, self.case())
+
+    def cases(self) -> dict:
+        """MASTER.dat's case table: mode (ic_cases_mat), OC_mat
+        [nproblems, 3 + ncases + 2] and the stop thresholds."""
+        mode, nc, npb = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.gfsph_inputs_cases(self.h, C.byref(mode), C.byref(nc), C.byref(npb), None, None))
+        table = np.zeros((npb.value, 3 + nc.value + 2))
+        stop = np.zeros(3)
+        self._check(self.lib.gfsph_inputs_cases(self.h, None, None, None, _dp(table), _dp(stop)))
+        return {"mode": mode.value, "ncases": nc.value, "table": table, "vstop": stop[0], "timestop": stop[1],
+                "dtimestop": stop[2]}
+
+    def batch(self, pixels: bool = False) -> BatchModel:
+        """gfsph_inputs_batch_create: every problem of the case table as one
+        realisation, under the reference's per-problem loop (case control)."""
+        ctx = C.c_void_p()
+        self._check(self.lib.gfsph_inputs_batch_create(self.h, int(pixels), C.byref(ctx)))
+        base = self.case()
+        npb = self.cases()["table"].shape[0]
+        return BatchModel.adopt(self.lib, ctx, [base] * npb)
 
     def close(self) -> None:
         if getattr(self, "h", None) is not None and self.h.value:

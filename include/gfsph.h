@@ -198,6 +198,32 @@ int gfsph_set_adaptive_dt(gfsph_ctx *ctx, int32_t mode);
  * pointer may be NULL).  Synchronises. */
 int gfsph_time(gfsph_ctx *ctx, double *time, double *dt);
 
+/* Case control: every realisation of the context runs the step loop the
+ * reference runs for each case of a case table or Monte Carlo set
+ * (Driver_Flow_23, "8 SPH_FEM_Driver_2019.f90":1444-1548), on the device and
+ * independently: each step's dt_sph is cut so that the step ends on the next
+ * multiple of time_plot (8:1481-1488) and restored after it; at each plot time
+ * the realisation's pixels are accumulated (Get_MC_pixel_Vars_SW,
+ * "6 SPH_SW_2019.f90":20717-20833, if pixels); after each step Stop_case_SW
+ * (6:20114-20177: mean |vx| below vstop after 10 steps, more than a fifth of
+ * the particles out of the domain, time_sph beyond timestop, dt below
+ * dtimestop) or time >= time_end stops the realisation, which then no longer
+ * changes while the others continue.  gfsph_time reports each realisation's
+ * time_sph and dt_sph.  Resets the loop state (t_plot_reset = 0, itimestep =
+ * 0, none stopped). */
+typedef struct gfsph_case_ctl {
+  double vstop, timestop, dtimestop; /* vstop_cases, timestop_cases, dtimestop_cases */
+  double dt;                         /* the driver's dt (Driver_Input, 8:330)        */
+  double time_end;                   /* time_end                                     */
+  double time_plot;                  /* plot_step*dt; 0: no plot times               */
+  int32_t pixels;                    /* 1: pixel statistics at plot times            */
+  int32_t pad_;
+} gfsph_case_ctl;
+int gfsph_batch_cases(gfsph_ctx *ctx, const gfsph_case_ctl *ctl);
+/* Per realisation: stopped (0/1) and loop iterations done (itimestep); the
+ * number stopped.  Any pointer may be NULL.  Synchronises. */
+int gfsph_batch_status(gfsph_ctx *ctx, int32_t *stopped, int32_t *nsteps, int32_t *nstopped);
+
 /* Sets time_sph of every realisation, e.g. to time_sph_restart
  * ("6 SPH_SW_2019.f90":3302) when a run continues from restart.dat. */
 int gfsph_set_time(gfsph_ctx *ctx, double time);
@@ -260,8 +286,36 @@ int gfsph_inputs_particles(gfsph_inputs *in, double *x, double *v, double *h, do
 /* gfsph_create on the inputs (init_input = 1), then ic_adapt_dt_sph of
  * MASTER.dat and time_sph_restart. */
 int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out);
+/* The case table of MASTER.dat (ic_cases_mat 1: one variable at a time, 2:
+ * every combination; Driver_Input, "8 SPH_FEM_Driver_2019.f90":475-639):
+ * table is OC_mat, [nproblems][3 + ncases + 2] (case, value index, variable
+ * index, the value of every case variable, the y shift of variable 521);
+ * stop3 = vstop_cases, timestop_cases, dtimestop_cases.  Any pointer may be
+ * NULL (query the sizes first). */
+int gfsph_inputs_cases(gfsph_inputs *in, int32_t *mode, int32_t *ncases, int32_t *nproblems, double *table,
+                       double *stop3);
+/* All problems of the case table as one batch (replaces the problem loop of
+ * the reference's main program, 8:60-130): realisation p is the release and
+ * const(:) as Input_SW changes them for problem p (6:1040-1232: variables 501
+ * depth factor, 511 expansion, 521 shift, const indices; 401 is rejected),
+ * with ic_adapt_dt_sph and the case control (gfsph_batch_cases) of the loop
+ * the reference runs for each problem: Driver_Flow for mode 1, Driver_Flow_23
+ * (steps cut at every plot_step*dt; pixel statistics if pixels) for mode 2. */
+int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out);
 const char *gfsph_inputs_error(gfsph_inputs *in);
 void gfsph_inputs_destroy(gfsph_inputs *in);
+
+/* Rasters on the terrain grid to a NetCDF classic file (64-bit offset
+ * variant): dimensions x (nx), y (ny), coordinate variables x and y, and one
+ * double variable (y, x) per raster with CF attributes (units, long_name,
+ * _FillValue unless fill is NaN).  data[k] is [ny][nx]; names are NetCDF
+ * names (a letter, then letters, digits, '_'); each raster below 4 GiB.  No
+ * reference counterpart in this format: replaces the hazard-map text outputs
+ * written after the Monte Carlo loop ("6 SPH_SW_2019.f90":21200-21560) for
+ * the statistics of gfsph_batch_hazard. */
+int gfsph_write_netcdf(const char *path, int32_t nx, int32_t ny, double xmin, double ymin, double dx, double dy,
+                       int32_t nvar, const char *const *names, const char *const *units,
+                       const char *const *long_names, const double *const *data, double fill, const char *title);
 
 const char *gfsph_last_error(gfsph_ctx *ctx);
 void gfsph_destroy(gfsph_ctx *ctx);

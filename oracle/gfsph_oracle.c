@@ -742,6 +742,11 @@ static double adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const 
   return fmin(1.1 * dt, dtmin);
 }
 
+double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *v, double *rho,
+                       const double *hsml, const int32_t *flag, double dt) {
+  return adaptive_dt(p, dem, n, v, rho, hsml, flag, dt);
+}
+
 int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double *v, double *rho,
             const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
             int nsteps, double *tau0_io, double *dt_time_io) {

@@ -40,6 +40,10 @@ void gfo_fdiff_derivs(int nx, int ny, double dxg, double dyg, const int32_t *in_
                       double *topol);
 int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, double *h,
              const double *mass, const double *hsml, double *u, double *uaux, int32_t *flag);
+/* Adaptive_dt_sph_SW (6:6521-6664) alone: the next dt_sph from dt; clamps
+ * rho to h_inf_SW in place where the mode reads it */
+double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *v, double *rho,
+                       const double *hsml, const int32_t *flag, double dt);
 int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,
             const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
             int nsteps, double *tau0_io /* Bingham Tau0 carry, in/out; NULL = 0 */,

@@ -56,6 +56,8 @@ def lib():
                               C.c_int, dp, dp]
         L.gfo_raster_idw.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, dp, dp]
         L.gfo_mc_pixel_accum.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, dp, ip]
+        L.gfo_adaptive_dt.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, C.c_double]
+        L.gfo_adaptive_dt.restype = C.c_double
         _lib = L
     return _lib
 
@@ -191,3 +193,63 @@ def mc_pixel_accum(dem, st, acc=None, cnt=None):
     lib().gfo_mc_pixel_accum(C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _ip(st.flag),
                              _dp(acc), _ip(cnt))
     return acc, cnt
+
+
+def run_case(case, st, vstop, timestop, dtimestop, dt_driver, time_end, time_plot, max_steps, pixels=True):
+    """The step loop the reference runs for one case of a case table or
+    Monte Carlo set (Driver_Flow_23, "8 SPH_FEM_Driver_2019.f90":1444-1548)
+    with Stop_case_SW ("6 SPH_SW_2019.f90":20114-20177): adaptive dt_sph,
+    the step cut to end on the next plot time and restored after it, pixel
+    accumulation at plot times (Get_MC_pixel_Vars_SW), the stop conditions.
+    Advances ``st``; returns (iterations, stopped, pixel acc, pixel cnt).
+    (The reference's Driver_Flow_23 lives in the file of its main program,
+    which the reference build here replaces with its harness.)"""
+    import copy
+    prm = make_params(case.params)
+    dh = DemHandle(case.dem)
+    n = st.h.shape[0]
+    adapt = int(getattr(case.params, "adapt_dt", 0))
+    if st.dt_time is None:
+        st.dt_time = np.array([case.params.dt, 0.0])
+    fixed = copy.deepcopy(case)
+    fixed.params.adapt_dt = 0
+    npg = case.dem.nx * case.dem.ny
+    acc, cnt = np.zeros((npg, 4)), np.zeros(npg, np.int32)
+    tplot, tglob = 0.0, float(st.dt_time[1])
+    dt_sph = float(st.dt_time[0])
+    it = 0
+    stopped = False
+    while it < max_steps and not stopped:
+        it += 1
+        if adapt:
+            dt_sph = lib().gfo_adaptive_dt(C.byref(prm), C.byref(dh.s), n, _dp(st.v), _dp(st.h), _dp(st.hsml),
+                                           _ip(st.flag), dt_sph)
+        dt_keep = dt_sph
+        if time_plot > 0.0:
+            tpr = tplot + dt_sph
+            if tpr >= time_plot:
+                dt_sph = int(tpr / time_plot) * time_plot - tplot
+            tplot = tplot + dt_sph
+        st.dt_time[0] = dt_sph
+        run(fixed, st, 1)
+        tglob = tglob + dt_sph
+        if time_plot > 0.0 and tplot >= time_plot:
+            if pixels:
+                mc_pixel_accum(case.dem, st, acc, cnt)
+            dt_sph = dt_keep
+            tplot = 0.0
+        act = st.flag == 0
+        v = float(np.sum(np.abs(st.v[act, 0]))) / n
+        if it > 1:
+            if v < vstop and it > 10:
+                stopped = True
+            if int(np.sum(st.flag != 0)) > n // 5:
+                stopped = True
+            if st.dt_time[1] > timestop:
+                stopped = True
+            if dt_driver < dtimestop:
+                stopped = True
+        if tglob >= time_end:
+            stopped = True
+    st.dt_time[0] = dt_sph
+    return it, stopped, acc, cnt

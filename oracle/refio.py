@@ -92,17 +92,31 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
         f.write("\n".join(rows) + "\n")
 
 
-def write_master(d: str, name: str, dt: float, adapt: int = 0) -> None:
+def write_master(d: str, name: str, dt: float, adapt: int = 0, cases: int = 0, case_rows=(), stop=(0.0, 1e9, 0.0),
+                 lists=(), dt_driver: float | None = None, time_end: float = 1.e9, plot_step: int = 1000000) -> None:
     """<name>.MASTER.dat in Driver_Input's layout ("8 SPH_FEM_Driver_2019.f90":
-    229-340): SPH only, RK4, dt_sph and ic_adapt_dt_sph."""
+    229-357, 475-639): SPH only, RK4, dt_sph and ic_adapt_dt_sph, optionally a
+    case table (ic_cases_mat 1 or 2: rows of Ccases_mat, statistic lists as
+    (values, probabilities) pairs, the stop thresholds)."""
+    dtd = dt if dt_driver is None else dt_driver
     L = ["1", "run control written for the reference's input path",
-         "if_sph if_gfl ic_cases_mat", "1 0 0",
+         "if_sph if_gfl ic_cases_mat", f"1 0 {int(cases)}",
          "SPH_problem_type SPH_t_integ_Alg", "1 4",
          "sph problem name", name,
-         "dt time_end maxtimesteps", f"{_r(dt)} 1.e9 1000000",
-         "print_step save_step plot_step", "1000000 1000000 1000000",
+         "dt time_end maxtimesteps", f"{_r(dtd)} {_r(time_end)} 1000000",
+         "print_step save_step plot_step", f"1000000 1000000 {int(plot_step)}",
          "dt_sph ic_adapt", f"{_r(dt)} {int(adapt)}",
-         "Ntime curves max pts", "0 0", "win", "0"]
+         "Ntime curves max pts", "0 0"]
+    if cases:
+        ninfo = max(len(r) for r in case_rows)
+        L += ["ncases_mat ninfo_mat", f"{len(case_rows)} {ninfo}", "index val0 nval min max law ..."]
+        L += [" ".join(_r(v) for v in list(r) + [0.0] * (ninfo - len(r))) for r in case_rows]
+        if lists:
+            L += ["statistic lists"]
+            for vals, probs in lists:
+                L += [" ".join(_r(v) for v in vals), " ".join(_r(v) for v in probs)]
+        L += ["vstop timestop dtimestop", " ".join(_r(v) for v in stop)]
+    L += ["win", "0"]
     with open(os.path.join(d, name + ".MASTER.dat"), "w") as f:
         f.write("\n".join(L) + "\n")
 

@@ -67,6 +67,27 @@ def test_batch_adaptive_dt_equals_single_runs():
             np.testing.assert_array_equal(per[r][k], st[k], err_msg=f"realisation {r} field {k}")
 
 
+def test_hazard_netcdf(tmp_path):
+    """The hazard rasters written as NetCDF read back (scipy's reader) equal
+    the arrays gfsph_batch_hazard returns."""
+    from scipy.io import netcdf_file
+    cases = realisations(R=3, seed=9)
+    with BatchModel(cases) as b:
+        for _ in range(4):
+            b.time_integration_sph(10)
+            b.pixel_accumulate()
+        hz = b.write_hazard(str(tmp_path / "hazard.nc"), 0.5)
+    dem = cases[0].dem
+    with netcdf_file(str(tmp_path / "hazard.nc"), "r", mmap=False) as f:
+        g = lambda k: f.variables[k][:].reshape(-1)  # noqa: E731
+        np.testing.assert_array_equal(g("max_h"), hz["max_h"])
+        np.testing.assert_array_equal(g("p_exceed"), hz["p_exceed"])
+        np.testing.assert_array_equal(g("mean_h"), hz["mean"][:, 0])
+        np.testing.assert_array_equal(g("std_v"), hz["std"][:, 2])
+        np.testing.assert_array_equal(g("n_reached"), hz["nprob"])
+        assert f.dimensions == {"x": dem.nx, "y": dem.ny}
+
+
 def test_batch_pixels_and_hazard():
     cases = realisations(R=4, seed=5)
     plots, every = 6, 15
