@@ -49,6 +49,13 @@ struct DevParams {
   double f1[4], f2[4], fin;
   double pi;
   double kc;  // kernel normalisation: 15/(7 pi) for skf 1, 7/(4 pi) for skf 4
+  // erosion (ic_erosion): law const(21) (1 Hungr, 2 slope angle, 3 basal
+  // stress), its constants const(22..25), grain and fluid densities
+  // const(17), const(18), the bottom-velocity factor from const(47);
+  // eros_single: the realisation holds one particle, whose depth then grows
+  // with the eroded depth too (6:8756-8758)
+  int eros_on, eros_law, eros_single, pad_e;
+  double keros, eros_c23, eros_c24, eros_c25, denss, densw_e, dens_bar_eff, slip_fact, avail0;
 };
 
 struct DevDem {
@@ -364,19 +371,59 @@ __device__ __forceinline__ double bingham_candidate(const DevParams &P, double r
   return __longlong_as_double(0x7ff8000000000000ll);
 }
 
+// Get_Erosion_Rate_SW (6:10289-10404), main-region laws, single phase.  h,
+// vmod, vxx, vyy, tanfi and frictx/fricty are the values ExtForces_SW's
+// shared variables hold after basal_friction_Ext_SW.
+__device__ __forceinline__ double erosion_rate(const DevParams &P, double h, double vmod, double vxx, double vyy,
+                                               double Zpx, double Zpy, double tanfi, double frictx,
+                                               double fricty) {
+  double er = 0.0;
+  if (P.eros_law == 1) {
+    er = P.keros * h * vmod;
+  } else if (P.eros_law == 2) {
+    const double denss = P.denss, densw = P.densw_e;
+    const double vol_conc = (P.dens - densw) / (denss - densw);
+    if (vmod > 1.e-5) {
+      const double cosa = vxx / vmod, sina = vyy / vmod;
+      const double gradZv = Zpx * cosa + Zpy * sina;
+      const double theta = atan2(-gradZv, 1.);
+      double xnum = (denss - densw) * vol_conc;
+      const double xden = xnum + densw;
+      xnum = xnum * tanfi;
+      const double thetae = atan(xnum / xden);
+      if (theta > thetae) er = P.keros * P.eros_c23 * vmod * tan(theta - thetae);
+    }
+  } else if (P.eros_law == 3) {
+    if (vmod > 1.e-5) {
+      const double Taub = P.dens * sqrt(frictx * frictx + fricty * fricty);  // dens_bar = dens (6:9007)
+      const double Tauf = P.eros_c23 + P.dens_bar_eff * P.cgra * h * (1. - P.eros_c25) * P.eros_c24;
+      if (Taub > Tauf && vmod > 1.e-3) er = P.keros * (Taub - Tauf) / P.dens_bar_eff;
+    }
+  }
+  return er;
+}
+
 // tau_in: for nfric = 4, the Tau0 resolved by the last-valid scan (the
-// particle's own value when it is in the intermediate regime)
+// particle's own value when it is in the intermediate regime).
+// EROS: erosion is compiled in.  Inside the RK stages (end = false) the
+// erosion rate adds its momentum term (6:8664-8693); after them (end = true,
+// ExtForces_SW(1)) exdvxdt is not cleared, so (acx, acy) -- the particle's
+// exdvxdt of the last stage -- is where the gravity term is added and what
+// the limiter reads, and only the rate is wanted (er).
+template <bool EROS = false>
 __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem, double xx, double yy,
                                           double vx1, double vy1, double rho_i, double &u_i,
                                           double uaux1, double inx, double iny, double tau_in, double &exo,
-                                          double &eyo, double &duo) {
+                                          double &eyo, double &duo, double acx = 0.0, double acy = 0.0,
+                                          bool end = false, double *er_out = nullptr) {
   const double dt = P.dt;
   double h, hu, hv, hinv, vmod, hvmod, hvmodinv;
-  double vxx, vyy;
+  double vxx = vx1, vyy = vy1;
   const double gravz = P.cgra;
   double frictx = 0, fricty = 0, TotFrict = 0, cf5 = 0, hfrict, pwp;
   double xi_voellmy = P.xi_voellmy;
-  double exx = 0.0, exy = 0.0, du = 0.0;
+  double exx = (EROS && end) ? acx : 0.0, exy = (EROS && end) ? acy : 0.0, du = 0.0;
+  double tanfi_m = 0.0;  // ExtForces_SW's tanfi after basal_friction_Ext_SW
 
   h = rho_i;
   hu = h * vx1;
@@ -477,6 +524,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       if (factpw > P.fact_pwp_max) factpw = P.fact_pwp_max;
       tanfi = tanfi * factpw;
     }
+    tanfi_m = tanfi;
     double gtanfi = (gravz + cv2byR) * tanfi * P.fact_dens;
     if (gtanfi < 0.0) gtanfi = 0.0;
     cf5 = gtanfi;
@@ -556,6 +604,14 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   }
   exo = exx - frictx / h;
   eyo = exy - fricty / h;
+  if (EROS && P.eros_on) {
+    const double er = erosion_rate(P, h, vmod, vxx, vyy, tp.Zx, tp.Zy, tanfi_m, frictx, fricty);
+    if (!end) {
+      exo = exo - er * vx1 * hinv * P.slip_fact;
+      eyo = eyo - er * vy1 * hinv * P.slip_fact;
+    }
+    if (er_out) *er_out = er;
+  }
   duo = du;
 }
 
@@ -1015,17 +1071,84 @@ __device__ __forceinline__ void write_frozen(const DevParams &P, const SortedSet
   o.flag[i] = 1;
 }
 
+// Erosion state of the terrain, per realisation r at offset r*np: the
+// current Z, eroded depth and available depth (topol(1), topol(14),
+// topol(15)) and the two inverse-distance sums the particles scatter each
+// step (auxv(1:2, :), 6:8733-8748).
+struct ErosGrid {
+  double *z, *eroded, *avail;
+  double *aux;  // [R][np][2]
+  long np;
+};
+
+// a particle's share of the eroded depth to the 4 x 4 nodes around it
+// (6:8732-8748); the sums over particles are atomic, so their order -- and
+// the last bits of the node values -- follow the hardware's order
+__device__ __forceinline__ void erosion_scatter(const DevDem &d, double *aux, double xx, double yy, double eb) {
+  const double fx = (xx - d.xmin) / d.dx + 1, fy = (yy - d.ymin) / d.dy + 1;
+  if (!(fx >= 1.0 && fx < d.nx + 1.0 && fy >= 1.0 && fy < d.ny + 1.0)) return;  // active particles are inside
+  const int ipx = (int)fx, ipy = (int)fy;
+  const int inx0 = ipx - 1 > 1 ? ipx - 1 : 1, iny0 = ipy - 1 > 1 ? ipy - 1 : 1;
+  const int inx1 = ipx + 2 < d.nx ? ipx + 2 : d.nx, iny1 = ipy + 2 < d.ny ? ipy + 2 : d.ny;
+  for (int inx = inx0; inx <= inx1; inx++)
+    for (int iny = iny0; iny <= iny1; iny++) {
+      const long ip = (long)(iny - 1) * d.nx + (inx - 1);
+      double dx2 = xx - (d.xmin + (inx - 1) * d.dx), dy2 = yy - (d.ymin + (iny - 1) * d.dy);
+      dx2 = dx2 * dx2;
+      dy2 = dy2 * dy2;
+      const double dist = fmax(1.e-6, sqrt(dx2 + dy2));
+      unsafeAtomicAdd(aux + 2 * ip, eb / dist);
+      unsafeAtomicAdd(aux + 2 * ip + 1, 1.0 / dist);
+    }
+}
+
+// initial erosion state: realisation 0's Z (uploaded) to every realisation
+__global__ void k_eros_init(long total, long np, const double *avail0, ErosGrid g) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const long r = k / np;
+  if (r > 0) g.z[k] = g.z[k - r * np];
+  g.eroded[k] = 0.0;
+  g.avail[k] = avail0[r];
+  g.aux[2 * k] = 0.0;
+  g.aux[2 * k + 1] = 0.0;
+}
+
+// the terrain update after the particles (6:8790-8806); clears the sums
+__global__ void k_erode_nodes(long total, ErosGrid g) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const double2 a = reinterpret_cast<const double2 *>(g.aux)[k];
+  if (a.x == 0.0 && a.y == 0.0) return;
+  if (a.y > 1.e-6) {
+    double xnum = a.x / a.y;
+    const double av = g.avail[k];
+    if (xnum > av) xnum = av;
+    if (av > 0.) {
+      g.eroded[k] = g.eroded[k] + xnum;
+      g.z[k] = g.z[k] - xnum;
+      g.avail[k] = av - xnum;
+    }
+  }
+  reinterpret_cast<double2 *>(g.aux)[k] = make_double2(0.0, 0.0);
+}
+
 // After the stages: final update (7:908-922), final depth (7:926-931), domain
-// flag, pore pressure npwp = 1 (6:11176-11310), u clamp (7:971-980),
-// h_upgrade sle = 1 (7:2925-2931).  Writes the particle's outputs.
+// flag, erosion (ExtForces_SW(1), 6:8724-8761), pore pressure npwp = 1
+// (6:11176-11310), u clamp (7:971-980), h_upgrade sle = 1 (7:2925-2931).
+// Writes the particle's outputs.  EROS: (exl, eyl) is the last stage's
+// external force, (inx, iny) the pressure force.
+template <bool EROS = false>
 __device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem &dem, const SortedSet &s, int i,
                                                 double Si, double rfi, double2 x, double2 v, double u, double Dx,
                                                 double Dy, double Dvx, double Dvy, double Du, bool flagged,
-                                                NewSet &o, bool &active_out, double2 &xo, double &hso) {
+                                                NewSet &o, bool &active_out, double2 &xo, double &hso,
+                                                double exl = 0.0, double eyl = 0.0, double inx = 0.0,
+                                                double iny = 0.0, double *eaux = nullptr) {
   const double dt = P.dt;
   const double2 x0 = s.pos[i], v0 = s.vel[i];
   const double u0 = s.u[i], h0 = s.h[i], uaux1 = s.uaux[i], mi = s.mass[i], hi = s.hsml[i];
-  double hnew;
+  double hnew, mnew = mi;
   if (!flagged) {
     x.x = x0.x + P.fin * Dx;
     x.y = x0.y + P.fin * Dy;
@@ -1037,12 +1160,22 @@ __device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem
     hnew = rfi;  // frozen mid-step: keeps its clamped stage depth
   }
   if (out_of_domain(dem, x.x, x.y)) flagged = true;
+  const double hS = hnew;  // Drho_RK is taken before ExtForces_SW(1) (7:928)
+  if (EROS && P.eros_on && !flagged) {
+    double er = 0.0, e1, e2, e3;
+    ext_force<true>(P, dem, x.x, x.y, v.x, v.y, hnew, u, uaux1, inx, iny, 0.0, e1, e2, e3, exl, eyl, true, &er);
+    erosion_scatter(dem, eaux, x.x, x.y, dt * er);
+    mnew = mi + dt * er * mi / hnew;
+    if (P.eros_single) hnew = hnew + dt * er;
+    s.mass[i] = mnew;
+    s.nb[i].w = mnew;  // XSPH reads the masses after erosion (7:987)
+  }
   double uaux_new = uaux1;
   if (P.nuaux) {
     if (flagged) {
       uaux_new = 0.0;
     } else {
-      double hh = hnew, dhh = hnew - h0, hinv;
+      double hh = hnew, dhh = hS - h0, hinv;
       if (hh < P.comp) {
         hh = P.comp;
         hinv = 1.0 / P.comp;
@@ -1061,7 +1194,7 @@ __device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem
     if (u < 0.0) u = 0.0;
   }
   double hs = hi;
-  if (P.sle == 1 && !flagged) hs = 2.0 * pow(mi / hnew, 1. / 2);
+  if (P.sle == 1 && !flagged) hs = 2.0 * pow(mnew / hnew, 1. / 2);
   o.pos[i] = x;
   o.vel[i] = v;
   o.h[i] = hnew;
@@ -1100,9 +1233,10 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 // served one realisation at a time (normally one pass), so every parameter
 // read is a scalar load.  part = nullptr: no bounding-box partials (several
 // realisations: k_bbox_real computes their boxes).
+template <bool EROS>
 __global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
                                                   const double *S, const double2 *ain, NewSet o, BBoxAcc *part,
-                                                  const int *stopr) {
+                                                  const int *stopr, ErosGrid eg) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
@@ -1144,7 +1278,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, Dev
           flagged = true;
           break;
         }
-        ext_force(P, dem, x.x, x.y, v.x, v.y, rfi, u, uaux1, inx, iny, tau_unused, ex, ey, du);
+        ext_force<EROS>(P, dem, x.x, x.y, v.x, v.y, rfi, u, uaux1, inx, iny, tau_unused, ex, ey, du);
         const double f2 = P.f2[st];
         Dvx += f2 * (ex + inx);
         Dvy += f2 * (ey + iny);
@@ -1152,7 +1286,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, Dev
         Dy += f2 * v.y;
         Du += f2 * du;
       }
-      finish_particle(P, dem, s, i, Si, rfi, x, v, u, Dx, Dy, Dvx, Dvy, Du, flagged, o, active_out, xo, hso);
+      finish_particle<EROS>(P, dem, s, i, Si, rfi, x, v, u, Dx, Dy, Dvx, Dvy, Du, flagged, o, active_out, xo, hso,
+                            ex, ey, inx, iny, EROS ? eg.aux + 2 * eg.np * r0 : nullptr);
     }
   }
   if (part) bbox_block(part, active_out, xo, hso);
@@ -2392,6 +2527,12 @@ __global__ void k_unsort(int n, CurView s, double2 *x, double2 *v, double *h, do
   flag[d] = s.flag[i];
 }
 
+__global__ void k_unsort_mass(int n, CurView s, double *mass) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mass[s.id[i]] = s.mass[i];
+}
+
 __global__ void k_nonfinite(int n, CurView s, int *bad) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -2792,6 +2933,9 @@ struct gfsph_ctx {
   CaseCtl cctl{};
   CaseState *cs = nullptr;
   int *stopr = nullptr, *due = nullptr, *outc = nullptr, *cnt_r = nullptr;
+  // erosion (some realisation has ic_erosion): the terrain's erosion state
+  int eros = 0;
+  ErosGrid eg{};
   std::vector<void *> allocs;
 };
 
@@ -2871,9 +3015,44 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
     err = "only ic_fct_dens = 0 and ic_hrelSat = 0 are implemented";
     return GFSPH_ERR_UNSUPPORTED;
   }
-  if (c[21] > 1e-9 || c[4] > 1e-4 || c[36] > 0.01 || c[43] > 0) {
-    err = "erosion, crushing and end-solid laws are outside the implemented path";
+  if (c[36] > 0.01 || c[43] > 0) {
+    err = "crushing and end-solid laws are outside the implemented path";
     return GFSPH_ERR_UNSUPPORTED;
+  }
+  // erosion: Input_SW maps const(4) > 1e-4 to law 1 with K = const(4)
+  // (6:1180-1184); ic_erosion = const(21) > 1e-9 (6:1288-1290)
+  {
+    double c21 = c[21], c22 = c[22];
+    if (c[4] > 1.e-4) {
+      c21 = 1;
+      c22 = c[4];
+    }
+    P.eros_on = c21 > 1.e-9;
+    P.eros_law = (int)(c21 + 0.1);
+    P.keros = c22;
+    P.eros_c23 = c[23];
+    P.eros_c24 = c[24];
+    P.eros_c25 = c[25];
+    P.denss = c[17];
+    P.densw_e = c[18];
+    P.dens_bar_eff = c[2] - (c[18] < 1.e-3 ? 1000.0 : c[18]);  // 6:9001-9008
+    P.slip_fact = 1.0;                                          // nconst = 110 > 45 (6:8672-8679)
+    if (c[47] > 1.e-6) P.slip_fact = 2 * c[47] / (1. + c[47]);
+    P.avail0 = (c[48] > 1.e-6) ? c[48] : 999.;                 // topol(15,:) (6:1294-1301)
+    if (P.eros_on) {
+      if ((P.eros_law == 2 || P.eros_law == 3) && (c[17] <= 1.e-3 || c[18] <= 1.e-3)) {
+        err = "erosion laws 2 and 3 need const(17) and const(18) (the reference stops here)";
+        return GFSPH_ERR_UNSUPPORTED;
+      }
+      if (P.eros_law == 2 && !P.if_frictional) {
+        err = "erosion law 2 reads the frictional law's tan(phi): it needs a frictional nfric";
+        return GFSPH_ERR_UNSUPPORTED;
+      }
+      if (P.nfric == 4) {
+        err = "erosion with the Bingham law (nfric = 4) is outside the implemented path";
+        return GFSPH_ERR_UNSUPPORTED;
+      }
+    }
   }
   if (p->summation_density != 1 || p->nor_density != 0) {
     err = "summation density without normalisation is required";
@@ -3055,8 +3234,16 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   hipStream_t st = c->stream;
   const int nb = nblk(n);
   if (c->P.nfric != 4) {
-    k_rk<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw, c->R == 1 ? c->part : nullptr,
-                                        c->cases ? c->stopr : nullptr);
+    if (c->eros) {
+      k_rk<true><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
+                                                c->R == 1 ? c->part : nullptr, c->cases ? c->stopr : nullptr, c->eg);
+      const long tot = c->eg.np * c->R;
+      k_erode_nodes<<<(int)((tot + GF_BLOCK - 1) / GF_BLOCK), GF_BLOCK, 0, st>>>(tot, c->eg);
+    } else {
+      k_rk<false><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
+                                                 c->R == 1 ? c->part : nullptr, c->cases ? c->stopr : nullptr,
+                                                 c->eg);
+    }
     return;
   }
   BinghamState &b = c->bs;
@@ -3291,6 +3478,10 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
     *out = c;
     return rc;
   }
+  for (int r = 0; r < R; r++) {
+    Ph[r].eros_single = (counts[r] == 1);
+    c->eros = c->eros || Ph[r].eros_on;
+  }
   c->P = Ph[0];
   c->R = R;
   if (dem->nx < 2 || dem->ny < 2 || !(dem->dx > 0) || !(dem->dy > 0)) {
@@ -3342,6 +3533,23 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   }
   c->dem.zxy = c->zxy;
   c->dem.top = c->top;
+  if (c->eros) {
+    // topol(1) = Z, topol(14) = 0, topol(15) = the available depth of each
+    // realisation (6:1294-1301), sums cleared
+    ErosGrid &g = c->eg;
+    g.np = np;
+    const size_t tot = (size_t)np * R;
+    if ((rc = dalloc(c, &g.z, tot)) || (rc = dalloc(c, &g.eroded, tot)) || (rc = dalloc(c, &g.avail, tot)) ||
+        (rc = dalloc(c, &g.aux, 2 * tot)))
+      return rc;
+    HIPCHK(c, hipMemcpyAsync(g.z, dem->Z, sizeof(double) * np, hipMemcpyHostToDevice, st));
+    std::vector<double> av(R);
+    for (int r = 0; r < R; r++) av[r] = Ph[r].avail0;
+    double *avd;
+    if ((rc = dalloc(c, &avd, R))) return rc;
+    HIPCHK(c, hipMemcpyAsync(avd, av.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
+    k_eros_init<<<(int)((tot + GF_BLOCK - 1) / GF_BLOCK), GF_BLOCK, 0, st>>>((long)tot, np, avd, g);
+  }
 
   // particles -> set[0] in caller order; the first neighbour build sorts them
   SortedSet &s0 = c->set[0];
@@ -3476,6 +3684,52 @@ int gfsph_get_state(gfsph_ctx *c, double *x, double *v, double *h, double *u, do
   for (void *q : tmp) (void)hipFree(q);
   if (e) return fail(c, GFSPH_ERR_HIP, std::string("get_state: ") + hipGetErrorString(e));
   return GFSPH_OK;
+}
+
+int gfsph_get_mass(gfsph_ctx *c, double *mass) {
+  if (!c || c->n <= 0 || !mass) return GFSPH_ERR_ARG;
+  const int n = c->n;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double *dm = nullptr;
+  hipError_t e = hipMalloc((void **)&dm, sizeof(double) * n);
+  if (!e) {
+    k_unsort_mass<<<nblk(n), GF_BLOCK, 0, c->stream>>>(n, cur_view(c), dm);
+    e = hipGetLastError();
+  }
+  if (!e) e = hipMemcpyAsync(mass, dm, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (dm) (void)hipFree(dm);
+  if (e) return fail(c, GFSPH_ERR_HIP, std::string("get_mass: ") + hipGetErrorString(e));
+  return GFSPH_OK;
+}
+
+static int erosion_copy(gfsph_ctx *c, int32_t r, double *z, double *eroded, double *avail, bool up) {
+  if (!c || c->n <= 0) return GFSPH_ERR_ARG;
+  if (!c->eros) return fail(c, GFSPH_ERR_ARG, "erosion is off (const(21) = 0 and const(4) <= 1e-4)");
+  if (r < 0 || r >= c->R) return fail(c, GFSPH_ERR_ARG, "realisation index out of range");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const long np = c->eg.np;
+  const size_t off = (size_t)np * r, b = sizeof(double) * np;
+  double *dev[3] = {c->eg.z + off, c->eg.eroded + off, c->eg.avail + off};
+  double *host[3] = {z, eroded, avail};
+  for (int k = 0; k < 3; k++) {
+    if (!host[k]) continue;
+    if (up)
+      HIPCHK(c, hipMemcpyAsync(dev[k], host[k], b, hipMemcpyHostToDevice, c->stream));
+    else
+      HIPCHK(c, hipMemcpyAsync(host[k], dev[k], b, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFSPH_OK;
+}
+
+int gfsph_erosion_read(gfsph_ctx *c, int32_t r, double *z, double *eroded, double *avail) {
+  return erosion_copy(c, r, z, eroded, avail, false);
+}
+
+int gfsph_erosion_set(gfsph_ctx *c, int32_t r, const double *z, const double *eroded, const double *avail) {
+  return erosion_copy(c, r, const_cast<double *>(z), const_cast<double *>(eroded), const_cast<double *>(avail),
+                      true);
 }
 
 int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double *hsml, const int32_t *flag,
@@ -3836,6 +4090,7 @@ static int adjoint_supported(gfsph_ctx *c) {
   if (c->R != 1) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs on a single realisation");
   if (c->adapt) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs with a fixed dt_sph (ic_adapt_dt_sph = 0)");
   if (c->cases) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without case control");
+  if (c->eros) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without erosion");
   if (P.nfric != 5 || P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 ||
       P.xi_voellmy < 0.0)
     return fail(c, GFSPH_ERR_UNSUPPORTED,

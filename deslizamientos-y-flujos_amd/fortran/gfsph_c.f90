@@ -43,6 +43,7 @@ module gfsph_c
   public :: gfsph_create, gfsph_step, gfsph_sync, gfsph_get_state, gfsph_destroy, gfsph_last_error_f
   public :: gfsph_raster_idw, gfsph_mc_pixel_accumulate, gfsph_mc_pixel_read, gfsph_counts
   public :: gfsph_set_adaptive_dt, gfsph_time
+  public :: gfsph_get_mass, gfsph_erosion_read, gfsph_erosion_set
 
   interface
      integer(c_int) function gfsph_create(p, dem, n, x, v, h, mass, hsml, u, itype, ctx) bind(C, name="gfsph_create")
@@ -103,6 +104,26 @@ module gfsph_c
        type(c_ptr), value        :: ctx
        integer(c_int32_t), value :: mode
      end function gfsph_set_adaptive_dt
+
+     integer(c_int) function gfsph_get_mass(ctx, mass) bind(C, name="gfsph_get_mass")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value          :: ctx
+       real(c_double), intent(out) :: mass(*)
+     end function gfsph_get_mass
+
+     integer(c_int) function gfsph_erosion_read(ctx, r, z, eroded, avail) bind(C, name="gfsph_erosion_read")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value          :: ctx
+       integer(c_int32_t), value   :: r
+       real(c_double), intent(out) :: z(*), eroded(*), avail(*)
+     end function gfsph_erosion_read
+
+     integer(c_int) function gfsph_erosion_set(ctx, r, z, eroded, avail) bind(C, name="gfsph_erosion_set")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value         :: ctx
+       integer(c_int32_t), value  :: r
+       real(c_double), intent(in) :: z(*), eroded(*), avail(*)
+     end function gfsph_erosion_set
 
      integer(c_int) function gfsph_time(ctx, time, dt) bind(C, name="gfsph_time")
        import :: c_int, c_double, c_ptr

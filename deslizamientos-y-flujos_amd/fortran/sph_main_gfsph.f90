@@ -25,6 +25,11 @@
 !  ("6 SPH_SW_2019.f90":6521-6664) on the device at the start of every step
 !  and each call returns the dt_sph that step used, so the driver drops its
 !  own "call adaptive_dt_sph" (8:1590) and keeps "time_sph = time_sph + dt_sph".
+!
+!  With erosion (ic_erosion = 1) the context starts from the terrain's
+!  erosion state topol(1), topol(14), topol(15) as Input_SW left it (restart
+!  included), and every call copies back the masses and that state as
+!  ExtForces_SW(1) updates them ("6 SPH_SW_2019.f90":8724-8806).
 
 module SPH_MAIN_GFSPH
 
@@ -117,6 +122,7 @@ subroutine create_context
       rc = gfsph_set_adaptive_dt(ctx, int(ic_adapt_dt_sph, c_int32_t))
       call check(rc, 'gfsph_set_adaptive_dt')
    endif
+   if (ic_erosion .eq. 1) call push_erosion
    deallocate (t9, zz, inm, ity)
 
 end subroutine create_context
@@ -134,8 +140,44 @@ subroutine pull_state
    if (allocated(Uaux)) Uaux(1, 1:npoin) = ua(:)
    If_Out_Domain(1:npoin) = fl(:)
    deallocate (ua, fl)
+   if (ic_erosion .eq. 1) call pull_erosion
 
 end subroutine pull_state
+
+
+subroutine push_erosion
+
+   real(c_double), allocatable :: z(:), e(:), a(:)
+   integer(c_int) rc
+
+   allocate (z(npoig), e(npoig), a(npoig))
+   z(:) = topol(1, 1:npoig)
+   e(:) = topol(14, 1:npoig)
+   a(:) = topol(15, 1:npoig)
+   rc = gfsph_erosion_set(ctx, 0_c_int32_t, z, e, a)
+   call check(rc, 'gfsph_erosion_set')
+   deallocate (z, e, a)
+
+end subroutine push_erosion
+
+
+subroutine pull_erosion
+
+   real(c_double), allocatable :: m(:), z(:), e(:), a(:)
+   integer(c_int) rc
+
+   allocate (m(npoin), z(npoig), e(npoig), a(npoig))
+   rc = gfsph_get_mass(ctx, m)
+   call check(rc, 'gfsph_get_mass')
+   mass(1:npoin) = m(:)
+   rc = gfsph_erosion_read(ctx, 0_c_int32_t, z, e, a)
+   call check(rc, 'gfsph_erosion_read')
+   topol(1, 1:npoig)  = z(:)
+   topol(14, 1:npoig) = e(:)
+   topol(15, 1:npoig) = a(:)
+   deallocate (m, z, e, a)
+
+end subroutine pull_erosion
 
 
 subroutine gfsph_release

@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_inputs_parse", "gfsph_inputs_read", "gfsph_inputs_info", "gfsph_inputs_dem", "gfsph_inputs_particles",
     "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy", "gfsph_write_netcdf",
     "gfsph_batch_cases", "gfsph_batch_status", "gfsph_inputs_cases", "gfsph_inputs_batch_create",
+    "gfsph_get_mass", "gfsph_erosion_read", "gfsph_erosion_set",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -117,6 +118,9 @@ def load_library(path: str | None = None):
     L.gfsph_inputs_cases.argtypes = [vp, ip, ip, ip, dp, dp]
     L.gfsph_inputs_batch_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     L.gfsph_batch_status.argtypes = [vp, ip, ip, ip]
+    L.gfsph_get_mass.argtypes = [vp, dp]
+    L.gfsph_erosion_read.argtypes = [vp, C.c_int32, dp, dp, dp]
+    L.gfsph_erosion_set.argtypes = [vp, C.c_int32, dp, dp, dp]
     for f in (L.gfsph_inputs_parse, L.gfsph_inputs_read):
         f.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(vp)]
     L.gfsph_inputs_info.argtypes = [vp, C.POINTER(CParams), C.POINTER(CRun), C.POINTER(CGrid), ip]
@@ -255,6 +259,32 @@ class SPHModel:
         self._check(self.lib.gfsph_get_state(self.ctx, _dp(out["x"]), _dp(out["v"]), _dp(out["h"]), _dp(out["u"]),
                                              _dp(out["uaux"]), _dp(out["hsml"]), _ip(out["flag"])))
         return out
+
+    def mass(self) -> np.ndarray:
+        """Particle masses (volumes) in the caller's order; erosion changes them."""
+        m = np.zeros(self.n)
+        self._check(self.lib.gfsph_get_mass(self.ctx, _dp(m)))
+        return m
+
+    def erosion(self, r: int = 0) -> dict:
+        """The terrain's erosion state of realisation r: current Z
+        (topol(1,:)), eroded depth (topol(14,:)), available depth
+        (topol(15,:)), each [ny*nx]."""
+        npg = self.case.dem.nx * self.case.dem.ny
+        out = {k: np.zeros(npg) for k in ("z", "eroded", "avail")}
+        self._check(self.lib.gfsph_erosion_read(self.ctx, int(r), _dp(out["z"]), _dp(out["eroded"]),
+                                                _dp(out["avail"])))
+        return out
+
+    def set_erosion(self, r: int = 0, z=None, eroded=None, avail=None) -> None:
+        """Replaces parts of realisation r's erosion state (None: keep), as
+        the reference's restart.top does."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (z, eroded, avail)]
+        npg = self.case.dem.nx * self.case.dem.ny
+        if any(a is not None and a.size != npg for a in arrs):
+            raise ValueError("erosion arrays must have nx*ny values")
+        ptr = [C.POINTER(C.c_double)() if a is None else _dp(a) for a in arrs]
+        self._check(self.lib.gfsph_erosion_set(self.ctx, int(r), *ptr))
 
     def raster_idw(self):
         npg = self.case.dem.nx * self.case.dem.ny

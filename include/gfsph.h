@@ -94,6 +94,25 @@ int gfsph_sync(gfsph_ctx *ctx);
 int gfsph_get_state(gfsph_ctx *ctx, double *x, double *v, double *h, double *u, double *uaux,
                     double *hsml, int32_t *out_flag);
 
+/* Particle masses in the caller's order.  Erosion (below) changes them:
+ * mass(:) after ExtForces_SW(1) ("6 SPH_SW_2019.f90":8755). */
+int gfsph_get_mass(gfsph_ctx *ctx, double *mass);
+
+/* Erosion.  With const(21) > 0 (law 1 Hungr, 2 slope angle, 3 basal
+ * stress; or the older form const(4) > 1e-4) every RK stage adds the
+ * erosion momentum term and every step ends with ExtForces_SW(1): the
+ * particles gain mass at the erosion rate and the terrain loses the eroded
+ * depth ("6 SPH_SW_2019.f90":8664-8812, Get_Erosion_Rate_SW 10289-10404).
+ * The terrain's erosion state of realisation r (0 for a single run) is three
+ * [ny*nx] arrays: the current Z (topol(1,:)), the eroded depth
+ * (topol(14,:)) and the depth still available (topol(15,:)).  At create Z is
+ * dem->Z, the eroded depth 0 and the available depth const(48) if given,
+ * else 999 (6:1288-1308).  gfsph_erosion_set replaces any of them (NULL:
+ * keep), as Get_Topo_restart (2:2880-2921) reads them from restart.top.
+ * Both fail with GFSPH_ERR_ARG when erosion is off. */
+int gfsph_erosion_read(gfsph_ctx *ctx, int32_t r, double *z, double *eroded, double *avail);
+int gfsph_erosion_set(gfsph_ctx *ctx, int32_t r, const double *z, const double *eroded, const double *avail);
+
 /* Neighbour set of grid_find_NEW_tools3 ("5 a My_SPH_Tools_2019.f90":802-1104)
  * for the given positions: pairs (i<j) with r < scale_k*(hsml_i+hsml_j)/2.
  * Stateless; used by the parity tests and the Pint_Update export.  On

@@ -49,6 +49,11 @@ typedef struct {
   double fact_dens, fact_pwp_max, fact_pwp_min, fpwp_max, fpwp_min;
   int scale_k;
   double pi;
+  /* erosion (Input_SW 6:1180-1184, 6:1288-1308; Get_Erosion_Rate_SW
+     6:10289-10510; ExtForces_SW 6:8664-8693, 6:8724-8812) */
+  int eros_on;   /* ic_erosion: const(21) > 1e-9 (after the const(4) form)  */
+  int eros_law;  /* const(21) + 0.1: 1 Hungr, 2 slope-angle, 3 basal stress */
+  double keros, eros_c23, eros_c24, eros_c25, denss, densw_e, dens_bar_eff, slip_fact, avail0;
 } derived_t;
 
 static void derive(const gfo_params *p, derived_t *d) {
@@ -88,6 +93,27 @@ static void derive(const gfo_params *p, derived_t *d) {
   d->fpwp_min = 0.0;
   if (p->skf == 1 || p->skf == 4) d->scale_k = 2; /* 5a:850-856 */
   else d->scale_k = 3;
+  /* erosion constants: const(4) > 1e-4 is the older form of law 1 (6:1180-1184) */
+  double c21 = c[21], c22 = c[22];
+  if (c[4] > 1.e-4) {
+    c21 = 1;
+    c22 = c[4];
+  }
+  d->eros_on = c21 > 1.e-9;
+  d->eros_law = (int)(c21 + 0.1);
+  d->keros = c22;
+  d->eros_c23 = c[23];
+  d->eros_c24 = c[24];
+  d->eros_c25 = c[25];
+  d->denss = c[17];
+  d->densw_e = c[18];
+  /* DFs_FSs_dens_and_pwp_factors, ic_fct_dens = 0 (6:9001-9008) */
+  d->dens_bar_eff = d->dens - (c[18] < 1.e-3 ? 1000.0 : c[18]);
+  /* nconst = 110 > 45: Vbottom = alpha Vtop (6:8672-8679) */
+  d->slip_fact = 1.0;
+  if (c[47] > 1.e-6) d->slip_fact = 2 * c[47] / (1. + c[47]);
+  /* available erosion depth topol(15,:) (6:1294-1301) */
+  d->avail0 = (c[48] > 1.e-6) ? c[48] : 999.;
 }
 
 /* ------------------------------------------------------------------ */
@@ -449,7 +475,42 @@ static void sum_density(const gfo_params *p, int n, const pairvec *pv, const dou
 
 typedef struct {
   double ex, ey, du;
+  double er; /* erosion_rate of the call (0 without erosion) */
 } extout;
+
+/* Get_Erosion_Rate_SW (6:10289-10404), main-region laws, single phase.  h,
+   vmod, vxx, vyy, tanfi and frictx/fricty are the values ExtForces_SW's
+   shared variables hold after basal_friction_Ext_SW. */
+static double erosion_rate(const derived_t *D, double h, double vmod, double vxx, double vyy, double Zpx,
+                           double Zpy, double tanfi, double frictx, double fricty) {
+  double er = 0.0;
+  if (D->eros_law == 1) {
+    er = D->keros * h * vmod;
+  } else if (D->eros_law == 2) {
+    const double Cstar = D->eros_c23, denss = D->denss, densw = D->densw_e;
+    double vol_conc = (D->dens - densw) / (denss - densw);
+    if (vmod > 1.e-5) {
+      double cosa = vxx / vmod, sina = vyy / vmod;
+      double gradZv = Zpx * cosa + Zpy * sina;
+      double theta = atan2(-gradZv, 1.);
+      double xnum = (denss - densw) * vol_conc;
+      double xden = xnum + densw;
+      xnum = xnum * tanfi;
+      double tanthetae = xnum / xden;
+      double thetae = atan(tanthetae);
+      if (theta > thetae) er = D->keros * Cstar * vmod * tan(theta - thetae);
+    }
+  } else if (D->eros_law == 3) {
+    const double C_basal = D->eros_c23, tanfi_basal = D->eros_c24, beta_basal = D->eros_c25;
+    if (vmod > 1.e-5) {
+      double dens_bar = D->dens; /* ic_fct_dens = 0 (6:9007) */
+      double Taub = dens_bar * sqrt(frictx * frictx + fricty * fricty);
+      double Tauf = C_basal + D->dens_bar_eff * D->cgra * h * (1. - beta_basal) * tanfi_basal;
+      if (Taub > Tauf && vmod > 1.e-3) er = D->keros * (Taub - Tauf) / D->dens_bar_eff;
+    }
+  }
+  return er;
+}
 
 /* tau0 carries the Bingham Tau0 between particles: ExtForces_SW applies
    cf4b = Tau0/dens after every nfric=4 branch (6:9619-9622) but sets Tau0
@@ -457,9 +518,14 @@ typedef struct {
    regimes use the Tau0 of the last particle, in particle order, that was in
    the intermediate regime.  The compiled reference behaves this way (checked
    against its dumps); the value carries across calls, starting from 0. */
+/* acc0: NULL inside the RK stages (ExtForces_SW(0) clears exdvxdt first,
+   6:8482-8485); after them (ExtForces_SW(1), erosion) the particle's
+   exdvxdt of the last stage, which the gravity term is added to and the
+   limiter reads. */
 static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem *dem, double xx,
                           double yy, double vx1, double vy1, double rho_i, double *u_i,
-                          double uaux1, double inx, double iny, double *tau0, extout *o) {
+                          double uaux1, double inx, double iny, double *tau0, const double *acc0,
+                          extout *o) {
   const double dt = p->dt;
   double h, hu, hv, hinv, vmod, hvmod, hvmodinv;
   double vxx = vx1, vyy = vy1;
@@ -467,7 +533,8 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
   double frictx = 0, fricty = 0, TotFrict = 0, cf5 = 0, hfrict, pwp;
   int nfric = D->nfric, if_frictional = D->if_frictional, if_V2byR = D->if_V2byR;
   double xi_voellmy = D->xi_voellmy;
-  double exx = 0.0, exy = 0.0, du = 0.0;
+  double exx = acc0 ? acc0[0] : 0.0, exy = acc0 ? acc0[1] : 0.0, du = 0.0;
+  double tanfi_m = 0.0; /* ExtForces_SW's tanfi after basal_friction_Ext_SW */
 
   h = rho_i;
   hu = h * vx1;
@@ -575,6 +642,7 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
       if (factpw > D->fact_pwp_max) factpw = D->fact_pwp_max;
       tanfi = tanfi * factpw;
     }
+    tanfi_m = tanfi;
     double gtanfi = (gravz + cv2byR) * tanfi * D->fact_dens;
     if (gtanfi < 0.0) gtanfi = 0.0;
     cf5 = gtanfi;
@@ -658,6 +726,14 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
   }
   exx = exx - frictx / h;
   exy = exy - fricty / h;
+  o->er = 0.0;
+  if (D->eros_on) { /* 6:8664-8693; the momentum term only inside the stages */
+    o->er = erosion_rate(D, h, vmod, vxx, vyy, tp.Zx, tp.Zy, tanfi_m, frictx, fricty);
+    if (!acc0) {
+      exx = exx - o->er * vx1 * hinv * D->slip_fact;
+      exy = exy - o->er * vy1 * hinv * D->slip_fact;
+    }
+  }
   o->ex = exx;
   o->ey = exy;
   o->du = du;
@@ -747,11 +823,25 @@ double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const dou
   return adaptive_dt(p, dem, n, v, rho, hsml, flag, dt);
 }
 
+/* Node coordinates coorg(:, ipoig) of the terrain grid (2:285-296). */
+static void node_xy(const gfo_dem *d, int64_t ip, double *xg, double *yg) {
+  if (d->coorg) {
+    *xg = d->coorg[2 * ip];
+    *yg = d->coorg[2 * ip + 1];
+  } else {
+    *xg = d->xmin + (ip % d->nx) * d->dx;
+    *yg = d->ymin + (ip / d->nx) * d->dy;
+  }
+}
+
 int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double *v, double *rho,
-            const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
-            int nsteps, double *tau0_io, double *dt_time_io) {
+            double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
+            int nsteps, double *tau0_io, double *dt_time_io, double *eros) {
   derived_t D;
   derive(p_in, &D);
+  if (D.eros_on && !eros) return -2;
+  const int64_t npg = (int64_t)dem->nx * dem->ny;
+  double *auxv = D.eros_on ? (double *)calloc(2 * npg, sizeof(double)) : NULL;
   double dt = dt_time_io ? dt_time_io[0] : p_in->dt;
   double time_sph = dt_time_io ? dt_time_io[1] : 0.0;
   gfo_params p_step = *p_in;
@@ -873,7 +963,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         if (flag[i]) continue;
         extout o;
         ext_force_one(p, &D, dem, x[2 * i], x[2 * i + 1], v[2 * i], v[2 * i + 1], rho[i], &u[i],
-                      nuaux ? uaux[i] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, &o);
+                      nuaux ? uaux[i] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, NULL, &o);
         ex[2 * i] = o.ex;
         ex[2 * i + 1] = o.ey;
         du[i] = o.du;
@@ -901,6 +991,49 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
     }
     for (int i = 0; i < n; i++)
       if (out_of_domain(dem, x[2 * i], x[2 * i + 1])) flag[i] = 1;
+    /* 7. ExtForces_SW(1) (7:941): erosion once the step's velocities are
+          known (6:8724-8761), then the terrain (6:8790-8806) */
+    if (D.eros_on) {
+      const double dxg = dem->dx, dyg = dem->dy;
+      memset(auxv, 0, sizeof(double) * 2 * npg);
+      for (int i = 0; i < n; i++) {
+        if (flag[i]) continue;
+        extout o;
+        ext_force_one(p, &D, dem, x[2 * i], x[2 * i + 1], v[2 * i], v[2 * i + 1], rho[i], &u[i],
+                      nuaux ? uaux[i] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, &ex[2 * i], &o);
+        const double eb = dt * o.er;
+        const double xx = x[2 * i], yy = x[2 * i + 1];
+        int ipx = (int)((xx - dem->xmin) / dxg + 1), ipy = (int)((yy - dem->ymin) / dyg + 1);
+        int inx0 = ipx - 1 > 1 ? ipx - 1 : 1, iny0 = ipy - 1 > 1 ? ipy - 1 : 1;
+        int inx1 = ipx + 2 < dem->nx ? ipx + 2 : dem->nx, iny1 = ipy + 2 < dem->ny ? ipy + 2 : dem->ny;
+        for (int inx = inx0; inx <= inx1; inx++)
+          for (int iny = iny0; iny <= iny1; iny++) {
+            int64_t ip = (int64_t)(iny - 1) * dem->nx + (inx - 1);
+            double xg, yg;
+            node_xy(dem, ip, &xg, &yg);
+            double dx2 = (xx - xg), dy2 = (yy - yg);
+            dx2 = dx2 * dx2;
+            dy2 = dy2 * dy2;
+            double dist = fmax(1.e-6, sqrt(dx2 + dy2));
+            auxv[2 * ip] = auxv[2 * ip] + eb / dist;
+            auxv[2 * ip + 1] = auxv[2 * ip + 1] + 1.0 / dist;
+          }
+        mass[i] = mass[i] + dt * o.er * mass[i] / rho[i];
+        if (n == 1) rho[i] = rho[i] + dt * o.er; /* npoin_s = 1 (6:8756-8758) */
+      }
+      for (int64_t ip = 0; ip < npg; ip++) {
+        if (auxv[2 * ip + 1] > 1.e-6) {
+          double xnum = auxv[2 * ip] / auxv[2 * ip + 1];
+          double *t = eros + 3 * ip; /* topol(1), topol(14), topol(15) */
+          if (xnum > t[2]) xnum = t[2];
+          if (t[2] > 0.) {
+            t[1] = t[1] + xnum;
+            t[0] = t[0] - xnum;
+            t[2] = t[2] - xnum;
+          }
+        }
+      }
+    }
     /* 8. pore pressure, npwp = 1 (Get_Pwp_SW 6:11176-11310) */
     if (nuaux >= 1) {
       for (int i = 0; i < n; i++) {
@@ -974,6 +1107,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
   free(pdw);
   free(pv.i);
   free(pv.j);
+  free(auxv);
   if (tau0_io) *tau0_io = tau0_carry;
   if (dt_time_io) {
     dt_time_io[0] = dt;

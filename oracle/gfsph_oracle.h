@@ -44,10 +44,13 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
  * rho to h_inf_SW in place where the mode reads it */
 double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *v, double *rho,
                        const double *hsml, const int32_t *flag, double dt);
+/* mass is updated in place by erosion; eros is [nx*ny][3] = topol(1),
+ * topol(14), topol(15) in/out, required when erosion is on (else -2) */
 int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,
-            const double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
+            double *mass, double *hsml, double *u, double *uaux, int32_t *flag,
             int nsteps, double *tau0_io /* Bingham Tau0 carry, in/out; NULL = 0 */,
-            double *dt_time_io /* [dt_sph, time_sph] in/out; NULL: p->dt, 0 */);
+            double *dt_time_io /* [dt_sph, time_sph] in/out; NULL: p->dt, 0 */,
+            double *eros /* erosion terrain state or NULL */);
 void gfo_raster_idw(const gfo_dem *d, int n, const double *x, const double *v, const double *rho,
                     const double *u, const int32_t *flag, double *out, double *wsum);
 void gfo_mc_pixel_accum(const gfo_dem *d, int n, const double *x, const double *v,

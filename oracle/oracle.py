@@ -53,7 +53,7 @@ def lib():
         L.gfo_fdiff_derivs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp]
         L.gfo_init.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, ip]
         L.gfo_run.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, dp, ip,
-                              C.c_int, dp, dp]
+                              C.c_int, dp, dp, dp]
         L.gfo_raster_idw.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, dp, dp]
         L.gfo_mc_pixel_accum.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, dp, ip]
         L.gfo_adaptive_dt.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, C.c_double]
@@ -146,6 +146,25 @@ class State:
         self.flag = np.zeros(self.h.shape[0], np.int32)
         self.tau0 = np.zeros(1)  # Bingham Tau0 carried between calls
         self.dt_time = None      # [dt_sph, time_sph] carried between calls
+        self.eros = None         # [npoig, 3] topol(1), topol(14), topol(15) when erosion is on
+
+
+def erosion_on(prm) -> bool:
+    """ic_erosion (6:1180-1184, 6:1288-1290): const(21) > 1e-9, or the older
+    form const(4) > 1e-4."""
+    c = prm.cst()
+    return bool(c[21] > 1e-9 or c[4] > 1e-4)
+
+
+def erosion_grid(case):
+    """Initial erosion state of the terrain: topol(1) = Z, topol(14) = 0 and
+    the available depth topol(15) = const(48) if given, else 999
+    (6:1294-1301)."""
+    c = case.params.cst()
+    g = np.zeros((case.dem.nx * case.dem.ny, 3))
+    g[:, 0] = np.asarray(case.dem.Z, dtype=np.float64).ravel()
+    g[:, 2] = c[48] if c[48] > 1e-6 else 999.0
+    return g
 
 
 def init(case, state=None):
@@ -155,6 +174,8 @@ def init(case, state=None):
     dh = DemHandle(case.dem)
     lib().gfo_init(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.h), _dp(st.mass), _dp(st.hsml),
                    _dp(st.u), _dp(st.uaux), _ip(st.flag))
+    if erosion_on(case.params) and st.eros is None:
+        st.eros = erosion_grid(case)
     return st
 
 
@@ -167,9 +188,11 @@ def run(case, st, nsteps, rev_order=False):
     if st.dt_time is None:
         st.dt_time = np.array([case.params.dt, 0.0])
     dh = DemHandle(case.dem)
+    if erosion_on(case.params) and st.eros is None:
+        st.eros = erosion_grid(case)
     rc = lib().gfo_run(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _dp(st.mass),
                        _dp(st.hsml), _dp(st.u), _dp(st.uaux), _ip(st.flag), nsteps, _dp(st.tau0),
-                       _dp(st.dt_time))
+                       _dp(st.dt_time), None if st.eros is None else _dp(st.eros))
     if rc != 0:
         raise RuntimeError(f"gfo_run failed: {rc}")
     return st

@@ -22,6 +22,8 @@
 !     real64 x0(ndimn,npoin)        positions the pair list was built from
 !     int32  pair_i(npairs) pair_j(npairs)
 !     real64 w(npairs) dwdx(ndimn,npairs)
+!  and, when erosion is on (ic_erosion = 1), eros_<step>.bin:
+!     real64 (topol(1,ip), topol(14,ip), topol(15,ip), ip = 1, npoig)
 !  It is built only by oracle/build_ref.sh and never ships with the product.
 !  Built with -DGFSPH_GPU it calls time_integration_sph_gfsph (module
 !  SPH_MAIN_GFSPH of the product's Fortran layer) instead, i.e. the
@@ -196,6 +198,12 @@ subroutine write_dump (is)
    endif
    close (iu)
    deallocate (pi_, pj_, pw, pdw)
+   if (ic_erosion.eq.1) then
+      write (fname,'(a,i0,a)') 'eros_', is, '.bin'
+      open (newunit=iu, file=trim(fname), access='stream', form='unformatted', status='replace')
+      write (iu) (real(topol(1,ip),8), real(topol(14,ip),8), real(topol(15,ip),8), ip=1,npoigx*npoigy)
+      close (iu)
+   endif
 end subroutine write_dump
 
 end program gfsph_ref_harness

@@ -35,8 +35,17 @@ def _r(v: float) -> str:
 def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
     prm, dem, P = case.params, case.dem, case.particles
     c = prm.cst()
-    if any(c[k] != 0.0 for k in range(16, 111)):
-        raise ValueError("only const(1:15) are written by this helper")
+    # const(16:30), (31:45), (46:60) follow on lines of their own when const(11),
+    # const(30), const(45) are negative (Input_SW 6:1186-1204)
+    if any(c[k] != 0.0 for k in range(61, 111)):
+        raise ValueError("only const(1:60) are written by this helper")
+    nblk = 0
+    for b, (lo, flag) in enumerate(((16, 11), (31, 30), (46, 45))):
+        if any(c[k] != 0.0 for k in range(lo, lo + 15)):
+            nblk = b + 1
+    for b, flag in enumerate((11, 30, 45)):
+        if b < nblk and not c[flag] < 0:
+            raise ValueError(f"const({flag}) must be negative to pass const({16 + 15 * b}:...)")
     L = []
     L += ["1", "synthetic case written for the reference oracle"]
     L += ["ic_SW_Alg", "0"]
@@ -50,6 +59,9 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
     L += ["sum_den av_vel virt_part nor_dens",
           f"{tf(prm.summation_density)} {tf(prm.average_velocity)} F {tf(prm.nor_density)}"]
     L += ["const 1..15", " ".join(_r(c[k]) for k in range(1, 16))]
+    for b in range(nblk):
+        lo = 16 + 15 * b
+        L += [f"const {lo}..{lo + 14}", " ".join(_r(c[k]) for k in range(lo, lo + 15))]
     nfric = int(c[5] + 0.01)
     if (5 <= nfric <= 10) or nfric in (26, 27):
         L += ["K0 activated?", "0"]
@@ -80,6 +92,22 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
                 k += 1
         f.write("\n".join(rows) + "\n")
         f.write("topo props (zones) 0 = none\n0\n")
+
+    if c[21] > 1e-9 or c[4] > 1e-4:
+        # erosion with restart.dat input reads the terrain's erosion state from
+        # restart.top (Get_Topo_restart, "2 SPH_FEM_topo_2019.f90":2880-2921):
+        # Z, eroded depth 0, available depth (reset by Input_SW right after)
+        with open(os.path.join(d, "restart.top"), "w") as f:
+            f.write("terrain erosion state written for the reference oracle\n")
+            f.write(f"{nx * ny} 0.0\nipoig topol(1) topol(14) topol(15)\n")
+            avail = c[48] if c[48] > 1e-6 else 999.0
+            rows = []
+            k = 1
+            for iy in range(ny):
+                for ix in range(nx):
+                    rows.append(f"{k} {_r(Z[iy, ix])} 0.0 {_r(avail)}")
+                    k += 1
+            f.write("\n".join(rows) + "\n")
 
     with open(os.path.join(d, "restart.dat"), "w") as f:
         f.write("restart written for the reference oracle\n")
@@ -158,6 +186,13 @@ def read_dump(path: str) -> dict:
     return out
 
 
+def read_eros(path: str) -> np.ndarray:
+    """eros_<step>.bin written by the harness when erosion is on:
+    [npoig][3] = topol(1), topol(14), topol(15)."""
+    raw = open(path, "rb").read()
+    return np.frombuffer(raw, dtype=np.float64).reshape(-1, 3).copy()
+
+
 def read_topo(path: str) -> dict:
     """topo.bin written by the harness after Init_sph (the terrain grid,
     topol(1:9,:) and poig_in_model)."""
@@ -218,6 +253,9 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
     if r.returncode != 0:
         raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
     out = {s: read_dump(os.path.join(tmp, f"dump_{s}.bin")) for s in steps}
+    for s in steps:
+        pe = os.path.join(tmp, f"eros_{s}.bin")
+        out[s]["eros"] = read_eros(pe) if os.path.exists(pe) else None
     if not keep and workdir is None:
         subprocess.run(["rm", "-rf", tmp])
     return out

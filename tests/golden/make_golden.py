@@ -120,6 +120,29 @@ def case_defs():
     defs["g_domain_exit"] = dict(
         case=C.Case("g_domain_exit", C.Params(integrator=4, dt=0.05, h_inf=1e-3, average_velocity=0,
                                               const=C.voellmy(0.1, 1000.0)), pl, pe, 80), dumps=[1, 20, 40, 80])
+    # erosion (Get_Erosion_Rate_SW 6:10289-10404, ExtForces_SW 6:8664-8812):
+    # law 1 (Hungr, dh/dt = K h |v|) with the bottom-velocity factor const(47)
+    # and an available depth const(48) that runs out; law 2 (slope angle
+    # against the equilibrium angle) on a frictional flow with pore pressure;
+    # law 3 (basal stress above a Coulomb threshold) under Voellmy.  const(11),
+    # const(30), const(45) < 0 make the reference read const(16:60).
+    ce1 = C.voellmy(0.2, 500.0)
+    ce1.update({11: -1.0, 21: 1, 22: 0.002, 30: -1.0, 45: -1.0, 47: 0.5, 48: 0.05})
+    defs["g_erosion_hungr"] = dict(
+        case=C.Case("g_erosion_hungr", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=ce1),
+                    small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
+        dumps=[1, 10, 100])
+    ce2 = C.frictional(28.0, cv=0.01)
+    ce2.update({11: -1.0, 17: 2650.0, 18: 1000.0, 21: 2, 22: 0.05, 23: 0.6})
+    defs["g_erosion_slope"] = dict(
+        case=C.Case("g_erosion_slope", C.Params(integrator=4, dt=0.02, h_inf=1e-3, icpwp=3, pwp0rel=0.6,
+                                                const=ce2), dem, scar_on(dem, 300, 31), 100), dumps=[1, 10, 100])
+    ce3 = C.voellmy(0.2, 500.0)
+    ce3.update({11: -1.0, 17: 2650.0, 18: 1000.0, 21: 3, 22: 0.002, 23: 50.0, 24: 0.3, 25: 0.2})
+    defs["g_erosion_stress"] = dict(
+        case=C.Case("g_erosion_stress", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=ce3),
+                    small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
+        dumps=[1, 10, 100])
     # random cloud with varied smoothing lengths: neighbour set only
     rng = np.random.Generator(np.random.Philox(99))
     n = 600
@@ -163,6 +186,10 @@ def save_case(name, d, out_dir=HERE):
         arr[f"s{s}_hsml"] = dd["hsml"]
         arr[f"s{s}_flag"] = dd["flag"]
         arr[f"s{s}_time"] = np.array(dd["time"])
+        if dd.get("eros") is not None:
+            # erosion changes the masses and the terrain: topol(1), (14), (15)
+            arr[f"s{s}_mass"] = dd["mass"]
+            arr[f"s{s}_eros"] = dd["eros"]
         if dd["uaux"] is not None:
             arr[f"s{s}_uaux"] = dd["uaux"]
         if s > 0:

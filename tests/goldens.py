@@ -14,7 +14,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 STATE_CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_curvature7", "g_mui8", "g_icpwp1_sle1",
                "g_manning1", "g_perzyna9", "g_hb25", "g_bagnold11", "g_kernel_skf2", "g_kernel_skf3",
-               "g_kernel_skf4", "g_domain_exit", "g_adaptive_dt2", "g_adaptive_dt10"]
+               "g_kernel_skf4", "g_domain_exit", "g_adaptive_dt2", "g_adaptive_dt10", "g_erosion_hungr",
+               "g_erosion_slope", "g_erosion_stress"]
+EROSION_CASES = ["g_erosion_hungr", "g_erosion_slope", "g_erosion_stress"]
 ALL_CASES = STATE_CASES + ["g_random_neighbours"]
 
 

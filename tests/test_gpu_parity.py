@@ -96,6 +96,12 @@ def test_gpu_state_matches_reference(name):
                 assert goldens.rel_err(st["uaux"], f[f"s{s}_uaux"]) <= tol_for(s)
             if f"s{s}_time" in f:
                 assert m.time()[0] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)
+            if f"s{s}_eros" in f:  # erosion: masses and the terrain's Z / eroded / available depth
+                assert goldens.rel_err(m.mass(), f[f"s{s}_mass"]) <= tol_for(s), f"{name} step {s} mass"
+                e = m.erosion()
+                for j, k in enumerate(("z", "eroded", "avail")):
+                    err = goldens.rel_err(e[k], f[f"s{s}_eros"][:, j])
+                    assert err <= tol_for(s), f"{name} step {s} {k}: {err:.3e}"
 
 
 @pytest.mark.parametrize("name", ["g_voellmy_plane", "g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1"])

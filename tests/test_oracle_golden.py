@@ -48,6 +48,11 @@ def test_state_matches_reference(name):
             assert goldens.rel_err(st.uaux, f[f"s{s}_uaux"]) <= tol_for(s)
         if f"s{s}_time" in f:  # time_sph: the sum of the (adaptive) dt_sph of the steps
             assert st.dt_time[1] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)
+        if f"s{s}_eros" in f:  # erosion: masses, and the terrain's Z / eroded / available depth
+            assert goldens.rel_err(st.mass, f[f"s{s}_mass"]) <= tol_for(s)
+            for j, what in enumerate(("topol(1)", "topol(14)", "topol(15)")):
+                err = goldens.rel_err(st.eros[:, j], f[f"s{s}_eros"][:, j])
+                assert err <= tol_for(s), f"{name} step {s} {what}: {err:.3e}"
 
 
 @pytest.mark.parametrize("name", goldens.ALL_CASES)
