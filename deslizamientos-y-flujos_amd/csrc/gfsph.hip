@@ -2,19 +2,23 @@
 // integrated SPH propagation step behind the C ABI of include/gfsph.h.
 //
 // One step (frozen pair geometry, "7 SPH_MAIN_2019.f90":784-799):
-//   k_grid/k_clear grid of the active bounding box (cell = scale_k*max hsml)
+//   k_grid         grid of the active bounding box (cell = half the largest
+//                  support radius)
 //   k_keys         cell key per particle + atomic count per cell
-//   k_scan_*       exclusive scan of the cell counts -> cell starts
+//   k_scan_*       exclusive scan of the cell counts -> cell starts (and
+//                  the counts cleared for the next build)
 //   k_scatter      counting-sort scatter
-//   k_cellsort     order each cell by previous slot (deterministic order)
-//   k_reorder      gather the particle state into cell order
+//   k_reorder      gather the particle state into cell order, each cell
+//                  ordered by previous slot (deterministic order)
 //   k_density      summed depth h_i = W(0)m_i + sum m_j w_ij  (5a:1763)
 //   k_pforce       pressure-force gather (IntForces_SW, 6:6669)
 //   k_rk           every RK stage per particle: DEM bilinear lookup
 //                  (2:2395), gravity, basal rheology and limiter
-//                  (6:9355-9901), stage update (7:808-922), pore pressure
-//                  (6:11176-11310), domain flag (7:1764), h update (7:2866),
-//                  next-step bounding box (Bingham: per-stage kernels)
+//                  (6:9355-9901), erosion (6:8664-8812), stage update
+//                  (7:808-922), pore pressure (6:11176-11310), domain flag
+//                  (7:1764), h update (7:2866), next-step bounding box
+//                  (Bingham: per-stage kernels)
+//   k_erode_nodes  terrain update of erosion (6:8790-8806)
 //   k_xsph         XSPH velocity correction (7:3042-3099, 7:987-993)
 // All arithmetic is FP64.  The file is compiled with -ffp-contract=off so
 // expressions round as the reference's FP64 build writes them.
@@ -862,10 +866,6 @@ __global__ void k_bbox_real(const int2 *rng, const double2 *pos, const double *h
   }
 }
 
-__global__ void k_clear(const int *ntot, int *cnt) {
-  int nb = *ntot + 1;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += gridDim.x * blockDim.x) cnt[c] = 0;
-}
 
 __device__ __forceinline__ void cell_of(const GridInfo &g, double2 p, int &cx, int &cy) {
   cx = (int)((p.x - g.xmin) / g.cell);
@@ -878,8 +878,9 @@ __global__ void k_keys(int n, const double2 *pos, const int *flag, const int *ri
                        const int *coff, const int *ntot, int *key, int *rank, int *cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const bool out = flag[i] != 0;
   int k;
-  if (flag[i]) {
+  if (out) {
     k = *ntot;
   } else {
     const int r = rid[i];
@@ -889,7 +890,19 @@ __global__ void k_keys(int n, const double2 *pos, const int *flag, const int *ri
     k = coff[r] + cy * g.ncx + cx;
   }
   key[i] = k;
-  rank[i] = atomicAdd(&cnt[k], 1);
+  // every out-of-domain particle shares one bucket: one atomic per wave
+  // for them (their order inside the bucket is free, they never interact)
+  const unsigned long long m = __ballot(out);
+  if (out) {
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[k], __popcll(m));
+    base = __shfl(base, leader);
+    rank[i] = base + __popcll(m & ((1ull << lane) - 1ull));
+  } else {
+    rank[i] = atomicAdd(&cnt[k], 1);
+  }
 }
 
 // exclusive scan of cnt[0..ncell] -> start[0..ncell+1]; 2048 items per block
@@ -940,24 +953,20 @@ __global__ void k_scan_reduce(const int *ntot, const int *cnt, int *bsum) {
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-__global__ void k_scan_top(int nblocks, int *bsum) {
-  __shared__ int lds[GF_BLOCK / 64 + 1];
-  int carry = 0;
-  for (int base = 0; base < nblocks; base += GF_BLOCK) {
-    int i = base + threadIdx.x;
-    int v = i < nblocks ? bsum[i] : 0;
-    int total;
-    int ex = block_exclusive_scan(v, lds, total);
-    if (i < nblocks) bsum[i] = ex + carry;
-    carry += total;
-  }
-}
 
-__global__ void k_scan_down(const int *ntot, const int *cnt, const int *bsum, int *start, int n) {
+// Down-sweep: each tile's offset is the sum of the tile sums before it
+// (summed by the block itself: the used tiles are few), then the tile's
+// exclusive scan.  The counts are cleared as they are consumed, so the next
+// build finds them zero (they start zero: cleared at allocation).
+__global__ void k_scan_down(const int *ntot, int *cnt, const int *bsum, int *start, int n) {
   __shared__ int lds[GF_BLOCK / 64 + 1];
   int nb = *ntot + 1;
   int base = blockIdx.x * SCAN_TILE;
   if (base >= nb) return;
+  int pre = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += blockDim.x) pre += bsum[b];
+  int pre_total;
+  (void)block_exclusive_scan(pre, lds, pre_total);
   int v[SCAN_ITEMS];
   int s = 0;
   for (int k = 0; k < SCAN_ITEMS; k++) {
@@ -966,10 +975,13 @@ __global__ void k_scan_down(const int *ntot, const int *cnt, const int *bsum, in
     s += v[k];
   }
   int total;
-  int ex = block_exclusive_scan(s, lds, total) + bsum[blockIdx.x];
+  int ex = block_exclusive_scan(s, lds, total) + pre_total;
   for (int k = 0; k < SCAN_ITEMS; k++) {
     int c = base + threadIdx.x * SCAN_ITEMS + k;
-    if (c < nb) start[c] = ex;
+    if (c < nb) {
+      start[c] = ex;
+      cnt[c] = 0;
+    }
     ex += v[k];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) start[nb] = n;
@@ -3177,6 +3189,8 @@ static int alloc_sets(gfsph_ctx *c, int n) {
     return rc;
   c->gridN = ((nblk(n) + 7) / 8) * 8;
   if ((rc = dalloc(c, &c->part, c->gridN))) return rc;
+  // the cell counts start zero; every build clears the ones it used
+  HIPCHK(c, hipMemsetAsync(c->cnt, 0, sizeof(int) * ((size_t)c->cap_cells + 2), c->stream));
   return GFSPH_OK;
 }
 
@@ -3266,7 +3280,6 @@ static int neighbour_build(gfsph_ctx *c, int src, int dst) {
   const int n = c->n, R = c->R;
   hipStream_t st = c->stream;
   const int nb = nblk(n);
-  int clear_blocks = 1024;
   const int *ntot = c->coff + R;
   if (R == 1) {
     k_grid<<<1, GF_BLOCK, 0, st>>>(c->gridN, c->part, c->acc, c->grid, c->coff, c->P.scale_k, c->cap_cells);
@@ -3276,11 +3289,9 @@ static int neighbour_build(gfsph_ctx *c, int src, int dst) {
     k_coff<<<1, GF_BLOCK, 0, st>>>(R, c->grid, c->coff);
   }
   c->first_build = false;
-  k_clear<<<clear_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt);
   k_keys<<<nb, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.flag, c->set[src].rid, c->grid, c->coff, ntot, c->key,
                                   c->rank, c->cnt);
   k_scan_reduce<<<c->scan_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt, c->bsum);
-  k_scan_top<<<1, GF_BLOCK, 0, st>>>(c->scan_blocks, c->bsum);
   k_scan_down<<<c->scan_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt, c->bsum, c->start, n);
   k_scatter<<<nb, GF_BLOCK, 0, st>>>(n, c->key, c->rank, c->start, c->perm);
   k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, ntot, c->perm, c->key, c->start, c->nw, c->set[src], c->set[dst]);
