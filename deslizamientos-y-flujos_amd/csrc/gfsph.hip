@@ -1500,6 +1500,50 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 #endif
 #define GATHER_MAX_BLOCK 512
 
+// Each particle's gather is shared by a team of GATHER_TEAM adjacent lanes:
+// lane t of the team takes batches t, t+T, t+2T, ... of the candidate stream
+// (density) or the neighbours of those batches (force, XSPH), keeps its own
+// partial sums, and the team adds them in a fixed tree (so every run, batch
+// and work-group size gives the same bits).  More lanes per particle keep
+// more loads in flight per SIMD when the particles alone fill it only a few
+// waves deep.
+#ifndef GATHER_TEAM
+#define GATHER_TEAM 2
+#endif
+static_assert(GATHER_TEAM == 1 || GATHER_TEAM == 2 || GATHER_TEAM == 4, "team of 1, 2 or 4 lanes");
+
+// particle and team lane of this thread
+__device__ __forceinline__ void team_index(int &i, int &t) {
+  const int g = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  i = g / GATHER_TEAM;
+  t = g % GATHER_TEAM;
+}
+
+// sum over the team (identical in every lane: each level adds two equal
+// pairs of partial sums in either order)
+__device__ __forceinline__ double team_sum(double v) {
+#pragma unroll
+  for (int o = 1; o < GATHER_TEAM; o <<= 1) v = v + __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long team_or(unsigned long long v) {
+#pragma unroll
+  for (int o = 1; o < GATHER_TEAM; o <<= 1) {
+    const unsigned lo = __shfl_xor((unsigned)v, o), hi = __shfl_xor((unsigned)(v >> 32), o);
+    v |= ((unsigned long long)hi << 32) | lo;
+  }
+  return v;
+}
+
+// mask bits of the batches (nibbles of NB_BATCH = 4 positions) team lane t takes
+__device__ __forceinline__ unsigned long long team_bits(int t) {
+  static_assert(NB_BATCH == 4, "the team split is by 4-position nibbles");
+  if (GATHER_TEAM == 1) return ~0ull;
+  if (GATHER_TEAM == 2) return 0x0F0F0F0F0F0F0F0Full << (4 * t);
+  return 0x000F000F000F000Full << (4 * t);
+}
+
 struct CandStream {
   int off[NROWS];  // first stream position of each row (off[0] = 0)
   int inc[NROWS];  // slot of stream position p: p + sum of inc[rr] over the rows with off[rr] <= p
@@ -1587,10 +1631,10 @@ __device__ __forceinline__ PredBand pred_band(double t) {
 
 // The mask words of a particle's stream (words past the stream read as 0).
 __device__ __forceinline__ void load_words(const CandStream &cs, const unsigned long long *mrow,
-                                           unsigned long long (&w)[NB_WORDS]) {
+                                           unsigned long long (&w)[NB_WORDS], unsigned long long lane_bits) {
   const int nw = (cs.total + 63) >> 6;
 #pragma unroll
-  for (int k = 0; k < NB_WORDS; k++) w[k] = k < nw ? mrow[k] : 0ull;
+  for (int k = 0; k < NB_WORDS; k++) w[k] = k < nw ? (mrow[k] & lane_bits) : 0ull;
 }
 
 // Calls f(js, ok) for batches of NB_BATCH neighbours taken from the mask
@@ -1627,10 +1671,11 @@ __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned lo
 
 // Every neighbour in the stream, with the exact predicate, for particles
 // whose streams are too long for the mask (not batched: rare).
-template <typename F>
-__device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, int i, double2 pi_, double hi,
+// Positions t, t+T, t+2T, ... of the stream (T = 1: all of them).
+template <int T = GATHER_TEAM, typename F>
+__device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, int i, int t, double2 pi_, double hi,
                                              const double4 *nb, F &&f) {
-  for (int p = 0; p < cs.total; p++) {
+  for (int p = t; p < cs.total; p += T) {
     const int j = slot_of(cs, p);
     if (j == i) continue;
     const double4 q = nb[j];
@@ -1646,13 +1691,16 @@ __device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, 
 // smoothing lengths are equal, so the support radius scale_k*(hi+hj)/2 =
 // scale_k*hi and the kernel's 1/h are the particle's own.
 template <int SKF, bool UNIFORM>
-__device__ __forceinline__ double density_stream(const CandStream &cs, const DevParams &P, int i, double2 pi_,
-                                                 double hi, const double4 *nb, unsigned long long *mrow) {
+__device__ __forceinline__ double density_stream(const CandStream &cs, const DevParams &P, int i, int t,
+                                                 double2 pi_, double hi, const double4 *nb,
+                                                 unsigned long long *mrow) {
   const PredBand pu = pred_band(P.scale_k * hi);
   const double ihi = rcp_nr(hi);
   double acc = 0.0;
-  unsigned long long word = 0ull;
-  for (int k0 = 0; k0 < cs.total; k0 += NB_BATCH) {
+  unsigned long long wd[NB_WORDS];
+#pragma unroll
+  for (int k = 0; k < NB_WORDS; k++) wd[k] = 0ull;
+  for (int k0 = t * NB_BATCH; k0 < cs.total; k0 += GATHER_TEAM * NB_BATCH) {
     int j[NB_BATCH];
     double4 q[NB_BATCH];
 #pragma unroll
@@ -1687,11 +1735,15 @@ __device__ __forceinline__ double density_stream(const CandStream &cs, const Dev
       const double t = acc + q[u].w * w;
       acc = nbr ? t : acc;
     }
-    word |= (unsigned long long)nib << (k0 & 63);
-    if (((k0 + NB_BATCH) & 63) == 0 || k0 + NB_BATCH >= cs.total) {
-      mrow[k0 >> 6] = word;
-      word = 0ull;
-    }
+    const unsigned long long bits = (unsigned long long)nib << (k0 & 63);
+#pragma unroll
+    for (int k = 0; k < NB_WORDS; k++) wd[k] |= (k == (k0 >> 6)) ? bits : 0ull;
+  }
+  const int nw = (cs.total + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NB_WORDS; k++) {
+    const unsigned long long v = team_or(wd[k]);
+    if (k < nw && k % GATHER_TEAM == t) mrow[k] = v;
   }
   return acc;
 }
@@ -1700,11 +1752,14 @@ __device__ __forceinline__ double density_stream(const CandStream &cs, const Dev
 template <int SKF>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
                                                                  double *Pout, unsigned long long *mask) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  int i, t;
+  team_index(i, t);
   if (i >= n) return;
   if (s.flag[i]) {
-    S[i] = s.h[i];
-    Pout[i] = 0.0;
+    if (t == 0) {
+      S[i] = s.h[i];
+      Pout[i] = 0.0;
+    }
     return;
   }
   const int rr_ = s.rid[i];
@@ -1715,16 +1770,19 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
   const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
   double acc = 0.0;
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-      double w, gx, gy;
-      sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-      acc = acc + q.w * w;
-    });
+    stream_dense(cs, P.scale_k, i, t, pi_, hi, s.nb,
+                 [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                   double w, gx, gy;
+                   sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                   acc = acc + q.w * w;
+                 });
   } else {
     unsigned long long *mrow = mask + NB_WORDS * (long)i;
-    acc = g.uniform_h ? density_stream<SKF, true>(cs, P, i, pi_, hi, s.nb, mrow)
-                      : density_stream<SKF, false>(cs, P, i, pi_, hi, s.nb, mrow);
+    acc = g.uniform_h ? density_stream<SKF, true>(cs, P, i, t, pi_, hi, s.nb, mrow)
+                      : density_stream<SKF, false>(cs, P, i, t, pi_, hi, s.nb, mrow);
   }
+  acc = team_sum(acc);
+  if (t != 0) return;
   const double Si = sph_self_w(SKF, P.pi, hi) * s.mass[i] + acc;
   S[i] = Si;
   const double rf = Si < P.h_inf ? P.h_inf : Si;
@@ -1757,10 +1815,11 @@ template <int SKF, int PA>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl,
                                                                 const unsigned long long *mask, SortedSet s,
                                                                 const double *S, const double *Pp, double2 *ain) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  int i, t;
+  team_index(i, t);
   if (i >= n) return;
   if (s.flag[i]) {
-    ain[i] = make_double2(0.0, 0.0);
+    if (t == 0) ain[i] = make_double2(0.0, 0.0);
     return;
   }
   const int rr_ = s.rid[i];
@@ -1774,14 +1833,15 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
   const double Pi = Pp[i];
   double inx = 0.0, iny = 0.0;
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-      double w, gx, gy;
-      sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-      pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
-    });
+    stream_dense(cs, P.scale_k, i, t, pi_, hi, s.nb,
+                 [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                   double w, gx, gy;
+                   sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                   pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
+                 });
   } else {
     unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w);
+    load_words(cs, mask + NB_WORDS * (long)i, w, team_bits(t));
     const double ihi = rcp_nr(hi);
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
@@ -1813,7 +1873,9 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
         body(std::false_type{}, js, ok);
       });
   }
-  ain[i] = make_double2(inx, iny);
+  inx = team_sum(inx);
+  iny = team_sum(iny);
+  if (t == 0) ain[i] = make_double2(inx, iny);
 }
 
 // a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
@@ -1821,11 +1883,12 @@ template <int SKF>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl,
                                                               const unsigned long long *mask, SortedSet s, NewSet o,
                                                               double2 *vout, const int *stopr) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  int i, t;
+  team_index(i, t);
   if (i >= n) return;
   double2 vi = o.vel[i];
   if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0 || (stopr && stopr[s.rid[i]])) {
-    vout[i] = vi;
+    if (t == 0) vout[i] = vi;
     return;
   }
   const int rr_ = s.rid[i];
@@ -1838,7 +1901,7 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
   const bool skip0 = P.npoin0 > 0;
   double ax = 0.0, ay = 0.0;
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+    stream_dense(cs, P.scale_k, i, t, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
       if (skip0 && s.id[j] < P.npoin0) return;
       double w, gx, gy;
       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
@@ -1849,7 +1912,7 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
     });
   } else {
     unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w);
+    load_words(cs, mask + NB_WORDS * (long)i, w, team_bits(t));
     const double ihi = rcp_nr(hi);
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
@@ -1884,6 +1947,9 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
         body(std::false_type{}, js, ok);
       });
   }
+  ax = team_sum(ax);
+  ay = team_sum(ay);
+  if (t != 0) return;
   vi.x = vi.x + 0.3 * ax;
   vi.y = vi.y + 0.3 * ay;
   vout[i] = vi;
@@ -2251,11 +2317,11 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
     gya += f * gy;
   };
   if (full) {
-    stream_dense(cs, P.scale_k, i, pi_, hi, t.nb,
+    stream_dense<1>(cs, P.scale_k, i, 0, pi_, hi, t.nb,
                  [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
   } else {
     unsigned long long mw[NB_WORDS];
-    load_words(cs, mrow, mw);
+    load_words(cs, mrow, mw, ~0ull);
     masked_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
@@ -2367,11 +2433,11 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
     lmi += lSj * wv;
   };
   if (full) {
-    stream_dense(cs, P.scale_k, i, pi_, hi, t.nb,
+    stream_dense<1>(cs, P.scale_k, i, 0, pi_, hi, t.nb,
                  [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
   } else {
     unsigned long long mw[NB_WORDS];
-    load_words(cs, mrow, mw);
+    load_words(cs, mrow, mw, ~0ull);
     masked_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
@@ -3219,7 +3285,7 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
   } while (0)
 
 #define PFORCE_PA(SKFV, PAV) \
-  k_pforce_m<SKFV, PAV><<<xcd_grid(c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS)
+  k_pforce_m<SKFV, PAV><<<xcd_grid((long)c->n * GATHER_TEAM, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS)
 #define PFORCE_CASE(SKFV)                      \
   switch (c->P.pa_sph) {                       \
     case 1: PFORCE_PA(SKFV, 1); break;         \
@@ -3306,7 +3372,7 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[0];
-  SKF_DISPATCH(k_density_m, xcd_grid(c->n, bsz), bsz, 0,
+  SKF_DISPATCH(k_density_m, xcd_grid((long)c->n * GATHER_TEAM, bsz), bsz, 0,
                c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
 }
 
@@ -3314,7 +3380,7 @@ static void launch_density(gfsph_ctx *c, SortedSet &s) {
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[2];
   if (c->P.av_vel)
-    SKF_DISPATCH(k_xsph_m, xcd_grid(c->n, bsz), bsz, 0,
+    SKF_DISPATCH(k_xsph_m, xcd_grid((long)c->n * GATHER_TEAM, bsz), bsz, 0,
                  c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, c->cases ? c->stopr : nullptr));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
