@@ -351,10 +351,10 @@ __device__ __forceinline__ void get_z_topo(const DevDem &d, double xx, double yy
 // Bingham intermediate regime (6:9609-9618): returns true and Tau0 when
 // 0.01 <= a_hat <= 50.
 __device__ __forceinline__ bool bingham_tau0(const DevParams &P, double hvmod, double hfrict, double &Tau0) {
-  double a_hat = (6. * P.constK * hvmod) / (P.tauy0 * pow(hfrict, 2.));
+  double a_hat = (6. * P.constK * hvmod) / (P.tauy0 * (hfrict * hfrict));
   if (a_hat > 50 || a_hat < 0.01) return false;
   double a2 = 48., b2 = -32. * a_hat - 114., c2 = 65.;
-  double xh = (-b2 - pow(b2 * b2 - 4. * a2 * c2, 0.5)) / (2. * a2);
+  double xh = (-b2 - sqrt(b2 * b2 - 4. * a2 * c2)) / (2. * a2);
   Tau0 = P.tauy0 / xh;
   return true;
 }
@@ -465,13 +465,13 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     double hp = pow(hfrict, P.c73);
     frictx = cf1 * hu * hvmod / hp;
     fricty = cf1 * hv * hvmod / hp;
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (nfric == 2) {
     double cf2a = P.constK / (8. * P.dens);
-    double h2 = pow(hfrict, 2.);
+    double h2 = (hfrict * hfrict);
     frictx = cf2a * hu / h2;
     fricty = cf2a * hv / h2;
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (nfric == 4 || nfric == 24 || nfric == 25) {
     const double Tauy0 = P.tauy0, constK = P.constK, m_vexp = P.visco;
     if (nfric == 4) {
@@ -491,13 +491,13 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       frictx = cf4b * hu * hvmodinv;
       fricty = cf4b * hv * hvmodinv;
     }
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (P.if_frictional == 1) {
     double cv2byR = 0.0, tanfi;
     if (P.if_V2byR == 1) {
       vxx = hu * hinv;
       vyy = hv * hinv;
-      vmod = pow(vxx * vxx + vyy * vyy, 0.5);
+      vmod = sqrt(vxx * vxx + vyy * vyy);
       if (vmod >= 1e-5) {
         double cosa = vxx / vmod, sina = vyy / vmod;
         double cos2 = cosa * cosa, sin2 = sina * sina, cbs = sina * cosa;
@@ -509,7 +509,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     tanfi = P.tanfi8;
     if (nfric == 8 || nfric == 26 || nfric == 27) {
       xi_voellmy = 0.0;
-      double Froude = vmod / pow(gravz * h, 0.5);
+      double Froude = vmod / sqrt(gravz * h);
       if (nfric == 8 || nfric == 26) {
         double ff = (1 + P.c3 * pow(Froude * P.c6 / h, P.c7));
         tanfi = tanfi * ff;
@@ -551,19 +551,19 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     }
     if (nfric == 9 || nfric == 10) {
       double C67 = P.c6 + P.c7;
-      double frict_abs = pow(frictx * frictx + fricty * fricty, 0.5);
+      double frict_abs = sqrt(frictx * frictx + fricty * fricty);
       double fact9 = pow(2. * P.c8 * vmod / hfrict, 1. / C67) * pow(frict_abs, -P.c6 / C67);
       if (P.c47 > 1.e-6) fact9 = fact9 * ((1 - P.c47) / (1 + P.c47));
       frictx = frictx * (1 + fact9);
       fricty = fricty * (1 + fact9);
     }
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (nfric == 11) {
     vxx = hu * hinv;
     vyy = hv * hinv;
     frictx = P.K_Bagnold * vxx * vmod / (hfrict * hfrict);
     fricty = P.K_Bagnold * vyy * vmod / (hfrict * hfrict);
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   }
 
   // limiter (6:9836-9895)
@@ -584,13 +584,13 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       }
     }
   } else {
-    double accelT = pow(ax * ax + ay * ay, 0.5);
+    double accelT = sqrt(ax * ax + ay * ay);
     if (accelT > 1.e-5) {
       double ctf = TotFrict;
       if (P.if_frictional == 1) ctf = cf5;
       frictx = ctf * ax / accelT;
       fricty = ctf * ay / accelT;
-      double fricT = pow(frictx * frictx + fricty * fricty, 0.5);
+      double fricT = sqrt(frictx * frictx + fricty * fricty);
       if (accelT <= fricT) {
         frictx = ax;
         fricty = ay;
@@ -1511,6 +1511,10 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 #define GATHER_TEAM 2
 #endif
 static_assert(GATHER_TEAM == 1 || GATHER_TEAM == 2 || GATHER_TEAM == 4, "team of 1, 2 or 4 lanes");
+// default work-group size of the gathers (gfsph_autotune may change it)
+#ifndef GATHER_BLOCK
+#define GATHER_BLOCK 64
+#endif
 
 // particle and team lane of this thread
 __device__ __forceinline__ void team_index(int &i, int &t) {
@@ -2487,7 +2491,7 @@ __global__ void k_misfit_vjp(int n, DevDem dem, CurView s, const double *acc5, c
   }
   const double2 p = s.pos[i];
   const double h = s.h[i];
-  const double distg = pow(dem.dx * dem.dx + dem.dy * dem.dy, 0.5);
+  const double distg = sqrt(dem.dx * dem.dx + dem.dy * dem.dy);
   int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
   int inx0 = ipx - 1 > 1 ? ipx - 1 : 1, iny0 = ipy - 1 > 1 ? ipy - 1 : 1;
   int inx1 = ipx + 2 < dem.nx ? ipx + 2 : dem.nx, iny1 = ipy + 2 < dem.ny ? ipy + 2 : dem.ny;
@@ -2500,7 +2504,7 @@ __global__ void k_misfit_vjp(int n, DevDem dem, CurView s, const double *acc5, c
       const double R = res[2 * g], e2 = 2.0 * res[2 * g + 1];
       double gx = dem.xmin + (inx - 1) * dem.dx, gy = dem.ymin + (iny - 1) * dem.dy;
       double a = p.x - gx, b = p.y - gy;
-      const double rho = pow(a * a + b * b, 0.5);
+      const double rho = sqrt(a * a + b * b);
       const double dist = fmax(1.e-6, rho) / distg;
       lh += e2 / (dist * ws);
       if (rho > 1.e-6) {
@@ -2566,7 +2570,7 @@ __global__ void k_fdiff(int nx, int ny, double dxg, double dyg, const double *Z,
     double zTL = Z[k + nx - 1], zTR = Z[k + nx + 1], zBL = Z[k - nx - 1], zBR = Z[k - nx + 1];
     zx = (zR - zL) / del2x;
     zy = (zT - zB) / del2y;
-    double rt = pow(1. + zx * zx + zy * zy, 0.5);
+    double rt = sqrt(1. + zx * zx + zy * zy);
     double zxx = (zL - 2. * zA + zR) / delx2;
     double zyy = (zB - 2. * zA + zT) / dely2;
     double zxy2 = (zTR - zBR - zTL + zBL) / (4. * delxy);
@@ -2834,7 +2838,7 @@ __global__ void k_raster_idw(int n, DevDem dem, CurView s, double *acc5) {
   double2 p = s.pos[i];
   double2 v = s.vel[i];
   double h = s.h[i], uu = s.u[i];
-  double distg = pow(dem.dx * dem.dx + dem.dy * dem.dy, 0.5);
+  double distg = sqrt(dem.dx * dem.dx + dem.dy * dem.dy);
   int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
   int inx0 = ipx - 1 > 1 ? ipx - 1 : 1, iny0 = ipy - 1 > 1 ? ipy - 1 : 1;
   int inx1 = ipx + 2 < dem.nx ? ipx + 2 : dem.nx, iny1 = ipy + 2 < dem.ny ? ipy + 2 : dem.ny;
@@ -2845,7 +2849,7 @@ __global__ void k_raster_idw(int n, DevDem dem, CurView s, double *acc5) {
       double a = p.x - gx, b = p.y - gy;
       a = a * a;
       b = b * b;
-      double dist = fmax(1.e-6, pow(a + b, 0.5));
+      double dist = fmax(1.e-6, sqrt(a + b));
       dist = dist / distg;
       atomicAdd(&acc5[5 * gidx + 0], h / dist);
       atomicAdd(&acc5[5 * gidx + 1], v.x / dist);
@@ -2964,7 +2968,7 @@ struct gfsph_ctx {
   hipGraphExec_t graphG[2] = {nullptr, nullptr};  // GRAPH_STEPS steps from set[k]
   // work-group size of each gather kernel (density, force, XSPH), chosen by
   // gfsph_autotune; all sizes give identical results
-  int gbs[3] = {GF_BLOCK, GF_BLOCK, GF_BLOCK};
+  int gbs[3] = {GATHER_BLOCK, GATHER_BLOCK, GATHER_BLOCK};
   int *bad = nullptr;
   double2 *zxy = nullptr;
   double *top = nullptr;

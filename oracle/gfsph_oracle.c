@@ -408,7 +408,7 @@ void gfo_fdiff_derivs(int nx, int ny, double dxg, double dyg, const int32_t *in_
 #define Z(n) topol[(n)*9]
       double zx = (Z(iAR) - Z(iAL)) / del2x;
       double zy = (Z(iAT) - Z(iAB)) / del2y;
-      double rt = pow(1. + zx * zx + zy * zy, 0.5);
+      double rt = sqrt(1. + zx * zx + zy * zy);
       double zxx = (Z(iAL) - 2. * Z(iA) + Z(iAR)) / delx2;
       double zyy = (Z(iAB) - 2. * Z(iA) + Z(iAT)) / dely2;
       double zxy = (Z(iATR) - Z(iABR) - Z(iATL) + Z(iABL)) / (4. * delxy);
@@ -572,21 +572,21 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
     double hp = pow(hfrict, D->c73);
     frictx = cf1 * hu * hvmod / hp;
     fricty = cf1 * hv * hvmod / hp;
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (nfric == 2) { /* 6:9432 */
     double cf2a = D->constK / (8. * D->dens);
-    frictx = cf2a * hu / pow(hfrict, 2.);
-    fricty = cf2a * hv / pow(hfrict, 2.);
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    frictx = cf2a * hu / (hfrict * hfrict);
+    fricty = cf2a * hv / (hfrict * hfrict);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (nfric == 4 || nfric == 24 || nfric == 25) { /* 6:9538-9623 */
     double Tauy0 = D->tauy0, m_vexp = D->visco, Tau0 = *tau0;
     double constMu_CF = D->constK;
     if (nfric == 4) {
       double constK = D->constK;
-      double a_hat = (6. * constK * hvmod) / (Tauy0 * pow(hfrict, 2.));
+      double a_hat = (6. * constK * hvmod) / (Tauy0 * (hfrict * hfrict));
       if (!(a_hat > 50) && !(a_hat < 0.01)) { /* intermediate regime */
         double a2 = 48., b2 = -32. * a_hat - 114., c2 = 65.;
-        double xh = (-b2 - pow(b2 * b2 - 4. * a2 * c2, 0.5)) / (2. * a2);
+        double xh = (-b2 - sqrt(b2 * b2 - 4. * a2 * c2)) / (2. * a2);
         Tau0 = Tauy0 / xh;
         *tau0 = Tau0;
       }
@@ -601,13 +601,13 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
       frictx = cf4b * hu * hvmodinv;
       fricty = cf4b * hv * hvmodinv;
     }
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (if_frictional == 1) { /* 6:9624-9789 */
     double cv2byR = 0.0, tanfi;
     if (if_V2byR == 1) {
       vxx = hu * hinv;
       vyy = hv * hinv;
-      vmod = pow(vxx * vxx + vyy * vyy, 0.5);
+      vmod = sqrt(vxx * vxx + vyy * vyy);
       cv2byR = 0.0;
       if (vmod >= 1e-5) {
         double cosa = vxx / vmod, sina = vyy / vmod;
@@ -622,12 +622,12 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
       xi_voellmy = 0.0;
       if (nfric == 8 || nfric == 26) {
         double b_muI = p->cst[3], r_muI = p->cst[6], n_muI = p->cst[7];
-        double Froude = vmod / pow(gravz * h, 0.5);
+        double Froude = vmod / sqrt(gravz * h);
         double ff = (1 + b_muI * pow(Froude * r_muI / h, n_muI));
         tanfi = tanfi * ff;
       } else {
         double Mu2 = p->cst[3], r_muI = p->cst[6], Inert0 = p->cst[7];
-        double Froude = vmod / pow(gravz * h, 0.5);
+        double Froude = vmod / sqrt(gravz * h);
         double Inert = Froude * r_muI / h;
         tanfi = tanfi + (Mu2 - tanfi) / (1.0 + (Inert0 / Inert));
       }
@@ -665,7 +665,7 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
     }
     if (nfric == 9 || nfric == 10) { /* Perzyna 6:9767-9787 */
       double C6 = p->cst[6], C7 = p->cst[7], C8 = p->cst[8], C67 = C6 + C7;
-      double frict_abs = pow(frictx * frictx + fricty * fricty, 0.5);
+      double frict_abs = sqrt(frictx * frictx + fricty * fricty);
       double fact9 = pow(2. * C8 * vmod / hfrict, 1. / C67) * pow(frict_abs, -C6 / C67);
       if (p->cst[47] > 1.e-6) { /* 6:9780-9784 */
         double alpha_slip = p->cst[47];
@@ -674,13 +674,13 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
       frictx = frictx * (1 + fact9);
       fricty = fricty * (1 + fact9);
     }
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (nfric == 11) { /* Bagnold 6:9791 */
     vxx = hu * hinv;
     vyy = hv * hinv;
     frictx = D->K_Bagnold * vxx * vmod / (hfrict * hfrict);
     fricty = D->K_Bagnold * vyy * vmod / (hfrict * hfrict);
-    TotFrict = pow(frictx * frictx + fricty * fricty, 0.5);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   }
 
   /* limiter (6:9836-9895) */
@@ -701,13 +701,13 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
       }
     }
   } else {
-    double accelT = pow(ax * ax + ay * ay, 0.5);
+    double accelT = sqrt(ax * ax + ay * ay);
     if (accelT > 1.e-5) {
       double ctf = TotFrict;
       if (if_frictional == 1) ctf = cf5;
       frictx = ctf * ax / accelT;
       fricty = ctf * ay / accelT;
-      double fricT = pow(frictx * frictx + fricty * fricty, 0.5);
+      double fricT = sqrt(frictx * frictx + fricty * fricty);
       if (accelT <= fricT) {
         frictx = ax;
         fricty = ay;
@@ -1126,7 +1126,7 @@ void gfo_raster_idw(const gfo_dem *d, int n, const double *x, const double *v, c
   int64_t np = (int64_t)d->nx * d->ny;
   memset(out, 0, sizeof(double) * 4 * np);
   memset(wsum, 0, sizeof(double) * np);
-  double distg = pow(d->dx * d->dx + d->dy * d->dy, 0.5);
+  double distg = sqrt(d->dx * d->dx + d->dy * d->dy);
   for (int i = 0; i < n; i++) {
     if (flag[i]) continue;
     double xx = x[2 * i], yy = x[2 * i + 1];
@@ -1144,7 +1144,7 @@ void gfo_raster_idw(const gfo_dem *d, int n, const double *x, const double *v, c
         double a = xx - gx, b = yy - gy;
         a = a * a;
         b = b * b;
-        double dist = fmax(1.e-6, pow(a + b, 0.5));
+        double dist = fmax(1.e-6, sqrt(a + b));
         dist = dist / distg;
         out[4 * g + 0] += rho[i] / dist;
         out[4 * g + 1] += v[2 * i] / dist;
@@ -1177,7 +1177,7 @@ void gfo_mc_pixel_accum(const gfo_dem *d, int n, const double *x, const double *
     int64_t g1 = (int64_t)(ipy - 1) * d->nx + ipx - 1;
     int64_t nodes[4] = {g1, g1 + 1, g1 + 1 + d->nx, g1 + d->nx};
     double hh = rho[i];
-    double vv = pow(v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1], 0.5);
+    double vv = sqrt(v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1]);
     for (int k = 0; k < 4; k++) {
       int64_t g = nodes[k];
       cnt[g] += 1;
