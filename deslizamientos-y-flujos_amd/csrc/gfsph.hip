@@ -59,6 +59,10 @@ struct DevParams {
   // eros_single: the realisation holds one particle, whose depth then grows
   // with the eroded depth too (6:8756-8758)
   int eros_on, eros_law, eros_single, pad_e;
+  // icpwp = 2: column of npwp nodes (0: no column), its bed / surface
+  // conditions and the consolidation coefficient Cv = 4 Bfact / pi^2
+  int npwp, bc_t1, bc_t2, pad_c;
+  double bc_v1, bc_v2, Cv;
   double keros, eros_c23, eros_c24, eros_c25, denss, densw_e, dens_bar_eff, slip_fact, avail0;
 };
 
@@ -94,7 +98,8 @@ struct SortedSet {
   double2 *pos, *vel;
   double *h, *mass, *hsml, *u, *uaux;
   int *flag, *id;
-  int *rid;  // realisation of the particle (0 for a single run)
+  int *rid;     // realisation of the particle (0 for a single run)
+  double *col;  // icpwp = 2: pore-pressure columns [npwp][n] (node-major), else null
 };
 
 // Cell index of one or several independent realisations: realisation r has
@@ -112,6 +117,8 @@ struct NewSet {
   double2 *pos, *vel;
   double *h, *hsml, *u, *uaux;
   int *flag;
+  double *col;  // icpwp = 2: columns after the step [npwp][n]
+  double *dh;   // icpwp = 2: the step's depth change Drho_RK, read by the column update
 };
 
 // The state after a step: positions, depths, smoothing lengths, pore
@@ -455,7 +462,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   exy = exy + (-P.cgra * tp.Zy);
 
   if (P.icpwp == 1) pwp = u_i;
-  else if (P.icpwp == 3) pwp = uaux1;
+  else if (P.icpwp == 2 || P.icpwp == 3) pwp = uaux1;  // Uaux(1,:), the bed value
   else pwp = 0.0;
   hfrict = fmax(h, P.hfrict0);
   const int nfric = P.nfric;
@@ -522,7 +529,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     if (pwp > P.fpwp_max * hhh) pwp = P.fpwp_max * hhh;
     if (pwp < P.fpwp_min * hhh) pwp = P.fpwp_min * hhh;
     u_i = pwp;  // 6:9675
-    if (P.icpwp == 1 || P.icpwp == 3) {
+    if (P.icpwp == 1 || P.icpwp == 2 || P.icpwp == 3) {
       double factpw = 1. - pwp * hinv;
       if (factpw < P.fact_pwp_min) factpw = P.fact_pwp_min;
       if (factpw > P.fact_pwp_max) factpw = P.fact_pwp_max;
@@ -1016,7 +1023,7 @@ __global__ void k_scatter(int n, const int *key, const int *rank, const int *sta
 // length / pore pressure / flags from the previous step's outputs (src_new),
 // mass / id / velocity from the previous sorted set (src_old).
 __global__ void k_reorder(int n, const int *ntot, const int *perm, const int *key, const int *start,
-                          NewSet src_new, SortedSet src_old, SortedSet dst) {
+                          NewSet src_new, SortedSet src_old, SortedSet dst, int ncol) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int i = perm[t];
@@ -1040,6 +1047,7 @@ __global__ void k_reorder(int n, const int *ntot, const int *perm, const int *ke
   dst.flag[d] = src_new.flag[i];
   dst.id[d] = src_old.id[i];
   dst.rid[d] = src_old.rid[i];
+  for (int k = 0; k < ncol; k++) dst.col[(long)k * n + d] = src_new.col[(long)k * n + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1183,7 +1191,13 @@ __device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem
     s.nb[i].w = mnew;  // XSPH reads the masses after erosion (7:987)
   }
   double uaux_new = uaux1;
-  if (P.nuaux) {
+  if (P.npwp) {
+    // the column is updated by k_pwp_column after the stages (it needs the
+    // step's depth change); the bed value and u follow it there
+    o.dh[i] = hS - h0;
+    if (flagged) uaux_new = 0.0;
+    u = uaux_new;
+  } else if (P.nuaux) {
     if (flagged) {
       uaux_new = 0.0;
     } else {
@@ -2543,8 +2557,108 @@ __global__ void k_init_pwp(int n, DevParams P, double alfs, double pwp0rel, int 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pore pressure on a column of N = npwp nodes per particle (icpwp = 2):
+// Get_Pwp_SW for npwp > 1 (6:10966-11174), single phase, no boundary zones.
+// Each particle's column, bed (node 0) to surface, is consolidated
+// explicitly (FTCS) over the saturated depth h alphaS in ntpwp sub-steps of
+// dt_sph, with the loading dh/ntpwp (1 - z/h), clamps that depend on the
+// height, and the bed / surface conditions (1 value, 2 gradient).  The
+// columns are node-major ([N][n]) so that the lanes' accesses to one node
+// are contiguous; the sweep runs in place with the previous node's old value
+// kept in a register.  The sub-step count is capped at PWP_MAX_SUBSTEPS so
+// that every lane ends (the cap is reached only for a vanishing depth).
+// ---------------------------------------------------------------------------
+#define PWP_MAX_SUBSTEPS (1 << 16)
+
+__global__ void k_pwp_column(int n, const DevParams *Pr, SortedSet s, NewSet o, const int *stopr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = s.rid[i];
+  const DevParams &P = Pr[r];
+  const int N = P.npwp;
+  const double *src = s.col + i;
+  double *col = o.col + i;
+  if (stopr && stopr[r]) {  // a stopped case does not change
+    for (int k = 0; k < N; k++) col[(long)k * n] = src[(long)k * n];
+    return;
+  }
+  if (o.flag[i]) {  // Uaux(:, ipoin) = 0 out of the domain (6:10977-10980)
+    for (int k = 0; k < N; k++) col[(long)k * n] = 0.0;
+    o.uaux[i] = 0.0;
+    o.u[i] = 0.0;
+    return;
+  }
+  for (int k = 0; k < N; k++) col[(long)k * n] = src[(long)k * n];
+  const double Cv = P.Cv, alphaS = P.alphaS, dt = P.dt;
+  const double hh = o.h[i], hpwp = hh * alphaS, dhh = o.dh[i];
+  const double dxpwp = hpwp / (N - 1), dxpwp2 = dxpwp * dxpwp;
+  double dtpwp = (Cv > 1.e-20) ? dxpwp2 / (2 * Cv) : dxpwp2 / (2.e-20);
+  const double q = ceil(dt / dtpwp);
+  const int ntpwp = (q >= 1.0 && q <= PWP_MAX_SUBSTEPS) ? (int)q : (q >= 1.0 ? PWP_MAX_SUBSTEPS : 1);
+  dtpwp = dt / (double)ntpwp;
+  const double dhhstep = dhh / ntpwp, dpwp_dn_step = 0.0 / ntpwp;
+  const double r_ = Cv * dtpwp / dxpwp2;
+  const double fmax_ = P.fpwp_max, fmin_ = P.fpwp_min;
+  for (int it = 0; it < ntpwp; it++) {
+    double prev = col[0], cur = col[(long)n];
+    for (int k = 1; k < N - 1; k++) {
+      const double next = col[(long)(k + 1) * n];
+      const double zz = dxpwp * k, zzhh = hh - zz, xi = zz / hh;
+      const double sumdhh = dhhstep * (1. - xi);
+      double v = cur + r_ * (prev - 2. * cur + next) + sumdhh + dpwp_dn_step;
+      if (v > (zzhh * fmax_)) v = (zzhh * fmax_);
+      else if (v < (zzhh * fmin_)) v = (zzhh * fmin_);
+      col[(long)k * n] = v;
+      prev = cur;
+      cur = next;
+    }
+    double b = col[0];
+    if (P.bc_t1 == 1) b = P.bc_v1 * hh * alphaS;
+    else if (P.bc_t1 == 2) b = (4. * col[(long)n] - col[(long)2 * n] - 2. * P.bc_v1 * dxpwp) / 3.;
+    if (b > hh) b = hh;
+    col[0] = b;  // before the surface condition, which reads node 0 when N = 3
+    double t = col[(long)(N - 1) * n];
+    if (P.bc_t2 == 1) t = P.bc_v2 * hh;
+    else if (P.bc_t2 == 2) t = (4. * col[(long)(N - 2) * n] - col[(long)(N - 3) * n] + 2. * P.bc_v2 * dxpwp) / 3.;
+    if (t > 0.0) t = 0.0;
+    col[(long)(N - 1) * n] = t;
+  }
+  o.uaux[i] = col[0];
+  o.u[i] = col[0];  // u = Uaux(1, :) (7:956-958)
+}
+
+// Initialize_Pwp for icpwp = 2 (6:1769-1803): the initial column profile of
+// every particle from its (re-summed) depth
+__global__ void k_init_column(int n, const DevParams *Pr, int pwp0_type, double pwp0rel, double xirel, SortedSet s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int N = Pr[s.rid[i]].npwp;
+  const double hh = s.h[i], pi = 4. * atan(1.0);
+  for (int k = 0; k < N; k++) {
+    double xi, v = 0.0;
+    if (pwp0_type == 0) {
+      v = pwp0rel * hh;
+    } else if (pwp0_type == 1) {
+      xi = 1. * k / (N - 1);
+      v = (1 - xi) * pwp0rel * hh;
+    } else if (pwp0_type == 2) {
+      xi = (k + 1 - 1.) / (N - 1.);
+      v = cos(pi * xi / 2) * pwp0rel * hh;
+    } else if (pwp0_type == 3) {
+      v = (k == 0) ? pwp0rel * hh : 0.0;
+    } else {
+      xi = (k + 1 - 1.) / (N - 1.);
+      v = (xi < xirel) ? hh * pwp0rel * (1 - xi) : 0.0;
+    }
+    s.col[(long)k * n + i] = v;
+  }
+  s.uaux[i] = s.col[i];
+  s.u[i] = s.col[i];
+}
+
 // new-set view of a sorted set (used to re-sort the initial state)
-__global__ void k_copy_to_new(int n, SortedSet s, NewSet o) {
+__global__ void k_copy_to_new(int n, SortedSet s, NewSet o, int ncol) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   o.pos[i] = s.pos[i];
@@ -2553,6 +2667,7 @@ __global__ void k_copy_to_new(int n, SortedSet s, NewSet o) {
   o.u[i] = s.u[i];
   o.uaux[i] = s.uaux[i];
   o.flag[i] = s.flag[i];
+  for (int k = 0; k < ncol; k++) o.col[(long)k * n + i] = s.col[(long)k * n + i];
 }
 
 // FD nodal derivatives (Fdiff_derivs, 2:2812-2877)
@@ -2607,6 +2722,28 @@ __global__ void k_unsort(int n, CurView s, double2 *x, double2 *v, double *h, do
   uaux[d] = s.uaux[i];
   hsml[d] = s.hsml[i];
   flag[d] = s.flag[i];
+}
+
+// pore-pressure columns between the caller's layout [n][N] (by id) and the
+// current state's node-major slots
+__global__ void k_col_get(int n, int N, const int *id, const double *col, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long d = (long)id[i] * N;
+  for (int k = 0; k < N; k++) out[d + k] = col[(long)k * n + i];
+}
+
+__global__ void k_col_set(int n, int N, const int *id, const double *in, double *col_new, double *col_sorted,
+                          double *uaux, double *u) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long d = (long)id[i] * N;
+  for (int k = 0; k < N; k++) {
+    col_new[(long)k * n + i] = in[d + k];
+    col_sorted[(long)k * n + i] = in[d + k];
+  }
+  uaux[i] = in[d];
+  u[i] = in[d];
 }
 
 __global__ void k_unsort_mass(int n, CurView s, double *mass) {
@@ -3152,8 +3289,8 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
     err = "sle must be 0 or 1";
     return GFSPH_ERR_UNSUPPORTED;
   }
-  if (!(p->icpwp == 0 || p->icpwp == 1 || p->icpwp == 3)) {
-    err = "icpwp must be 0, 1 or 3";
+  if (!(p->icpwp == 0 || p->icpwp == 1 || p->icpwp == 2 || p->icpwp == 3)) {
+    err = "icpwp must be 0, 1, 2 or 3";
     return GFSPH_ERR_UNSUPPORTED;
   }
   if (!(p->integrator == 4 || p->integrator == 2)) {
@@ -3172,8 +3309,32 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
     err = "Bingham yield stress const(6) must exceed 1e-32";
     return GFSPH_ERR_ARG;
   }
+  if (p->icpwp == 2) {
+    if (p->nuaux < 3 || p->nuaux > 1024) {
+      err = "icpwp = 2 needs a column of 3 to 1024 nodes (nuaux)";
+      return GFSPH_ERR_ARG;
+    }
+    for (int k = 0; k < 2; k++)
+      if (!(p->pwp_bc_type[k] == 1 || p->pwp_bc_type[k] == 2)) {
+        err = "pore-pressure column conditions must be of type 1 (value) or 2 (gradient)";
+        return GFSPH_ERR_UNSUPPORTED;
+      }
+    if (p->pwp_bc_val[0] < -900) {
+      err = "the drained-bed value (BC_pwp_val(1) < -900) is outside the implemented path";
+      return GFSPH_ERR_UNSUPPORTED;
+    }
+    if (p->pwp0_type < 0 || p->pwp0_type > 4) {
+      err = "pwp0_type must be 0..4";
+      return GFSPH_ERR_ARG;
+    }
+  }
   P.icpwp = p->icpwp;
-  P.nuaux = (p->icpwp == 3) ? 1 : 0;
+  P.nuaux = (p->icpwp == 3 || p->icpwp == 2) ? 1 : 0;  // a bed value Uaux(1,:) drives the friction
+  P.npwp = (p->icpwp == 2) ? p->nuaux : 0;
+  P.bc_t1 = p->pwp_bc_type[0];
+  P.bc_t2 = p->pwp_bc_type[1];
+  P.bc_v1 = p->pwp_bc_val[0];
+  P.bc_v2 = p->pwp_bc_val[1];
   P.pa_sph = p->pa_sph;
   P.skf = p->skf;
   P.scale_k = (p->skf == 1 || p->skf == 4) ? 2 : 3;
@@ -3209,6 +3370,7 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
   P.c8 = c[8];
   P.c47 = c[47];
   P.pi = 4.0 * atan(1.0);
+  P.Cv = 4 * c[13] / (P.pi * P.pi);  // Get_Pwp_SW (6:10926)
   P.kc = p->skf == 1 ? (15.0 / 7.0) / P.pi : (7.0 / 4.0) / P.pi;
   if (p->integrator == 4) {
     const double f1[4] = {0., 0.5, 0.5, 1.0}, f2[4] = {1., 2., 2., 1.0};
@@ -3244,6 +3406,14 @@ static int alloc_sets(gfsph_ctx *c, int n) {
       (rc = dalloc(c, &o.hsml, n)) || (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) ||
       (rc = dalloc(c, &o.flag, n)))
     return rc;
+  if (c->P.npwp) {  // pore-pressure columns, node-major, cleared
+    const size_t nc = (size_t)c->P.npwp * n;
+    if ((rc = dalloc(c, &c->set[0].col, nc)) || (rc = dalloc(c, &c->set[1].col, nc)) || (rc = dalloc(c, &o.col, nc)) ||
+        (rc = dalloc(c, &o.dh, n)))
+      return rc;
+    for (double *q : {c->set[0].col, c->set[1].col, o.col})
+      HIPCHK(c, hipMemsetAsync(q, 0, sizeof(double) * nc, c->stream));
+  }
   if ((rc = dalloc(c, &c->mask, NB_WORDS * (size_t)n)) || (rc = dalloc(c, &c->ain, n))) return rc;
   if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
       (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
@@ -3312,6 +3482,12 @@ static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
 #undef PFORCE_ARGS
 }
 
+// the pore-pressure columns after the stages (icpwp = 2)
+static void launch_column(gfsph_ctx *c, SortedSet &s) {
+  if (c->P.npwp)
+    k_pwp_column<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->Pr, s, c->nw, c->cases ? c->stopr : nullptr);
+}
+
 // RK stages of one step on the sorted set s (pressure force in c->ain).
 static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   const int n = c->n;
@@ -3328,6 +3504,7 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
                                                  c->R == 1 ? c->part : nullptr, c->cases ? c->stopr : nullptr,
                                                  c->eg);
     }
+    launch_column(c, s);
     return;
   }
   BinghamState &b = c->bs;
@@ -3340,6 +3517,7 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
     k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->P, c->Pr, c->dem, s, c->S, b);
   }
   k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->Pr, c->dem, s, c->S, b, c->nw, c->part);
+  launch_column(c, s);
 }
 
 // Neighbour build into set[dst] from (nw, set[src]).  A single run takes its
@@ -3364,7 +3542,8 @@ static int neighbour_build(gfsph_ctx *c, int src, int dst) {
   k_scan_reduce<<<c->scan_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt, c->bsum);
   k_scan_down<<<c->scan_blocks, GF_BLOCK, 0, st>>>(ntot, c->cnt, c->bsum, c->start, n);
   k_scatter<<<nb, GF_BLOCK, 0, st>>>(n, c->key, c->rank, c->start, c->perm);
-  k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, ntot, c->perm, c->key, c->start, c->nw, c->set[src], c->set[dst]);
+  k_reorder<<<nb, GF_BLOCK, 0, st>>>(n, ntot, c->perm, c->key, c->start, c->nw, c->set[src], c->set[dst],
+                                     c->P.npwp);
   HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
 }
@@ -3532,6 +3711,9 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
       if (Ph[r].nfric == 4) {
         err = "Bingham (nfric = 4) couples the particles of a run; it is not batched";
         rc = GFSPH_ERR_UNSUPPORTED;
+      } else if (Ph[r].npwp != Ph[0].npwp) {
+        err = "the realisations of a batch must share the pore-pressure column size";
+        rc = GFSPH_ERR_ARG;
       } else if (Ph[r].cgra != Ph[0].cgra || Ph[r].alphaS != Ph[0].alphaS) {
         err = "the realisations of a batch must share const(1) and const(14)";
         rc = GFSPH_ERR_ARG;
@@ -3645,7 +3827,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   HIPCHK(c, hipMemcpyAsync(s0.u, u ? u : zd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
   // a state that is already initialised (init_input = 0) carries its bottom
   // pore pressure in u: the reference keeps u = Uaux(1,:) (7:956-958)
-  const double *ua0 = (!p->init_input && p->icpwp == 3 && u) ? u : zd.data();
+  const double *ua0 = (!p->init_input && (p->icpwp == 3 || p->icpwp == 2) && u) ? u : zd.data();
   HIPCHK(c, hipMemcpyAsync(s0.uaux, ua0, sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.flag, zeros.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(s0.id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
@@ -3667,7 +3849,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   HIPCHK(c, hipMemcpyAsync(c->Pr, Ph.data(), sizeof(DevParams) * R, hipMemcpyHostToDevice, st));
   const int nb = nblk(n);
   k_domain_flag<<<nb, GF_BLOCK, 0, st>>>(n, c->dem, s0.pos, s0.flag);
-  k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw);
+  k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw, c->P.npwp);
   k_bbox<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
   if ((rc = neighbour_build(c, 0, 1))) return rc;
   c->cur = 1;
@@ -3675,8 +3857,10 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   int resum = (p->init_input && p->icpwp != 0 && p->summation_density) ? 1 : 0;
   if (resum) launch_density(c, s1);
   if (p->init_input) k_init_pwp<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->P.alphaS, p->pwp0rel, resum, c->S, s1);
+  if (p->init_input && c->P.npwp)
+    k_init_column<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, p->pwp0_type, p->pwp0rel, p->xirel, s1);
   // the step begins by re-sorting from the "new" set: publish the state there
-  k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s1, c->nw);
+  k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s1, c->nw, c->P.npwp);
   k_bbox<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(st));
@@ -3784,6 +3968,34 @@ int gfsph_get_mass(gfsph_ctx *c, double *mass) {
   return GFSPH_OK;
 }
 
+static int column_copy(gfsph_ctx *c, double *col, bool up) {
+  if (!c || c->n <= 0 || !col) return GFSPH_ERR_ARG;
+  if (!c->P.npwp) return fail(c, GFSPH_ERR_ARG, "no pore-pressure column (icpwp /= 2)");
+  const int n = c->n, N = c->P.npwp;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double *tmp = nullptr;
+  const size_t b = sizeof(double) * (size_t)n * N;
+  hipError_t e = hipMalloc((void **)&tmp, b);
+  SortedSet &s = c->set[c->cur];
+  if (!e && up) e = hipMemcpyAsync(tmp, col, b, hipMemcpyHostToDevice, c->stream);
+  if (!e) {
+    if (up)
+      k_col_set<<<nblk(n), GF_BLOCK, 0, c->stream>>>(n, N, s.id, tmp, c->nw.col, s.col, c->nw.uaux, c->nw.u);
+    else
+      k_col_get<<<nblk(n), GF_BLOCK, 0, c->stream>>>(n, N, s.id, c->nw.col, tmp);
+    e = hipGetLastError();
+  }
+  if (!e && !up) e = hipMemcpyAsync(col, tmp, b, hipMemcpyDeviceToHost, c->stream);
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (tmp) (void)hipFree(tmp);
+  if (e) return fail(c, GFSPH_ERR_HIP, std::string("pwp column: ") + hipGetErrorString(e));
+  return GFSPH_OK;
+}
+
+int gfsph_get_pwp_column(gfsph_ctx *c, double *col) { return column_copy(c, col, false); }
+
+int gfsph_set_pwp_column(gfsph_ctx *c, const double *col) { return column_copy(c, const_cast<double *>(col), true); }
+
 static int erosion_copy(gfsph_ctx *c, int32_t r, double *z, double *eroded, double *avail, bool up) {
   if (!c || c->n <= 0) return GFSPH_ERR_ARG;
   if (!c->eros) return fail(c, GFSPH_ERR_ARG, "erosion is off (const(21) = 0 and const(4) <= 1e-4)");
@@ -3850,7 +4062,7 @@ int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double 
     if (e) rc = GFSPH_ERR_HIP;
     if (!rc) {
       const int nb = nblk(n);
-      k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw);
+      k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw, c->P.npwp);
       k_bbox<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
       rc = neighbour_build(c, 0, 1);
     }

@@ -825,8 +825,31 @@ int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
     RD(R.text());
     RD(R.reals(rv, 1));
     P.pwp0rel = rv[0];
+  } else if (P.icpwp == 2) {  // Initialize_Pwp, the pore-pressure column (6:1743-1768)
+    RD(R.text());
+    RD(R.ints(iv, 1));
+    P.nuaux = (int32_t)iv[0];
+    RD(R.text());
+    RD(R.reals(rv, 4));  // BcT1 BcV1 BcT2 BcV2: the types are integers
+    if (rv[0] != (double)(long)rv[0] || rv[2] != (double)(long)rv[2])
+      return ifail(in, GFSPH_ERR_ARG, path + ": pore-pressure condition types must be integers");
+    P.pwp_bc_type[0] = (int32_t)rv[0];
+    P.pwp_bc_val[0] = rv[1];
+    P.pwp_bc_type[1] = (int32_t)rv[2];
+    P.pwp_bc_val[1] = rv[3];
+    RD(R.text());
+    RD(R.reals(rv, 2));  // pwp0_type (a REAL in the reference) and pwp0rel
+    P.pwp0_type = (int32_t)rv[0];
+    P.pwp0rel = rv[1];
+    if (P.pwp0_type == 4) {
+      RD(R.text());
+      RD(R.reals(rv, 1));
+      P.xirel = rv[0];
+    }
+  } else if (P.icpwp == 20) {
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": icpwp = 20 (pore pressure with Stefan erosion) is not supported");
   } else if (P.icpwp != 0) {
-    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": icpwp must be 0, 1 or 3");
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": icpwp must be 0, 1, 2 or 3");
   }
   P.integrator = 4;
   P.init_input = 1;

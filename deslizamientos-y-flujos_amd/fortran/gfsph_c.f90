@@ -30,6 +30,11 @@ module gfsph_c
      real(c_double)     :: h_inf             = 0.0_c_double
      real(c_double)     :: pwp0rel           = 0.0_c_double
      real(c_double)     :: cst(0:110)        = 0.0_c_double
+     integer(c_int32_t) :: nuaux             = 0          ! icpwp = 2: nodes of the column
+     integer(c_int32_t) :: pwp_bc_type(2)    = [2, 1]     ! BC_pwp_Type
+     integer(c_int32_t) :: pwp0_type         = 0
+     real(c_double)     :: pwp_bc_val(2)     = 0.0_c_double  ! BC_pwp_Val
+     real(c_double)     :: xirel             = 0.0_c_double
   end type gfsph_params_t
 
   type, bind(C), public :: gfsph_dem_t
@@ -44,6 +49,7 @@ module gfsph_c
   public :: gfsph_raster_idw, gfsph_mc_pixel_accumulate, gfsph_mc_pixel_read, gfsph_counts
   public :: gfsph_set_adaptive_dt, gfsph_time
   public :: gfsph_get_mass, gfsph_erosion_read, gfsph_erosion_set
+  public :: gfsph_get_pwp_column, gfsph_set_pwp_column
 
   interface
      integer(c_int) function gfsph_create(p, dem, n, x, v, h, mass, hsml, u, itype, ctx) bind(C, name="gfsph_create")
@@ -104,6 +110,18 @@ module gfsph_c
        type(c_ptr), value        :: ctx
        integer(c_int32_t), value :: mode
      end function gfsph_set_adaptive_dt
+
+     integer(c_int) function gfsph_get_pwp_column(ctx, col) bind(C, name="gfsph_get_pwp_column")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value          :: ctx
+       real(c_double), intent(out) :: col(*)
+     end function gfsph_get_pwp_column
+
+     integer(c_int) function gfsph_set_pwp_column(ctx, col) bind(C, name="gfsph_set_pwp_column")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value         :: ctx
+       real(c_double), intent(in) :: col(*)
+     end function gfsph_set_pwp_column
 
      integer(c_int) function gfsph_get_mass(ctx, mass) bind(C, name="gfsph_get_mass")
        import :: c_int, c_double, c_ptr

@@ -26,6 +26,9 @@
 !  and each call returns the dt_sph that step used, so the driver drops its
 !  own "call adaptive_dt_sph" (8:1590) and keeps "time_sph = time_sph + dt_sph".
 !
+!  With a pore-pressure column (icpwp = 2) the whole Uaux(1:nUaux,:) goes to
+!  the context at the first call and comes back after every call.
+!
 !  With erosion (ic_erosion = 1) the context starts from the terrain's
 !  erosion state topol(1), topol(14), topol(15) as Input_SW left it (restart
 !  included), and every call copies back the masses and that state as
@@ -103,6 +106,11 @@ subroutine create_context
    do k = 1, min(110, size(const))
       p%cst(k) = real(const(k), c_double)      ! REAL(4) values, widened
    enddo
+   if (icpwp .eq. 2) then                       ! the column Initialize_Pwp set up (6:1743-1768)
+      p%nuaux          = nUaux
+      p%pwp_bc_type(:) = BC_pwp_Type(:)
+      p%pwp_bc_val(:)  = BC_pwp_Val(:)
+   endif
 
    allocate (t9(9, npoig), zz(npoig), inm(npoig), ity(npoin))
    t9(1:9, :) = topol(1:9, 1:npoig)
@@ -123,6 +131,10 @@ subroutine create_context
       call check(rc, 'gfsph_set_adaptive_dt')
    endif
    if (ic_erosion .eq. 1) call push_erosion
+   if (icpwp .eq. 2) then                       ! the columns Init_sph produced
+      rc = gfsph_set_pwp_column(ctx, Uaux)
+      call check(rc, 'gfsph_set_pwp_column')
+   endif
    deallocate (t9, zz, inm, ity)
 
 end subroutine create_context
@@ -137,7 +149,12 @@ subroutine pull_state
    allocate (ua(npoin), fl(npoin))
    rc = gfsph_get_state(ctx, x, vx, rho, u, ua, hsml, fl)
    call check(rc, 'gfsph_get_state')
-   if (allocated(Uaux)) Uaux(1, 1:npoin) = ua(:)
+   if (icpwp .eq. 2) then
+      rc = gfsph_get_pwp_column(ctx, Uaux)
+      call check(rc, 'gfsph_get_pwp_column')
+   elseif (allocated(Uaux)) then
+      Uaux(1, 1:npoin) = ua(:)
+   endif
    If_Out_Domain(1:npoin) = fl(:)
    deallocate (ua, fl)
    if (ic_erosion .eq. 1) call pull_erosion

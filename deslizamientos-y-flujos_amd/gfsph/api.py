@@ -28,7 +28,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_inputs_parse", "gfsph_inputs_read", "gfsph_inputs_info", "gfsph_inputs_dem", "gfsph_inputs_particles",
     "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy", "gfsph_write_netcdf",
     "gfsph_batch_cases", "gfsph_batch_status", "gfsph_inputs_cases", "gfsph_inputs_batch_create",
-    "gfsph_get_mass", "gfsph_erosion_read", "gfsph_erosion_set",
+    "gfsph_get_mass", "gfsph_erosion_read", "gfsph_erosion_set", "gfsph_get_pwp_column", "gfsph_set_pwp_column",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -46,7 +46,9 @@ class CParams(C.Structure):
                 ("skf", C.c_int32), ("summation_density", C.c_int32), ("average_velocity", C.c_int32),
                 ("nor_density", C.c_int32), ("icpwp", C.c_int32), ("npoin0", C.c_int32),
                 ("init_input", C.c_int32), ("check_finite", C.c_int32),
-                ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double), ("cst", C.c_double * 111)]
+                ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double), ("cst", C.c_double * 111),
+                ("nuaux", C.c_int32), ("pwp_bc_type", C.c_int32 * 2), ("pwp0_type", C.c_int32),
+                ("pwp_bc_val", C.c_double * 2), ("xirel", C.c_double)]
 
 
 class CDem(C.Structure):
@@ -119,6 +121,8 @@ def load_library(path: str | None = None):
     L.gfsph_inputs_batch_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     L.gfsph_batch_status.argtypes = [vp, ip, ip, ip]
     L.gfsph_get_mass.argtypes = [vp, dp]
+    L.gfsph_get_pwp_column.argtypes = [vp, dp]
+    L.gfsph_set_pwp_column.argtypes = [vp, dp]
     L.gfsph_erosion_read.argtypes = [vp, C.c_int32, dp, dp, dp]
     L.gfsph_erosion_set.argtypes = [vp, C.c_int32, dp, dp, dp]
     for f in (L.gfsph_inputs_parse, L.gfsph_inputs_read):
@@ -174,6 +178,12 @@ def make_cparams(prm, init_input: bool = True, check_finite: bool = False) -> CP
     c = prm.cst()
     for k in range(111):
         p.cst[k] = c[k]
+    p.nuaux = int(getattr(prm, "nuaux", 0))
+    bt, bv = getattr(prm, "pwp_bc_type", (2, 1)), getattr(prm, "pwp_bc_val", (0.0, 0.0))
+    p.pwp_bc_type[0], p.pwp_bc_type[1] = int(bt[0]), int(bt[1])
+    p.pwp_bc_val[0], p.pwp_bc_val[1] = float(bv[0]), float(bv[1])
+    p.pwp0_type = int(getattr(prm, "pwp0_type", 0))
+    p.xirel = float(getattr(prm, "xirel", 0.0))
     return p
 
 
@@ -259,6 +269,18 @@ class SPHModel:
         self._check(self.lib.gfsph_get_state(self.ctx, _dp(out["x"]), _dp(out["v"]), _dp(out["h"]), _dp(out["u"]),
                                              _dp(out["uaux"]), _dp(out["hsml"]), _ip(out["flag"])))
         return out
+
+    def pwp_column(self) -> np.ndarray:
+        """icpwp = 2: the pore-pressure columns [n][nuaux], node 0 at the bed."""
+        col = np.zeros((self.n, int(self.case.params.nuaux)))
+        self._check(self.lib.gfsph_get_pwp_column(self.ctx, _dp(col)))
+        return col
+
+    def set_pwp_column(self, col) -> None:
+        c = np.ascontiguousarray(col, dtype=np.float64)
+        if c.shape != (self.n, int(self.case.params.nuaux)):
+            raise ValueError("columns must be [n][nuaux]")
+        self._check(self.lib.gfsph_set_pwp_column(self.ctx, _dp(c)))
 
     def mass(self) -> np.ndarray:
         """Particle masses (volumes) in the caller's order; erosion changes them."""

@@ -52,6 +52,14 @@ class Params:
     adapt_dt: int = 0             # ic_adapt_dt_sph: 0 fixed dt; 1, 2, 4, 10 (6:6521-6664)
     h_inf: float = 1.0e-3
     pwp0rel: float = 0.0
+    # icpwp = 2: pore pressure on a column of nuaux nodes (Initialize_Pwp,
+    # "6 SPH_SW_2019.f90":1743-1800): bottom / top conditions (type 1 value,
+    # 2 gradient; values relative to the depth) and the initial profile
+    nuaux: int = 0
+    pwp_bc_type: tuple = (2, 1)
+    pwp_bc_val: tuple = (0.0, 0.0)
+    pwp0_type: int = 0
+    xirel: float = 0.0
     const: dict = dataclasses.field(default_factory=dict)   # 1-based index -> value
 
     def cst(self) -> np.ndarray:

@@ -88,7 +88,9 @@ class Inputs:
         prm = Cs.Params(integrator=p.integrator, pa_sph=p.pa_sph, nnps=p.nnps, sle=p.sle, skf=p.skf,
                         summation_density=p.summation_density, average_velocity=p.average_velocity,
                         nor_density=p.nor_density, icpwp=p.icpwp, npoin0=p.npoin0, dt=p.dt, h_inf=p.h_inf,
-                        pwp0rel=p.pwp0rel, adapt_dt=self.run.adapt_dt if self.run.present else 0, const=const)
+                        pwp0rel=p.pwp0rel, adapt_dt=self.run.adapt_dt if self.run.present else 0, const=const,
+                        nuaux=p.nuaux, pwp_bc_type=(p.pwp_bc_type[0], p.pwp_bc_type[1]),
+                        pwp_bc_val=(p.pwp_bc_val[0], p.pwp_bc_val[1]), pwp0_type=p.pwp0_type, xirel=p.xirel)
         d = self.dem()
         g = self.grid
         dem = FileDEM(d["Z"], g.xmin, g.ymin, g.dx, g.dy, d["in_model"], g.xmax, g.ymax, d["topol"])

@@ -55,6 +55,14 @@ typedef struct gfsph_params {
   double h_inf;              /* h_inf_SW                                       */
   double pwp0rel;            /* initial pore-pressure ratio                    */
   double cst[111];           /* const(1..110), index 0 unused                  */
+  /* icpwp = 2: pore pressure on a column of nuaux >= 3 nodes per particle,
+   * bed to surface (Uaux(1:nUaux,:), Initialize_Pwp "6 SPH_SW_2019.f90":
+   * 1743-1800, Get_Pwp_SW 6:10966-11174).  Ignored for other icpwp. */
+  int32_t nuaux;
+  int32_t pwp_bc_type[2];    /* BC_pwp_type bed, surface: 1 value, 2 gradient  */
+  int32_t pwp0_type;         /* initial profile 0..4 (init_input = 1)          */
+  double pwp_bc_val[2];      /* BC_pwp_val (relative to the depth)             */
+  double xirel;              /* pwp0_type = 4: relative height of the profile  */
 } gfsph_params;
 
 /* Regular DEM grid ("2 SPH_FEM_topo_2019.f90":94-110, 223-252).  Z is
@@ -93,6 +101,13 @@ int gfsph_sync(gfsph_ctx *ctx);
  * NULL).  h is the depth ("rho"), uaux the bottom pore pressure Uaux(1,:). */
 int gfsph_get_state(gfsph_ctx *ctx, double *x, double *v, double *h, double *u, double *uaux,
                     double *hsml, int32_t *out_flag);
+
+/* icpwp = 2: the pore-pressure columns [n][nuaux] (Uaux(:, ipoin), node 1
+ * at the bed) in the caller's order; set replaces them, e.g. with the
+ * columns the reference's Initialize_Pwp or a restart produced (u and the
+ * bed value follow node 1). */
+int gfsph_get_pwp_column(gfsph_ctx *ctx, double *col);
+int gfsph_set_pwp_column(gfsph_ctx *ctx, const double *col);
 
 /* Particle masses in the caller's order.  Erosion (below) changes them:
  * mass(:) after ExtForces_SW(1) ("6 SPH_SW_2019.f90":8755). */

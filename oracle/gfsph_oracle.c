@@ -54,6 +54,7 @@ typedef struct {
   int eros_on;   /* ic_erosion: const(21) > 1e-9 (after the const(4) form)  */
   int eros_law;  /* const(21) + 0.1: 1 Hungr, 2 slope-angle, 3 basal stress */
   double keros, eros_c23, eros_c24, eros_c25, denss, densw_e, dens_bar_eff, slip_fact, avail0;
+  double Cv; /* consolidation coefficient 4 Bfact / pi^2 (6:10926) */
 } derived_t;
 
 static void derive(const gfo_params *p, derived_t *d) {
@@ -114,6 +115,7 @@ static void derive(const gfo_params *p, derived_t *d) {
   if (c[47] > 1.e-6) d->slip_fact = 2 * c[47] / (1. + c[47]);
   /* available erosion depth topol(15,:) (6:1294-1301) */
   d->avail0 = (c[48] > 1.e-6) ? c[48] : 999.;
+  d->Cv = 4 * c[13] / (d->pi * d->pi);
 }
 
 /* ------------------------------------------------------------------ */
@@ -774,9 +776,30 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
     free(pv.j);
   }
   double alfs = p->cst[14];
+  if (p->icpwp == 2 && p->nuaux > 1) { /* Initialize_Pwp, icpwp = 2 (6:1769-1803) */
+    const int N = p->nuaux;
+    const double pi = 4. * atan(1.0);
+    for (int i = 0; i < n; i++) {
+      double hh = h[i], *U = uaux + (int64_t)i * N;
+      for (int k = 0; k < N; k++) {
+        double xi, v = 0.0;
+        switch (p->pwp0_type) {
+          case 0: v = p->pwp0rel * hh; break;
+          case 1: xi = 1. * k / (N - 1); v = (1 - xi) * p->pwp0rel * hh; break;
+          case 2: xi = (k + 1 - 1.) / (N - 1.); v = cos(pi * xi / 2) * p->pwp0rel * hh; break;
+          case 3: v = (k == 0) ? p->pwp0rel * hh : 0.0; break;
+          case 4: xi = (k + 1 - 1.) / (N - 1.); v = (xi < p->xirel) ? hh * p->pwp0rel * (1 - xi) : 0.0; break;
+          default: return -2;
+        }
+        U[k] = v;
+      }
+      u[i] = U[0];
+    }
+    return 0;
+  }
   if (p->icpwp == 1) {
     for (int i = 0; i < n; i++) u[i] = p->pwp0rel * (h[i] * alfs);
-  } else if (p->icpwp == 3) {
+  } else if (p->icpwp == 3 || p->icpwp == 2) {
     for (int i = 0; i < n; i++) {
       uaux[i] = p->pwp0rel * (h[i] * alfs);
       u[i] = uaux[i];
@@ -823,6 +846,62 @@ double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const dou
   return adaptive_dt(p, dem, n, v, rho, hsml, flag, dt);
 }
 
+/* Get_Pwp_SW for npwp > 1 (6:10966-11174), single phase without topo
+   boundary zones: per particle an explicit FTCS consolidation of the column
+   Uaux(1:N) over the saturated depth h*alphaS, in ntpwp sub-steps, with the
+   loading term dh/ntpwp*(1-z/h), the clamps z-dependent, and the bottom /
+   top conditions (1 value, 2 gradient).  The sub-step count is capped at
+   PWP_MAX_SUBSTEPS (the reference has no cap; a cap is only reached for a
+   vanishing depth). */
+#define PWP_MAX_SUBSTEPS (1 << 16)
+static void pwp_column(const gfo_params *p, const derived_t *D, int n, int N, double dt, const double *rho,
+                       const double *drho, const int32_t *flag, double *uaux) {
+  const double Cv = D->Cv, alphaS = p->cst[14];
+  double *pwp0 = (double *)malloc(sizeof(double) * N);
+  for (int i = 0; i < n; i++) {
+    double *pwp = uaux + (int64_t)i * N;
+    if (flag[i]) {
+      for (int k = 0; k < N; k++) pwp[k] = 0.0;
+      continue;
+    }
+    const int LBc1 = p->pwp_bc_type[0], LBc2 = p->pwp_bc_type[1];
+    double VBc1 = p->pwp_bc_val[0];
+    const double VBc2 = p->pwp_bc_val[1];
+    double fpwp_min = D->fpwp_min;
+    const double fpwp_max = D->fpwp_max;
+    const double hh = rho[i], hpwp = hh * alphaS, dhh = drho[i], dpwp_dn = 0.0;
+    if (VBc1 < -900) { /* 6:11053-11060 */
+      VBc1 = -((p->cst[18] < 1.e-3 ? 1000.0 : p->cst[18]) / D->dens_bar_eff);
+      fpwp_min = VBc1;
+    }
+    const double dxpwp = hpwp / (N - 1), dxpwp2 = dxpwp * dxpwp;
+    double dtpwp = (Cv > 1.e-20) ? dxpwp2 / (2 * Cv) : dxpwp2 / (2.e-20);
+    dtpwp = 1.0 * dtpwp; /* safetpwp = 1 */
+    double q = ceil(dt / dtpwp);
+    const int ntpwp = (q >= 1.0 && q <= PWP_MAX_SUBSTEPS) ? (int)q : (q >= 1.0 ? PWP_MAX_SUBSTEPS : 1);
+    dtpwp = dt / (double)ntpwp;
+    const double dhhstep = dhh / ntpwp, dpwp_dn_step = dpwp_dn / ntpwp;
+    for (int it = 0; it < ntpwp; it++) {
+      for (int k = 0; k < N; k++) pwp0[k] = pwp[k];
+      for (int k = 1; k < N - 1; k++) { /* ip = k + 1 */
+        const double zz = dxpwp * k, zzhh = hh - zz, xi = zz / hh;
+        const double sumdhh = dhhstep * (1. - xi);
+        pwp[k] = pwp0[k] + (Cv * dtpwp / dxpwp2) * (pwp0[k - 1] - 2. * pwp0[k] + pwp0[k + 1]) + sumdhh +
+                 dpwp_dn_step;
+        if (pwp[k] > (zzhh * fpwp_max)) pwp[k] = (zzhh * fpwp_max);
+        else if (pwp[k] < (zzhh * fpwp_min)) pwp[k] = (zzhh * fpwp_min);
+      }
+      if (LBc1 == 1) pwp[0] = VBc1 * hh * alphaS;
+      else if (LBc1 == 2) pwp[0] = (4. * pwp[1] - pwp[2] - 2. * VBc1 * dxpwp) / 3.;
+      if (pwp[0] > hh) pwp[0] = hh;
+      if (LBc2 == 1) pwp[N - 1] = VBc2 * hh;
+      else if (LBc2 == 2) pwp[N - 1] = (4. * pwp[N - 2] - pwp[N - 3] + 2. * VBc2 * dxpwp) / 3.;
+      if (pwp[N - 1] > 0.0) pwp[N - 1] = 0.0;
+    }
+  }
+  free(pwp0);
+}
+
 /* Node coordinates coorg(:, ipoig) of the terrain grid (2:285-296). */
 static void node_xy(const gfo_dem *d, int64_t ip, double *xg, double *yg) {
   if (d->coorg) {
@@ -853,6 +932,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
   const double *f1 = (nst == 4) ? f1rk4 : f1rk2, *f2 = (nst == 4) ? f2rk4 : f2rk2;
   double fin = (nst == 4) ? dt / 6. : dt / 2.;
   const int nuaux = (p->icpwp == 2 || p->icpwp == 3) ? 1 : 0;
+  const int ncol = (p->icpwp == 2 && p->nuaux > 1) ? p->nuaux : 1; /* Uaux(1:ncol, ipoin) */
   int scale_k = D.scale_k;
 
   double *x0 = (double *)malloc(sizeof(double) * 2 * n), *v0 = (double *)malloc(sizeof(double) * 2 * n);
@@ -963,7 +1043,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         if (flag[i]) continue;
         extout o;
         ext_force_one(p, &D, dem, x[2 * i], x[2 * i + 1], v[2 * i], v[2 * i + 1], rho[i], &u[i],
-                      nuaux ? uaux[i] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, NULL, &o);
+                      nuaux ? uaux[(int64_t)i * ncol] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, NULL, &o);
         ex[2 * i] = o.ex;
         ex[2 * i + 1] = o.ey;
         du[i] = o.du;
@@ -1000,7 +1080,8 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         if (flag[i]) continue;
         extout o;
         ext_force_one(p, &D, dem, x[2 * i], x[2 * i + 1], v[2 * i], v[2 * i + 1], rho[i], &u[i],
-                      nuaux ? uaux[i] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, &ex[2 * i], &o);
+                      nuaux ? uaux[(int64_t)i * ncol] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, &ex[2 * i],
+                      &o);
         const double eb = dt * o.er;
         const double xx = x[2 * i], yy = x[2 * i + 1];
         int ipx = (int)((xx - dem->xmin) / dxg + 1), ipy = (int)((yy - dem->ymin) / dyg + 1);
@@ -1034,8 +1115,11 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         }
       }
     }
-    /* 8. pore pressure, npwp = 1 (Get_Pwp_SW 6:11176-11310) */
-    if (nuaux >= 1) {
+    /* 8. pore pressure on the column, npwp > 1 (Get_Pwp_SW 6:10966-11174) */
+    if (ncol > 1) {
+      pwp_column(p, &D, n, ncol, dt, rho, Drho, flag, uaux);
+      for (int i = 0; i < n; i++) u[i] = uaux[(int64_t)i * ncol];
+    } else if (nuaux >= 1) { /* npwp = 1 (Get_Pwp_SW 6:11176-11310) */
       for (int i = 0; i < n; i++) {
         if (flag[i]) {
           uaux[i] = 0.0;

@@ -19,6 +19,14 @@ typedef struct {
   double h_inf;               /* h_inf_SW */
   double pwp0rel;             /* initial pwp ratio (6:1811-1852) */
   double cst[111];            /* const(1..110); index 0 unused; REAL(4) values */
+  /* icpwp = 2: pore pressure on a column of nuaux nodes per particle
+   * (Initialize_Pwp 6:1743-1800, Get_Pwp_SW 6:10966-11174); uaux is then
+   * [n][nuaux] (Uaux(1:nuaux, ipoin)), node 1 at the bed */
+  int32_t nuaux;
+  int32_t pwp_bc_type[2];     /* BC_pwp_type: bottom, top (1 value, 2 gradient) */
+  int32_t pwp0_type;          /* initial profile 0..4 */
+  double pwp_bc_val[2];       /* BC_pwp_val */
+  double xirel;               /* pwp0_type 4 */
 } gfo_params;
 
 typedef struct {

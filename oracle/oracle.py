@@ -21,7 +21,8 @@ class GfoParams(C.Structure):
                 ("summation_density", C.c_int32), ("average_velocity", C.c_int32), ("nor_density", C.c_int32),
                 ("icpwp", C.c_int32), ("npoin0", C.c_int32), ("rev_order", C.c_int32),
                 ("adapt_dt", C.c_int32), ("pad2_", C.c_int32), ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double),
-                ("cst", C.c_double * 111)]
+                ("cst", C.c_double * 111), ("nuaux", C.c_int32), ("pwp_bc_type", C.c_int32 * 2),
+                ("pwp0_type", C.c_int32), ("pwp_bc_val", C.c_double * 2), ("xirel", C.c_double)]
 
 
 class GfoDem(C.Structure):
@@ -80,7 +81,18 @@ def make_params(prm) -> GfoParams:
     c = prm.cst()
     for k in range(111):
         p.cst[k] = c[k]
+    p.nuaux = int(getattr(prm, "nuaux", 0)) if prm.icpwp == 2 else 0
+    bt, bv = getattr(prm, "pwp_bc_type", (2, 1)), getattr(prm, "pwp_bc_val", (0.0, 0.0))
+    p.pwp_bc_type[0], p.pwp_bc_type[1] = int(bt[0]), int(bt[1])
+    p.pwp_bc_val[0], p.pwp_bc_val[1] = float(bv[0]), float(bv[1])
+    p.pwp0_type = int(getattr(prm, "pwp0_type", 0))
+    p.xirel = float(getattr(prm, "xirel", 0.0))
     return p
+
+
+def ncol(prm) -> int:
+    """Nodes of the pore-pressure column per particle (1: a single value)."""
+    return int(prm.nuaux) if prm.icpwp == 2 and int(getattr(prm, "nuaux", 0)) > 1 else 1
 
 
 class DemHandle:
@@ -170,6 +182,8 @@ def erosion_grid(case):
 def init(case, state=None):
     """Input_SW initialisation (re-summed h, pwp) -> State."""
     st = State(case.particles) if state is None else state
+    if ncol(case.params) > 1 and st.uaux.ndim == 1:
+        st.uaux = np.zeros((st.h.shape[0], ncol(case.params)))
     p = make_params(case.params)
     dh = DemHandle(case.dem)
     lib().gfo_init(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.h), _dp(st.mass), _dp(st.hsml),

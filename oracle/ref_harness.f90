@@ -22,6 +22,8 @@
 !     real64 x0(ndimn,npoin)        positions the pair list was built from
 !     int32  pair_i(npairs) pair_j(npairs)
 !     real64 w(npairs) dwdx(ndimn,npairs)
+!  with a pore-pressure column (nUaux > 1), uaux_<step>.bin:
+!     real64 (Uaux(1:nUaux,ip), ip = 1, npoin)
 !  and, when erosion is on (ic_erosion = 1), eros_<step>.bin:
 !     real64 (topol(1,ip), topol(14,ip), topol(15,ip), ip = 1, npoig)
 !  It is built only by oracle/build_ref.sh and never ships with the product.
@@ -198,6 +200,14 @@ subroutine write_dump (is)
    endif
    close (iu)
    deallocate (pi_, pj_, pw, pdw)
+   if (allocated(Uaux)) then
+      if (size(Uaux, 1).gt.1) then
+         write (fname,'(a,i0,a)') 'uaux_', is, '.bin'
+         open (newunit=iu, file=trim(fname), access='stream', form='unformatted', status='replace')
+         write (iu) (real(Uaux(:,ip),8), ip=1,npoin)
+         close (iu)
+      endif
+   endif
    if (ic_erosion.eq.1) then
       write (fname,'(a,i0,a)') 'eros_', is, '.bin'
       open (newunit=iu, file=trim(fname), access='stream', form='unformatted', status='replace')

@@ -68,8 +68,15 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
     L += ["icpwp", str(prm.icpwp)]
     if prm.icpwp in (1, 3):
         L += ["pwp0rel", _r(prm.pwp0rel)]
+    elif prm.icpwp == 2:  # Initialize_Pwp (6:1743-1768)
+        bt, bv = prm.pwp_bc_type, prm.pwp_bc_val
+        L += ["nUaux", str(int(prm.nuaux))]
+        L += ["BcT1 BcV1 BcT2 BcV2", f"{int(bt[0])} {_r(bv[0])} {int(bt[1])} {_r(bv[1])}"]
+        L += ["pwp0_type pwp0rel", f"{int(prm.pwp0_type)} {_r(prm.pwp0rel)}"]
+        if int(prm.pwp0_type) == 4:
+            L += ["xirel", _r(prm.xirel)]
     elif prm.icpwp != 0:
-        raise ValueError("icpwp must be 0, 1 or 3")
+        raise ValueError("icpwp must be 0, 1, 2 or 3")
     L += ["coarse mesh saving utility?", "0"]
     L += ["control points?", "0"]
     L += ["GID filter", "1 1 1 1 0 1 0 0 0 1 0 0"]
@@ -256,6 +263,10 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
     for s in steps:
         pe = os.path.join(tmp, f"eros_{s}.bin")
         out[s]["eros"] = read_eros(pe) if os.path.exists(pe) else None
+        pc = os.path.join(tmp, f"uaux_{s}.bin")
+        if os.path.exists(pc):  # the whole pore-pressure column Uaux(1:nUaux, :)
+            out[s]["ucol"] = np.frombuffer(open(pc, "rb").read(), dtype=np.float64).reshape(
+                out[s]["h"].shape[0], -1).copy()
     if not keep and workdir is None:
         subprocess.run(["rm", "-rf", tmp])
     return out

@@ -143,6 +143,19 @@ def case_defs():
         case=C.Case("g_erosion_stress", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=ce3),
                     small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
         dumps=[1, 10, 100])
+    # pore pressure on a column of nUaux nodes (icpwp = 2, Initialize_Pwp
+    # 6:1743-1800, Get_Pwp_SW 6:10966-11174): linear initial profile with a
+    # closed bed; a cosine profile with a prescribed bed value and several
+    # sub-steps per step; a partial profile (xirel) with gradient conditions
+    for nm, N, cv, bt, bv, p0t, p0rel, xirel, seed in (
+            ("g_pwp_column_lin", 11, 0.01, (2, 1), (0.0, 0.0), 1, 0.6, 0.0, 41),
+            ("g_pwp_column_cos", 21, 0.5, (1, 1), (0.3, 0.0), 2, 0.8, 0.0, 42),
+            ("g_pwp_column_step", 9, 0.05, (2, 2), (0.0, 0.0), 4, 0.7, 0.5, 43)):
+        defs[nm] = dict(
+            case=C.Case(nm, C.Params(integrator=4, dt=0.02, h_inf=1e-3, icpwp=2, pwp0rel=p0rel, nuaux=N,
+                                     pwp_bc_type=bt, pwp_bc_val=bv, pwp0_type=p0t, xirel=xirel,
+                                     const=C.frictional(28.0, cv=cv)),
+                        dem, scar_on(dem, 300, seed), 100), dumps=[1, 10, 100])
     # random cloud with varied smoothing lengths: neighbour set only
     rng = np.random.Generator(np.random.Philox(99))
     n = 600
@@ -186,6 +199,8 @@ def save_case(name, d, out_dir=HERE):
         arr[f"s{s}_hsml"] = dd["hsml"]
         arr[f"s{s}_flag"] = dd["flag"]
         arr[f"s{s}_time"] = np.array(dd["time"])
+        if dd.get("ucol") is not None:  # the whole pore-pressure column
+            arr[f"s{s}_ucol"] = dd["ucol"]
         if dd.get("eros") is not None:
             # erosion changes the masses and the terrain: topol(1), (14), (15)
             arr[f"s{s}_mass"] = dd["mass"]

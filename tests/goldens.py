@@ -15,7 +15,8 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STATE_CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_curvature7", "g_mui8", "g_icpwp1_sle1",
                "g_manning1", "g_perzyna9", "g_hb25", "g_bagnold11", "g_kernel_skf2", "g_kernel_skf3",
                "g_kernel_skf4", "g_domain_exit", "g_adaptive_dt2", "g_adaptive_dt10", "g_erosion_hungr",
-               "g_erosion_slope", "g_erosion_stress"]
+               "g_erosion_slope", "g_erosion_stress", "g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step"]
+PWP_COLUMN_CASES = ["g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step"]
 EROSION_CASES = ["g_erosion_hungr", "g_erosion_slope", "g_erosion_stress"]
 ALL_CASES = STATE_CASES + ["g_random_neighbours"]
 
@@ -30,6 +31,9 @@ def case_of(name: str):
     pj = json.loads(bytes(f["params_json"]).decode())
     cst = f["const"]
     const = {k: float(cst[k]) for k in range(1, 111) if cst[k] != 0.0}
+    for k in ("pwp_bc_type", "pwp_bc_val"):
+        if k in pj:
+            pj[k] = tuple(pj[k])
     prm = C.Params(**pj, const=const)
     xmin, ymin, dx, dy = (float(v) for v in f["dem_geom"])
     dem = DEM(Z=f["dem_Z"], xmin=xmin, ymin=ymin, dx=dx, dy=dy)

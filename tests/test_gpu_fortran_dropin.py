@@ -14,7 +14,7 @@ from oracle import refio
 pytestmark = pytest.mark.gpu
 
 CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_icpwp1_sle1", "g_curvature7", "g_adaptive_dt10",
-         "g_erosion_hungr", "g_erosion_slope"]
+         "g_erosion_hungr", "g_erosion_slope", "g_pwp_column_cos"]
 
 
 @pytest.mark.skipif(not os.path.exists(refio.REF_GPU_EXE),
@@ -32,6 +32,8 @@ def test_fortran_driver_with_gpu_step_matches_reference(name, tmp_path):
         assert np.array_equal(out[s]["flag"], f[f"s{s}_flag"])
         if f"s{s}_time" in f:
             assert out[s]["time"] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)
+        if f"s{s}_ucol" in f:  # Uaux(1:nUaux, :) copied back by the layer
+            assert goldens.rel_err(out[s]["ucol"], f[f"s{s}_ucol"]) <= tol, (name, s)
         if f"s{s}_eros" in f:  # masses and topol(1), (14), (15) copied back by the layer
             assert goldens.rel_err(out[s]["mass"], f[f"s{s}_mass"]) <= tol
             for j in range(3):

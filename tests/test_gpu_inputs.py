@@ -81,7 +81,8 @@ def test_picacho_release_and_run_match_reference(picacho):
                 assert t == pytest.approx(float(f[f"s{s}_time"]), rel=1e-9)
 
 
-@pytest.mark.parametrize("name", ["g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1", "g_adaptive_dt10"])
+@pytest.mark.parametrize("name", ["g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1", "g_adaptive_dt10",
+                                  "g_pwp_column_step", "g_erosion_hungr"])
 def test_file_path_equals_array_path(name, tmp_path):
     case, _ = goldens.case_of(name)
     refio.write_inputs(str(tmp_path), case, name="case", topo="topo")
@@ -94,3 +95,7 @@ def test_file_path_equals_array_path(name, tmp_path):
             for k in ("x", "v", "h", "u", "uaux", "hsml", "flag"):
                 np.testing.assert_array_equal(sa[k], sb[k], err_msg=f"{name} {k}")
             assert a.time() == b.time()
+            if case.params.icpwp == 2:
+                np.testing.assert_array_equal(a.pwp_column(), b.pwp_column())
+            if name.startswith("g_erosion"):
+                np.testing.assert_array_equal(a.mass(), b.mass())

@@ -83,6 +83,8 @@ def test_gpu_state_matches_reference(name):
         s0 = m.state()
         assert goldens.rel_err(s0["h"], f["s0_h"]) <= 1e-13
         assert goldens.rel_err(s0["u"], f["s0_u"]) <= 1e-13
+        if "s0_ucol" in f:  # icpwp = 2: Initialize_Pwp's column profile
+            assert goldens.rel_err(m.pwp_column(), f["s0_ucol"]) <= 1e-13
         done = 0
         for s in [int(v) for v in f["dumps"] if v > 0]:
             m.time_integration_sph(s - done)
@@ -94,6 +96,9 @@ def test_gpu_state_matches_reference(name):
             assert np.array_equal(st["flag"], f[f"s{s}_flag"]), f"{name} step {s}: flags"
             if f"s{s}_uaux" in f:
                 assert goldens.rel_err(st["uaux"], f[f"s{s}_uaux"]) <= tol_for(s)
+            if f"s{s}_ucol" in f:  # every node of every pore-pressure column
+                err = goldens.rel_err(m.pwp_column(), f[f"s{s}_ucol"])
+                assert err <= tol_for(s), f"{name} step {s} column: {err:.3e}"
             if f"s{s}_time" in f:
                 assert m.time()[0] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)
             if f"s{s}_eros" in f:  # erosion: masses and the terrain's Z / eroded / available depth

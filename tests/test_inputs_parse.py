@@ -52,7 +52,8 @@ def test_picacho_grid_and_run_control(tmp_path):
         assert inp.n == 0  # the gridded release is built on the device
 
 
-@pytest.mark.parametrize("name", ["g_frictional_pwp", "g_icpwp1_sle1", "g_bingham", "g_hb25"])
+@pytest.mark.parametrize("name", ["g_frictional_pwp", "g_icpwp1_sle1", "g_bingham", "g_hb25", "g_pwp_column_step",
+                                  "g_erosion_hungr"])
 def test_restart_inputs_round_trip(name, tmp_path):
     """Inputs written in the reference's formats (explicit-grid .top with
     finite-difference slopes, restart.dat) read back exactly."""
@@ -64,8 +65,11 @@ def test_restart_inputs_round_trip(name, tmp_path):
         assert (p.summation_density, p.average_velocity, p.nor_density) == \
             (cp.summation_density, cp.average_velocity, cp.nor_density)
         assert p.h_inf == cp.h_inf
-        if cp.icpwp in (1, 3):
+        if cp.icpwp in (1, 2, 3):
             assert p.pwp0rel == cp.pwp0rel
+        if cp.icpwp == 2:  # the column block of Initialize_Pwp
+            assert (p.nuaux, tuple(p.pwp_bc_type), tuple(p.pwp_bc_val), p.pwp0_type, p.xirel) == \
+                (cp.nuaux, tuple(cp.pwp_bc_type), tuple(cp.pwp_bc_val), cp.pwp0_type, cp.xirel)
         c = cp.cst()
         assert all(p.cst[k] == c[k] for k in range(1, 16))
         g, dem = inp.grid, case.dem

@@ -36,6 +36,8 @@ def test_state_matches_reference(name):
     # step-0 dump is the state after Init_sph (re-summed h, initial pwp)
     np.testing.assert_allclose(st.h, f["s0_h"], rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(st.u, f["s0_u"], rtol=1e-13, atol=1e-13)
+    if "s0_ucol" in f:  # icpwp = 2: the initial column profile
+        assert goldens.rel_err(st.uaux, f["s0_ucol"]) <= 1e-13
     done = 0
     for s in [int(v) for v in f["dumps"] if v > 0]:
         oracle.run(case, st, s - done)
@@ -44,7 +46,9 @@ def test_state_matches_reference(name):
             err = goldens.rel_err(getattr(st, k), f[f"s{s}_{k}"])
             assert err <= tol_for(s), f"{name} step {s} field {k}: {err:.3e}"
         assert np.array_equal(st.flag, f[f"s{s}_flag"]), f"{name} step {s}: domain flags differ"
-        if f"s{s}_uaux" in f:
+        if f"s{s}_ucol" in f:  # icpwp = 2: every node of every column
+            assert goldens.rel_err(st.uaux, f[f"s{s}_ucol"]) <= tol_for(s), f"{name} step {s} column"
+        elif f"s{s}_uaux" in f:
             assert goldens.rel_err(st.uaux, f[f"s{s}_uaux"]) <= tol_for(s)
         if f"s{s}_time" in f:  # time_sph: the sum of the (adaptive) dt_sph of the steps
             assert st.dt_time[1] == pytest.approx(float(f[f"s{s}_time"]), rel=1e-12)
