@@ -63,6 +63,10 @@ struct DevParams {
   // conditions and the consolidation coefficient Cv = 4 Bfact / pi^2
   int npwp, bc_t1, bc_t2, pad_c;
   double bc_v1, bc_v2, Cv;
+  // pressure re-evaluated in every stage: g along the terrain normal
+  // (ic_cosTheta = 1) and / or K0 active / passive (ic_K0 = 1)
+  int costheta, ic_k0;
+  double k0a, k0p;
   double keros, eros_c23, eros_c24, eros_c25, denss, densw_e, dens_bar_eff, slip_fact, avail0;
 };
 
@@ -430,7 +434,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   const double dt = P.dt;
   double h, hu, hv, hinv, vmod, hvmod, hvmodinv;
   double vxx = vx1, vyy = vy1;
-  const double gravz = P.cgra;
+  double gravz = P.cgra;  // g along the terrain normal with ic_cosTheta = 1 (6:9264-9270)
   double frictx = 0, fricty = 0, TotFrict = 0, cf5 = 0, hfrict, pwp;
   double xi_voellmy = P.xi_voellmy;
   double exx = (EROS && end) ? acx : 0.0, exy = (EROS && end) ? acy : 0.0, du = 0.0;
@@ -458,8 +462,15 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   // the curvature terms are interpolated only when the law uses them (V^2/R);
   // the reference interpolates them for every frictional law (6:9257-9259)
   get_z_topo(dem, xx, yy, P.if_V2byR != 0, tp);
-  exx = exx + (-P.cgra * tp.Zx);
-  exy = exy + (-P.cgra * tp.Zy);
+  if (P.costheta) {
+    const double deng = sqrt(1. + tp.Zx * tp.Zx + tp.Zy * tp.Zy);
+    gravz = P.cgra / deng;
+    exx = exx + (-P.cgra * tp.Zx / deng);
+    exy = exy + (-P.cgra * tp.Zy / deng);
+  } else {
+    exx = exx + (-P.cgra * tp.Zx);
+    exy = exy + (-P.cgra * tp.Zy);
+  }
 
   if (P.icpwp == 1) pwp = u_i;
   else if (P.icpwp == 2 || P.icpwp == 3) pwp = uaux1;  // Uaux(1,:), the bed value
@@ -1974,6 +1985,182 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
 }
 
 // ---------------------------------------------------------------------------
+// Pressure re-evaluated in every RK stage (ic_K0 = 1 or ic_cosTheta = 1).
+// RK45 calls IntForces_SW in every stage (7:849-866) with the pair geometry
+// of the step start; its pressures then change with the stage: g along the
+// terrain normal at the stage position (6:6774-6783) and the K0 factor by the
+// sign of the stage velocity divergence (6:6736-6760, 6:6857-6863).  Each
+// stage is
+//   k_stg_update   stage update, domain flag, the particle's pressure
+//   k_stg_k0       (ic_K0) velocity divergence over the neighbours, K0
+//   k_pforce_m     the pressure force with these pressures
+//   k_stg_ext      external forces and the RK sums
+// and k_stg_finish ends the step as k_rk does.  A particle that leaves the
+// domain keeps the pressure of its last stage inside (p is not recomputed
+// for it), as its neighbours see it in the reference.  Buffers: BinghamState.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GF_BLOCK) void k_stg_init(int n, SortedSet s, BinghamState b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  b.x[i] = s.pos[i];
+  b.v[i] = s.vel[i];
+  b.ex[i] = make_double2(0.0, 0.0);
+  b.u[i] = s.u[i];
+  b.du[i] = 0.0;
+  for (int k = 0; k < 5; k++) b.D[5 * i + k] = 0.0;
+  b.fl[i] = 0;
+}
+
+__global__ __launch_bounds__(GF_BLOCK) void k_stg_update(int n, int st, const DevParams *Pr, DevDem dem,
+                                                          SortedSet s, const double *S, const double2 *ain,
+                                                          BinghamState b, double *Pp, const int *stopr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = s.rid[i];
+  if (s.flag[i] || b.fl[i] || (stopr && stopr[r])) return;
+  const DevParams &P = Pr[r];
+  const double Si = S[i];
+  const double rfi = Si < P.h_inf ? P.h_inf : Si;
+  // the first stage's weight is 0: its force terms are not read
+  const double2 in = st == 0 ? make_double2(0.0, 0.0) : ain[i];
+  double2 x = b.x[i], v = b.v[i];
+  const double2 ex = b.ex[i];
+  double u = b.u[i];
+  stage_update(P, st, s.pos[i], s.vel[i], s.u[i], s.h[i], rfi, ex.x, ex.y, b.du[i], in.x, in.y, x, v, u);
+  b.x[i] = x;
+  b.v[i] = v;
+  b.u[i] = u;
+  if (out_of_domain(dem, x.x, x.y)) {
+    b.fl[i] = 1;
+    return;
+  }
+  double cgra05 = P.cgra05;
+  if (P.costheta) {
+    TopoPt tp;
+    get_z_topo(dem, x.x, x.y, false, tp);
+    const double deng = sqrt(1. + tp.Zx * tp.Zx + tp.Zy * tp.Zy);
+    cgra05 = 0.5 * (P.cgra / deng);
+  }
+  const double p = cgra05 * rfi * rfi;
+  if (P.ic_k0)
+    b.cand[i] = p;  // the K0 factor follows in k_stg_k0
+  else
+    Pp[i] = (P.pa_sph == 2) ? p / (rfi * rfi) : p;
+}
+
+template <int SKF>
+__global__ __launch_bounds__(GF_BLOCK) void k_stg_k0(int n, const DevParams *Pr, Cells cl,
+                                                      const unsigned long long *mask, SortedSet s, const double *S,
+                                                      BinghamState b, double *Pp, const int *stopr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = s.rid[i];
+  if (s.flag[i] || b.fl[i] || (stopr && stopr[r])) return;
+  const DevParams &P = Pr[r];
+  const GridInfo g = cl.grid[r];
+  const int *start = cl.start + cl.coff[r];
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
+  const double2 vi = b.v[i];
+  double dv = 0.0;  // sum_j m_j (v_i - v_j) . grad_i W_ij
+  auto term = [&](double mj, double2 vj, double gx, double gy) {
+    double vcc = (vi.x - vj.x) * gx;
+    vcc = vcc + (vi.y - vj.y) * gy;
+    dv = dv + mj * vcc;
+  };
+  if (stream_full(cs)) {
+    stream_dense<1>(cs, P.scale_k, i, 0, pi_, hi, s.nb,
+                    [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                      double w, gx, gy;
+                      sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                      term(q.w, b.v[j], gx, gy);
+                    });
+  } else {
+    unsigned long long w[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+    masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        if (!ok[u]) continue;
+        const double4 q = s.nb[js[u]];
+        double dx, dy, dr, mh, wk, gx, gy;
+        pair_geom(pi_, hi, q, dx, dy, dr, mh);
+        sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, wk, gx, gy);
+        term(q.w, b.v[js[u]], gx, gy);
+      }
+    });
+  }
+  const double Si = S[i];
+  const double rfi = Si < P.h_inf ? P.h_inf : Si;
+  const double p = b.cand[i] * (dv >= 0 ? P.k0a : P.k0p);
+  Pp[i] = (P.pa_sph == 2) ? p / (rfi * rfi) : p;
+}
+
+template <bool EROS>
+__global__ __launch_bounds__(GF_BLOCK) void k_stg_ext(int n, int st, const DevParams *Pr, DevDem dem, SortedSet s,
+                                                       const double *S, const double2 *ain, BinghamState b,
+                                                       const int *stopr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = s.rid[i];
+  if (s.flag[i] || b.fl[i] || (stopr && stopr[r])) return;
+  const DevParams &P = Pr[r];
+  const double Si = S[i];
+  const double rfi = Si < P.h_inf ? P.h_inf : Si;
+  const double2 in = ain[i], x = b.x[i], v = b.v[i];
+  double u = b.u[i], ex, ey, du;
+  ext_force<EROS>(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, 0.0, ex, ey, du);
+  b.u[i] = u;
+  b.ex[i] = make_double2(ex, ey);
+  b.du[i] = du;
+  const double f2 = P.f2[st];
+  double *D = b.D + 5 * i;
+  D[2] += f2 * (ex + in.x);
+  D[3] += f2 * (ey + in.y);
+  D[0] += f2 * v.x;
+  D[1] += f2 * v.y;
+  D[4] += f2 * du;
+}
+
+template <bool EROS>
+__global__ __launch_bounds__(GF_BLOCK) void k_stg_finish(int n, const DevParams *Pr, DevDem dem, SortedSet s,
+                                                          const double *S, const double2 *ain, BinghamState b,
+                                                          NewSet o, BBoxAcc *part, const int *stopr, ErosGrid eg) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  bool active_out = false;
+  double2 xo = make_double2(0, 0);
+  double hso = 0;
+  if (i < n) {
+    const int r = s.rid[i];
+    const DevParams &P = Pr[r];
+    if (stopr && stopr[r]) {  // a realisation whose case has stopped does not change
+      o.pos[i] = s.pos[i];
+      o.vel[i] = s.vel[i];
+      o.h[i] = s.h[i];
+      o.hsml[i] = s.hsml[i];
+      o.u[i] = s.u[i];
+      o.uaux[i] = s.uaux[i];
+      o.flag[i] = s.flag[i];
+      active_out = !s.flag[i];
+      xo = s.pos[i];
+      hso = s.hsml[i];
+    } else if (s.flag[i]) {
+      write_frozen(P, s, i, o);
+    } else {
+      const double Si = S[i];
+      const double rfi = Si < P.h_inf ? P.h_inf : Si;
+      const double *D = b.D + 5 * i;
+      const double2 ex = b.ex[i], in = ain[i];
+      finish_particle<EROS>(P, dem, s, i, Si, rfi, b.x[i], b.v[i], b.u[i], D[0], D[1], D[2], D[3], D[4],
+                            b.fl[i] != 0, o, active_out, xo, hso, ex.x, ex.y, in.x, in.y,
+                            EROS ? eg.aux + 2 * eg.np * r : nullptr);
+    }
+  }
+  if (part) bbox_block(part, active_out, xo, hso);
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint (reverse mode) of one step, for the frictional / Voellmy laws
 // (nfric 5, 6) with pa_sph = 2, skf = 1, icpwp = 0, sle = 0.  With the
 // neighbour set of the step fixed, one step is
@@ -3154,6 +3341,8 @@ struct gfsph_ctx {
   int *stopr = nullptr, *due = nullptr, *outc = nullptr, *cnt_r = nullptr;
   // erosion (some realisation has ic_erosion): the terrain's erosion state
   int eros = 0;
+  // pressure re-evaluated in every stage (some realisation has ic_K0 or ic_cosTheta)
+  int staged = 0;
   ErosGrid eg{};
   std::vector<void *> allocs;
 };
@@ -3226,8 +3415,17 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
     err = "basal law nfric=" + std::to_string(P.nfric) + " is outside the implemented set";
     return GFSPH_ERR_UNSUPPORTED;
   }
-  if ((int)(c[16] + 0.01) != 0) {
-    err = "ic_cosTheta (const(16)) must be 0";
+  P.costheta = (int)(c[16] + 0.01);
+  if (P.costheta != 0 && P.costheta != 1) {
+    err = "ic_cosTheta (const(16)) must be 0 or 1";
+    return GFSPH_ERR_UNSUPPORTED;
+  }
+  // the reference reads ic_K0 for frictional laws only (6:1373-1390)
+  P.ic_k0 = (p->ic_k0 == 1 && P.if_frictional) ? 1 : 0;
+  P.k0a = p->k0_active;
+  P.k0p = p->k0_pasive;
+  if ((P.costheta || P.ic_k0) && P.nfric == 4) {
+    err = "ic_cosTheta = 1 with the Bingham law (nfric = 4) is outside the implemented path";
     return GFSPH_ERR_UNSUPPORTED;
   }
   if (c[20] != 0.0 || c[34] != 0.0) {
@@ -3488,11 +3686,40 @@ static void launch_column(gfsph_ctx *c, SortedSet &s) {
     k_pwp_column<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->Pr, s, c->nw, c->cases ? c->stopr : nullptr);
 }
 
+static void launch_pforce(gfsph_ctx *c, SortedSet &s);
+
 // RK stages of one step on the sorted set s (pressure force in c->ain).
 static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   const int n = c->n;
   hipStream_t st = c->stream;
   const int nb = nblk(n);
+  if (c->staged) {
+    BinghamState &b = c->bs;
+    const int *stp = c->cases ? c->stopr : nullptr;
+    k_stg_init<<<nb, GF_BLOCK, 0, st>>>(n, s, b);
+    for (int k = 0; k < c->P.nstages; k++) {
+      k_stg_update<<<nb, GF_BLOCK, 0, st>>>(n, k, c->Pr, c->dem, s, c->S, c->ain, b, c->Pp, stp);
+      if (c->P.ic_k0)
+        SKF_DISPATCH(k_stg_k0, nb, GF_BLOCK, 0, st>>>(n, c->Pr, cells_of(c), c->mask, s, c->S, b, c->Pp, stp));
+      launch_pforce(c, s);
+      if (c->eros)
+        k_stg_ext<true><<<nb, GF_BLOCK, 0, st>>>(n, k, c->Pr, c->dem, s, c->S, c->ain, b, stp);
+      else
+        k_stg_ext<false><<<nb, GF_BLOCK, 0, st>>>(n, k, c->Pr, c->dem, s, c->S, c->ain, b, stp);
+    }
+    BBoxAcc *part = c->R == 1 ? c->part : nullptr;
+    if (c->eros) {
+      k_stg_finish<true><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, b, c->nw, part, stp,
+                                                        c->eg);
+      const long tot = c->eg.np * c->R;
+      k_erode_nodes<<<(int)((tot + GF_BLOCK - 1) / GF_BLOCK), GF_BLOCK, 0, st>>>(tot, c->eg);
+    } else {
+      k_stg_finish<false><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, b, c->nw, part, stp,
+                                                         c->eg);
+    }
+    launch_column(c, s);
+    return;
+  }
   if (c->P.nfric != 4) {
     if (c->eros) {
       k_rk<true><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
@@ -3624,7 +3851,7 @@ static int enqueue_step(gfsph_ctx *c, int src, int dst) {
   if (rc) return rc;
   SortedSet &s = c->set[dst];
   launch_density(c, s);
-  launch_pforce(c, s);
+  if (!c->staged) launch_pforce(c, s);  // staged: once per stage instead
   launch_stages(c, s);
   launch_velocity(c, s);
   enqueue_case_end(c, dst);
@@ -3744,6 +3971,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   for (int r = 0; r < R; r++) {
     Ph[r].eros_single = (counts[r] == 1);
     c->eros = c->eros || Ph[r].eros_on;
+    c->staged = c->staged || Ph[r].ic_k0 || Ph[r].costheta;
   }
   c->P = Ph[0];
   c->R = R;
@@ -3762,7 +3990,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   *out = c;
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   if ((rc = alloc_sets(c, n))) return rc;
-  if (c->P.nfric == 4 && (rc = alloc_bingham(c, n))) return rc;
+  if ((c->P.nfric == 4 || c->staged) && (rc = alloc_bingham(c, n))) return rc;
   hipStream_t st = c->stream;
 
   // DEM
@@ -4384,6 +4612,7 @@ static int adjoint_supported(gfsph_ctx *c) {
   if (c->adapt) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs with a fixed dt_sph (ic_adapt_dt_sph = 0)");
   if (c->cases) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without case control");
   if (c->eros) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without erosion");
+  if (c->staged) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without ic_K0 and ic_cosTheta");
   if (P.nfric != 5 || P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 ||
       P.xi_voellmy < 0.0)
     return fail(c, GFSPH_ERR_UNSUPPORTED,

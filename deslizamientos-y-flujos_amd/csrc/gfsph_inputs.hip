@@ -813,10 +813,16 @@ int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
   if (nfric == 201 || nfric == 202 || nfric == 203)
     return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": nfric 201-203 (water inflow) is not supported");
   if (P.cst[1] < 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": inclined gravity (const(1) < 0)");
-  if (frictional) {
+  if (frictional) {  // ic_K0 and its coefficients (6:1373-1390)
     RD(R.text());
     RD(R.ints(iv, 1));
-    if (iv[0] != 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": K0 (ic_K0 = 1) is not supported");
+    P.ic_k0 = (int32_t)iv[0];
+    if (P.ic_k0 == 1) {
+      RD(R.text());
+      RD(R.reals(rv, 2));
+      P.k0_active = rv[0];
+      P.k0_pasive = rv[1];
+    }
   }
   RD(R.text());
   RD(R.ints(iv, 1));

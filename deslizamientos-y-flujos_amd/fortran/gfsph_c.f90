@@ -35,6 +35,10 @@ module gfsph_c
      integer(c_int32_t) :: pwp0_type         = 0
      real(c_double)     :: pwp_bc_val(2)     = 0.0_c_double  ! BC_pwp_Val
      real(c_double)     :: xirel             = 0.0_c_double
+     integer(c_int32_t) :: ic_k0             = 0          ! K0 active / passive (frictional laws)
+     integer(c_int32_t) :: pad_k0            = 0
+     real(c_double)     :: k0_active         = 1.0_c_double
+     real(c_double)     :: k0_pasive         = 1.0_c_double
   end type gfsph_params_t
 
   type, bind(C), public :: gfsph_dem_t

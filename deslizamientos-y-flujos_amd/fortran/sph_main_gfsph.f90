@@ -106,6 +106,9 @@ subroutine create_context
    do k = 1, min(110, size(const))
       p%cst(k) = real(const(k), c_double)      ! REAL(4) values, widened
    enddo
+   p%ic_k0     = ic_K0                          ! read for frictional laws only (6:1373-1390)
+   p%k0_active = K0_active
+   p%k0_pasive = K0_pasive
    if (icpwp .eq. 2) then                       ! the column Initialize_Pwp set up (6:1743-1768)
       p%nuaux          = nUaux
       p%pwp_bc_type(:) = BC_pwp_Type(:)

@@ -60,6 +60,11 @@ class Params:
     pwp_bc_val: tuple = (0.0, 0.0)
     pwp0_type: int = 0
     xirel: float = 0.0
+    # K0 (frictional laws, "6 SPH_SW_2019.f90":1373-1390): the pressure times
+    # k0_active where the velocity divergence is >= 0, else k0_pasive
+    ic_k0: int = 0
+    k0_active: float = 1.0
+    k0_pasive: float = 1.0
     const: dict = dataclasses.field(default_factory=dict)   # 1-based index -> value
 
     def cst(self) -> np.ndarray:

@@ -90,7 +90,8 @@ class Inputs:
                         nor_density=p.nor_density, icpwp=p.icpwp, npoin0=p.npoin0, dt=p.dt, h_inf=p.h_inf,
                         pwp0rel=p.pwp0rel, adapt_dt=self.run.adapt_dt if self.run.present else 0, const=const,
                         nuaux=p.nuaux, pwp_bc_type=(p.pwp_bc_type[0], p.pwp_bc_type[1]),
-                        pwp_bc_val=(p.pwp_bc_val[0], p.pwp_bc_val[1]), pwp0_type=p.pwp0_type, xirel=p.xirel)
+                        pwp_bc_val=(p.pwp_bc_val[0], p.pwp_bc_val[1]), pwp0_type=p.pwp0_type, xirel=p.xirel,
+                        ic_k0=p.ic_k0, k0_active=p.k0_active, k0_pasive=p.k0_pasive)
         d = self.dem()
         g = self.grid
         dem = FileDEM(d["Z"], g.xmin, g.ymin, g.dx, g.dy, d["in_model"], g.xmax, g.ymax, d["topol"])

@@ -63,6 +63,14 @@ typedef struct gfsph_params {
   int32_t pwp0_type;         /* initial profile 0..4 (init_input = 1)          */
   double pwp_bc_val[2];      /* BC_pwp_val (relative to the depth)             */
   double xirel;              /* pwp0_type = 4: relative height of the profile  */
+  /* ic_K0 = 1 (read by the reference for frictional laws only,
+   * 6:1373-1390): the pressure times k0_active where the velocity divergence
+   * is >= 0, else k0_pasive (6:6736-6760, 6:6857-6863).  With it, or with
+   * ic_cosTheta = const(16) = 1, the pressure force is re-evaluated in every
+   * RK stage, as IntForces_SW is. */
+  int32_t ic_k0;
+  int32_t pad_k0;
+  double k0_active, k0_pasive;
 } gfsph_params;
 
 /* Regular DEM grid ("2 SPH_FEM_topo_2019.f90":94-110, 223-252).  Z is

@@ -531,7 +531,7 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
   const double dt = p->dt;
   double h, hu, hv, hinv, vmod, hvmod, hvmodinv;
   double vxx = vx1, vyy = vy1;
-  double gravz = D->cgra;
+  double gravz = D->cgra; /* Get_Grav_Ext_SW sets it per particle when ic_cosTheta = 1 */
   double frictx = 0, fricty = 0, TotFrict = 0, cf5 = 0, hfrict, pwp;
   int nfric = D->nfric, if_frictional = D->if_frictional, if_V2byR = D->if_V2byR;
   double xi_voellmy = D->xi_voellmy;
@@ -556,10 +556,20 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
     hvmodinv = (hvmod >= 1.e-5) ? 1. / hvmod : 0.0;
   }
 
-  /* Get_Grav_Ext_SW, ic_cosTheta = 0 (6:9257-9299) */
+  /* Get_Grav_Ext_SW (6:9257-9299): g along the terrain normal when
+     ic_cosTheta = 1, else vertical g and the slope terms */
   topo_pt tp;
   get_z_topo(dem, xx, yy, if_frictional, &tp);
-  double gravx = -D->cgra * tp.Zx, gravy = -D->cgra * tp.Zy;
+  double gravx, gravy;
+  if (D->ic_cosTheta == 1) {
+    const double deng = sqrt(1. + tp.Zx * tp.Zx + tp.Zy * tp.Zy);
+    gravz = D->cgra / deng;
+    gravx = -D->cgra * tp.Zx / deng;
+    gravy = -D->cgra * tp.Zy / deng;
+  } else {
+    gravx = -D->cgra * tp.Zx;
+    gravy = -D->cgra * tp.Zy;
+  }
   exx = exx + gravx;
   exy = exy + gravy;
 
@@ -902,6 +912,64 @@ static void pwp_column(const gfo_params *p, const derived_t *D, int n, int N, do
   free(pwp0);
 }
 
+/* a5. IntForces_SW (6:6669-6941) with the frozen pair geometry of the step:
+   the pressures p(i) of the active particles -- 0.5 g h^2, with g along
+   the terrain normal at the particle when ic_cosTheta = 1 (6:6774-6783),
+   times K0_active / K0_pasive by the sign of the velocity divergence when
+   ic_K0 = 1 (6:6736-6760, 6:6857-6863) -- then the pair sum.  p keeps the
+   values of particles that are out of the domain (not recomputed).  Called
+   once per step when neither applies (the force is then the same in every
+   stage), else in every stage with that stage's positions and velocities. */
+static void pressure_force(const gfo_params *p, const derived_t *D, const gfo_dem *dem, int n, const pairvec *pv,
+                           const double *pdw, const double *mass, const double *rhoF, const int32_t *flag,
+                           const double *x, const double *v, double *p_, double *divv, double *inf) {
+  if (p->ic_k0) { /* divvK0 over the pairs, stage velocities (6:6736-6756) */
+    memset(divv, 0, sizeof(double) * n);
+    for (int64_t k = 0; k < pv->n; k++) {
+      int i = pv->i[k], j = pv->j[k];
+      double vcc = (v[2 * i] - v[2 * j]) * pdw[2 * k];
+      vcc = vcc + (v[2 * i + 1] - v[2 * j + 1]) * pdw[2 * k + 1];
+      divv[i] = divv[i] + mass[j] * vcc;
+      divv[j] = divv[j] + mass[i] * vcc;
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    if (flag[i]) continue;
+    double cgra05 = D->cgra * 0.5;
+    if (D->ic_cosTheta == 1) {
+      topo_pt tp;
+      get_z_topo(dem, x[2 * i], x[2 * i + 1], 0, &tp);
+      const double deng = sqrt(1. + tp.Zx * tp.Zx + tp.Zy * tp.Zy);
+      cgra05 = 0.5 * (D->cgra / deng);
+    }
+    p_[i] = cgra05 * rhoF[i] * rhoF[i];
+    if (p->ic_k0) p_[i] = p_[i] * (divv[i] >= 0 ? p->k0_active : p->k0_pasive);
+  }
+  memset(inf, 0, sizeof(double) * 2 * n);
+  for (int64_t kk = 0; kk < pv->n; kk++) {
+    int64_t k = p->rev_order ? pv->n - 1 - kk : kk;
+    int i = pv->i[k], j = pv->j[k];
+    double mI = mass[i], mJ = mass[j];
+    for (int d = 0; d < 2; d++) {
+      double hh = 0, gw = pdw[2 * k + d];
+      if (p->pa_sph == 1) {
+        double rhoij = 1.e0 / (rhoF[i] * rhoF[j]);
+        hh = -(p_[i] + p_[j]) * gw;
+        hh = hh * rhoij;
+        inf[2 * i + d] += mJ * hh;
+        inf[2 * j + d] -= mI * hh;
+      } else if (p->pa_sph == 2) {
+        hh = -(p_[i] / (rhoF[i] * rhoF[i]) + p_[j] / (rhoF[j] * rhoF[j])) * gw;
+        inf[2 * i + d] += mJ * hh;
+        inf[2 * j + d] -= mI * hh;
+      } else if (p->pa_sph == 4) {
+        inf[2 * i + d] -= ((p_[j] - p_[i]) * gw * mJ / rhoF[j]);
+        inf[2 * j + d] -= ((p_[j] - p_[i]) * gw * mI / rhoF[i]);
+      }
+    }
+  }
+}
+
 /* Node coordinates coorg(:, ipoig) of the terrain grid (2:285-296). */
 static void node_xy(const gfo_dem *d, int64_t ip, double *xg, double *yg) {
   if (d->coorg) {
@@ -942,6 +1010,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
   double *Dx = (double *)malloc(sizeof(double) * 2 * n), *Dv = (double *)malloc(sizeof(double) * 2 * n);
   double *Du = (double *)malloc(sizeof(double) * n), *p_ = (double *)malloc(sizeof(double) * n);
   double *av = (double *)malloc(sizeof(double) * 2 * n);
+  double *divv = (double *)calloc(n, sizeof(double));
   pairvec pv = {0, 0, NULL, NULL};
   double *pw = NULL, *pdw = NULL;
   int64_t pcap = 0;
@@ -990,34 +1059,11 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
     sum_density(p, n, &pv, pw, mass, hsml, flag, S);
 
     /* 3. pressure force with frozen dwdx (IntForces_SW, 6:6769-6941):
-          stage-independent when ic_K0 = 0 and ic_cosTheta = 0 */
+          the same in every stage unless ic_K0 = 1 or ic_cosTheta = 1 */
     double *rhoF = (double *)malloc(sizeof(double) * n);
     for (int i = 0; i < n; i++) rhoF[i] = (S[i] < p->h_inf) ? p->h_inf : S[i];
-    double cgra05 = D.cgra * 0.5;
-    for (int i = 0; i < n; i++)
-      if (!flag[i]) p_[i] = cgra05 * rhoF[i] * rhoF[i];
-    for (int64_t kk = 0; kk < pv.n; kk++) {
-      int64_t k = p->rev_order ? pv.n - 1 - kk : kk;
-      int i = pv.i[k], j = pv.j[k];
-      double mI = mass[i], mJ = mass[j];
-      for (int d = 0; d < 2; d++) {
-        double hh = 0, gw = pdw[2 * k + d];
-        if (p->pa_sph == 1) {
-          double rhoij = 1.e0 / (rhoF[i] * rhoF[j]);
-          hh = -(p_[i] + p_[j]) * gw;
-          hh = hh * rhoij;
-          inf[2 * i + d] += mJ * hh;
-          inf[2 * j + d] -= mI * hh;
-        } else if (p->pa_sph == 2) {
-          hh = -(p_[i] / (rhoF[i] * rhoF[i]) + p_[j] / (rhoF[j] * rhoF[j])) * gw;
-          inf[2 * i + d] += mJ * hh;
-          inf[2 * j + d] -= mI * hh;
-        } else if (p->pa_sph == 4) {
-          inf[2 * i + d] -= ((p_[j] - p_[i]) * gw * mJ / rhoF[j]);
-          inf[2 * j + d] -= ((p_[j] - p_[i]) * gw * mI / rhoF[i]);
-        }
-      }
-    }
+    const int staged = p->ic_k0 || D.ic_cosTheta == 1;
+    if (!staged) pressure_force(p, &D, dem, n, &pv, pdw, mass, rhoF, flag, x, v, p_, divv, inf);
 
     /* 4. RK stages (7:808-906) */
     for (int s = 0; s < nst; s++) {
@@ -1039,6 +1085,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         rho[i] = flag[i] ? rho[i] : S[i];
         if (rho[i] < p->h_inf) rho[i] = p->h_inf; /* 6:6769 applies to all */
       }
+      if (staged) pressure_force(p, &D, dem, n, &pv, pdw, mass, rhoF, flag, x, v, p_, divv, inf);
       for (int i = 0; i < n; i++) {
         if (flag[i]) continue;
         extout o;
@@ -1187,6 +1234,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
   free(Du);
   free(p_);
   free(av);
+  free(divv);
   free(pw);
   free(pdw);
   free(pv.i);

@@ -27,6 +27,10 @@ typedef struct {
   int32_t pwp0_type;          /* initial profile 0..4 */
   double pwp_bc_val[2];       /* BC_pwp_val */
   double xirel;               /* pwp0_type 4 */
+  /* K0 (frictional laws, 6:1373-1390): pressure times K0_active where the
+   * velocity divergence is >= 0, else K0_pasive (6:6736-6760, 6:6857-6863) */
+  int32_t ic_k0, pad3_;
+  double k0_active, k0_pasive;
 } gfo_params;
 
 typedef struct {

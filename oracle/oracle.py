@@ -22,7 +22,8 @@ class GfoParams(C.Structure):
                 ("icpwp", C.c_int32), ("npoin0", C.c_int32), ("rev_order", C.c_int32),
                 ("adapt_dt", C.c_int32), ("pad2_", C.c_int32), ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double),
                 ("cst", C.c_double * 111), ("nuaux", C.c_int32), ("pwp_bc_type", C.c_int32 * 2),
-                ("pwp0_type", C.c_int32), ("pwp_bc_val", C.c_double * 2), ("xirel", C.c_double)]
+                ("pwp0_type", C.c_int32), ("pwp_bc_val", C.c_double * 2), ("xirel", C.c_double),
+                ("ic_k0", C.c_int32), ("pad3_", C.c_int32), ("k0_active", C.c_double), ("k0_pasive", C.c_double)]
 
 
 class GfoDem(C.Structure):
@@ -87,6 +88,9 @@ def make_params(prm) -> GfoParams:
     p.pwp_bc_val[0], p.pwp_bc_val[1] = float(bv[0]), float(bv[1])
     p.pwp0_type = int(getattr(prm, "pwp0_type", 0))
     p.xirel = float(getattr(prm, "xirel", 0.0))
+    p.ic_k0 = int(getattr(prm, "ic_k0", 0))
+    p.k0_active = float(getattr(prm, "k0_active", 1.0))
+    p.k0_pasive = float(getattr(prm, "k0_pasive", 1.0))
     return p
 
 

@@ -64,7 +64,12 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
         L += [f"const {lo}..{lo + 14}", " ".join(_r(c[k]) for k in range(lo, lo + 15))]
     nfric = int(c[5] + 0.01)
     if (5 <= nfric <= 10) or nfric in (26, 27):
-        L += ["K0 activated?", "0"]
+        k0 = int(getattr(prm, "ic_k0", 0))
+        L += ["K0 activated?", str(k0)]
+        if k0 == 1:  # 6:1383-1389
+            L += ["K0_active K0_pasive", f"{_r(prm.k0_active)} {_r(prm.k0_pasive)}"]
+    elif int(getattr(prm, "ic_k0", 0)):
+        raise ValueError("the reference reads ic_K0 only for frictional laws")
     L += ["icpwp", str(prm.icpwp)]
     if prm.icpwp in (1, 3):
         L += ["pwp0rel", _r(prm.pwp0rel)]

@@ -156,6 +156,25 @@ def case_defs():
                                      pwp_bc_type=bt, pwp_bc_val=bv, pwp0_type=p0t, xirel=xirel,
                                      const=C.frictional(28.0, cv=cv)),
                         dem, scar_on(dem, 300, seed), 100), dumps=[1, 10, 100])
+    # pressure that changes within the step (IntForces_SW in every stage):
+    # K0 active / passive by the sign of the velocity divergence (6:6736-6760,
+    # 6:6857-6863), g along the terrain normal (ic_cosTheta = 1, 6:6774-6783,
+    # 6:9264-9270), both with curvature, K0 with pore pressure
+    ck = C.voellmy(0.35, 0.0)
+    defs["g_k0"] = dict(
+        case=C.Case("g_k0", C.Params(integrator=4, dt=0.02, h_inf=1e-3, ic_k0=1, k0_active=0.6, k0_pasive=1.8,
+                                     const=ck), dem, scar_on(dem, 300, 51), 100), dumps=[1, 10, 100])
+    cc = C.voellmy(0.25, 400.0, nfric=7)
+    cc.update({11: -1.0, 16: 1})
+    defs["g_costheta_k0"] = dict(
+        case=C.Case("g_costheta_k0", C.Params(integrator=4, dt=0.02, h_inf=1e-3, ic_k0=1, k0_active=0.7,
+                                              k0_pasive=1.5, const=cc), dem, scar_on(dem, 300, 52), 100),
+        dumps=[1, 10, 100])
+    cp = C.frictional(28.0, cv=0.01)
+    defs["g_k0_pwp"] = dict(
+        case=C.Case("g_k0_pwp", C.Params(integrator=4, dt=0.02, h_inf=1e-3, icpwp=3, pwp0rel=0.6, ic_k0=1,
+                                         k0_active=0.5, k0_pasive=2.0, const=cp), dem, scar_on(dem, 300, 53), 100),
+        dumps=[1, 10, 100])
     # random cloud with varied smoothing lengths: neighbour set only
     rng = np.random.Generator(np.random.Philox(99))
     n = 600

@@ -15,7 +15,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STATE_CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_curvature7", "g_mui8", "g_icpwp1_sle1",
                "g_manning1", "g_perzyna9", "g_hb25", "g_bagnold11", "g_kernel_skf2", "g_kernel_skf3",
                "g_kernel_skf4", "g_domain_exit", "g_adaptive_dt2", "g_adaptive_dt10", "g_erosion_hungr",
-               "g_erosion_slope", "g_erosion_stress", "g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step"]
+               "g_erosion_slope", "g_erosion_stress", "g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step",
+               "g_k0", "g_costheta_k0", "g_k0_pwp"]
+STAGED_CASES = ["g_k0", "g_costheta_k0", "g_k0_pwp"]
 PWP_COLUMN_CASES = ["g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step"]
 EROSION_CASES = ["g_erosion_hungr", "g_erosion_slope", "g_erosion_stress"]
 ALL_CASES = STATE_CASES + ["g_random_neighbours"]

@@ -82,7 +82,7 @@ def test_picacho_release_and_run_match_reference(picacho):
 
 
 @pytest.mark.parametrize("name", ["g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1", "g_adaptive_dt10",
-                                  "g_pwp_column_step", "g_erosion_hungr"])
+                                  "g_pwp_column_step", "g_erosion_hungr", "g_k0"])
 def test_file_path_equals_array_path(name, tmp_path):
     case, _ = goldens.case_of(name)
     refio.write_inputs(str(tmp_path), case, name="case", topo="topo")

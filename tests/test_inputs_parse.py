@@ -53,7 +53,7 @@ def test_picacho_grid_and_run_control(tmp_path):
 
 
 @pytest.mark.parametrize("name", ["g_frictional_pwp", "g_icpwp1_sle1", "g_bingham", "g_hb25", "g_pwp_column_step",
-                                  "g_erosion_hungr"])
+                                  "g_erosion_hungr", "g_k0", "g_costheta_k0"])
 def test_restart_inputs_round_trip(name, tmp_path):
     """Inputs written in the reference's formats (explicit-grid .top with
     finite-difference slopes, restart.dat) read back exactly."""
@@ -67,6 +67,8 @@ def test_restart_inputs_round_trip(name, tmp_path):
         assert p.h_inf == cp.h_inf
         if cp.icpwp in (1, 2, 3):
             assert p.pwp0rel == cp.pwp0rel
+        if cp.ic_k0:  # K0 coefficients (6:1383-1389)
+            assert (p.ic_k0, p.k0_active, p.k0_pasive) == (1, cp.k0_active, cp.k0_pasive)
         if cp.icpwp == 2:  # the column block of Initialize_Pwp
             assert (p.nuaux, tuple(p.pwp_bc_type), tuple(p.pwp_bc_val), p.pwp0_type, p.xirel) == \
                 (cp.nuaux, tuple(cp.pwp_bc_type), tuple(cp.pwp_bc_val), cp.pwp0_type, cp.xirel)

@@ -32,10 +32,11 @@ def test_library_loads_and_exports_every_symbol():
 def test_params_struct_layout_matches_header():
     # gfsph_params: 12 int32, 3 doubles, 111 doubles, then the pore-pressure
     # column block: 4 int32 (nuaux, bc types, pwp0_type) and 3 doubles
-    assert ctypes.sizeof(api.CParams) == 12 * 4 + 3 * 8 + 111 * 8 + 4 * 4 + 3 * 8
+    assert ctypes.sizeof(api.CParams) == 12 * 4 + 3 * 8 + 111 * 8 + 4 * 4 + 3 * 8 + 2 * 4 + 2 * 8
     assert api.CParams.dt.offset == 48 and api.CParams.cst.offset == 72
     assert api.CParams.nuaux.offset == 960 and api.CParams.pwp_bc_val.offset == 976
     assert api.CParams.xirel.offset == 992
+    assert api.CParams.ic_k0.offset == 1000 and api.CParams.k0_active.offset == 1008
     assert api.CDem.Z.offset == 56
 
 
