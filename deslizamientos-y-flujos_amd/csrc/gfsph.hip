@@ -828,18 +828,49 @@ __global__ void k_rng(int R, const int *coff, const int *start, int2 *rng) {
   if (r < R) rng[r] = make_int2(start[coff[r]], start[coff[r + 1]]);
 }
 
-// One block per realisation: bounding box of its active particles (slot
-// range rng[r] of the step's output set) and its grid, capped at
-// 16 cells per particle + 64.
-__global__ void k_bbox_real(const int2 *rng, const double2 *pos, const double *hsml, const int *flag, int scale_k,
-                            GridInfo *grid) {
+
+
+// Bounding boxes of the realisations of a batch: realisation r's slot range
+// (rng[r]) is cut into chunks of BOX_CHUNK slots, one work-group per (r,
+// chunk) reduces its chunk to a partial (no atomics), and k_grid_real folds
+// each realisation's partials into its grid.  Min / max / count do not
+// depend on the order, so the grids are those of a serial reduction.
+#define BOX_CHUNK 4096
+
+__device__ __forceinline__ BBoxAcc bbox_identity() {
+  BBoxAcc a;
+  a.xmin = ~0ull;
+  a.ymin = ~0ull;
+  a.xmax = 0ull;
+  a.ymax = 0ull;
+  a.hmax = 0ull;
+  a.hmin = ~0ull;
+  a.nactive = 0u;
+  a.pad = 0u;
+  return a;
+}
+
+__device__ __forceinline__ void bbox_fold(BBoxAcc &a, const BBoxAcc &b) {
+  a.xmin = b.xmin < a.xmin ? b.xmin : a.xmin;
+  a.xmax = b.xmax > a.xmax ? b.xmax : a.xmax;
+  a.ymin = b.ymin < a.ymin ? b.ymin : a.ymin;
+  a.ymax = b.ymax > a.ymax ? b.ymax : a.ymax;
+  a.hmax = b.hmax > a.hmax ? b.hmax : a.hmax;
+  a.hmin = b.hmin < a.hmin ? b.hmin : a.hmin;
+  a.nactive += b.nactive;
+}
+
+// grid: (R, nchunk) work-groups of GF_BLOCK threads; part: [R][nchunk]
+__global__ void k_bbox_chunks(const int2 *rng, const double2 *pos, const double *hsml, const int *flag,
+                              BBoxAcc *part) {
   __shared__ BBoxAcc wp[GF_BLOCK / 64];
-  const int r = blockIdx.x;
+  const int r = blockIdx.x, ch = blockIdx.y;
   const int2 rg = rng[r];
+  const int b = rg.x + ch * BOX_CHUNK, e = min(rg.y, b + BOX_CHUNK);
   const unsigned long long BIG = ~0ull, SMALL = 0ull;
   unsigned long long oxmin = BIG, oxmax = SMALL, oymin = BIG, oymax = SMALL, ohmax = SMALL, ohmin = BIG;
   unsigned cnt = 0u;
-  for (int i = rg.x + (int)threadIdx.x; i < rg.y; i += GF_BLOCK) {
+  for (int i = b + (int)threadIdx.x; i < e; i += GF_BLOCK) {
     if (flag[i]) continue;
     const double2 p = pos[i];
     const unsigned long long ox = ord_of(p.x), oy = ord_of(p.y), oh = ord_of(hsml[i]);
@@ -871,19 +902,21 @@ __global__ void k_bbox_real(const int2 *rng, const double2 *pos, const double *h
   __syncthreads();
   if (threadIdx.x == 0) {
     BBoxAcc a = wp[0];
-    for (int w = 1; w < GF_BLOCK / 64; w++) {
-      a.xmin = wp[w].xmin < a.xmin ? wp[w].xmin : a.xmin;
-      a.xmax = wp[w].xmax > a.xmax ? wp[w].xmax : a.xmax;
-      a.ymin = wp[w].ymin < a.ymin ? wp[w].ymin : a.ymin;
-      a.ymax = wp[w].ymax > a.ymax ? wp[w].ymax : a.ymax;
-      a.hmax = wp[w].hmax > a.hmax ? wp[w].hmax : a.hmax;
-      a.hmin = wp[w].hmin < a.hmin ? wp[w].hmin : a.hmin;
-      a.nactive += wp[w].nactive;
-    }
-    grid[r] = make_grid(a, scale_k, 16 * (rg.y - rg.x) + 64);
+    for (int w = 1; w < GF_BLOCK / 64; w++) bbox_fold(a, wp[w]);
+    part[(long)r * gridDim.y + ch] = a;
   }
 }
 
+// grid of every realisation from its partials, capped at 16 cells per
+// particle of its slot range + 64
+__global__ void k_grid_real(int R, int nchunk, const int2 *rng, const BBoxAcc *part, int scale_k, GridInfo *grid) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int2 rg = rng[r];
+  BBoxAcc a = bbox_identity();
+  for (int ch = 0; ch < nchunk; ch++) bbox_fold(a, part[(long)r * nchunk + ch]);
+  grid[r] = make_grid(a, scale_k, 16 * (rg.y - rg.x) + 64);
+}
 
 __device__ __forceinline__ void cell_of(const GridInfo &g, double2 p, int &cx, int &cy) {
   cx = (int)((p.x - g.xmin) / g.cell);
@@ -1269,7 +1302,7 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 // Pr holds one parameter block per realisation; the lanes of a wave are
 // served one realisation at a time (normally one pass), so every parameter
 // read is a scalar load.  part = nullptr: no bounding-box partials (several
-// realisations: k_bbox_real computes their boxes).
+// realisations: k_bbox_chunks computes their boxes).
 template <bool EROS>
 __global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
                                                   const double *S, const double2 *ain, NewSet o, BBoxAcc *part,
@@ -1525,52 +1558,55 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 #endif
 #define GATHER_MAX_BLOCK 512
 
-// Each particle's gather is shared by a team of GATHER_TEAM adjacent lanes:
-// lane t of the team takes batches t, t+T, t+2T, ... of the candidate stream
-// (density) or the neighbours of those batches (force, XSPH), keeps its own
-// partial sums, and the team adds them in a fixed tree (so every run, batch
-// and work-group size gives the same bits).  More lanes per particle keep
-// more loads in flight per SIMD when the particles alone fill it only a few
-// waves deep.
-#ifndef GATHER_TEAM
-#define GATHER_TEAM 2
+// Every gather splits a particle's candidate stream into two halves: the
+// 4-position batches 0, 2, 4, ... and 1, 3, 5, ... (the force and XSPH passes:
+// the neighbours in those batches).  Each half is summed in stream order and
+// the result is half 0 + half 1.  A context runs the halves either on two
+// adjacent lanes (TM = 2: more loads in flight when the particles alone fill
+// each SIMD only a few waves deep) or one after the other on one lane (TM =
+// 1: large batches); both give the same bits, for every work-group size.
+#define SPLIT 2
+// particles from which one lane per particle is the default
+#ifndef TEAM1_MIN_N
+#define TEAM1_MIN_N 1000000
 #endif
-static_assert(GATHER_TEAM == 1 || GATHER_TEAM == 2 || GATHER_TEAM == 4, "team of 1, 2 or 4 lanes");
 // default work-group size of the gathers (gfsph_autotune may change it)
 #ifndef GATHER_BLOCK
 #define GATHER_BLOCK 64
 #endif
 
-// particle and team lane of this thread
+// particle and lane in its team of this thread
+template <int TM>
 __device__ __forceinline__ void team_index(int &i, int &t) {
   const int g = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  i = g / GATHER_TEAM;
-  t = g % GATHER_TEAM;
+  i = g / TM;
+  t = g % TM;
 }
 
-// sum over the team (identical in every lane: each level adds two equal
-// pairs of partial sums in either order)
-__device__ __forceinline__ double team_sum(double v) {
-#pragma unroll
-  for (int o = 1; o < GATHER_TEAM; o <<= 1) v = v + __shfl_xor(v, o);
-  return v;
+// half 0 + half 1: the lane's two partial sums, or the pair's (lane 0 adds
+// its own and its partner's; lane 1 gets the same value in the other order)
+template <int TM>
+__device__ __forceinline__ double halves_sum(const double (&a)[SPLIT / TM]) {
+  if constexpr (TM == 1)
+    return a[0] + a[1];
+  else
+    return a[0] + __shfl_xor(a[0], 1);
 }
 
-__device__ __forceinline__ unsigned long long team_or(unsigned long long v) {
-#pragma unroll
-  for (int o = 1; o < GATHER_TEAM; o <<= 1) {
-    const unsigned lo = __shfl_xor((unsigned)v, o), hi = __shfl_xor((unsigned)(v >> 32), o);
-    v |= ((unsigned long long)hi << 32) | lo;
+template <int TM>
+__device__ __forceinline__ unsigned long long halves_or(unsigned long long v) {
+  if constexpr (TM == 1) {
+    return v;
+  } else {
+    const unsigned lo = __shfl_xor((unsigned)v, 1), hi = __shfl_xor((unsigned)(v >> 32), 1);
+    return v | ((unsigned long long)hi << 32) | lo;
   }
-  return v;
 }
 
-// mask bits of the batches (nibbles of NB_BATCH = 4 positions) team lane t takes
-__device__ __forceinline__ unsigned long long team_bits(int t) {
-  static_assert(NB_BATCH == 4, "the team split is by 4-position nibbles");
-  if (GATHER_TEAM == 1) return ~0ull;
-  if (GATHER_TEAM == 2) return 0x0F0F0F0F0F0F0F0Full << (4 * t);
-  return 0x000F000F000F000Full << (4 * t);
+// mask bits of half h (nibbles of NB_BATCH = 4 positions)
+__device__ __forceinline__ unsigned long long split_bits(int h) {
+  static_assert(NB_BATCH == 4, "the halves are made of 4-position nibbles");
+  return 0x0F0F0F0F0F0F0F0Full << (4 * h);
 }
 
 struct CandStream {
@@ -1701,7 +1737,7 @@ __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned lo
 // Every neighbour in the stream, with the exact predicate, for particles
 // whose streams are too long for the mask (not batched: rare).
 // Positions t, t+T, t+2T, ... of the stream (T = 1: all of them).
-template <int T = GATHER_TEAM, typename F>
+template <int T = SPLIT, typename F>
 __device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, int i, int t, double2 pi_, double hi,
                                              const double4 *nb, F &&f) {
   for (int p = t; p < cs.total; p += T) {
@@ -1720,16 +1756,13 @@ __device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, 
 // smoothing lengths are equal, so the support radius scale_k*(hi+hj)/2 =
 // scale_k*hi and the kernel's 1/h are the particle's own.
 template <int SKF, bool UNIFORM>
-__device__ __forceinline__ double density_stream(const CandStream &cs, const DevParams &P, int i, int t,
+__device__ __forceinline__ double density_stream(const CandStream &cs, const DevParams &P, int i, int h,
                                                  double2 pi_, double hi, const double4 *nb,
-                                                 unsigned long long *mrow) {
+                                                 unsigned long long (&wd)[NB_WORDS]) {
   const PredBand pu = pred_band(P.scale_k * hi);
   const double ihi = rcp_nr(hi);
   double acc = 0.0;
-  unsigned long long wd[NB_WORDS];
-#pragma unroll
-  for (int k = 0; k < NB_WORDS; k++) wd[k] = 0ull;
-  for (int k0 = t * NB_BATCH; k0 < cs.total; k0 += GATHER_TEAM * NB_BATCH) {
+  for (int k0 = h * NB_BATCH; k0 < cs.total; k0 += SPLIT * NB_BATCH) {
     int j[NB_BATCH];
     double4 q[NB_BATCH];
 #pragma unroll
@@ -1768,21 +1801,15 @@ __device__ __forceinline__ double density_stream(const CandStream &cs, const Dev
 #pragma unroll
     for (int k = 0; k < NB_WORDS; k++) wd[k] |= (k == (k0 >> 6)) ? bits : 0ull;
   }
-  const int nw = (cs.total + 63) >> 6;
-#pragma unroll
-  for (int k = 0; k < NB_WORDS; k++) {
-    const unsigned long long v = team_or(wd[k]);
-    if (k < nw && k % GATHER_TEAM == t) mrow[k] = v;
-  }
   return acc;
 }
 
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
-template <int SKF>
+template <int SKF, int TM>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
                                                                  double *Pout, unsigned long long *mask) {
   int i, t;
-  team_index(i, t);
+  team_index<TM>(i, t);
   if (i >= n) return;
   if (s.flag[i]) {
     if (t == 0) {
@@ -1797,20 +1824,36 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
-  double acc = 0.0;
+  double a[SPLIT / TM];
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, t, pi_, hi, s.nb,
-                 [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                   double w, gx, gy;
-                   sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                   acc = acc + q.w * w;
-                 });
+#pragma unroll
+    for (int hh = 0; hh < SPLIT / TM; hh++) {
+      double acc = 0.0;
+      stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
+                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                     double w, gx, gy;
+                     sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                     acc = acc + q.w * w;
+                   });
+      a[hh] = acc;
+    }
   } else {
+    unsigned long long wd[NB_WORDS];
+#pragma unroll
+    for (int k = 0; k < NB_WORDS; k++) wd[k] = 0ull;
+#pragma unroll
+    for (int hh = 0; hh < SPLIT / TM; hh++)
+      a[hh] = g.uniform_h ? density_stream<SKF, true>(cs, P, i, t + hh * TM, pi_, hi, s.nb, wd)
+                          : density_stream<SKF, false>(cs, P, i, t + hh * TM, pi_, hi, s.nb, wd);
     unsigned long long *mrow = mask + NB_WORDS * (long)i;
-    acc = g.uniform_h ? density_stream<SKF, true>(cs, P, i, t, pi_, hi, s.nb, mrow)
-                      : density_stream<SKF, false>(cs, P, i, t, pi_, hi, s.nb, mrow);
+    const int nw = (cs.total + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NB_WORDS; k++) {
+      const unsigned long long v = halves_or<TM>(wd[k]);
+      if (k < nw && k % TM == t) mrow[k] = v;
+    }
   }
-  acc = team_sum(acc);
+  const double acc = halves_sum<TM>(a);
   if (t != 0) return;
   const double Si = sph_self_w(SKF, P.pi, hi) * s.mass[i] + acc;
   S[i] = Si;
@@ -1840,12 +1883,12 @@ __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, do
 }
 
 // a5 pressure force (IntForces_SW pair sum, 6:6892-6941) -> ain[i]
-template <int SKF, int PA>
+template <int SKF, int PA, int TM>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl,
                                                                 const unsigned long long *mask, SortedSet s,
                                                                 const double *S, const double *Pp, double2 *ain) {
   int i, t;
-  team_index(i, t);
+  team_index<TM>(i, t);
   if (i >= n) return;
   if (s.flag[i]) {
     if (t == 0) ain[i] = make_double2(0.0, 0.0);
@@ -1860,9 +1903,13 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
   const double Si = S[i];
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
-  double inx = 0.0, iny = 0.0;
+  double ax[SPLIT / TM], ay[SPLIT / TM];
+#pragma unroll
+  for (int hh = 0; hh < SPLIT / TM; hh++) {
+    const int h = t + hh * TM;
+    double inx = 0.0, iny = 0.0;
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, t, pi_, hi, s.nb,
+    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb,
                  [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
                    double w, gx, gy;
                    sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
@@ -1870,7 +1917,7 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
                  });
   } else {
     unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, team_bits(t));
+    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
     const double ihi = rcp_nr(hi);
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
@@ -1902,18 +1949,20 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
         body(std::false_type{}, js, ok);
       });
   }
-  inx = team_sum(inx);
-  iny = team_sum(iny);
+    ax[hh] = inx;
+    ay[hh] = iny;
+  }
+  const double inx = halves_sum<TM>(ax), iny = halves_sum<TM>(ay);
   if (t == 0) ain[i] = make_double2(inx, iny);
 }
 
 // a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
-template <int SKF>
+template <int SKF, int TM>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl,
                                                               const unsigned long long *mask, SortedSet s, NewSet o,
                                                               double2 *vout, const int *stopr) {
   int i, t;
-  team_index(i, t);
+  team_index<TM>(i, t);
   if (i >= n) return;
   double2 vi = o.vel[i];
   if (s.flag[i] || o.flag[i] || s.id[i] < P.npoin0 || (stopr && stopr[s.rid[i]])) {
@@ -1928,9 +1977,13 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
   const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
-  double ax = 0.0, ay = 0.0;
+  double hx[SPLIT / TM], hy[SPLIT / TM];
+#pragma unroll
+  for (int hh = 0; hh < SPLIT / TM; hh++) {
+    const int h = t + hh * TM;
+    double ax = 0.0, ay = 0.0;
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, t, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
       if (skip0 && s.id[j] < P.npoin0) return;
       double w, gx, gy;
       sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
@@ -1941,7 +1994,7 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
     });
   } else {
     unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, team_bits(t));
+    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
     const double ihi = rcp_nr(hi);
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
@@ -1976,8 +2029,10 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
         body(std::false_type{}, js, ok);
       });
   }
-  ax = team_sum(ax);
-  ay = team_sum(ay);
+    hx[hh] = ax;
+    hy[hh] = ay;
+  }
+  const double ax = halves_sum<TM>(hx), ay = halves_sum<TM>(hy);
   if (t != 0) return;
   vi.x = vi.x + 0.3 * ax;
   vi.y = vi.y + 0.3 * ay;
@@ -3175,11 +3230,11 @@ __global__ void k_raster_idw(int n, DevDem dem, CurView s, double *acc5) {
       b = b * b;
       double dist = fmax(1.e-6, sqrt(a + b));
       dist = dist / distg;
-      atomicAdd(&acc5[5 * gidx + 0], h / dist);
-      atomicAdd(&acc5[5 * gidx + 1], v.x / dist);
-      atomicAdd(&acc5[5 * gidx + 2], v.y / dist);
-      atomicAdd(&acc5[5 * gidx + 3], (uu / h) / dist);
-      atomicAdd(&acc5[5 * gidx + 4], 1.00 / dist);
+      unsafeAtomicAdd(&acc5[5 * gidx + 0], h / dist);
+      unsafeAtomicAdd(&acc5[5 * gidx + 1], v.x / dist);
+      unsafeAtomicAdd(&acc5[5 * gidx + 2], v.y / dist);
+      unsafeAtomicAdd(&acc5[5 * gidx + 3], (uu / h) / dist);
+      unsafeAtomicAdd(&acc5[5 * gidx + 4], 1.00 / dist);
     }
 }
 
@@ -3216,8 +3271,8 @@ __global__ void k_mc_pixel(int n, DevDem dem, CurView s, long stride, double *ac
   for (int k = 0; k < 4; k++) {
     long g = nodes[k];
     atomicAdd(&cnt[g], 1);
-    atomicAdd(&acc4[4 * g + 0], hh);
-    atomicAdd(&acc4[4 * g + 2], vv);
+    unsafeAtomicAdd(&acc4[4 * g + 0], hh);  // hardware FP64 add
+    unsafeAtomicAdd(&acc4[4 * g + 2], vv);
     atomic_max_pos(&acc4[4 * g + 1], hh);
     atomic_max_pos(&acc4[4 * g + 3], vv);
   }
@@ -3287,12 +3342,17 @@ struct gfsph_ctx {
   int cap_cells = 0, scan_blocks = 0;
   GridInfo *grid = nullptr;
   BBoxAcc *acc = nullptr, *part = nullptr;
+  BBoxAcc *boxpart = nullptr;  // batches: per-realisation box partials [R][box_chunks]
+  int box_chunks = 1;
   int gridN = 0;  // blocks of the per-particle kernels (multiple of 8)
   hipGraphExec_t graph[2] = {nullptr, nullptr};   // one step from set[k]
   hipGraphExec_t graphG[2] = {nullptr, nullptr};  // GRAPH_STEPS steps from set[k]
   // work-group size of each gather kernel (density, force, XSPH), chosen by
   // gfsph_autotune; all sizes give identical results
   int gbs[3] = {GATHER_BLOCK, GATHER_BLOCK, GATHER_BLOCK};
+  // lanes per particle of each gather (1 or 2; the same bits either way):
+  // 2 below TEAM1_MIN_N particles, else 1; gfsph_autotune may change them
+  int gteam[3] = {2, 2, 2};
   int *bad = nullptr;
   double2 *zxy = nullptr;
   double *top = nullptr;
@@ -3627,6 +3687,12 @@ static int alloc_sets(gfsph_ctx *c, int n) {
     return rc;
   c->gridN = ((nblk(n) + 7) / 8) * 8;
   if ((rc = dalloc(c, &c->part, c->gridN))) return rc;
+  if (c->R > 1) {  // a realisation's slot range never exceeds its particle count
+    int mx = 1;
+    for (int v : c->counts_h) mx = v > mx ? v : mx;
+    c->box_chunks = (mx + BOX_CHUNK - 1) / BOX_CHUNK;
+    if ((rc = dalloc(c, &c->boxpart, (size_t)c->R * c->box_chunks))) return rc;
+  }
   // the cell counts start zero; every build clears the ones it used
   HIPCHK(c, hipMemsetAsync(c->cnt, 0, sizeof(int) * ((size_t)c->cap_cells + 2), c->stream));
   return GFSPH_OK;
@@ -3656,8 +3722,24 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
     }                                               \
   } while (0)
 
-#define PFORCE_PA(SKFV, PAV) \
-  k_pforce_m<SKFV, PAV><<<xcd_grid((long)c->n * GATHER_TEAM, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS)
+// gathers of team size TMV (1 or 2 lanes per particle) for each kernel smoothing function
+#define SKF_TM_DISPATCH(KERNEL, TMV, ...)                \
+  do {                                                   \
+    switch (c->P.skf) {                                  \
+      case 1: KERNEL<1, TMV><<<__VA_ARGS__; break;       \
+      case 2: KERNEL<2, TMV><<<__VA_ARGS__; break;       \
+      case 3: KERNEL<3, TMV><<<__VA_ARGS__; break;       \
+      default: KERNEL<4, TMV><<<__VA_ARGS__; break;      \
+    }                                                    \
+  } while (0)
+
+#define PFORCE_PA(SKFV, PAV)                                                                                  \
+  do {                                                                                                        \
+    if (tm == 1)                                                                                              \
+      k_pforce_m<SKFV, PAV, 1><<<xcd_grid((long)c->n, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS);                \
+    else                                                                                                      \
+      k_pforce_m<SKFV, PAV, 2><<<xcd_grid((long)c->n * 2, bsz), bsz, 0, c->stream>>>(PFORCE_ARGS);            \
+  } while (0)
 #define PFORCE_CASE(SKFV)                      \
   switch (c->P.pa_sph) {                       \
     case 1: PFORCE_PA(SKFV, 1); break;         \
@@ -3669,7 +3751,7 @@ static int alloc_bingham(gfsph_ctx *c, int n) {
 static inline int xcd_grid(long n, int bsz) { return (int)((((n + bsz - 1) / bsz) + 7) / 8 * 8); }
 
 static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
-  const int bsz = c->gbs[1];
+  const int bsz = c->gbs[1], tm = c->gteam[1];
 #define PFORCE_ARGS c->n, c->P, cells_of(c), c->mask, s, c->S, c->Pp, c->ain
   switch (c->P.skf) {
     case 1: PFORCE_CASE(1) break;
@@ -3760,7 +3842,10 @@ static int neighbour_build(gfsph_ctx *c, int src, int dst) {
     k_grid<<<1, GF_BLOCK, 0, st>>>(c->gridN, c->part, c->acc, c->grid, c->coff, c->P.scale_k, c->cap_cells);
   } else {
     if (!c->first_build) k_rng<<<(R + GF_BLOCK - 1) / GF_BLOCK, GF_BLOCK, 0, st>>>(R, c->coff, c->start, c->rng);
-    k_bbox_real<<<R, GF_BLOCK, 0, st>>>(c->rng, c->nw.pos, c->nw.hsml, c->nw.flag, c->P.scale_k, c->grid);
+    k_bbox_chunks<<<dim3(R, c->box_chunks), GF_BLOCK, 0, st>>>(c->rng, c->nw.pos, c->nw.hsml, c->nw.flag,
+                                                                c->boxpart);
+    k_grid_real<<<(R + GF_BLOCK - 1) / GF_BLOCK, GF_BLOCK, 0, st>>>(R, c->box_chunks, c->rng, c->boxpart,
+                                                                    c->P.scale_k, c->grid);
     k_coff<<<1, GF_BLOCK, 0, st>>>(R, c->grid, c->coff);
   }
   c->first_build = false;
@@ -3782,16 +3867,24 @@ __global__ void k_copy_vel(int n, const double2 *src, double2 *dst) {
 
 static void launch_density(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[0];
-  SKF_DISPATCH(k_density_m, xcd_grid((long)c->n * GATHER_TEAM, bsz), bsz, 0,
-               c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
+  if (c->gteam[0] == 1)
+    SKF_TM_DISPATCH(k_density_m, 1, xcd_grid((long)c->n, bsz), bsz, 0,
+                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
+  else
+    SKF_TM_DISPATCH(k_density_m, 2, xcd_grid((long)c->n * 2, bsz), bsz, 0,
+                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
 }
 
 // XSPH (or the plain copy of the post-RK velocity) into the sorted set
 static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[2];
-  if (c->P.av_vel)
-    SKF_DISPATCH(k_xsph_m, xcd_grid((long)c->n * GATHER_TEAM, bsz), bsz, 0,
-                 c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, c->cases ? c->stopr : nullptr));
+  const int *stp = c->cases ? c->stopr : nullptr;
+  if (c->P.av_vel && c->gteam[2] == 1)
+    SKF_TM_DISPATCH(k_xsph_m, 1, xcd_grid((long)c->n, bsz), bsz, 0,
+                    c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, stp));
+  else if (c->P.av_vel)
+    SKF_TM_DISPATCH(k_xsph_m, 2, xcd_grid((long)c->n * 2, bsz), bsz, 0,
+                    c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, stp));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
@@ -3987,8 +4080,15 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
       }
   c->n = n;
   c->check_finite = p->check_finite;
+  for (int k = 0; k < 3; k++) c->gteam[k] = n >= TEAM1_MIN_N ? 1 : 2;
+  if (const char *e = getenv("GFSPH_GATHER_LANES")) {  // 1 or 2 lanes per particle (same results)
+    const int v = atoi(e);
+    if (v == 1 || v == 2)
+      for (int k = 0; k < 3; k++) c->gteam[k] = v;
+  }
   *out = c;
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->counts_h.assign(counts, counts + R);
   if ((rc = alloc_sets(c, n))) return rc;
   if ((c->P.nfric == 4 || c->staged) && (rc = alloc_bingham(c, n))) return rc;
   hipStream_t st = c->stream;
@@ -4477,31 +4577,37 @@ int gfsph_autotune(gfsph_ctx *c, int32_t reps, int32_t *chosen) {
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
-  int best[3];
+  int best[3], best_tm[3];
   for (int k = 0; k < 3; k++) {
     float tbest = 1e30f;
     best[k] = c->gbs[k];
-    for (int bsz : sizes) {
-      if (k == 2 && !c->P.av_vel) break;
-      c->gbs[k] = bsz;
-      for (int w = 0; w < 2; w++) {  // warm-up, then timed
-        HIPCHK(c, hipEventRecord(e0, c->stream));
-        for (int r = 0; r < reps; r++) {
-          if (k == 0) launch_density(c, s);
-          else if (k == 1) launch_pforce(c, s);
-          else launch_velocity(c, s);
+    best_tm[k] = c->gteam[k];
+    for (int tm = 2; tm >= 1; tm--) {  // lanes per particle: the same bits either way
+      for (int bsz : sizes) {
+        if (k == 2 && !c->P.av_vel) break;
+        c->gbs[k] = bsz;
+        c->gteam[k] = tm;
+        for (int w = 0; w < 2; w++) {  // warm-up, then timed
+          HIPCHK(c, hipEventRecord(e0, c->stream));
+          for (int r = 0; r < reps; r++) {
+            if (k == 0) launch_density(c, s);
+            else if (k == 1) launch_pforce(c, s);
+            else launch_velocity(c, s);
+          }
+          HIPCHK(c, hipEventRecord(e1, c->stream));
         }
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-      }
-      HIPCHK(c, hipEventSynchronize(e1));
-      float t;
-      HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
-      if (t < tbest) {
-        tbest = t;
-        best[k] = bsz;
+        HIPCHK(c, hipEventSynchronize(e1));
+        float t;
+        HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+        if (t < tbest) {
+          tbest = t;
+          best[k] = bsz;
+          best_tm[k] = tm;
+        }
       }
     }
     c->gbs[k] = best[k];
+    c->gteam[k] = best_tm[k];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

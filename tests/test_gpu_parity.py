@@ -109,6 +109,23 @@ def test_gpu_state_matches_reference(name):
                     assert err <= tol_for(s), f"{name} step {s} {k}: {err:.3e}"
 
 
+@pytest.mark.parametrize("name", ["g_frictional_pwp", "g_kernel_skf3", "g_k0", "g_pwp_column_cos"])
+def test_gpu_gather_lanes_give_the_same_bits(name, monkeypatch):
+    """The gathers' two halves of every candidate stream run on two lanes or
+    one after the other on one lane (GFSPH_GATHER_LANES, normally chosen by
+    size or by gfsph_autotune): the results are the same bits."""
+    g = _gpu()
+    case, _ = goldens.case_of(name)
+    out = []
+    for lanes in ("1", "2"):
+        monkeypatch.setenv("GFSPH_GATHER_LANES", lanes)
+        with g.SPHModel(case) as m:
+            m.time_integration_sph(30)
+            out.append(m.state())
+    for k in ("x", "v", "h", "u", "uaux", "hsml", "flag"):
+        np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
+
+
 @pytest.mark.parametrize("name", ["g_voellmy_plane", "g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1"])
 def test_gpu_rk2_matches_oracle(name):
     """RK2 (Heun) has no reference counterpart: GPU against the oracle."""
