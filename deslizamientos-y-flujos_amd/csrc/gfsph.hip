@@ -1706,7 +1706,7 @@ __device__ __forceinline__ void load_words(const CandStream &cs, const unsigned 
 // words w of a particle in stream order; js are slots.  Unused entries of a
 // batch have ok = false and js = self (a valid index), so f may read them
 // unconditionally.  Emptied words are shifted out at batch boundaries.
-template <typename F>
+template <bool HALVES = false, typename F>
 __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned long long (&w)[NB_WORDS], int self,
                                                F &&f) {
   int pbase = 0;
@@ -1723,14 +1723,20 @@ __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned lo
     }
     int js[NB_BATCH];
     bool ok[NB_BATCH];
+    int hs[NB_BATCH];  // half of the entry: parity of its 4-position nibble
 #pragma unroll
     for (int u = 0; u < NB_BATCH; u++) {
       ok[u] = w[0] != 0ull;
-      const int sl = slot_of(cs, pbase + __ffsll((long long)w[0]) - 1);
+      const int pos = pbase + __ffsll((long long)w[0]) - 1;
+      const int sl = slot_of(cs, pos);
       js[u] = ok[u] ? sl : self;
+      hs[u] = (pos >> 2) & 1;
       w[0] &= w[0] - 1ull;
     }
-    f(js, ok);
+    if constexpr (HALVES)
+      f(js, ok, hs);
+    else
+      f(js, ok);
   }
 }
 
@@ -1755,14 +1761,19 @@ __device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, 
 // neighbour); mask word k holds stream positions 64k .. 64k+63.  UNIFORM: all
 // smoothing lengths are equal, so the support radius scale_k*(hi+hj)/2 =
 // scale_k*hi and the kernel's 1/h are the particle's own.
-template <int SKF, bool UNIFORM>
-__device__ __forceinline__ double density_stream(const CandStream &cs, const DevParams &P, int i, int h,
-                                                 double2 pi_, double hi, const double4 *nb,
-                                                 unsigned long long (&wd)[NB_WORDS]) {
+// BOTH: one pass over every batch, each summed into its half (acc[0], acc[1]);
+// else the batches of half h only, into acc[0].  Either way each half is
+// summed in stream order.
+template <int SKF, bool UNIFORM, bool BOTH>
+__device__ __forceinline__ void density_stream(const CandStream &cs, const DevParams &P, int i, int h,
+                                               double2 pi_, double hi, const double4 *nb,
+                                               unsigned long long (&wd)[NB_WORDS], double (&acc)[2]) {
   const PredBand pu = pred_band(P.scale_k * hi);
   const double ihi = rcp_nr(hi);
-  double acc = 0.0;
-  for (int k0 = h * NB_BATCH; k0 < cs.total; k0 += SPLIT * NB_BATCH) {
+  acc[0] = 0.0;
+  acc[1] = 0.0;
+  for (int k0 = BOTH ? 0 : h * NB_BATCH; k0 < cs.total; k0 += (BOTH ? 1 : SPLIT) * NB_BATCH) {
+    const bool h1 = BOTH && ((k0 >> 2) & 1);
     int j[NB_BATCH];
     double4 q[NB_BATCH];
 #pragma unroll
@@ -1794,14 +1805,15 @@ __device__ __forceinline__ double density_stream(const CandStream &cs, const Dev
       nib |= nbr ? (1u << u) : 0u;
       double w, gx, gy;
       sph_kernel_t<SKF, UNIFORM>(P.kc, P.pi, dr[u], dx[u], dy[u], mh[u], w, gx, gy, ihi);
-      const double t = acc + q[u].w * w;
-      acc = nbr ? t : acc;
+      const double cur = h1 ? acc[1] : acc[0];
+      const double t = cur + q[u].w * w;
+      acc[0] = (nbr && !h1) ? t : acc[0];
+      acc[1] = (nbr && h1) ? t : acc[1];
     }
     const unsigned long long bits = (unsigned long long)nib << (k0 & 63);
 #pragma unroll
     for (int k = 0; k < NB_WORDS; k++) wd[k] |= (k == (k0 >> 6)) ? bits : 0ull;
   }
-  return acc;
 }
 
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
@@ -1841,10 +1853,13 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
     unsigned long long wd[NB_WORDS];
 #pragma unroll
     for (int k = 0; k < NB_WORDS; k++) wd[k] = 0ull;
+    double two[2];
+    if (g.uniform_h)
+      density_stream<SKF, true, TM == 1>(cs, P, i, t, pi_, hi, s.nb, wd, two);
+    else
+      density_stream<SKF, false, TM == 1>(cs, P, i, t, pi_, hi, s.nb, wd, two);
 #pragma unroll
-    for (int hh = 0; hh < SPLIT / TM; hh++)
-      a[hh] = g.uniform_h ? density_stream<SKF, true>(cs, P, i, t + hh * TM, pi_, hi, s.nb, wd)
-                          : density_stream<SKF, false>(cs, P, i, t + hh * TM, pi_, hi, s.nb, wd);
+    for (int hh = 0; hh < SPLIT / TM; hh++) a[hh] = two[hh];
     unsigned long long *mrow = mask + NB_WORDS * (long)i;
     const int nw = (cs.total + 63) >> 6;
 #pragma unroll
@@ -1904,8 +1919,51 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
   double ax[SPLIT / TM], ay[SPLIT / TM];
+  bool done = false;
+  if constexpr (TM == 1) {
+    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
+      double hx[2] = {0.0, 0.0}, hy[2] = {0.0, 0.0};
+      unsigned long long w[NB_WORDS];
+      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+      const double ihi = rcp_nr(hi);
+      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
+        constexpr bool UH = decltype(uh)::value;
+        double4 q[NB_BATCH];
+        double pj[NB_BATCH], sj[NB_BATCH];
 #pragma unroll
-  for (int hh = 0; hh < SPLIT / TM; hh++) {
+        for (int u = 0; u < NB_BATCH; u++) {
+          q[u] = s.nb[js[u]];
+          pj[u] = Pp[js[u]];
+          sj[u] = PA == 2 ? 0.0 : S[js[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < NB_BATCH; u++) {
+          double dx, dy, dr, mh, w, gx, gy;
+          pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+          sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
+          double tx = hs[u] ? hx[1] : hx[0], ty = hs[u] ? hy[1] : hy[0];
+          pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
+          hx[0] = (ok[u] && !hs[u]) ? tx : hx[0];
+          hy[0] = (ok[u] && !hs[u]) ? ty : hy[0];
+          hx[1] = (ok[u] && hs[u]) ? tx : hx[1];
+          hy[1] = (ok[u] && hs[u]) ? ty : hy[1];
+        }
+      };
+      if (g.uniform_h)
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
+      else
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
+      ax[0] = hx[0];
+      ay[0] = hy[0];
+      ax[1] = hx[1];
+      ay[1] = hy[1];
+      done = true;
+    }
+  }
+#pragma unroll
+  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
     const int h = t + hh * TM;
     double inx = 0.0, iny = 0.0;
   if (stream_full(cs)) {
@@ -1978,8 +2036,55 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
   double hx[SPLIT / TM], hy[SPLIT / TM];
+  bool done = false;
+  if constexpr (TM == 1) {
+    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
+      double sx[2] = {0.0, 0.0}, sy[2] = {0.0, 0.0};
+      unsigned long long w[NB_WORDS];
+      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+      const double ihi = rcp_nr(hi);
+      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
+        constexpr bool UH = decltype(uh)::value;
+        double4 q[NB_BATCH];
+        double2 vj[NB_BATCH];
+        double hj[NB_BATCH];
+        bool use[NB_BATCH];
 #pragma unroll
-  for (int hh = 0; hh < SPLIT / TM; hh++) {
+        for (int u = 0; u < NB_BATCH; u++) {
+          q[u] = s.nb[js[u]];
+          vj[u] = o.vel[js[u]];
+          hj[u] = o.h[js[u]];
+          use[u] = ok[u] && !(skip0 && s.id[js[u]] < P.npoin0);
+        }
+#pragma unroll
+        for (int u = 0; u < NB_BATCH; u++) {
+          double dx, dy, dr, mh, w, gx, gy;
+          pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+          sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
+          const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
+          const double cx = hs[u] ? sx[1] : sx[0], cy = hs[u] ? sy[1] : sy[0];
+          const double tx = cx - c * (vi.x - vj[u].x), ty = cy - c * (vi.y - vj[u].y);
+          sx[0] = (use[u] && !hs[u]) ? tx : sx[0];
+          sy[0] = (use[u] && !hs[u]) ? ty : sy[0];
+          sx[1] = (use[u] && hs[u]) ? tx : sx[1];
+          sy[1] = (use[u] && hs[u]) ? ty : sy[1];
+        }
+      };
+      if (g.uniform_h)
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
+      else
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
+      hx[0] = sx[0];
+      hy[0] = sy[0];
+      hx[1] = sx[1];
+      hy[1] = sy[1];
+      done = true;
+    }
+  }
+#pragma unroll
+  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
     const int h = t + hh * TM;
     double ax = 0.0, ay = 0.0;
   if (stream_full(cs)) {
