@@ -925,34 +925,53 @@ __device__ __forceinline__ void cell_of(const GridInfo &g, double2 p, int &cx, i
   cy = cy < 0 ? 0 : (cy >= g.ncy ? g.ncy - 1 : cy);
 }
 
+// Runs of consecutive lanes of a wave that share a key: the particles arrive
+// in the previous step's cell order, so the lanes of a wave mostly fall in
+// a few cells.  Every lane of the wave must call these together.
+struct WaveRun {
+  int head;  // first lane of this lane's run
+  int len;   // length of the run (valid in its head lane)
+};
+
+__device__ __forceinline__ WaveRun wave_run(long key, int lane) {
+  const long prev = __shfl_up(key, 1);
+  const bool is_head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(is_head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  WaveRun w;
+  w.head = 63 - __clzll((long long)(heads & upto));
+  const unsigned long long after = lane == 63 ? 0ull : (heads >> (lane + 1));
+  const int next_head = after ? lane + __ffsll((long long)after) : 64;
+  w.len = next_head - lane;
+  return w;
+}
+
+// Cell keys and counts: one atomic per run of equal keys in a wave, whose
+// lanes take consecutive ranks (the rank order inside a cell is fixed later
+// by k_reorder).  Lanes past n carry keys of their own.
 __global__ void k_keys(int n, const double2 *pos, const int *flag, const int *rid, const GridInfo *grid,
                        const int *coff, const int *ntot, int *key, int *rank, int *cnt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const bool out = flag[i] != 0;
-  int k;
-  if (out) {
-    k = *ntot;
-  } else {
-    const int r = rid[i];
-    const GridInfo g = grid[r];
-    int cx, cy;
-    cell_of(g, pos[i], cx, cy);
-    k = coff[r] + cy * g.ncx + cx;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)__lane_id();
+  long k = -1 - lane;
+  if (i < n) {
+    if (flag[i]) {
+      k = *ntot;
+    } else {
+      const int r = rid[i];
+      const GridInfo g = grid[r];
+      int cx, cy;
+      cell_of(g, pos[i], cx, cy);
+      k = coff[r] + cy * g.ncx + cx;
+    }
   }
-  key[i] = k;
-  // every out-of-domain particle shares one bucket: one atomic per wave
-  // for them (their order inside the bucket is free, they never interact)
-  const unsigned long long m = __ballot(out);
-  if (out) {
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&cnt[k], __popcll(m));
-    base = __shfl(base, leader);
-    rank[i] = base + __popcll(m & ((1ull << lane) - 1ull));
-  } else {
-    rank[i] = atomicAdd(&cnt[k], 1);
+  const WaveRun w = wave_run(k, lane);
+  int base = 0;
+  if (lane == w.head && i < n) base = atomicAdd(&cnt[k], w.len);
+  base = __shfl(base, w.head);
+  if (i < n) {
+    key[i] = (int)k;
+    rank[i] = base + (lane - w.head);
   }
 }
 
@@ -3359,27 +3378,52 @@ __device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
 
 // stride > 0: per-realisation accumulators, realisation r at offset r*stride nodes
 __global__ void k_mc_pixel(int n, DevDem dem, CurView s, long stride, double *acc4, int *cnt, const int *due) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || s.flag[i]) return;
-  if (due && !due[s.rid[i]]) return;
-  const long base = stride * s.rid[i];
-  double2 p = s.pos[i];
-  double2 v = s.vel[i];
-  int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
-  if (ipx == dem.nx && fabs(dem.xmax - p.x) <= 1.e-3) ipx--;
-  if (ipy == dem.ny && fabs(dem.ymax - p.y) <= 1.e-3) ipy--;
-  if (ipx < 1 || ipy < 1 || ipx > dem.nx - 1 || ipy > dem.ny - 1) return;
-  long g1 = base + (long)(ipy - 1) * dem.nx + ipx - 1;
-  long nodes[4] = {g1, g1 + 1, g1 + 1 + dem.nx, g1 + dem.nx};
-  double hh = s.h[i];
-  double vv = sqrt(v.x * v.x + v.y * v.y);  // **0.5: flang emits a square root
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)__lane_id();
+  // lanes with a cell take part; the others carry keys of their own
+  long key = -1 - lane, g1 = 0;
+  double hh = 0.0, vv = 0.0;
+  if (i < n && !s.flag[i] && !(due && !due[s.rid[i]])) {
+    const long base = stride * s.rid[i];
+    const double2 p = s.pos[i];
+    const double2 v = s.vel[i];
+    int ipx = (int)((p.x - dem.xmin) / dem.dx + 1), ipy = (int)((p.y - dem.ymin) / dem.dy + 1);
+    if (ipx == dem.nx && fabs(dem.xmax - p.x) <= 1.e-3) ipx--;
+    if (ipy == dem.ny && fabs(dem.ymax - p.y) <= 1.e-3) ipy--;
+    if (!(ipx < 1 || ipy < 1 || ipx > dem.nx - 1 || ipy > dem.ny - 1)) {
+      g1 = base + (long)(ipy - 1) * dem.nx + ipx - 1;
+      key = g1;
+      hh = s.h[i];
+      vv = sqrt(v.x * v.x + v.y * v.y);  // **0.5: flang emits a square root
+    }
+  }
+  // particles of a wave that share a cell (they arrive in cell order) are
+  // combined first: one set of atomics per run of equal cells
+  const WaveRun w = wave_run(key, lane);
+  const int end = lane + w.len;
+  double sh = hh, sv = vv, mh = hh, mv = vv;
+  int c = 1;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double th = __shfl_down(sh, o), tv = __shfl_down(sv, o), xh = __shfl_down(mh, o), xv = __shfl_down(mv, o);
+    const int tc = __shfl_down(c, o);
+    if (lane + o < end) {
+      sh += th;
+      sv += tv;
+      mh = xh > mh ? xh : mh;
+      mv = xv > mv ? xv : mv;
+      c += tc;
+    }
+  }
+  if (lane != w.head || key < 0) return;
+  const long nodes[4] = {g1, g1 + 1, g1 + 1 + dem.nx, g1 + dem.nx};
   for (int k = 0; k < 4; k++) {
-    long g = nodes[k];
-    atomicAdd(&cnt[g], 1);
-    unsafeAtomicAdd(&acc4[4 * g + 0], hh);  // hardware FP64 add
-    unsafeAtomicAdd(&acc4[4 * g + 2], vv);
-    atomic_max_pos(&acc4[4 * g + 1], hh);
-    atomic_max_pos(&acc4[4 * g + 3], vv);
+    const long g = nodes[k];
+    atomicAdd(&cnt[g], c);
+    unsafeAtomicAdd(&acc4[4 * g + 0], sh);  // hardware FP64 add
+    unsafeAtomicAdd(&acc4[4 * g + 2], sv);
+    atomic_max_pos(&acc4[4 * g + 1], mh);
+    atomic_max_pos(&acc4[4 * g + 3], mv);
   }
 }
 
