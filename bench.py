@@ -199,11 +199,17 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
     print(f"# MC: {R} realisations, {n} particles, {nsteps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b = BatchModel(cs)
-    b.time_integration_sph(1)
-    blocks = b.autotune(reps=2)
-    done = 1
+    # the work-group sizes and lanes per particle are tuned once the flows
+    # have developed (the first steps, from rest, favour other shapes)
+    tune_at = min(100, nsteps)
+    blocks = None
+    done = 0
     while done < nsteps:
+        if done == tune_at and blocks is None:
+            blocks = b.autotune(reps=2)
         k = min(plot_every - done % plot_every, nsteps - done)
+        if done < tune_at:
+            k = min(k, tune_at - done)
         b.time_integration_sph(k)
         done += k
         if done % plot_every == 0:
