@@ -86,8 +86,13 @@ struct GridInfo {
 
 // Cells are half the largest support radius, so a particle's candidates lie
 // in the 5 x 5 cells around its own (rows cy-2 .. cy+2).
-#define CELL_FRAC 0.5
+#ifndef NROWS
 #define NROWS 5
+#endif
+#define RAD ((NROWS - 1) / 2)
+#define CELL_FRAC (1.0 / RAD)
+// cells per particle the grids may use (the same area per particle for any RAD)
+#define CAP_PER_PART (4 * RAD * RAD)
 
 // bounding-box accumulator of active particles (ordered-integer encoding)
 struct BBoxAcc {
@@ -907,7 +912,7 @@ __global__ void k_bbox_chunks(const int2 *rng, const double2 *pos, const double 
   }
 }
 
-// grid of every realisation from its partials, capped at 16 cells per
+// grid of every realisation from its partials, capped at CAP_PER_PART cells per
 // particle of its slot range + 64
 __global__ void k_grid_real(int R, int nchunk, const int2 *rng, const BBoxAcc *part, int scale_k, GridInfo *grid) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -915,7 +920,7 @@ __global__ void k_grid_real(int R, int nchunk, const int2 *rng, const BBoxAcc *p
   const int2 rg = rng[r];
   BBoxAcc a = bbox_identity();
   for (int ch = 0; ch < nchunk; ch++) bbox_fold(a, part[(long)r * nchunk + ch]);
-  grid[r] = make_grid(a, scale_k, 16 * (rg.y - rg.x) + 64);
+  grid[r] = make_grid(a, scale_k, CAP_PER_PART * (rg.y - rg.x) + 64);
 }
 
 __device__ __forceinline__ void cell_of(const GridInfo &g, double2 p, int &cx, int &cy) {
@@ -1121,8 +1126,8 @@ template <typename F>
 __device__ __forceinline__ void for_candidates(const GridInfo &g, const int *start, double2 p, F &&f) {
   int cx, cy;
   cell_of(g, p, cx, cy);
-  int c0 = cx > 1 ? cx - 2 : 0, c1 = cx < g.ncx - 2 ? cx + 2 : g.ncx - 1;
-  int r0 = cy > 1 ? cy - 2 : 0, r1 = cy < g.ncy - 2 ? cy + 2 : g.ncy - 1;
+  int c0 = cx >= RAD ? cx - RAD : 0, c1 = cx < g.ncx - RAD ? cx + RAD : g.ncx - 1;
+  int r0 = cy >= RAD ? cy - RAD : 0, r1 = cy < g.ncy - RAD ? cy + RAD : g.ncy - 1;
   for (int r = r0; r <= r1; r++) {
     int b = start[r * g.ncx + c0], e = start[r * g.ncx + c1 + 1];
     for (int j = b; j < e; j++) f(j);
@@ -1322,8 +1327,11 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 // served one realisation at a time (normally one pass), so every parameter
 // read is a scalar load.  part = nullptr: no bounding-box partials (several
 // realisations: k_bbox_chunks computes their boxes).
+#ifndef RK_WAVES_PER_SIMD
+#define RK_WAVES_PER_SIMD 1
+#endif
 template <bool EROS>
-__global__ __launch_bounds__(GF_BLOCK) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
+__global__ __launch_bounds__(GF_BLOCK, RK_WAVES_PER_SIMD) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
                                                   const double *S, const double2 *ain, NewSet o, BBoxAcc *part,
                                                   const int *stopr, ErosGrid eg) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -1640,13 +1648,13 @@ __device__ __forceinline__ CandStream cand_stream(const GridInfo &g, const int *
   cell_of(g, p, cx, cy);
   const double T = scale_k * (hi + g.hmax) * 0.5 * (1.0 + 1.0e-9);
   const double T2 = T * T;
-  const int cmin = cx > 1 ? cx - 2 : 0, cmax = cx < g.ncx - 2 ? cx + 2 : g.ncx - 1;
+  const int cmin = cx >= RAD ? cx - RAD : 0, cmax = cx < g.ncx - RAD ? cx + RAD : g.ncx - 1;
   CandStream cs;
   int tot = 0, dprev = 0;
 #pragma unroll
   for (int rr = 0; rr < NROWS; rr++) {
     int bb = 0, cnt = 0;
-    const int r = cy - 2 + rr;
+    const int r = cy - RAD + rr;
     if (r >= 0 && r < g.ncy) {
       const double y0 = g.ymin + r * g.cell, y1 = y0 + g.cell;
       const double dy = p.y < y0 ? y0 - p.y : (p.y > y1 ? p.y - y1 : 0.0);
@@ -3825,9 +3833,9 @@ static int alloc_sets(gfsph_ctx *c, int n) {
   if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
       (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
     return rc;
-  // a single run caps its grid at 16n + 4096 cells, realisation r of a batch
-  // at 16 n_r + 64
-  c->cap_cells = 16 * n + 64 * c->R + 4096;
+  // a single run caps its grid at CAP_PER_PART n + 4096 cells, realisation r of a batch
+  // at CAP_PER_PART n_r + 64
+  c->cap_cells = CAP_PER_PART * n + 64 * c->R + 4096;
   c->scan_blocks = (c->cap_cells + 2 + SCAN_TILE - 1) / SCAN_TILE;
   if ((rc = dalloc(c, &c->cnt, (size_t)c->cap_cells + 2)) || (rc = dalloc(c, &c->start, (size_t)c->cap_cells + 2)) ||
       (rc = dalloc(c, &c->bsum, (size_t)c->scan_blocks)) || (rc = dalloc(c, &c->grid, c->R)) ||
