@@ -1682,6 +1682,39 @@ __device__ __forceinline__ CandStream cand_stream(const GridInfo &g, const int *
 // a stream longer than the mask holds: the passes re-test the predicate
 __device__ __forceinline__ bool stream_full(const CandStream &cs) { return cs.total > 64 * NB_WORDS; }
 
+// The density pass stores each particle's stream (32 bytes: the row
+// increments and the 16-bit row offsets and length) for the force and XSPH
+// passes of the step; a full stream stores a marker and is recomputed.
+static_assert(NROWS == 5, "the stream record holds 5 rows");
+#define STREAM_FULL_MARK 0xffffu
+
+__device__ __forceinline__ void store_stream(int4 *rec, int i, const CandStream &cs) {
+  const bool full = stream_full(cs);
+  const unsigned o12 = (unsigned)cs.off[1] | ((unsigned)cs.off[2] << 16);
+  const unsigned o34 = (unsigned)cs.off[3] | ((unsigned)cs.off[4] << 16);
+  const unsigned tot = full ? STREAM_FULL_MARK : (unsigned)cs.total;
+  rec[2 * i] = make_int4(cs.inc[0], cs.inc[1], cs.inc[2], cs.inc[3]);
+  rec[2 * i + 1] = make_int4(cs.inc[4], (int)o12, (int)o34, (int)tot);
+}
+
+// false: a full stream (the caller recomputes it)
+__device__ __forceinline__ bool load_stream(const int4 *rec, int i, CandStream &cs) {
+  const int4 a = rec[2 * i], b = rec[2 * i + 1];
+  if ((unsigned)b.w == STREAM_FULL_MARK) return false;
+  cs.inc[0] = a.x;
+  cs.inc[1] = a.y;
+  cs.inc[2] = a.z;
+  cs.inc[3] = a.w;
+  cs.inc[4] = b.x;
+  cs.off[0] = 0;
+  cs.off[1] = (int)((unsigned)b.y & 0xffffu);
+  cs.off[2] = (int)((unsigned)b.y >> 16);
+  cs.off[3] = (int)((unsigned)b.z & 0xffffu);
+  cs.off[4] = (int)((unsigned)b.z >> 16);
+  cs.total = b.w;
+  return true;
+}
+
 // Slot of stream position p: p + delta of the last row with off[rr] <= p
 // (an empty row shares its off with the next one, so its increment is
 // cancelled by the next row's).  Summed as masked increments so that it
@@ -1846,7 +1879,8 @@ __device__ __forceinline__ void density_stream(const CandStream &cs, const DevPa
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
 template <int SKF, int TM>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
-                                                                 double *Pout, unsigned long long *mask) {
+                                                                 double *Pout, unsigned long long *mask,
+                                                                 int4 *srec) {
   int i, t;
   team_index<TM>(i, t);
   if (i >= n) return;
@@ -1863,6 +1897,7 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
   const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
+  if (t == 0) store_stream(srec, i, cs);
   double a[SPLIT / TM];
   if (stream_full(cs)) {
 #pragma unroll
@@ -1928,7 +1963,8 @@ __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, do
 template <int SKF, int PA, int TM>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl,
                                                                 const unsigned long long *mask, SortedSet s,
-                                                                const double *S, const double *Pp, double2 *ain) {
+                                                                const double *S, const double *Pp, double2 *ain,
+                                                                const int4 *srec) {
   int i, t;
   team_index<TM>(i, t);
   if (i >= n) return;
@@ -1941,7 +1977,8 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
   const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
-  const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
+  CandStream cs;
+  if (!load_stream(srec, i, cs)) cs = cand_stream(g, start, pi_, hi, P.scale_k);
   const double Si = S[i];
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
@@ -2045,7 +2082,7 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
 template <int SKF, int TM>
 __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl,
                                                               const unsigned long long *mask, SortedSet s, NewSet o,
-                                                              double2 *vout, const int *stopr) {
+                                                              double2 *vout, const int *stopr, const int4 *srec) {
   int i, t;
   team_index<TM>(i, t);
   if (i >= n) return;
@@ -2059,7 +2096,8 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P,
   const int *start = cl.start + cl.coff[rr_];
   const double2 pi_ = s.pos[i];
   const double hi = s.hsml[i];
-  const CandStream cs = cand_stream(g, start, pi_, hi, P.scale_k);
+  CandStream cs;
+  if (!load_stream(srec, i, cs)) cs = cand_stream(g, start, pi_, hi, P.scale_k);
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
   double hx[SPLIT / TM], hy[SPLIT / TM];
@@ -3494,6 +3532,7 @@ struct gfsph_ctx {
   NewSet nw{};
   double *S = nullptr, *Pp = nullptr;
   unsigned long long *mask = nullptr;
+  int4 *srec = nullptr;  // the step's candidate-stream records [n][2]
   double2 *ain = nullptr;  // pressure force per particle
   int *key = nullptr, *rank = nullptr, *perm = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
   int cap_cells = 0, scan_blocks = 0;
@@ -3830,6 +3869,7 @@ static int alloc_sets(gfsph_ctx *c, int n) {
       HIPCHK(c, hipMemsetAsync(q, 0, sizeof(double) * nc, c->stream));
   }
   if ((rc = dalloc(c, &c->mask, NB_WORDS * (size_t)n)) || (rc = dalloc(c, &c->ain, n))) return rc;
+  if ((rc = dalloc(c, &c->srec, 2 * (size_t)n))) return rc;
   if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
       (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
     return rc;
@@ -3909,7 +3949,7 @@ static inline int xcd_grid(long n, int bsz) { return (int)((((n + bsz - 1) / bsz
 
 static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[1], tm = c->gteam[1];
-#define PFORCE_ARGS c->n, c->P, cells_of(c), c->mask, s, c->S, c->Pp, c->ain
+#define PFORCE_ARGS c->n, c->P, cells_of(c), c->mask, s, c->S, c->Pp, c->ain, c->srec
   switch (c->P.skf) {
     case 1: PFORCE_CASE(1) break;
     case 2: PFORCE_CASE(2) break;
@@ -4026,10 +4066,10 @@ static void launch_density(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[0];
   if (c->gteam[0] == 1)
     SKF_TM_DISPATCH(k_density_m, 1, xcd_grid((long)c->n, bsz), bsz, 0,
-                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
+                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask, c->srec));
   else
     SKF_TM_DISPATCH(k_density_m, 2, xcd_grid((long)c->n * 2, bsz), bsz, 0,
-                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask));
+                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask, c->srec));
 }
 
 // XSPH (or the plain copy of the post-RK velocity) into the sorted set
@@ -4038,10 +4078,10 @@ static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
   const int *stp = c->cases ? c->stopr : nullptr;
   if (c->P.av_vel && c->gteam[2] == 1)
     SKF_TM_DISPATCH(k_xsph_m, 1, xcd_grid((long)c->n, bsz), bsz, 0,
-                    c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, stp));
+                    c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, stp, c->srec));
   else if (c->P.av_vel)
     SKF_TM_DISPATCH(k_xsph_m, 2, xcd_grid((long)c->n * 2, bsz), bsz, 0,
-                    c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, stp));
+                    c->stream>>>(c->n, c->P, cells_of(c), c->mask, s, c->nw, s.vel, stp, c->srec));
   else
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
