@@ -81,7 +81,7 @@ def main() -> int:
     ap.add_argument("--no-mc", action="store_true", help="skip the batched Monte Carlo measurement")
     ap.add_argument("--mc-real", type=int, default=256)
     ap.add_argument("--mc-steps", type=int, default=3000)
-    ap.add_argument("--mc-seq-sample", type=int, default=8,
+    ap.add_argument("--mc-seq-sample", type=int, default=16,
                     help="realisations timed one by one to estimate the sequential cost")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--adj-steps", type=int, default=1000)
@@ -190,8 +190,11 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
     LogNormal(ln 400, 0.5), RK2 dt 0.05 s, pixel statistics every
     ``plot_every`` steps, then the hazard rasters): the whole batch is timed
     from context creation to the rasters on the host; the sequential cost is
-    the mean of ``nseq`` randomly chosen realisations run one by one through
-    the plain drop-in (same steps and pixel accumulation) times R."""
+    estimated from ``nseq`` randomly chosen realisations run one by one
+    through the plain drop-in (same steps and pixel accumulation): a least-
+    squares line of run time against particle count over the sample, summed
+    over the R particle counts (the sample mean times R is reported beside
+    it)."""
     from gfsph import BatchModel, SPHModel, cases
 
     cs = cases.cfg_mc_hazard(R=R, nsteps=nsteps)
@@ -219,8 +222,9 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
     b.close()
     print(f"# MC batch {t_batch:.2f}s (blocks {blocks})", file=sys.stderr, flush=True)
     rng = np.random.default_rng(1)
-    ts = []
+    ts, ns = [], []
     for r in rng.choice(R, size=min(nseq, R), replace=False):
+        ns.append(cs[r].particles.n)
         t1 = time.perf_counter()
         m = SPHModel(cs[r])
         for _ in range(nsteps // plot_every):
@@ -230,12 +234,19 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
         m.mc_pixel_read()
         m.close()
         ts.append(time.perf_counter() - t1)
-    t_seq = float(np.mean(ts)) * R
-    print(f"# MC sequential estimate {t_seq:.2f}s from {len(ts)} runs", file=sys.stderr, flush=True)
+    t_mean = float(np.mean(ts)) * R
+    if len(ts) >= 3 and len(set(ns)) > 1:
+        b1, b0 = np.polyfit(np.asarray(ns, float), np.asarray(ts), 1)
+        t_seq = float(R * b0 + b1 * n)
+    else:
+        t_seq = t_mean
+    print(f"# MC sequential estimate {t_seq:.2f}s (sample mean x R: {t_mean:.2f}s) from {len(ts)} runs",
+          file=sys.stderr, flush=True)
     return {"metric": "MC realisations/hr", "value": round(R * 3600.0 / t_batch, 1), "unit": "realisations/hr",
             "realisations": R, "steps": nsteps, "particles": n, "batch_s": round(t_batch, 3),
             "particle_steps_per_s": round(n * nsteps / t_batch, 1),
-            "sequential_s_estimate": round(t_seq, 3), "sequential_sample": len(ts),
+            "sequential_s_estimate": round(t_seq, 3), "sequential_s_mean_estimate": round(t_mean, 3),
+            "sequential_sample": len(ts),
             "speedup_vs_sequential": round(t_seq / t_batch, 2), "autotuned_blocks": list(blocks),
             "max_h": round(float(hz["max_h"].max()), 3), "exceed_nodes": int((hz["p_exceed"] > 0).sum())}
 

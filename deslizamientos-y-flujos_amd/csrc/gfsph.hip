@@ -1584,6 +1584,22 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 #define NB_BATCH 4
 #endif
 #define GATHER_MAX_BLOCK 512
+// minimum waves per SIMD of each gather (register budget), if set
+#ifdef DENSITY_WAVES
+#define DENSITY_LB __launch_bounds__(GATHER_MAX_BLOCK, DENSITY_WAVES)
+#else
+#define DENSITY_LB __launch_bounds__(GATHER_MAX_BLOCK)
+#endif
+#ifdef PFORCE_WAVES
+#define PFORCE_LB __launch_bounds__(GATHER_MAX_BLOCK, PFORCE_WAVES)
+#else
+#define PFORCE_LB __launch_bounds__(GATHER_MAX_BLOCK)
+#endif
+#ifdef XSPH_WAVES
+#define XSPH_LB __launch_bounds__(GATHER_MAX_BLOCK, XSPH_WAVES)
+#else
+#define XSPH_LB __launch_bounds__(GATHER_MAX_BLOCK)
+#endif
 
 // Every gather splits a particle's candidate stream into two halves: the
 // 4-position batches 0, 2, 4, ... and 1, 3, 5, ... (the force and XSPH passes:
@@ -1878,7 +1894,7 @@ __device__ __forceinline__ void density_stream(const CandStream &cs, const DevPa
 
 // a4 density with the neighbour-mask build (pair predicate of 5a:1010-1019)
 template <int SKF, int TM>
-__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
+__global__ DENSITY_LB void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
                                                                  double *Pout, unsigned long long *mask,
                                                                  int4 *srec) {
   int i, t;
@@ -1961,7 +1977,7 @@ __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, do
 
 // a5 pressure force (IntForces_SW pair sum, 6:6892-6941) -> ain[i]
 template <int SKF, int PA, int TM>
-__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams P, Cells cl,
+__global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
                                                                 const unsigned long long *mask, SortedSet s,
                                                                 const double *S, const double *Pp, double2 *ain,
                                                                 const int4 *srec) {
@@ -2080,7 +2096,7 @@ __global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_pforce_m(int n, DevParams 
 
 // a11 XSPH (av_vel, 7:3042-3099) with the frozen step-start geometry
 template <int SKF, int TM>
-__global__ __launch_bounds__(GATHER_MAX_BLOCK) void k_xsph_m(int n, DevParams P, Cells cl,
+__global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
                                                               const unsigned long long *mask, SortedSet s, NewSet o,
                                                               double2 *vout, const int *stopr, const int4 *srec) {
   int i, t;
