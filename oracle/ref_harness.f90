@@ -26,6 +26,8 @@
 !     real64 (Uaux(1:nUaux,ip), ip = 1, npoin)
 !  and, when erosion is on (ic_erosion = 1), eros_<step>.bin:
 !     real64 (topol(1,ip), topol(14,ip), topol(15,ip), ip = 1, npoig)
+!  If gid_final.ctl exists in the run directory, the reference's GiD result
+!  writer (OutputRes_SW) also runs once after the last step.
 !  It is built only by oracle/build_ref.sh and never ships with the product.
 !  Built with -DGFSPH_GPU it calls time_integration_sph_gfsph (module
 !  SPH_MAIN_GFSPH of the product's Fortran layer) instead, i.e. the
@@ -39,6 +41,7 @@ use SPH_FEM_Driver_time_vars_2019
 use SPH_MAIN_2019, only: Init_SPH, time_integration_sph, adaptive_dt_sph
 use SPH_Main_Vars_2019
 use SPH_SW_Vars_2019
+use SPH_SW_2019, only: OutputRes_SW
 use SPH_FEM_topo_2019, only: npoigx, npoigy, Xming, Xmaxg, Yming, Ymaxg, deltxg, deltyg, &
                              xshiftg, yshiftg, topol, poig_in_model
 #ifdef GFSPH_GPU
@@ -49,6 +52,7 @@ implicit none
 
 character(16)  name
 integer        nsteps, ndump, istep, k, ictl, iadapt
+logical        gid_final
 integer, allocatable :: dump_steps(:)
 real(8)        dtin
 
@@ -121,6 +125,10 @@ do istep = 1, nsteps
    time     = time + dt
    call maybe_dump (istep)
 enddo
+!      ------  with gid_final.ctl present: the GiD result blocks of the final
+!              state (OutputRes_SW, appended to <name>.post.res)
+inquire (file='gid_final.ctl', exist=gid_final)
+if (gid_final) call OutputRes_SW
 #ifdef GFSPH_GPU
 call gfsph_release
 #endif

@@ -246,7 +246,7 @@ def run_reference_files(src_dir: str, name: str, files, nsteps: int, dt: float, 
 
 
 def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, timeout: float = 3600.0,
-                  keep: bool = False, exe: str = REF_EXE) -> dict:
+                  keep: bool = False, exe: str = REF_EXE, gid_final: bool = False) -> dict:
     """Run the reference on ``case`` for ``nsteps`` RK4 steps; return
     {step: dump}.  Raises if the reference build is absent."""
     if not os.path.exists(exe):
@@ -260,6 +260,8 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
         f.write("case\n")
         f.write(f"{nsteps} {_r(case.params.dt)} {int(getattr(case.params, 'adapt_dt', 0))}\n")
         f.write(f"{len(steps)} " + " ".join(str(s) for s in steps) + "\n")
+    if gid_final:  # the reference's GiD writer also runs after the last step
+        open(os.path.join(tmp, "gid_final.ctl"), "w").close()
     r = subprocess.run([exe], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
                        timeout=timeout)
     if r.returncode != 0:
