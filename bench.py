@@ -4,14 +4,18 @@ Workload (the 200k-particle drop-in configuration): fractal DEM 2048 x 2048
 at 2 m (diamond-square, H 0.8, 400 m relief, carved 25-35 deg valley);
 elliptical release of 200,000 particles, thickness ~ U(2, 8) m; frictional
 phi = 28 deg with pore pressure (initial ratio 0.6, Cv 0.01 m2/s); RK2,
-dt = 0.02 s; FP64.  Synthetic data generated from fixed seeds.
+dt = 0.02 s, 5,000 steps; FP64.  Synthetic data generated from fixed seeds.
 
 A "step" is one time_integration_sph call over all particles (neighbour
 build, density, pressure force, RK stages with rheology and pore pressure,
-XSPH).  The timed region is the whole configured run -- 5,000 steps from
-t = 0 on a fresh context whose inputs are already resident in HBM -- because
-the cost of a step changes as the slide evolves (the flow converges in the
-valley and neighbour counts grow).  Warmup steps run on a separate context.
+XSPH).  The cost of a step changes as the slide evolves (the flow converges
+in the valley and neighbour counts grow), so the timed window is taken from
+the developed flow: a fresh context whose inputs are resident in HBM is
+advanced --develop steps (default: half the configured run) untimed, then
+--warmup steps, then exactly --steps steps are timed.  The whole configured
+run (5,000 steps from t = 0) is timed as well and reported under
+"full_run".  Per-kernel device times ("kernel_ms", the roofline) are taken
+with HIP events over the next --steps steps of the same context.
 
 Prints one JSON line (see the metric contract in the repository docs).
 """
@@ -54,21 +58,40 @@ B_ALG = {"neighbour_build": 64.0, "k_density": 44.0, "k_pforce": 56.0, "k_rk": 1
 MEASURED_TRAFFIC_200K = {"k_density": 32.22e6, "k_pforce": 33.21e6, "k_xsph": 38.48e6, "k_rk": 35.75e6}
 
 
-def cpu_model() -> str:
+def cpu_info() -> dict:
+    """The host CPU as the OS reports it: the full model string, the vendor,
+    family/model/stepping, the logical CPUs online (nproc) and those this
+    process may run on."""
+    info = {"model": platform.processor() or "unknown"}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and "model_name" not in info:
+                info["model"] = v
+                info["model_name"] = v
+            elif k in ("vendor_id", "cpu family", "model", "stepping", "cpu MHz") and k.replace(" ", "_") not in info:
+                info[k.replace(" ", "_")] = v
     except OSError:
         pass
-    return platform.processor() or "unknown"
+    info.pop("model_name", None)
+    info["nproc"] = os.cpu_count()
+    try:
+        info["nproc_affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return info
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--run-steps", type=int, default=5000, help="steps of the configured run")
+    ap.add_argument("--develop", type=int, default=-1,
+                    help="untimed steps before the timed window (default: half of --run-steps)")
+    ap.add_argument("--no-full-run", action="store_true", help="skip timing the whole configured run")
     ap.add_argument("--profile-steps", type=int, default=0, help="steps of the event-timed run (0: --steps)")
     ap.add_argument("--traffic", type=float, default=None,
                     help="measured HBM bytes per launch of the dominant kernel (from rocprofv3 --pmc)")
@@ -81,8 +104,8 @@ def main() -> int:
     ap.add_argument("--no-mc", action="store_true", help="skip the batched Monte Carlo measurement")
     ap.add_argument("--mc-real", type=int, default=256)
     ap.add_argument("--mc-steps", type=int, default=3000)
-    ap.add_argument("--mc-seq-sample", type=int, default=16,
-                    help="realisations timed one by one to estimate the sequential cost")
+    ap.add_argument("--mc-seq-sample", type=int, default=0,
+                    help="realisations timed one by one for the sequential cost (0: all of them)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--adj-steps", type=int, default=1000)
     args = ap.parse_args()
@@ -97,33 +120,50 @@ def main() -> int:
         return calibrate_copy()
 
     t0 = time.time()
-    case = cases.cfg_fractal_frictional(n=args.n, integrator=args.integrator, nsteps=args.steps)
+    case = cases.cfg_fractal_frictional(n=args.n, integrator=args.integrator, nsteps=args.run_steps)
     print(f"# workload generated in {time.time() - t0:.1f}s: n={case.particles.n} dem={case.dem.nx}x{case.dem.ny}",
           file=sys.stderr, flush=True)
-    if args.warmup > 0:
-        with SPHModel(case) as w:
-            w.time_integration_sph(args.warmup)
+    n = case.particles.n
+    develop = args.run_steps // 2 if args.develop < 0 else args.develop
+
+    # the whole configured run from t = 0 on a fresh context
+    full = None
+    if not args.no_full_run:
+        with SPHModel(case) as w:  # first-use costs (graph capture, code load) off the clock
+            w.time_integration_sph(16)
             w.sync()
+        with SPHModel(case) as mf:
+            mf.sync()
+            t1 = time.perf_counter()
+            mf.time_integration_sph(args.run_steps)
+            mf.sync()
+            tf = time.perf_counter() - t1
+        full = {"steps": args.run_steps, "s": round(tf, 4), "ms_per_step": round(1e3 * tf / args.run_steps, 5),
+                "particle_steps_per_s": round(n * args.run_steps / tf, 1)}
+        print(f"# full run: {args.run_steps} steps in {tf:.3f}s", file=sys.stderr, flush=True)
+
+    # the timed window: develop, warm up, then exactly --steps steps
     m = SPHModel(case)
+    m.time_integration_sph(develop + args.warmup)
     m.sync()
     t1 = time.perf_counter()
     m.time_integration_sph(args.steps)
     m.sync()
     t2 = time.perf_counter()
     elapsed = t2 - t1
-    n = case.particles.n
     value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
+    print(f"# timed steps {develop + args.warmup + 1}..{develop + args.warmup + args.steps} in {elapsed:.4f}s",
+          file=sys.stderr, flush=True)
+    window_state = None if args.no_cpu_baseline else (m.state(), m.mass())
+
+    # per-kernel device time: the next --steps steps of the same context with
+    # HIP events around every kernel group on the library's stream
+    prof_steps = args.steps if args.profile_steps <= 0 else args.profile_steps
+    prof = m.profile_step(nrep=prof_steps)
     st = m.state()
     n_active = int((st["flag"] == 0).sum())
     m.close()
-    print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
-
-    # per-kernel device time: the same run again with HIP events around every
-    # kernel group on the library's stream, averaged over all its steps
-    prof_steps = args.steps if args.profile_steps <= 0 else args.profile_steps
-    with SPHModel(case) as mp:
-        prof = mp.profile_step(nrep=prof_steps)
     kern = {k: v for k, v in prof.items() if k in B_ALG and k != "neighbour_build"}
     dom = max(kern, key=kern.get)
     ms = kern[dom]
@@ -136,11 +176,12 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": B_ALG[dom] * n,
                 "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                "window": f"steps {develop + args.warmup + args.steps + 1}..{develop + args.warmup + args.steps + prof_steps}",
                 "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(case, args.cpu_sample_steps)
+        cpu = cpu_baseline(case, window_state, develop + args.warmup, args.cpu_sample_steps)
     mc = None if args.no_mc else mc_hazard(args.mc_real, args.mc_steps, args.mc_seq_sample)
     adj = None if args.no_adjoint else adjoint_cost(args.adj_steps)
 
@@ -157,9 +198,12 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (fractal DEM seed 1002, release seed 2002)",
-        "config": {"workload": "fractal DEM 2048x2048 @2m, 200k-particle elliptical release, frictional 28deg + "
-                               "pore pressure, RK2 dt=0.02 s", "particles": n, "active_after": n_active,
+        "config": {"workload": f"fractal DEM 2048x2048 @2m, 200k-particle elliptical release, frictional 28deg + "
+                               f"pore pressure, RK2 dt=0.02 s, {args.run_steps} steps",
+                   "particles": n, "active_after": n_active,
+                   "timed_window": f"steps {develop + args.warmup + 1}..{develop + args.warmup + args.steps}",
                    "integrator": "RK2" if args.integrator == 2 else "RK4", "parallelism": "single GPU"},
+        "full_run": full,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mc": mc,
@@ -190,11 +234,11 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
     LogNormal(ln 400, 0.5), RK2 dt 0.05 s, pixel statistics every
     ``plot_every`` steps, then the hazard rasters): the whole batch is timed
     from context creation to the rasters on the host; the sequential cost is
-    estimated from ``nseq`` randomly chosen realisations run one by one
-    through the plain drop-in (same steps and pixel accumulation): a least-
-    squares line of run time against particle count over the sample, summed
-    over the R particle counts (the sample mean times R is reported beside
-    it)."""
+    the time of all R realisations run one by one through the plain drop-in
+    (same steps and pixel accumulation), each from context creation to its
+    pixel accumulators on the host (with 0 < ``nseq`` < R only a sample is
+    run and the total is extrapolated by a least-squares line of run time
+    against particle count; the JSON says which)."""
     from gfsph import BatchModel, SPHModel, cases
 
     cs = cases.cfg_mc_hazard(R=R, nsteps=nsteps)
@@ -221,9 +265,12 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
     t_batch = time.perf_counter() - t0
     b.close()
     print(f"# MC batch {t_batch:.2f}s (blocks {blocks})", file=sys.stderr, flush=True)
-    rng = np.random.default_rng(1)
+    if nseq <= 0 or nseq >= R:
+        picks = list(range(R))
+    else:
+        picks = sorted(np.random.default_rng(1).choice(R, size=nseq, replace=False).tolist())
     ts, ns = [], []
-    for r in rng.choice(R, size=min(nseq, R), replace=False):
+    for r in picks:
         ns.append(cs[r].particles.n)
         t1 = time.perf_counter()
         m = SPHModel(cs[r])
@@ -234,19 +281,23 @@ def mc_hazard(R, nsteps, nseq, plot_every=20):
         m.mc_pixel_read()
         m.close()
         ts.append(time.perf_counter() - t1)
-    t_mean = float(np.mean(ts)) * R
-    if len(ts) >= 3 and len(set(ns)) > 1:
+        if len(ts) % 32 == 0:
+            print(f"# MC sequential: {len(ts)} of {len(picks)} runs, {sum(ts):.1f}s", file=sys.stderr, flush=True)
+    measured = len(picks) == R
+    if measured:
+        t_seq = float(np.sum(ts))
+    elif len(ts) >= 3 and len(set(ns)) > 1:
         b1, b0 = np.polyfit(np.asarray(ns, float), np.asarray(ts), 1)
         t_seq = float(R * b0 + b1 * n)
     else:
-        t_seq = t_mean
-    print(f"# MC sequential estimate {t_seq:.2f}s (sample mean x R: {t_mean:.2f}s) from {len(ts)} runs",
+        t_seq = float(np.mean(ts)) * R
+    print(f"# MC sequential {'measured' if measured else 'extrapolated'} {t_seq:.2f}s from {len(ts)} runs",
           file=sys.stderr, flush=True)
     return {"metric": "MC realisations/hr", "value": round(R * 3600.0 / t_batch, 1), "unit": "realisations/hr",
             "realisations": R, "steps": nsteps, "particles": n, "batch_s": round(t_batch, 3),
             "particle_steps_per_s": round(n * nsteps / t_batch, 1),
-            "sequential_s_estimate": round(t_seq, 3), "sequential_s_mean_estimate": round(t_mean, 3),
-            "sequential_sample": len(ts),
+            "sequential_s": round(t_seq, 3), "sequential_kind": "measured" if measured else "extrapolated",
+            "sequential_runs": len(ts), "sequential_run_s_min_max": [round(min(ts), 4), round(max(ts), 4)],
             "speedup_vs_sequential": round(t_seq / t_batch, 2), "autotuned_blocks": list(blocks),
             "max_h": round(float(hz["max_h"].max()), 3), "exceed_nodes": int((hz["p_exceed"] > 0).sum())}
 
@@ -321,20 +372,31 @@ def adjoint_cost(nsteps, ckpt=50, fd_steps=50):
             "grad_mu": g["mu"], "grad_xi": g["xi"], "fd_check_steps": fd_steps, "fd_check": fd}
 
 
-def cpu_baseline(case, nsteps):
-    """The oracle (serial C restatement, 1 thread) on a bounded sample of the
-    same workload: all particles, ``nsteps`` steps."""
+def cpu_baseline(case, window_state, start_step, nsteps):
+    """The oracle (serial C restatement of the reference's step, 1 thread) on
+    a bounded sample of the same workload: all particles, ``nsteps`` steps
+    from the state the GPU run reached at the start of the timed window (so
+    both time the same developed flow)."""
     try:
         from oracle import oracle
     except Exception as e:  # the oracle ships with the repository
         return {"error": str(e)}
     st = oracle.init(case)
+    if window_state is not None:
+        g, mass = window_state
+        st.x, st.v, st.h, st.u, st.hsml = (np.ascontiguousarray(g[k], dtype=np.float64).copy()
+                                           for k in ("x", "v", "h", "u", "hsml"))
+        st.uaux = np.ascontiguousarray(g["uaux"], dtype=np.float64).copy()
+        st.flag = np.ascontiguousarray(g["flag"], dtype=np.int32).copy()
+        st.mass = np.ascontiguousarray(mass, dtype=np.float64).copy()
     t = time.perf_counter()
     oracle.run(case, st, nsteps)
     dt = time.perf_counter() - t
+    where = f"steps {start_step + 1}..{start_step + nsteps}" if window_state is not None else f"steps 1..{nsteps}"
     return {"value": round(case.particles.n * nsteps / dt, 1), "unit": "particle-steps/s", "cores": 1,
-            "kind": "port", "sample": f"{case.particles.n} particles x {nsteps} steps (oracle, serial C FP64)",
-            "cpu": cpu_model()}
+            "kind": "port", "sample": f"{case.particles.n} particles x {nsteps} steps ({where}, the GPU run's "
+                                      f"state at step {start_step} as input; oracle, serial C FP64)",
+            "seconds": round(dt, 3), "cpu": cpu_info()}
 
 
 if __name__ == "__main__":

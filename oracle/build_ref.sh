@@ -1,8 +1,11 @@
 #!/usr/bin/env bash
 # Builds the reference Geoflow-SPH Fortran sources (read where they lie under
-# /root/reference) into oracle/_ref/gfsph_ref, an FP64 executable driven by
-# oracle/ref_harness.f90.  Test infrastructure only: nothing in the product
-# path links or runs it.
+# /root/reference) into FP64 executables under oracle/_ref:
+#   gfsph_ref       the modules driven by oracle/ref_harness.f90
+#   gfsph_ref_main  the reference's own main program (step recorder attached)
+#   gfsph_ref_fma   gfsph_ref built with fused multiply-adds
+#   gfsph_ref_gpu   ref_harness.f90 driving the product's Fortran drop-in
+# Test infrastructure only: nothing in the product path links or runs them.
 #
 # Precision recipe (mirrors what the reference's own kinds imply):
 #   -fdefault-real-8   promotes REAL / kind(0.0) (irk) to FP64; const(:) stays
@@ -21,6 +24,8 @@
 #   5 SPH_Main_Vars_2019.f90:175  "integer (irk) iffix" -> integer (ink); with
 #                                 irk promoted to 8 the semi-implicit solver's
 #                                 INTEGER(4) dummy no longer matches (off path)
+#   8 SPH_FEM_Driver_2019.f90:1993 "real (irk) ndout" -> integer: used as an
+#                                 implied-do bound in Write_CSV
 # The values tested are 0/1, so the logical edits keep the semantics.
 set -euo pipefail
 
@@ -37,51 +42,75 @@ if [ ! -d "$SRC" ]; then
 fi
 command -v "$FC" >/dev/null 2>&1 || { echo "build_ref.sh: $FC not found; skipping" >&2; exit 0; }
 
-mkdir -p "$BLD"
-cd "$BLD"
-
-compile_asis() {   # $1 = file stem under $SRC, $2 = object name
-  "$FC" -c $FFLAGS -module-dir "$BLD" "$SRC/$1.f90" -o "$BLD/$2.o"
+# compile_all <build dir> <flags>: every reference module into <build dir>
+compile_all() {
+  local B="$1" FL="$2"
+  mkdir -p "$B"
+  asis() { "$FC" -c $FL -module-dir "$B" "$SRC/$1.f90" -o "$B/$2.o"; }
+  asis "1 SPH_FEM_variable_types_2019" r1
+  asis "3 SPH_FEM_Driver_time_vars_2019" r3
+  asis "2 SPH_FEM_topo_2019" r2
+  asis "4 SPH_SW_Vars_2019" r4
+  sed -e '175s/integer (irk), allocatable:: iffix/integer (ink), allocatable:: iffix/' \
+      "$SRC/5 SPH_Main_Vars_2019.f90" > "$B/r5_edit.f90"
+  "$FC" -c $FL -module-dir "$B" "$B/r5_edit.f90" -o "$B/r5.o"
+  rm -f "$B/r5_edit.f90"
+  asis "5a My_SPH_Tools_2019" r5a
+  asis "5b GFL_Global_Vars_2019" r5b
+  asis "5c SPH_GFL_SW_Interactions_2019" r5c
+  asis "5d SPH_GCsolver_2019" r5d
+  asis "5e SPH_Quicksort_2019" r5e
+  asis "5f SPH_Vulnerability" r5f
+  sed -e '395s/((\s*h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
+      -e '536s/((\s*h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
+      -e '402s/(\s*(time, h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(time, h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
+      -e '543s/(\s*(time, h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(time, h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
+      -e '9258s/if (if_frictional)/if (if_frictional.ne.0)/' \
+      "$SRC/6 SPH_SW_2019.f90" > "$B/r6_edit.f90"
+  "$FC" -c $FL -module-dir "$B" "$B/r6_edit.f90" -o "$B/r6.o"
+  rm -f "$B/r6_edit.f90"
+  sed -e '3264s/if (if_density_comput)/if (if_density_comput.ne.0)/' \
+      -e '3279s/if (if_density_comput)/if (if_density_comput.ne.0)/' \
+      "$SRC/7 SPH_MAIN_2019.f90" > "$B/r7_edit.f90"
+  "$FC" -c $FL -module-dir "$B" "$B/r7_edit.f90" -o "$B/r7.o"
+  rm -f "$B/r7_edit.f90"
+  asis "7b GFL_MAIN_2019" r7b
+  "$FC" -c $FL -cpp -module-dir "$B" "$HERE/ref_harness.f90" -o "$B/harness.o"
+}
+refobjs() {  # the reference's module objects in link order
+  local B="$1"
+  echo "$B/r1.o $B/r3.o $B/r2.o $B/r4.o $B/r5.o $B/r5a.o $B/r5b.o $B/r5c.o $B/r5d.o $B/r5e.o $B/r5f.o" \
+       "$B/r6.o $B/r7.o $B/r7b.o"
 }
 
-compile_asis "1 SPH_FEM_variable_types_2019" r1
-compile_asis "3 SPH_FEM_Driver_time_vars_2019" r3
-compile_asis "2 SPH_FEM_topo_2019" r2
-compile_asis "4 SPH_SW_Vars_2019" r4
-sed -e '175s/integer (irk), allocatable:: iffix/integer (ink), allocatable:: iffix/' \
-    "$SRC/5 SPH_Main_Vars_2019.f90" > "$BLD/r5_edit.f90"
-"$FC" -c $FFLAGS -module-dir "$BLD" "$BLD/r5_edit.f90" -o "$BLD/r5.o"
-rm -f "$BLD/r5_edit.f90"
-compile_asis "5a My_SPH_Tools_2019" r5a
-compile_asis "5b GFL_Global_Vars_2019" r5b
-compile_asis "5c SPH_GFL_SW_Interactions_2019" r5c
-compile_asis "5d SPH_GCsolver_2019" r5d
-compile_asis "5e SPH_Quicksort_2019" r5e
-compile_asis "5f SPH_Vulnerability" r5f
-
-sed -e '395s/((\s*h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
-    -e '536s/((\s*h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
-    -e '402s/(\s*(time, h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(time, h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
-    -e '543s/(\s*(time, h_cw(icw),vn_cw(icw),q_cw(icw)),icw/(time, h_cw(icw),vn_cw(icw),q_cw(icw),icw/' \
-    -e '9258s/if (if_frictional)/if (if_frictional.ne.0)/' \
-    "$SRC/6 SPH_SW_2019.f90" > "$BLD/r6_edit.f90"
-"$FC" -c $FFLAGS -module-dir "$BLD" "$BLD/r6_edit.f90" -o "$BLD/r6.o"
-rm -f "$BLD/r6_edit.f90"
-
-sed -e '3264s/if (if_density_comput)/if (if_density_comput.ne.0)/' \
-    -e '3279s/if (if_density_comput)/if (if_density_comput.ne.0)/' \
-    "$SRC/7 SPH_MAIN_2019.f90" > "$BLD/r7_edit.f90"
-"$FC" -c $FFLAGS -module-dir "$BLD" "$BLD/r7_edit.f90" -o "$BLD/r7.o"
-rm -f "$BLD/r7_edit.f90"
-
-compile_asis "7b GFL_MAIN_2019" r7b
-
-"$FC" -c $FFLAGS -cpp -module-dir "$BLD" "$HERE/ref_harness.f90" -o "$BLD/harness.o"
-"$FC" $FFLAGS -o "$OUT/gfsph_ref" \
-  "$BLD"/r1.o "$BLD"/r3.o "$BLD"/r2.o "$BLD"/r4.o "$BLD"/r5.o "$BLD"/r5a.o \
-  "$BLD"/r5b.o "$BLD"/r5c.o "$BLD"/r5d.o "$BLD"/r5e.o "$BLD"/r5f.o \
-  "$BLD"/r6.o "$BLD"/r7.o "$BLD"/r7b.o "$BLD"/harness.o
+compile_all "$BLD" "$FFLAGS"
+"$FC" $FFLAGS -o "$OUT/gfsph_ref" $(refobjs "$BLD") "$BLD/harness.o"
 echo "build_ref.sh: built $OUT/gfsph_ref"
+
+# The reference's own main program (PROGRAM SPH_FEM_Driver_2019) with the
+# compile fix flang needs ("8 SPH_FEM_Driver_2019.f90":1993, a REAL variable
+# used as an implied-do bound -> INTEGER, the value is integral) and the step
+# recorder of ref_main_probe.f90 wrapped around stop_case_SPH at link time.
+sed -e '1993s/real   (irk) ndout/integer(ink) ndout/' \
+    "$SRC/8 SPH_FEM_Driver_2019.f90" > "$BLD/r8_edit.f90"
+"$FC" -c $FFLAGS -module-dir "$BLD" "$BLD/r8_edit.f90" -o "$BLD/r8.o"
+rm -f "$BLD/r8_edit.f90"
+"$FC" -c $FFLAGS -module-dir "$BLD" "$HERE/ref_main_probe.f90" -o "$BLD/probe.o"
+"$FC" $FFLAGS -o "$OUT/gfsph_ref_main" $(refobjs "$BLD") "$BLD/r8.o" "$BLD/probe.o" \
+  -Wl,--wrap=_QMsph_main_2019Pstop_case_sph
+echo "build_ref.sh: built $OUT/gfsph_ref_main"
+
+# The same reference built with fused multiply-adds (contraction on, as the
+# compilers' defaults for the reference's shipped Windows build allow): an
+# equally valid rounding of the same program.  The spread between the two
+# builds is the reference's own reproducibility on a given case.
+if [ "${GFSPH_REF_FMA:-1}" = "1" ]; then
+  BLDF="$OUT/build_fma"
+  FFLAGS_FMA="-O2 -fdefault-real-8 -fdefault-double-8 -ffp-contract=fast -march=x86-64-v3"
+  compile_all "$BLDF" "$FFLAGS_FMA"
+  "$FC" $FFLAGS_FMA -o "$OUT/gfsph_ref_fma" $(refobjs "$BLDF") "$BLDF/harness.o"
+  echo "build_ref.sh: built $OUT/gfsph_ref_fma"
+fi
 
 # The same harness driving the GPU step through the product's ISO_C_BINDING
 # layer (SPH_MAIN_GFSPH), linked against the in-tree libgfsph.so.
@@ -91,9 +120,7 @@ if [ -f "$PKG/gfsph/libgfsph.so" ]; then
   "$FC" -c $FFLAGS -module-dir "$BLD" "$PKG/fortran/sph_main_gfsph.f90" -o "$BLD/sph_main_gfsph.o"
   "$FC" -c $FFLAGS -cpp -DGFSPH_GPU -module-dir "$BLD" "$HERE/ref_harness.f90" -o "$BLD/harness_gpu.o"
   "$FC" $FFLAGS -o "$OUT/gfsph_ref_gpu" \
-    "$BLD"/r1.o "$BLD"/r3.o "$BLD"/r2.o "$BLD"/r4.o "$BLD"/r5.o "$BLD"/r5a.o \
-    "$BLD"/r5b.o "$BLD"/r5c.o "$BLD"/r5d.o "$BLD"/r5e.o "$BLD"/r5f.o \
-    "$BLD"/r6.o "$BLD"/r7.o "$BLD"/r7b.o "$BLD"/gfsph_c.o "$BLD"/sph_main_gfsph.o "$BLD"/harness_gpu.o \
+    $(refobjs "$BLD") "$BLD"/gfsph_c.o "$BLD"/sph_main_gfsph.o "$BLD"/harness_gpu.o \
     -L"$PKG/gfsph" -lgfsph -Wl,-rpath,'$ORIGIN/../../deslizamientos-y-flujos_amd/gfsph'
   echo "build_ref.sh: built $OUT/gfsph_ref_gpu"
 fi

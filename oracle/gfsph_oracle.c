@@ -1320,3 +1320,58 @@ void gfo_mc_pixel_accum(const gfo_dem *d, int n, const double *x, const double *
     }
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* a14 (iii). Norm_MC_pixel_Vars_SW (6:20838-20908), called with an      */
+/* offset after every problem (7:3521, 8:1539-1541): per node reached    */
+/* by a problem (cnt > 0), the time averages OCMC(1), OCMC(3) = sums /   */
+/* visits (6:20857-20860) and the two maxima enter MC_Pr(1,:) += OCMC,   */
+/* MC_Pr(2,:) += OCMC**2 (6:20864-20869), nr_problem_pix += 1; after the */
+/* last problem MC_Pr(3,:) = mean = MC_Pr(1,:)/nr, MC_Pr(4,:) =          */
+/* sqrt(MC_Pr(2,:)/nr - mean**2), 0 below 1e-6 (6:20885-20898).         */
+/* acc[R][npoig][4], cnt[R][npoig] as gfo_mc_pixel_accum leaves them.   */
+/* Outputs: pr[npoig][4][4] = MC_Pr(row, var) for rows 1..4 as          */
+/* pr[g][row][var]; nprob[npoig]; maxs[npoig][2] = max over problems of */
+/* max h, max |v|; pexc[npoig] = fraction of problems with max h > hthr */
+/* (the exceedance raster, no reference counterpart).                   */
+/* ------------------------------------------------------------------ */
+void gfo_mc_hazard(int64_t npoig, int R, const double *acc, const int32_t *cnt, double hthr, double *pr,
+                   int32_t *nprob, double *maxs, double *pexc) {
+  for (int64_t g = 0; g < npoig; g++) {
+    double *P = pr + 16 * g;
+    for (int k = 0; k < 16; k++) P[k] = 0.0;
+    int nr = 0, nexc = 0;
+    double mh = 0.0, mv = 0.0;
+    for (int r = 0; r < R; r++) {
+      const int64_t k = (int64_t)r * npoig + g;
+      const int c = cnt[k];
+      if (c == 0) continue;
+      const double *a = acc + 4 * k;
+      double o[4];
+      o[0] = a[0] / c;
+      o[1] = a[1];
+      o[2] = a[2] / c;
+      o[3] = a[3];
+      for (int q = 0; q < 4; q++) {
+        P[q] = P[q] + o[q];
+        P[4 + q] = P[4 + q] + o[q] * o[q];
+      }
+      nr++;
+      if (a[1] > mh) mh = a[1];
+      if (a[3] > mv) mv = a[3];
+      if (a[1] > hthr) nexc++;
+    }
+    const int nd = nr > 0 ? nr : 1;
+    for (int q = 0; q < 4; q++) {
+      const double mean = P[q] / nd, sig2 = P[4 + q] / nd;
+      double t = sig2 - mean * mean;
+      if (t < 1.e-6) t = 0.0;
+      P[8 + q] = mean;
+      P[12 + q] = sqrt(t);
+    }
+    nprob[g] = nr;
+    maxs[2 * g] = mh;
+    maxs[2 * g + 1] = mv;
+    pexc[g] = R > 0 ? (double)nexc / R : 0.0;
+  }
+}

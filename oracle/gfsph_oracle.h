@@ -67,4 +67,9 @@ void gfo_raster_idw(const gfo_dem *d, int n, const double *x, const double *v, c
                     const double *u, const int32_t *flag, double *out, double *wsum);
 void gfo_mc_pixel_accum(const gfo_dem *d, int n, const double *x, const double *v,
                         const double *rho, const int32_t *flag, double *acc, int32_t *cnt);
+/* Norm_MC_pixel_Vars_SW over R problems' accumulators: pr[npoig][4][4]
+ * (rows: sum, sum of squares, mean, standard deviation of average h, max h,
+ * average |v|, max |v|), nprob, maxs[npoig][2], pexc */
+void gfo_mc_hazard(int64_t npoig, int R, const double *acc, const int32_t *cnt, double hthr, double *pr,
+                   int32_t *nprob, double *maxs, double *pexc);
 #endif

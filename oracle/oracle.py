@@ -58,6 +58,7 @@ def lib():
                               C.c_int, dp, dp, dp]
         L.gfo_raster_idw.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, dp, dp]
         L.gfo_mc_pixel_accum.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, dp, ip]
+        L.gfo_mc_hazard.argtypes = [C.c_int64, C.c_int, dp, ip, C.c_double, dp, ip, dp, dp]
         L.gfo_adaptive_dt.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, C.c_double]
         L.gfo_adaptive_dt.restype = C.c_double
         _lib = L
@@ -234,6 +235,23 @@ def mc_pixel_accum(dem, st, acc=None, cnt=None):
     lib().gfo_mc_pixel_accum(C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _ip(st.flag),
                              _dp(acc), _ip(cnt))
     return acc, cnt
+
+
+def mc_hazard(acc, cnt, hthr=0.5):
+    """Norm_MC_pixel_Vars_SW ("6 SPH_SW_2019.f90":20838-20908) over the
+    per-problem pixel accumulators acc[R][npoig][4], cnt[R][npoig]: returns
+    {"pr": [npoig][4][4] MC_Pr rows (sum, sum of squares, mean, standard
+    deviation) x variables (average h, max h, average |v|, max |v|),
+    "nprob", "maxs" [npoig][2], "pexc"}."""
+    acc = np.ascontiguousarray(acc, dtype=np.float64)
+    cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+    R, npg = cnt.shape
+    pr = np.zeros((npg, 4, 4))
+    nprob = np.zeros(npg, np.int32)
+    maxs = np.zeros((npg, 2))
+    pexc = np.zeros(npg)
+    lib().gfo_mc_hazard(npg, R, _dp(acc), _ip(cnt), float(hthr), _dp(pr), _ip(nprob), _dp(maxs), _dp(pexc))
+    return {"pr": pr, "nprob": nprob, "maxs": maxs, "pexc": pexc}
 
 
 def run_case(case, st, vstop, timestop, dtimestop, dt_driver, time_end, time_plot, max_steps, pixels=True):

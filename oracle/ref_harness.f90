@@ -12,6 +12,18 @@
 !     line 1: problem name (the .dat stem)
 !     line 2: nsteps  dt_sph  ic_adapt_dt_sph
 !     line 3: ndump   followed by ndump step numbers (0 = after Init_sph)
+!     line 4 (optional): nprob  plot_every  -- Monte Carlo pixel mode: the
+!             problems named on line 5 (one .dat stem each, sharing the
+!             release and terrain) run one after another, each from its own
+!             Init_sph, with Get_MC_pixel_Vars_SW ("6 SPH_SW_2019.f90":
+!             20717-20833) every plot_every steps and Norm_MC_pixel_Vars_SW(1)
+!             (6:20838-20908) after its last step, as Driver_Flow_23 calls
+!             them through Manage_pixel_Hazard_SPH (11) and (12)
+!             ("8 SPH_FEM_Driver_2019.f90":1504-1541); then mc_pixel.bin:
+!                int32  magic npoig nstat
+!                per pixel with MC_Pr allocated: int32 ipoig nr_problem_pix,
+!                real64 MC_Pr(1:4, 1:4)
+!             Dumps of line 3 are written for the last problem only.
 !
 !  After Init_sph it writes topo.bin (the terrain, see write_topo) and at
 !  each listed step dump_<step>.bin (stream, native endian):
@@ -41,7 +53,7 @@ use SPH_FEM_Driver_time_vars_2019
 use SPH_MAIN_2019, only: Init_SPH, time_integration_sph, adaptive_dt_sph
 use SPH_Main_Vars_2019
 use SPH_SW_Vars_2019
-use SPH_SW_2019, only: OutputRes_SW
+use SPH_SW_2019, only: OutputRes_SW, Get_MC_pixel_Vars_SW, Norm_MC_pixel_Vars_SW
 use SPH_FEM_topo_2019, only: npoigx, npoigy, Xming, Xmaxg, Yming, Ymaxg, deltxg, deltyg, &
                              xshiftg, yshiftg, topol, poig_in_model
 #ifdef GFSPH_GPU
@@ -55,6 +67,8 @@ integer        nsteps, ndump, istep, k, ictl, iadapt
 logical        gid_final
 integer, allocatable :: dump_steps(:)
 real(8)        dtin
+integer        nprob, plot_every, iprob, ios
+character(16), allocatable :: pnames(:)
 
 open (newunit=ictl, file='harness.ctl', status='old')
 read (ictl,'(a16)') name
@@ -65,6 +79,11 @@ if (ndump.gt.0) then
    backspace (ictl)
    read (ictl,*) ndump, (dump_steps(k), k=1,ndump)
 endif
+nprob = 0; plot_every = 0
+read (ictl,*,iostat=ios) nprob, plot_every
+if (ios.ne.0) nprob = 0
+allocate (pnames(max(nprob,1)))
+if (nprob.gt.0) read (ictl,*) (pnames(k), k=1,nprob)
 close (ictl)
 
 !      ------  what Driver_Input would have read (8:229-470)
@@ -98,6 +117,14 @@ nproblems        = 1
 nproblems_index  = 1
 time_sph_restart = 0.0
 
+do iprob = 1, max(nprob,1)
+if (nprob.gt.0) then
+   problem_name_Global = pnames(iprob)
+   SPH_SW_problem_name = pnames(iprob)
+   nproblems_index     = iprob
+   dt = dtin; dt_sph = dtin
+   t_print_reset = 0.; t_save_reset = 0.; t_plot_reset = 0.
+endif
 call Init_sph
 
 !      ------  Driver_Flow (8:1565-1620) for one SPH substep per step
@@ -108,8 +135,10 @@ if (time_sph_restart.gt.0.0) then
    time_sph = time_sph_restart
 endif
 
-call maybe_dump (0)
-call write_topo
+if (iprob.eq.1) then
+   call maybe_dump (0)
+   call write_topo
+endif
 do istep = 1, nsteps
    t_print_reset = t_print_reset + dt
 #ifndef GFSPH_GPU
@@ -123,8 +152,14 @@ do istep = 1, nsteps
 #endif
    time_sph = time_sph + dt_sph
    time     = time + dt
-   call maybe_dump (istep)
+   if (nprob.eq.0.or.iprob.eq.nprob) call maybe_dump (istep)
+   if (nprob.gt.0) then
+      if (mod(istep, plot_every).eq.0) call Get_MC_pixel_Vars_SW
+   endif
 enddo
+if (nprob.gt.0) call Norm_MC_pixel_Vars_SW (1)
+enddo
+if (nprob.gt.0) call write_mc_pixel
 !      ------  with gid_final.ctl present: the GiD result blocks of the final
 !              state (OutputRes_SW, appended to <name>.post.res)
 inquire (file='gid_final.ctl', exist=gid_final)
@@ -134,6 +169,24 @@ call gfsph_release
 #endif
 
 contains
+
+subroutine write_mc_pixel
+   integer iu, ip, nst, np9
+   np9 = npoigx*npoigy
+   nst = 0
+   do ip = 1, np9
+      if (MC_pixel(ip)%if_MC_Pr_alloc.eq.1) nst = nst + 1
+   enddo
+   open (newunit=iu, file='mc_pixel.bin', access='stream', form='unformatted', status='replace')
+   write (iu) int(1195725648,4), int(np9,4), int(nst,4)
+   do ip = 1, np9
+      if (MC_pixel(ip)%if_MC_Pr_alloc.eq.1) then
+         write (iu) int(ip,4), int(MC_pixel(ip)%nr_problem_pix,4)
+         write (iu) real(MC_pixel(ip)%MC_Pr(1:4,1:4),8)
+      endif
+   enddo
+   close (iu)
+end subroutine write_mc_pixel
 
 ! topo.bin: the terrain Init_sph built (Topo_main's module state)
 !    int32  magic(=1195725648) version(=2) npoigx npoigy
@@ -146,8 +199,12 @@ subroutine write_topo
    write (iu) int(1195725648,4), int(2,4), int(npoigx,4), int(npoigy,4)
    write (iu) real(Xming,8), real(Xmaxg,8), real(Yming,8), real(Ymaxg,8), real(deltxg,8), real(deltyg,8), &
               real(xshiftg,8), real(yshiftg,8)
-   write (iu) (real(topol(1:9,ip),8), ip=1,np9)
-   write (iu) (int(poig_in_model(ip),4), ip=1,np9)
+   do ip = 1, np9, 8192   ! bounded records: large single records are slow to build
+      write (iu) real(topol(1:9,ip:min(ip+8191,np9)),8)
+   enddo
+   do ip = 1, np9, 8192
+      write (iu) int(poig_in_model(ip:min(ip+8191,np9)),4)
+   enddo
    close (iu)
 end subroutine write_topo
 
@@ -191,20 +248,20 @@ subroutine write_dump (is)
    open (newunit=iu, file=trim(fname), access='stream', form='unformatted', status='replace')
    write (iu) int(1195725648,4), int(1,4), int(npoin,4), int(ndimn,4), int(has_uaux,4), int(np,4)
    write (iu) real(time_sph,8)
-   write (iu) (real(x(:,ip),8), ip=1,npoin)
-   write (iu) (real(vx(:,ip),8), ip=1,npoin)
-   write (iu) (real(rho(ip),8), ip=1,npoin)
-   write (iu) (real(mass(ip),8), ip=1,npoin)
-   write (iu) (real(hsml(ip),8), ip=1,npoin)
-   write (iu) (real(u(ip),8), ip=1,npoin)
+   call wr_d (iu, x, ndimn*npoin)
+   call wr_d (iu, vx, ndimn*npoin)
+   call wr_d (iu, rho, npoin)
+   call wr_d (iu, mass, npoin)
+   call wr_d (iu, hsml, npoin)
+   call wr_d (iu, u, npoin)
    if (has_uaux.eq.1) write (iu) (real(Uaux(1,ip),8), ip=1,npoin)
-   write (iu) (int(If_Out_Domain(ip),4), ip=1,npoin)
-   write (iu) (real(x0(:,ip),8), ip=1,npoin)
+   call wr_i (iu, If_Out_Domain, npoin)
+   call wr_d (iu, x0, ndimn*npoin)
    if (np.gt.0) then
-      write (iu) pi_(1:np)
-      write (iu) pj_(1:np)
-      write (iu) pw(1:np)
-      write (iu) pdw(:,1:np)
+      call wr_i (iu, pi_, np)
+      call wr_i (iu, pj_, np)
+      call wr_d (iu, pw, np)
+      call wr_d (iu, pdw, ndimn*np)
    endif
    close (iu)
    deallocate (pi_, pj_, pw, pdw)
@@ -223,5 +280,26 @@ subroutine write_dump (is)
       close (iu)
    endif
 end subroutine write_dump
+
+! wr_d / wr_i: n values of an array (any shape, contiguous) as consecutive
+! stream records of at most 8192 values (large single records are slow to
+! build in the runtime); the bytes are those of one record of all n
+subroutine wr_d (iu, a, n)
+   integer, intent(in) :: iu, n
+   real(8), intent(in) :: a(*)
+   integer k
+   do k = 1, n, 8192
+      write (iu) a(k:min(k+8191,n))
+   enddo
+end subroutine wr_d
+
+subroutine wr_i (iu, a, n)
+   integer, intent(in) :: iu, n
+   integer(4), intent(in) :: a(*)
+   integer k
+   do k = 1, n, 8192
+      write (iu) a(k:min(k+8191,n))
+   enddo
+end subroutine wr_i
 
 end program gfsph_ref_harness

@@ -277,3 +277,106 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
     if not keep and workdir is None:
         subprocess.run(["rm", "-rf", tmp])
     return out
+
+
+def read_mc_pixel(path: str) -> dict:
+    """mc_pixel.bin of the harness's Monte Carlo pixel mode: for every DEM
+    node whose MC_Pr the reference allocated, the node (0-based), the number
+    of problems that reached it (nr_problem_pix) and MC_Pr(1:4, 1:4) as
+    [row][variable] (rows: sum over problems, sum of squares, mean, standard
+    deviation; variables: average h, max h, average |v|, max |v|)."""
+    raw = open(path, "rb").read()
+    magic, npg, nst = (int(v) for v in np.frombuffer(raw, dtype=np.int32, count=3))
+    assert magic == 1195725648
+    node = np.zeros(nst, np.int64)
+    nprob = np.zeros(nst, np.int32)
+    pr = np.zeros((nst, 4, 4))
+    off = 12
+    for k in range(nst):
+        a, b = np.frombuffer(raw, dtype=np.int32, count=2, offset=off)
+        off += 8
+        node[k], nprob[k] = a - 1, b
+        # MC_Pr(1:4, 1:4) column-major: element (row r, var q) at r + 4 q
+        pr[k] = np.frombuffer(raw, dtype=np.float64, count=16, offset=off).reshape(4, 4).T
+        off += 128
+    return {"npoig": npg, "node": node, "nprob": nprob, "mc_pr": pr}
+
+
+def run_reference_pixels(cases, nsteps: int, plot_every: int, timeout: float = 3600.0) -> dict:
+    """The reference's Monte Carlo pixel statistics over ``cases`` (one
+    problem each: same terrain and release, its own const(:)), run one after
+    another by the harness: Get_MC_pixel_Vars_SW every ``plot_every`` steps
+    and Norm_MC_pixel_Vars_SW(1) after each problem's ``nsteps`` steps
+    ("6 SPH_SW_2019.f90":20717-20908).  Returns read_mc_pixel's dict."""
+    if not os.path.exists(REF_EXE):
+        raise FileNotFoundError(REF_EXE)
+    tmp = tempfile.mkdtemp(prefix="gfsph_refpix_")
+    try:
+        names = []
+        for k, c in enumerate(cases):
+            if c.params.integrator != 4:
+                raise ValueError("the reference integrates with RK4 only")
+            nm = f"p{k + 1}"
+            write_inputs(tmp, c, name=nm)
+            names.append(nm)
+        dt = cases[0].params.dt
+        with open(os.path.join(tmp, "harness.ctl"), "w") as f:
+            f.write(f"{names[0]}\n{nsteps} {_r(dt)} {int(getattr(cases[0].params, 'adapt_dt', 0))}\n0\n")
+            f.write(f"{len(names)} {int(plot_every)}\n" + " ".join(names) + "\n")
+        r = subprocess.run([REF_EXE], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
+                           timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        return read_mc_pixel(os.path.join(tmp, "mc_pixel.bin"))
+    finally:
+        subprocess.run(["rm", "-rf", tmp])
+
+
+REF_MAIN_EXE = os.path.join(HERE, "_ref", "gfsph_ref_main")
+
+
+def read_case_trace(path: str) -> list:
+    """case_trace.bin of the reference's main program (oracle/ref_main_probe.f90):
+    one record per Driver_Flow_23 step, in order."""
+    raw = open(path, "rb").read()
+    out = []
+    off = 0
+    while off < len(raw):
+        magic, prob, it, stop, n = (int(v) for v in np.frombuffer(raw, dtype=np.int32, count=5, offset=off))
+        assert magic == 1195725648
+        off += 20
+        t = np.frombuffer(raw, dtype=np.float64, count=4, offset=off)
+        off += 32
+        rec = {"problem": prob, "itimestep": it, "stop": stop, "time_sph": float(t[0]), "dt_sph": float(t[1]),
+               "time": float(t[2]), "t_plot_reset": float(t[3])}
+        for k, cnt in (("x", 2 * n), ("v", 2 * n), ("h", n), ("u", n)):
+            a = np.frombuffer(raw, dtype=np.float64, count=cnt, offset=off)
+            off += a.nbytes
+            rec[k] = a.reshape(n, 2).copy() if cnt == 2 * n else a.copy()
+        rec["flag"] = np.frombuffer(raw, dtype=np.int32, count=n, offset=off).copy()
+        off += 4 * n
+        out.append(rec)
+    return out
+
+
+def run_reference_main(case, name: str = "case", timeout: float = 3600.0, **master) -> list:
+    """Runs the reference's own main program (PROGRAM SPH_FEM_Driver_2019,
+    built as oracle/_ref/gfsph_ref_main) on ``case``'s input files and a
+    <name>.MASTER.dat written by write_master(**master); returns the per-step
+    records of its case loop (read_case_trace)."""
+    if not os.path.exists(REF_MAIN_EXE):
+        raise FileNotFoundError(REF_MAIN_EXE)
+    tmp = tempfile.mkdtemp(prefix="gfsph_refmain_")
+    try:
+        write_inputs(tmp, case, name=name, topo="topo")
+        write_master(tmp, name, case.params.dt, **master)
+        # the main program opens <name>.master.dat (8:217); the reference was
+        # written on a file system that ignores case
+        os.rename(os.path.join(tmp, name + ".MASTER.dat"), os.path.join(tmp, name + ".master.dat"))
+        r = subprocess.run([REF_MAIN_EXE], input=f"{name}\ntopo\n" + "\n" * 16, cwd=tmp, capture_output=True,
+                           text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference main failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-2000:]}")
+        return read_case_trace(os.path.join(tmp, "case_trace.bin"))
+    finally:
+        subprocess.run(["rm", "-rf", tmp])

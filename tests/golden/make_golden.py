@@ -262,12 +262,158 @@ def save_cfg1(out_dir=HERE, nsteps=100, sample_every=37):
     np.savez_compressed(os.path.join(out_dir, "g_cfg1_full.npz"), **arr)
 
 
+REF_FMA = os.path.join(REPO, "oracle", "_ref", "gfsph_ref_fma")
+GROWTH_STEPS = (1, 2, 5, 10, 20, 30, 50, 75, 100, 150, 200, 300, 500, 1000, 1500, 2000)
+
+
+def _integrals(dd):
+    """Mass-free integral measures of a state: centroid, mean speed and the
+    extent of the flow (active particles)."""
+    act = dd["flag"] == 0
+    x, v = dd["x"][act], dd["v"][act]
+    sp = np.sqrt((v * v).sum(axis=1))
+    return np.array([x[:, 0].mean(), x[:, 1].mean(), sp.mean(), x[:, 0].max(), x[:, 0].min(),
+                     dd["h"][act].mean(), float(act.sum())])
+
+
+def save_cfg1_growth(out_dir=HERE, every=23):
+    """cfg1 (10,011 particles, RK4 dt 0.05 s) for 2,000 steps, run by the
+    reference build and by the same reference built with fused
+    multiply-adds: a sample of the state at GROWTH_STEPS for both, the
+    whole-field spread between the two builds (field-normwise, x v h u),
+    integral measures of the flow, and the reference's whole state at step
+    100.  The spread is the reference's own build-to-build reproducibility
+    on this case; it bounds what any other FP64 evaluation can be held to."""
+    case = C.cfg_plane_voellmy(nsteps=GROWTH_STEPS[-1])
+    ref = refio.run_reference(case, GROWTH_STEPS[-1], GROWTH_STEPS)
+    fma = refio.run_reference(case, GROWTH_STEPS[-1], GROWTH_STEPS, exe=REF_FMA)
+    idx = np.arange(0, case.particles.n, every)
+    arr = {"steps": np.array(GROWTH_STEPS), "idx": idx}
+    for s in GROWTH_STEPS:
+        for k in ("x", "v", "h", "u"):
+            arr[f"s{s}_{k}"] = ref[s][k][idx]
+            arr[f"f{s}_{k}"] = fma[s][k][idx]
+            a, b = fma[s][k], ref[s][k]
+            arr[f"spread{s}_{k}"] = np.array(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+        arr[f"s{s}_flag"] = ref[s]["flag"][idx]
+        arr[f"s{s}_int"] = _integrals(ref[s])
+        arr[f"f{s}_int"] = _integrals(fma[s])
+    for k in ("x", "v", "h", "u", "flag"):
+        arr[f"full100_{k}"] = ref[100][k]
+    np.savez_compressed(os.path.join(out_dir, "g_cfg1_growth.npz"), **arr)
+
+
+def _pairs_sha(dd):
+    pr = np.sort(dd["pairs"], axis=1)
+    pr = pr[np.lexsort((pr[:, 1], pr[:, 0]))].astype(np.int32)
+    return pr.shape[0], np.frombuffer(hashlib.sha256(pr.tobytes()).hexdigest().encode(), dtype=np.uint8)
+
+
+def save_cfg2(out_dir=HERE, nsteps=100, every=101):
+    """The 200,000-particle fractal workload (2048 x 2048 DEM at 2 m,
+    frictional 28 deg + pore pressure) at full size with the reference's
+    integrator (RK4, dt 0.02 s): reference state after 1, 10 and 100 steps
+    as a particle sample and global sums, the SHA-256 of the sorted pair
+    sets of steps 1 and 100, and the whole-field spread of the reference
+    built with fused multiply-adds.  The inputs are regenerated from their
+    seeds (gfsph.cases.cfg_fractal_frictional)."""
+    case = C.cfg_fractal_frictional(integrator=4, nsteps=nsteps)
+    dumps = [1, 10, nsteps]
+    ref = refio.run_reference(case, nsteps, dumps)
+    fma = refio.run_reference(case, nsteps, dumps, exe=REF_FMA)
+    idx = np.arange(0, case.particles.n, every)
+    arr = {"nsteps": np.array(nsteps), "dumps": np.array(dumps), "idx": idx}
+    for s in dumps:
+        for k in ("x", "v", "h", "u"):
+            arr[f"s{s}_{k}"] = ref[s][k][idx]
+            arr[f"s{s}_{k}_sum"] = np.array(np.sum(ref[s][k], axis=0))
+            a, b = fma[s][k], ref[s][k]
+            arr[f"spread{s}_{k}"] = np.array(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+        arr[f"s{s}_flag"] = ref[s]["flag"][idx]
+        arr[f"s{s}_nactive"] = np.array(int((ref[s]["flag"] == 0).sum()))
+        arr[f"s{s}_time"] = np.array(ref[s]["time"])
+        if s in (1, nsteps):
+            arr[f"s{s}_npairs"], arr[f"s{s}_pairs_sha256"] = _pairs_sha(ref[s])
+            arr[f"f{s}_npairs"], arr[f"f{s}_pairs_sha256"] = _pairs_sha(fma[s])
+    np.savez_compressed(os.path.join(out_dir, "g_cfg2_full.npz"), **arr)
+
+
+HAZARD_DRAWS = ((0.12, 200.0), (0.2, 500.0), (0.3, 800.0), (0.45, 1200.0))
+
+
+def hazard_cases():
+    """Monte Carlo pixel-statistics problems: the frictional fixture's terrain
+    and release under Voellmy with the (mu, xi) of HAZARD_DRAWS, RK4 dt
+    0.02 s."""
+    import copy
+    sys.path.insert(0, os.path.dirname(HERE))
+    import goldens
+    base, _ = goldens.case_of("g_frictional_pwp")
+    out = []
+    for mu, xi in HAZARD_DRAWS:
+        c = copy.deepcopy(base)
+        c.params = C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=C.voellmy(mu, xi))
+        out.append(c)
+    return out
+
+
+def save_mc_hazard(out_dir=HERE, nsteps=100, plot_every=10):
+    """The reference's own pixel statistics (Get_MC_pixel_Vars_SW every
+    plot_every steps, Norm_MC_pixel_Vars_SW(1) after each problem) over the
+    problems of hazard_cases(): MC_Pr(1:4, 1:4) and nr_problem_pix of every
+    node the reference allocated."""
+    r = refio.run_reference_pixels(hazard_cases(), nsteps, plot_every)
+    np.savez_compressed(os.path.join(out_dir, "g_mc_hazard_ref.npz"), node=r["node"], nprob=r["nprob"],
+                        mc_pr=r["mc_pr"], npoig=np.array(r["npoig"]), draws=np.array(HAZARD_DRAWS),
+                        nsteps=np.array(nsteps), plot_every=np.array(plot_every))
+
+
+# Case tables run through the reference's own main program: name -> (rows of
+# Ccases_mat, write_master options).  t2: every combination (ic_cases_mat 2,
+# Driver_Flow_23 with steps cut at the plot times), adaptive dt mode 2; t1:
+# one variable at a time (ic_cases_mat 1, Driver_Flow), adaptive dt mode 1.
+CASE_TABLES = {
+    "t2": ([[9, 0.2, 3, 0.12, 0.28, 1], [501, 1.0, 2, 0.8, 1.2, 1]],
+           dict(adapt=2, cases=2, stop=(0.05, 2.5, 0.0), dt_driver=0.5, plot_step=1, time_end=1e9)),
+    "t1": ([[9, 0.2, 2, 0.15, 0.3, 1, 0, 0], [3, 500.0, 2, 300.0, 900.0, 1, 0, 0]],
+           dict(adapt=1, cases=1, stop=(0.1, 1.7, 0.0), dt_driver=0.05, plot_step=1000000, time_end=1e9)),
+}
+
+
+def save_case_driver(out_dir=HERE):
+    """The reference's own main program (PROGRAM SPH_FEM_Driver_2019) on the
+    case tables of CASE_TABLES over the plane fixture's release: for every
+    problem the steps its case loop took, whether Stop_case_SW stopped it,
+    time_sph after every step and the final state, as recorded after each
+    stop_case_SPH call (oracle/ref_main_probe.f90)."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import goldens
+    case, _ = goldens.case_of("g_voellmy_plane")
+    arr = {}
+    for tname, (rows, opts) in CASE_TABLES.items():
+        tr = refio.run_reference_main(case, case_rows=rows, **opts)
+        probs = sorted(set(r["problem"] for r in tr))
+        arr[f"{tname}_nprob"] = np.array(len(probs))
+        for p in probs:
+            recs = [r for r in tr if r["problem"] == p]
+            k = f"{tname}_p{p}"
+            arr[k + "_iters"] = np.array(len(recs))
+            arr[k + "_stop"] = np.array([r["stop"] for r in recs], np.int32)
+            arr[k + "_time"] = np.array([r["time_sph"] for r in recs])
+            for f in ("x", "v", "h", "u", "flag"):
+                arr[k + "_" + f] = recs[-1][f]
+    np.savez_compressed(os.path.join(out_dir, "g_case_driver_ref.npz"), **arr)
+
+
+SPECIAL = {"g_case_driver_ref": save_case_driver, "g_mc_hazard_ref": save_mc_hazard, "g_cfg1_full": save_cfg1, "g_cfg1_growth": save_cfg1_growth, "g_cfg2_full": save_cfg2}
+
+
 def main(argv):
     defs = case_defs()
-    names = argv[1:] or list(defs) + ["g_cfg1_full"]
+    names = argv[1:] or list(defs) + list(SPECIAL)
     for nm in names:
-        if nm == "g_cfg1_full":
-            save_cfg1()
+        if nm in SPECIAL:
+            SPECIAL[nm]()
         else:
             save_case(nm, defs[nm])
         print("wrote", nm, flush=True)

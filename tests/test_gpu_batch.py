@@ -1,10 +1,13 @@
 """Batched Monte Carlo realisations against the same realisations run one
 by one through the plain drop-in (the batch must not change any
-realisation's result), and the hazard statistics against their definition
-(Norm_MC_pixel_Vars_SW, "6 SPH_SW_2019.f90":20838-20908) evaluated from the
-per-realisation accumulators."""
+realisation's result), and the hazard statistics against the oracle's
+Norm_MC_pixel_Vars_SW ("6 SPH_SW_2019.f90":20838-20908) and against the
+reference's own statistics over the same problems."""
 import numpy as np
 import pytest
+
+import goldens
+from oracle import oracle
 
 from gfsph import BatchModel, SPHModel
 from gfsph import cases as C
@@ -105,24 +108,17 @@ def test_batch_pixels_and_hazard():
             acc, cnt = m.mc_pixel_read()
         np.testing.assert_array_equal(accs[r][1], cnt)
         np.testing.assert_allclose(accs[r][0], acc, rtol=1e-12, atol=0)
-    # hazard statistics from their definition
+    # hazard statistics against the oracle's Norm_MC_pixel_Vars_SW (pinned to
+    # the reference's own statistics by tests/test_oracle_hazard.py)
     A = np.stack([a for a, _ in accs])            # [R, np, 4]
     N = np.stack([c for _, c in accs])            # [R, np]
-    hit = N > 0
-    nprob = hit.sum(0)
-    with np.errstate(invalid="ignore", divide="ignore"):
-        o = np.stack([A[:, :, 0] / N, A[:, :, 1], A[:, :, 2] / N, A[:, :, 3]], axis=2)
-    o = np.where(hit[:, :, None], o, 0.0)
-    nd = np.maximum(nprob, 1)[:, None]
-    mean = o.sum(0) / nd
-    var = (o * o).sum(0) / nd - mean * mean
-    std = np.sqrt(np.where(var < 1e-6, 0.0, var))
-    np.testing.assert_array_equal(hz["nprob"], nprob)
-    np.testing.assert_allclose(hz["mean"], mean, rtol=1e-12, atol=1e-300)
-    np.testing.assert_allclose(hz["std"], std, rtol=1e-9, atol=1e-9)
-    np.testing.assert_array_equal(hz["max_h"], A[:, :, 1].max(0))
-    np.testing.assert_array_equal(hz["max_v"], A[:, :, 3].max(0))
-    np.testing.assert_allclose(hz["p_exceed"], (A[:, :, 1] > 0.5).sum(0) / len(cases), rtol=0, atol=1e-15)
+    ref = oracle.mc_hazard(A, N, 0.5)
+    np.testing.assert_array_equal(hz["nprob"], ref["nprob"])
+    np.testing.assert_allclose(hz["mean"], ref["pr"][:, 2, :], rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(hz["std"], ref["pr"][:, 3, :], rtol=1e-9, atol=1e-9)
+    np.testing.assert_array_equal(hz["max_h"], ref["maxs"][:, 0])
+    np.testing.assert_array_equal(hz["max_v"], ref["maxs"][:, 1])
+    np.testing.assert_allclose(hz["p_exceed"], ref["pexc"], rtol=0, atol=1e-15)
     assert hz["p_exceed"].max() > 0
 
 
@@ -151,3 +147,34 @@ def test_autotune_keeps_results():
         sa, sb = a.state(), b.state()
     for k in ("x", "v", "h", "u"):
         np.testing.assert_array_equal(sa[k], sb[k])
+
+
+def test_batch_hazard_matches_reference_statistics():
+    """The reference's own Monte Carlo pixel statistics (the reference build
+    ran Get_MC_pixel_Vars_SW every 10 steps and Norm_MC_pixel_Vars_SW(1)
+    after each of 4 problems, tests/golden/g_mc_hazard_ref.npz) against the
+    batch of the same 4 problems: nodes reached, problems per node, mean and
+    standard deviation of average h, max h, average |v|, max |v|."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_golden import hazard_cases
+
+    f = goldens.load("g_mc_hazard_ref")
+    cs = hazard_cases()
+    nsteps, every = int(f["nsteps"]), int(f["plot_every"])
+    with BatchModel(cs) as b:
+        for _ in range(nsteps // every):
+            b.time_integration_sph(every)
+            b.pixel_accumulate()
+        hz = b.hazard(0.5)
+    node = f["node"]
+    reached = np.flatnonzero(hz["nprob"] > 0)
+    np.testing.assert_array_equal(reached, node)
+    np.testing.assert_array_equal(hz["nprob"][node], f["nprob"])
+    pr = f["mc_pr"]
+    scale = np.max(np.abs(pr[:, 2, :]), axis=0)
+    for q in range(4):
+        err_m = np.max(np.abs(hz["mean"][node, q] - pr[:, 2, q])) / scale[q]
+        err_s = np.max(np.abs(hz["std"][node, q] - pr[:, 3, q])) / scale[q]
+        assert err_m <= 1e-10 and err_s <= 1e-8, (q, err_m, err_s)

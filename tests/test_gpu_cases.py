@@ -121,3 +121,62 @@ def test_case_table_batch_equals_each_problem(tmp_path):
             np.testing.assert_array_equal(per[p][key], st[key], err_msg=f"problem {p} {key}")
         np.testing.assert_array_equal(pix[p][1], c1)
         np.testing.assert_allclose(pix[p][0], a1, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("tname", ["t2", "t1"])
+def test_case_table_batch_matches_reference_main(tname, tmp_path):
+    """MASTER.dat's case tables as one batch against the reference's own main
+    program (tests/golden/g_case_driver_ref.npz, recorded step by step from
+    PROGRAM SPH_FEM_Driver_2019): steps per problem, stop flags, time_sph and
+    the final states.  With ic_cases_mat 1 the reference starts each problem
+    from the dt_sph the previous one ended with (its mode-1 loop does not
+    reset it), a sequential dependence the batch does not imitate: every
+    realisation starts from MASTER.dat's dt_sph, as the reference's
+    ic_cases_mat >= 2 loop does.  There, problem 1 is checked against the
+    reference and the others against the oracle's loop started the same
+    way."""
+    import os
+    import sys
+
+    from gfsph import Inputs
+    from oracle import refio
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_golden import CASE_TABLES
+    from test_oracle_case_driver import problems
+
+    f = goldens.load("g_case_driver_ref")
+    rows, opts = CASE_TABLES[tname]
+    case, _ = goldens.case_of("g_voellmy_plane")
+    d = tmp_path / "run"
+    d.mkdir()
+    refio.write_inputs(str(d), case, name="case", topo="topo")
+    refio.write_master(str(d), "case", case.params.dt, case_rows=rows, **opts)
+    with Inputs(str(d), "case", "topo") as inp:
+        with inp.batch(pixels=False) as b:
+            b.run_cases(10000, chunk=8)
+            stp, its, k = b.case_status()
+            per = b.split(b.state())
+            tb, _ = b.time()
+    nprob = int(f[f"{tname}_nprob"])
+    assert k == nprob == len(per)
+    od = tmp_path / "oracle"
+    od.mkdir()
+    cs, _ = problems(od, rows, opts)
+    time_plot = opts["plot_step"] * opts["dt_driver"] if opts["cases"] == 2 else 0.0
+    for p in range(nprob):
+        key = f"{tname}_p{p + 1}"
+        if opts["cases"] == 1 and p > 0:
+            st = oracle.init(cs[p])
+            it, stopped, _, _ = oracle.run_case(cs[p], st, *opts["stop"], opts["dt_driver"], opts["time_end"],
+                                                time_plot, max_steps=10000, pixels=False)
+            want = {"iters": it, "stop": int(stopped), "time": float(st.dt_time[1]),
+                    **{fl: getattr(st, fl) for fl in ("x", "v", "h", "u", "flag")}}
+        else:
+            want = {"iters": int(f[key + "_iters"]), "stop": int(f[key + "_stop"][-1]),
+                    "time": float(f[key + "_time"][-1]), **{fl: f[key + "_" + fl] for fl in ("x", "v", "h", "u", "flag")}}
+        assert (int(its[p]), int(stp[p])) == (want["iters"], want["stop"]), (p, its[p], stp[p])
+        assert tb[p] == pytest.approx(want["time"], rel=2e-15), p
+        for fl in ("x", "v", "h", "u"):
+            err = goldens.rel_err(per[p][fl], want[fl])
+            assert err <= 1e-10, (p, fl, err)
+        np.testing.assert_array_equal(per[p]["flag"], want["flag"])
