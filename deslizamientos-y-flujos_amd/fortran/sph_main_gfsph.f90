@@ -15,11 +15,19 @@
 !
 !  The first call uploads the state (already initialised by Init_sph, so
 !  init_input = 0) and the DEM with the reference's own nodal slopes
-!  topol(1:9,:); every call advances one step and copies the particle state
-!  back into the module arrays, so the reference's output routines
-!  (Out_Plot_SPH, Out_Save_SPH, Manage_pixel_Hazard_SPH) see the current
-!  state.  The build must use the same real kind as the reference modules
-!  (irk = double): -fdefault-real-8.
+!  topol(1:9,:); every call advances one step.  The particle state stays on
+!  the GPU between steps and is copied back into the module arrays only when
+!  the driver is about to read them: when its print, save or plot counter
+!  fires after this step (t_print_reset >= time_print, t_save_reset >=
+!  time_save, t_plot_reset >= time_plot -- Driver_Flow and Driver_Flow_23
+!  advance the counters before the step, "8 SPH_FEM_Driver_2019.f90":1578-1580,
+!  1488-1492, and test them after it, 1663-1686, 1500-1525), in a case-table
+!  run (stop_case_SPH reads vx and If_Out_Domain after every step), and on
+!  the step after which the driver's loop ends (time reaching time_end,
+!  maxtimestep).  A caller that reads the arrays at other times calls
+!  gfsph_sync_state first, or sets gfsph_pull_every_step.  The build must use
+!  the same real kind as the reference modules (irk = double):
+!  -fdefault-real-8.
 !
 !  With ic_adapt_dt_sph /= 0 the context performs Adaptive_dt_sph_SW
 !  ("6 SPH_SW_2019.f90":6521-6664) on the device at the start of every step
@@ -27,12 +35,12 @@
 !  own "call adaptive_dt_sph" (8:1590) and keeps "time_sph = time_sph + dt_sph".
 !
 !  With a pore-pressure column (icpwp = 2) the whole Uaux(1:nUaux,:) goes to
-!  the context at the first call and comes back after every call.
+!  the context at the first call and comes back with the state.
 !
 !  With erosion (ic_erosion = 1) the context starts from the terrain's
 !  erosion state topol(1), topol(14), topol(15) as Input_SW left it (restart
-!  included), and every call copies back the masses and that state as
-!  ExtForces_SW(1) updates them ("6 SPH_SW_2019.f90":8724-8806).
+!  included), and the masses and that state come back with the particle
+!  state as ExtForces_SW(1) updates them ("6 SPH_SW_2019.f90":8724-8806).
 
 module SPH_MAIN_GFSPH
 
@@ -50,8 +58,11 @@ private
 type(c_ptr), save :: ctx = c_null_ptr
 integer, save, public :: gfsph_integrator = 4     ! 4: RK4 as RK45 (7:746); 2: RK2
 integer, save, public :: gfsph_status = 0
+logical, save, public :: gfsph_pull_every_step = .false.
+integer, save, public :: gfsph_pulls = 0          ! copies back so far (diagnostics)
+logical, save :: stale = .false.                  ! module arrays behind the GPU state
 
-public :: time_integration_sph_gfsph, gfsph_release
+public :: time_integration_sph_gfsph, gfsph_release, gfsph_sync_state
 
 contains
 
@@ -68,9 +79,33 @@ subroutine time_integration_sph_gfsph
       call check(rc, 'gfsph_time')
       dt_sph = dtnow(1)
    endif
-   call pull_state
+   stale = .true.
+   if (driver_reads_state()) call gfsph_sync_state
 
 end subroutine time_integration_sph_gfsph
+
+
+! Whether the driver reads the particle arrays after the step now ending:
+! an output counter fires, a case table's stop test runs, or its loop ends.
+logical function driver_reads_state()
+   driver_reads_state = gfsph_pull_every_step                  &
+        .or. t_print_reset .ge. time_print                     &
+        .or. t_save_reset  .ge. time_save                      &
+        .or. t_plot_reset  .ge. time_plot                      &
+        .or. ic_cases_mat  .ge. 1                              &
+        .or. time + dt_sph .ge. time_end                       &
+        .or. itimestep     .ge. nstart + maxtimestep
+end function driver_reads_state
+
+
+! Copies the GPU state back into the module arrays if they are behind it.
+subroutine gfsph_sync_state
+   if (stale .and. c_associated(ctx)) then
+      call pull_state
+      gfsph_pulls = gfsph_pulls + 1
+   endif
+   stale = .false.
+end subroutine gfsph_sync_state
 
 
 subroutine create_context
@@ -201,6 +236,7 @@ end subroutine pull_erosion
 
 
 subroutine gfsph_release
+   call gfsph_sync_state
    if (c_associated(ctx)) call gfsph_destroy(ctx)
    ctx = c_null_ptr
 end subroutine gfsph_release

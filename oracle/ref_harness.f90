@@ -57,7 +57,8 @@ use SPH_SW_2019, only: OutputRes_SW, Get_MC_pixel_Vars_SW, Norm_MC_pixel_Vars_SW
 use SPH_FEM_topo_2019, only: npoigx, npoigy, Xming, Xmaxg, Yming, Ymaxg, deltxg, deltyg, &
                              xshiftg, yshiftg, topol, poig_in_model
 #ifdef GFSPH_GPU
-use SPH_MAIN_GFSPH, only: time_integration_sph_gfsph, gfsph_release
+use SPH_MAIN_GFSPH, only: time_integration_sph_gfsph, gfsph_release, gfsph_sync_state, gfsph_pulls, &
+                          gfsph_pull_every_step
 #endif
 
 implicit none
@@ -69,6 +70,8 @@ integer, allocatable :: dump_steps(:)
 real(8)        dtin
 integer        nprob, plot_every, iprob, ios
 character(16), allocatable :: pnames(:)
+integer(8)     clk0, clk1, clkrate, cd0, cd1, clkdump
+character(8)   envv
 
 open (newunit=ictl, file='harness.ctl', status='old')
 read (ictl,'(a16)') name
@@ -139,6 +142,12 @@ if (iprob.eq.1) then
    call maybe_dump (0)
    call write_topo
 endif
+#ifdef GFSPH_GPU
+call get_environment_variable('GFSPH_PULL_EVERY_STEP', envv)
+gfsph_pull_every_step = envv(1:1).eq.'1'
+#endif
+call system_clock (clk0, clkrate)
+clkdump = 0
 do istep = 1, nsteps
    t_print_reset = t_print_reset + dt
 #ifndef GFSPH_GPU
@@ -152,12 +161,21 @@ do istep = 1, nsteps
 #endif
    time_sph = time_sph + dt_sph
    time     = time + dt
+   call system_clock (cd0)
    if (nprob.eq.0.or.iprob.eq.nprob) call maybe_dump (istep)
+   call system_clock (cd1)
+   clkdump = clkdump + (cd1 - cd0)
    if (nprob.gt.0) then
       if (mod(istep, plot_every).eq.0) call Get_MC_pixel_Vars_SW
    endif
 enddo
 if (nprob.gt.0) call Norm_MC_pixel_Vars_SW (1)
+#ifdef GFSPH_GPU
+call gfsph_sync_state
+#endif
+call system_clock (clk1)
+write (*,'(a,i0,a,es12.5)') ' harness: ', nsteps, ' steps (dumps excluded), seconds ', &
+      real(clk1-clk0-clkdump,8)/real(clkrate,8)
 enddo
 if (nprob.gt.0) call write_mc_pixel
 !      ------  with gid_final.ctl present: the GiD result blocks of the final
@@ -165,6 +183,7 @@ if (nprob.gt.0) call write_mc_pixel
 inquire (file='gid_final.ctl', exist=gid_final)
 if (gid_final) call OutputRes_SW
 #ifdef GFSPH_GPU
+write (*,'(a,i0)') ' gfsph: state copies back to the host: ', gfsph_pulls
 call gfsph_release
 #endif
 
@@ -212,7 +231,12 @@ subroutine maybe_dump (is)
    integer, intent(in) :: is
    integer k2
    do k2 = 1, ndump
-      if (dump_steps(k2).eq.is) call write_dump (is)
+      if (dump_steps(k2).eq.is) then
+#ifdef GFSPH_GPU
+         call gfsph_sync_state
+#endif
+         call write_dump (is)
+      endif
    enddo
 end subroutine maybe_dump
 

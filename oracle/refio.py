@@ -24,6 +24,10 @@ REF_EXE = os.path.join(HERE, "_ref", "gfsph_ref")
 REF_GPU_EXE = os.path.join(HERE, "_ref", "gfsph_ref_gpu")
 
 
+# stdout of the last harness run (the harness prints its step-loop time)
+LAST_STDOUT = ""
+
+
 def available() -> bool:
     return os.path.exists(REF_EXE)
 
@@ -246,7 +250,7 @@ def run_reference_files(src_dir: str, name: str, files, nsteps: int, dt: float, 
 
 
 def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, timeout: float = 3600.0,
-                  keep: bool = False, exe: str = REF_EXE, gid_final: bool = False) -> dict:
+                  keep: bool = False, exe: str = REF_EXE, gid_final: bool = False, env: dict | None = None) -> dict:
     """Run the reference on ``case`` for ``nsteps`` RK4 steps; return
     {step: dump}.  Raises if the reference build is absent."""
     if not os.path.exists(exe):
@@ -263,7 +267,9 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
     if gid_final:  # the reference's GiD writer also runs after the last step
         open(os.path.join(tmp, "gid_final.ctl"), "w").close()
     r = subprocess.run([exe], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
-                       timeout=timeout)
+                       timeout=timeout, env=env)
+    global LAST_STDOUT
+    LAST_STDOUT = r.stdout
     if r.returncode != 0:
         raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
     out = {s: read_dump(os.path.join(tmp, f"dump_{s}.bin")) for s in steps}

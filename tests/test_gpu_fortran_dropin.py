@@ -38,3 +38,64 @@ def test_fortran_driver_with_gpu_step_matches_reference(name, tmp_path):
             assert goldens.rel_err(out[s]["mass"], f[f"s{s}_mass"]) <= tol
             for j in range(3):
                 assert goldens.rel_err(out[s]["eros"][:, j], f[f"s{s}_eros"][:, j]) <= max(tol, 1e-12), (name, s, j)
+
+
+def _pulls(stdout):
+    for line in stdout.splitlines():
+        if "state copies back to the host:" in line:
+            return int(line.split(":")[-1])
+    raise AssertionError("the harness did not report its copies")
+
+
+def _loop_seconds(stdout):
+    for line in stdout.splitlines():
+        if line.strip().startswith("harness:") and "seconds" in line:
+            return float(line.split()[-1])
+    raise AssertionError("the harness did not report its step-loop time")
+
+
+@pytest.mark.skipif(not os.path.exists(refio.REF_GPU_EXE),
+                    reason="oracle/_ref/gfsph_ref_gpu is built only where the reference sources are present")
+def test_fortran_layer_copies_the_state_back_only_when_read(tmp_path):
+    """The layer copies the particle state to the module arrays only when the
+    driver reads them (here: the harness's dump steps), and the dumps still
+    equal the reference's."""
+    case, f = goldens.case_of("g_frictional_pwp")
+    steps = [int(s) for s in f["dumps"] if s > 0]
+    out = refio.run_reference(case, case.nsteps, steps, workdir=str(tmp_path), exe=refio.REF_GPU_EXE)
+    assert _pulls(refio.LAST_STDOUT) == len(set(steps))
+    s = max(steps)
+    for k in ("x", "v", "h", "u"):
+        assert goldens.rel_err(out[s][k], f[f"s{s}_{k}"]) <= 1e-10
+
+
+@pytest.mark.skipif(not os.path.exists(refio.REF_GPU_EXE),
+                    reason="oracle/_ref/gfsph_ref_gpu is built only where the reference sources are present")
+def test_fortran_layer_timing_200k(tmp_path):
+    """The reference's driver loop through the Fortran layer at 200,000
+    particles (the fractal release on a 512 x 512 DEM at 2 m, frictional +
+    pore pressure, RK4 dt 0.02 s, 200 steps): the loop time with the state
+    kept on the GPU against the same loop copying it back after every step
+    (the layer's former behaviour).  Both runs end in the same state."""
+    from gfsph import cases as C
+    from gfsph.dem import fractal_dem
+
+    dem = fractal_dem(512, 2.0, 1002)
+    L = (dem.nx - 1) * dem.dx
+    parts = C.elliptical_scar(0.3 * L, 0.5 * L, 200_000, 1.0, 2.0, 2002)
+    prm = C.Params(integrator=4, dt=0.02, h_inf=1.0e-3, icpwp=3, pwp0rel=0.6, const=C.frictional(28.0, cv=0.01))
+    case = C.Case("dropin200k", prm, dem, parts, 200)
+    res = {}
+    for mode in ("0", "1"):
+        d = tmp_path / f"m{mode}"
+        d.mkdir()
+        env = {**os.environ, "GFSPH_PULL_EVERY_STEP": mode}
+        out = refio.run_reference(case, 200, [200], workdir=str(d), exe=refio.REF_GPU_EXE, env=env)
+        res[mode] = (_loop_seconds(refio.LAST_STDOUT), _pulls(refio.LAST_STDOUT), out[200])
+    lazy, every = res["0"], res["1"]
+    print(f"\nFortran layer, 200k particles x 200 steps: {1e3 * lazy[0] / 200:.3f} ms/step with {lazy[1]} copies "
+          f"back; {1e3 * every[0] / 200:.3f} ms/step copying back after every step ({every[1]} copies)")
+    assert lazy[1] == 1 and every[1] == 200
+    for k in ("x", "v", "h", "u"):
+        np.testing.assert_array_equal(lazy[2][k], every[2][k])
+    assert lazy[0] < every[0]

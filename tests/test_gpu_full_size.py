@@ -50,8 +50,8 @@ def test_gpu_cfg1_divergence_growth():
     particles.  Up to step 100 the GPU stays within max(1e-10, 2 x spread);
     the whole field at step 100 likewise.  Beyond step 200 the trajectories
     of the two reference builds themselves decorrelate; there the flow's
-    integral measures (centroid, mean speed, extent, mean depth) are held to
-    the FMA build's distance from the reference."""
+    integral measures (centroid, mean speed, front and tail, mean depth) are
+    compared on the flow's own scale."""
     g = goldens.load("g_cfg1_growth")
     steps = [int(s) for s in g["steps"]]
     idx = g["idx"]
@@ -87,14 +87,21 @@ def test_gpu_cfg1_divergence_growth():
             for k in FIELDS:
                 assert full[k] <= max(BAR, SPREAD_FACTOR * sp[k]), (s, k, full[k], sp[k])
         if s >= 200:
-            # centroid x, y; mean speed; front and tail x; mean depth (not the count)
-            scale = np.array([1.0, 1.0, max(abs(iref[2]), 1e-3), 1.0, 1.0, max(abs(iref[5]), 1e-3)])
+            # The two reference builds' trajectories have decorrelated; the
+            # flow is compared by integral measures instead (centroid x, y;
+            # mean speed; front and tail x; mean depth), each within a
+            # tolerance on the flow's own scale (E = front - tail), or the
+            # FMA build's distance if larger.  Errors in the physics show up
+            # here as metres of runout; rounding shows up as centimetres.
+            E = iref[3] - iref[4]
+            tol = np.array([2e-3 * E, 2e-3 * E, max(0.05 * abs(iref[2]), 1e-3), 1e-2 * E, 1e-2 * E,
+                            1e-2 * abs(iref[5])])
             d_gpu = np.abs(ints[:6] - iref[:6])
             d_fma = np.abs(ifma[:6] - iref[:6])
-            print(f"step {s}: integral distances gpu {np.array2string(d_gpu, precision=3)} "
+            print(f"step {s}: extent {E:.1f} m; integral distances gpu {np.array2string(d_gpu, precision=3)} "
                   f"fma {np.array2string(d_fma, precision=3)}")
-            assert np.all(d_gpu <= np.maximum(4.0 * d_fma, 0.02 * scale)), (s, d_gpu, d_fma)
-            assert ints[6] == iref[6], s  # every particle stays in the domain
+            assert np.all(d_gpu <= np.maximum(tol, d_fma)), (s, d_gpu, d_fma, tol)
+            assert ints[6] == iref[6], s  # the same number of particles left in the domain
 
 
 def test_gpu_cfg2_full_size_reference():
@@ -160,8 +167,8 @@ def test_gpu_cfg3_full_batch_equals_single_runs():
     """All 256 realisations (5k-80k particles each, 7.4M in total) on the
     1024 x 1024 DEM as one batch for 100 steps with pixel statistics every
     20: the smallest, the median and the largest realisation and two
-    random ones equal their runs alone bit for bit (state and pixel
-    accumulators)."""
+    random ones equal their runs alone bit for bit (state, pixel counts and
+    maxima; the pixel sums to the rounding of their atomic order)."""
     cs = C.cfg_mc_hazard(R=256, nsteps=100)
     counts = np.array([c.particles.n for c in cs])
     order = np.argsort(counts)
@@ -186,7 +193,10 @@ def test_gpu_cfg3_full_batch_equals_single_runs():
         for k in ("x", "v", "h", "u", "hsml", "flag"):
             np.testing.assert_array_equal(per[r][k], st[k], err_msg=f"realisation {r} {k}")
         np.testing.assert_array_equal(pix[r][1], cnt)
-        np.testing.assert_array_equal(pix[r][0], acc)
+        # counts and maxima are exact; the sums are accumulated with hardware
+        # FP64 atomics, whose order (so the last bit) differs run to run
+        np.testing.assert_array_equal(pix[r][0][:, [1, 3]], acc[:, [1, 3]])
+        np.testing.assert_allclose(pix[r][0][:, [0, 2]], acc[:, [0, 2]], rtol=1e-13, atol=0)
 
 
 def _adjoint_loss(case, target, nsteps):
