@@ -32,6 +32,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -4642,6 +4643,134 @@ int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double 
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free neighbour set (gfsph_neighbour_pairs_noalloc): the same
+// cell-list build and predicate as gfsph_neighbour_pairs, on caller-owned
+// device arrays, with every intermediate buffer carved out of one file-scope
+// device array.  Each GPU holds its own copy of the array for the lifetime of
+// the process; calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+#define NOALLOC_MAX_N 262144
+#define NOALLOC_CELLS (4 * NOALLOC_MAX_N + 4096)
+#define NOALLOC_SCAN_BLOCKS ((NOALLOC_CELLS + 2 + SCAN_TILE - 1) / SCAN_TILE)
+#define NOALLOC_PARTS ((((NOALLOC_MAX_N + GF_BLOCK - 1) / GF_BLOCK) + 7) / 8 * 8)
+#define NA_AL(b) (((size_t)(b) + 255) / 256 * 256)
+// byte offsets of the pieces (each 256-aligned)
+#define NA_ZERO 0
+#define NA_KEY (NA_ZERO + NA_AL(4ull * NOALLOC_MAX_N))
+#define NA_RANK (NA_KEY + NA_AL(4ull * NOALLOC_MAX_N))
+#define NA_PERM (NA_RANK + NA_AL(4ull * NOALLOC_MAX_N))
+#define NA_CNT (NA_PERM + NA_AL(4ull * NOALLOC_MAX_N))
+#define NA_START (NA_CNT + NA_AL(4ull * (NOALLOC_CELLS + 2)))
+#define NA_BSUM (NA_START + NA_AL(4ull * (NOALLOC_CELLS + 2)))
+#define NA_PART (NA_BSUM + NA_AL(4ull * NOALLOC_SCAN_BLOCKS))
+#define NA_ACC (NA_PART + NA_AL(sizeof(BBoxAcc) * NOALLOC_PARTS))
+#define NA_GRID (NA_ACC + NA_AL(sizeof(BBoxAcc)))
+#define NA_COFF (NA_GRID + NA_AL(sizeof(GridInfo)))
+#define NA_BYTES (NA_COFF + NA_AL(4 * 2))
+
+// the scratch memory of gfsph_neighbour_pairs_noalloc (about 13 MB)
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+__global__ void k_fill_i32(long n, int v, int *p) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long *p) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = 0ull;
+}
+
+// Pairs i < j (caller indices) from the counting sort of the caller's
+// unsorted arrays: slot -> caller index through perm.  The predicate and
+// its argument order (the smaller index first) are those of k_pairs.
+__global__ void k_pairs_perm(int n, int scale_k, const GridInfo *grid, const int *start, const int *perm,
+                             const double2 *pos, const double *hsml, const int *flag, long long cap,
+                             unsigned long long *count, int *ij) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || flag[i]) return;
+  const GridInfo g = *grid;
+  const double2 pi_ = pos[i];
+  const double hi = hsml[i];
+  for_candidates(g, start, pi_, [&](int slot) {
+    const int j = perm[slot];
+    if (j <= i) return;
+    double dx, dy, r, mh;
+    if (pair_pred(scale_k, pi_, pos[j], hi, hsml[j], dx, dy, r, mh)) {
+      const unsigned long long k = atomicAdd(count, 1ull);
+      if ((long long)k < cap) {
+        ij[2 * k] = i;
+        ij[2 * k + 1] = j;
+      }
+    }
+  });
+}
+
+// device address of scratch, per device: taken once, with that device current
+static std::mutex noalloc_mu;
+static unsigned char *noalloc_ptr[64] = {};
+static thread_local std::string noalloc_err;
+
+static int noalloc_fail(int code, const std::string &msg) {
+  noalloc_err = msg;
+  return code;
+}
+
+static int noalloc_scratch(unsigned char **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return noalloc_fail(GFSPH_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= 64) return noalloc_fail(GFSPH_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return noalloc_fail(GFSPH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    noalloc_ptr[dev] = (unsigned char *)p;
+  }
+  *out = noalloc_ptr[dev];
+  return GFSPH_OK;
+}
+
+int gfsph_neighbour_pairs_noalloc(int32_t skf, int32_t n, const double *x, const double *hsml,
+                                  const int32_t *flag, int64_t cap, uint64_t *npairs, int32_t *ij, void *stream) {
+  if (n <= 0 || !x || !hsml || !npairs || skf < 1 || skf > 4 || cap < 0 || (cap > 0 && !ij))
+    return noalloc_fail(GFSPH_ERR_ARG, "gfsph_neighbour_pairs_noalloc: invalid argument");
+  if (n > NOALLOC_MAX_N)
+    return noalloc_fail(GFSPH_ERR_CAPACITY, "gfsph_neighbour_pairs_noalloc: n = " + std::to_string(n) +
+                                                " exceeds its maximum of " + std::to_string(NOALLOC_MAX_N) +
+                                                " particles (gfsph_neighbour_pairs has no limit)");
+  unsigned char *base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  int *zero = (int *)(base + NA_ZERO), *key = (int *)(base + NA_KEY), *rank = (int *)(base + NA_RANK);
+  int *perm = (int *)(base + NA_PERM), *cnt = (int *)(base + NA_CNT), *start = (int *)(base + NA_START);
+  int *bsum = (int *)(base + NA_BSUM), *coff = (int *)(base + NA_COFF);
+  BBoxAcc *part = (BBoxAcc *)(base + NA_PART), *acc = (BBoxAcc *)(base + NA_ACC);
+  GridInfo *grid = (GridInfo *)(base + NA_GRID);
+  const int scale_k = (skf == 1 || skf == 4) ? 2 : 3;
+  const int nb = nblk(n), gridN = (nb + 7) / 8 * 8;
+  unsigned long long *count = (unsigned long long *)npairs;
+  // the array keeps the last call's contents: clear what must start zero
+  k_fill_i32<<<nb, GF_BLOCK, 0, st>>>(n, 0, zero);
+  k_fill_i32<<<(int)((NOALLOC_CELLS + 2 + GF_BLOCK - 1) / GF_BLOCK), GF_BLOCK, 0, st>>>(NOALLOC_CELLS + 2, 0, cnt);
+  k_zero_u64<<<1, 64, 0, st>>>(count);
+  const int *fl = flag ? flag : zero;
+  k_bbox<<<gridN, GF_BLOCK, 0, st>>>(n, (const double2 *)x, hsml, fl, part);
+  k_grid<<<1, GF_BLOCK, 0, st>>>(gridN, part, acc, grid, coff, scale_k, NOALLOC_CELLS);
+  k_keys<<<nb, GF_BLOCK, 0, st>>>(n, (const double2 *)x, fl, zero, grid, coff, coff + 1, key, rank, cnt);
+  k_scan_reduce<<<NOALLOC_SCAN_BLOCKS, GF_BLOCK, 0, st>>>(coff + 1, cnt, bsum);
+  k_scan_down<<<NOALLOC_SCAN_BLOCKS, GF_BLOCK, 0, st>>>(coff + 1, cnt, bsum, start, n);
+  k_scatter<<<nb, GF_BLOCK, 0, st>>>(n, key, rank, start, perm);
+  k_pairs_perm<<<nb, GF_BLOCK, 0, st>>>(n, scale_k, grid, start, perm, (const double2 *)x, hsml, fl, cap, count,
+                                        ij);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return noalloc_fail(GFSPH_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return GFSPH_OK;
+}
+
 int gfsph_raster_idw(gfsph_ctx *c, double *out, double *wsum) {
   if (!c || !out || !wsum) return GFSPH_ERR_ARG;
   long np = (long)c->dem.nx * c->dem.ny;
@@ -5227,7 +5356,7 @@ int gfsph_batch_status(gfsph_ctx *c, int32_t *stopped, int32_t *nsteps, int32_t 
   return GFSPH_OK;
 }
 
-const char *gfsph_last_error(gfsph_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *gfsph_last_error(gfsph_ctx *c) { return c ? c->err.c_str() : noalloc_err.c_str(); }
 
 void gfsph_destroy(gfsph_ctx *c) {
   if (!c) return;

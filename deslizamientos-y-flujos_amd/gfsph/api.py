@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy", "gfsph_write_netcdf",
     "gfsph_batch_cases", "gfsph_batch_status", "gfsph_inputs_cases", "gfsph_inputs_batch_create",
     "gfsph_get_mass", "gfsph_erosion_read", "gfsph_erosion_set", "gfsph_get_pwp_column", "gfsph_set_pwp_column",
+    "gfsph_neighbour_pairs_noalloc",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -96,6 +97,8 @@ def load_library(path: str | None = None):
     L.gfsph_sync.argtypes = [vp]
     L.gfsph_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp, ip]
     L.gfsph_neighbour_pairs.argtypes = [C.c_int32, C.c_int32, dp, dp, ip, C.c_int64, C.POINTER(C.c_int64), ip]
+    # device pointers and a hipStream_t, passed as addresses
+    L.gfsph_neighbour_pairs_noalloc.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_int64, vp, vp, vp]
     L.gfsph_raster_idw.argtypes = [vp, dp, dp]
     L.gfsph_mc_pixel_accumulate.argtypes = [vp]
     L.gfsph_mc_pixel_read.argtypes = [vp, dp, ip, C.c_int32]

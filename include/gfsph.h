@@ -143,6 +143,21 @@ int gfsph_erosion_set(gfsph_ctx *ctx, int32_t r, const double *z, const double *
 int gfsph_neighbour_pairs(int32_t skf, int32_t n, const double *x, const double *hsml,
                           const int32_t *flag, int64_t cap, int64_t *npairs, int32_t *ij);
 
+/* The neighbour set of gfsph_neighbour_pairs without allocating: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  x [n][2], hsml [n], flag [n] (NULL: all active)
+ * and the outputs are caller-owned DEVICE memory; the work is enqueued on
+ * stream (a hipStream_t; NULL: the null stream) and nothing is waited for.
+ * *npairs (device) receives the number of pairs (i < j, caller indices);
+ * the first min(*npairs, cap) are written to ij [cap][2], in no particular
+ * order.  Every intermediate buffer lives in one static device array of
+ * about 13 MB per GPU, held until the process ends: n is limited to
+ * 262,144 (GFSPH_ERR_CAPACITY beyond, with a message in
+ * gfsph_last_error(NULL)), and calls that overlap on one GPU race. */
+int gfsph_neighbour_pairs_noalloc(int32_t skf, int32_t n, const double *x, const double *hsml,
+                                  const int32_t *flag, int64_t cap, uint64_t *npairs, int32_t *ij, void *stream);
+
 /* Particle-to-DEM-node inverse-distance raster of OutputRes_WIR
  * ("6 SPH_SW_2019.f90":14312-14466): out[ny*nx][4] = h, vx, vy, u/h,
  * wsum[ny*nx] = sum of 1/d. */
@@ -359,6 +374,8 @@ int gfsph_write_netcdf(const char *path, int32_t nx, int32_t ny, double xmin, do
                        int32_t nvar, const char *const *names, const char *const *units,
                        const char *const *long_names, const double *const *data, double fill, const char *title);
 
+/* The message of the last failure on ctx; with ctx = NULL, of the calling
+ * thread's last failed context-free call (gfsph_neighbour_pairs_noalloc). */
 const char *gfsph_last_error(gfsph_ctx *ctx);
 void gfsph_destroy(gfsph_ctx *ctx);
 
