@@ -431,7 +431,13 @@ __device__ __forceinline__ double erosion_rate(const DevParams &P, double h, dou
 // ExtForces_SW(1)) exdvxdt is not cleared, so (acx, acy) -- the particle's
 // exdvxdt of the last stage -- is where the gravity term is added and what
 // the limiter reads, and only the rate is wanted (er).
-template <bool EROS = false>
+// LAW = 5: compiled for nfric = 5 alone (frictional with optional Voellmy
+// turbulence, no curvature term, g vertical) -- every realisation of the
+// context has that law, so the other laws' code and registers are left out;
+// LAW = 0: every law, chosen at run time.
+#define LAW_ANY 0
+#define LAW_FRIC5 5
+template <bool EROS = false, int LAW = LAW_ANY>
 __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem, double xx, double yy,
                                           double vx1, double vy1, double rho_i, double &u_i,
                                           double uaux1, double inx, double iny, double tau_in, double &exo,
@@ -467,8 +473,9 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   TopoPt tp;
   // the curvature terms are interpolated only when the law uses them (V^2/R);
   // the reference interpolates them for every frictional law (6:9257-9259)
-  get_z_topo(dem, xx, yy, P.if_V2byR != 0, tp);
-  if (P.costheta) {
+  constexpr bool F5 = LAW == LAW_FRIC5;
+  get_z_topo(dem, xx, yy, !F5 && P.if_V2byR != 0, tp);
+  if (!F5 && P.costheta) {
     const double deng = sqrt(1. + tp.Zx * tp.Zx + tp.Zy * tp.Zy);
     gravz = P.cgra / deng;
     exx = exx + (-P.cgra * tp.Zx / deng);
@@ -482,21 +489,21 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   else if (P.icpwp == 2 || P.icpwp == 3) pwp = uaux1;  // Uaux(1,:), the bed value
   else pwp = 0.0;
   hfrict = fmax(h, P.hfrict0);
-  const int nfric = P.nfric;
+  const int nfric = F5 ? 5 : P.nfric;
 
-  if (nfric == 1) {
+  if (!F5 && nfric == 1) {
     double cf1 = P.cgra * xi_voellmy * xi_voellmy;
     double hp = pow(hfrict, P.c73);
     frictx = cf1 * hu * hvmod / hp;
     fricty = cf1 * hv * hvmod / hp;
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
-  } else if (nfric == 2) {
+  } else if (!F5 && nfric == 2) {
     double cf2a = P.constK / (8. * P.dens);
     double h2 = (hfrict * hfrict);
     frictx = cf2a * hu / h2;
     fricty = cf2a * hv / h2;
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
-  } else if (nfric == 4 || nfric == 24 || nfric == 25) {
+  } else if (!F5 && (nfric == 4 || nfric == 24 || nfric == 25)) {
     const double Tauy0 = P.tauy0, constK = P.constK, m_vexp = P.visco;
     if (nfric == 4) {
       // every regime ends with cf4b = Tau0/dens (6:9619-9622); Tau0 is this
@@ -516,9 +523,9 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       fricty = cf4b * hv * hvmodinv;
     }
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
-  } else if (P.if_frictional == 1) {
+  } else if (F5 || P.if_frictional == 1) {
     double cv2byR = 0.0, tanfi;
-    if (P.if_V2byR == 1) {
+    if (!F5 && P.if_V2byR == 1) {
       vxx = hu * hinv;
       vyy = hv * hinv;
       vmod = sqrt(vxx * vxx + vyy * vyy);
@@ -531,7 +538,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       }
     }
     tanfi = P.tanfi8;
-    if (nfric == 8 || nfric == 26 || nfric == 27) {
+    if (!F5 && (nfric == 8 || nfric == 26 || nfric == 27)) {
       xi_voellmy = 0.0;
       double Froude = vmod / sqrt(gravz * h);
       if (nfric == 8 || nfric == 26) {
@@ -573,7 +580,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       frictx = frictx + P.constMu_CF * vmod * vxx / (hfrict * hfrict);
       fricty = fricty + P.constMu_CF * vmod * vyy / (hfrict * hfrict);
     }
-    if (nfric == 9 || nfric == 10) {
+    if (!F5 && (nfric == 9 || nfric == 10)) {
       double C67 = P.c6 + P.c7;
       double frict_abs = sqrt(frictx * frictx + fricty * fricty);
       double fact9 = pow(2. * P.c8 * vmod / hfrict, 1. / C67) * pow(frict_abs, -P.c6 / C67);
@@ -582,7 +589,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       fricty = fricty * (1 + fact9);
     }
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
-  } else if (nfric == 11) {
+  } else if (!F5 && nfric == 11) {
     vxx = hu * hinv;
     vyy = hv * hinv;
     frictx = P.K_Bagnold * vxx * vmod / (hfrict * hfrict);
@@ -611,7 +618,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     double accelT = sqrt(ax * ax + ay * ay);
     if (accelT > 1.e-5) {
       double ctf = TotFrict;
-      if (P.if_frictional == 1) ctf = cf5;
+      if (F5 || P.if_frictional == 1) ctf = cf5;
       frictx = ctf * ax / accelT;
       fricty = ctf * ay / accelT;
       double fricT = sqrt(frictx * frictx + fricty * fricty);
@@ -1331,7 +1338,7 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 #ifndef RK_WAVES_PER_SIMD
 #define RK_WAVES_PER_SIMD 1
 #endif
-template <bool EROS>
+template <bool EROS, int LAW>
 __global__ __launch_bounds__(GF_BLOCK, RK_WAVES_PER_SIMD) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
                                                   const double *S, const double2 *ain, NewSet o, BBoxAcc *part,
                                                   const int *stopr, ErosGrid eg) {
@@ -1376,7 +1383,7 @@ __global__ __launch_bounds__(GF_BLOCK, RK_WAVES_PER_SIMD) void k_rk(int n, const
           flagged = true;
           break;
         }
-        ext_force<EROS>(P, dem, x.x, x.y, v.x, v.y, rfi, u, uaux1, inx, iny, tau_unused, ex, ey, du);
+        ext_force<EROS, LAW>(P, dem, x.x, x.y, v.x, v.y, rfi, u, uaux1, inx, iny, tau_unused, ex, ey, du);
         const double f2 = P.f2[st];
         Dvx += f2 * (ex + inx);
         Dvy += f2 * (ey + iny);
@@ -1602,13 +1609,14 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 #define XSPH_LB __launch_bounds__(GATHER_MAX_BLOCK)
 #endif
 
-// Every gather splits a particle's candidate stream into two halves: the
-// 4-position batches 0, 2, 4, ... and 1, 3, 5, ... (the force and XSPH passes:
-// the neighbours in those batches).  Each half is summed in stream order and
-// the result is half 0 + half 1.  A context runs the halves either on two
-// adjacent lanes (TM = 2: more loads in flight when the particles alone fill
-// each SIMD only a few waves deep) or one after the other on one lane (TM =
-// 1: large batches); both give the same bits, for every work-group size.
+// Every gather splits a particle's sum into two halves: the density pass the
+// 4-position batches 0, 2, 4, ... and 1, 3, 5, ... of its candidate stream,
+// the force and XSPH passes the first and the second half of its neighbours
+// (split_halves).  Each half is summed in stream order and the result is
+// half 0 + half 1.  A context runs the halves either on two adjacent lanes
+// (TM = 2: more loads in flight when the particles alone fill each SIMD only
+// a few waves deep) or one after the other on one lane (TM = 1: large
+// batches); both give the same bits, for every work-group size.
 #define SPLIT 2
 // particles from which one lane per particle is the default
 #ifndef TEAM1_MIN_N
@@ -1645,12 +1653,6 @@ __device__ __forceinline__ unsigned long long halves_or(unsigned long long v) {
     const unsigned lo = __shfl_xor((unsigned)v, 1), hi = __shfl_xor((unsigned)(v >> 32), 1);
     return v | ((unsigned long long)hi << 32) | lo;
   }
-}
-
-// mask bits of half h (nibbles of NB_BATCH = 4 positions)
-__device__ __forceinline__ unsigned long long split_bits(int h) {
-  static_assert(NB_BATCH == 4, "the halves are made of 4-position nibbles");
-  return 0x0F0F0F0F0F0F0F0Full << (4 * h);
 }
 
 struct CandStream {
@@ -1783,7 +1785,7 @@ __device__ __forceinline__ void load_words(const CandStream &cs, const unsigned 
 // words w of a particle in stream order; js are slots.  Unused entries of a
 // batch have ok = false and js = self (a valid index), so f may read them
 // unconditionally.  Emptied words are shifted out at batch boundaries.
-template <bool HALVES = false, typename F>
+template <typename F>
 __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned long long (&w)[NB_WORDS], int self,
                                                F &&f) {
   int pbase = 0;
@@ -1800,20 +1802,15 @@ __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned lo
     }
     int js[NB_BATCH];
     bool ok[NB_BATCH];
-    int hs[NB_BATCH];  // half of the entry: parity of its 4-position nibble
 #pragma unroll
     for (int u = 0; u < NB_BATCH; u++) {
       ok[u] = w[0] != 0ull;
       const int pos = pbase + __ffsll((long long)w[0]) - 1;
       const int sl = slot_of(cs, pos);
       js[u] = ok[u] ? sl : self;
-      hs[u] = (pos >> 2) & 1;
       w[0] &= w[0] - 1ull;
     }
-    if constexpr (HALVES)
-      f(js, ok, hs);
-    else
-      f(js, ok);
+    f(js, ok);
   }
 }
 
@@ -1833,15 +1830,90 @@ __device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, 
   }
 }
 
-// Density sum over the stream with the neighbour-mask build: every candidate
-// takes the predicate and the kernel (selected into the sum when it is a
-// neighbour); mask word k holds stream positions 64k .. 64k+63.  UNIFORM: all
-// smoothing lengths are equal, so the support radius scale_k*(hi+hj)/2 =
-// scale_k*hi and the kernel's 1/h are the particle's own.
-// BOTH: one pass over every batch, each summed into its half (acc[0], acc[1]);
-// else the batches of half h only, into acc[0].  Either way each half is
-// summed in stream order.
-template <int SKF, bool UNIFORM, bool BOTH>
+// Kernel value alone (the density and XSPH sums) for a compile-time kernel
+// type, from the squared separation dr: the polynomials of kernel_TOOLS
+// ("5a My_SPH_Tools_2019.f90":2062-2180) in nested form, one branch-free
+// select between the two pieces of the cubic spline.  Agrees with
+// kernel_TOOLS to a few ulps for q = r/h <= 2, the only values the sums use
+// (a neighbour has r < 2h); beyond q = 2 the result is not the kernel.
+template <int SKF, bool UH = false>
+__device__ __forceinline__ double sph_w_t(double kc, double pi, double dr, double hsml, double ihu = 0.0) {
+  if (SKF == 1 || SKF == 4) {
+    double r, ir;
+    root_nr(dr, r, ir);
+    const double ih = UH ? ihu : rcp_nr(hsml);
+    const double q = r * ih;
+    const double factor = kc * (ih * ih);
+    if (SKF == 1) {
+      const double w1 = fma(q * q, fma(0.5, q, -1.0), 2.0 / 3.0);  // 2/3 - q^2 + q^3/2
+      const double t = 2.0 - q;
+      const double w2 = (t * t) * (t * (1.0 / 6.0));                // (2 - q)^3 / 6
+      return factor * (q <= 1.0 ? w1 : w2);
+    } else {
+      const double b = fma(-0.5, q, 1.0);                            // 1 - q/2
+      const double b2 = b * b;
+      return factor * ((b2 * b2) * fma(2.0, q, 1.0));                // b^4 (2q + 1)
+    }
+  } else {
+    double w, gx, gy;
+    sph_kernel(SKF, pi, sqrt(dr), 0.0, 0.0, hsml, w, gx, gy);
+    return w;
+  }
+}
+
+// One batch of NB_BATCH consecutive stream positions k0 .. k0+3 of the
+// density pass: every candidate takes the predicate and the kernel value,
+// the neighbours' m_j W_ij are summed into acc in stream order, and the
+// returned nibble has bit u set for a neighbour at position k0 + u.
+// UNIFORM: all smoothing lengths are equal, so the support radius
+// scale_k*(hi+hj)/2 = scale_k*hi and the kernel's 1/h are the particle's own.
+template <int SKF, bool UNIFORM>
+__device__ __forceinline__ unsigned density_batch(const CandStream &cs, const DevParams &P, int i, int k0,
+                                                  double2 pi_, double hi, const PredBand &pu, double ihi,
+                                                  const double4 *nb, double &acc) {
+  int j[NB_BATCH];
+  double4 q[NB_BATCH];
+#pragma unroll
+  for (int u = 0; u < NB_BATCH; u++) {
+    j[u] = slot_of(cs, min(k0 + u, cs.total - 1));
+    q[u] = nb[j[u]];
+  }
+  double dx[NB_BATCH], dy[NB_BATCH], dr[NB_BATCH], mh[NB_BATCH];
+  PredBand pb[NB_BATCH];
+  bool in[NB_BATCH], band[NB_BATCH];
+  bool anyband = false;
+#pragma unroll
+  for (int u = 0; u < NB_BATCH; u++) {
+    pair_geom(pi_, hi, q[u], dx[u], dy[u], dr[u], mh[u]);
+    pb[u] = UNIFORM ? pu : pred_band(P.scale_k * mh[u]);
+    in[u] = dr[u] < pb[u].lo;
+    band[u] = !in[u] && !(dr[u] > pb[u].hi);
+    anyband = anyband || band[u];
+  }
+  if (__any(anyband)) {
+#pragma unroll
+    for (int u = 0; u < NB_BATCH; u++)
+      if (band[u]) in[u] = sqrt(dr[u]) < pb[u].t;
+  }
+  unsigned nib = 0u;
+#pragma unroll
+  for (int u = 0; u < NB_BATCH; u++) {
+    const bool nbr = in[u] && k0 + u < cs.total && j[u] != i;
+    nib |= nbr ? (1u << u) : 0u;
+    const double t = acc + q[u].w * sph_w_t<SKF, UNIFORM>(P.kc, P.pi, dr[u], mh[u], ihi);
+    acc = nbr ? t : acc;
+  }
+  return nib;
+}
+
+// Density sum over the stream with the neighbour-mask build.  The stream is
+// cut into batches of 4 positions; batch b belongs to half b & 1, and each
+// half is summed in stream order (acc[0], acc[1]).  TM = 1: one lane takes
+// the batches two at a time, the first into acc[0], the second into acc[1];
+// TM = 2: lane h takes the batches of half h into acc[0].  Mask word k holds
+// stream positions 64k .. 64k+63; its bits are gathered in a register while
+// the word's positions are walked.
+template <int SKF, bool UNIFORM, int TM>
 __device__ __forceinline__ void density_stream(const CandStream &cs, const DevParams &P, int i, int h,
                                                double2 pi_, double hi, const double4 *nb,
                                                unsigned long long (&wd)[NB_WORDS], double (&acc)[2]) {
@@ -1849,47 +1921,17 @@ __device__ __forceinline__ void density_stream(const CandStream &cs, const DevPa
   const double ihi = rcp_nr(hi);
   acc[0] = 0.0;
   acc[1] = 0.0;
-  for (int k0 = BOTH ? 0 : h * NB_BATCH; k0 < cs.total; k0 += (BOTH ? 1 : SPLIT) * NB_BATCH) {
-    const bool h1 = BOTH && ((k0 >> 2) & 1);
-    int j[NB_BATCH];
-    double4 q[NB_BATCH];
 #pragma unroll
-    for (int u = 0; u < NB_BATCH; u++) {
-      j[u] = slot_of(cs, min(k0 + u, cs.total - 1));
-      q[u] = nb[j[u]];
+  for (int wk = 0; wk < NB_WORDS; wk++) {
+    unsigned long long bits = 0ull;
+    for (int k = (TM == 1 ? 0 : NB_BATCH * h); k < 64 && 64 * wk + k < cs.total; k += SPLIT * NB_BATCH) {
+      const int k0 = 64 * wk + k;
+      unsigned nib = density_batch<SKF, UNIFORM>(cs, P, i, k0, pi_, hi, pu, ihi, nb, acc[0]);
+      if constexpr (TM == 1)
+        nib |= density_batch<SKF, UNIFORM>(cs, P, i, k0 + NB_BATCH, pi_, hi, pu, ihi, nb, acc[1]) << NB_BATCH;
+      bits |= (unsigned long long)nib << k;
     }
-    double dx[NB_BATCH], dy[NB_BATCH], dr[NB_BATCH], mh[NB_BATCH];
-    PredBand pb[NB_BATCH];
-    bool in[NB_BATCH], band[NB_BATCH];
-    bool anyband = false;
-#pragma unroll
-    for (int u = 0; u < NB_BATCH; u++) {
-      pair_geom(pi_, hi, q[u], dx[u], dy[u], dr[u], mh[u]);
-      pb[u] = UNIFORM ? pu : pred_band(P.scale_k * mh[u]);
-      in[u] = dr[u] < pb[u].lo;
-      band[u] = !in[u] && !(dr[u] > pb[u].hi);
-      anyband = anyband || band[u];
-    }
-    if (__any(anyband)) {
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++)
-        if (band[u]) in[u] = sqrt(dr[u]) < pb[u].t;
-    }
-    unsigned nib = 0u;
-#pragma unroll
-    for (int u = 0; u < NB_BATCH; u++) {
-      const bool nbr = in[u] && k0 + u < cs.total && j[u] != i;
-      nib |= nbr ? (1u << u) : 0u;
-      double w, gx, gy;
-      sph_kernel_t<SKF, UNIFORM>(P.kc, P.pi, dr[u], dx[u], dy[u], mh[u], w, gx, gy, ihi);
-      const double cur = h1 ? acc[1] : acc[0];
-      const double t = cur + q[u].w * w;
-      acc[0] = (nbr && !h1) ? t : acc[0];
-      acc[1] = (nbr && h1) ? t : acc[1];
-    }
-    const unsigned long long bits = (unsigned long long)nib << (k0 & 63);
-#pragma unroll
-    for (int k = 0; k < NB_WORDS; k++) wd[k] |= (k == (k0 >> 6)) ? bits : 0ull;
+    wd[wk] = bits;
   }
 }
 
@@ -1922,9 +1964,7 @@ __global__ DENSITY_LB void k_density_m(int n, DevParams P, Cells cl, SortedSet s
       double acc = 0.0;
       stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
                    [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                     double w, gx, gy;
-                     sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                     acc = acc + q.w * w;
+                     acc = acc + q.w * sph_w_t<SKF>(P.kc, P.pi, dr, mh);
                    });
       a[hh] = acc;
     }
@@ -1934,9 +1974,9 @@ __global__ DENSITY_LB void k_density_m(int n, DevParams P, Cells cl, SortedSet s
     for (int k = 0; k < NB_WORDS; k++) wd[k] = 0ull;
     double two[2];
     if (g.uniform_h)
-      density_stream<SKF, true, TM == 1>(cs, P, i, t, pi_, hi, s.nb, wd, two);
+      density_stream<SKF, true, TM>(cs, P, i, t, pi_, hi, s.nb, wd, two);
     else
-      density_stream<SKF, false, TM == 1>(cs, P, i, t, pi_, hi, s.nb, wd, two);
+      density_stream<SKF, false, TM>(cs, P, i, t, pi_, hi, s.nb, wd, two);
 #pragma unroll
     for (int hh = 0; hh < SPLIT / TM; hh++) a[hh] = two[hh];
     unsigned long long *mrow = mask + NB_WORDS * (long)i;
@@ -1954,6 +1994,40 @@ __global__ DENSITY_LB void k_density_m(int n, DevParams P, Cells cl, SortedSet s
   const double rf = Si < P.h_inf ? P.h_inf : Si;
   const double p = P.cgra05 * rf * rf;
   Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
+}
+
+// The two halves of a particle's neighbour set for the force and XSPH
+// sums: lo[] holds its first ceil(n/2) neighbours in stream order, hi[] the
+// rest.  A context sums them on two lanes (TM = 2) or one after the other on
+// one lane (TM = 1); either way each half is summed in stream order and the
+// result is half 0 + half 1, so both give the same bits.
+__device__ __forceinline__ void split_halves(const unsigned long long (&w)[NB_WORDS], unsigned long long (&lo)[NB_WORDS],
+                                             unsigned long long (&hi)[NB_WORDS]) {
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < NB_WORDS; k++) n += __popcll(w[k]);
+  int need = (n + 1) >> 1;
+#pragma unroll
+  for (int k = 0; k < NB_WORDS; k++) {
+    const int c = __popcll(w[k]);
+    unsigned long long keep = w[k];
+    if (need < c) {  // the lowest `need` set bits of the word
+      unsigned long long m = w[k];
+      for (int t = 0; t < need; t++) m &= m - 1ull;
+      keep = w[k] ^ m;
+    }
+    lo[k] = need > 0 ? keep : 0ull;
+    hi[k] = w[k] ^ lo[k];
+    need -= c;
+  }
+}
+
+// this lane's half of a neighbour set (TM = 2: lane t takes half t)
+template <int TM>
+__device__ __forceinline__ void lane_half(int t, const unsigned long long (&lo)[NB_WORDS],
+                                          const unsigned long long (&hi)[NB_WORDS], unsigned long long (&m)[NB_WORDS]) {
+#pragma unroll
+  for (int k = 0; k < NB_WORDS; k++) m[k] = (TM == 2 && t) ? hi[k] : lo[k];
 }
 
 // pressure-force contribution of neighbour j (IntForces_SW pair term)
@@ -1999,15 +2073,28 @@ __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
   const double Si = S[i];
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
-  double ax[SPLIT / TM], ay[SPLIT / TM];
-  bool done = false;
-  if constexpr (TM == 1) {
-    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
-      double hx[2] = {0.0, 0.0}, hy[2] = {0.0, 0.0};
-      unsigned long long w[NB_WORDS];
-      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
-      const double ihi = rcp_nr(hi);
-      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
+  double ax[2] = {0.0, 0.0}, ay[2] = {0.0, 0.0};
+  if (stream_full(cs)) {
+#pragma unroll
+    for (int hh = 0; hh < SPLIT / TM; hh++) {
+      double inx = 0.0, iny = 0.0;
+      stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
+                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                     double w, gx, gy;
+                     sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                     pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
+                   });
+      ax[hh] = inx;
+      ay[hh] = iny;
+    }
+  } else {
+    unsigned long long w[NB_WORDS], lo[NB_WORDS], up[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+    split_halves(w, lo, up);
+    const double ihi = rcp_nr(hi);
+    // the pressure force of the neighbours in m, in stream order
+    auto half = [&](unsigned long long (&m)[NB_WORDS], double &inx, double &iny) {
+      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         constexpr bool UH = decltype(uh)::value;
         double4 q[NB_BATCH];
         double pj[NB_BATCH], sj[NB_BATCH];
@@ -2022,76 +2109,37 @@ __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
           double dx, dy, dr, mh, w, gx, gy;
           pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
           sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
-          double tx = hs[u] ? hx[1] : hx[0], ty = hs[u] ? hy[1] : hy[0];
+          double tx = inx, ty = iny;
           pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
-          hx[0] = (ok[u] && !hs[u]) ? tx : hx[0];
-          hy[0] = (ok[u] && !hs[u]) ? ty : hy[0];
-          hx[1] = (ok[u] && hs[u]) ? tx : hx[1];
-          hy[1] = (ok[u] && hs[u]) ? ty : hy[1];
+          inx = ok[u] ? tx : inx;
+          iny = ok[u] ? ty : iny;
         }
       };
       if (g.uniform_h)
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
+        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+          body(std::true_type{}, js, ok);
+        });
       else
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
-      ax[0] = hx[0];
-      ay[0] = hy[0];
-      ax[1] = hx[1];
-      ay[1] = hy[1];
-      done = true;
+        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+          body(std::false_type{}, js, ok);
+        });
+    };
+    if constexpr (TM == 1) {
+      half(lo, ax[0], ay[0]);
+      half(up, ax[1], ay[1]);
+    } else {
+      unsigned long long m[NB_WORDS];
+      lane_half<TM>(t, lo, up, m);
+      half(m, ax[0], ay[0]);
     }
   }
+  double hx[SPLIT / TM], hy[SPLIT / TM];
 #pragma unroll
-  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
-    const int h = t + hh * TM;
-    double inx = 0.0, iny = 0.0;
-  if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb,
-                 [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                   double w, gx, gy;
-                   sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                   pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
-                 });
-  } else {
-    unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
-    const double ihi = rcp_nr(hi);
-    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-      constexpr bool UH = decltype(uh)::value;
-      double4 q[NB_BATCH];
-      double pj[NB_BATCH], sj[NB_BATCH];
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        q[u] = s.nb[js[u]];
-        pj[u] = Pp[js[u]];
-        sj[u] = PA == 2 ? 0.0 : S[js[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        double dx, dy, dr, mh, w, gx, gy;
-        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
-        double tx = inx, ty = iny;
-        pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
-        inx = ok[u] ? tx : inx;
-        iny = ok[u] ? ty : iny;
-      }
-    };
-    if (g.uniform_h)
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-        body(std::true_type{}, js, ok);
-      });
-    else
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-        body(std::false_type{}, js, ok);
-      });
+  for (int hh = 0; hh < SPLIT / TM; hh++) {
+    hx[hh] = ax[hh];
+    hy[hh] = ay[hh];
   }
-    ax[hh] = inx;
-    ay[hh] = iny;
-  }
-  const double inx = halves_sum<TM>(ax), iny = halves_sum<TM>(ay);
+  const double inx = halves_sum<TM>(hx), iny = halves_sum<TM>(hy);
   if (t == 0) ain[i] = make_double2(inx, iny);
 }
 
@@ -2117,15 +2165,31 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
   if (!load_stream(srec, i, cs)) cs = cand_stream(g, start, pi_, hi, P.scale_k);
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
-  double hx[SPLIT / TM], hy[SPLIT / TM];
-  bool done = false;
-  if constexpr (TM == 1) {
-    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
-      double sx[2] = {0.0, 0.0}, sy[2] = {0.0, 0.0};
-      unsigned long long w[NB_WORDS];
-      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
-      const double ihi = rcp_nr(hi);
-      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
+  double sx[2] = {0.0, 0.0}, sy[2] = {0.0, 0.0};
+  if (stream_full(cs)) {
+#pragma unroll
+    for (int hh = 0; hh < SPLIT / TM; hh++) {
+      double ax = 0.0, ay = 0.0;
+      stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
+                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                     if (skip0 && s.id[j] < P.npoin0) return;
+                     const double w = sph_w_t<SKF>(P.kc, P.pi, dr, mh);
+                     const double2 vj = o.vel[j];
+                     const double c = (2 * q.w) * w / (rhoi + o.h[j]);
+                     ax = ax - c * (vi.x - vj.x);
+                     ay = ay - c * (vi.y - vj.y);
+                   });
+      sx[hh] = ax;
+      sy[hh] = ay;
+    }
+  } else {
+    unsigned long long w[NB_WORDS], lo[NB_WORDS], up[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+    split_halves(w, lo, up);
+    const double ihi = rcp_nr(hi);
+    // the XSPH sum of the neighbours in m, in stream order
+    auto half = [&](unsigned long long (&m)[NB_WORDS], double &ax, double &ay) {
+      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         constexpr bool UH = decltype(uh)::value;
         double4 q[NB_BATCH];
         double2 vj[NB_BATCH];
@@ -2140,84 +2204,38 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
         }
 #pragma unroll
         for (int u = 0; u < NB_BATCH; u++) {
-          double dx, dy, dr, mh, w, gx, gy;
+          double dx, dy, dr, mh;
           pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-          sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
+          const double w = sph_w_t<SKF, UH>(P.kc, P.pi, dr, mh, ihi);
           const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
-          const double cx = hs[u] ? sx[1] : sx[0], cy = hs[u] ? sy[1] : sy[0];
-          const double tx = cx - c * (vi.x - vj[u].x), ty = cy - c * (vi.y - vj[u].y);
-          sx[0] = (use[u] && !hs[u]) ? tx : sx[0];
-          sy[0] = (use[u] && !hs[u]) ? ty : sy[0];
-          sx[1] = (use[u] && hs[u]) ? tx : sx[1];
-          sy[1] = (use[u] && hs[u]) ? ty : sy[1];
+          const double tx = ax - c * (vi.x - vj[u].x), ty = ay - c * (vi.y - vj[u].y);
+          ax = use[u] ? tx : ax;
+          ay = use[u] ? ty : ay;
         }
       };
       if (g.uniform_h)
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
+        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+          body(std::true_type{}, js, ok);
+        });
       else
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
-      hx[0] = sx[0];
-      hy[0] = sy[0];
-      hx[1] = sx[1];
-      hy[1] = sy[1];
-      done = true;
+        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+          body(std::false_type{}, js, ok);
+        });
+    };
+    if constexpr (TM == 1) {
+      half(lo, sx[0], sy[0]);
+      half(up, sx[1], sy[1]);
+    } else {
+      unsigned long long m[NB_WORDS];
+      lane_half<TM>(t, lo, up, m);
+      half(m, sx[0], sy[0]);
     }
   }
+  double hx[SPLIT / TM], hy[SPLIT / TM];
 #pragma unroll
-  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
-    const int h = t + hh * TM;
-    double ax = 0.0, ay = 0.0;
-  if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-      if (skip0 && s.id[j] < P.npoin0) return;
-      double w, gx, gy;
-      sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-      const double2 vj = o.vel[j];
-      const double c = (2 * q.w) * w / (rhoi + o.h[j]);
-      ax = ax - c * (vi.x - vj.x);
-      ay = ay - c * (vi.y - vj.y);
-    });
-  } else {
-    unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
-    const double ihi = rcp_nr(hi);
-    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-      constexpr bool UH = decltype(uh)::value;
-      double4 q[NB_BATCH];
-      double2 vj[NB_BATCH];
-      double hj[NB_BATCH];
-      bool use[NB_BATCH];
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        q[u] = s.nb[js[u]];
-        vj[u] = o.vel[js[u]];
-        hj[u] = o.h[js[u]];
-        use[u] = ok[u] && !(skip0 && s.id[js[u]] < P.npoin0);
-      }
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        double dx, dy, dr, mh, w, gx, gy;
-        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
-        const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
-        const double tx = ax - c * (vi.x - vj[u].x), ty = ay - c * (vi.y - vj[u].y);
-        ax = use[u] ? tx : ax;
-        ay = use[u] ? ty : ay;
-      }
-    };
-    if (g.uniform_h)
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-        body(std::true_type{}, js, ok);
-      });
-    else
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-        body(std::false_type{}, js, ok);
-      });
-  }
-    hx[hh] = ax;
-    hy[hh] = ay;
+  for (int hh = 0; hh < SPLIT / TM; hh++) {
+    hx[hh] = sx[hh];
+    hy[hh] = sy[hh];
   }
   const double ax = halves_sum<TM>(hx), ay = halves_sum<TM>(hy);
   if (t != 0) return;
@@ -3616,6 +3634,9 @@ struct gfsph_ctx {
   int eros = 0;
   // pressure re-evaluated in every stage (some realisation has ic_K0 or ic_cosTheta)
   int staged = 0;
+  // the basal-law class every realisation shares (LAW_FRIC5: nfric = 5 without
+  // cosTheta; else LAW_ANY), selecting the specialised RK kernel
+  int law = LAW_ANY;
   ErosGrid eg{};
   std::vector<void *> allocs;
 };
@@ -4018,14 +4039,18 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   }
   if (c->P.nfric != 4) {
     if (c->eros) {
-      k_rk<true><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
+      k_rk<true, LAW_ANY><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
                                                 c->R == 1 ? c->part : nullptr, c->cases ? c->stopr : nullptr, c->eg);
       const long tot = c->eg.np * c->R;
       k_erode_nodes<<<(int)((tot + GF_BLOCK - 1) / GF_BLOCK), GF_BLOCK, 0, st>>>(tot, c->eg);
+    } else if (c->law == LAW_FRIC5) {
+      k_rk<false, LAW_FRIC5><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
+                                                            c->R == 1 ? c->part : nullptr,
+                                                            c->cases ? c->stopr : nullptr, c->eg);
     } else {
-      k_rk<false><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
-                                                 c->R == 1 ? c->part : nullptr, c->cases ? c->stopr : nullptr,
-                                                 c->eg);
+      k_rk<false, LAW_ANY><<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, c->ain, c->nw,
+                                                          c->R == 1 ? c->part : nullptr, c->cases ? c->stopr : nullptr,
+                                                          c->eg);
     }
     launch_column(c, s);
     return;
@@ -4294,6 +4319,8 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
     Ph[r].eros_single = (counts[r] == 1);
     c->eros = c->eros || Ph[r].eros_on;
     c->staged = c->staged || Ph[r].ic_k0 || Ph[r].costheta;
+    if (r == 0) c->law = LAW_FRIC5;
+    if (Ph[r].nfric != 5 || Ph[r].costheta || Ph[r].if_V2byR) c->law = LAW_ANY;
   }
   c->P = Ph[0];
   c->R = R;

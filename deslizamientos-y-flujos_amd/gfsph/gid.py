@@ -13,9 +13,10 @@ the reference's writers:
 * results: ``OutputRes_WIR`` ("6 SPH_SW_2019.f90":14500-14917), blocks of
   the GiD mask (``Gid_Mask_SW``, 6:1536-1559): 1 height soil, 2
   displacement, 3 velocity, 4 pore pressure (icpwp 1-3), 6 Z topo (first
-  output only: the two-phase ``hs_plus_Z_topo`` form is not built) and 10
-  joint height.  A mask entry of 1 writes the block; 9 (skip empty nodes)
-  and the blocks 5, 7, 8, 9, 11, 12 raise ``ValueError``.
+  output only: the two-phase ``hs_plus_Z_topo`` form is not built), 7 with
+  the value 999 (the mass-based height "hs", 6:14661-14713) and 10 joint
+  height.  A mask entry of 1 writes the block; 9 (skip empty nodes) and the
+  blocks 5, 7 (other than 999), 8, 9, 11, 12 raise ``ValueError``.
 
 Numbers are written as shortest round-trip decimals; GiD reads the files
 as whitespace-separated tokens, so the values (not the spacing of the
@@ -66,6 +67,35 @@ def z_at(dem, x: np.ndarray) -> np.ndarray:
     return sh1 * Z[n1] + sh2 * Z[n2] + sh3 * Z[n3] + sh4 * Z[n4]
 
 
+def cell_nodes(dem, x: np.ndarray) -> np.ndarray:
+    """The four nodes n1, n2, n3, n4 (0-based, counter-clockwise from the one
+    below-left) of each position's DEM cell, as Get_Z_Topo and the result
+    writers find it: cell index (x - xmin)/dx + 1 truncated, moved one cell
+    back on the last row or column within 1e-3 of the grid's edge."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 2)
+    nx, ny = dem.nx, dem.ny
+    xx, yy = x[:, 0], x[:, 1]
+    ipx = ((xx - dem.xmin) / dem.dx + 1).astype(np.int64)
+    ipy = ((yy - dem.ymin) / dem.dy + 1).astype(np.int64)
+    ipx = np.where((ipx == nx) & (np.abs(dem.xmax - xx) <= 1.0e-3), ipx - 1, ipx)
+    ipy = np.where((ipy == ny) & (np.abs(dem.ymax - yy) <= 1.0e-3), ipy - 1, ipy)
+    ipx = np.clip(ipx, 1, nx - 1)
+    ipy = np.clip(ipy, 1, ny - 1)
+    n1 = (ipy - 1) * nx + (ipx - 1)
+    return np.stack([n1, n1 + 1, n1 + 1 + nx, n1 + nx], axis=1)
+
+
+def hs_nodes(dem, x: np.ndarray, mass: np.ndarray, live: np.ndarray) -> np.ndarray:
+    """The mass-based height of the "hs" block (6:14661-14713): every particle
+    in the domain adds its mass to the four nodes of its cell, in particle
+    order; the sums are divided by 4 dx dy."""
+    nodes = cell_nodes(dem, x)[live]
+    m = np.asarray(mass, dtype=np.float64)[live]
+    hg = np.zeros(dem.nx * dem.ny)
+    np.add.at(hg, nodes.ravel(), np.repeat(m, 4))  # per particle n1..n4, particles in order
+    return hg / (4.0 * (dem.dx * dem.dy))
+
+
 def quads(dem) -> np.ndarray:
     """Q1 quadrilaterals [keleg][4] (1-based DEM nodes) over the cells whose
     four nodes are in the model, in the reference's row-by-row order."""
@@ -114,12 +144,13 @@ def _block(title: str, time: float, rows) -> list[str]:
 
 def post_res_blocks(dem, time: float, raster: np.ndarray, x: np.ndarray, v: np.ndarray, x00: np.ndarray,
                     z00: np.ndarray, out_domain: np.ndarray | None = None, mask=(1, 1, 1, 1, 0, 1, 0, 0, 0, 1, 0, 0),
-                    first: bool = True, icpwp: int = 0) -> list[str]:
+                    first: bool = True, icpwp: int = 0, mass: np.ndarray | None = None) -> list[str]:
     """Lines of one output time: raster [npoig][4] = (h, vx, vy, pwp) of the
     IDW raster, x/v [n][2] the particles, x00/z00 where they started
     (write_post_msh's return), out_domain [n] nonzero for particles out of
     the domain, mask the 12 GiD mask entries, first for the first output of
-    the run (the Z topo block), icpwp the pore-pressure model."""
+    the run (the Z topo block), icpwp the pore-pressure model, mass [n] the
+    particle masses (needed by the "hs" block, mask entry 7 = 999)."""
     mask = [int(m) for m in mask] + [0] * (12 - len(mask))
     for k, m in enumerate(mask, 1):
         if m not in (0, 1) and not (k == 7 and m == 999):
@@ -151,6 +182,11 @@ def post_res_blocks(dem, time: float, raster: np.ndarray, x: np.ndarray, v: np.n
         L += _block('"Pwp" "pwpress"', time, [f" {k + 1} 0.0 0.0 {_f(raster[k, 3])}" for k in range(npoig)])
     if mask[5] and first:
         L += _block('" Z topo " "   Z   "', time, [f" {k + 1} 0. 0. {_f(Ztop[k])}" for k in range(npoig)])
+    if mask[6] == 999:
+        if mass is None:
+            raise ValueError("GiD mask entry 7 = 999 (the hs block) needs the particle masses")
+        hg = hs_nodes(dem, x, mass, live)
+        L += _block('"hs" "hs"', time, [f" {k + 1} 0. 0. {_f(hg[k])}" for k in range(npoig) if hg[k] > 0.005])
     if mask[9]:
         rows = [f" {k + 1} 0. 0. {_f(raster[k, 0])}" for k in range(npoig)]
         rows += [f" {npoig + k + 1} 0. 0. {_f(raster[k, 0])}" for k in range(npoig)]

@@ -88,7 +88,7 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
         raise ValueError("icpwp must be 0, 1, 2 or 3")
     L += ["coarse mesh saving utility?", "0"]
     L += ["control points?", "0"]
-    L += ["GID filter", "1 1 1 1 0 1 0 0 0 1 0 0"]
+    L += ["GID filter", " ".join(str(int(m)) for m in getattr(case, "gid_mask", (1, 1, 1, 1, 0, 1, 0, 0, 0, 1, 0, 0)))]
     L += ["T_change_to_W", "1.e+12"]
     with open(os.path.join(d, name + ".dat"), "w") as f:
         f.write("\n".join(L) + "\n")

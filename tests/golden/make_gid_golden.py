@@ -30,6 +30,14 @@ def main():
     out["nsteps"] = np.array(NSTEPS)
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "gid_frictional_pwp.npz"), **out)
     print(sorted(out))
+    # the mass-based height block ("hs", GiD mask entry 7 = 999)
+    case.gid_mask = (1, 0, 0, 0, 0, 0, 999, 0, 0, 0, 0, 0)
+    tmp = tempfile.mkdtemp(prefix="gfsph_gid_")
+    refio.run_reference(case, NSTEPS, [NSTEPS], workdir=tmp, gid_final=True)
+    out = {f"res: {k}": v for k, v in gid.read_tokens(os.path.join(tmp, "case.post.res")).items()}
+    out["nsteps"] = np.array(NSTEPS)
+    np.savez_compressed(os.path.join(ROOT, "tests", "golden", "gid_hs_frictional_pwp.npz"), **out)
+    print(sorted(out))
 
 
 if __name__ == "__main__":

@@ -68,9 +68,36 @@ def test_gid_writer_rejects_unbuilt_blocks(tmp_path):
     case, _ = goldens.case_of("g_frictional_pwp")
     st = oracle.init(case)
     ras = np.zeros((case.dem.nx * case.dem.ny, 4))
-    for mask in ((1, 0, 0, 0, 1), (9, 0, 0), (0,) * 10 + (1,)):
-        with pytest.raises(ValueError):
+    for mask in ((1, 0, 0, 0, 1), (9, 0, 0), (0,) * 10 + (1,), (0,) * 6 + (1,), (0,) * 6 + (999,)):
+        with pytest.raises(ValueError):  # the last: 999 (hs) without the masses
             gid.post_res_blocks(case.dem, 0.0, ras, st.x, st.v, st.x, np.zeros(st.x.shape[0]), mask=mask)
+
+
+def test_gid_hs_block_matches_reference_writer():
+    """The mass-based height block (GiD mask entry 7 = 999, 6:14661-14713)
+    against the reference writer's file for the same case and mask, at the
+    initial state and after 100 RK4 steps (tests/golden/gid_hs_frictional_pwp.npz)."""
+    fx = goldens.load("gid_hs_frictional_pwp")
+    case, _ = goldens.case_of("g_frictional_pwp")
+    st = oracle.init(case)
+    mask = (1, 0, 0, 0, 0, 0, 999, 0, 0, 0, 0, 0)
+    for k, steps in ((0, 0), (1, int(fx["nsteps"]))):
+        if steps:
+            oracle.run(case, st, steps)
+        ras, _ = oracle.raster_idw(case.dem, st)
+        L = gid.post_res_blocks(case.dem, float(fx[f"res: time hs #{k}"][0]), ras, st.x, st.v, st.x,
+                                np.zeros(st.x.shape[0]), out_domain=st.flag, mask=mask, mass=st.mass)
+        rows = []
+        on = False
+        for line in L:
+            if line.startswith(' Result "hs"'):
+                on = True
+                continue
+            if on and line.strip() == "End Values":
+                break
+            if on and line.strip() != "Values":
+                rows.append([float(t) for t in line.split()])
+        _close(np.array(rows), fx[f"res: result hs #{k}"], 1e-12 if k else 1e-15, f"hs #{k}")
 
 
 def test_gid_displacement_sign():
