@@ -1338,8 +1338,13 @@ __device__ __forceinline__ void stage_update(const DevParams &P, int st, double2
 #ifndef RK_WAVES_PER_SIMD
 #define RK_WAVES_PER_SIMD 1
 #endif
+// waves per SIMD the specialised kernel is compiled for (its registers fit
+// three waves with a few spilled words; measured faster than two)
+#ifndef RK5_WAVES_PER_SIMD
+#define RK5_WAVES_PER_SIMD 3
+#endif
 template <bool EROS, int LAW>
-__global__ __launch_bounds__(GF_BLOCK, RK_WAVES_PER_SIMD) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
+__global__ __launch_bounds__(GF_BLOCK, LAW == LAW_FRIC5 ? RK5_WAVES_PER_SIMD : RK_WAVES_PER_SIMD) void k_rk(int n, const DevParams *Pr, DevDem dem, SortedSet s,
                                                   const double *S, const double2 *ain, NewSet o, BBoxAcc *part,
                                                   const int *stopr, ErosGrid eg) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -1416,8 +1421,11 @@ struct BinghamState {
   int *fl;       // flagged in mid-step
   double *cand;  // [n] by caller id: Tau0 or NaN
   double *tau;   // [n] by caller id: resolved Tau0
-  double *carry; // [1]
-  double *bsum;  // per scan block: last valid value (NaN if none)
+  double *carry; // [R]: each realisation's Tau0 carried from its earlier calls
+  double *bsum;  // per scan tile: last valid value (NaN if none)
+  int4 *tiles;   // [ntiles]: realisation, first id, end id of each scan tile
+  int *rtile;    // [R + 1]: first tile of each realisation
+  int ntiles;
 };
 
 __global__ __launch_bounds__(GF_BLOCK) void k_bingham_init(int n, SortedSet s, const double2 *ain, BinghamState b) {
@@ -1433,64 +1441,72 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_init(int n, SortedSet s, c
   b.fl[i] = 0;
 }
 
-// Pd: the device parameter block, whose dt_sph (and RK weight) the adaptive
-// time step updates between steps
-__global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, int st, DevParams P,
-                                                             const DevParams *Pd, DevDem dem, SortedSet s,
-                                                             const double *S, BinghamState b) {
+// Pr: the realisations' parameter blocks on the device (dt_sph and the RK
+// weight change with the adaptive time step between steps); each wave runs
+// its realisations one after the other (their slot ranges are contiguous)
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, int st, const DevParams *Pr,
+                                                             DevDem dem, SortedSet s, const double *S,
+                                                             BinghamState b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  P.dt = Pd->dt;
-  P.fin = Pd->fin;
-  const int id = s.id[i];
-  if (s.flag[i] || b.fl[i]) {
-    if (phase == 0) b.cand[id] = __longlong_as_double(0x7ff8000000000000ll);
-    return;
-  }
-  const double Si = S[i];
-  const double rfi = Si < P.h_inf ? P.h_inf : Si;
-  const double2 in = b.in[i];
-  if (phase == 0) {
-    double2 x = b.x[i], v = b.v[i], ex = b.ex[i];
-    double u = b.u[i];
-    stage_update(P, st, s.pos[i], s.vel[i], s.u[i], s.h[i], rfi, ex.x, ex.y, b.du[i], in.x, in.y, x, v, u);
-    b.x[i] = x;
-    b.v[i] = v;
-    b.u[i] = u;
-    if (out_of_domain(dem, x.x, x.y)) {
-      b.fl[i] = 1;
-      b.cand[id] = __longlong_as_double(0x7ff8000000000000ll);
-      return;
+  const int myr = i < n ? s.rid[i] : 0;
+  for (bool todo = i < n; todo;) {
+    const int r0 = __builtin_amdgcn_readfirstlane(myr);
+    if (myr != r0) continue;
+    todo = false;
+    const DevParams &P = Pr[r0];
+    const int id = s.id[i];
+    if (s.flag[i] || b.fl[i]) {
+      if (phase == 0) b.cand[id] = __longlong_as_double(0x7ff8000000000000ll);
+      continue;
     }
-    b.cand[id] = bingham_candidate(P, rfi, v.x, v.y);
-  } else {
-    double2 x = b.x[i], v = b.v[i];
-    double u = b.u[i], ex, ey, du;
-    double tau = b.tau[id];
-    ext_force(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, tau, ex, ey, du);
-    b.u[i] = u;
-    b.ex[i] = make_double2(ex, ey);
-    b.du[i] = du;
-    const double f2 = P.f2[st];
-    double *D = b.D + 5 * i;
-    D[2] += f2 * (ex + in.x);
-    D[3] += f2 * (ey + in.y);
-    D[0] += f2 * v.x;
-    D[1] += f2 * v.y;
-    D[4] += f2 * du;
+    const double Si = S[i];
+    const double rfi = Si < P.h_inf ? P.h_inf : Si;
+    const double2 in = b.in[i];
+    if (phase == 0) {
+      double2 x = b.x[i], v = b.v[i], ex = b.ex[i];
+      double u = b.u[i];
+      stage_update(P, st, s.pos[i], s.vel[i], s.u[i], s.h[i], rfi, ex.x, ex.y, b.du[i], in.x, in.y, x, v, u);
+      b.x[i] = x;
+      b.v[i] = v;
+      b.u[i] = u;
+      if (out_of_domain(dem, x.x, x.y)) {
+        b.fl[i] = 1;
+        b.cand[id] = __longlong_as_double(0x7ff8000000000000ll);
+        continue;
+      }
+      b.cand[id] = bingham_candidate(P, rfi, v.x, v.y);
+    } else {
+      double2 x = b.x[i], v = b.v[i];
+      double u = b.u[i], ex, ey, du;
+      double tau = b.tau[id];
+      ext_force(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, tau, ex, ey, du);
+      b.u[i] = u;
+      b.ex[i] = make_double2(ex, ey);
+      b.du[i] = du;
+      const double f2 = P.f2[st];
+      double *D = b.D + 5 * i;
+      D[2] += f2 * (ex + in.x);
+      D[3] += f2 * (ey + in.y);
+      D[0] += f2 * v.x;
+      D[1] += f2 * v.y;
+      D[4] += f2 * du;
+    }
   }
 }
 
-__global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P, const DevParams *Pd, DevDem dem,
-                                                              SortedSet s, const double *S, BinghamState b,
-                                                              NewSet o, BBoxAcc *part) {
+__global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, const DevParams *Pr, DevDem dem, SortedSet s,
+                                                              const double *S, BinghamState b, NewSet o,
+                                                              BBoxAcc *part) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
-  P.dt = Pd->dt;
-  P.fin = Pd->fin;
   bool active_out = false;
   double2 xo = make_double2(0, 0);
   double hso = 0;
-  if (i < n) {
+  const int myr = i < n ? s.rid[i] : 0;
+  for (bool todo = i < n; todo;) {
+    const int r0 = __builtin_amdgcn_readfirstlane(myr);
+    if (myr != r0) continue;
+    todo = false;
+    const DevParams &P = Pr[r0];
     if (s.flag[i]) {
       write_frozen(P, s, i, o);
     } else {
@@ -1501,7 +1517,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P,
                       o, active_out, xo, hso);
     }
   }
-  bbox_block(part, active_out, xo, hso);
+  if (part) bbox_block(part, active_out, xo, hso);
 }
 
 // last-valid inclusive scan over particle ids: tau[k] = cand of the largest
@@ -1511,13 +1527,17 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_finish(int n, DevParams P,
 
 __device__ __forceinline__ bool is_valid(double v) { return v == v; }
 
-__global__ void k_lv_reduce(int n, const double *cand, double *bsum) {
+// Scan tiles never cross a realisation: each realisation carries its own
+// Tau0 (realisations never interact), and ids are stacked realisation after
+// realisation.  Tile b covers ids tiles[b].y .. tiles[b].z - 1 of
+// realisation tiles[b].x.
+__global__ void k_lv_reduce(const int4 *tiles, const double *cand, double *bsum) {
   __shared__ double lds[GF_BLOCK];
-  int base = blockIdx.x * LV_TILE;
+  const int4 tl = tiles[blockIdx.x];
   double last = __longlong_as_double(0x7ff8000000000000ll);
   for (int k = 0; k < LV_ITEMS; k++) {
-    int idx = base + threadIdx.x * LV_ITEMS + k;
-    if (idx < n && is_valid(cand[idx])) last = cand[idx];
+    int idx = tl.y + threadIdx.x * LV_ITEMS + k;
+    if (idx < tl.z && is_valid(cand[idx])) last = cand[idx];
   }
   lds[threadIdx.x] = last;
   __syncthreads();
@@ -1529,26 +1549,28 @@ __global__ void k_lv_reduce(int n, const double *cand, double *bsum) {
   }
 }
 
-// sequential over blocks (one thread): bsum becomes the value valid before
-// each block; the carry becomes the last valid value of this call
-__global__ void k_lv_top(int nblocks, double *bsum, double *carry) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double c = *carry;
-  for (int b = 0; b < nblocks; b++) {
+// one thread per realisation, sequential over its tiles: bsum becomes the
+// value valid before each tile; the realisation's carry becomes the last
+// valid value of this call
+__global__ void k_lv_top(int R, const int *rtile, double *bsum, double *carry) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  double c = carry[r];
+  for (int b = rtile[r]; b < rtile[r + 1]; b++) {
     double t = bsum[b];
     bsum[b] = c;
     if (is_valid(t)) c = t;
   }
-  *carry = c;
+  carry[r] = c;
 }
 
-__global__ void k_lv_down(int n, const double *cand, const double *bsum, double *tau) {
+__global__ void k_lv_down(const int4 *tiles, const double *cand, const double *bsum, double *tau) {
   __shared__ double lds[GF_BLOCK];
-  int base = blockIdx.x * LV_TILE;
+  const int4 tl = tiles[blockIdx.x];
   double last = __longlong_as_double(0x7ff8000000000000ll);
   for (int k = 0; k < LV_ITEMS; k++) {
-    int idx = base + threadIdx.x * LV_ITEMS + k;
-    if (idx < n && is_valid(cand[idx])) last = cand[idx];
+    int idx = tl.y + threadIdx.x * LV_ITEMS + k;
+    if (idx < tl.z && is_valid(cand[idx])) last = cand[idx];
   }
   lds[threadIdx.x] = last;
   __syncthreads();
@@ -1563,8 +1585,8 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
   double c = threadIdx.x > 0 ? lds[threadIdx.x - 1] : __longlong_as_double(0x7ff8000000000000ll);
   if (!is_valid(c)) c = bsum[blockIdx.x];
   for (int k = 0; k < LV_ITEMS; k++) {
-    int idx = base + threadIdx.x * LV_ITEMS + k;
-    if (idx < n) {
+    int idx = tl.y + threadIdx.x * LV_ITEMS + k;
+    if (idx < tl.z) {
       if (is_valid(cand[idx])) c = cand[idx];
       tau[idx] = c;
     }
@@ -1598,22 +1620,21 @@ __global__ void k_lv_down(int n, const double *cand, const double *bsum, double 
 #else
 #define DENSITY_LB __launch_bounds__(GATHER_MAX_BLOCK)
 #endif
-#ifdef PFORCE_WAVES
-#define PFORCE_LB __launch_bounds__(GATHER_MAX_BLOCK, PFORCE_WAVES)
-#else
-#define PFORCE_LB __launch_bounds__(GATHER_MAX_BLOCK)
+// the force pass fits five waves per SIMD without spilling
+#ifndef PFORCE_WAVES
+#define PFORCE_WAVES 5
 #endif
+#define PFORCE_LB __launch_bounds__(GATHER_MAX_BLOCK, PFORCE_WAVES)
 #ifdef XSPH_WAVES
 #define XSPH_LB __launch_bounds__(GATHER_MAX_BLOCK, XSPH_WAVES)
 #else
 #define XSPH_LB __launch_bounds__(GATHER_MAX_BLOCK)
 #endif
 
-// Every gather splits a particle's sum into two halves: the density pass the
-// 4-position batches 0, 2, 4, ... and 1, 3, 5, ... of its candidate stream,
-// the force and XSPH passes the first and the second half of its neighbours
-// (split_halves).  Each half is summed in stream order and the result is
-// half 0 + half 1.  A context runs the halves either on two adjacent lanes
+// Every gather splits a particle's candidate stream into two halves: the
+// 4-position batches 0, 2, 4, ... and 1, 3, 5, ... (the force and XSPH passes:
+// the neighbours in those batches).  Each half is summed in stream order and
+// the result is half 0 + half 1.  A context runs the halves either on two adjacent lanes
 // (TM = 2: more loads in flight when the particles alone fill each SIMD only
 // a few waves deep) or one after the other on one lane (TM = 1: large
 // batches); both give the same bits, for every work-group size.
@@ -1653,6 +1674,12 @@ __device__ __forceinline__ unsigned long long halves_or(unsigned long long v) {
     const unsigned lo = __shfl_xor((unsigned)v, 1), hi = __shfl_xor((unsigned)(v >> 32), 1);
     return v | ((unsigned long long)hi << 32) | lo;
   }
+}
+
+// mask bits of half h (nibbles of NB_BATCH = 4 positions)
+__device__ __forceinline__ unsigned long long split_bits(int h) {
+  static_assert(NB_BATCH == 4, "the halves are made of 4-position nibbles");
+  return 0x0F0F0F0F0F0F0F0Full << (4 * h);
 }
 
 struct CandStream {
@@ -1785,7 +1812,7 @@ __device__ __forceinline__ void load_words(const CandStream &cs, const unsigned 
 // words w of a particle in stream order; js are slots.  Unused entries of a
 // batch have ok = false and js = self (a valid index), so f may read them
 // unconditionally.  Emptied words are shifted out at batch boundaries.
-template <typename F>
+template <bool HALVES = false, typename F>
 __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned long long (&w)[NB_WORDS], int self,
                                                F &&f) {
   int pbase = 0;
@@ -1802,15 +1829,20 @@ __device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned lo
     }
     int js[NB_BATCH];
     bool ok[NB_BATCH];
+    int hs[NB_BATCH];  // half of the entry: parity of its 4-position nibble
 #pragma unroll
     for (int u = 0; u < NB_BATCH; u++) {
       ok[u] = w[0] != 0ull;
       const int pos = pbase + __ffsll((long long)w[0]) - 1;
       const int sl = slot_of(cs, pos);
       js[u] = ok[u] ? sl : self;
+      hs[u] = (pos >> 2) & 1;
       w[0] &= w[0] - 1ull;
     }
-    f(js, ok);
+    if constexpr (HALVES)
+      f(js, ok, hs);
+    else
+      f(js, ok);
   }
 }
 
@@ -1996,40 +2028,6 @@ __global__ DENSITY_LB void k_density_m(int n, DevParams P, Cells cl, SortedSet s
   Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
 }
 
-// The two halves of a particle's neighbour set for the force and XSPH
-// sums: lo[] holds its first ceil(n/2) neighbours in stream order, hi[] the
-// rest.  A context sums them on two lanes (TM = 2) or one after the other on
-// one lane (TM = 1); either way each half is summed in stream order and the
-// result is half 0 + half 1, so both give the same bits.
-__device__ __forceinline__ void split_halves(const unsigned long long (&w)[NB_WORDS], unsigned long long (&lo)[NB_WORDS],
-                                             unsigned long long (&hi)[NB_WORDS]) {
-  int n = 0;
-#pragma unroll
-  for (int k = 0; k < NB_WORDS; k++) n += __popcll(w[k]);
-  int need = (n + 1) >> 1;
-#pragma unroll
-  for (int k = 0; k < NB_WORDS; k++) {
-    const int c = __popcll(w[k]);
-    unsigned long long keep = w[k];
-    if (need < c) {  // the lowest `need` set bits of the word
-      unsigned long long m = w[k];
-      for (int t = 0; t < need; t++) m &= m - 1ull;
-      keep = w[k] ^ m;
-    }
-    lo[k] = need > 0 ? keep : 0ull;
-    hi[k] = w[k] ^ lo[k];
-    need -= c;
-  }
-}
-
-// this lane's half of a neighbour set (TM = 2: lane t takes half t)
-template <int TM>
-__device__ __forceinline__ void lane_half(int t, const unsigned long long (&lo)[NB_WORDS],
-                                          const unsigned long long (&hi)[NB_WORDS], unsigned long long (&m)[NB_WORDS]) {
-#pragma unroll
-  for (int k = 0; k < NB_WORDS; k++) m[k] = (TM == 2 && t) ? hi[k] : lo[k];
-}
-
 // pressure-force contribution of neighbour j (IntForces_SW pair term)
 template <int PA>
 __device__ __forceinline__ void pforce_term(double Pi, double rfi, double Pj, double Sj, double h_inf, double mj,
@@ -2073,28 +2071,15 @@ __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
   const double Si = S[i];
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
-  double ax[2] = {0.0, 0.0}, ay[2] = {0.0, 0.0};
-  if (stream_full(cs)) {
-#pragma unroll
-    for (int hh = 0; hh < SPLIT / TM; hh++) {
-      double inx = 0.0, iny = 0.0;
-      stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
-                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                     double w, gx, gy;
-                     sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                     pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
-                   });
-      ax[hh] = inx;
-      ay[hh] = iny;
-    }
-  } else {
-    unsigned long long w[NB_WORDS], lo[NB_WORDS], up[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
-    split_halves(w, lo, up);
-    const double ihi = rcp_nr(hi);
-    // the pressure force of the neighbours in m, in stream order
-    auto half = [&](unsigned long long (&m)[NB_WORDS], double &inx, double &iny) {
-      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+  double ax[SPLIT / TM], ay[SPLIT / TM];
+  bool done = false;
+  if constexpr (TM == 1) {
+    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
+      double hx[2] = {0.0, 0.0}, hy[2] = {0.0, 0.0};
+      unsigned long long w[NB_WORDS];
+      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+      const double ihi = rcp_nr(hi);
+      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
         constexpr bool UH = decltype(uh)::value;
         double4 q[NB_BATCH];
         double pj[NB_BATCH], sj[NB_BATCH];
@@ -2109,37 +2094,76 @@ __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
           double dx, dy, dr, mh, w, gx, gy;
           pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
           sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
-          double tx = inx, ty = iny;
+          double tx = hs[u] ? hx[1] : hx[0], ty = hs[u] ? hy[1] : hy[0];
           pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
-          inx = ok[u] ? tx : inx;
-          iny = ok[u] ? ty : iny;
+          hx[0] = (ok[u] && !hs[u]) ? tx : hx[0];
+          hy[0] = (ok[u] && !hs[u]) ? ty : hy[0];
+          hx[1] = (ok[u] && hs[u]) ? tx : hx[1];
+          hy[1] = (ok[u] && hs[u]) ? ty : hy[1];
         }
       };
       if (g.uniform_h)
-        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-          body(std::true_type{}, js, ok);
-        });
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
       else
-        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-          body(std::false_type{}, js, ok);
-        });
-    };
-    if constexpr (TM == 1) {
-      half(lo, ax[0], ay[0]);
-      half(up, ax[1], ay[1]);
-    } else {
-      unsigned long long m[NB_WORDS];
-      lane_half<TM>(t, lo, up, m);
-      half(m, ax[0], ay[0]);
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
+      ax[0] = hx[0];
+      ay[0] = hy[0];
+      ax[1] = hx[1];
+      ay[1] = hy[1];
+      done = true;
     }
   }
-  double hx[SPLIT / TM], hy[SPLIT / TM];
 #pragma unroll
-  for (int hh = 0; hh < SPLIT / TM; hh++) {
-    hx[hh] = ax[hh];
-    hy[hh] = ay[hh];
+  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
+    const int h = t + hh * TM;
+    double inx = 0.0, iny = 0.0;
+  if (stream_full(cs)) {
+    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb,
+                 [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                   double w, gx, gy;
+                   sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                   pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
+                 });
+  } else {
+    unsigned long long w[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
+    const double ihi = rcp_nr(hi);
+    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      constexpr bool UH = decltype(uh)::value;
+      double4 q[NB_BATCH];
+      double pj[NB_BATCH], sj[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        q[u] = s.nb[js[u]];
+        pj[u] = Pp[js[u]];
+        sj[u] = PA == 2 ? 0.0 : S[js[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh, w, gx, gy;
+        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
+        double tx = inx, ty = iny;
+        pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
+        inx = ok[u] ? tx : inx;
+        iny = ok[u] ? ty : iny;
+      }
+    };
+    if (g.uniform_h)
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::true_type{}, js, ok);
+      });
+    else
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::false_type{}, js, ok);
+      });
   }
-  const double inx = halves_sum<TM>(hx), iny = halves_sum<TM>(hy);
+    ax[hh] = inx;
+    ay[hh] = iny;
+  }
+  const double inx = halves_sum<TM>(ax), iny = halves_sum<TM>(ay);
   if (t == 0) ain[i] = make_double2(inx, iny);
 }
 
@@ -2165,31 +2189,15 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
   if (!load_stream(srec, i, cs)) cs = cand_stream(g, start, pi_, hi, P.scale_k);
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
-  double sx[2] = {0.0, 0.0}, sy[2] = {0.0, 0.0};
-  if (stream_full(cs)) {
-#pragma unroll
-    for (int hh = 0; hh < SPLIT / TM; hh++) {
-      double ax = 0.0, ay = 0.0;
-      stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
-                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                     if (skip0 && s.id[j] < P.npoin0) return;
-                     const double w = sph_w_t<SKF>(P.kc, P.pi, dr, mh);
-                     const double2 vj = o.vel[j];
-                     const double c = (2 * q.w) * w / (rhoi + o.h[j]);
-                     ax = ax - c * (vi.x - vj.x);
-                     ay = ay - c * (vi.y - vj.y);
-                   });
-      sx[hh] = ax;
-      sy[hh] = ay;
-    }
-  } else {
-    unsigned long long w[NB_WORDS], lo[NB_WORDS], up[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
-    split_halves(w, lo, up);
-    const double ihi = rcp_nr(hi);
-    // the XSPH sum of the neighbours in m, in stream order
-    auto half = [&](unsigned long long (&m)[NB_WORDS], double &ax, double &ay) {
-      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+  double hx[SPLIT / TM], hy[SPLIT / TM];
+  bool done = false;
+  if constexpr (TM == 1) {
+    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
+      double sx[2] = {0.0, 0.0}, sy[2] = {0.0, 0.0};
+      unsigned long long w[NB_WORDS];
+      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+      const double ihi = rcp_nr(hi);
+      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
         constexpr bool UH = decltype(uh)::value;
         double4 q[NB_BATCH];
         double2 vj[NB_BATCH];
@@ -2208,34 +2216,79 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
           pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
           const double w = sph_w_t<SKF, UH>(P.kc, P.pi, dr, mh, ihi);
           const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
-          const double tx = ax - c * (vi.x - vj[u].x), ty = ay - c * (vi.y - vj[u].y);
-          ax = use[u] ? tx : ax;
-          ay = use[u] ? ty : ay;
+          const double cx = hs[u] ? sx[1] : sx[0], cy = hs[u] ? sy[1] : sy[0];
+          const double tx = cx - c * (vi.x - vj[u].x), ty = cy - c * (vi.y - vj[u].y);
+          sx[0] = (use[u] && !hs[u]) ? tx : sx[0];
+          sy[0] = (use[u] && !hs[u]) ? ty : sy[0];
+          sx[1] = (use[u] && hs[u]) ? tx : sx[1];
+          sy[1] = (use[u] && hs[u]) ? ty : sy[1];
         }
       };
       if (g.uniform_h)
-        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-          body(std::true_type{}, js, ok);
-        });
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
       else
-        masked_batches(cs, m, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-          body(std::false_type{}, js, ok);
-        });
-    };
-    if constexpr (TM == 1) {
-      half(lo, sx[0], sy[0]);
-      half(up, sx[1], sy[1]);
-    } else {
-      unsigned long long m[NB_WORDS];
-      lane_half<TM>(t, lo, up, m);
-      half(m, sx[0], sy[0]);
+        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
+                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
+      hx[0] = sx[0];
+      hy[0] = sy[0];
+      hx[1] = sx[1];
+      hy[1] = sy[1];
+      done = true;
     }
   }
-  double hx[SPLIT / TM], hy[SPLIT / TM];
 #pragma unroll
-  for (int hh = 0; hh < SPLIT / TM; hh++) {
-    hx[hh] = sx[hh];
-    hy[hh] = sy[hh];
+  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
+    const int h = t + hh * TM;
+    double ax = 0.0, ay = 0.0;
+  if (stream_full(cs)) {
+    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+      if (skip0 && s.id[j] < P.npoin0) return;
+      const double w = sph_w_t<SKF>(P.kc, P.pi, dr, mh);
+      const double2 vj = o.vel[j];
+      const double c = (2 * q.w) * w / (rhoi + o.h[j]);
+      ax = ax - c * (vi.x - vj.x);
+      ay = ay - c * (vi.y - vj.y);
+    });
+  } else {
+    unsigned long long w[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
+    const double ihi = rcp_nr(hi);
+    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      constexpr bool UH = decltype(uh)::value;
+      double4 q[NB_BATCH];
+      double2 vj[NB_BATCH];
+      double hj[NB_BATCH];
+      bool use[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        q[u] = s.nb[js[u]];
+        vj[u] = o.vel[js[u]];
+        hj[u] = o.h[js[u]];
+        use[u] = ok[u] && !(skip0 && s.id[js[u]] < P.npoin0);
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh;
+        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+        const double w = sph_w_t<SKF, UH>(P.kc, P.pi, dr, mh, ihi);
+        const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
+        const double tx = ax - c * (vi.x - vj[u].x), ty = ay - c * (vi.y - vj[u].y);
+        ax = use[u] ? tx : ax;
+        ay = use[u] ? ty : ay;
+      }
+    };
+    if (g.uniform_h)
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::true_type{}, js, ok);
+      });
+    else
+      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::false_type{}, js, ok);
+      });
+  }
+    hx[hh] = ax;
+    hy[hh] = ay;
   }
   const double ax = halves_sum<TM>(hx), ay = halves_sum<TM>(hy);
   if (t != 0) return;
@@ -3936,14 +3989,30 @@ static int alloc_sets(gfsph_ctx *c, int n) {
 static int alloc_bingham(gfsph_ctx *c, int n) {
   int rc;
   BinghamState &b = c->bs;
-  c->lv_blocks = (n + LV_TILE - 1) / LV_TILE;
+  // scan tiles of LV_TILE ids inside each realisation (ids are stacked in
+  // realisation order)
+  std::vector<int4> tiles;
+  std::vector<int> rtile(c->R + 1, 0);
+  int first = 0;
+  for (int r = 0; r < c->R; r++) {
+    const int nr = c->R == 1 ? n : c->counts_h[r];
+    rtile[r] = (int)tiles.size();
+    for (int b0 = first; b0 < first + nr; b0 += LV_TILE) tiles.push_back(make_int4(r, b0, std::min(b0 + LV_TILE, first + nr), 0));
+    first += nr;
+  }
+  rtile[c->R] = (int)tiles.size();
+  b.ntiles = (int)tiles.size();
+  c->lv_blocks = b.ntiles;
   if ((rc = dalloc(c, &b.in, n)) || (rc = dalloc(c, &b.x, n)) || (rc = dalloc(c, &b.v, n)) ||
       (rc = dalloc(c, &b.ex, n)) || (rc = dalloc(c, &b.u, n)) || (rc = dalloc(c, &b.du, n)) ||
       (rc = dalloc(c, &b.D, 5 * (size_t)n)) || (rc = dalloc(c, &b.fl, n)) || (rc = dalloc(c, &b.cand, n)) ||
-      (rc = dalloc(c, &b.tau, n)) || (rc = dalloc(c, &b.carry, 1)) || (rc = dalloc(c, &b.bsum, c->lv_blocks)))
+      (rc = dalloc(c, &b.tau, n)) || (rc = dalloc(c, &b.carry, c->R)) || (rc = dalloc(c, &b.bsum, b.ntiles)) ||
+      (rc = dalloc(c, &b.tiles, b.ntiles)) || (rc = dalloc(c, &b.rtile, c->R + 1)))
     return rc;
-  double zero = 0.0;  // Tau0 before any intermediate-regime particle
-  HIPCHK(c, hipMemcpy(b.carry, &zero, sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> zero(c->R, 0.0);  // Tau0 before any intermediate-regime particle
+  HIPCHK(c, hipMemcpy(b.carry, zero.data(), sizeof(double) * c->R, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.tiles, tiles.data(), sizeof(int4) * b.ntiles, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.rtile, rtile.data(), sizeof(int) * (c->R + 1), hipMemcpyHostToDevice));
   return GFSPH_OK;
 }
 
@@ -4058,13 +4127,14 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
   BinghamState &b = c->bs;
   k_bingham_init<<<nb, GF_BLOCK, 0, st>>>(n, s, c->ain, b);
   for (int k = 0; k < c->P.nstages; k++) {
-    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 0, k, c->P, c->Pr, c->dem, s, c->S, b);
-    k_lv_reduce<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum);
-    k_lv_top<<<1, 64, 0, st>>>(c->lv_blocks, b.bsum, b.carry);
-    k_lv_down<<<c->lv_blocks, GF_BLOCK, 0, st>>>(n, b.cand, b.bsum, b.tau);
-    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->P, c->Pr, c->dem, s, c->S, b);
+    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 0, k, c->Pr, c->dem, s, c->S, b);
+    k_lv_reduce<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, b.cand, b.bsum);
+    k_lv_top<<<(c->R + 63) / 64, 64, 0, st>>>(c->R, b.rtile, b.bsum, b.carry);
+    k_lv_down<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, b.cand, b.bsum, b.tau);
+    k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->Pr, c->dem, s, c->S, b);
   }
-  k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->P, c->Pr, c->dem, s, c->S, b, c->nw, c->part);
+  k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, b, c->nw,
+                                                  c->R == 1 ? c->part : nullptr);
   launch_column(c, s);
 }
 
@@ -4282,8 +4352,8 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
     if (cstR) memcpy(pr.cst, cstR + 111 * (size_t)r, sizeof(pr.cst));
     rc = derive_params(&pr, Ph[r], err);
     if (!rc && R > 1) {
-      if (Ph[r].nfric == 4) {
-        err = "Bingham (nfric = 4) couples the particles of a run; it is not batched";
+      if ((Ph[r].nfric == 4) != (Ph[0].nfric == 4)) {
+        err = "a batch is either all Bingham (nfric = 4, its Tau0 carried per realisation) or none";
         rc = GFSPH_ERR_UNSUPPORTED;
       } else if (Ph[r].npwp != Ph[0].npwp) {
         err = "the realisations of a batch must share the pore-pressure column size";

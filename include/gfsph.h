@@ -180,7 +180,9 @@ int gfsph_mc_pixel_read(gfsph_ctx *ctx, double *acc /*[npoig][4]*/, int32_t *cnt
  * (cst == NULL: p->cst for all; Update_Const_SW, 6:20035).  The switches of
  * p are shared, and so must be const(1) and const(14).  Realisations never
  * interact; each advances exactly as gfsph_create + gfsph_step would advance
- * it alone.  Bingham (nfric = 4) is not batched (GFSPH_ERR_UNSUPPORTED).
+ * it alone.  Bingham (nfric = 4): either every realisation or none; each
+ * carries its own Tau0 from particle to particle and call to call, as a run
+ * alone does (a batch mixing Bingham with other laws: GFSPH_ERR_UNSUPPORTED).
  * gfsph_get_state returns all particles in the same stacked order. */
 int gfsph_batch_create(const gfsph_params *p, const gfsph_dem *dem, int32_t nreal, const int32_t *counts,
                        const double *cst, const double *x, const double *v, const double *h,

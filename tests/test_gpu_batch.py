@@ -122,11 +122,50 @@ def test_batch_pixels_and_hazard():
     assert hz["p_exceed"].max() > 0
 
 
+def bingham_realisations(R=4, seed=5, integrator=2):
+    """Bingham realisations with tau_y ~ U(500, 5000) Pa and mu_B ~ U(10, 200)
+    Pa s (the calibration workload's ranges) and releases of uneven size."""
+    dem = fractal_dem(129, 2.0, 31, relief=60.0)
+    rng = np.random.default_rng(seed)
+    sizes = [400, 1100, 250, 800, 600][:R]
+    out = []
+    for r in range(R):
+        tau, mub = float(rng.uniform(500.0, 5000.0)), float(rng.uniform(10.0, 200.0))
+        parts = C.elliptical_scar(60.0 + rng.uniform(-20, 20), 128.0 + rng.uniform(-30, 30), sizes[r], 1.0, 2.0,
+                                  seed + r, h_range=(1.0, 4.0))
+        prm = C.Params(integrator=integrator, dt=0.02, h_inf=1e-3, const=C.bingham(tau, mub))
+        out.append(C.Case(f"b{r}", prm, dem, parts, 0))
+    return out
+
+
+@pytest.mark.parametrize("integrator,adapt", [(2, 0), (4, 0), (2, 2)])
+def test_bingham_batch_equals_single_runs(integrator, adapt):
+    """Bingham realisations as one batch: each realisation's Tau0 is carried
+    from particle to particle and call to call within that realisation only
+    (a segmented last-valid scan), so every realisation ends exactly where
+    it ends alone."""
+    cases = bingham_realisations(integrator=integrator)
+    for c in cases:
+        c.params.adapt_dt = adapt
+    with BatchModel(cases) as b:
+        b.time_integration_sph(60)
+        per = b.split(b.state())
+        tb, dtb = b.time()
+    for r, case in enumerate(cases):
+        with SPHModel(case) as m:
+            m.time_integration_sph(60)
+            st = m.state()
+            t, dt = m.time()
+        assert (t, dt) == (tb[r], dtb[r]), r
+        for k in ("x", "v", "h", "u", "hsml", "flag"):
+            np.testing.assert_array_equal(per[r][k], st[k], err_msg=f"realisation {r} {k}")
+
+
 def test_batch_rejects_bingham_and_mismatched_gravity():
     cases = realisations(R=2)
     bing = C.Case("b", C.Params(integrator=2, dt=0.02, h_inf=1e-3, const=C.bingham(1000.0, 50.0)), cases[0].dem,
                   cases[0].particles, 0)
-    with pytest.raises(Exception, match="unsupported|Bingham"):
+    with pytest.raises(Exception, match="unsupported|Bingham"):  # Bingham mixed with Voellmy
         BatchModel([cases[0], bing])
     g2 = dict(C.voellmy(0.2, 400.0))
     g2[1] = 9.7
