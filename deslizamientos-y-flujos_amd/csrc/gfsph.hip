@@ -1631,13 +1631,16 @@ __global__ void k_lv_down(const int4 *tiles, const double *cand, const double *b
 #define XSPH_LB __launch_bounds__(GATHER_MAX_BLOCK)
 #endif
 
-// Every gather splits a particle's candidate stream into two halves: the
-// 4-position batches 0, 2, 4, ... and 1, 3, 5, ... (the force and XSPH passes:
-// the neighbours in those batches).  Each half is summed in stream order and
-// the result is half 0 + half 1.  A context runs the halves either on two adjacent lanes
-// (TM = 2: more loads in flight when the particles alone fill each SIMD only
-// a few waves deep) or one after the other on one lane (TM = 1: large
-// batches); both give the same bits, for every work-group size.
+// Every gather splits its sum into two halves, each summed in stream order,
+// and the result is half 0 + half 1.  The density pass (which walks stream
+// positions) puts the 4-position batches 0, 2, 4, ... in half 0 and 1, 3,
+// 5, ... in half 1; the force and XSPH passes (which walk the neighbour
+// mask) put the neighbours of even ordinal among the set bits of their mask
+// word in half 0 and the others in half 1.  A context runs the halves
+// either on two adjacent lanes (TM = 2: more loads in flight when the
+// particles alone fill each SIMD only a few waves deep) or both on one lane
+// (TM = 1: large batches); both give the same bits, for every work-group
+// size.
 #define SPLIT 2
 // particles from which one lane per particle is the default
 #ifndef TEAM1_MIN_N
@@ -1676,10 +1679,18 @@ __device__ __forceinline__ unsigned long long halves_or(unsigned long long v) {
   }
 }
 
-// mask bits of half h (nibbles of NB_BATCH = 4 positions)
-__device__ __forceinline__ unsigned long long split_bits(int h) {
-  static_assert(NB_BATCH == 4, "the halves are made of 4-position nibbles");
-  return 0x0F0F0F0F0F0F0F0Full << (4 * h);
+// The neighbours of mask word v of ordinal parity h among its set bits (h = 0:
+// the first, third, ... set bit): v masked by the inclusive prefix parity of
+// its bits (or its complement).
+__device__ __forceinline__ unsigned long long ordinal_half(unsigned long long v, int h) {
+  unsigned long long x = v;
+  x ^= x << 1;
+  x ^= x << 2;
+  x ^= x << 4;
+  x ^= x << 8;
+  x ^= x << 16;
+  x ^= x << 32;
+  return v & (h ? ~x : x);
 }
 
 struct CandStream {
@@ -1811,38 +1822,33 @@ __device__ __forceinline__ void load_words(const CandStream &cs, const unsigned 
 // Calls f(js, ok) for batches of NB_BATCH neighbours taken from the mask
 // words w of a particle in stream order; js are slots.  Unused entries of a
 // batch have ok = false and js = self (a valid index), so f may read them
-// unconditionally.  Emptied words are shifted out at batch boundaries.
-template <bool HALVES = false, typename F>
-__device__ __forceinline__ void masked_batches(const CandStream &cs, unsigned long long (&w)[NB_WORDS], int self,
-                                               F &&f) {
-  int pbase = 0;
+// unconditionally.  A batch takes its neighbours from one word (an emptied
+// word ends the batch and is shifted out before the next), so entry u of a
+// batch has the ordinal parity u & 1 among its word's set bits.
+template <typename F>
+__device__ __forceinline__ void word_batches(const CandStream &cs, unsigned long long (&w)[NB_WORDS], int self,
+                                             F &&f) {
+  const int nw = (cs.total + 63) >> 6;
+  int k = 0;
   for (;;) {
-    unsigned long long rest = 0ull;
+    while (w[0] == 0ull && k + 1 < nw) {
 #pragma unroll
-    for (int k = 1; k < NB_WORDS; k++) rest |= w[k];
-    if (!(w[0] | rest)) break;
-    if (!w[0]) {
-#pragma unroll
-      for (int k = 0; k + 1 < NB_WORDS; k++) w[k] = w[k + 1];
+      for (int m = 0; m + 1 < NB_WORDS; m++) w[m] = w[m + 1];
       w[NB_WORDS - 1] = 0ull;
-      pbase += 64;
+      k++;
     }
+    if (w[0] == 0ull) break;
+    const int pbase = k << 6;
     int js[NB_BATCH];
     bool ok[NB_BATCH];
-    int hs[NB_BATCH];  // half of the entry: parity of its 4-position nibble
 #pragma unroll
     for (int u = 0; u < NB_BATCH; u++) {
       ok[u] = w[0] != 0ull;
-      const int pos = pbase + __ffsll((long long)w[0]) - 1;
-      const int sl = slot_of(cs, pos);
+      const int sl = slot_of(cs, pbase + __ffsll((long long)w[0]) - 1);
       js[u] = ok[u] ? sl : self;
-      hs[u] = (pos >> 2) & 1;
       w[0] &= w[0] - 1ull;
     }
-    if constexpr (HALVES)
-      f(js, ok, hs);
-    else
-      f(js, ok);
+    f(js, ok);
   }
 }
 
@@ -1862,6 +1868,23 @@ __device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, 
   }
 }
 
+// r = sqrt(dr) for the pair sums: the hardware reciprocal-root estimate
+// (relative error below 2^-24), one Newton-Raphson step and one correction
+// of the root.  An emulation in exact arithmetic over 20,000 arguments with
+// estimates at that error bound gave the correctly rounded root every time.
+// dr = 0 (a particle and itself, or two coincident particles) is taken as
+// 2^-1000: its root 2^-500 gives the kernel's q = 0 values exactly.
+__device__ __forceinline__ double root_pair(double dr) {
+  dr = fmax(dr, 0x1p-1000);
+  double y = __builtin_amdgcn_rsq(dr);
+  const double hx = 0.5 * dr;
+  const double e = fma(-(hx * y), y, 0.5);
+  y = fma(y, e, y);
+  double g = dr * y;
+  const double d = fma(-g, g, dr);
+  return fma(d, 0.5 * y, g);
+}
+
 // Kernel value alone (the density and XSPH sums) for a compile-time kernel
 // type, from the squared separation dr: the polynomials of kernel_TOOLS
 // ("5a My_SPH_Tools_2019.f90":2062-2180) in nested form, one branch-free
@@ -1871,8 +1894,7 @@ __device__ __forceinline__ void stream_dense(const CandStream &cs, int scale_k, 
 template <int SKF, bool UH = false>
 __device__ __forceinline__ double sph_w_t(double kc, double pi, double dr, double hsml, double ihu = 0.0) {
   if (SKF == 1 || SKF == 4) {
-    double r, ir;
-    root_nr(dr, r, ir);
+    const double r = root_pair(dr);
     const double ih = UH ? ihu : rcp_nr(hsml);
     const double q = r * ih;
     const double factor = kc * (ih * ih);
@@ -1890,6 +1912,38 @@ __device__ __forceinline__ double sph_w_t(double kc, double pi, double dr, doubl
     double w, gx, gy;
     sph_kernel(SKF, pi, sqrt(dr), 0.0, 0.0, hsml, w, gx, gy);
     return w;
+  }
+}
+
+// Kernel gradient alone for a pair sum, rounded as sph_kernel_t rounds it: the cubic spline's inner piece needs no 1/r, the
+// outer one takes 1/r from two Newton-Raphson steps on the estimate.
+template <int SKF, bool UH = false>
+__device__ __forceinline__ void sph_grad_nb(double kc, double pi, double dr, double dx, double dy, double hsml,
+                                            double ihu, double &gx, double &gy) {
+  if (SKF == 1) {
+    dr = fmax(dr, 0x1p-1000);
+    double y = __builtin_amdgcn_rsq(dr);
+    const double hx = 0.5 * dr;
+    double e = fma(-(hx * y), y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-(hx * y), y, 0.5);
+    y = fma(y, e, y);
+    double r = dr * y;
+    const double d = fma(-r, r, dr);
+    r = fma(d, 0.5 * y, r);
+    const double ih = UH ? ihu : rcp_nr(hsml);
+    const double ih2 = ih * ih;
+    const double q = r * ih;
+    const double factor = kc * ih2;
+    const double a1 = factor * (-2. + 1.5 * q) * ih2;
+    const double t = 2. - q;
+    const double a2 = -(factor * 0.5 * (t * t) * ih) * y;
+    const double a = q <= 1.0 ? a1 : (q <= 2.0 ? a2 : 0.0);
+    gx = a * dx;
+    gy = a * dy;
+  } else {
+    double w;
+    sph_kernel_t<SKF, UH>(kc, pi, dr, dx, dy, hsml, w, gx, gy, ihu);
   }
 }
 
@@ -2072,64 +2126,27 @@ __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double Pi = Pp[i];
   double ax[SPLIT / TM], ay[SPLIT / TM];
-  bool done = false;
-  if constexpr (TM == 1) {
-    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
-      double hx[2] = {0.0, 0.0}, hy[2] = {0.0, 0.0};
-      unsigned long long w[NB_WORDS];
-      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
-      const double ihi = rcp_nr(hi);
-      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
-        constexpr bool UH = decltype(uh)::value;
-        double4 q[NB_BATCH];
-        double pj[NB_BATCH], sj[NB_BATCH];
 #pragma unroll
-        for (int u = 0; u < NB_BATCH; u++) {
-          q[u] = s.nb[js[u]];
-          pj[u] = Pp[js[u]];
-          sj[u] = PA == 2 ? 0.0 : S[js[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < NB_BATCH; u++) {
-          double dx, dy, dr, mh, w, gx, gy;
-          pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-          sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
-          double tx = hs[u] ? hx[1] : hx[0], ty = hs[u] ? hy[1] : hy[0];
-          pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
-          hx[0] = (ok[u] && !hs[u]) ? tx : hx[0];
-          hy[0] = (ok[u] && !hs[u]) ? ty : hy[0];
-          hx[1] = (ok[u] && hs[u]) ? tx : hx[1];
-          hy[1] = (ok[u] && hs[u]) ? ty : hy[1];
-        }
-      };
-      if (g.uniform_h)
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
-      else
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
-      ax[0] = hx[0];
-      ay[0] = hy[0];
-      ax[1] = hx[1];
-      ay[1] = hy[1];
-      done = true;
-    }
-  }
-#pragma unroll
-  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
-    const int h = t + hh * TM;
-    double inx = 0.0, iny = 0.0;
+  for (int hh = 0; hh < SPLIT / TM; hh++) ax[hh] = ay[hh] = 0.0;
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb,
-                 [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-                   double w, gx, gy;
-                   sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
-                   pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, inx, iny);
-                 });
+#pragma unroll
+    for (int hh = 0; hh < SPLIT / TM; hh++)
+      stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
+                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                     double w, gx, gy;
+                     sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                     pforce_term<PA>(Pi, rfi, Pp[j], PA == 2 ? 0.0 : S[j], P.h_inf, q.w, gx, gy, ax[hh], ay[hh]);
+                   });
   } else {
     unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
+    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+    if constexpr (TM == 2) {
+#pragma unroll
+      for (int k = 0; k < NB_WORDS; k++) w[k] = ordinal_half(w[k], t);
+    }
     const double ihi = rcp_nr(hi);
+    // an unused entry (ok = false: the particle itself) adds m = 0 times a
+    // finite term, which leaves the sum's bits as they are
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
       double4 q[NB_BATCH];
@@ -2142,26 +2159,21 @@ __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
       }
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++) {
-        double dx, dy, dr, mh, w, gx, gy;
+        double dx, dy, dr, mh, gx, gy;
         pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-        sph_kernel_t<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy, ihi);
-        double tx = inx, ty = iny;
-        pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, q[u].w, gx, gy, tx, ty);
-        inx = ok[u] ? tx : inx;
-        iny = ok[u] ? ty : iny;
+        sph_grad_nb<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, ihi, gx, gy);
+        const int hh = TM == 1 ? (u & 1) : 0;
+        pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, ok[u] ? q[u].w : 0.0, gx, gy, ax[hh], ay[hh]);
       }
     };
     if (g.uniform_h)
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::true_type{}, js, ok);
       });
     else
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::false_type{}, js, ok);
       });
-  }
-    ax[hh] = inx;
-    ay[hh] = iny;
   }
   const double inx = halves_sum<TM>(ax), iny = halves_sum<TM>(ay);
   if (t == 0) ain[i] = make_double2(inx, iny);
@@ -2190,70 +2202,29 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
   const double rhoi = o.h[i];
   const bool skip0 = P.npoin0 > 0;
   double hx[SPLIT / TM], hy[SPLIT / TM];
-  bool done = false;
-  if constexpr (TM == 1) {
-    if (!stream_full(cs)) {  // one pass over every neighbour, each summed into its half
-      double sx[2] = {0.0, 0.0}, sy[2] = {0.0, 0.0};
-      unsigned long long w[NB_WORDS];
-      load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
-      const double ihi = rcp_nr(hi);
-      auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH], const int(&hs)[NB_BATCH]) {
-        constexpr bool UH = decltype(uh)::value;
-        double4 q[NB_BATCH];
-        double2 vj[NB_BATCH];
-        double hj[NB_BATCH];
-        bool use[NB_BATCH];
 #pragma unroll
-        for (int u = 0; u < NB_BATCH; u++) {
-          q[u] = s.nb[js[u]];
-          vj[u] = o.vel[js[u]];
-          hj[u] = o.h[js[u]];
-          use[u] = ok[u] && !(skip0 && s.id[js[u]] < P.npoin0);
-        }
-#pragma unroll
-        for (int u = 0; u < NB_BATCH; u++) {
-          double dx, dy, dr, mh;
-          pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
-          const double w = sph_w_t<SKF, UH>(P.kc, P.pi, dr, mh, ihi);
-          const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
-          const double cx = hs[u] ? sx[1] : sx[0], cy = hs[u] ? sy[1] : sy[0];
-          const double tx = cx - c * (vi.x - vj[u].x), ty = cy - c * (vi.y - vj[u].y);
-          sx[0] = (use[u] && !hs[u]) ? tx : sx[0];
-          sy[0] = (use[u] && !hs[u]) ? ty : sy[0];
-          sx[1] = (use[u] && hs[u]) ? tx : sx[1];
-          sy[1] = (use[u] && hs[u]) ? ty : sy[1];
-        }
-      };
-      if (g.uniform_h)
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::true_type{}, js, ok, hs); });
-      else
-        masked_batches<true>(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH],
-                                           const int(&hs)[NB_BATCH]) { body(std::false_type{}, js, ok, hs); });
-      hx[0] = sx[0];
-      hy[0] = sy[0];
-      hx[1] = sx[1];
-      hy[1] = sy[1];
-      done = true;
-    }
-  }
-#pragma unroll
-  for (int hh = 0; hh < SPLIT / TM && !done; hh++) {
-    const int h = t + hh * TM;
-    double ax = 0.0, ay = 0.0;
+  for (int hh = 0; hh < SPLIT / TM; hh++) hx[hh] = hy[hh] = 0.0;
   if (stream_full(cs)) {
-    stream_dense(cs, P.scale_k, i, h, pi_, hi, s.nb, [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
-      if (skip0 && s.id[j] < P.npoin0) return;
-      const double w = sph_w_t<SKF>(P.kc, P.pi, dr, mh);
-      const double2 vj = o.vel[j];
-      const double c = (2 * q.w) * w / (rhoi + o.h[j]);
-      ax = ax - c * (vi.x - vj.x);
-      ay = ay - c * (vi.y - vj.y);
-    });
+#pragma unroll
+    for (int hh = 0; hh < SPLIT / TM; hh++)
+      stream_dense(cs, P.scale_k, i, t + hh * TM, pi_, hi, s.nb,
+                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                     if (skip0 && s.id[j] < P.npoin0) return;
+                     const double w = sph_w_t<SKF>(P.kc, P.pi, dr, mh);
+                     const double2 vj = o.vel[j];
+                     const double c = (2 * q.w) * w / (rhoi + o.h[j]);
+                     hx[hh] = hx[hh] - c * (vi.x - vj.x);
+                     hy[hh] = hy[hh] - c * (vi.y - vj.y);
+                   });
   } else {
     unsigned long long w[NB_WORDS];
-    load_words(cs, mask + NB_WORDS * (long)i, w, split_bits(h));
+    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+    if constexpr (TM == 2) {
+#pragma unroll
+      for (int k = 0; k < NB_WORDS; k++) w[k] = ordinal_half(w[k], t);
+    }
     const double ihi = rcp_nr(hi);
+    // an unused entry adds c = 0 times a finite difference (bits unchanged)
     auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
       constexpr bool UH = decltype(uh)::value;
       double4 q[NB_BATCH];
@@ -2272,23 +2243,21 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
         double dx, dy, dr, mh;
         pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
         const double w = sph_w_t<SKF, UH>(P.kc, P.pi, dr, mh, ihi);
-        const double c = (2 * q[u].w) * w * rcp_nr(rhoi + hj[u]);
-        const double tx = ax - c * (vi.x - vj[u].x), ty = ay - c * (vi.y - vj[u].y);
-        ax = use[u] ? tx : ax;
-        ay = use[u] ? ty : ay;
+        const double mj = use[u] ? q[u].w : 0.0;
+        const double c = (2 * mj) * w * rcp_nr(rhoi + hj[u]);
+        const int hh = TM == 1 ? (u & 1) : 0;
+        hx[hh] = hx[hh] - c * (vi.x - vj[u].x);
+        hy[hh] = hy[hh] - c * (vi.y - vj[u].y);
       }
     };
     if (g.uniform_h)
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::true_type{}, js, ok);
       });
     else
-      masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::false_type{}, js, ok);
       });
-  }
-    hx[hh] = ax;
-    hy[hh] = ay;
   }
   const double ax = halves_sum<TM>(hx), ay = halves_sum<TM>(hy);
   if (t != 0) return;
@@ -2392,7 +2361,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_stg_k0(int n, const DevParams *Pr,
   } else {
     unsigned long long w[NB_WORDS];
     load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
-    masked_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+    word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++) {
         if (!ok[u]) continue;
@@ -2840,7 +2809,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
   } else {
     unsigned long long mw[NB_WORDS];
     load_words(cs, mrow, mw, ~0ull);
-    masked_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    word_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
         if (ok[u]) term(js[u], t.nb[js[u]]);
@@ -2956,7 +2925,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
   } else {
     unsigned long long mw[NB_WORDS];
     load_words(cs, mrow, mw, ~0ull);
-    masked_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+    word_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
       for (int u = 0; u < NB_BATCH; u++)
         if (ok[u]) term(js[u], t.nb[js[u]]);
