@@ -372,9 +372,13 @@ def save_mc_hazard(out_dir=HERE, nsteps=100, plot_every=10):
 # Ccases_mat, write_master options).  t2: every combination (ic_cases_mat 2,
 # Driver_Flow_23 with steps cut at the plot times), adaptive dt mode 2; t1:
 # one variable at a time (ic_cases_mat 1, Driver_Flow), adaptive dt mode 1.
+# t2's time stop (2.53 s) lies inside a step: its steps are cut to end on the
+# plot times 0.5, 1.0, ..., where time_sph equals a multiple of 0.5 only to
+# within rounding, so a stop time on a plot time would decide the stop by
+# the last bit of time_sph.
 CASE_TABLES = {
     "t2": ([[9, 0.2, 3, 0.12, 0.28, 1], [501, 1.0, 2, 0.8, 1.2, 1]],
-           dict(adapt=2, cases=2, stop=(0.05, 2.5, 0.0), dt_driver=0.5, plot_step=1, time_end=1e9)),
+           dict(adapt=2, cases=2, stop=(0.05, 2.53, 0.0), dt_driver=0.5, plot_step=1, time_end=1e9)),
     "t1": ([[9, 0.2, 2, 0.15, 0.3, 1, 0, 0], [3, 500.0, 2, 300.0, 900.0, 1, 0, 0]],
            dict(adapt=1, cases=1, stop=(0.1, 1.7, 0.0), dt_driver=0.05, plot_step=1000000, time_end=1e9)),
 }
