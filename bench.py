@@ -53,9 +53,10 @@ PEAK_HBM_GBS = 8000.0
 B_ALG = {"neighbour_build": 64.0, "k_density": 44.0, "k_pforce": 56.0, "k_rk": 128.0, "k_xsph": 72.0}
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
-# this workload (200k particles, first 300 steps), each scaled by what the
-# counter reads for a device copy of known size (fetch x2.0, write x1.0)
-MEASURED_TRAFFIC_200K = {"k_density": 32.22e6, "k_pforce": 33.21e6, "k_xsph": 38.48e6, "k_rk": 35.75e6}
+# this workload (200k particles, the steps of the developed-flow window after
+# 1,000 untimed steps), each scaled by what the counter reads for a device
+# copy of known size (fetch x2.0, write x1.0)
+MEASURED_TRAFFIC_200K = {"k_density": 32.69e6, "k_pforce": 33.33e6, "k_xsph": 38.63e6, "k_rk": 50.00e6}
 
 
 def cpu_info() -> dict:
