@@ -2762,12 +2762,27 @@ __global__ void k_adj_load(int n, int npoin0, TapeRec t, const double2 *LX, cons
   w.act[i] = (!t.sflag[i] && !t.oflag[i] && k >= npoin0) ? 1 : 0;
 }
 
+// Adjoint gathers run two lanes per particle: lane t sums the neighbours of
+// ordinal parity t in their mask words (or stream positions t, t+2, ... of a
+// full stream), and the lanes add their partial sums in a fixed order, so the
+// result does not depend on the work-group size.  Loads of a batch are issued
+// for every entry (an unused one reads the particle itself) and the sums take
+// only the used entries.
+#define ADJ_TM 2
+
+template <int NS>
+__device__ __forceinline__ void adj_pair_sum(double (&a)[NS]) {
+#pragma unroll
+  for (int k = 0; k < NS; k++) a[k] = a[k] + __shfl_xor(a[k], 1);
+}
+
 // K2: XSPH adjoint (gather over the step's neighbour masks)
 __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeRec t, AdjWork w) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i, ln;
+  team_index<ADJ_TM>(i, ln);
   if (i >= n) return;
   if (t.sflag[i]) {
-    w.lvp[i] = w.lvpp[i];
+    if (ln == 0) w.lvp[i] = w.lvpp[i];
     return;
   }
   const Cells cl = tape_cells(t);
@@ -2776,51 +2791,61 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
   const CandStream cs = cand_stream(g, cl.start, pi_, hi, P.scale_k);
-  const bool full = stream_full(cs);
   const unsigned long long *mrow = t.mask + NB_WORDS * (long)i;
   const double2 vi = t.vp[i], li = w.lvpp[i];
   const double Si = t.S[i];
   const int acti = w.act[i];
-  double csum = 0.0, lSi = 0.0, lmi = 0.0, gxa = 0.0, gya = 0.0, lvx = 0.0, lvy = 0.0;
-  auto term = [&](int j, const double4 &q) {
+  // csum, lSi, lmi, gxa, gya, lvx, lvy
+  double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  auto term = [&](bool ok, const double4 &q, const double2 vj, const double2 lj, double Sj, int actj) {
     double dx, dy, dr, mh, wv, gx, gy;
     pair_geom(pi_, hi, q, dx, dy, dr, mh);
     sph_kernel_t<1>(P.kc, P.pi, dr, dx, dy, mh, wv, gx, gy);
-    const double2 vj = t.vp[j], lj = w.lvpp[j];
-    const double Ssum = Si + t.S[j];
+    const double Ssum = Si + Sj;
     const double cij = (2 * q.w) * wv / Ssum, cji = (2 * mi) * wv / Ssum;
     const double dvx = vj.x - vi.x, dvy = vj.y - vi.y;
-    const double pi_side = acti ? li.x * dvx + li.y * dvy : 0.0;        // lambda_i . (v_j - v_i)
-    const double pj_side = w.act[j] ? -(lj.x * dvx + lj.y * dvy) : 0.0;  // lambda_j . (v_i - v_j)
-    if (acti) csum += cij;
-    if (w.act[j]) {
-      lvx += cji * lj.x;
-      lvy += cji * lj.y;
-    }
-    lSi += -(pi_side * cij + pj_side * cji) / Ssum;
-    lmi += pj_side * 2.0 * wv / Ssum;
+    const double pi_side = acti ? li.x * dvx + li.y * dvy : 0.0;       // lambda_i . (v_j - v_i)
+    const double pj_side = actj ? -(lj.x * dvx + lj.y * dvy) : 0.0;    // lambda_j . (v_i - v_j)
     const double f = pi_side * 2.0 * q.w / Ssum + pj_side * 2.0 * mi / Ssum;
-    gxa += f * gx;
-    gya += f * gy;
+    const double c[7] = {acti ? cij : 0.0, -(pi_side * cij + pj_side * cji) / Ssum, pj_side * 2.0 * wv / Ssum,
+                         f * gx, f * gy, actj ? cji * lj.x : 0.0, actj ? cji * lj.y : 0.0};
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] = ok ? acc[k] + c[k] : acc[k];
   };
-  if (full) {
-    stream_dense<1>(cs, P.scale_k, i, 0, pi_, hi, t.nb,
-                 [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
+  if (stream_full(cs)) {
+    stream_dense(cs, P.scale_k, i, ln, pi_, hi, t.nb, [&](int j, const double4 &q, double, double, double, double) {
+      term(true, q, t.vp[j], w.lvpp[j], t.S[j], w.act[j]);
+    });
   } else {
     unsigned long long mw[NB_WORDS];
     load_words(cs, mrow, mw, ~0ull);
-    word_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
-      for (int u = 0; u < NB_BATCH; u++)
-        if (ok[u]) term(js[u], t.nb[js[u]]);
+    for (int k = 0; k < NB_WORDS; k++) mw[k] = ordinal_half(mw[k], ln);
+    word_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+      double4 q[NB_BATCH];
+      double2 vj[NB_BATCH], lj[NB_BATCH];
+      double Sj[NB_BATCH];
+      int aj[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        q[u] = t.nb[js[u]];
+        vj[u] = t.vp[js[u]];
+        lj[u] = w.lvpp[js[u]];
+        Sj[u] = t.S[js[u]];
+        aj[u] = w.act[js[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) term(ok[u], q[u], vj[u], lj[u], Sj[u], aj[u]);
     });
   }
-  const double keep = acti ? 1.0 - 0.3 * csum : 1.0;
-  w.lvp[i] = make_double2(li.x * keep + 0.3 * lvx, li.y * keep + 0.3 * lvy);
-  w.lS[i] += 0.3 * lSi;
-  w.lm[i] += 0.3 * lmi;
+  adj_pair_sum(acc);
+  if (ln != 0) return;
+  const double keep = acti ? 1.0 - 0.3 * acc[0] : 1.0;
+  w.lvp[i] = make_double2(li.x * keep + 0.3 * acc[5], li.y * keep + 0.3 * acc[6]);
+  w.lS[i] += 0.3 * acc[1];
+  w.lm[i] += 0.3 * acc[2];
   const double2 lx = w.lx[i];
-  w.lx[i] = make_double2(lx.x + 0.3 * gxa, lx.y + 0.3 * gya);
+  w.lx[i] = make_double2(lx.x + 0.3 * acc[3], lx.y + 0.3 * acc[4]);
 }
 
 // K3: per-particle stage adjoint; mu / xi cotangents accumulate per block
@@ -2882,12 +2907,15 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_stage(int n, DevParams P, DevD
 // x^n, v^n and the mass back to caller order
 __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, TapeRec t, AdjWork w, double2 *LX,
                                                         double2 *LV, double *LM) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i, ln;
+  team_index<ADJ_TM>(i, ln);
   if (i >= n) return;
   const int k = t.id[i];
   if (t.sflag[i]) {
-    LX[k] = w.lxn[i];
-    LV[k] = w.lvn[i];
+    if (ln == 0) {
+      LX[k] = w.lxn[i];
+      LV[k] = w.lvn[i];
+    }
     return;
   }
   const Cells cl = tape_cells(t);
@@ -2896,44 +2924,54 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
   const CandStream cs = cand_stream(g, cl.start, pi_, hi, P.scale_k);
-  const bool full = stream_full(cs);
   const unsigned long long *mrow = t.mask + NB_WORDS * (long)i;
   const double2 lai = w.la[i];
   const double lSi = w.lS[i];
   const double grav = 2.0 * P.cgra05;
-  double lxx = 0.0, lxy = 0.0, lmi = 0.0;
-  auto term = [&](int j, const double4 &q) {
+  double acc[3] = {0.0, 0.0, 0.0};  // lxx, lxy, lmi
+  auto term = [&](bool ok, const double4 &q, const double2 laj, double lSj) {
     double dx, dy, dr, mh, wv, a, hxx, hxy, hyy;
     pair_geom(pi_, hi, q, dx, dy, dr, mh);
     kernel_hess(P.kc, dr, dx, dy, mh, wv, a, hxx, hxy, hyy);
-    const double2 laj = w.la[j];
     // pressure force: -g sum_j H_ij (m_j la_i - m_i la_j); mass: sum_j la_j . (g grad W_ij)
     const double bx = q.w * lai.x - mi * laj.x, by = q.w * lai.y - mi * laj.y;
-    lxx += -grav * (hxx * bx + hxy * by);
-    lxy += -grav * (hxy * bx + hyy * by);
-    lmi += grav * a * (laj.x * dx + laj.y * dy);
+    double cx = -grav * (hxx * bx + hxy * by);
+    double cy = -grav * (hxy * bx + hyy * by);
+    double cm = grav * a * (laj.x * dx + laj.y * dy);
     // density: sum_j (lS_i m_j + lS_j m_i) grad W_ij; mass: sum_j lS_j W_ij
-    const double lSj = w.lS[j];
     const double c = (lSi * q.w + lSj * mi) * a;
-    lxx += c * dx;
-    lxy += c * dy;
-    lmi += lSj * wv;
+    acc[0] = ok ? (acc[0] + cx) + c * dx : acc[0];
+    acc[1] = ok ? (acc[1] + cy) + c * dy : acc[1];
+    acc[2] = ok ? (acc[2] + cm) + lSj * wv : acc[2];
   };
-  if (full) {
-    stream_dense<1>(cs, P.scale_k, i, 0, pi_, hi, t.nb,
-                 [&](int j, const double4 &q, double, double, double, double) { term(j, q); });
+  if (stream_full(cs)) {
+    stream_dense(cs, P.scale_k, i, ln, pi_, hi, t.nb, [&](int j, const double4 &q, double, double, double, double) {
+      term(true, q, w.la[j], w.lS[j]);
+    });
   } else {
     unsigned long long mw[NB_WORDS];
     load_words(cs, mrow, mw, ~0ull);
-    word_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
 #pragma unroll
-      for (int u = 0; u < NB_BATCH; u++)
-        if (ok[u]) term(js[u], t.nb[js[u]]);
+    for (int kk = 0; kk < NB_WORDS; kk++) mw[kk] = ordinal_half(mw[kk], ln);
+    word_batches(cs, mw, i, [&](const int (&js)[NB_BATCH], const bool (&ok)[NB_BATCH]) {
+      double4 q[NB_BATCH];
+      double2 laj[NB_BATCH];
+      double lSj[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        q[u] = t.nb[js[u]];
+        laj[u] = w.la[js[u]];
+        lSj[u] = w.lS[js[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) term(ok[u], q[u], laj[u], lSj[u]);
     });
   }
-  lmi += lSi * sph_self_w(1, P.pi, hi);
+  adj_pair_sum(acc);
+  if (ln != 0) return;
+  const double lmi = acc[2] + lSi * sph_self_w(1, P.pi, hi);
   const double2 lxn = w.lxn[i], lxp = w.lx[i];
-  LX[k] = make_double2(lxn.x + lxp.x + lxx, lxn.y + lxp.y + lxy);
+  LX[k] = make_double2(lxn.x + lxp.x + acc[0], lxn.y + lxp.y + acc[1]);
   LV[k] = w.lvn[i];
   LM[k] += w.lm[i] + lmi;
 }
@@ -5216,12 +5254,12 @@ int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_
       const bool last = (seg == nseg - 1) && (t == L - 1);
       k_adj_load<<<nb, GF_BLOCK, 0, st>>>(n, c->P.npoin0, tr, c->LX, c->LV, last ? c->LH : nullptr, w);
       if (c->P.av_vel) {
-        k_adj_xsph<<<nb, GF_BLOCK, 0, st>>>(n, c->P, tr, w);
+        k_adj_xsph<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w);
       } else {
         HIPCHK(c, hipMemcpyAsync(w.lvp, w.lvpp, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
       }
       k_adj_stage<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->dem, tr, w, c->gmu, c->gxi);
-      k_adj_pairs<<<nb, GF_BLOCK, 0, st>>>(n, c->P, tr, w, c->LX, c->LV, c->LM);
+      k_adj_pairs<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w, c->LX, c->LV, c->LM);
     }
     HIPCHK(c, hipGetLastError());
   }
