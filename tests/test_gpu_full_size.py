@@ -68,6 +68,16 @@ def test_gpu_cfg1_divergence_growth():
             ints = np.array([st["x"][act, 0].mean(), st["x"][act, 1].mean(), sp.mean(), st["x"][act, 0].max(),
                              st["x"][act, 0].min(), st["h"][act].mean(), float(act.sum())])
             errs = {k: goldens.rel_err(st[k][idx], g[f"s{s}_{k}"]) for k in FIELDS}
+            if s in (1, 10, 100):
+                # per-element relative errors |a - b| / |b| over the sampled
+                # particles (elements with b = 0 left out), median and maximum
+                pe = []
+                for k in ("x", "v", "h"):
+                    a, b = np.ravel(st[k][idx]), np.ravel(g[f"s{s}_{k}"])
+                    nz = b != 0
+                    r = np.abs(a[nz] - b[nz]) / np.abs(b[nz])
+                    pe.append(f"{k} median {np.median(r):.1e} max {r.max():.1e}")
+                print(f"cfg1 step {s} per element: " + "; ".join(pe))
             fma = {k: goldens.rel_err(g[f"f{s}_{k}"], g[f"s{s}_{k}"]) for k in FIELDS}
             spread = {k: float(g[f"spread{s}_{k}"]) for k in FIELDS}
             full = None
