@@ -15,6 +15,12 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 TOL_EARLY = 1e-8
+# Where the oracle's own rounding spread (the same sums taken in reverse
+# order, or the reference built with fused multiply-adds) exceeds the bar,
+# the bound is SPREAD_FACTOR times that spread: the GPU and the oracle are
+# two roundings of the same arithmetic, each about a spread away from any
+# third, so their distance is at most the sum of two such distances.
+SPREAD_FACTOR = 2.0
 FULL_SUPPORT_ONLY = ("g_kernel_skf2", "g_kernel_skf3")
 
 
@@ -134,6 +140,7 @@ def test_gpu_rk2_matches_oracle(name):
     case.params.integrator = 2
     st = oracle.init(case)
     alt = st.copy()
+    done = 0
     with g.SPHModel(case) as m:
         for chunk in (1, 49, 50):
             m.time_integration_sph(chunk)
@@ -143,7 +150,9 @@ def test_gpu_rk2_matches_oracle(name):
             for k in ("x", "v", "h", "u"):
                 spread = goldens.rel_err(getattr(alt, k), getattr(st, k))
                 err = goldens.rel_err(got[k], getattr(st, k))
-                assert err <= max(TOL_EARLY if chunk < 50 else TOL, 10 * spread), (name, chunk, k, err, spread)
+                print(f"{name} RK2 step {done + chunk} {k}: {err:.2e} (oracle order spread {spread:.2e})")
+                assert err <= max(TOL_EARLY if chunk < 50 else TOL, SPREAD_FACTOR * spread), (name, chunk, k, err, spread)
+            done += chunk
 
 
 @pytest.mark.parametrize("name,mode,integrator", [("g_bingham", 1, 4), ("g_frictional_pwp", 4, 2),
@@ -167,7 +176,8 @@ def test_gpu_adaptive_dt_matches_oracle(name, mode, integrator):
             for k in ("x", "v", "h", "u"):
                 spread = goldens.rel_err(getattr(alt, k), getattr(st, k))
                 err = goldens.rel_err(got[k], getattr(st, k))
-                assert err <= max(TOL_EARLY if chunk < 40 else TOL, 10 * spread), (name, chunk, k, err, spread)
+                print(f"{name} adaptive dt {mode} chunk {chunk} {k}: {err:.2e} (oracle order spread {spread:.2e})")
+                assert err <= max(TOL_EARLY if chunk < 40 else TOL, SPREAD_FACTOR * spread), (name, chunk, k, err, spread)
             t, dt = m.time()
             assert dt == pytest.approx(st.dt_time[0], rel=1e-8)
             assert t == pytest.approx(st.dt_time[1], rel=1e-8)
@@ -202,19 +212,19 @@ def test_gpu_adaptive_dt_switch():
 
 
 def test_gpu_cfg1_full_size():
-    """cfg1 at full size against the reference fixture (pair-set hash after
-    99 steps, state after 100 steps within 10x the oracle's order spread)."""
+    """cfg1 at full size against the reference fixture: pair-set hash after 99
+    steps; state after 100 steps within max(1e-10, SPREAD_FACTOR x the
+    reference's own build-to-build spread), the spread being the distance of
+    the reference built with fused multiply-adds from the reference build,
+    whole field, at step 100 (tests/golden/g_cfg1_growth.npz)."""
     import hashlib
 
     g = _gpu()
     from gfsph import cases as C
 
     f = goldens.load("g_cfg1_full")
+    growth = goldens.load("g_cfg1_growth")
     case = C.cfg_plane_voellmy(nsteps=100)
-    st = oracle.init(case)
-    alt = st.copy()
-    oracle.run(case, st, 100)
-    oracle.run(case, alt, 100, rev_order=True)
     with g.SPHModel(case) as m:
         m.time_integration_sph(99)
         mid = m.state()
@@ -225,9 +235,10 @@ def test_gpu_cfg1_full_size():
         got = m.state()
     idx = f["s100_idx"]
     for k in ("x", "v", "h", "u"):
-        spread = goldens.rel_err(getattr(alt, k), getattr(st, k))
+        spread = float(growth[f"spread100_{k}"])
         err = goldens.rel_err(got[k][idx], f[f"s100_{k}"])
-        assert err <= max(TOL, 10 * spread), (k, err, spread)
+        print(f"cfg1 step 100 {k}: {err:.2e} (reference build-to-build spread {spread:.2e})")
+        assert err <= max(TOL, SPREAD_FACTOR * spread), (k, err, spread)
 
 
 def test_gpu_raster_idw_matches_oracle():
