@@ -2687,6 +2687,156 @@ __device__ __forceinline__ void stage_map(const DevParams &P, const DevDem &dem,
   voy = v0y + P.fin * Dvy;
 }
 
+// Bingham (nfric 4) in T, the same arithmetic as bingham_tau0,
+// bingham_candidate and ext_force's Bingham branch: the intermediate-regime
+// Tau0 from (tauy0, K) = (const(6), const(7)), the law's friction, the
+// limiter (at rest its friction bound is |friction| itself).
+template <typename T>
+__device__ __forceinline__ bool bingham_tau0_t(const T &tauy0, const T &K, const T &hvmod, const T &hfrict,
+                                               T &Tau0) {
+  const T a_hat = ((6. * K) * hvmod) / (tauy0 * (hfrict * hfrict));
+  if (val(a_hat) > 50 || val(a_hat) < 0.01) return false;
+  const double a2 = 48., c2 = 65.;
+  const T b2 = (-32.) * a_hat - 114.;
+  const T xh = (-b2 - tsqrt(b2 * b2 - 4. * a2 * c2)) / (2. * a2);
+  Tau0 = tauy0 / xh;
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ T tmax_const(const T &a, double b) {
+  return val(a) >= b ? a : tconst<T>(b);
+}
+
+template <typename T>
+__device__ __forceinline__ bool bingham_candidate_t(const DevParams &P, const T &rho_i, const T &vx1, const T &vy1,
+                                                    const T &tauy0, const T &K, T &cand) {
+  T h = rho_i, hvmod;
+  if (val(h) < P.comp) {
+    h = tconst<T>(P.comp);
+    hvmod = tconst<T>(0);
+  } else {
+    const T hu = h * vx1, hv = h * vy1;
+    hvmod = tsqrt(hu * hu + hv * hv);
+  }
+  return bingham_tau0_t(tauy0, K, hvmod, tmax_const(h, P.hfrict0), cand);
+}
+
+template <typename T>
+__device__ __forceinline__ void ext_bingham(const DevParams &P, const DevDem &dem, const T &xx, const T &yy,
+                                            const T &vx1, const T &vy1, const T &rho_i, const T &inx,
+                                            const T &iny, const T &tauy0, const T &K, const T &tau_in, T &exo,
+                                            T &eyo) {
+  const double dt = P.dt;
+  T h, hu, hv, hinv, vmod, hvmod, hvmodinv;
+  h = rho_i;
+  hu = h * vx1;
+  hv = h * vy1;
+  if (val(h) < P.comp) {
+    h = tconst<T>(P.comp);
+    hu = tconst<T>(0);
+    hv = tconst<T>(0);
+    hinv = tconst<T>(0);
+    vmod = tconst<T>(0);
+    hvmod = tconst<T>(0);
+    hvmodinv = tconst<T>(0);
+  } else {
+    hinv = 1 / h;
+    hvmod = tsqrt(hu * hu + hv * hv);
+    vmod = hvmod * hinv;
+    hvmodinv = (val(hvmod) >= 1.e-5) ? 1. / hvmod : tconst<T>(0);
+  }
+  T Zx, Zy;
+  slope_at(dem, xx, yy, Zx, Zy);
+  T exx = tconst<T>(0) + (-P.cgra) * Zx;
+  T exy = tconst<T>(0) + (-P.cgra) * Zy;
+  T Tau0 = tau_in, own;
+  if (bingham_tau0_t(tauy0, K, hvmod, tmax_const(h, P.hfrict0), own)) Tau0 = own;
+  const T cf4b = Tau0 / P.dens;
+  T frictx = cf4b * hu * hvmodinv;
+  T fricty = cf4b * hv * hvmodinv;
+  const T TotFrict = tsqrt(frictx * frictx + fricty * fricty);
+  T dcx = -frictx * hinv, dcy = -fricty * hinv;
+  T ax = exx + inx, ay = exy + iny;
+  if (val(vmod) > 1.e-5) {
+    T vsx = vx1 + dt * (ax + dcx), vsy = vy1 + dt * (ay + dcy);
+    T vdot = vsx * vx1 + vsy * vy1;
+    if (val(vdot) < 0) {
+      T tx = vx1 / vmod, ty = vy1 / vmod;
+      T accelT = ax * tx + ay * ty;
+      T deccelT = dcx * tx + dcy * ty;
+      if (val(accelT) * val(deccelT) < 0) {
+        dcx = -vx1 / dt - ax;
+        dcy = -vy1 / dt - ay;
+        frictx = -dcx * h;
+        fricty = -dcy * h;
+      }
+    }
+  } else {
+    T accelT = tsqrt(ax * ax + ay * ay);
+    if (val(accelT) > 1.e-5) {
+      frictx = TotFrict * ax / accelT;
+      fricty = TotFrict * ay / accelT;
+      T fricT = tsqrt(frictx * frictx + fricty * fricty);
+      if (val(accelT) <= val(fricT)) {
+        frictx = ax;
+        fricty = ay;
+      }
+    } else {
+      frictx = tconst<T>(0);
+      fricty = tconst<T>(0);
+    }
+    frictx = frictx * h;
+    fricty = fricty * h;
+  }
+  exo = exx - frictx / h;
+  eyo = exy - fricty / h;
+}
+
+// The Bingham stages of one particle given the Tau0 each stage resolved
+// (tau[st], taken where the particle is not itself in the intermediate
+// regime); cand[st] / ok[st]: the particle's own candidate at stage st.
+#define BING_MAX_STAGES 4
+template <typename T>
+__device__ __forceinline__ void bing_stage_map(const DevParams &P, const DevDem &dem, const T &x0x, const T &x0y,
+                                               const T &v0x, const T &v0y, const T &rf, const T &inx, const T &iny,
+                                               const T &tauy0, const T &K, const T (&tau)[BING_MAX_STAGES],
+                                               T &xox, T &xoy, T &vox, T &voy, T (&cand)[BING_MAX_STAGES],
+                                               bool (&ok)[BING_MAX_STAGES], bool &mid) {
+  const double dt = P.dt;
+  T xx = x0x, xy = x0y, vx = v0x, vy = v0y;
+  T ex = tconst<T>(0), ey = tconst<T>(0);
+  T Dx = tconst<T>(0), Dy = tconst<T>(0), Dvx = tconst<T>(0), Dvy = tconst<T>(0);
+  mid = false;
+#pragma unroll
+  for (int st = 0; st < BING_MAX_STAGES; st++) {
+    ok[st] = false;
+    cand[st] = tconst<T>(0);
+  }
+  for (int st = 0; st < P.nstages; st++) {
+    const double f1 = P.f1[st];
+    xx = x0x + f1 * dt * vx;
+    xy = x0y + f1 * dt * vy;
+    vx = v0x + f1 * dt * (ex + inx);
+    vy = v0y + f1 * dt * (ey + iny);
+    if (out_of_domain(dem, val(xx), val(xy))) {
+      mid = true;
+      break;
+    }
+    ok[st] = bingham_candidate_t(P, rf, vx, vy, tauy0, K, cand[st]);
+    ext_bingham(P, dem, xx, xy, vx, vy, rf, inx, iny, tauy0, K, tau[st], ex, ey);
+    const double f2 = P.f2[st];
+    Dvx = Dvx + f2 * (ex + inx);
+    Dvy = Dvy + f2 * (ey + iny);
+    Dx = Dx + f2 * vx;
+    Dy = Dy + f2 * vy;
+  }
+  xox = x0x + P.fin * Dx;
+  xoy = x0y + P.fin * Dy;
+  vox = v0x + P.fin * Dvx;
+  voy = v0y + P.fin * Dvy;
+}
+
 // cubic-spline kernel with its gradient factor a (grad W = a d) and Hessian
 // H = W'' dd^T/r^2 + (W'/r)(I - dd^T/r^2), from the squared separation dr
 __device__ __forceinline__ void kernel_hess(double kc, double dr, double dx, double dy, double hsml, double &w,
@@ -2730,7 +2880,8 @@ struct AdjWork {
 // recorded forward step (sorted order of that step)
 struct TapeRec {
   double4 *nb;
-  int *id, *sflag, *oflag;
+  int *id, *rid, *sflag, *oflag;
+  double *bt;  // Bingham: [2S][n] by caller id, each stage's resolved Tau0, then its candidates
   double *S;
   double2 *ain, *vp, *vn;  // vn: velocity at step start, caller order
   unsigned long long *mask;
@@ -2786,11 +2937,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
     return;
   }
   const Cells cl = tape_cells(t);
-  const GridInfo g = cl.grid[0];
+  const int rr = t.rid[i];
+  const GridInfo g = cl.grid[rr];
   const double4 qi = t.nb[i];
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
-  const CandStream cs = cand_stream(g, cl.start, pi_, hi, P.scale_k);
+  const CandStream cs = cand_stream(g, cl.start + cl.coff[rr], pi_, hi, P.scale_k);
   const unsigned long long *mrow = t.mask + NB_WORDS * (long)i;
   const double2 vi = t.vp[i], li = w.lvpp[i];
   const double Si = t.S[i];
@@ -2848,58 +3000,226 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_xsph(int n, DevParams P, TapeR
   w.lx[i] = make_double2(lx.x + 0.3 * acc[3], lx.y + 0.3 * acc[4]);
 }
 
-// K3: per-particle stage adjoint; mu / xi cotangents accumulate per block
-// (block b always adds to gmu[b], gxi[b]: a fixed summation order)
-__global__ __launch_bounds__(GF_BLOCK) void k_adj_stage(int n, DevParams P, DevDem dem, TapeRec t, AdjWork w,
-                                                        double *gmu, double *gxi) {
-  __shared__ double red[2][GF_BLOCK / 64];
+// K3: per-particle stage adjoint (Pr: the realisations' parameter blocks).
+// The cotangents of the law's two parameters accumulate per particle, by
+// caller id, over the steps (GP[id], GQ[id]); their sums per realisation are
+// taken in id order at the end, so no order depends on the launch.
+__global__ __launch_bounds__(GF_BLOCK) void k_adj_stage(int n, const DevParams *Pr, DevDem dem, TapeRec t,
+                                                        AdjWork w, double *GP, double *GQ) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double dmu = 0.0, dxi = 0.0;
-  if (i < n) {
-    const double2 lx = w.lxp[i], lv = w.lvp[i];
-    if (t.sflag[i]) {
-      w.lxn[i] = lx;
-      w.lvn[i] = lv;
-      w.la[i] = make_double2(0.0, 0.0);
-    } else {
-      const double4 q = t.nb[i];
-      const double2 v0 = t.vn[t.id[i]], a = t.ain[i];
-      const double Si = t.S[i];
-      const double rf = Si < P.h_inf ? P.h_inf : Si;
-      Dual X0 = dvar(q.x, 0), Y0 = dvar(q.y, 1), VX = dvar(v0.x, 2), VY = dvar(v0.y, 3), RF = dvar(rf, 4);
-      Dual AX = dvar(a.x, 5), AY = dvar(a.y, 6), MU = dvar(P.tanfi8, 7), XI = dvar(P.xi_voellmy, 8);
-      Dual xo, yo, vxo, vyo;
-      bool mid;
-      stage_map(P, dem, X0, Y0, VX, VY, RF, AX, AY, MU, XI, xo, yo, vxo, vyo, mid);
-      double g[AD_N];
+  if (i >= n) return;
+  const double2 lx = w.lxp[i], lv = w.lvp[i];
+  if (t.sflag[i]) {
+    w.lxn[i] = lx;
+    w.lvn[i] = lv;
+    w.la[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  const DevParams &P = Pr[t.rid[i]];
+  const int id = t.id[i];
+  const double4 q = t.nb[i];
+  const double2 v0 = t.vn[id], a = t.ain[i];
+  const double Si = t.S[i];
+  const double rf = Si < P.h_inf ? P.h_inf : Si;
+  Dual X0 = dvar(q.x, 0), Y0 = dvar(q.y, 1), VX = dvar(v0.x, 2), VY = dvar(v0.y, 3), RF = dvar(rf, 4);
+  Dual AX = dvar(a.x, 5), AY = dvar(a.y, 6), MU = dvar(P.tanfi8, 7), XI = dvar(P.xi_voellmy, 8);
+  Dual xo, yo, vxo, vyo;
+  bool mid;
+  stage_map(P, dem, X0, Y0, VX, VY, RF, AX, AY, MU, XI, xo, yo, vxo, vyo, mid);
+  double g[AD_N];
 #pragma unroll
-      for (int k = 0; k < AD_N; k++)
-        g[k] = mid ? 0.0 : lx.x * xo.d[k] + lx.y * yo.d[k] + lv.x * vxo.d[k] + lv.y * vyo.d[k];
-      w.lxn[i] = make_double2(g[0], g[1]);
-      w.lvn[i] = make_double2(g[2], g[3]);
-      if (Si >= P.h_inf) w.lS[i] += g[4];
-      w.la[i] = make_double2(g[5], g[6]);
-      dmu = g[7];
-      dxi = g[8];
+  for (int k = 0; k < AD_N; k++)
+    g[k] = mid ? 0.0 : lx.x * xo.d[k] + lx.y * yo.d[k] + lv.x * vxo.d[k] + lv.y * vyo.d[k];
+  w.lxn[i] = make_double2(g[0], g[1]);
+  w.lvn[i] = make_double2(g[2], g[3]);
+  if (Si >= P.h_inf) w.lS[i] += g[4];
+  w.la[i] = make_double2(g[5], g[6]);
+  GP[id] += g[7];
+  GQ[id] += g[8];
+}
+
+// ---------------------------------------------------------------------------
+// Bingham stage adjoint.  In stage st a particle outside the intermediate
+// regime takes the Tau0 of the last particle before it (caller ids, its own
+// realisation) whose candidate was valid, or the value carried from the
+// previous scan.  With the resolved values tau_st held as inputs, a
+// particle's stages are a map of its own inputs and of tau_0 .. tau_{S-1};
+// the candidates it offers are outputs.  The cotangent of tau_st of a
+// particle belongs to the candidate it was taken from: per stage, in reverse
+// order, (1) k_adj_bing_tau gives every particle its lambda(tau_st) (its
+// outputs' cotangents and those of its later candidates, forward-mode in the
+// one direction tau_st), (2) a segmented reverse scan over ids adds each
+// segment [valid k, next valid) to lambda(cand_st[k]); what precedes the
+// realisation's first valid candidate belongs to the carried value, i.e.
+// to the last valid candidate of the previous scan (or its carry), and is
+// passed on in bcar[r]; (3) after the stages k_adj_stage_bing contracts the
+// outputs' and the candidates' cotangents over the particle's own inputs.
+// The tape holds tau_st and cand_st by id ([2S][n]: tau_0.., cand_0..).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool cand_valid(double c) { return c == c; }
+
+__global__ __launch_bounds__(GF_BLOCK) void k_adj_bing_tau(int n, const DevParams *Pr, DevDem dem, TapeRec t,
+                                                           AdjWork w, int sd, const double *lamCand,
+                                                           double *lamTau) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int id = t.id[i];
+  if (t.sflag[i]) {
+    lamTau[id] = 0.0;
+    return;
+  }
+  const DevParams &P = Pr[t.rid[i]];
+  const int S = P.nstages;
+  const double4 q = t.nb[i];
+  const double2 v0 = t.vn[id], a = t.ain[i];
+  const double Si = t.S[i];
+  const double rf = Si < P.h_inf ? P.h_inf : Si;
+  Dual tau[BING_MAX_STAGES];
+#pragma unroll
+  for (int st = 0; st < BING_MAX_STAGES; st++)
+    tau[st] = st < S ? (st == sd ? dvar(t.bt[(long)st * n + id], 0) : dconst(t.bt[(long)st * n + id])) : dconst(0.0);
+  Dual xo, yo, vxo, vyo, cand[BING_MAX_STAGES];
+  bool ok[BING_MAX_STAGES], mid;
+  bing_stage_map(P, dem, dconst(q.x), dconst(q.y), dconst(v0.x), dconst(v0.y), dconst(rf), dconst(a.x),
+                 dconst(a.y), dconst(P.tauy0), dconst(P.constK), tau, xo, yo, vxo, vyo, cand, ok, mid);
+  double g = 0.0;
+  if (!mid) {
+    const double2 lx = w.lxp[i], lv = w.lvp[i];
+    g = lx.x * xo.d[0] + lx.y * yo.d[0] + lv.x * vxo.d[0] + lv.y * vyo.d[0];
+#pragma unroll
+    for (int st = 0; st < BING_MAX_STAGES; st++)
+      if (st > sd && st < S && ok[st]) g += lamCand[(long)st * n + id] * cand[st].d[0];
+  }
+  lamTau[id] = g;
+}
+
+__global__ __launch_bounds__(GF_BLOCK) void k_adj_stage_bing(int n, const DevParams *Pr, DevDem dem, TapeRec t,
+                                                             AdjWork w, const double *lamCand, double *GP,
+                                                             double *GQ) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 lx = w.lxp[i], lv = w.lvp[i];
+  if (t.sflag[i]) {
+    w.lxn[i] = lx;
+    w.lvn[i] = lv;
+    w.la[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  const DevParams &P = Pr[t.rid[i]];
+  const int S = P.nstages;
+  const int id = t.id[i];
+  const double4 q = t.nb[i];
+  const double2 v0 = t.vn[id], a = t.ain[i];
+  const double Si = t.S[i];
+  const double rf = Si < P.h_inf ? P.h_inf : Si;
+  Dual tau[BING_MAX_STAGES];
+#pragma unroll
+  for (int st = 0; st < BING_MAX_STAGES; st++) tau[st] = dconst(st < S ? t.bt[(long)st * n + id] : 0.0);
+  Dual xo, yo, vxo, vyo, cand[BING_MAX_STAGES];
+  bool ok[BING_MAX_STAGES], mid;
+  bing_stage_map(P, dem, dvar(q.x, 0), dvar(q.y, 1), dvar(v0.x, 2), dvar(v0.y, 3), dvar(rf, 4), dvar(a.x, 5),
+                 dvar(a.y, 6), dvar(P.tauy0, 7), dvar(P.constK, 8), tau, xo, yo, vxo, vyo, cand, ok, mid);
+  double g[AD_N];
+#pragma unroll
+  for (int k = 0; k < AD_N; k++) {
+    double v = lx.x * xo.d[k] + lx.y * yo.d[k] + lv.x * vxo.d[k] + lv.y * vyo.d[k];
+#pragma unroll
+    for (int st = 0; st < BING_MAX_STAGES; st++)
+      if (st < S && ok[st]) v += lamCand[(long)st * n + id] * cand[st].d[k];
+    g[k] = mid ? 0.0 : v;
+  }
+  w.lxn[i] = make_double2(g[0], g[1]);
+  w.lvn[i] = make_double2(g[2], g[3]);
+  if (Si >= P.h_inf) w.lS[i] += g[4];
+  w.la[i] = make_double2(g[5], g[6]);
+  GP[id] += g[7];
+  GQ[id] += g[8];
+}
+
+// Segmented reverse scan over the Bingham scan tiles (each tile a range of
+// ids of one realisation; LV_ITEMS ids per thread).  A thread's "lead" is the
+// sum of its ids before its first valid candidate (all of them if none);
+// walking the threads (then the tiles) from the right, the carry into a range
+// is what arrives from its right up to the next valid candidate.
+__device__ void seg_thread_cin(const double *lam, const double *cand, int b, int e, double cin_tile,
+                               double *cin_out, double *lead_out, bool *has_out) {
+  __shared__ double lead[GF_BLOCK];
+  __shared__ int has[GF_BLOCK];
+  __shared__ double cin[GF_BLOCK];
+  const int t = threadIdx.x;
+  double l = 0.0;
+  int hv = 0;
+  for (int k = 0; k < LV_ITEMS; k++) {
+    const int id = b + t * LV_ITEMS + k;
+    if (id < e && !hv) {
+      if (cand_valid(cand[id]))
+        hv = 1;
+      else
+        l = l + lam[id];
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    dmu += __shfl_xor(dmu, o, 64);
-    dxi += __shfl_xor(dxi, o, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = dmu;
-    red[1][threadIdx.x >> 6] = dxi;
+  lead[t] = l;
+  has[t] = hv;
+  __syncthreads();
+  if (t == 0) {
+    double carry = cin_tile;
+    int any = 0;
+    for (int u = GF_BLOCK - 1; u >= 0; u--) {
+      cin[u] = carry;
+      carry = has[u] ? lead[u] : lead[u] + carry;
+      any |= has[u];
+    }
+    if (lead_out) *lead_out = carry;
+    if (has_out) *has_out = any != 0;
   }
   __syncthreads();
+  *cin_out = cin[t];
+}
+
+__global__ __launch_bounds__(GF_BLOCK) void k_seg_reduce(const int4 *tiles, const double *lam, const double *cand,
+                                                         double *tlead, int *thas) {
+  const int4 tl = tiles[blockIdx.x];
+  double cin, lead;
+  bool has;
+  seg_thread_cin(lam, cand, tl.y, tl.z, 0.0, &cin, &lead, &has);
   if (threadIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int k = 0; k < GF_BLOCK / 64; k++) {
-      s0 += red[0][k];
-      s1 += red[1][k];
+    tlead[blockIdx.x] = lead;
+    thas[blockIdx.x] = has ? 1 : 0;
+  }
+}
+
+// one thread per realisation: the carry into each of its tiles, from its
+// last tile (which receives the pending cotangent of the later scan's carry)
+// to its first; what is left is the cotangent of this scan's carried value
+__global__ void k_seg_top(int R, const int *rtile, const double *tlead, const int *thas, double *tcin,
+                          double *bcar) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  double carry = bcar[r];
+  for (int b = rtile[r + 1] - 1; b >= rtile[r]; b--) {
+    tcin[b] = carry;
+    carry = thas[b] ? tlead[b] : tlead[b] + carry;
+  }
+  bcar[r] = carry;
+}
+
+__global__ __launch_bounds__(GF_BLOCK) void k_seg_down(const int4 *tiles, const double *lam, const double *cand,
+                                                       const double *tcin, double *lamCand) {
+  const int4 tl = tiles[blockIdx.x];
+  double cin;
+  seg_thread_cin(lam, cand, tl.y, tl.z, tcin[blockIdx.x], &cin, nullptr, nullptr);
+  const int b0 = tl.y + threadIdx.x * LV_ITEMS;
+  double acc = cin;
+  for (int k = LV_ITEMS - 1; k >= 0; k--) {
+    const int id = b0 + k;
+    if (id >= tl.z) continue;
+    acc = lam[id] + acc;
+    if (cand_valid(cand[id])) {
+      lamCand[id] = acc;
+      acc = 0.0;
+    } else {
+      lamCand[id] = 0.0;
     }
-    gmu[blockIdx.x] += s0;
-    gxi[blockIdx.x] += s1;
   }
 }
 
@@ -2919,11 +3239,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_adj_pairs(int n, DevParams P, Tape
     return;
   }
   const Cells cl = tape_cells(t);
-  const GridInfo g = cl.grid[0];
+  const int rr = t.rid[i];
+  const GridInfo g = cl.grid[rr];
   const double4 qi = t.nb[i];
   const double2 pi_ = make_double2(qi.x, qi.y);
   const double hi = qi.z, mi = qi.w;
-  const CandStream cs = cand_stream(g, cl.start, pi_, hi, P.scale_k);
+  const CandStream cs = cand_stream(g, cl.start + cl.coff[rr], pi_, hi, P.scale_k);
   const unsigned long long *mrow = t.mask + NB_WORDS * (long)i;
   const double2 lai = w.la[i];
   const double lSi = w.lS[i];
@@ -3001,9 +3322,9 @@ __global__ void k_misfit_nodes(long np, const double *acc5, const double *target
 }
 
 __global__ void k_misfit_vjp(int n, DevDem dem, CurView s, const double *acc5, const double *res, double2 *LX,
-                             double *LH) {
+                             double *LH, int rsel) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (rsel >= 0 && s.rid[i] != rsel)) return;
   const int k = s.id[i];
   if (s.flag[i]) {
     LX[k] = make_double2(0.0, 0.0);
@@ -3476,9 +3797,10 @@ __global__ void k_pairs(int n, int skf, int scale_k, Cells cl, SortedSet s, long
 }
 
 // a14 (i): IDW particle -> node scatter (6:14312-14440), soil particles
-__global__ void k_raster_idw(int n, DevDem dem, CurView s, double *acc5) {
+// rsel >= 0: the particles of realisation rsel only
+__global__ void k_raster_idw(int n, DevDem dem, CurView s, double *acc5, int rsel) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || s.flag[i]) return;
+  if (i >= n || s.flag[i] || (rsel >= 0 && s.rid[i] != rsel)) return;
   double2 p = s.pos[i];
   double2 v = s.vel[i];
   double h = s.h[i], uu = s.u[i];
@@ -3668,13 +3990,22 @@ struct gfsph_ctx {
     SortedSet s;
     NewSet o;
     int cur;
+    double *carry;  // Bingham: each realisation's carried Tau0
   };
   int ad_ckpt = 0, ad_nsteps = 0;
   std::vector<Snap> snaps;  // [nseg + 1]: segment starts and the final state
   std::vector<TapeRec> tape;
   AdjWork aw{};
   double2 *LX = nullptr, *LV = nullptr;
-  double *LM = nullptr, *LH = nullptr, *gmu = nullptr, *gxi = nullptr;
+  double *LM = nullptr, *LH = nullptr;
+  double *GP = nullptr, *GQ = nullptr;  // parameter cotangents per particle, by caller id
+  // Bingham adjoint: lambda of each particle's resolved Tau0 and candidates,
+  // the segmented-scan tile sums, the pending cotangent of each carried Tau0;
+  // tape_bt: where the step being recorded writes its stages' Tau0
+  double *lamTau = nullptr, *lamCand = nullptr, *tlead = nullptr, *tcin = nullptr, *bcar = nullptr;
+  int *thas = nullptr;
+  double *tape_bt = nullptr;
+  std::vector<DevParams> Ph;  // the realisations' parameter blocks (host copy)
   // time step: ic_adapt_dt_sph (0: fixed dt_sph); with adaptation dt_sph and
   // time_sph of every realisation live on the device (Pr[r].dt, tm, dtm the
   // per-step minima), otherwise time_sph is summed here from dt_h
@@ -4138,6 +4469,12 @@ static void launch_stages(gfsph_ctx *c, SortedSet &s) {
     k_lv_reduce<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, b.cand, b.bsum);
     k_lv_top<<<(c->R + 63) / 64, 64, 0, st>>>(c->R, b.rtile, b.bsum, b.carry);
     k_lv_down<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, b.cand, b.bsum, b.tau);
+    if (c->tape_bt) {  // a step recorded for the adjoint (launched directly, not in a graph)
+      const int S = c->P.nstages;
+      (void)hipMemcpyAsync(c->tape_bt + (size_t)k * n, b.tau, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+      (void)hipMemcpyAsync(c->tape_bt + (size_t)(S + k) * n, b.cand, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                           st);
+    }
     k_bingham_stage<<<nb, GF_BLOCK, 0, st>>>(n, 1, k, c->Pr, c->dem, s, c->S, b);
   }
   k_bingham_finish<<<c->gridN, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, s, c->S, b, c->nw,
@@ -4508,6 +4845,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   HIPCHK(c, hipMemcpyAsync(s0.rid, rid.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->rng, rg.data(), sizeof(int2) * R, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->Pr, Ph.data(), sizeof(DevParams) * R, hipMemcpyHostToDevice, st));
+  c->Ph = Ph;
   const int nb = nblk(n);
   k_domain_flag<<<nb, GF_BLOCK, 0, st>>>(n, c->dem, s0.pos, s0.flag);
   k_copy_to_new<<<nb, GF_BLOCK, 0, st>>>(n, s0, c->nw, c->P.npwp);
@@ -4882,7 +5220,7 @@ int gfsph_raster_idw(gfsph_ctx *c, double *out, double *wsum) {
   if (!c->raster5 && (rc = dalloc(c, &c->raster5, 5 * np + 5 * np))) return rc;
   double *acc5 = c->raster5, *nrm = c->raster5 + 5 * np;
   HIPCHK(c, hipMemsetAsync(acc5, 0, sizeof(double) * 5 * np, c->stream));
-  k_raster_idw<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), acc5);
+  k_raster_idw<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->dem, cur_view(c), acc5, -1);
   k_raster_norm<<<nblk(np), GF_BLOCK, 0, c->stream>>>(np, acc5, nrm, nrm + 4 * np);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(out, nrm, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream));
@@ -5094,6 +5432,8 @@ static int alloc_snap(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
       (rc = dalloc(c, &o.vel, n)) || (rc = dalloc(c, &o.h, n)) || (rc = dalloc(c, &o.hsml, n)) ||
       (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) || (rc = dalloc(c, &o.flag, n)))
     return rc;
+  sn.carry = nullptr;
+  if (c->P.nfric == 4 && (rc = dalloc(c, &sn.carry, c->R))) return rc;
   return GFSPH_OK;
 }
 
@@ -5126,11 +5466,13 @@ static int copy_state(gfsph_ctx *c, SortedSet &sd, NewSet &od, const SortedSet &
 
 static int snap_save(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
   sn.cur = c->cur;
+  if (sn.carry) DCOPY(sn.carry, c->bs.carry, c->R);
   return copy_state(c, sn.s, sn.o, c->set[c->cur], c->nw);
 }
 
 static int snap_restore(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
   c->cur = sn.cur;
+  if (sn.carry) DCOPY(c->bs.carry, sn.carry, c->R);
   return copy_state(c, c->set[c->cur], c->nw, sn.s, sn.o);
 }
 
@@ -5142,12 +5484,14 @@ __global__ void k_scatter_vel(int n, const int *id, const double2 *vel, double2 
 static int alloc_tape(gfsph_ctx *c, TapeRec &t) {
   const int n = c->n;
   int rc;
-  if ((rc = dalloc(c, &t.nb, n)) || (rc = dalloc(c, &t.id, n)) || (rc = dalloc(c, &t.sflag, n)) ||
-      (rc = dalloc(c, &t.oflag, n)) || (rc = dalloc(c, &t.S, n)) || (rc = dalloc(c, &t.ain, n)) ||
-      (rc = dalloc(c, &t.vp, n)) || (rc = dalloc(c, &t.vn, n)) || (rc = dalloc(c, &t.mask, NB_WORDS * (size_t)n)) ||
-      (rc = dalloc(c, &t.start, (size_t)c->cap_cells + 2)) || (rc = dalloc(c, &t.grid, 1)) ||
-      (rc = dalloc(c, &t.coff, 2)))
+  if ((rc = dalloc(c, &t.nb, n)) || (rc = dalloc(c, &t.id, n)) || (rc = dalloc(c, &t.rid, n)) ||
+      (rc = dalloc(c, &t.sflag, n)) || (rc = dalloc(c, &t.oflag, n)) || (rc = dalloc(c, &t.S, n)) ||
+      (rc = dalloc(c, &t.ain, n)) || (rc = dalloc(c, &t.vp, n)) || (rc = dalloc(c, &t.vn, n)) ||
+      (rc = dalloc(c, &t.mask, NB_WORDS * (size_t)n)) || (rc = dalloc(c, &t.start, (size_t)c->cap_cells + 2)) ||
+      (rc = dalloc(c, &t.grid, c->R)) || (rc = dalloc(c, &t.coff, c->R + 1)))
     return rc;
+  t.bt = nullptr;
+  if (c->P.nfric == 4 && (rc = dalloc(c, &t.bt, 2 * (size_t)c->P.nstages * n))) return rc;
   return GFSPH_OK;
 }
 
@@ -5156,11 +5500,20 @@ static int taped_step(gfsph_ctx *c, TapeRec &t) {
   const int n = c->n;
   const SortedSet &s0 = c->set[c->cur];
   k_scatter_vel<<<nblk(n), GF_BLOCK, 0, c->stream>>>(n, s0.id, s0.vel, t.vn);
-  int rc = run_steps(c, 1);
-  if (rc) return rc;
+  int rc;
+  if (t.bt) {  // Bingham: the stages' Tau0 are recorded from inside the step, launched directly
+    c->tape_bt = t.bt;
+    rc = enqueue_step(c, c->cur, 1 - c->cur);
+    c->tape_bt = nullptr;
+    if (rc) return rc;
+    c->cur = 1 - c->cur;
+  } else if ((rc = run_steps(c, 1))) {
+    return rc;
+  }
   const SortedSet &s = c->set[c->cur];
   DCOPY(t.nb, s.nb, n);
   DCOPY(t.id, s.id, n);
+  DCOPY(t.rid, s.rid, n);
   DCOPY(t.sflag, s.flag, n);
   DCOPY(t.oflag, c->nw.flag, n);
   DCOPY(t.S, c->S, n);
@@ -5168,23 +5521,24 @@ static int taped_step(gfsph_ctx *c, TapeRec &t) {
   DCOPY(t.vp, c->nw.vel, n);
   DCOPY(t.mask, c->mask, NB_WORDS * (size_t)n);
   DCOPY(t.start, c->start, (size_t)c->cap_cells + 2);
-  DCOPY(t.grid, c->grid, 1);
-  DCOPY(t.coff, c->coff, 2);
+  DCOPY(t.grid, c->grid, c->R);
+  DCOPY(t.coff, c->coff, c->R + 1);
   return GFSPH_OK;
 }
 
 static int adjoint_supported(gfsph_ctx *c) {
-  const DevParams &P = c->P;
-  if (c->R != 1) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs on a single realisation");
   if (c->adapt) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs with a fixed dt_sph (ic_adapt_dt_sph = 0)");
   if (c->cases) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without case control");
   if (c->eros) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without erosion");
   if (c->staged) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without ic_K0 and ic_cosTheta");
-  if (P.nfric != 5 || P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 ||
-      P.xi_voellmy < 0.0)
-    return fail(c, GFSPH_ERR_UNSUPPORTED,
-                "the adjoint covers nfric = 5 (frictional / Voellmy, xi >= 0) with pa_sph = 2, skf = 1, "
-                "icpwp = 0, sle = 0");
+  for (const DevParams &P : c->Ph) {
+    if (P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 || P.nfric != c->Ph[0].nfric)
+      return fail(c, GFSPH_ERR_UNSUPPORTED,
+                  "the adjoint covers pa_sph = 2, skf = 1, icpwp = 0, sle = 0 and one law for every realisation");
+    if (!((P.nfric == 5 && P.xi_voellmy >= 0.0) || (P.nfric == 4 && P.nstages <= BING_MAX_STAGES)))
+      return fail(c, GFSPH_ERR_UNSUPPORTED,
+                  "the adjoint covers nfric = 5 (frictional / Voellmy, xi >= 0) and nfric = 4 (Bingham)");
+  }
   return GFSPH_OK;
 }
 
@@ -5223,17 +5577,23 @@ int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_
                            double *g_mu, double *g_xi, double *g_mass, double *g_x0, double *g_v0) {
   if (!c || c->n <= 0) return GFSPH_ERR_ARG;
   if (c->ad_nsteps <= 0) return fail(c, GFSPH_ERR_ARG, "gfsph_adjoint_forward has not run");
-  const int n = c->n, K = c->ad_ckpt, N = c->ad_nsteps, nseg = (N + K - 1) / K;
+  const int n = c->n, K = c->ad_ckpt, N = c->ad_nsteps, nseg = (N + K - 1) / K, R = c->R;
+  const bool bing = c->P.nfric == 4;
+  const int S = c->P.nstages;
   hipStream_t st = c->stream;
   int rc;
   AdjWork &w = c->aw;
   if (!c->LX) {
     if ((rc = dalloc(c, &c->LX, n)) || (rc = dalloc(c, &c->LV, n)) || (rc = dalloc(c, &c->LM, n)) ||
-        (rc = dalloc(c, &c->LH, n)) || (rc = dalloc(c, &c->gmu, nblk(n))) || (rc = dalloc(c, &c->gxi, nblk(n))) ||
+        (rc = dalloc(c, &c->LH, n)) || (rc = dalloc(c, &c->GP, n)) || (rc = dalloc(c, &c->GQ, n)) ||
         (rc = dalloc(c, &w.lxp, n)) || (rc = dalloc(c, &w.lvpp, n)) || (rc = dalloc(c, &w.lvp, n)) ||
         (rc = dalloc(c, &w.la, n)) || (rc = dalloc(c, &w.lx, n)) || (rc = dalloc(c, &w.lxn, n)) ||
         (rc = dalloc(c, &w.lvn, n)) || (rc = dalloc(c, &w.lS, n)) || (rc = dalloc(c, &w.lm, n)) ||
         (rc = dalloc(c, &w.act, n)))
+      return rc;
+    if (bing && ((rc = dalloc(c, &c->lamTau, n)) || (rc = dalloc(c, &c->lamCand, (size_t)S * n)) ||
+                 (rc = dalloc(c, &c->tlead, c->bs.ntiles)) || (rc = dalloc(c, &c->tcin, c->bs.ntiles)) ||
+                 (rc = dalloc(c, &c->thas, c->bs.ntiles)) || (rc = dalloc(c, &c->bcar, R))))
       return rc;
   }
   std::vector<double> zero(2 * (size_t)n, 0.0);
@@ -5241,9 +5601,11 @@ int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_
   HIPCHK(c, hipMemcpyAsync(c->LV, lam_v ? lam_v : zero.data(), sizeof(double2) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->LH, lam_h ? lam_h : zero.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemsetAsync(c->LM, 0, sizeof(double) * n, st));
-  HIPCHK(c, hipMemsetAsync(c->gmu, 0, sizeof(double) * nblk(n), st));
-  HIPCHK(c, hipMemsetAsync(c->gxi, 0, sizeof(double) * nblk(n), st));
+  HIPCHK(c, hipMemsetAsync(c->GP, 0, sizeof(double) * n, st));
+  HIPCHK(c, hipMemsetAsync(c->GQ, 0, sizeof(double) * n, st));
+  if (bing) HIPCHK(c, hipMemsetAsync(c->bcar, 0, sizeof(double) * R, st));
   const int nb = nblk(n);
+  const BinghamState &b = c->bs;
   for (int seg = nseg - 1; seg >= 0; seg--) {
     if ((rc = snap_restore(c, c->snaps[seg]))) return rc;
     const int L = std::min(K, N - seg * K);
@@ -5258,62 +5620,85 @@ int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_
       } else {
         HIPCHK(c, hipMemcpyAsync(w.lvp, w.lvpp, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
       }
-      k_adj_stage<<<nb, GF_BLOCK, 0, st>>>(n, c->P, c->dem, tr, w, c->gmu, c->gxi);
+      if (bing) {
+        for (int sd = S - 1; sd >= 0; sd--) {
+          k_adj_bing_tau<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, sd, c->lamCand, c->lamTau);
+          const double *cand = tr.bt + (size_t)(S + sd) * n;
+          k_seg_reduce<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, c->lamTau, cand, c->tlead, c->thas);
+          k_seg_top<<<(R + 63) / 64, 64, 0, st>>>(R, b.rtile, c->tlead, c->thas, c->tcin, c->bcar);
+          k_seg_down<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, c->lamTau, cand, c->tcin, c->lamCand + (size_t)sd * n);
+        }
+        k_adj_stage_bing<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, c->lamCand, c->GP, c->GQ);
+      } else {
+        k_adj_stage<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, c->GP, c->GQ);
+      }
       k_adj_pairs<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w, c->LX, c->LV, c->LM);
     }
     HIPCHK(c, hipGetLastError());
   }
   // back to the state the forward pass ended in
   if ((rc = snap_restore(c, c->snaps[nseg]))) return rc;
-  std::vector<double> gm(nb), gx(nb);
-  HIPCHK(c, hipMemcpyAsync(gm.data(), c->gmu, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipMemcpyAsync(gx.data(), c->gxi, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+  std::vector<double> gp(n), gq(n);
+  HIPCHK(c, hipMemcpyAsync(gp.data(), c->GP, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(gq.data(), c->GQ, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   if (g_mass) HIPCHK(c, hipMemcpyAsync(g_mass, c->LM, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   if (g_x0) HIPCHK(c, hipMemcpyAsync(g_x0, c->LX, sizeof(double2) * n, hipMemcpyDeviceToHost, st));
   if (g_v0) HIPCHK(c, hipMemcpyAsync(g_v0, c->LV, sizeof(double2) * n, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipStreamSynchronize(st));
-  double smu = 0.0, sxi = 0.0;
-  for (int b = 0; b < nb; b++) {
-    smu += gm[b];
-    sxi += gx[b];
+  // each realisation's parameter cotangents: its particles' sums, in id order
+  int first = 0;
+  for (int r = 0; r < R; r++) {
+    const int nr = R == 1 ? n : c->counts_h[r];
+    double sp = 0.0, sq = 0.0;
+    for (int k = first; k < first + nr; k++) {
+      sp += gp[k];
+      sq += gq[k];
+    }
+    if (g_mu) g_mu[r] = sp;
+    if (g_xi) g_xi[r] = sq;
+    first += nr;
   }
-  if (g_mu) *g_mu = smu;
-  if (g_xi) *g_xi = sxi;
   return GFSPH_OK;
 }
 
 int gfsph_raster_misfit(gfsph_ctx *c, const double *target, double *loss, double *lam_x, double *lam_h) {
   if (!c || c->n <= 0 || !target || !loss) return GFSPH_ERR_ARG;
   const long np = (long)c->dem.nx * c->dem.ny;
-  const int n = c->n;
+  const int n = c->n, R = c->R;
   hipStream_t st = c->stream;
   int rc;
   if (!c->raster5 && (rc = dalloc(c, &c->raster5, 5 * np + 5 * np))) return rc;
   const int nbn = nblk(np);
   void *tmp = nullptr;
-  const size_t bytes = sizeof(double) * (np + 2 * np + nbn) + sizeof(double2) * n + sizeof(double) * n;
+  const size_t bytes = sizeof(double) * (np + 2 * np + (size_t)nbn * R) + sizeof(double2) * n + sizeof(double) * n;
   HIPCHK(c, hipMalloc(&tmp, bytes));
   double *d_t = (double *)tmp, *d_res = d_t + np, *d_part = d_res + 2 * np;
-  double2 *d_lx = (double2 *)(d_part + nbn);
+  double2 *d_lx = (double2 *)(d_part + (size_t)nbn * R);
   double *d_lh = (double *)(d_lx + n);
-  std::vector<double> part(nbn);
+  std::vector<double> part((size_t)nbn * R);
   hipError_t e = hipMemcpyAsync(d_t, target, sizeof(double) * np, hipMemcpyHostToDevice, st);
-  if (!e) e = hipMemsetAsync(c->raster5, 0, sizeof(double) * 5 * np, st);
-  if (!e) {
-    k_raster_idw<<<nblk(n), GF_BLOCK, 0, st>>>(n, c->dem, cur_view(c), c->raster5);
-    k_misfit_nodes<<<nbn, GF_BLOCK, 0, st>>>(np, c->raster5, d_t, d_res, d_part);
-    k_misfit_vjp<<<nblk(n), GF_BLOCK, 0, st>>>(n, c->dem, cur_view(c), c->raster5, d_res, d_lx, d_lh);
-    e = hipGetLastError();
+  // each realisation against the same target, one after the other on the one raster
+  for (int r = 0; r < R && !e; r++) {
+    const int rs = R == 1 ? -1 : r;
+    e = hipMemsetAsync(c->raster5, 0, sizeof(double) * 5 * np, st);
+    if (!e) {
+      k_raster_idw<<<nblk(n), GF_BLOCK, 0, st>>>(n, c->dem, cur_view(c), c->raster5, rs);
+      k_misfit_nodes<<<nbn, GF_BLOCK, 0, st>>>(np, c->raster5, d_t, d_res, d_part + (size_t)nbn * r);
+      k_misfit_vjp<<<nblk(n), GF_BLOCK, 0, st>>>(n, c->dem, cur_view(c), c->raster5, d_res, d_lx, d_lh, rs);
+      e = hipGetLastError();
+    }
   }
-  if (!e) e = hipMemcpyAsync(part.data(), d_part, sizeof(double) * nbn, hipMemcpyDeviceToHost, st);
+  if (!e) e = hipMemcpyAsync(part.data(), d_part, sizeof(double) * nbn * R, hipMemcpyDeviceToHost, st);
   if (!e && lam_x) e = hipMemcpyAsync(lam_x, d_lx, sizeof(double2) * n, hipMemcpyDeviceToHost, st);
   if (!e && lam_h) e = hipMemcpyAsync(lam_h, d_lh, sizeof(double) * n, hipMemcpyDeviceToHost, st);
   if (!e) e = hipStreamSynchronize(st);
   (void)hipFree(tmp);
   if (e) return fail(c, GFSPH_ERR_HIP, std::string("raster misfit: ") + hipGetErrorString(e));
-  double L = 0.0;
-  for (double v : part) L += v;
-  *loss = L;
+  for (int r = 0; r < R; r++) {
+    double L = 0.0;
+    for (int k = 0; k < nbn; k++) L += part[(size_t)nbn * r + k];
+    loss[r] = L;
+  }
   return GFSPH_OK;
 }
 

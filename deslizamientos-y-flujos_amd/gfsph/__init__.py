@@ -8,5 +8,5 @@ and generates the synthetic DEMs and particle releases of the workloads
 """
 from . import cases, dem  # noqa: F401
 from ._build import LIB, build  # noqa: F401
-from .api import AdjointModel, BatchModel, GfsphError, SPHModel, load_library, neighbour_pairs, write_netcdf  # noqa: F401,E501
+from .api import AdjointModel, BatchAdjointModel, BatchModel, GfsphError, SPHModel, load_library, neighbour_pairs, write_netcdf  # noqa: F401,E501
 from .inputs import Inputs  # noqa: F401

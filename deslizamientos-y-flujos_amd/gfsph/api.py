@@ -499,40 +499,58 @@ class BatchModel(SPHModel):
         return hz
 
 
-class AdjointModel(SPHModel):
-    """A run with its reverse mode: ``forward(nsteps, ckpt)`` advances and
+class _Adjoint:
+    """The reverse mode of a context: ``forward(nsteps, ckpt)`` advances and
     keeps checkpoints, ``misfit(target)`` gives the deposit misfit of the
-    current state and its cotangents, ``backward(...)`` the gradients with
-    respect to mu (const(9)), xi (const(3)), the particle masses (volumes)
-    and the initial positions and velocities."""
+    current state (per realisation) and its cotangents, ``backward(...)`` the
+    gradients with respect to the law's two parameters — (mu, xi) =
+    (const(9), const(3)) for nfric 5, (tau_y, mu_B) = (const(6), const(7)) for
+    Bingham, nfric 4 — the particle masses (volumes) and the initial
+    positions and velocities."""
 
     def forward(self, nsteps: int, ckpt_every: int = 50) -> None:
         self._check(self.lib.gfsph_adjoint_forward(self.ctx, int(nsteps), int(ckpt_every)))
         self.nsteps_done += int(nsteps)
 
+    def _nreal(self) -> int:
+        return int(getattr(self, "R", 1))
+
     def misfit(self, target):
         t = np.ascontiguousarray(target, dtype=np.float64).ravel()
         lx = np.zeros((self.n, 2))
         lh = np.zeros(self.n)
-        L = C.c_double()
-        self._check(self.lib.gfsph_raster_misfit(self.ctx, _dp(t), C.byref(L), _dp(lx), _dp(lh)))
-        return L.value, lx, lh
+        L = np.zeros(self._nreal())
+        self._check(self.lib.gfsph_raster_misfit(self.ctx, _dp(t), _dp(L), _dp(lx), _dp(lh)))
+        return (float(L[0]) if self._nreal() == 1 else L), lx, lh
 
     def backward(self, lam_x=None, lam_v=None, lam_h=None) -> dict:
         def arg(a, shape):
             return None if a is None else np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float64)
 
         lx, lv, lh = arg(lam_x, (self.n, 2)), arg(lam_v, (self.n, 2)), arg(lam_h, (self.n,))
-        gmu, gxi = C.c_double(), C.c_double()
+        R = self._nreal()
+        gmu, gxi = np.zeros(R), np.zeros(R)
         gm = np.zeros(self.n)
         gx = np.zeros((self.n, 2))
         gv = np.zeros((self.n, 2))
         nul = C.POINTER(C.c_double)()
         self._check(self.lib.gfsph_adjoint_backward(self.ctx, _dp(lx) if lx is not None else nul,
                                                     _dp(lv) if lv is not None else nul,
-                                                    _dp(lh) if lh is not None else nul, C.byref(gmu), C.byref(gxi),
+                                                    _dp(lh) if lh is not None else nul, _dp(gmu), _dp(gxi),
                                                     _dp(gm), _dp(gx), _dp(gv)))
-        return {"mu": gmu.value, "xi": gxi.value, "mass": gm, "x0": gx, "v0": gv}
+        if R == 1:
+            return {"mu": float(gmu[0]), "xi": float(gxi[0]), "mass": gm, "x0": gx, "v0": gv}
+        return {"mu": gmu, "xi": gxi, "mass": gm, "x0": gx, "v0": gv}
+
+
+class AdjointModel(_Adjoint, SPHModel):
+    """One run with its reverse mode (see _Adjoint)."""
+
+
+class BatchAdjointModel(_Adjoint, BatchModel):
+    """A batch of realisations with their reverse mode: every realisation's
+    misfit against one target raster and its own parameter gradients, from
+    one batched forward and one batched backward sweep."""
 
 
 def neighbour_pairs(x, hsml, flag=None, skf: int = 1) -> np.ndarray:

@@ -210,9 +210,12 @@ int gfsph_batch_hazard(gfsph_ctx *ctx, double hthr, double *stats, int32_t *npro
  * receives the sizes. */
 int gfsph_autotune(gfsph_ctx *ctx, int32_t reps, int32_t *chosen);
 /* Reverse mode (adjoint) of the step, for back-analysis of rheology.  No
- * reference counterpart: the reference has no derivatives.  Covered: nfric
- * = 5 (frictional with Voellmy turbulence, xi = const(3) >= 0), pa_sph = 2,
- * skf = 1, icpwp = 0, sle = 0, one realisation (else GFSPH_ERR_UNSUPPORTED).
+ * reference counterpart: the reference has no derivatives.  Covered, for
+ * one run or a batch whose realisations share the law: nfric = 5
+ * (frictional with Voellmy turbulence, xi = const(3) >= 0) and nfric = 4
+ * (Bingham, its sticky Tau0 included: the cotangent of a carried Tau0 goes
+ * to the particle, stage or step that set it), with pa_sph = 2, skf = 1,
+ * icpwp = 0, sle = 0 (else GFSPH_ERR_UNSUPPORTED).
  *
  * gfsph_adjoint_forward advances nsteps steps from the current state like
  * gfsph_step, keeping a copy of the device state every ckpt_every steps.
@@ -220,17 +223,21 @@ int gfsph_autotune(gfsph_ctx *ctx, int32_t reps, int32_t *chosen);
  * nsteps steps: given the cotangents of the final state lam_x [n][2], lam_v
  * [n][2], lam_h [n] (caller order; NULL = 0), it recomputes each segment
  * from its checkpoint with each step's neighbour set fixed as in the forward
- * pass and sweeps backwards, giving dL/dmu (mu = const(9)), dL/dxi
- * (const(3)), dL/dmass [n], dL/dx0 [n][2] and dL/dv0 [n][2] at the state
- * the forward pass started from (any output may be NULL).  The context is
- * left in the state the forward pass ended in.  Particles that leave the
- * domain inside a step contribute no derivative for that step. */
+ * pass and sweeps backwards, giving, per realisation ([R]; R = 1 for a
+ * single run), the cotangents of the law's two parameters -- g_mu, g_xi =
+ * dL/dmu (mu = const(9)), dL/dxi (const(3)) for nfric 5; dL/dtau_y
+ * (const(6)), dL/dmu_B (const(7)) for Bingham -- and dL/dmass [n], dL/dx0
+ * [n][2], dL/dv0 [n][2] at the state the forward pass started from (any
+ * output may be NULL).  The context is left in the state the forward pass
+ * ended in.  Particles that leave the domain inside a step contribute no
+ * derivative for that step. */
 int gfsph_adjoint_forward(gfsph_ctx *ctx, int32_t nsteps, int32_t ckpt_every);
 int gfsph_adjoint_backward(gfsph_ctx *ctx, const double *lam_x, const double *lam_v, const double *lam_h,
                            double *g_mu, double *g_xi, double *g_mass, double *g_x0, double *g_v0);
 /* Deposit misfit of the current state: L = sum over DEM nodes of (R - T)^2,
  * R the IDW depth raster of OutputRes_WIR ("6 SPH_SW_2019.f90":14312-14466)
- * and T = target [npoig]; lam_x [n][2] and lam_h [n] (may be NULL) receive
+ * and T = target [npoig]; in a batch each realisation's raster against the
+ * same target, loss [R]; lam_x [n][2] and lam_h [n] (may be NULL) receive
  * dL/dx and dL/dh in caller order.  No reference counterpart. */
 int gfsph_raster_misfit(gfsph_ctx *ctx, const double *target, double *loss, double *lam_x, double *lam_h);
 /* Average device time per launch of each step kernel over nrep steps, in

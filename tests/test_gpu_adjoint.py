@@ -78,10 +78,86 @@ def test_adjoint_matches_central_differences(integrator):
 
 def test_adjoint_rejects_uncovered_laws():
     case = base_case()
-    prm = C.Params(integrator=2, dt=0.02, h_inf=1e-3, const=C.bingham(1000.0, 50.0))
+    prm = C.Params(integrator=2, dt=0.02, h_inf=1e-3, const=C.voellmy(0.2, 500.0, nfric=7))
     with AdjointModel(C.Case("b", prm, case.dem, case.particles, NSTEPS)) as m:
         with pytest.raises(Exception, match="adjoint"):
             m.forward(10, 5)
+
+
+def bingham_case(tau_y=1000.0, mu_b=50.0, integrator=2, n=400, seed=5):
+    dem = fractal_dem(65, 2.0, 17, relief=30.0)
+    parts = C.elliptical_scar(40.0, 64.0, n, 1.0, 2.0, seed, h_range=(1.0, 3.0))
+    prm = C.Params(integrator=integrator, dt=0.02, h_inf=1e-3, const=C.bingham(tau_y, mu_b))
+    return C.Case("adjb", prm, dem, parts, NSTEPS)
+
+
+@pytest.mark.parametrize("integrator", [2, 4])
+def test_bingham_adjoint_matches_central_differences(integrator):
+    """Bingham (nfric 4): its Tau0 is carried from particle to particle, stage
+    to stage and step to step (the reference's sticky value), so a particle's
+    friction depends on other particles' states; the adjoint routes those
+    cotangents back.  dL/dtau_y (const(6)), dL/dmu_B (const(7)) and dL/dmass
+    of sampled particles against central differences of the misfit."""
+    T = target_raster()
+    case = bingham_case(integrator=integrator)
+    with AdjointModel(case) as m:
+        m.forward(NSTEPS, CKPT)
+        L, lx, lh = m.misfit(T)
+        g = m.backward(lam_x=lx, lam_h=lh)
+        st = m.state()
+    assert np.all(st["flag"] == 0)
+    assert L > 0
+    assert loss_of(case, T) == pytest.approx(L, rel=1e-13)
+    eps = 1e-6
+    for k, name in ((6, "mu"), (7, "xi")):
+        v = case.params.const[k]
+        hi, lo = C.f32(v * (1 + eps)), C.f32(v * (1 - eps))
+        fd = (loss_of(with_const(case, k, hi), T) - loss_of(with_const(case, k, lo), T)) / (hi - lo)
+        print(f"Bingham RK{integrator} dL/dconst({k}): adjoint {g[name]:.10e} central difference {fd:.10e}")
+        assert g[name] == pytest.approx(fd, rel=1e-5), (name, g[name], fd)
+    rng = np.random.default_rng(0)
+    for i in rng.choice(case.particles.n, size=4, replace=False):
+        m0 = case.particles.mass[i]
+        P1, P2 = case.particles.copy(), case.particles.copy()
+        P1.mass[i] = m0 * (1 + eps)
+        P2.mass[i] = m0 * (1 - eps)
+        c1 = C.Case(case.name, case.params, case.dem, P1, NSTEPS)
+        c2 = C.Case(case.name, case.params, case.dem, P2, NSTEPS)
+        fd = (loss_of(c1, T) - loss_of(c2, T)) / (P1.mass[i] - P2.mass[i])
+        assert g["mass"][i] == pytest.approx(fd, rel=1e-5, abs=1e-9 * abs(L) / m0), (i, g["mass"][i], fd)
+
+
+@pytest.mark.parametrize("law", ["voellmy", "bingham"])
+def test_batched_adjoint_equals_single_runs(law):
+    """Three realisations (different parameters and releases) as one batched
+    forward + misfit + backward: each realisation's loss and gradients equal
+    those of its run alone (to the rounding of the misfit raster's atomic
+    sums)."""
+    from gfsph import BatchAdjointModel
+
+    T = target_raster()
+    if law == "voellmy":
+        cases = [base_case(mu, xi) for mu, xi in ((0.2, 500.0), (0.15, 800.0), (0.25, 300.0))]
+    else:
+        cases = [bingham_case(ty, mb, seed=sd) for ty, mb, sd in ((1000.0, 50.0, 5), (2500.0, 20.0, 6),
+                                                                  (600.0, 120.0, 7))]
+    single = []
+    for c in cases:
+        with AdjointModel(c) as m:
+            m.forward(NSTEPS, CKPT)
+            L, lx, lh = m.misfit(T)
+            single.append((L, m.backward(lam_x=lx, lam_h=lh)))
+    with BatchAdjointModel(cases) as b:
+        b.forward(NSTEPS, CKPT)
+        L, lx, lh = b.misfit(T)
+        g = b.backward(lam_x=lx, lam_h=lh)
+    edges = np.concatenate([[0], np.cumsum([c.particles.n for c in cases])])
+    for r, (Ls, gs) in enumerate(single):
+        assert L[r] == pytest.approx(Ls, rel=1e-12), r
+        assert g["mu"][r] == pytest.approx(gs["mu"], rel=1e-9), (r, g["mu"][r], gs["mu"])
+        assert g["xi"][r] == pytest.approx(gs["xi"], rel=1e-9), (r, g["xi"][r], gs["xi"])
+        gm = g["mass"][edges[r]:edges[r + 1]]
+        np.testing.assert_allclose(gm, gs["mass"], rtol=1e-9, atol=1e-12 * np.abs(gs["mass"]).max())
 
 
 @pytest.mark.parametrize("integrator", [2, 4])
