@@ -3991,6 +3991,7 @@ struct gfsph_ctx {
     NewSet o;
     int cur;
     double *carry;  // Bingham: each realisation's carried Tau0
+    int2 *rng;      // batches: each realisation's slot range in the saved sorted set
   };
   int ad_ckpt = 0, ad_nsteps = 0;
   std::vector<Snap> snaps;  // [nseg + 1]: segment starts and the final state
@@ -5433,7 +5434,9 @@ static int alloc_snap(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
       (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) || (rc = dalloc(c, &o.flag, n)))
     return rc;
   sn.carry = nullptr;
+  sn.rng = nullptr;
   if (c->P.nfric == 4 && (rc = dalloc(c, &sn.carry, c->R))) return rc;
+  if (c->R > 1 && (rc = dalloc(c, &sn.rng, c->R))) return rc;
   return GFSPH_OK;
 }
 
@@ -5464,16 +5467,36 @@ static int copy_state(gfsph_ctx *c, SortedSet &sd, NewSet &od, const SortedSet &
   return GFSPH_OK;
 }
 
+// A snapshot holds what the next step reads: the sorted set, the step
+// output, the Bingham carry, and what the next neighbour build takes its
+// grids from -- a batch's slot ranges of the saved sorted set (a single
+// run's bounding box is recomputed from the restored step output at
+// restore: min / max reductions, so the grid is the one the forward built).
 static int snap_save(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
   sn.cur = c->cur;
   if (sn.carry) DCOPY(sn.carry, c->bs.carry, c->R);
+  if (sn.rng) {
+    if (c->first_build)
+      DCOPY(sn.rng, c->rng, c->R);
+    else
+      k_rng<<<(c->R + GF_BLOCK - 1) / GF_BLOCK, GF_BLOCK, 0, c->stream>>>(c->R, c->coff, c->start, sn.rng);
+  }
   return copy_state(c, sn.s, sn.o, c->set[c->cur], c->nw);
 }
 
 static int snap_restore(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
   c->cur = sn.cur;
   if (sn.carry) DCOPY(c->bs.carry, sn.carry, c->R);
-  return copy_state(c, c->set[c->cur], c->nw, sn.s, sn.o);
+  int rc = copy_state(c, c->set[c->cur], c->nw, sn.s, sn.o);
+  if (rc) return rc;
+  if (sn.rng) {
+    DCOPY(c->rng, sn.rng, c->R);
+    c->first_build = true;  // the next build takes these ranges as they are
+  } else {
+    k_bbox<<<c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
+  }
+  HIPCHK(c, hipGetLastError());
+  return GFSPH_OK;
 }
 
 __global__ void k_scatter_vel(int n, const int *id, const double2 *vel, double2 *vn) {
