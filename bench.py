@@ -108,6 +108,14 @@ def main() -> int:
     ap.add_argument("--mc-seq-sample", type=int, default=0,
                     help="realisations timed one by one for the sequential cost (0: all of them)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--no-cal", action="store_true", help="skip the batched calibration (Bingham forward + adjoint)")
+    ap.add_argument("--cal-draws", type=int, default=64)
+    ap.add_argument("--cal-particles", type=int, default=1_000_000)
+    ap.add_argument("--cal-steps", type=int, default=20,
+                    help="steps of the bounded calibration sample (the configured run is 2,000)")
+    ap.add_argument("--cal-ckpt", type=int, default=5)
+    ap.add_argument("--cal-seq-sample", type=int, default=0,
+                    help="sequential draws timed (0: all; else a sample, the total extrapolated)")
     ap.add_argument("--adj-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -185,6 +193,8 @@ def main() -> int:
         cpu = cpu_baseline(case, window_state, develop + args.warmup, args.cpu_sample_steps)
     mc = None if args.no_mc else mc_hazard(args.mc_real, args.mc_steps, args.mc_seq_sample)
     adj = None if args.no_adjoint else adjoint_cost(args.adj_steps)
+    cal = None if args.no_cal else calibration(args.cal_draws, args.cal_particles, args.cal_steps, args.cal_ckpt,
+                                               args.cal_seq_sample)
 
     out = {
         "metric": "particle-steps/s at 200k",
@@ -209,9 +219,67 @@ def main() -> int:
         "cpu_baseline": cpu,
         "mc": mc,
         "adjoint": adj,
+        "calibration": cal,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def calibration(R, n, nsteps, ckpt, nseq):
+    """Batched adjoint calibration (fractal DEM 4096 x 4096 at 1 m, one
+    release of n particles, R Bingham draws tau_y ~ U(500, 5000) Pa, mu_B ~
+    U(10, 200) Pa s, RK2 dt 0.02 s): one batched forward + deposit misfit +
+    backward over all draws, from context creation to the R gradients on the
+    host, against the same for each draw run alone through the single-run
+    adjoint.  The misfit target is the deposit of a forward run at (tau_y,
+    mu_B) = (2000 Pa, 60 Pa s).  A bounded sample of ``nsteps`` steps of the
+    configured 2,000 (the JSON says so); the checkpoints every ``ckpt``
+    steps and one segment's records are what fit the GPU's memory for R x n
+    particles at this horizon."""
+    from gfsph import AdjointModel, BatchAdjointModel, SPHModel, cases
+
+    t0 = time.perf_counter()
+    cs = cases.cfg_calibration(R=R, n=n, nsteps=nsteps)
+    print(f"# calibration: {R} draws x {n} particles, {nsteps} steps, workload {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    tprm = cases.Params(integrator=2, dt=cs[0].params.dt, h_inf=1.0e-3, const=cases.bingham(2000.0, 60.0))
+    with SPHModel(cases.Case("target", tprm, cs[0].dem, cs[0].particles, nsteps)) as m:
+        m.time_integration_sph(nsteps)
+        T = m.raster_idw()[0][:, 0].copy()
+    t0 = time.perf_counter()
+    with BatchAdjointModel(cs) as b:
+        b.forward(nsteps, ckpt)
+        L, lx, lh = b.misfit(T)
+        g = b.backward(lam_x=lx, lam_h=lh)
+    t_batch = time.perf_counter() - t0
+    print(f"# calibration batch {t_batch:.2f}s", file=sys.stderr, flush=True)
+    picks = list(range(R)) if nseq <= 0 or nseq >= R else list(range(nseq))
+    ts, check = [], []
+    for r in picks:
+        t1 = time.perf_counter()
+        with AdjointModel(cs[r]) as a:
+            a.forward(nsteps, ckpt)
+            Lr, lxr, lhr = a.misfit(T)
+            gr = a.backward(lam_x=lxr, lam_h=lhr)
+        ts.append(time.perf_counter() - t1)
+        if r < 3:
+            check.append({"draw": r, "loss_rel": abs(L[r] - Lr) / abs(Lr),
+                          "grad_tau_y_rel": abs(g["mu"][r] - gr["mu"]) / max(abs(gr["mu"]), 1e-300),
+                          "grad_mu_B_rel": abs(g["xi"][r] - gr["xi"]) / max(abs(gr["xi"]), 1e-300)})
+        if len(ts) % 16 == 0:
+            print(f"# calibration sequential: {len(ts)} of {len(picks)} draws, {sum(ts):.1f}s", file=sys.stderr,
+                  flush=True)
+    measured = len(picks) == R
+    t_seq = float(np.sum(ts)) if measured else float(np.mean(ts)) * R
+    print(f"# calibration sequential {'measured' if measured else 'extrapolated'} {t_seq:.2f}s", file=sys.stderr,
+          flush=True)
+    return {"metric": "gradients/hr", "value": round(R * 3600.0 / t_batch, 1), "unit": "gradients/hr",
+            "draws": R, "particles_per_draw": n, "particles": R * n, "steps": nsteps,
+            "steps_configured": 2000, "sample": f"{nsteps} of the configured 2,000 steps (bounded sample)",
+            "ckpt_every": ckpt, "batch_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
+            "sequential_kind": "measured" if measured else "extrapolated", "sequential_runs": len(ts),
+            "speedup_vs_sequential": round(t_seq / t_batch, 2), "batch_vs_single_check": check,
+            "grad_tau_y_first": [float(v) for v in g["mu"][:3]], "grad_mu_B_first": [float(v) for v in g["xi"][:3]]}
 
 
 def calibrate_copy() -> int:
