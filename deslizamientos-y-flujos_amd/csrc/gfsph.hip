@@ -2886,6 +2886,7 @@ struct TapeRec {
   double2 *ain, *vp, *vn;  // vn: velocity at step start, caller order
   unsigned long long *mask;
   int *start;
+  int start_cap;  // entries of start
   GridInfo *grid;
   int *coff;
 };
@@ -3994,6 +3995,10 @@ struct gfsph_ctx {
     int2 *rng;      // batches: each realisation's slot range in the saved sorted set
   };
   int ad_ckpt = 0, ad_nsteps = 0;
+  int ad_tape = 0;         // most steps recorded at once (0: ckpt); longer segments get a second level
+  int ad_cells = 0;        // largest cell count the forward pass's checkpoints saw
+  int *ad_over = nullptr;  // device: cells a recorded step needed beyond its buffer (0: none)
+  std::vector<Snap> snaps2;  // second-level checkpoints inside one segment
   std::vector<Snap> snaps;  // [nseg + 1]: segment starts and the final state
   std::vector<TapeRec> tape;
   AdjWork aw{};
@@ -5421,17 +5426,23 @@ int gfsph_autotune(gfsph_ctx *c, int32_t reps, int32_t *chosen) {
 // ---------------------------------------------------------------------------
 // adjoint host side
 // ---------------------------------------------------------------------------
+// A checkpoint holds what the next step reads: of the sorted set the masses,
+// ids, velocities and realisations (the neighbour build takes the rest from
+// the step output), of the step output the positions, depths, smoothing
+// lengths, pore pressures and flags; the Bingham carry; and what the next
+// neighbour build takes its grids from -- a batch's slot ranges of the saved
+// sorted set (a single run's bounding box is recomputed from the restored
+// step output at restore: min / max reductions, so the grid is the one the
+// forward built).  84 B per particle.
 static int alloc_snap(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
   const int n = c->n;
   int rc;
-  SortedSet &s = sn.s;
-  NewSet &o = sn.o;
-  if ((rc = dalloc(c, &s.nb, n)) || (rc = dalloc(c, &s.pos, n)) || (rc = dalloc(c, &s.vel, n)) ||
-      (rc = dalloc(c, &s.h, n)) || (rc = dalloc(c, &s.mass, n)) || (rc = dalloc(c, &s.hsml, n)) ||
-      (rc = dalloc(c, &s.u, n)) || (rc = dalloc(c, &s.uaux, n)) || (rc = dalloc(c, &s.flag, n)) ||
-      (rc = dalloc(c, &s.id, n)) || (rc = dalloc(c, &s.rid, n)) || (rc = dalloc(c, &o.pos, n)) ||
-      (rc = dalloc(c, &o.vel, n)) || (rc = dalloc(c, &o.h, n)) || (rc = dalloc(c, &o.hsml, n)) ||
-      (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) || (rc = dalloc(c, &o.flag, n)))
+  sn.s = SortedSet{};
+  sn.o = NewSet{};
+  if ((rc = dalloc(c, &sn.s.mass, n)) || (rc = dalloc(c, &sn.s.id, n)) || (rc = dalloc(c, &sn.s.vel, n)) ||
+      (rc = dalloc(c, &sn.s.rid, n)) || (rc = dalloc(c, &sn.o.pos, n)) || (rc = dalloc(c, &sn.o.h, n)) ||
+      (rc = dalloc(c, &sn.o.hsml, n)) || (rc = dalloc(c, &sn.o.u, n)) || (rc = dalloc(c, &sn.o.uaux, n)) ||
+      (rc = dalloc(c, &sn.o.flag, n)))
     return rc;
   sn.carry = nullptr;
   sn.rng = nullptr;
@@ -5443,37 +5454,20 @@ static int alloc_snap(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
 #define DCOPY(dst, src, count) \
   HIPCHK(c, hipMemcpyAsync((dst), (src), sizeof(*(src)) * (size_t)(count), hipMemcpyDeviceToDevice, c->stream))
 
-// copies the step state (the current sorted set, the new set, the parity)
-static int copy_state(gfsph_ctx *c, SortedSet &sd, NewSet &od, const SortedSet &ss, const NewSet &os) {
-  const int n = c->n;
-  DCOPY(sd.nb, ss.nb, n);
-  DCOPY(sd.pos, ss.pos, n);
-  DCOPY(sd.vel, ss.vel, n);
-  DCOPY(sd.h, ss.h, n);
-  DCOPY(sd.mass, ss.mass, n);
-  DCOPY(sd.hsml, ss.hsml, n);
-  DCOPY(sd.u, ss.u, n);
-  DCOPY(sd.uaux, ss.uaux, n);
-  DCOPY(sd.flag, ss.flag, n);
-  DCOPY(sd.id, ss.id, n);
-  DCOPY(sd.rid, ss.rid, n);
-  DCOPY(od.pos, os.pos, n);
-  DCOPY(od.vel, os.vel, n);
-  DCOPY(od.h, os.h, n);
-  DCOPY(od.hsml, os.hsml, n);
-  DCOPY(od.u, os.u, n);
-  DCOPY(od.uaux, os.uaux, n);
-  DCOPY(od.flag, os.flag, n);
-  return GFSPH_OK;
-}
-
-// A snapshot holds what the next step reads: the sorted set, the step
-// output, the Bingham carry, and what the next neighbour build takes its
-// grids from -- a batch's slot ranges of the saved sorted set (a single
-// run's bounding box is recomputed from the restored step output at
-// restore: min / max reductions, so the grid is the one the forward built).
 static int snap_save(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
+  const int n = c->n;
+  const SortedSet &s = c->set[c->cur];
   sn.cur = c->cur;
+  DCOPY(sn.s.mass, s.mass, n);
+  DCOPY(sn.s.id, s.id, n);
+  DCOPY(sn.s.vel, s.vel, n);
+  DCOPY(sn.s.rid, s.rid, n);
+  DCOPY(sn.o.pos, c->nw.pos, n);
+  DCOPY(sn.o.h, c->nw.h, n);
+  DCOPY(sn.o.hsml, c->nw.hsml, n);
+  DCOPY(sn.o.u, c->nw.u, n);
+  DCOPY(sn.o.uaux, c->nw.uaux, n);
+  DCOPY(sn.o.flag, c->nw.flag, n);
   if (sn.carry) DCOPY(sn.carry, c->bs.carry, c->R);
   if (sn.rng) {
     if (c->first_build)
@@ -5481,14 +5475,24 @@ static int snap_save(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
     else
       k_rng<<<(c->R + GF_BLOCK - 1) / GF_BLOCK, GF_BLOCK, 0, c->stream>>>(c->R, c->coff, c->start, sn.rng);
   }
-  return copy_state(c, sn.s, sn.o, c->set[c->cur], c->nw);
+  return GFSPH_OK;
 }
 
 static int snap_restore(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
+  const int n = c->n;
   c->cur = sn.cur;
+  SortedSet &s = c->set[c->cur];
+  DCOPY(s.mass, sn.s.mass, n);
+  DCOPY(s.id, sn.s.id, n);
+  DCOPY(s.vel, sn.s.vel, n);
+  DCOPY(s.rid, sn.s.rid, n);
+  DCOPY(c->nw.pos, sn.o.pos, n);
+  DCOPY(c->nw.h, sn.o.h, n);
+  DCOPY(c->nw.hsml, sn.o.hsml, n);
+  DCOPY(c->nw.u, sn.o.u, n);
+  DCOPY(c->nw.uaux, sn.o.uaux, n);
+  DCOPY(c->nw.flag, sn.o.flag, n);
   if (sn.carry) DCOPY(c->bs.carry, sn.carry, c->R);
-  int rc = copy_state(c, c->set[c->cur], c->nw, sn.s, sn.o);
-  if (rc) return rc;
   if (sn.rng) {
     DCOPY(c->rng, sn.rng, c->R);
     c->first_build = true;  // the next build takes these ranges as they are
@@ -5497,6 +5501,17 @@ static int snap_restore(gfsph_ctx *c, gfsph_ctx::Snap &sn) {
   }
   HIPCHK(c, hipGetLastError());
   return GFSPH_OK;
+}
+
+// copies cell starts [0, *ntot + 2) to dst when they fit its cap entries, else
+// raises *over (the tape is then recorded again with larger buffers)
+__global__ void k_copy_starts(const int *ntot, const int *src, int *dst, int cap, int *over) {
+  const int need = *ntot + 2;
+  if (need > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(over, need);
+    return;
+  }
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < need; k += gridDim.x * blockDim.x) dst[k] = src[k];
 }
 
 __global__ void k_scatter_vel(int n, const int *id, const double2 *vel, double2 *vn) {
@@ -5510,9 +5525,13 @@ static int alloc_tape(gfsph_ctx *c, TapeRec &t) {
   if ((rc = dalloc(c, &t.nb, n)) || (rc = dalloc(c, &t.id, n)) || (rc = dalloc(c, &t.rid, n)) ||
       (rc = dalloc(c, &t.sflag, n)) || (rc = dalloc(c, &t.oflag, n)) || (rc = dalloc(c, &t.S, n)) ||
       (rc = dalloc(c, &t.ain, n)) || (rc = dalloc(c, &t.vp, n)) || (rc = dalloc(c, &t.vn, n)) ||
-      (rc = dalloc(c, &t.mask, NB_WORDS * (size_t)n)) || (rc = dalloc(c, &t.start, (size_t)c->cap_cells + 2)) ||
-      (rc = dalloc(c, &t.grid, c->R)) || (rc = dalloc(c, &t.coff, c->R + 1)))
+      (rc = dalloc(c, &t.mask, NB_WORDS * (size_t)n)) || (rc = dalloc(c, &t.grid, c->R)) ||
+      (rc = dalloc(c, &t.coff, c->R + 1)))
     return rc;
+  // the cell starts at the size the forward pass used (its largest cell count
+  // with a margin; grown if a recomputed step needs more)
+  t.start_cap = std::min(c->cap_cells + 2, std::max(4096, c->ad_cells + c->ad_cells / 2 + 2));
+  if ((rc = dalloc(c, &t.start, (size_t)t.start_cap))) return rc;
   t.bt = nullptr;
   if (c->P.nfric == 4 && (rc = dalloc(c, &t.bt, 2 * (size_t)c->P.nstages * n))) return rc;
   return GFSPH_OK;
@@ -5543,7 +5562,8 @@ static int taped_step(gfsph_ctx *c, TapeRec &t) {
   DCOPY(t.ain, c->ain, n);
   DCOPY(t.vp, c->nw.vel, n);
   DCOPY(t.mask, c->mask, NB_WORDS * (size_t)n);
-  DCOPY(t.start, c->start, (size_t)c->cap_cells + 2);
+  k_copy_starts<<<std::max(1, std::min(1024, (t.start_cap + GF_BLOCK - 1) / GF_BLOCK)), GF_BLOCK, 0, c->stream>>>(
+      c->coff + c->R, c->start, t.start, t.start_cap, c->ad_over);
   DCOPY(t.grid, c->grid, c->R);
   DCOPY(t.coff, c->coff, c->R + 1);
   return GFSPH_OK;
@@ -5565,31 +5585,48 @@ static int adjoint_supported(gfsph_ctx *c) {
   return GFSPH_OK;
 }
 
+static int cells_now(gfsph_ctx *c, int *out) {
+  HIPCHK(c, hipMemcpyAsync(out, c->coff + c->R, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFSPH_OK;
+}
+
+static void dfree(gfsph_ctx *c, void *p) {
+  for (size_t k = 0; k < c->allocs.size(); k++)
+    if (c->allocs[k] == p) {
+      c->allocs.erase(c->allocs.begin() + (long)k);
+      break;
+    }
+  (void)hipFree(p);
+}
+
+int gfsph_adjoint_set_tape(gfsph_ctx *c, int32_t tape_steps) {
+  if (!c || tape_steps < 0) return GFSPH_ERR_ARG;
+  c->ad_tape = tape_steps;
+  return GFSPH_OK;
+}
+
 int gfsph_adjoint_forward(gfsph_ctx *c, int32_t nsteps, int32_t ckpt_every) {
   if (!c || c->n <= 0 || nsteps <= 0 || ckpt_every <= 0) return GFSPH_ERR_ARG;
   int rc = adjoint_supported(c);
   if (rc) return rc;
   const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
-  if (c->ad_ckpt != ckpt_every || (int)c->snaps.size() < nseg + 1) {
-    if ((int)c->snaps.size() < nseg + 1) {
-      const size_t old = c->snaps.size();
-      c->snaps.resize(nseg + 1);
-      for (size_t k = old; k < c->snaps.size(); k++)
-        if ((rc = alloc_snap(c, c->snaps[k]))) return rc;
-    }
-    if ((int)c->tape.size() < ckpt_every) {
-      const size_t old = c->tape.size();
-      c->tape.resize(ckpt_every);
-      for (size_t k = old; k < c->tape.size(); k++)
-        if ((rc = alloc_tape(c, c->tape[k]))) return rc;
-    }
+  if ((int)c->snaps.size() < nseg + 1) {
+    const size_t old = c->snaps.size();
+    c->snaps.resize(nseg + 1);
+    for (size_t k = old; k < c->snaps.size(); k++)
+      if ((rc = alloc_snap(c, c->snaps[k]))) return rc;
   }
+  if (!c->ad_over && (rc = dalloc(c, &c->ad_over, 1))) return rc;
   c->ad_ckpt = ckpt_every;
   c->ad_nsteps = nsteps;
+  int cells = 0;
   for (int seg = 0; seg < nseg; seg++) {
     if ((rc = snap_save(c, c->snaps[seg]))) return rc;
     const int L = std::min(ckpt_every, nsteps - seg * ckpt_every);
     if ((rc = run_steps(c, L))) return rc;
+    if ((rc = cells_now(c, &cells))) return rc;
+    c->ad_cells = std::max(c->ad_cells, cells);
   }
   if ((rc = snap_save(c, c->snaps[nseg]))) return rc;
   HIPCHK(c, hipGetLastError());
@@ -5629,14 +5666,72 @@ int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_
   if (bing) HIPCHK(c, hipMemsetAsync(c->bcar, 0, sizeof(double) * R, st));
   const int nb = nblk(n);
   const BinghamState &b = c->bs;
+  // at most T steps are recorded at once: a longer segment is advanced once
+  // more from its checkpoint, leaving second-level checkpoints every T steps
+  const int T = (c->ad_tape > 0 && c->ad_tape < K) ? c->ad_tape : K;
+  const int nsub_max = (K + T - 1) / T;
+  if (T < K && (int)c->snaps2.size() < nsub_max) {
+    const size_t old = c->snaps2.size();
+    c->snaps2.resize(nsub_max);
+    for (size_t k = old; k < c->snaps2.size(); k++)
+      if ((rc = alloc_snap(c, c->snaps2[k]))) return rc;
+  }
+  if ((int)c->tape.size() < T) {
+    const size_t old = c->tape.size();
+    c->tape.resize(T);
+    for (size_t k = old; k < c->tape.size(); k++)
+      if ((rc = alloc_tape(c, c->tape[k]))) return rc;
+  }
+  struct Piece {
+    int seg, sub, first, len;  // checkpoint (level 1 or 2), first step, steps
+  };
+  std::vector<Piece> pieces;  // in reverse time order
   for (int seg = nseg - 1; seg >= 0; seg--) {
-    if ((rc = snap_restore(c, c->snaps[seg]))) return rc;
     const int L = std::min(K, N - seg * K);
+    const int nsub = (L + T - 1) / T;
+    for (int j = nsub - 1; j >= 0; j--) pieces.push_back({seg, nsub > 1 ? j : -1, j * T, std::min(T, L - j * T)});
+  }
+  int cur_seg = -1;
+  for (const Piece &pc : pieces) {
+    if (pc.sub < 0) {
+      if ((rc = snap_restore(c, c->snaps[pc.seg]))) return rc;
+    } else {
+      if (pc.seg != cur_seg) {  // the segment's second-level checkpoints
+        if ((rc = snap_restore(c, c->snaps[pc.seg]))) return rc;
+        const int L = std::min(K, N - pc.seg * K);
+        for (int j = 0; j * T < L; j++) {
+          if ((rc = snap_save(c, c->snaps2[j]))) return rc;
+          if ((rc = run_steps(c, std::min(T, L - j * T)))) return rc;
+        }
+        cur_seg = pc.seg;
+      }
+      if ((rc = snap_restore(c, c->snaps2[pc.sub]))) return rc;
+    }
+    const int L = pc.len;
+    HIPCHK(c, hipMemsetAsync(c->ad_over, 0, sizeof(int), st));
     for (int t = 0; t < L; t++)
       if ((rc = taped_step(c, c->tape[t]))) return rc;
+    int over = 0;
+    HIPCHK(c, hipMemcpyAsync(&over, c->ad_over, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (over > 0) {  // a recorded step had more cells than the buffers: grow them and record again
+      for (TapeRec &tr : c->tape) {
+        dfree(c, tr.start);
+        tr.start = nullptr;
+        tr.start_cap = std::min(c->cap_cells + 2, over + over / 2);
+        if ((rc = dalloc(c, &tr.start, (size_t)tr.start_cap))) return rc;
+      }
+      if (pc.sub < 0) {
+        if ((rc = snap_restore(c, c->snaps[pc.seg]))) return rc;
+      } else if ((rc = snap_restore(c, c->snaps2[pc.sub]))) {
+        return rc;
+      }
+      for (int t = 0; t < L; t++)
+        if ((rc = taped_step(c, c->tape[t]))) return rc;
+    }
     for (int t = L - 1; t >= 0; t--) {
       TapeRec &tr = c->tape[t];
-      const bool last = (seg == nseg - 1) && (t == L - 1);
+      const bool last = (pc.seg == nseg - 1) && (pc.first + t == std::min(K, N - pc.seg * K) - 1);
       k_adj_load<<<nb, GF_BLOCK, 0, st>>>(n, c->P.npoin0, tr, c->LX, c->LV, last ? c->LH : nullptr, w);
       if (c->P.av_vel) {
         k_adj_xsph<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w);

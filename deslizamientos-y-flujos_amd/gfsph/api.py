@@ -23,7 +23,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_create", "gfsph_step", "gfsph_sync", "gfsph_get_state", "gfsph_neighbour_pairs",
     "gfsph_raster_idw", "gfsph_mc_pixel_accumulate", "gfsph_mc_pixel_read", "gfsph_profile_step",
     "gfsph_counts", "gfsph_last_error", "gfsph_destroy", "gfsph_batch_create", "gfsph_batch_pixel_accumulate",
-    "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune", "gfsph_adjoint_forward",
+    "gfsph_batch_pixel_read", "gfsph_batch_hazard", "gfsph_autotune", "gfsph_adjoint_forward", "gfsph_adjoint_set_tape",
     "gfsph_adjoint_backward", "gfsph_raster_misfit", "gfsph_set_adaptive_dt", "gfsph_time", "gfsph_set_time",
     "gfsph_inputs_parse", "gfsph_inputs_read", "gfsph_inputs_info", "gfsph_inputs_dem", "gfsph_inputs_particles",
     "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy", "gfsph_write_netcdf",
@@ -115,6 +115,7 @@ def load_library(path: str | None = None):
     L.gfsph_batch_hazard.argtypes = [vp, C.c_double, dp, ip, dp, dp]
     L.gfsph_autotune.argtypes = [vp, C.c_int32, ip]
     L.gfsph_adjoint_forward.argtypes = [vp, C.c_int32, C.c_int32]
+    L.gfsph_adjoint_set_tape.argtypes = [vp, C.c_int32]
     L.gfsph_adjoint_backward.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp]
     L.gfsph_raster_misfit.argtypes = [vp, dp, dp, dp, dp]
     L.gfsph_set_adaptive_dt.argtypes = [vp, C.c_int32]
@@ -508,7 +509,10 @@ class _Adjoint:
     Bingham, nfric 4 — the particle masses (volumes) and the initial
     positions and velocities."""
 
-    def forward(self, nsteps: int, ckpt_every: int = 50) -> None:
+    def forward(self, nsteps: int, ckpt_every: int = 50, tape_steps: int = 0) -> None:
+        """Advance nsteps with checkpoints every ckpt_every steps; the backward
+        pass records at most tape_steps steps at once (0: a whole segment)."""
+        self._check(self.lib.gfsph_adjoint_set_tape(self.ctx, int(tape_steps)))
         self._check(self.lib.gfsph_adjoint_forward(self.ctx, int(nsteps), int(ckpt_every)))
         self.nsteps_done += int(nsteps)
 

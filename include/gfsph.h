@@ -232,6 +232,13 @@ int gfsph_autotune(gfsph_ctx *ctx, int32_t reps, int32_t *chosen);
  * ended in.  Particles that leave the domain inside a step contribute no
  * derivative for that step. */
 int gfsph_adjoint_forward(gfsph_ctx *ctx, int32_t nsteps, int32_t ckpt_every);
+/* Records at most tape_steps steps at once in gfsph_adjoint_backward (0: a
+ * whole checkpoint segment): a longer segment is advanced once more from its
+ * checkpoint, leaving checkpoints every tape_steps steps, and recorded piece
+ * by piece -- a third level of recomputation for horizons whose records do
+ * not fit beside the checkpoints.  The gradients do not depend on it.  No
+ * reference counterpart. */
+int gfsph_adjoint_set_tape(gfsph_ctx *ctx, int32_t tape_steps);
 int gfsph_adjoint_backward(gfsph_ctx *ctx, const double *lam_x, const double *lam_v, const double *lam_h,
                            double *g_mu, double *g_xi, double *g_mass, double *g_x0, double *g_v0);
 /* Deposit misfit of the current state: L = sum over DEM nodes of (R - T)^2,

@@ -199,20 +199,25 @@ def test_adjoint_matches_fp64_reference_gradient(integrator):
 
 
 @pytest.mark.parametrize("batch", [False, True])
-def test_adjoint_segments_recompute_the_forward_grids(batch):
-    """A block sliding ~60 m down a 30 degree plane over 120 steps, checkpoints
-    every 40: each segment is recomputed from its checkpoint with the grids
-    the forward pass built (a single run's box from the restored state, a
-    batch's slot ranges saved with the checkpoint) -- not from the state the
-    context last held, whose box no longer covers the particles -- so every
-    recomputed step equals the forward's bit for bit: the gradient is the same
-    for any checkpoint spacing, and equals central differences."""
+@pytest.mark.parametrize("law", ["voellmy", "bingham"])
+def test_adjoint_segments_recompute_the_forward_grids(batch, law):
+    """A block sliding ~60 m down a 30 degree plane over 120 steps.  Each
+    segment is recomputed from its checkpoint with the grids the forward pass
+    built (a single run's box from the restored state, a batch's slot ranges
+    saved with the checkpoint) -- not from the state the context last held,
+    whose box no longer covers the particles -- so every recomputed step
+    equals the forward's bit for bit: the gradient is the same for any
+    checkpoint scheme (checkpoints every 40 steps; one segment of 120; one
+    segment recorded 15 steps at a time from second-level checkpoints), and
+    equals central differences."""
     from gfsph import BatchAdjointModel
     from gfsph.dem import plane_runout_dem
 
     dem = plane_runout_dem(nx=61, ny=21, d=5.0, x_break=250.0, blend=50.0)
     parts = C.lattice_block(20.0, 40.0, 20.0, 10.0, 10, 5, 3.0)
-    prm = C.Params(integrator=2, dt=0.05, h_inf=1e-3, const=C.voellmy(0.1, 1000.0))
+    const = C.voellmy(0.1, 1000.0) if law == "voellmy" else C.bingham(800.0, 30.0)
+    k_mu = 9 if law == "voellmy" else 6
+    prm = C.Params(integrator=2, dt=0.05, h_inf=1e-3, const=const)
     case = C.Case("slide", prm, dem, parts, 120)
     W = np.random.default_rng(2).normal(size=(parts.n, 2))
 
@@ -225,21 +230,21 @@ def test_adjoint_segments_recompute_the_forward_grids(batch):
     L0, xe = lin_loss(case)
     assert xe[:, 0].min() > parts.x[:, 0].max()  # the block left its initial box
     grads = {}
-    for ckpt in (40, 120):
+    for ckpt, tape in ((40, 0), (120, 0), (120, 15)):
         if batch:
             with BatchAdjointModel([case, case]) as b:
-                b.forward(120, ckpt)
+                b.forward(120, ckpt, tape)
                 g = b.backward(lam_x=np.concatenate([W, W]))
-            grads[ckpt] = float(g["mu"][0])
+            grads[ckpt, tape] = float(g["mu"][0])
             assert g["mu"][1] == g["mu"][0]
         else:
             with AdjointModel(case) as m:
-                m.forward(120, ckpt)
-                grads[ckpt] = m.backward(lam_x=W)["mu"]
+                m.forward(120, ckpt, tape)
+                grads[ckpt, tape] = m.backward(lam_x=W)["mu"]
     eps = 1e-6
-    v = case.params.const[9]
+    v = case.params.const[k_mu]
     hi, lo = C.f32(v * (1 + eps)), C.f32(v * (1 - eps))
-    fd = (lin_loss(with_const(case, 9, hi))[0] - lin_loss(with_const(case, 9, lo))[0]) / (hi - lo)
-    print(f"slide dL/dmu: ckpt 40 {grads[40]:.10e}, ckpt 120 {grads[120]:.10e}, central difference {fd:.10e}")
-    assert grads[40] == grads[120]
-    assert grads[40] == pytest.approx(fd, rel=1e-5)
+    fd = (lin_loss(with_const(case, k_mu, hi))[0] - lin_loss(with_const(case, k_mu, lo))[0]) / (hi - lo)
+    print(f"{law} slide dL/dconst({k_mu}): {grads}, central difference {fd:.10e}")
+    assert grads[40, 0] == grads[120, 0] == grads[120, 15]
+    assert grads[40, 0] == pytest.approx(fd, rel=1e-5)
