@@ -32,6 +32,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -2921,6 +2922,8 @@ __global__ void k_adj_load(int n, int npoin0, TapeRec t, const double2 *LX, cons
 // for every entry (an unused one reads the particle itself) and the sums take
 // only the used entries.
 #define ADJ_TM 2
+// pieces per level when a checkpoint segment is longer than the records kept at once
+#define ADJ_BRANCH 8
 
 template <int NS>
 __device__ __forceinline__ void adj_pair_sum(double (&a)[NS]) {
@@ -3998,7 +4001,7 @@ struct gfsph_ctx {
   int ad_tape = 0;         // most steps recorded at once (0: ckpt); longer segments get a second level
   int ad_cells = 0;        // largest cell count the forward pass's checkpoints saw
   int *ad_over = nullptr;  // device: cells a recorded step needed beyond its buffer (0: none)
-  std::vector<Snap> snaps2;  // second-level checkpoints inside one segment
+  std::vector<std::vector<Snap>> snapsL;  // checkpoints inside a segment, per level
   std::vector<Snap> snaps;  // [nseg + 1]: segment starts and the final state
   std::vector<TapeRec> tape;
   AdjWork aw{};
@@ -5666,51 +5669,45 @@ int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_
   if (bing) HIPCHK(c, hipMemsetAsync(c->bcar, 0, sizeof(double) * R, st));
   const int nb = nblk(n);
   const BinghamState &b = c->bs;
-  // at most T steps are recorded at once: a longer segment is advanced once
-  // more from its checkpoint, leaving second-level checkpoints every T steps
+  // at most T steps are recorded at once: a longer segment is split, from
+  // its checkpoint, into at most ADJ_BRANCH pieces with checkpoints of their
+  // own, recursively, each piece swept back in reverse order
   const int T = (c->ad_tape > 0 && c->ad_tape < K) ? c->ad_tape : K;
-  const int nsub_max = (K + T - 1) / T;
-  if (T < K && (int)c->snaps2.size() < nsub_max) {
-    const size_t old = c->snaps2.size();
-    c->snaps2.resize(nsub_max);
-    for (size_t k = old; k < c->snaps2.size(); k++)
-      if ((rc = alloc_snap(c, c->snaps2[k]))) return rc;
-  }
   if ((int)c->tape.size() < T) {
     const size_t old = c->tape.size();
     c->tape.resize(T);
     for (size_t k = old; k < c->tape.size(); k++)
       if ((rc = alloc_tape(c, c->tape[k]))) return rc;
   }
-  struct Piece {
-    int seg, sub, first, len;  // checkpoint (level 1 or 2), first step, steps
-  };
-  std::vector<Piece> pieces;  // in reverse time order
-  for (int seg = nseg - 1; seg >= 0; seg--) {
-    const int L = std::min(K, N - seg * K);
-    const int nsub = (L + T - 1) / T;
-    for (int j = nsub - 1; j >= 0; j--) pieces.push_back({seg, nsub > 1 ? j : -1, j * T, std::min(T, L - j * T)});
-  }
-  int cur_seg = -1;
-  for (const Piece &pc : pieces) {
-    if (pc.sub < 0) {
-      if ((rc = snap_restore(c, c->snaps[pc.seg]))) return rc;
-    } else {
-      if (pc.seg != cur_seg) {  // the segment's second-level checkpoints
-        if ((rc = snap_restore(c, c->snaps[pc.seg]))) return rc;
-        const int L = std::min(K, N - pc.seg * K);
-        for (int j = 0; j * T < L; j++) {
-          if ((rc = snap_save(c, c->snaps2[j]))) return rc;
-          if ((rc = run_steps(c, std::min(T, L - j * T)))) return rc;
-        }
-        cur_seg = pc.seg;
+  bool first_back = true;  // the step swept back first (the last step) takes lam_h
+  auto step_back = [&](TapeRec &tr) {
+    k_adj_load<<<nb, GF_BLOCK, 0, st>>>(n, c->P.npoin0, tr, c->LX, c->LV, first_back ? c->LH : nullptr, w);
+    first_back = false;
+    if (c->P.av_vel)
+      k_adj_xsph<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w);
+    else
+      (void)hipMemcpyAsync(w.lvp, w.lvpp, sizeof(double2) * n, hipMemcpyDeviceToDevice, st);
+    if (bing) {
+      for (int sd = S - 1; sd >= 0; sd--) {
+        k_adj_bing_tau<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, sd, c->lamCand, c->lamTau);
+        const double *cand = tr.bt + (size_t)(S + sd) * n;
+        k_seg_reduce<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, c->lamTau, cand, c->tlead, c->thas);
+        k_seg_top<<<(R + 63) / 64, 64, 0, st>>>(R, b.rtile, c->tlead, c->thas, c->tcin, c->bcar);
+        k_seg_down<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, c->lamTau, cand, c->tcin, c->lamCand + (size_t)sd * n);
       }
-      if ((rc = snap_restore(c, c->snaps2[pc.sub]))) return rc;
+      k_adj_stage_bing<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, c->lamCand, c->GP, c->GQ);
+    } else {
+      k_adj_stage<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, c->GP, c->GQ);
     }
-    const int L = pc.len;
+    k_adj_pairs<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w, c->LX, c->LV, c->LM);
+  };
+  // records the L <= T steps from checkpoint sn and sweeps them back
+  auto record_and_back = [&](gfsph_ctx::Snap &sn, int L) -> int {
+    int r;
+    if ((r = snap_restore(c, sn))) return r;
     HIPCHK(c, hipMemsetAsync(c->ad_over, 0, sizeof(int), st));
     for (int t = 0; t < L; t++)
-      if ((rc = taped_step(c, c->tape[t]))) return rc;
+      if ((r = taped_step(c, c->tape[t]))) return r;
     int over = 0;
     HIPCHK(c, hipMemcpyAsync(&over, c->ad_over, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
@@ -5719,41 +5716,39 @@ int gfsph_adjoint_backward(gfsph_ctx *c, const double *lam_x, const double *lam_
         dfree(c, tr.start);
         tr.start = nullptr;
         tr.start_cap = std::min(c->cap_cells + 2, over + over / 2);
-        if ((rc = dalloc(c, &tr.start, (size_t)tr.start_cap))) return rc;
+        if ((r = dalloc(c, &tr.start, (size_t)tr.start_cap))) return r;
       }
-      if (pc.sub < 0) {
-        if ((rc = snap_restore(c, c->snaps[pc.seg]))) return rc;
-      } else if ((rc = snap_restore(c, c->snaps2[pc.sub]))) {
-        return rc;
-      }
+      if ((r = snap_restore(c, sn))) return r;
       for (int t = 0; t < L; t++)
-        if ((rc = taped_step(c, c->tape[t]))) return rc;
+        if ((r = taped_step(c, c->tape[t]))) return r;
     }
-    for (int t = L - 1; t >= 0; t--) {
-      TapeRec &tr = c->tape[t];
-      const bool last = (pc.seg == nseg - 1) && (pc.first + t == std::min(K, N - pc.seg * K) - 1);
-      k_adj_load<<<nb, GF_BLOCK, 0, st>>>(n, c->P.npoin0, tr, c->LX, c->LV, last ? c->LH : nullptr, w);
-      if (c->P.av_vel) {
-        k_adj_xsph<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w);
-      } else {
-        HIPCHK(c, hipMemcpyAsync(w.lvp, w.lvpp, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
-      }
-      if (bing) {
-        for (int sd = S - 1; sd >= 0; sd--) {
-          k_adj_bing_tau<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, sd, c->lamCand, c->lamTau);
-          const double *cand = tr.bt + (size_t)(S + sd) * n;
-          k_seg_reduce<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, c->lamTau, cand, c->tlead, c->thas);
-          k_seg_top<<<(R + 63) / 64, 64, 0, st>>>(R, b.rtile, c->tlead, c->thas, c->tcin, c->bcar);
-          k_seg_down<<<b.ntiles, GF_BLOCK, 0, st>>>(b.tiles, c->lamTau, cand, c->tcin, c->lamCand + (size_t)sd * n);
-        }
-        k_adj_stage_bing<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, c->lamCand, c->GP, c->GQ);
-      } else {
-        k_adj_stage<<<nb, GF_BLOCK, 0, st>>>(n, c->Pr, c->dem, tr, w, c->GP, c->GQ);
-      }
-      k_adj_pairs<<<xcd_grid((long)n * ADJ_TM, GF_BLOCK), GF_BLOCK, 0, st>>>(n, c->P, tr, w, c->LX, c->LV, c->LM);
-    }
+    for (int t = L - 1; t >= 0; t--) step_back(c->tape[t]);
     HIPCHK(c, hipGetLastError());
-  }
+    return GFSPH_OK;
+  };
+  std::function<int(gfsph_ctx::Snap &, int, int)> sweep = [&](gfsph_ctx::Snap &sn, int L, int depth) -> int {
+    if (L <= T) return record_and_back(sn, L);
+    const int sub = (L + ADJ_BRANCH - 1) / ADJ_BRANCH, np = (L + sub - 1) / sub;
+    int r;
+    if ((int)c->snapsL.size() <= depth) c->snapsL.resize(depth + 1);
+    std::vector<gfsph_ctx::Snap> &lv = c->snapsL[depth];
+    if ((int)lv.size() < np) {
+      const size_t old = lv.size();
+      lv.resize(np);
+      for (size_t k = old; k < lv.size(); k++)
+        if ((r = alloc_snap(c, lv[k]))) return r;
+    }
+    if ((r = snap_restore(c, sn))) return r;
+    for (int j = 0; j < np; j++) {
+      if ((r = snap_save(c, lv[j]))) return r;
+      if ((r = run_steps(c, std::min(sub, L - j * sub)))) return r;
+    }
+    for (int j = np - 1; j >= 0; j--)
+      if ((r = sweep(c->snapsL[depth][j], std::min(sub, L - j * sub), depth + 1))) return r;
+    return GFSPH_OK;
+  };
+  for (int seg = nseg - 1; seg >= 0; seg--)
+    if ((rc = sweep(c->snaps[seg], std::min(K, N - seg * K), 0))) return rc;
   // back to the state the forward pass ended in
   if ((rc = snap_restore(c, c->snaps[nseg]))) return rc;
   std::vector<double> gp(n), gq(n);

@@ -233,11 +233,13 @@ int gfsph_autotune(gfsph_ctx *ctx, int32_t reps, int32_t *chosen);
  * derivative for that step. */
 int gfsph_adjoint_forward(gfsph_ctx *ctx, int32_t nsteps, int32_t ckpt_every);
 /* Records at most tape_steps steps at once in gfsph_adjoint_backward (0: a
- * whole checkpoint segment): a longer segment is advanced once more from its
- * checkpoint, leaving checkpoints every tape_steps steps, and recorded piece
- * by piece -- a third level of recomputation for horizons whose records do
- * not fit beside the checkpoints.  The gradients do not depend on it.  No
- * reference counterpart. */
+ * whole checkpoint segment).  A longer segment is split from its checkpoint
+ * into at most 8 pieces, each with a checkpoint of its own, recursively,
+ * until the pieces fit: every further level costs one more forward pass and
+ * 8 checkpoints, the records of tape_steps steps stay the largest buffer --
+ * the form of recomputation for horizons whose records do not fit beside the
+ * checkpoints.  The gradients do not depend on it (every recomputed step
+ * equals the forward's).  No reference counterpart. */
 int gfsph_adjoint_set_tape(gfsph_ctx *ctx, int32_t tape_steps);
 int gfsph_adjoint_backward(gfsph_ctx *ctx, const double *lam_x, const double *lam_v, const double *lam_h,
                            double *g_mu, double *g_xi, double *g_mass, double *g_x0, double *g_v0);

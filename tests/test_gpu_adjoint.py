@@ -208,7 +208,8 @@ def test_adjoint_segments_recompute_the_forward_grids(batch, law):
     whose box no longer covers the particles -- so every recomputed step
     equals the forward's bit for bit: the gradient is the same for any
     checkpoint scheme (checkpoints every 40 steps; one segment of 120; one
-    segment recorded 15 steps at a time from second-level checkpoints), and
+    segment recorded 15 steps at a time from second-level checkpoints, or 2
+    at a time from three levels), and
     equals central differences."""
     from gfsph import BatchAdjointModel
     from gfsph.dem import plane_runout_dem
@@ -230,7 +231,7 @@ def test_adjoint_segments_recompute_the_forward_grids(batch, law):
     L0, xe = lin_loss(case)
     assert xe[:, 0].min() > parts.x[:, 0].max()  # the block left its initial box
     grads = {}
-    for ckpt, tape in ((40, 0), (120, 0), (120, 15)):
+    for ckpt, tape in ((40, 0), (120, 0), (120, 15), (120, 2)):
         if batch:
             with BatchAdjointModel([case, case]) as b:
                 b.forward(120, ckpt, tape)
@@ -246,5 +247,5 @@ def test_adjoint_segments_recompute_the_forward_grids(batch, law):
     hi, lo = C.f32(v * (1 + eps)), C.f32(v * (1 - eps))
     fd = (lin_loss(with_const(case, k_mu, hi))[0] - lin_loss(with_const(case, k_mu, lo))[0]) / (hi - lo)
     print(f"{law} slide dL/dconst({k_mu}): {grads}, central difference {fd:.10e}")
-    assert grads[40, 0] == grads[120, 0] == grads[120, 15]
+    assert grads[40, 0] == grads[120, 0] == grads[120, 15] == grads[120, 2]
     assert grads[40, 0] == pytest.approx(fd, rel=1e-5)
