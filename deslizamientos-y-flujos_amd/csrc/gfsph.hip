@@ -70,6 +70,11 @@ struct DevParams {
   int costheta, ic_k0;
   double k0a, k0p;
   double keros, eros_c23, eros_c24, eros_c25, denss, densw_e, dens_bar_eff, slip_fact, avail0;
+  // nfric = 23 (exact Herschel-Bulkley root): hb_m = 1 (m = 1), 2 (m = 2) or
+  // 3 (m = 1/3 or 1/2, a_hat = 4.5 K |hv| / (tau_y h^2)); hb_A = (1/Cm)
+  // (K/tau_y)^(1/m), the run-constant factor of a_hat for m = 1 and 2
+  int hb_m, pad_h;
+  double hb_A;
 };
 
 struct DevDem {
@@ -111,6 +116,7 @@ struct SortedSet {
   int *flag, *id;
   int *rid;     // realisation of the particle (0 for a single run)
   double *col;  // icpwp = 2: pore-pressure columns [npwp][n] (node-major), else null
+  const double *mass0;  // nfric = 3: masses before any erosion by caller id (mass0, 5:41), else null
 };
 
 // Cell index of one or several independent realisations: realisation r has
@@ -438,10 +444,12 @@ __device__ __forceinline__ double erosion_rate(const DevParams &P, double h, dou
 // LAW = 0: every law, chosen at run time.
 #define LAW_ANY 0
 #define LAW_FRIC5 5
+// nsoil: (mass - mass0)/mass of the particle, read by nfric = 3 alone.
 template <bool EROS = false, int LAW = LAW_ANY>
 __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem, double xx, double yy,
                                           double vx1, double vy1, double rho_i, double &u_i,
-                                          double uaux1, double inx, double iny, double tau_in, double &exo,
+                                          double uaux1, double inx, double iny, double tau_in, double nsoil,
+                                          double &exo,
                                           double &eyo, double &duo, double acx = 0.0, double acy = 0.0,
                                           bool end = false, double *er_out = nullptr) {
   const double dt = P.dt;
@@ -503,6 +511,56 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     double h2 = (hfrict * hfrict);
     frictx = cf2a * hu / h2;
     fricty = cf2a * hv / h2;
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
+  } else if (!F5 && nfric == 3) {
+    // Bingham erosion (6:9443-9488): Bingham on the eroded fraction
+    // nsoil = (m - m0)/m, bounded below by Manning's law with n = const(8)
+    const double fsoilB = 1. - exp(-P.c3 * nsoil);
+    const double constKB = P.constK * fsoilB, tauy0B = P.tauy0 * fsoilB;
+    const double densB = 1000. + (P.dens - 1000.) * nsoil;
+    if (nsoil > 0.01) {
+      const double h2 = hfrict * hfrict;
+      const double a_hat = (6. * constKB * hvmod) / (tauy0B * h2);
+      if (a_hat > 50) {
+        const double cf4b = 3 * constKB / densB;
+        frictx = cf4b * hu / h2;
+        fricty = cf4b * hv / h2;
+      } else if (a_hat < 0.01) {
+        const double cf4b = tauy0B / densB;
+        frictx = cf4b * hu * hvmodinv;
+        fricty = cf4b * hv * hvmodinv;
+      } else {
+        const double a2 = 48., b2 = -32. * a_hat - 114., c2 = 65.;
+        const double xh = (-b2 - sqrt(b2 * b2 - 4. * a2 * c2)) / (2. * a2);
+        const double cf4b = (tauy0B / xh) / densB;
+        frictx = cf4b * hu * hvmodinv;
+        fricty = cf4b * hv * hvmodinv;
+      }
+    }
+    const double cf1 = P.cgra * P.c8 * P.c8;
+    const double hp = pow(hfrict, P.c73);
+    const double fxx = cf1 * hu * hvmod / hp, fyy = cf1 * hv * hvmod / hp;
+    if (fabs(fxx) > fabs(frictx)) frictx = fxx;
+    if (fabs(fyy) > fabs(fricty)) fricty = fyy;
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
+  } else if (!F5 && nfric == 23) {
+    // Herschel-Bulkley with the exact root of the regime polynomial
+    // (6:9538-9545, 6:9576-9605); Tau0 is the particle's own (not carried)
+    const double h2 = hfrict * hfrict;
+    double a_hat, a2 = 48., b2, c2 = 65.;
+    if (P.hb_m == 3) a_hat = (4.5 * P.constK * hvmod) / (P.tauy0 * h2);
+    else a_hat = P.hb_A * (hvmod / h2);
+    if (P.hb_m == 2) {
+      a2 = 10.;
+      b2 = -a_hat - 19.2188;
+      c2 = 9.1875;
+    } else {
+      b2 = -32. * a_hat - 114.;
+    }
+    const double xh = (-b2 - sqrt(b2 * b2 - 4. * a2 * c2)) / (2. * a2);
+    const double cf4b = (P.tauy0 / xh) / P.dens;
+    frictx = cf4b * hu * hvmodinv;
+    fricty = cf4b * hv * hvmodinv;
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (!F5 && (nfric == 4 || nfric == 24 || nfric == 25)) {
     const double Tauy0 = P.tauy0, constK = P.constK, m_vexp = P.visco;
@@ -596,6 +654,14 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     frictx = P.K_Bagnold * vxx * vmod / (hfrict * hfrict);
     fricty = P.K_Bagnold * vyy * vmod / (hfrict * hfrict);
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
+  } else if (!F5 && nfric == 22) {
+    // simplified Bingham + power law (6:9803-9816), as written there: a
+    // negative velocity component adds nothing and the y term goes to frictx
+    double tb_abs = P.tauy0 + (P.constK * pow(vmod / hfrict, P.visco));
+    tb_abs = tb_abs / P.dens;
+    if (vxx >= 1.e-4) frictx = tb_abs * vxx / vmod;
+    if (vyy >= 1.e-4) frictx = frictx + tb_abs * vyy / vmod;
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
   }
 
   // limiter (6:9836-9895)
@@ -649,6 +715,13 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
     if (er_out) *er_out = er;
   }
   duo = du;
+}
+
+// (mass - mass0)/mass of slot i for the Bingham-erosion law (6:9451)
+__device__ __forceinline__ double nsoil_of(const DevParams &P, const SortedSet &s, int i) {
+  if (P.nfric != 3) return 0.0;
+  const double m = s.mass[i];
+  return (m - s.mass0[s.id[i]]) / m;
 }
 
 // a13. Check_Out_Domain (7:1764-1788)
@@ -1260,7 +1333,8 @@ __device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem
   const double hS = hnew;  // Drho_RK is taken before ExtForces_SW(1) (7:928)
   if (EROS && P.eros_on && !flagged) {
     double er = 0.0, e1, e2, e3;
-    ext_force<true>(P, dem, x.x, x.y, v.x, v.y, hnew, u, uaux1, inx, iny, 0.0, e1, e2, e3, exl, eyl, true, &er);
+    ext_force<true>(P, dem, x.x, x.y, v.x, v.y, hnew, u, uaux1, inx, iny, 0.0, nsoil_of(P, s, i), e1, e2, e3, exl,
+                    eyl, true, &er);
     erosion_scatter(dem, eaux, x.x, x.y, dt * er);
     mnew = mi + dt * er * mi / hnew;
     if (P.eros_single) hnew = hnew + dt * er;
@@ -1383,13 +1457,14 @@ __global__ __launch_bounds__(GF_BLOCK, LAW == LAW_FRIC5 ? RK5_WAVES_PER_SIMD : R
       double Dx = 0, Dy = 0, Dvx = 0, Dvy = 0, Du = 0;
       bool flagged = false;
       double tau_unused = 0.0;
+      const double nsoil = LAW == LAW_FRIC5 ? 0.0 : nsoil_of(P, s, i);
       for (int st = 0; st < P.nstages; st++) {
         stage_update(P, st, x0, v0, u0, h0, rfi, ex, ey, du, inx, iny, x, v, u);
         if (out_of_domain(dem, x.x, x.y)) {
           flagged = true;
           break;
         }
-        ext_force<EROS, LAW>(P, dem, x.x, x.y, v.x, v.y, rfi, u, uaux1, inx, iny, tau_unused, ex, ey, du);
+        ext_force<EROS, LAW>(P, dem, x.x, x.y, v.x, v.y, rfi, u, uaux1, inx, iny, tau_unused, nsoil, ex, ey, du);
         const double f2 = P.f2[st];
         Dvx += f2 * (ex + inx);
         Dvy += f2 * (ey + iny);
@@ -1480,7 +1555,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_bingham_stage(int n, int phase, in
       double2 x = b.x[i], v = b.v[i];
       double u = b.u[i], ex, ey, du;
       double tau = b.tau[id];
-      ext_force(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, tau, ex, ey, du);
+      ext_force(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, tau, 0.0, ex, ey, du);
       b.u[i] = u;
       b.ex[i] = make_double2(ex, ey);
       b.du[i] = du;
@@ -2267,6 +2342,110 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
   vout[i] = vi;
 }
 
+// a12 h_upgrade (7:2866-2961) for sle = 2 and 3 over the step-start pair set
+// (the neighbour masks), with the step's final velocities, depths and masses
+// (after erosion), before XSPH; writes the new smoothing length of every
+// particle inside the domain.
+//   sle = 2: dh/dt = -(h/2)(1/h_d) dh_d/dt, i.e. vcc_i = sum_j m_j (v_j - v_i)
+//            . grad_i W_ij / h_j and hsml += dt (hsml/2) vcc_i, undone if it
+//            would reach 0 (7:2893-2919);
+//   sle = 3: McDougall & Hungr, hsml = 4 / sqrt(S_i / N_i) with N_i the
+//            particle's neighbours and S_i its own h_i/m_i added N_i times
+//            (7:2933-2959).  A particle without neighbours keeps its
+//            smoothing length (the reference takes 0/0 there and carries a
+//            NaN smoothing length into the next neighbour search).
+// The sums run in two halves (ordinal parity of a neighbour among its mask
+// word's set bits) on one lane, as the XSPH pass runs them.
+template <int SKF>
+__global__ XSPH_LB void k_hsml_m(int n, const DevParams *Pr, Cells cl, const unsigned long long *mask, SortedSet s,
+                                 NewSet o, const int *stopr, const int4 *srec) {
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= n) return;
+  const int rr_ = s.rid[i];
+  if (s.flag[i] || o.flag[i] || (stopr && stopr[rr_])) return;
+  const DevParams &P = Pr[rr_];
+  const GridInfo g = cl.grid[rr_];
+  const int *start = cl.start + cl.coff[rr_];
+  const double2 pi_ = s.pos[i];
+  const double hi = s.hsml[i];
+  CandStream cs;
+  if (!load_stream(srec, i, cs)) cs = cand_stream(g, start, pi_, hi, P.scale_k);
+  if (P.sle == 3) {
+    int N = 0;
+    if (stream_full(cs)) {
+      stream_dense<1>(cs, P.scale_k, i, 0, pi_, hi, s.nb,
+                      [&](int, const double4 &, double, double, double, double) { N++; });
+    } else {
+      const unsigned long long *mrow = mask + NB_WORDS * (long)i;
+      const int nw = (cs.total + 63) >> 6;
+#pragma unroll
+      for (int k = 0; k < NB_WORDS; k++)
+        if (k < nw) N += __popcll(mrow[k]);
+    }
+    if (N == 0) return;
+    const double ci = o.h[i] / s.mass[i];
+    double S = 0.0;
+    for (int k = 0; k < N; k++) S = S + ci;
+    const double Ls = sqrt(S / N);
+    o.hsml[i] = 4 / Ls;
+    return;
+  }
+  const double2 vi = o.vel[i];
+  double a[SPLIT] = {0.0, 0.0};
+  auto term = [&](double mj, double gx, double gy, double2 vj, double hj) {
+    double hv = (vj.x - vi.x) * gx;
+    hv = hv + (vj.y - vi.y) * gy;
+    return mj * hv / hj;
+  };
+  if (stream_full(cs)) {
+#pragma unroll
+    for (int hh = 0; hh < SPLIT; hh++)
+      stream_dense(cs, P.scale_k, i, hh, pi_, hi, s.nb,
+                   [&](int j, const double4 &q, double dx, double dy, double dr, double mh) {
+                     double w, gx, gy;
+                     sph_kernel_t<SKF>(P.kc, P.pi, dr, dx, dy, mh, w, gx, gy);
+                     a[hh] = a[hh] + term(q.w, gx, gy, o.vel[j], o.h[j]);
+                   });
+  } else {
+    unsigned long long w[NB_WORDS];
+    load_words(cs, mask + NB_WORDS * (long)i, w, ~0ull);
+    const double ihi = rcp_nr(hi);
+    // an unused entry (the particle itself) adds 0 * (v_i - v_i) . grad / h_i
+    auto body = [&](auto uh, const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      constexpr bool UH = decltype(uh)::value;
+      double4 q[NB_BATCH];
+      double2 vj[NB_BATCH];
+      double hj[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        q[u] = s.nb[js[u]];
+        vj[u] = o.vel[js[u]];
+        hj[u] = o.h[js[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh, gx, gy;
+        pair_geom(pi_, hi, q[u], dx, dy, dr, mh);
+        sph_grad_nb<SKF, UH>(P.kc, P.pi, dr, dx, dy, mh, ihi, gx, gy);
+        a[u & 1] = a[u & 1] + term(ok[u] ? q[u].w : 0.0, gx, gy, vj[u], hj[u]);
+      }
+    };
+    if (g.uniform_h)
+      word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::true_type{}, js, ok);
+      });
+    else
+      word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+        body(std::false_type{}, js, ok);
+      });
+  }
+  const double vcc = a[0] + a[1];
+  const double dh = (hi / 2) * vcc;
+  double hs = hi + P.dt * dh;
+  if (hs <= 0) hs = hs - P.dt * dh;
+  o.hsml[i] = hs;
+}
+
 // ---------------------------------------------------------------------------
 // Pressure re-evaluated in every RK stage (ic_K0 = 1 or ic_cosTheta = 1).
 // RK45 calls IntForces_SW in every stage (7:849-866) with the pair geometry
@@ -2393,7 +2572,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_stg_ext(int n, int st, const DevPa
   const double rfi = Si < P.h_inf ? P.h_inf : Si;
   const double2 in = ain[i], x = b.x[i], v = b.v[i];
   double u = b.u[i], ex, ey, du;
-  ext_force<EROS>(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, 0.0, ex, ey, du);
+  ext_force<EROS>(P, dem, x.x, x.y, v.x, v.y, rfi, u, s.uaux[i], in.x, in.y, 0.0, nsoil_of(P, s, i), ex, ey, du);
   b.u[i] = u;
   b.ex[i] = make_double2(ex, ey);
   b.du[i] = du;
@@ -4015,6 +4194,7 @@ struct gfsph_ctx {
   int *thas = nullptr;
   double *tape_bt = nullptr;
   std::vector<DevParams> Ph;  // the realisations' parameter blocks (host copy)
+  double *mass0 = nullptr;    // nfric = 3: masses before any erosion, by caller id
   // time step: ic_adapt_dt_sph (0: fixed dt_sph); with adaptation dt_sph and
   // time_sph of every realisation live on the device (Pr[r].dt, tm, dtm the
   // per-step minima), otherwise time_sph is summed here from dt_h
@@ -4102,12 +4282,27 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
     P.if_frictional = 1;
     if (P.nfric == 7 || P.nfric == 8 || P.nfric == 10 || P.nfric == 26 || P.nfric == 27) P.if_V2byR = 1;
   }
-  const int ok_nfric[] = {1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 24, 25, 26, 27};
+  // 201-203 read the porosity n_DF of the two-phase debris-flow model
+  // (6:9491-9536, 6:9728-9742), which single-phase runs do not allocate
+  const int ok_nfric[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 22, 23, 24, 25, 26, 27};
   bool ok = false;
   for (int v : ok_nfric) ok = ok || (v == P.nfric);
   if (!ok) {
     err = "basal law nfric=" + std::to_string(P.nfric) + " is outside the implemented set";
     return GFSPH_ERR_UNSUPPORTED;
+  }
+  if (P.nfric == 23) {
+    // the exponents the reference accepts (6:9578-9601; others stop it)
+    const double m = c[8];
+    if (m == 1) P.hb_m = 1;
+    else if (m == 2) P.hb_m = 2;
+    else if (fabs(m - 0.333333) <= 1.e-3 || m == 0.5) P.hb_m = 3;
+    else {
+      err = "nfric = 23 takes the exponent const(8) = 1, 2, 1/2 or 1/3 only";
+      return GFSPH_ERR_ARG;
+    }
+    const double Cm = m / ((m + 1) * (2 * m + 1));
+    P.hb_A = (1. / Cm) * pow(c[7] / c[6], 1. / m);
   }
   P.costheta = (int)(c[16] + 0.01);
   if (P.costheta != 0 && P.costheta != 1) {
@@ -4177,8 +4372,8 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
     err = "skf must be 1..4";
     return GFSPH_ERR_UNSUPPORTED;
   }
-  if (!(p->sle == 0 || p->sle == 1)) {
-    err = "sle must be 0 or 1";
+  if (p->sle < 0 || p->sle > 3) {
+    err = "sle must be 0, 1, 2 or 3";
     return GFSPH_ERR_UNSUPPORTED;
   }
   if (!(p->icpwp == 0 || p->icpwp == 1 || p->icpwp == 2 || p->icpwp == 3)) {
@@ -4197,8 +4392,8 @@ static int derive_params(const gfsph_params *p, DevParams &P, std::string &err) 
     err = "icpwp = 3 needs a positive dissipation rate Bfact (const(13))";
     return GFSPH_ERR_ARG;
   }
-  if (P.nfric == 4 && c[6] < 1.e-32) {
-    err = "Bingham yield stress const(6) must exceed 1e-32";
+  if ((P.nfric == 4 || P.nfric == 23 || P.nfric == 24 || P.nfric == 25) && c[6] < 1.e-32) {
+    err = "the yield stress const(6) must exceed 1e-32 (6:9547-9550)";
     return GFSPH_ERR_ARG;
   }
   if (p->icpwp == 2) {
@@ -4551,6 +4746,19 @@ static void launch_velocity(gfsph_ctx *c, SortedSet &s) {
     k_copy_vel<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->nw.vel, s.vel);
 }
 
+// h_upgrade for sle = 2 and 3 (sle = 1 is part of the RK kernels); a single
+// run's bounding box is then taken again from the new smoothing lengths (the
+// RK kernels accumulated it from the old ones)
+static void launch_hsml(gfsph_ctx *c, SortedSet &s) {
+  if (c->P.sle < 2) return;
+  const int bsz = c->gbs[2];
+  const int *stp = c->cases ? c->stopr : nullptr;
+  SKF_DISPATCH(k_hsml_m, xcd_grid((long)c->n, bsz), bsz, 0,
+               c->stream>>>(c->n, c->Pr, cells_of(c), c->mask, s, c->nw, stp, c->srec));
+  if (c->R == 1)
+    k_bbox<<<c->gridN, GF_BLOCK, 0, c->stream>>>(c->n, c->nw.pos, c->nw.hsml, c->nw.flag, c->part);
+}
+
 // adaptive_dt_sph before a step from set[src] (nothing with a fixed dt_sph)
 static void enqueue_dt(gfsph_ctx *c, int src) {
   const int *stopr = c->cases ? c->stopr : nullptr;
@@ -4608,6 +4816,7 @@ static int enqueue_step(gfsph_ctx *c, int src, int dst) {
   launch_density(c, s);
   if (!c->staged) launch_pforce(c, s);  // staged: once per stage instead
   launch_stages(c, s);
+  launch_hsml(c, s);
   launch_velocity(c, s);
   enqueue_case_end(c, dst);
   return GFSPH_OK;
@@ -4771,6 +4980,18 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   if ((rc = alloc_sets(c, n))) return rc;
   if ((c->P.nfric == 4 || c->staged) && (rc = alloc_bingham(c, n))) return rc;
   hipStream_t st = c->stream;
+  bool any_nfric3 = false;
+  for (int r = 0; r < R; r++) any_nfric3 = any_nfric3 || Ph[r].nfric == 3;
+  if (any_nfric3) {
+    // Bingham erosion reads (mass - mass0)/mass; mass0 = the masses as set up
+    // (6:6398) until gfsph_set_mass0 says otherwise.  A mass <= 1e-6 stops
+    // the reference (6:9450-9455).
+    for (int i = 0; i < n; i++)
+      if (!(mass[i] > 1.e-6)) return fail(c, GFSPH_ERR_ARG, "nfric = 3 needs every particle mass above 1e-6");
+    if ((rc = dalloc(c, &c->mass0, n))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->mass0, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    c->set[0].mass0 = c->set[1].mass0 = c->mass0;
+  }
 
   // DEM
   long np = (long)dem->nx * dem->ny;
@@ -5867,6 +6088,14 @@ int gfsph_set_adaptive_dt(gfsph_ctx *c, int32_t mode) {
   if (before && !mode && !c->cases && (rc = time_to_host(c))) return rc;
   drop_graphs(c);
   c->adapt = mode;
+  return GFSPH_OK;
+}
+
+int gfsph_set_mass0(gfsph_ctx *c, const double *mass0) {
+  if (!c || c->n <= 0 || !mass0) return GFSPH_ERR_ARG;
+  if (!c->mass0) return fail(c, GFSPH_ERR_ARG, "mass0 is read by nfric = 3 only");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->mass0, mass0, sizeof(double) * c->n, hipMemcpyHostToDevice));
   return GFSPH_OK;
 }
 

@@ -52,7 +52,7 @@ module gfsph_c
   public :: gfsph_create, gfsph_step, gfsph_sync, gfsph_get_state, gfsph_destroy, gfsph_last_error_f
   public :: gfsph_raster_idw, gfsph_mc_pixel_accumulate, gfsph_mc_pixel_read, gfsph_counts
   public :: gfsph_set_adaptive_dt, gfsph_time
-  public :: gfsph_get_mass, gfsph_erosion_read, gfsph_erosion_set
+  public :: gfsph_get_mass, gfsph_set_mass0, gfsph_erosion_read, gfsph_erosion_set
   public :: gfsph_get_pwp_column, gfsph_set_pwp_column
 
   interface
@@ -132,6 +132,12 @@ module gfsph_c
        type(c_ptr), value          :: ctx
        real(c_double), intent(out) :: mass(*)
      end function gfsph_get_mass
+
+     integer(c_int) function gfsph_set_mass0(ctx, mass0) bind(C, name="gfsph_set_mass0")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value         :: ctx
+       real(c_double), intent(in) :: mass0(*)
+     end function gfsph_set_mass0
 
      integer(c_int) function gfsph_erosion_read(ctx, r, z, eroded, avail) bind(C, name="gfsph_erosion_read")
        import :: c_int, c_int32_t, c_double, c_ptr

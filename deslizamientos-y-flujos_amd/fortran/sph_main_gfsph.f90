@@ -169,6 +169,10 @@ subroutine create_context
       call check(rc, 'gfsph_set_adaptive_dt')
    endif
    if (ic_erosion .eq. 1) call push_erosion
+   if (int(const(5) + 0.01) .eq. 3) then       ! Bingham erosion reads mass0 (6:9451)
+      rc = gfsph_set_mass0(ctx, mass0)
+      call check(rc, 'gfsph_set_mass0')
+   endif
    if (icpwp .eq. 2) then                       ! the columns Init_sph produced
       rc = gfsph_set_pwp_column(ctx, Uaux)
       call check(rc, 'gfsph_set_pwp_column')

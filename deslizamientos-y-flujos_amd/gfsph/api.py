@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy", "gfsph_write_netcdf",
     "gfsph_batch_cases", "gfsph_batch_status", "gfsph_inputs_cases", "gfsph_inputs_batch_create",
     "gfsph_get_mass", "gfsph_erosion_read", "gfsph_erosion_set", "gfsph_get_pwp_column", "gfsph_set_pwp_column",
-    "gfsph_neighbour_pairs_noalloc",
+    "gfsph_neighbour_pairs_noalloc", "gfsph_set_mass0",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -126,6 +126,7 @@ def load_library(path: str | None = None):
     L.gfsph_inputs_batch_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     L.gfsph_batch_status.argtypes = [vp, ip, ip, ip]
     L.gfsph_get_mass.argtypes = [vp, dp]
+    L.gfsph_set_mass0.argtypes = [vp, dp]
     L.gfsph_get_pwp_column.argtypes = [vp, dp]
     L.gfsph_set_pwp_column.argtypes = [vp, dp]
     L.gfsph_erosion_read.argtypes = [vp, C.c_int32, dp, dp, dp]
@@ -295,6 +296,14 @@ class SPHModel:
         m = np.zeros(self.n)
         self._check(self.lib.gfsph_get_mass(self.ctx, _dp(m)))
         return m
+
+    def set_mass0(self, mass0) -> None:
+        """The masses before any erosion (mass0), read by the Bingham-erosion
+        law nfric = 3; create takes the masses it is given."""
+        m = np.ascontiguousarray(mass0, dtype=np.float64)
+        if m.shape != (self.n,):
+            raise ValueError("mass0 must hold one value per particle")
+        self._check(self.lib.gfsph_set_mass0(self.ctx, _dp(m)))
 
     def erosion(self, r: int = 0) -> dict:
         """The terrain's erosion state of realisation r: current Z

@@ -121,6 +121,14 @@ int gfsph_set_pwp_column(gfsph_ctx *ctx, const double *col);
  * mass(:) after ExtForces_SW(1) ("6 SPH_SW_2019.f90":8755). */
 int gfsph_get_mass(gfsph_ctx *ctx, double *mass);
 
+/* mass0(:) ("5 SPH_Main_Vars_2019.f90":41), the masses before any erosion,
+ * in the caller's order: the Bingham-erosion law nfric = 3 reads (mass -
+ * mass0)/mass ("6 SPH_SW_2019.f90":9451).  gfsph_create takes the masses it
+ * is given, as Setup_Global_Arrays_SW does (6:6398); a caller that creates
+ * the context from a state that has already eroded hands the reference's
+ * mass0 over here.  GFSPH_ERR_ARG unless the context has nfric = 3. */
+int gfsph_set_mass0(gfsph_ctx *ctx, const double *mass0);
+
 /* Erosion.  With const(21) > 0 (law 1 Hungr, 2 slope angle, 3 basal
  * stress; or the older form const(4) > 1e-4) every RK stage adds the
  * erosion momentum term and every step ends with ExtForces_SW(1): the

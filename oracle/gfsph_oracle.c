@@ -16,11 +16,16 @@
  *                          momentum invariant
  *   rheology + limiter   - reference dumps (state after 100 RK4 steps),
  *                          inclined-plane closed forms (frictional, Voellmy
- *                          terminal speed), Bingham root identity
+ *                          terminal speed), Bingham root identity; laws 22
+ *                          and 23 (m = 1, 2, 1/3) reference dumps; law 3
+ *                          (Bingham erosion) reference dumps while the eroded
+ *                          fraction is <= 0.01, beyond it each Bingham regime
+ *                          equals a reference-pinned law bit for bit (the
+ *                          reference reads an unset local there)
  *   pore pressure        - reference dumps, exponential-decay closed form
  *   RK4 / RK2            - reference dumps (RK4), ODE order tests (RK2 has no
  *                          reference counterpart: pinned by order + closed form)
- *   XSPH / sle=1         - reference dumps
+ *   XSPH / sle=1,2,3     - reference dumps
  *   adaptive dt          - reference dumps of modes 2 and 10 (states and
  *                          time_sph after 1, 10, 100 steps); modes 1 and 4
  *                          share their terms (pinned through mode 10)
@@ -524,10 +529,11 @@ static double erosion_rate(const derived_t *D, double h, double vmod, double vxx
    6:8482-8485); after them (ExtForces_SW(1), erosion) the particle's
    exdvxdt of the last stage, which the gravity term is added to and the
    limiter reads. */
+/* nsoil: (mass - mass0)/mass of the particle, read by nfric = 3 only. */
 static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem *dem, double xx,
                           double yy, double vx1, double vy1, double rho_i, double *u_i,
-                          double uaux1, double inx, double iny, double *tau0, const double *acc0,
-                          extout *o) {
+                          double uaux1, double inx, double iny, double *tau0, double nsoil,
+                          const double *acc0, extout *o) {
   const double dt = p->dt;
   double h, hu, hv, hinv, vmod, hvmod, hvmodinv;
   double vxx = vx1, vyy = vy1;
@@ -589,6 +595,66 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
     double cf2a = D->constK / (8. * D->dens);
     frictx = cf2a * hu / (hfrict * hfrict);
     fricty = cf2a * hv / (hfrict * hfrict);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
+  } else if (nfric == 3) { /* Bingham erosion 6:9443-9488 */
+    double frictxx, frictyy;
+    const double nsoilB = nsoil; /* (mass - mass0)/mass, 6:9451 */
+    const double csoilB = p->cst[3];
+    const double fsoilB = 1. - exp(-csoilB * nsoilB);
+    const double constKB = D->constK * fsoilB, tauy0B = D->tauy0 * fsoilB;
+    const double densB = 1000. + (D->dens - 1000.) * nsoilB;
+    frictx = 0.0;
+    fricty = 0.0;
+    if (nsoilB > 0.01) {
+      double a_hat = (6. * constKB * hvmod) / (tauy0B * (hfrict * hfrict));
+      if (a_hat > 50) { /* viscous */
+        double cf4b = 3 * constKB / densB;
+        frictx = cf4b * hu / (hfrict * hfrict);
+        fricty = cf4b * hv / (hfrict * hfrict);
+      } else if (a_hat < 0.01) { /* cohesive */
+        double cf4b = tauy0B / densB;
+        frictx = cf4b * hu * hvmodinv;
+        fricty = cf4b * hv * hvmodinv;
+      } else { /* intermediate */
+        double a2 = 48., b2 = -32. * a_hat - 114., c2 = 65.;
+        double xh = (-b2 - sqrt(b2 * b2 - 4. * a2 * c2)) / (2. * a2);
+        double cf4b = (tauy0B / xh) / densB;
+        frictx = cf4b * hu * hvmodinv;
+        fricty = cf4b * hv * hvmodinv;
+      }
+    }
+    const double cmanning = p->cst[8];
+    const double cf1 = D->cgra * cmanning * cmanning;
+    const double hp = pow(hfrict, D->c73);
+    frictxx = cf1 * hu * hvmod / hp;
+    frictyy = cf1 * hv * hvmod / hp;
+    if (fabs(frictxx) > fabs(frictx)) frictx = frictxx;
+    if (fabs(frictyy) > fabs(fricty)) fricty = frictyy;
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
+  } else if (nfric == 23) { /* Herschel-Bulkley, exact root (6:9538-9545, 6:9576-9605, 6:9619-9622) */
+    const double Tauy0 = D->tauy0, constK = D->constK, m_vexp = D->visco;
+    const double Cm = m_vexp / ((m_vexp + 1) * (2 * m_vexp + 1));
+    double a_hat = (1. / Cm) * pow(constK / Tauy0, 1. / m_vexp) * (hvmod / (hfrict * hfrict));
+    double a2, b2, c2;
+    if (m_vexp == 1) {
+      a2 = 48.;
+      b2 = -32. * a_hat - 114.;
+      c2 = 65.;
+    } else if (m_vexp == 2) {
+      a2 = 10.;
+      b2 = -a_hat - 19.2188;
+      c2 = 9.1875;
+    } else { /* m = 1/3 or 1/2 (other exponents stop the reference) */
+      a_hat = (4.5 * constK * hvmod) / (Tauy0 * (hfrict * hfrict));
+      a2 = 48.;
+      b2 = -32. * a_hat - 114.;
+      c2 = 65.;
+    }
+    double xh = (-b2 - sqrt(b2 * b2 - 4. * a2 * c2)) / (2. * a2);
+    double Tau0 = Tauy0 / xh;
+    double cf4b = Tau0 / D->dens;
+    frictx = cf4b * hu * hvmodinv;
+    fricty = cf4b * hv * hvmodinv;
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
   } else if (nfric == 4 || nfric == 24 || nfric == 25) { /* 6:9538-9623 */
     double Tauy0 = D->tauy0, m_vexp = D->visco, Tau0 = *tau0;
@@ -692,6 +758,15 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
     vyy = hv * hinv;
     frictx = D->K_Bagnold * vxx * vmod / (hfrict * hfrict);
     fricty = D->K_Bagnold * vyy * vmod / (hfrict * hfrict);
+    TotFrict = sqrt(frictx * frictx + fricty * fricty);
+  } else if (nfric == 22) { /* simplified Bingham + power law 6:9803-9816 */
+    /* vxx, vyy are the particle's velocity and vmod = |hv|/h (6:8568-8589);
+       a negative component adds nothing and the y term goes to frictx, as
+       the reference writes it */
+    double tb_abs = D->tauy0 + (D->constK * pow(vmod / hfrict, D->visco));
+    tb_abs = tb_abs / D->dens;
+    if (vxx >= 1.e-4) frictx = tb_abs * vxx / vmod;
+    if (vyy >= 1.e-4) frictx = frictx + tb_abs * vyy / vmod;
     TotFrict = sqrt(frictx * frictx + fricty * fricty);
   }
 
@@ -971,6 +1046,12 @@ static void pressure_force(const gfo_params *p, const derived_t *D, const gfo_de
 }
 
 /* Node coordinates coorg(:, ipoig) of the terrain grid (2:285-296). */
+/* (mass - mass0)/mass for the Bingham-erosion law (6:9451); 0 for others */
+static double nsoil_of(const gfo_params *p, int nfric, const double *mass, int i) {
+  if (nfric != 3) return 0.0;
+  return (mass[i] - p->mass0[i]) / mass[i];
+}
+
 static void node_xy(const gfo_dem *d, int64_t ip, double *xg, double *yg) {
   if (d->coorg) {
     *xg = d->coorg[2 * ip];
@@ -987,6 +1068,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
   derived_t D;
   derive(p_in, &D);
   if (D.eros_on && !eros) return -2;
+  if (D.nfric == 3 && !p_in->mass0) return -3;
   const int64_t npg = (int64_t)dem->nx * dem->ny;
   double *auxv = D.eros_on ? (double *)calloc(2 * npg, sizeof(double)) : NULL;
   double dt = dt_time_io ? dt_time_io[0] : p_in->dt;
@@ -1090,7 +1172,8 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         if (flag[i]) continue;
         extout o;
         ext_force_one(p, &D, dem, x[2 * i], x[2 * i + 1], v[2 * i], v[2 * i + 1], rho[i], &u[i],
-                      nuaux ? uaux[(int64_t)i * ncol] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, NULL, &o);
+                      nuaux ? uaux[(int64_t)i * ncol] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry,
+                      nsoil_of(p, D.nfric, mass, i), NULL, &o);
         ex[2 * i] = o.ex;
         ex[2 * i + 1] = o.ey;
         du[i] = o.du;
@@ -1127,8 +1210,8 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         if (flag[i]) continue;
         extout o;
         ext_force_one(p, &D, dem, x[2 * i], x[2 * i + 1], v[2 * i], v[2 * i + 1], rho[i], &u[i],
-                      nuaux ? uaux[(int64_t)i * ncol] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry, &ex[2 * i],
-                      &o);
+                      nuaux ? uaux[(int64_t)i * ncol] : 0.0, inf[2 * i], inf[2 * i + 1], &tau0_carry,
+                      nsoil_of(p, D.nfric, mass, i), &ex[2 * i], &o);
         const double eb = dt * o.er;
         const double xx = x[2 * i], yy = x[2 * i + 1];
         int ipx = (int)((xx - dem->xmin) / dxg + 1), ipy = (int)((yy - dem->ymin) / dyg + 1);
@@ -1192,10 +1275,48 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
         if (u[i] < 0.0) u[i] = 0.0;
       }
     }
-    /* 10. h_upgrade sle=1 (7:2925-2931) then XSPH (7:3042-3099, 7:987-993) */
-    if (p->sle == 1) {
+    /* 10. h_upgrade (7:2866-2961) then XSPH (7:3042-3099, 7:987-993) */
+    if (p->sle == 1) { /* 7:2925-2931 */
       for (int i = 0; i < n; i++)
         if (!flag[i]) hsml[i] = 2.0 * pow(mass[i] / rho[i], 1. / 2);
+    } else if (p->sle == 2) { /* dh/dt = -(h/dim) (1/rho) drho/dt, 7:2893-2919 */
+      double *vcc = (double *)calloc(n, sizeof(double));
+      for (int64_t kk = 0; kk < pv.n; kk++) {
+        int64_t k = p->rev_order ? pv.n - 1 - kk : kk;
+        int i = pv.i[k], j = pv.j[k];
+        double dvx0 = v[2 * j] - v[2 * i], dvx1 = v[2 * j + 1] - v[2 * i + 1];
+        double hvcc = dvx0 * pdw[2 * k];
+        hvcc = hvcc + dvx1 * pdw[2 * k + 1];
+        vcc[i] = vcc[i] + mass[j] * hvcc / rho[j];
+        vcc[j] = vcc[j] + mass[i] * hvcc / rho[i];
+      }
+      for (int i = 0; i < n; i++) {
+        if (flag[i]) continue;
+        double dh = (hsml[i] / 2) * vcc[i];
+        hsml[i] = hsml[i] + dt * dh;
+        if (hsml[i] <= 0) hsml[i] = hsml[i] - dt * dh;
+      }
+      free(vcc);
+    } else if (p->sle == 3) { /* McDougall & Hungr: hsml = B / sqrt(mean h/m), 7:2933-2959 */
+      int *nnb = (int *)calloc(n, sizeof(int));
+      double *hbm = (double *)calloc(n, sizeof(double));
+      for (int64_t kk = 0; kk < pv.n; kk++) {
+        int64_t k = p->rev_order ? pv.n - 1 - kk : kk;
+        int i = pv.i[k], j = pv.j[k];
+        nnb[i]++;
+        nnb[j]++;
+        hbm[i] = hbm[i] + rho[i] / mass[i];
+        hbm[j] = hbm[j] + rho[j] / mass[j];
+      }
+      for (int i = 0; i < n; i++) {
+        /* no neighbours: 0/0 in the reference (a NaN smoothing length);
+           kept unchanged here */
+        if (flag[i] || nnb[i] == 0) continue;
+        double Ls = sqrt(hbm[i] / nnb[i]);
+        hsml[i] = 4 / Ls;
+      }
+      free(nnb);
+      free(hbm);
     }
     if (p->average_velocity) {
       memset(av, 0, sizeof(double) * 2 * n);

@@ -31,6 +31,9 @@ typedef struct {
    * velocity divergence is >= 0, else K0_pasive (6:6736-6760, 6:6857-6863) */
   int32_t ic_k0, pad3_;
   double k0_active, k0_pasive;
+  /* mass0 (5:41): the masses before any erosion, read by the Bingham-erosion
+   * law nfric = 3 (6:9451); required for it, else may be NULL */
+  const double *mass0;
 } gfo_params;
 
 typedef struct {

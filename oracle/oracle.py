@@ -23,7 +23,8 @@ class GfoParams(C.Structure):
                 ("adapt_dt", C.c_int32), ("pad2_", C.c_int32), ("dt", C.c_double), ("h_inf", C.c_double), ("pwp0rel", C.c_double),
                 ("cst", C.c_double * 111), ("nuaux", C.c_int32), ("pwp_bc_type", C.c_int32 * 2),
                 ("pwp0_type", C.c_int32), ("pwp_bc_val", C.c_double * 2), ("xirel", C.c_double),
-                ("ic_k0", C.c_int32), ("pad3_", C.c_int32), ("k0_active", C.c_double), ("k0_pasive", C.c_double)]
+                ("ic_k0", C.c_int32), ("pad3_", C.c_int32), ("k0_active", C.c_double), ("k0_pasive", C.c_double),
+                ("mass0", C.POINTER(C.c_double))]
 
 
 class GfoDem(C.Structure):
@@ -157,6 +158,7 @@ class State:
         self.v = np.ascontiguousarray(parts.v, dtype=np.float64).copy()
         self.h = np.ascontiguousarray(parts.h, dtype=np.float64).copy()
         self.mass = np.ascontiguousarray(parts.mass, dtype=np.float64).copy()
+        self.mass0 = self.mass.copy()  # masses before any erosion (mass0, 5:41)
         self.hsml = np.ascontiguousarray(parts.hsml, dtype=np.float64).copy()
         self.u = np.ascontiguousarray(parts.u, dtype=np.float64).copy()
         self.uaux = np.zeros_like(self.h)
@@ -204,6 +206,7 @@ def run(case, st, nsteps, rev_order=False):
     case amplifies rounding (its conditioning)."""
     p = make_params(case.params)
     p.rev_order = int(rev_order)
+    p.mass0 = _dp(st.mass0)
     if st.dt_time is None:
         st.dt_time = np.array([case.params.dt, 0.0])
     dh = DemHandle(case.dem)

@@ -102,6 +102,37 @@ def case_defs():
     c11 = {1: 9.81, 2: 1800.0, 3: 0.8, 5: 11, 10: 1e-2, 14: 1.0, 15: 1e-2}
     defs["g_bagnold11"] = dict(case=C.Case("g_bagnold11", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=c11),
                                            dem, scar_on(dem, 300, 19), 100), dumps=[1, 100])
+    # simplified Bingham + power law (nfric 22, 6:9803-9816): tau_y + mu (v/h)^m
+    c22 = {1: 9.81, 2: 1800.0, 5: 22, 6: 150.0, 7: 40.0, 8: 0.6, 10: 1e-2, 14: 1.0, 15: 1e-2}
+    defs["g_nfric22"] = dict(case=C.Case("g_nfric22", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=c22),
+                                         dem, scar_on(dem, 300, 61), 100), dumps=[1, 10, 100])
+    # Herschel-Bulkley with the exact regime root (nfric 23, 6:9576-9605) for
+    # the exponents m = 1, 2 and 1/3 (the 4.5 K |hv| / (tau_y h^2) form)
+    for nm, m, tau, k, seed in (("g_hb23_m1", 1.0, 800.0, 40.0, 62), ("g_hb23_m2", 2.0, 500.0, 20.0, 63),
+                                ("g_hb23_m3", 1.0 / 3.0, 600.0, 60.0, 64)):
+        c23 = {1: 9.81, 2: 1800.0, 5: 23, 6: tau, 7: k, 8: m, 10: 1e-2, 14: 1.0, 15: 1e-2}
+        defs[nm] = dict(case=C.Case(nm, C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=c23),
+                                    dem, scar_on(dem, 300, seed), 100), dumps=[1, 10, 100])
+    # Bingham erosion (nfric 3, 6:9443-9488): Bingham on the eroded fraction
+    # (m - m0)/m with const(3) = csoilB, bounded below by Manning n = const(8),
+    # under Hungr erosion (law 1).  Only the steps before the fraction reaches
+    # 0.01 (12 here): beyond it the reference reads the yield stress from
+    # basal_friction_Ext_SW's own uninitialised local Tauy0 (declared at
+    # 6:9377 for law 201, it hides ExtForces_SW's tauy0 = const(6)) and its
+    # run turns to NaN and then faults
+    c3 = {1: 9.81, 2: 2000.0, 3: 5.0, 5: 3, 6: 400.0, 7: 30.0, 8: 0.04, 10: 1e-2, 14: 1.0, 15: 1e-2,
+          11: -1.0, 21: 1, 22: 0.004}
+    defs["g_nfric3_erosion"] = dict(
+        case=C.Case("g_nfric3_erosion", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=c3),
+                    small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 12),
+        dumps=[1, 10, 12])
+    # h_upgrade (7:2866-2961): sle = 2 (dh/dt from the velocity divergence) and
+    # sle = 3 (McDougall & Hungr, hsml = 4 / sqrt(mean h/m)) under Voellmy
+    for sle, seed in ((2, 65), (3, 66)):
+        nm = f"g_sle{sle}"
+        defs[nm] = dict(case=C.Case(nm, C.Params(integrator=4, dt=0.02, h_inf=1e-3, sle=sle,
+                                                 const=C.voellmy(0.25, 400.0)),
+                                    dem, scar_on(dem, 300, seed), 100), dumps=[1, 10, 100])
     # other kernels: Gauss (scale_k 3), quintic, Wendland
     for skf in (2, 3, 4):
         defs[f"g_kernel_skf{skf}"] = dict(

@@ -48,6 +48,35 @@ def test_batch_equals_single_runs(integrator):
         np.testing.assert_array_equal(per[r]["flag"], st["flag"])
 
 
+@pytest.mark.parametrize("variant", ["sle3", "nfric23", "nfric3"])
+def test_batch_laws_and_sle_equal_single_runs(variant):
+    """Smoothing lengths evolved by h_upgrade (sle = 3, each realisation's grid
+    from its own new lengths), the exact Herschel-Bulkley law and the
+    Bingham-erosion law (mass0 per realisation) in a batch: every realisation
+    equals its run alone bit for bit."""
+    cases = realisations(R=3, seed=5, integrator=2)
+    for r, c in enumerate(cases):
+        if variant == "sle3":
+            c.params.sle = 3
+        elif variant == "nfric23":
+            c.params.const = {1: 9.81, 2: 1800.0, 5: 23, 6: 300.0 + 100 * r, 7: 20.0, 8: (1.0, 2.0, 0.5)[r],
+                              10: 1e-2, 14: 1.0, 15: 1e-2}
+        else:
+            c.params.const = {1: 9.81, 2: 2000.0, 3: 5.0, 5: 3, 6: 400.0, 7: 30.0, 8: 0.04, 10: 1e-2, 14: 1.0,
+                              15: 1e-2, 11: -1.0, 21: 1, 22: 0.004 * (r + 1)}
+    nsteps = 60
+    with BatchModel(cases) as b:
+        b.time_integration_sph(nsteps)
+        per = b.split(b.state())
+    for r, case in enumerate(cases):
+        with SPHModel(case) as m:
+            m.time_integration_sph(nsteps)
+            st = m.state()
+        for k in ("x", "v", "h", "u", "hsml"):
+            np.testing.assert_array_equal(per[r][k], st[k], err_msg=f"{variant} realisation {r} field {k}")
+        np.testing.assert_array_equal(per[r]["flag"], st["flag"])
+
+
 def test_batch_adaptive_dt_equals_single_runs():
     """Each realisation adapts its own dt_sph (ic_adapt_dt_sph = 2) exactly as
     it would alone; the minimum over particles does not depend on order."""

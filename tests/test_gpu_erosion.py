@@ -38,8 +38,12 @@ def _compare(m, st, tol, what):
 
 
 @pytest.mark.parametrize("name,integrator,adapt", [("g_erosion_hungr", 2, 0), ("g_erosion_slope", 2, 4),
-                                                   ("g_erosion_stress", 4, 2), ("g_erosion_hungr", 4, 10)])
+                                                   ("g_erosion_stress", 4, 2), ("g_erosion_hungr", 4, 10),
+                                                   ("g_nfric3_erosion", 4, 0), ("g_nfric3_erosion", 2, 0)])
 def test_erosion_matches_oracle(name, integrator, adapt):
+    """50 steps against the oracle; for the Bingham-erosion law (nfric 3) this
+    goes past the reference fixture's 12 steps, into the Bingham regimes the
+    oracle's law-equivalence tests pin (test_oracle_laws.py)."""
     case, _ = goldens.case_of(name)
     case.params.integrator = integrator
     case.params.adapt_dt = adapt

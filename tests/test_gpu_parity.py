@@ -115,7 +115,7 @@ def test_gpu_state_matches_reference(name):
                     assert err <= tol_for(s), f"{name} step {s} {k}: {err:.3e}"
 
 
-@pytest.mark.parametrize("name", ["g_frictional_pwp", "g_kernel_skf3", "g_k0", "g_pwp_column_cos"])
+@pytest.mark.parametrize("name", ["g_frictional_pwp", "g_kernel_skf3", "g_k0", "g_pwp_column_cos", "g_sle2"])
 def test_gpu_gather_lanes_give_the_same_bits(name, monkeypatch):
     """The gathers' two halves of every candidate stream run on two lanes or
     one after the other on one lane (GFSPH_GATHER_LANES, normally chosen by
@@ -132,7 +132,8 @@ def test_gpu_gather_lanes_give_the_same_bits(name, monkeypatch):
         np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
 
 
-@pytest.mark.parametrize("name", ["g_voellmy_plane", "g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1"])
+@pytest.mark.parametrize("name", ["g_voellmy_plane", "g_frictional_pwp", "g_curvature7", "g_icpwp1_sle1", "g_sle2",
+                                  "g_sle3", "g_nfric22", "g_hb23_m2"])
 def test_gpu_rk2_matches_oracle(name):
     """RK2 (Heun) has no reference counterpart: GPU against the oracle."""
     g = _gpu()
@@ -277,7 +278,12 @@ def test_gpu_mc_pixel_matches_oracle():
 def test_gpu_check_finite_and_errors():
     g = _gpu()
     case, _ = goldens.case_of("g_voellmy_plane")
-    case.params.const[5] = 23  # Herschel-Bulkley exact law: not implemented
+    case.params.const[5] = 201  # reads the two-phase porosity n_DF: not on this path
     with pytest.raises(g.GfsphError) as e:
         g.SPHModel(case)
     assert e.value.code == -2
+    case.params.const[5] = 23  # exact Herschel-Bulkley: exponents 1, 2, 1/2, 1/3 only (6:9598-9601)
+    case.params.const[8] = 1.5
+    with pytest.raises(g.GfsphError) as e:
+        g.SPHModel(case)
+    assert e.value.code == -1

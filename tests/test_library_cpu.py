@@ -15,7 +15,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     txt = open(os.path.join(REPO, "include", "gfsph.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(gfsph_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(gfsph_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_and_binding_agree():
