@@ -3831,6 +3831,34 @@ __global__ __launch_bounds__(GF_BLOCK) void k_dt_min(int n, int mode, double h_i
   }
 }
 
+// ic_adapt_dt_sph = 7, the one-dimensional free surface (6:6597-6607): the
+// particles in the caller's order, each with the next one of its
+// realisation, give 0.5 |x_(k+1) - x_k| / v where v = |vx_(k+1) + vx_k| / 2
+// exceeds 1e-6 -- every particle, inside the domain or not, no depth clamp.
+// k_dt_fs_scatter puts (x, vx) and the realisation in caller order first.
+__global__ void k_dt_fs_scatter(int n, const int *id, const double2 *pos, const double2 *vel, const int *rid,
+                                double2 *xv, int *rk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = id[i];
+  xv[k] = make_double2(pos[i].x, vel[i].x);
+  rk[k] = rid[i];
+}
+
+__global__ void k_dt_fs(int n, const double2 *xv, const int *rk, const int *stopr, unsigned long long *dtm) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k + 1 >= n) return;
+  const int r = rk[k];
+  if (rk[k + 1] != r || (stopr && stopr[r])) return;
+  const double2 a = xv[k], b = xv[k + 1];
+  const double Le = fabs(b.x - a.x);
+  const double vv = fabs(b.y + a.y) / 2.;
+  if (vv > 1.e-6) {
+    const double cc = 0.5 * Le / vv;
+    atomicMin(&dtm[r], (unsigned long long)__double_as_longlong(cc));
+  }
+}
+
 // New dt_sph of every realisation (6:6655-6660), its RK weight, time_sph
 // (8:1617; with case control k_case_pre adds the step's dt instead), and the
 // minimum slots reset for the next step.  Stopped realisations keep theirs.
@@ -4195,6 +4223,8 @@ struct gfsph_ctx {
   double *tape_bt = nullptr;
   std::vector<DevParams> Ph;  // the realisations' parameter blocks (host copy)
   double *mass0 = nullptr;    // nfric = 3: masses before any erosion, by caller id
+  double2 *fs_xv = nullptr;   // ic_adapt_dt_sph = 7: (x, vx) by caller id
+  int *fs_rk = nullptr;       //   and the realisation of each caller id
   // time step: ic_adapt_dt_sph (0: fixed dt_sph); with adaptation dt_sph and
   // time_sph of every realisation live on the device (Pr[r].dt, tm, dtm the
   // per-step minima), otherwise time_sph is summed here from dt_h
@@ -4762,7 +4792,12 @@ static void launch_hsml(gfsph_ctx *c, SortedSet &s) {
 // adaptive_dt_sph before a step from set[src] (nothing with a fixed dt_sph)
 static void enqueue_dt(gfsph_ctx *c, int src) {
   const int *stopr = c->cases ? c->stopr : nullptr;
-  if (c->adapt) {
+  if (c->adapt == 7) {
+    const SortedSet &o = c->set[src];
+    k_dt_fs_scatter<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, o.id, c->nw.pos, o.vel, o.rid, c->fs_xv, c->fs_rk);
+    k_dt_fs<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->fs_xv, c->fs_rk, stopr, c->dtm);
+    k_dt_apply<<<1, 64, 0, c->stream>>>(c->R, c->P.nstages, c->Pr, c->dtm, c->tm, stopr, c->cases ? 0 : 1);
+  } else if (c->adapt) {
     const SortedSet &o = c->set[src];
     k_dt_min<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->adapt, c->P.h_inf, c->P.cgra, c->dxg, o.vel, c->nw.h,
                                                       c->nw.hsml, c->nw.flag, o.rid, stopr, c->dtm);
@@ -6077,12 +6112,17 @@ static int time_to_host(gfsph_ctx *c) {
 
 int gfsph_set_adaptive_dt(gfsph_ctx *c, int32_t mode) {
   if (!c || c->n <= 0) return GFSPH_ERR_ARG;
-  if (mode != 0 && mode != 1 && mode != 2 && mode != 4 && mode != 10)
+  // mode 3 halves its running minimum at every pair of the pair list
+  // (6:6587-6590): with more than ~1,100 pairs it underflows to 0 and the
+  // reference's dt_sph becomes 0, below that it depends on the list order
+  if (mode != 0 && mode != 1 && mode != 2 && mode != 4 && mode != 7 && mode != 10)
     return fail(c, GFSPH_ERR_UNSUPPORTED,
-                "ic_adapt_dt_sph must be 0, 1, 2, 4 or 10 (3, pair-based, and 7, one-dimensional, are not provided)");
+                "ic_adapt_dt_sph must be 0, 1, 2, 4, 7 or 10 (3 drives dt_sph to 0 on any flow of more than ~1,100 "
+                "pairs)");
   if (mode == c->adapt) return GFSPH_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int rc;
+  if (mode == 7 && !c->fs_xv && ((rc = dalloc(c, &c->fs_xv, c->n)) || (rc = dalloc(c, &c->fs_rk, c->n)))) return rc;
   const bool before = device_time(c);
   if (!before && mode && (rc = time_to_device(c))) return rc;
   if (before && !mode && !c->cases && (rc = time_to_host(c))) return rc;

@@ -271,9 +271,12 @@ int gfsph_counts(gfsph_ctx *ctx, int32_t *n, int32_t *nactive);
  * 6521-6664), which Driver_Flow calls before each step ("8 SPH_FEM_Driver_2019.f90":
  * 1590).  mode 0 keeps dt_sph fixed; 1: min hsml/(3c), 2: min hsml/(3(c+|v|)),
  * 4: min |dx_DEM|/|v|, 10: min of 1 and 4 (c = sqrt(const(1) h), h clamped
- * to h_inf in place), dt_sph growing by at most 10 % per step.  Every
- * following step first computes the new dt_sph of each realisation on the
- * device.  Modes 3 and 7 give GFSPH_ERR_UNSUPPORTED. */
+ * to h_inf in place), 7: min 0.5 |x_(k+1) - x_k| / (|vx_(k+1) + vx_k| / 2)
+ * over consecutive particles of the caller's order (the one-dimensional free
+ * surface), dt_sph growing by at most 10 % per step.  Every following step
+ * first computes the new dt_sph of each realisation on the device.  Mode 3
+ * gives GFSPH_ERR_UNSUPPORTED: it halves its running minimum at every pair
+ * (6:6587-6590), so its dt_sph is 0 on any flow of more than ~1,100 pairs. */
 int gfsph_set_adaptive_dt(gfsph_ctx *ctx, int32_t mode);
 
 /* time_sph (the sum of the dt_sph of the steps taken, "8 SPH_FEM_Driver_2019.f90":

@@ -900,13 +900,23 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
    smallest per-particle bound over the active particles (clamping their depth
    to h_inf_SW first where the mode reads it), then dt_sph = dtmin if that is
    smaller, else min(1.1 dt_sph, dtmin). */
-static double adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *v, double *rho,
-                          const double *hsml, const int32_t *flag, double dt) {
+static double adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *x, const double *v,
+                          double *rho, const double *hsml, const int32_t *flag, double dt) {
   const int mode = p->adapt_dt;
   const double cgra = p->cst[1];
   const double dxg_mod = sqrt(dem->dx * dem->dx + dem->dy * dem->dy); /* **0.5: a square root */
   double dtmin = 1.e10;
-  for (int i = 0; i < n; i++) {
+  if (mode == 7) { /* one-dimensional free surface, particles in index order (6:6597-6607) */
+    for (int i = 0; i + 1 < n; i++) {
+      double Le = fabs(x[2 * (i + 1)] - x[2 * i]);
+      double vv = fabs(v[2 * (i + 1)] + v[2 * i]) / 2.;
+      if (vv > 1.e-6) {
+        double cc = 0.5 * Le / vv;
+        dtmin = fmin(cc, dtmin);
+      }
+    }
+  }
+  for (int i = 0; i < n && mode != 7; i++) {
     if (flag[i]) continue;
     double vv = sqrt(v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1]), cc;
     if (mode == 1 || mode == 2 || mode == 10) {
@@ -926,9 +936,9 @@ static double adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const 
   return fmin(1.1 * dt, dtmin);
 }
 
-double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *v, double *rho,
-                       const double *hsml, const int32_t *flag, double dt) {
-  return adaptive_dt(p, dem, n, v, rho, hsml, flag, dt);
+double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *x, const double *v,
+                       double *rho, const double *hsml, const int32_t *flag, double dt) {
+  return adaptive_dt(p, dem, n, x, v, rho, hsml, flag, dt);
 }
 
 /* Get_Pwp_SW for npwp > 1 (6:10966-11174), single phase without topo
@@ -1101,7 +1111,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
   for (int step = 0; step < nsteps; step++) {
     /* Driver_Flow: adaptive_dt_sph before time_integration_sph (8:1610-1616) */
     if (p->adapt_dt) {
-      dt = adaptive_dt(p, dem, n, v, rho, hsml, flag, dt);
+      dt = adaptive_dt(p, dem, n, x, v, rho, hsml, flag, dt);
       fin = (nst == 4) ? dt / 6. : dt / 2.;
       p_step.dt = dt;
     }

@@ -13,7 +13,7 @@ typedef struct {
   int32_t icpwp;              /* 0, 1 or 3 (6:1418) */
   int32_t npoin0;             /* particles [0,npoin0) skip XSPH */
   int32_t rev_order;          /* 1: every pair sum in reverse order (conditioning probe) */
-  int32_t adapt_dt;           /* ic_adapt_dt_sph: 0 fixed dt, or 1, 2, 4, 10 (6:6521-6664) */
+  int32_t adapt_dt;           /* ic_adapt_dt_sph: 0 fixed dt, or 1, 2, 4, 7, 10 (6:6521-6664) */
   int32_t pad2_;
   double dt;                  /* dt_sph (initial value when adapt_dt != 0) */
   double h_inf;               /* h_inf_SW */
@@ -57,8 +57,8 @@ int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, do
              const double *mass, const double *hsml, double *u, double *uaux, int32_t *flag);
 /* Adaptive_dt_sph_SW (6:6521-6664) alone: the next dt_sph from dt; clamps
  * rho to h_inf_SW in place where the mode reads it */
-double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *v, double *rho,
-                       const double *hsml, const int32_t *flag, double dt);
+double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const double *x, const double *v,
+                       double *rho, const double *hsml, const int32_t *flag, double dt);
 /* mass is updated in place by erosion; eros is [nx*ny][3] = topol(1),
  * topol(14), topol(15) in/out, required when erosion is on (else -2) */
 int gfo_run(const gfo_params *p, const gfo_dem *dem, int n, double *x, double *v, double *rho,

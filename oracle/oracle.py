@@ -60,7 +60,7 @@ def lib():
         L.gfo_raster_idw.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, dp, dp]
         L.gfo_mc_pixel_accum.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, dp, ip]
         L.gfo_mc_hazard.argtypes = [C.c_int64, C.c_int, dp, ip, C.c_double, dp, ip, dp, dp]
-        L.gfo_adaptive_dt.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, ip, C.c_double]
+        L.gfo_adaptive_dt.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, ip, C.c_double]
         L.gfo_adaptive_dt.restype = C.c_double
         _lib = L
     return _lib
@@ -284,8 +284,8 @@ def run_case(case, st, vstop, timestop, dtimestop, dt_driver, time_end, time_plo
     while it < max_steps and not stopped:
         it += 1
         if adapt:
-            dt_sph = lib().gfo_adaptive_dt(C.byref(prm), C.byref(dh.s), n, _dp(st.v), _dp(st.h), _dp(st.hsml),
-                                           _ip(st.flag), dt_sph)
+            dt_sph = lib().gfo_adaptive_dt(C.byref(prm), C.byref(dh.s), n, _dp(st.x), _dp(st.v), _dp(st.h),
+                                           _dp(st.hsml), _ip(st.flag), dt_sph)
         dt_keep = dt_sph
         if time_plot > 0.0:
             tpr = tplot + dt_sph

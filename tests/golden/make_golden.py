@@ -52,9 +52,10 @@ def case_defs():
         case=C.Case("g_voellmy_plane", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=C.voellmy(0.2, 500.0)),
                     small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
         dumps=[1, 10, 100])
-    # adaptive dt (Adaptive_dt_sph_SW, 6:6521-6664): modes 2 (hsml/(c+v)) and 10
+    # adaptive dt (Adaptive_dt_sph_SW, 6:6521-6664): modes 2 (hsml/(c+v)), 7
+    # (consecutive particles' spacing over their mean x speed) and 10
     # (min(hsml/c, dxg/v)) on the same release
-    for mode in (2, 10):
+    for mode in (2, 7, 10):
         defs[f"g_adaptive_dt{mode}"] = dict(
             case=C.Case(f"g_adaptive_dt{mode}", C.Params(integrator=4, dt=0.05, adapt_dt=mode, h_inf=1e-3,
                                                          const=C.voellmy(0.2, 500.0)),

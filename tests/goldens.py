@@ -14,7 +14,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 STATE_CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_curvature7", "g_mui8", "g_icpwp1_sle1",
                "g_manning1", "g_perzyna9", "g_hb25", "g_bagnold11", "g_kernel_skf2", "g_kernel_skf3",
-               "g_kernel_skf4", "g_domain_exit", "g_adaptive_dt2", "g_adaptive_dt10", "g_erosion_hungr",
+               "g_kernel_skf4", "g_domain_exit", "g_adaptive_dt2", "g_adaptive_dt7", "g_adaptive_dt10", "g_erosion_hungr",
                "g_erosion_slope", "g_erosion_stress", "g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step",
                "g_k0", "g_costheta_k0", "g_k0_pwp", "g_nfric22", "g_hb23_m1", "g_hb23_m2", "g_hb23_m3",
                "g_nfric3_erosion", "g_sle2", "g_sle3"]

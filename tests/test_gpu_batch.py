@@ -77,12 +77,14 @@ def test_batch_laws_and_sle_equal_single_runs(variant):
         np.testing.assert_array_equal(per[r]["flag"], st["flag"])
 
 
-def test_batch_adaptive_dt_equals_single_runs():
-    """Each realisation adapts its own dt_sph (ic_adapt_dt_sph = 2) exactly as
-    it would alone; the minimum over particles does not depend on order."""
+@pytest.mark.parametrize("mode", [2, 7])
+def test_batch_adaptive_dt_equals_single_runs(mode):
+    """Each realisation adapts its own dt_sph (ic_adapt_dt_sph = 2, or 7 over
+    consecutive particles of the realisation) exactly as it would alone; the
+    minimum over particles does not depend on order."""
     cases = realisations(R=4, seed=3, integrator=4)
     for c in cases:
-        c.params.adapt_dt = 2
+        c.params.adapt_dt = mode
     nsteps = 60
     with BatchModel(cases) as b:
         b.time_integration_sph(nsteps)

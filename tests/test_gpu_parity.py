@@ -157,7 +157,8 @@ def test_gpu_rk2_matches_oracle(name):
 
 
 @pytest.mark.parametrize("name,mode,integrator", [("g_bingham", 1, 4), ("g_frictional_pwp", 4, 2),
-                                                   ("g_voellmy_plane", 2, 2), ("g_curvature7", 10, 4)])
+                                                   ("g_voellmy_plane", 2, 2), ("g_curvature7", 10, 4),
+                                                   ("g_voellmy_plane", 7, 2)])
 def test_gpu_adaptive_dt_matches_oracle(name, mode, integrator):
     """Adaptive dt_sph on paths the reference fixtures do not cover (Bingham,
     RK2, pore pressure): GPU against the oracle, including dt_sph and time_sph
