@@ -136,6 +136,7 @@ struct NewSet {
   int *flag;
   double *col;  // icpwp = 2: columns after the step [npwp][n]
   double *dh;   // icpwp = 2: the step's depth change Drho_RK, read by the column update
+  double4 *vhm;  // (vx, vy, h, mass) after the step, one record for the XSPH gathers
 };
 
 // The state after a step: positions, depths, smoothing lengths, pore
@@ -1375,6 +1376,7 @@ __device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem
   o.pos[i] = x;
   o.vel[i] = v;
   o.h[i] = hnew;
+  o.vhm[i] = make_double4(v.x, v.y, hnew, mnew);
   o.hsml[i] = hs;
   o.u[i] = u;
   o.uaux[i] = uaux_new;
@@ -2030,7 +2032,7 @@ __device__ __forceinline__ void sph_grad_nb(double kc, double pi, double dr, dou
 // UNIFORM: all smoothing lengths are equal, so the support radius
 // scale_k*(hi+hj)/2 = scale_k*hi and the kernel's 1/h are the particle's own.
 template <int SKF, bool UNIFORM>
-__device__ __forceinline__ unsigned density_batch(const CandStream &cs, const DevParams &P, int i, int k0,
+__device__ __forceinline__ unsigned density_batch(const CandStream &cs, const DevParams &P, int self, int k0,
                                                   double2 pi_, double hi, const PredBand &pu, double ihi,
                                                   const double4 *nb, double &acc) {
   int j[NB_BATCH];
@@ -2060,7 +2062,7 @@ __device__ __forceinline__ unsigned density_batch(const CandStream &cs, const De
   unsigned nib = 0u;
 #pragma unroll
   for (int u = 0; u < NB_BATCH; u++) {
-    const bool nbr = in[u] && k0 + u < cs.total && j[u] != i;
+    const bool nbr = in[u] && k0 + u < cs.total && j[u] != self;
     nib |= nbr ? (1u << u) : 0u;
     const double t = acc + q[u].w * sph_w_t<SKF, UNIFORM>(P.kc, P.pi, dr[u], mh[u], ihi);
     acc = nbr ? t : acc;
@@ -2076,7 +2078,7 @@ __device__ __forceinline__ unsigned density_batch(const CandStream &cs, const De
 // stream positions 64k .. 64k+63; its bits are gathered in a register while
 // the word's positions are walked.
 template <int SKF, bool UNIFORM, int TM>
-__device__ __forceinline__ void density_stream(const CandStream &cs, const DevParams &P, int i, int h,
+__device__ __forceinline__ void density_stream(const CandStream &cs, const DevParams &P, int self, int h,
                                                double2 pi_, double hi, const double4 *nb,
                                                unsigned long long (&wd)[NB_WORDS], double (&acc)[2]) {
   const PredBand pu = pred_band(P.scale_k * hi);
@@ -2088,9 +2090,9 @@ __device__ __forceinline__ void density_stream(const CandStream &cs, const DevPa
     unsigned long long bits = 0ull;
     for (int k = (TM == 1 ? 0 : NB_BATCH * h); k < 64 && 64 * wk + k < cs.total; k += SPLIT * NB_BATCH) {
       const int k0 = 64 * wk + k;
-      unsigned nib = density_batch<SKF, UNIFORM>(cs, P, i, k0, pi_, hi, pu, ihi, nb, acc[0]);
+      unsigned nib = density_batch<SKF, UNIFORM>(cs, P, self, k0, pi_, hi, pu, ihi, nb, acc[0]);
       if constexpr (TM == 1)
-        nib |= density_batch<SKF, UNIFORM>(cs, P, i, k0 + NB_BATCH, pi_, hi, pu, ihi, nb, acc[1]) << NB_BATCH;
+        nib |= density_batch<SKF, UNIFORM>(cs, P, self, k0 + NB_BATCH, pi_, hi, pu, ihi, nb, acc[1]) << NB_BATCH;
       bits |= (unsigned long long)nib << k;
     }
     wd[wk] = bits;
@@ -2101,7 +2103,7 @@ __device__ __forceinline__ void density_stream(const CandStream &cs, const DevPa
 template <int SKF, int TM>
 __global__ DENSITY_LB void k_density_m(int n, DevParams P, Cells cl, SortedSet s, double *S,
                                                                  double *Pout, unsigned long long *mask,
-                                                                 int4 *srec) {
+                                                                 int4 *srec, double4 *nbp) {
   int i, t;
   team_index<TM>(i, t);
   if (i >= n) return;
@@ -2155,7 +2157,9 @@ __global__ DENSITY_LB void k_density_m(int n, DevParams P, Cells cl, SortedSet s
   S[i] = Si;
   const double rf = Si < P.h_inf ? P.h_inf : Si;
   const double p = P.cgra05 * rf * rf;
-  Pout[i] = (P.pa_sph == 2) ? p / (rf * rf) : p;
+  const double Pi = (P.pa_sph == 2) ? p / (rf * rf) : p;
+  Pout[i] = Pi;
+  nbp[i] = make_double4(pi_.x, pi_.y, Pi, s.mass[i]);
 }
 
 // pressure-force contribution of neighbour j (IntForces_SW pair term)
@@ -2183,7 +2187,7 @@ template <int SKF, int PA, int TM>
 __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
                                                                 const unsigned long long *mask, SortedSet s,
                                                                 const double *S, const double *Pp, double2 *ain,
-                                                                const int4 *srec) {
+                                                                const int4 *srec, const double4 *nbp) {
   int i, t;
   team_index<TM>(i, t);
   if (i >= n) return;
@@ -2242,7 +2246,24 @@ __global__ PFORCE_LB void k_pforce_m(int n, DevParams P, Cells cl,
         pforce_term<PA>(Pi, rfi, pj[u], sj[u], P.h_inf, ok[u] ? q[u].w : 0.0, gx, gy, ax[hh], ay[hh]);
       }
     };
-    if (g.uniform_h)
+    // equal smoothing lengths and pa_sph = 2: a neighbour's position, pressure
+    // and mass come as one 32-byte record (written by the density pass)
+    auto body_p = [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      double4 r[NB_BATCH];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) r[u] = nbp[js[u]];
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh, gx, gy;
+        pair_geom(pi_, hi, make_double4(r[u].x, r[u].y, hi, r[u].w), dx, dy, dr, mh);
+        sph_grad_nb<SKF, true>(P.kc, P.pi, dr, dx, dy, mh, ihi, gx, gy);
+        const int hh = TM == 1 ? (u & 1) : 0;
+        pforce_term<PA>(Pi, rfi, r[u].z, 0.0, P.h_inf, ok[u] ? r[u].w : 0.0, gx, gy, ax[hh], ay[hh]);
+      }
+    };
+    if (PA == 2 && nbp && g.uniform_h)
+      word_batches(cs, w, i, body_p);
+    else if (g.uniform_h)
       word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::true_type{}, js, ok);
       });
@@ -2326,7 +2347,33 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
         hy[hh] = hy[hh] - c * (vi.y - vj[u].y);
       }
     };
-    if (g.uniform_h)
+    // equal smoothing lengths, every particle averaged: a neighbour's
+    // position is the first half of its record, its new velocity, depth and
+    // mass one 32-byte record of the step output
+    auto body_v = [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
+      double2 xy[NB_BATCH];
+      double4 r[NB_BATCH];
+      const double2 *nxy = reinterpret_cast<const double2 *>(s.nb);
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        xy[u] = nxy[2 * (long)js[u]];
+        r[u] = o.vhm[js[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < NB_BATCH; u++) {
+        double dx, dy, dr, mh;
+        pair_geom(pi_, hi, make_double4(xy[u].x, xy[u].y, hi, r[u].w), dx, dy, dr, mh);
+        const double w = sph_w_t<SKF, true>(P.kc, P.pi, dr, mh, ihi);
+        const double mj = ok[u] ? r[u].w : 0.0;
+        const double c = (2 * mj) * w * rcp_nr(rhoi + r[u].z);
+        const int hh = TM == 1 ? (u & 1) : 0;
+        hx[hh] = hx[hh] - c * (vi.x - r[u].x);
+        hy[hh] = hy[hh] - c * (vi.y - r[u].y);
+      }
+    };
+    if (!skip0 && g.uniform_h)
+      word_batches(cs, w, i, body_v);
+    else if (g.uniform_h)
       word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::true_type{}, js, ok);
       });
@@ -4159,6 +4206,7 @@ struct gfsph_ctx {
   SortedSet set[2]{};
   NewSet nw{};
   double *S = nullptr, *Pp = nullptr;
+  double4 *nbp = nullptr;  // per slot (x, y, pressure, mass) from the density pass, read by the force pass
   unsigned long long *mask = nullptr;
   int4 *srec = nullptr;  // the step's candidate-stream records [n][2]
   double2 *ain = nullptr;  // pressure force per particle
@@ -4521,7 +4569,7 @@ static int alloc_sets(gfsph_ctx *c, int n) {
   NewSet &o = c->nw;
   if ((rc = dalloc(c, &o.pos, n)) || (rc = dalloc(c, &o.vel, n)) || (rc = dalloc(c, &o.h, n)) ||
       (rc = dalloc(c, &o.hsml, n)) || (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) ||
-      (rc = dalloc(c, &o.flag, n)))
+      (rc = dalloc(c, &o.flag, n)) || (rc = dalloc(c, &o.vhm, n)))
     return rc;
   if (c->P.npwp) {  // pore-pressure columns, node-major, cleared
     const size_t nc = (size_t)c->P.npwp * n;
@@ -4533,7 +4581,7 @@ static int alloc_sets(gfsph_ctx *c, int n) {
   }
   if ((rc = dalloc(c, &c->mask, NB_WORDS * (size_t)n)) || (rc = dalloc(c, &c->ain, n))) return rc;
   if ((rc = dalloc(c, &c->srec, 2 * (size_t)n))) return rc;
-  if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->key, n)) ||
+  if ((rc = dalloc(c, &c->S, n)) || (rc = dalloc(c, &c->Pp, n)) || (rc = dalloc(c, &c->nbp, n)) || (rc = dalloc(c, &c->key, n)) ||
       (rc = dalloc(c, &c->rank, n)) || (rc = dalloc(c, &c->perm, n)))
     return rc;
   // a single run caps its grid at CAP_PER_PART n + 4096 cells, realisation r of a batch
@@ -4628,7 +4676,7 @@ static inline int xcd_grid(long n, int bsz) { return (int)((((n + bsz - 1) / bsz
 
 static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[1], tm = c->gteam[1];
-#define PFORCE_ARGS c->n, c->P, cells_of(c), c->mask, s, c->S, c->Pp, c->ain, c->srec
+#define PFORCE_ARGS c->n, c->P, cells_of(c), c->mask, s, c->S, c->Pp, c->ain, c->srec, c->staged ? nullptr : c->nbp
   switch (c->P.skf) {
     case 1: PFORCE_CASE(1) break;
     case 2: PFORCE_CASE(2) break;
@@ -4756,10 +4804,10 @@ static void launch_density(gfsph_ctx *c, SortedSet &s) {
   const int bsz = c->gbs[0];
   if (c->gteam[0] == 1)
     SKF_TM_DISPATCH(k_density_m, 1, xcd_grid((long)c->n, bsz), bsz, 0,
-                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask, c->srec));
+                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask, c->srec, c->nbp));
   else
     SKF_TM_DISPATCH(k_density_m, 2, xcd_grid((long)c->n * 2, bsz), bsz, 0,
-                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask, c->srec));
+                    c->stream>>>(c->n, c->P, cells_of(c), s, c->S, c->Pp, c->mask, c->srec, c->nbp));
 }
 
 // XSPH (or the plain copy of the post-RK velocity) into the sorted set
