@@ -3715,6 +3715,23 @@ __global__ void k_init_column(int n, const DevParams *Pr, int pwp0_type, double 
   s.u[i] = s.col[i];
 }
 
+// Input_SW's case variable 401 (6:1453-1479): every particle's pore
+// pressure u and Uaux(:) times its realisation's factor (the state the next
+// step sorts from, nw, and the sorted set it is in the order of)
+__global__ void k_scale_pwp(int n, const double *f, SortedSet s, NewSet o, int ncol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double c = f[s.rid[i]];
+  o.u[i] = o.u[i] * c;
+  o.uaux[i] = o.uaux[i] * c;
+  s.u[i] = s.u[i] * c;
+  s.uaux[i] = s.uaux[i] * c;
+  for (int k = 0; k < ncol; k++) {
+    o.col[(long)k * n + i] = o.col[(long)k * n + i] * c;
+    s.col[(long)k * n + i] = s.col[(long)k * n + i] * c;
+  }
+}
+
 // new-set view of a sorted set (used to re-sort the initial state)
 __global__ void k_copy_to_new(int n, SortedSet s, NewSet o, int ncol) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -6176,6 +6193,23 @@ int gfsph_set_adaptive_dt(gfsph_ctx *c, int32_t mode) {
   if (before && !mode && !c->cases && (rc = time_to_host(c))) return rc;
   drop_graphs(c);
   c->adapt = mode;
+  return GFSPH_OK;
+}
+
+int gfsph_batch_scale_pwp(gfsph_ctx *c, const double *factor) {
+  if (!c || c->n <= 0 || !factor) return GFSPH_ERR_ARG;
+  if (!c->P.icpwp) return fail(c, GFSPH_ERR_ARG, "no pore pressure (icpwp = 0)");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double *f = nullptr;
+  hipError_t e = hipMalloc((void **)&f, sizeof(double) * c->R);
+  if (!e) e = hipMemcpy(f, factor, sizeof(double) * c->R, hipMemcpyHostToDevice);
+  if (!e) {
+    k_scale_pwp<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, f, c->set[c->cur], c->nw, c->P.npwp);
+    e = hipGetLastError();
+  }
+  if (!e) e = hipStreamSynchronize(c->stream);
+  if (f) (void)hipFree(f);
+  if (e) return fail(c, GFSPH_ERR_HIP, std::string("scale_pwp: ") + hipGetErrorString(e));
   return GFSPH_OK;
 }
 

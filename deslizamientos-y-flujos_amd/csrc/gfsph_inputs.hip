@@ -537,15 +537,111 @@ struct SrcRaw {  // Read_Source_plus_New after reading: shifted points and the r
   int nbx = 0, nby = 0;
 };
 
-// case table of MASTER.dat (Driver_Input, "8 SPH_FEM_Driver_2019.f90":475-639)
+// a sampled case variable (probab_law, Param_prob_law or the histogram of
+// GetMCarlo_Cont_values / GetMCarlo_Dscrt_values, 8:1282-1439)
+struct Sampler {
+  int law = 0;                    // 0 constant, 1 triangular, 2 normal, 3 log-normal, 100 histogram
+  double prm[10] = {};            // Param_prob_law(icase, 1:10)
+  std::vector<double> xh, Fh;     // histogram nodes and their accumulated, normalised frequencies
+};
+
+// case table of MASTER.dat (Driver_Input, "8 SPH_FEM_Driver_2019.f90":475-1236)
 struct CaseTable {
-  int mode = 0;                              // ic_cases_mat (0: none, 1 lines, 2 full factorial)
+  int mode = 0;  // ic_cases_mat as read: 1 lines, 2 full factorial, 3 sampled rows of it, 4 / 31 / 32
+                 // sampled values, 20 / 21 FOSM (first-order second-moment) sets
   int ncases = 0, ninfo = 0;
   std::vector<std::vector<double>> C;        // Ccases_mat rows (10 columns; unread ones 0)
   std::vector<std::vector<double>> list;     // List_Vals rows of statistic-list variables (law >= 100)
   double vstop = 0, timestop = 0, dtimestop = 0;
   std::vector<std::vector<double>> OC;       // OC_mat: [nproblems][3 + ncases + 2]
+  // sampled sets: the problems drawn (n_MCarlo_problems), each variable's law
+  long nmc = 0;
+  std::vector<Sampler> smp;
+  std::vector<std::vector<double>> full;     // mode 3: the full factorial table the rows are drawn from
+  uint64_t seed = 1;
 };
+
+// The uniform numbers of the sampled sets.  The reference draws them with
+// Fortran's random_number after RANDOM_SEED(PUT = System_clock count)
+// (8:1306-1312), so its sets are not reproducible; here every variable k has
+// its own stream from the set's seed: SplitMix64 from state seed +
+// 0x9E3779B97F4A7C15 (k + 1), each output u giving ((u >> 11) + 0.5) 2^-53 in
+// (0, 1).  mode 3 draws its row indices from stream k = 0.
+struct McStream {
+  uint64_t s;
+  McStream(uint64_t seed, int k) : s(seed + 0x9E3779B97F4A7C15ull * (uint64_t)(k + 1)) {}
+  double next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return ((double)(z >> 11) + 0.5) * 0x1p-53;
+  }
+};
+
+// One value of a sampled variable from the uniform y (8:1297-1366,
+// 8:1391-1436): the inverse of the law's distribution function.  Normal and
+// log-normal use the approximation Phi(t) ~ (1 + sign(t) sqrt(1 - exp(-5 t^2 / 8)))
+// / 2 the reference inverts (with y kept in [1e-5, 0.99999]); a histogram's
+// distribution is linear between its nodes (y on a node gives the node).
+static double mc_value(const Sampler &S, double y) {
+  const double *P = S.prm;
+  if (S.law == 0) return P[0];
+  if (S.law == 1) {
+    const double a = P[0], b = P[1], c = P[2];
+    const double ba = b - a, ca = c - a, cb = c - b;
+    const double baca = ba * ca, cbca = cb * ca, yb = ba / ca;
+    return y <= yb ? a + sqrt(baca * y) : c - sqrt(cbca * (1. - y));
+  }
+  if (S.law == 2 || S.law == 3) {
+    const double aa = 5. / 8., aainv = 1 / aa;
+    if (y < 0.00001) y = 0.00001;
+    if (y > 0.99999) y = 0.99999;
+    const double b = log(4 * y * (1. - y));
+    const double t = y >= 0.5 ? sqrt(-aainv * b) : -sqrt(-aainv * b);
+    if (S.law == 2) return P[0] + P[1] * t;
+    const double r = P[1] / P[0];
+    const double ls = sqrt(log(1. + r * r));
+    const double lm = log(P[0]) - 0.5 * ls * ls;
+    return exp(lm + t * ls);
+  }
+  // histogram
+  const size_t np = S.xh.size();
+  for (size_t ip = 0; ip + 1 < np; ip++) {
+    const double discrim = (S.Fh[ip] - y) * (S.Fh[ip + 1] - y);
+    if (discrim < 0.0) {
+      const double dd = (y - S.Fh[ip]) / (S.Fh[ip + 1] - S.Fh[ip]);
+      return S.xh[ip] + dd * (S.xh[ip + 1] - S.xh[ip]);
+    }
+    if (S.Fh[ip] == y) return S.xh[ip];
+  }
+  return S.xh[np - 1];
+}
+
+// OC_mat of a sampled set from its seed: modes 4, 31, 32 fill column 3 + k
+// with n_MCarlo_problems values of variable k; mode 3 takes n_MCarlo_problems
+// rows of the full factorial table at indices nint(y * nproblems), kept in
+// 1 .. nproblems (8:623-630).
+static void mc_fill(CaseTable &T) {
+  const int ncol = 3 + T.ncases + 2;
+  if (T.mode == 3) {
+    const long np = (long)T.full.size();
+    McStream st(T.seed, 0);
+    T.OC.clear();
+    for (long k = 0; k < T.nmc; k++) {
+      long ir = lround(st.next() * (double)np);  // nint: halves away from zero
+      if (ir == 0) ir = 1;
+      if (ir > np) ir = np;
+      T.OC.push_back(T.full[ir - 1]);
+    }
+    return;
+  }
+  T.OC.assign(T.nmc, std::vector<double>(ncol, 0.0));
+  for (int k = 0; k < T.ncases; k++) {
+    McStream st(T.seed, k);
+    for (long p = 0; p < T.nmc; p++) T.OC[p][3 + k] = mc_value(T.smp[k], st.next());
+  }
+}
 
 struct gfsph_inputs {
   std::string err;
@@ -576,6 +672,99 @@ int ifail(gfsph_inputs *in, int code, const std::string &m) {
     if (!(call)) return ifail(in, GFSPH_ERR_ARG, R.error()); \
   } while (0)
 
+// One variable's law: probab_law, then Param_prob_law(1:10) for a
+// continuous law or the histogram (np_h, nodes x_h, element counts p_h) that
+// GetMCarlo_Dscrt_values reads (8:735-765, 8:1391-1406).
+int read_law(gfsph_inputs *in, FReader &R, const std::string &path, Sampler &S) {
+  std::vector<long> iv;
+  std::vector<double> rv;
+  RD(R.text());
+  RD(R.ints(iv, 1));
+  S.law = (int)iv[0];
+  if (S.law < 100) {
+    RD(R.text());
+    RD(R.reals(rv, 10));
+    for (int k = 0; k < 10; k++) S.prm[k] = rv[k];
+  } else if (S.law == 100) {
+    RD(R.text());
+    RD(R.ints(iv, 1));
+    const long np = iv[0];
+    if (np < 2) return ifail(in, GFSPH_ERR_ARG, path + ": a histogram needs at least 2 nodes");
+    RD(R.text());
+    RD(R.reals(rv, np));
+    S.xh = rv;
+    RD(R.text());
+    RD(R.reals(rv, np - 1));
+    S.Fh.assign(np, 0.0);
+    for (long ip = 1; ip < np; ip++) S.Fh[ip] = S.Fh[ip - 1] + rv[ip - 1];
+    const double tot = S.Fh[np - 1];
+    if (!(tot > 0)) return ifail(in, GFSPH_ERR_ARG, path + ": histogram frequencies must add up to > 0");
+    for (long ip = 0; ip < np; ip++) S.Fh[ip] = S.Fh[ip] / tot;
+  }
+  return GFSPH_OK;
+}
+
+// the laws the samplers draw from (others leave the values 0 in the reference)
+int check_law(gfsph_inputs *in, const std::string &path, const Sampler &S) {
+  if (S.law == 1 && !(S.prm[2] > S.prm[0]))
+    return ifail(in, GFSPH_ERR_ARG, path + ": a triangular law needs max > min");
+  if (S.law == 3 && !(S.prm[0] > 0))
+    return ifail(in, GFSPH_ERR_ARG, path + ": a log-normal law needs a positive mean");
+  if (!(S.law == 0 || S.law == 1 || S.law == 2 || S.law == 3 || S.law == 100))
+    return ifail(in, GFSPH_ERR_UNSUPPORTED,
+                 path + ": probability law " + std::to_string(S.law) +
+                     " (0 constant, 1 triangular, 2 normal, 3 log-normal, 100 histogram are drawn)");
+  return GFSPH_OK;
+}
+
+// ic_cases_mat 32 (8:1138-1234): the stop thresholds, ncases_mat, mx_cases,
+// ninfo_mat, n_MCarlo_problems and n_GZones (taken as 1), then mx_cases
+// records (variable, law, 10 parameters); the records with a law > 0 become
+// the case variables in order, continuous laws only (a histogram stops the
+// reference).
+int read_case_set32(gfsph_inputs *in, FReader &R, const std::string &path) {
+  CaseTable &T = in->cases;
+  std::vector<long> iv;
+  std::vector<double> rv;
+  RD(R.text());
+  RD(R.reals(rv, 3));
+  T.vstop = rv[0];
+  T.timestop = rv[1];
+  T.dtimestop = rv[2];
+  RD(R.text());
+  RD(R.ints(iv, 3));
+  T.ncases = (int)iv[0];
+  const long mx = iv[1];
+  T.ninfo = (int)iv[2];
+  if (T.ncases < 1 || mx < T.ncases) return ifail(in, GFSPH_ERR_ARG, path + ": 1 <= ncases_mat <= mx_cases expected");
+  RD(R.text());
+  RD(R.ints(iv, 2));
+  T.nmc = iv[0];
+  if (T.nmc < 1 || T.nmc > 1000000) return ifail(in, GFSPH_ERR_ARG, path + ": 1 to 10^6 Monte Carlo problems");
+  T.C.assign(T.ncases, std::vector<double>(10, 0.0));
+  T.smp.assign(T.ncases, Sampler());
+  RD(R.text());
+  int nc = 0;
+  for (long r = 0; r < mx; r++) {
+    RD(R.reals(rv, 12));
+    if ((long)rv[1] > 0 && nc < T.ncases) {
+      T.C[nc][0] = rv[0];
+      T.smp[nc].law = (int)rv[1];
+      for (int k = 0; k < 10; k++) T.smp[nc].prm[k] = rv[2 + k];
+      nc++;
+    }
+  }
+  if (nc < T.ncases) return ifail(in, GFSPH_ERR_ARG, path + ": fewer records with a law than ncases_mat");
+  for (int c = 0; c < T.ncases; c++) {
+    if (T.smp[c].law >= 100)
+      return ifail(in, GFSPH_ERR_ARG, path + ": set 32 draws from continuous laws only (8:1218-1222)");
+    if (int rc = check_law(in, path, T.smp[c])) return rc;
+  }
+  T.list.assign(T.ncases, std::vector<double>());
+  mc_fill(T);
+  return GFSPH_OK;
+}
+
 // Driver_Input subset (8:229-340): SPH only, no GFL, no case tables
 // Case tables and the problem matrix OC_mat (8:475-639): mode 1 varies one
 // variable at a time over its values (the others at their defaults), mode 2
@@ -586,6 +775,7 @@ int read_case_table(gfsph_inputs *in, FReader &R, const std::string &path) {
   CaseTable &T = in->cases;
   std::vector<long> iv;
   std::vector<double> rv;
+  if (T.mode == 32) return read_case_set32(in, R, path);
   RD(R.text());
   RD(R.ints(iv, 2));
   T.ncases = (int)iv[0];
@@ -599,7 +789,7 @@ int read_case_table(gfsph_inputs *in, FReader &R, const std::string &path) {
     for (int k = 0; k < T.ninfo; k++) T.C[c][k] = rv[k];
   }
   T.list.assign(T.ncases, std::vector<double>());
-  if (T.mode == 2) {
+  if (T.mode == 2 || T.mode == 3) {
     bool any = false;
     for (int c = 0; c < T.ncases; c++) any = any || T.C[c][5] >= 100;
     if (any) {
@@ -620,6 +810,45 @@ int read_case_table(gfsph_inputs *in, FReader &R, const std::string &path) {
   T.dtimestop = rv[2];
   const int ncol = 3 + T.ncases + 2;
   T.OC.clear();
+  if (T.mode == 4 || T.mode == 31) {  // sampled values (8:693-772, 8:1053-1136)
+    RD(R.text());
+    long nz = 1;
+    if (T.mode == 31) {
+      RD(R.ints(iv, 2));
+      nz = iv[1];
+    } else {
+      RD(R.ints(iv, 1));
+    }
+    T.nmc = iv[0];
+    if (T.nmc < 1 || T.nmc > 1000000) return ifail(in, GFSPH_ERR_ARG, path + ": 1 to 10^6 Monte Carlo problems");
+    if (nz != 1)
+      return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": n_GZones > 1 (the terrain's geological zones) is not read");
+    T.smp.assign(T.ncases, Sampler());
+    for (int c = 0; c < T.ncases; c++) {
+      int rc = read_law(in, R, path, T.smp[c]);
+      if (!rc) rc = check_law(in, path, T.smp[c]);
+      if (rc) return rc;
+    }
+    mc_fill(T);
+    return GFSPH_OK;
+  }
+  if (T.mode == 20 || T.mode == 21) {  // FOSM: the means and the sigmas, 2 ncases + 1 problems (8:974-1051)
+    RD(R.text());
+    std::vector<double> mu, sg;
+    RD(R.reals(mu, T.ncases));
+    RD(R.reals(sg, T.ncases));
+    T.OC.assign(2 * T.ncases + 1, std::vector<double>(ncol, 0.0));
+    for (int c = 0; c < T.ncases; c++) T.OC[0][3 + c] = mu[c];
+    // as the reference fills rows 2c and 2c+1: every column the mean of
+    // variable c (8:1039-1045), then column c at mean +- sigma/10
+    for (int c = 0; c < T.ncases; c++)
+      for (int j = 0; j < T.ncases; j++) T.OC[2 * c + 1][3 + j] = T.OC[2 * c + 2][3 + j] = mu[c];
+    for (int c = 0; c < T.ncases; c++) {
+      T.OC[2 * c + 1][3 + c] = mu[c] + sg[c] / 10.;
+      T.OC[2 * c + 2][3 + c] = mu[c] - sg[c] / 10.;
+    }
+    return GFSPH_OK;
+  }
   auto value = [&](int c, long ival, int col_lo, bool logs) {  // min + dval*(ival-1), log scales
     double mn = T.C[c][col_lo], mx = T.C[c][col_lo + 1];
     const long nval = (long)T.C[c][2];
@@ -671,6 +900,14 @@ int read_case_table(gfsph_inputs *in, FReader &R, const std::string &path) {
       }
       T.OC.push_back(row);
     }
+    if (T.mode == 3) {  // n_MCarlo_problems rows drawn from the table (8:589-600, 8:623-634)
+      RD(R.text());
+      RD(R.ints(iv, 1));
+      T.nmc = iv[0] < np ? iv[0] : np;
+      if (T.nmc < 1) return ifail(in, GFSPH_ERR_ARG, path + ": n_MCarlo_problems >= 1 expected");
+      T.full.swap(T.OC);
+      mc_fill(T);
+    }
   }
   return GFSPH_OK;
 }
@@ -687,10 +924,11 @@ int read_master(gfsph_inputs *in, const std::string &path) {
   const long if_sph = iv[0], if_gfl = iv[1], cases = iv[2];
   if (if_sph != 1 || if_gfl != 0)
     return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": only if_sph = 1, if_gfl = 0 runs are read");
-  if (cases != 0 && cases != 1 && cases != 2)
+  if (!(cases == 0 || cases == 1 || cases == 2 || cases == 3 || cases == 4 || cases == 20 || cases == 21 ||
+        cases == 31 || cases == 32))
     return ifail(in, GFSPH_ERR_UNSUPPORTED,
                  path + ": ic_cases_mat " + std::to_string(cases) +
-                     " (0, 1: one variable at a time, 2: all combinations are read; the sampled sets are not)");
+                     " (0-4, 20, 21, 31, 32 are read; 5 and 6 draw per geological zone of the terrain)");
   in->cases.mode = (int)cases;
   RD(R.text());
   RD(R.ints(iv, 2));
@@ -1463,9 +1701,10 @@ int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out)
           px[2 * i + 1] = px[2 * i + 1] + dy;
         }
       } else if (idx == 401) {
-        return ifail(in, GFSPH_ERR_UNSUPPORTED, "case variable 401 (initial pore-pressure factor)");
+        // the initial pore pressure's factor, applied to the created context
+        // below (gfsph_batch_scale_pwp)
       } else if (idx >= 1 && idx <= 110) {
-        if (T.mode == 2 && idx == 2 && cv <= 1.01) {  // porosity given: mixture density (6:1216-1226)
+        if (T.mode >= 2 && idx == 2 && cv <= 1.01) {  // porosity given: mixture density (6:1216-1226)
           if (!(c[17] >= 1.e-4))
             return ifail(in, GFSPH_ERR_ARG, "a porosity for const(2) needs the solid and water densities const(17, 18)");
           c[2] = (double)(float)((1. - cv) * c[17] + cv * c[18]);
@@ -1498,6 +1737,16 @@ int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out)
     return rc;
   };
   if (rc) return bail("gfsph_batch_create");
+  // variable 401 (6:1453-1479, pore pressure only): u and Uaux of problem p
+  // times its value, one variable after another; in mode 1 only where it is
+  // the problem's varied variable
+  for (int ic = 0; ic < T.ncases && in->P.icpwp != 0; ic++) {
+    if ((int)(T.C[ic][0] + 0.01) != 401) continue;
+    std::vector<double> fac(R, 1.0);
+    for (int p = 0; p < R; p++)
+      if (!(T.mode == 1 && ic != (int)T.OC[p][0] - 1)) fac[p] = T.OC[p][3 + ic];
+    if ((rc = gfsph_batch_scale_pwp(*out, fac.data()))) return bail("gfsph_batch_scale_pwp");
+  }
   if (in->run.present && in->run.adapt_dt && (rc = gfsph_set_adaptive_dt(*out, in->run.adapt_dt)))
     return bail("gfsph_set_adaptive_dt");
   gfsph_case_ctl ctl;
@@ -1506,10 +1755,20 @@ int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out)
   ctl.dtimestop = T.dtimestop;
   ctl.dt = in->run.dt;
   ctl.time_end = in->run.time_end > 0 ? in->run.time_end : 1.e30;
-  ctl.time_plot = T.mode == 2 ? in->run.plot_step * in->run.dt : 0.0;
-  ctl.pixels = pixels && T.mode == 2 ? 1 : 0;
+  ctl.time_plot = T.mode >= 2 ? in->run.plot_step * in->run.dt : 0.0;
+  ctl.pixels = pixels && T.mode >= 2 ? 1 : 0;
   ctl.pad_ = 0;
   if ((rc = gfsph_batch_cases(*out, &ctl))) return bail("gfsph_batch_cases");
+  return GFSPH_OK;
+}
+
+int gfsph_inputs_resample(gfsph_inputs *in, uint64_t seed) {
+  if (!in) return GFSPH_ERR_ARG;
+  CaseTable &T = in->cases;
+  if (!(T.mode == 3 || T.mode == 4 || T.mode == 31 || T.mode == 32))
+    return ifail(in, GFSPH_ERR_ARG, "MASTER.dat holds no sampled case set (ic_cases_mat 3, 4, 31, 32)");
+  T.seed = seed;
+  mc_fill(T);
   return GFSPH_OK;
 }
 

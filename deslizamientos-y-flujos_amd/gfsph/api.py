@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_inputs_create", "gfsph_inputs_error", "gfsph_inputs_destroy", "gfsph_write_netcdf",
     "gfsph_batch_cases", "gfsph_batch_status", "gfsph_inputs_cases", "gfsph_inputs_batch_create",
     "gfsph_get_mass", "gfsph_erosion_read", "gfsph_erosion_set", "gfsph_get_pwp_column", "gfsph_set_pwp_column",
-    "gfsph_neighbour_pairs_noalloc", "gfsph_set_mass0",
+    "gfsph_neighbour_pairs_noalloc", "gfsph_set_mass0", "gfsph_batch_scale_pwp", "gfsph_inputs_resample",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -127,6 +127,8 @@ def load_library(path: str | None = None):
     L.gfsph_batch_status.argtypes = [vp, ip, ip, ip]
     L.gfsph_get_mass.argtypes = [vp, dp]
     L.gfsph_set_mass0.argtypes = [vp, dp]
+    L.gfsph_batch_scale_pwp.argtypes = [vp, dp]
+    L.gfsph_inputs_resample.argtypes = [vp, C.c_uint64]
     L.gfsph_get_pwp_column.argtypes = [vp, dp]
     L.gfsph_set_pwp_column.argtypes = [vp, dp]
     L.gfsph_erosion_read.argtypes = [vp, C.c_int32, dp, dp, dp]
@@ -296,6 +298,12 @@ class SPHModel:
         m = np.zeros(self.n)
         self._check(self.lib.gfsph_get_mass(self.ctx, _dp(m)))
         return m
+
+    def scale_pwp(self, factor) -> None:
+        """Initial pore pressure (u and the Uaux column) of every realisation
+        times its factor: case variable 401 of Input_SW."""
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(factor, dtype=np.float64), (self.R,)))
+        self._check(self.lib.gfsph_batch_scale_pwp(self.ctx, _dp(f)))
 
     def set_mass0(self, mass0) -> None:
         """The masses before any erosion (mass0), read by the Bingham-erosion

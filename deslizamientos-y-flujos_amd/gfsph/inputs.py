@@ -118,6 +118,11 @@ class Inputs:
         return {"mode": mode.value, "ncases": nc.value, "table": table, "vstop": stop[0], "timestop": stop[1],
                 "dtimestop": stop[2]}
 
+    def resample(self, seed: int) -> None:
+        """Draws a sampled case set (ic_cases_mat 3, 4, 31, 32) again from
+        another seed (the sets are drawn from seed 1 when read)."""
+        self._check(self.lib.gfsph_inputs_resample(self.h, C.c_uint64(int(seed))))
+
     def batch(self, pixels: bool = False) -> BatchModel:
         """gfsph_inputs_batch_create: every problem of the case table as one
         realisation, under the reference's per-problem loop (case control)."""

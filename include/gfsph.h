@@ -310,6 +310,13 @@ int gfsph_batch_cases(gfsph_ctx *ctx, const gfsph_case_ctl *ctl);
  * number stopped.  Any pointer may be NULL.  Synchronises. */
 int gfsph_batch_status(gfsph_ctx *ctx, int32_t *stopped, int32_t *nsteps, int32_t *nstopped);
 
+/* Multiplies the initial pore pressure of realisation r (u and the whole
+ * Uaux column of every particle) by factor[r]: Input_SW's case variable 401
+ * ("6 SPH_SW_2019.f90":1453-1479), applied after the context is created
+ * (gfsph_inputs_batch_create calls it for each 401 column).  GFSPH_ERR_ARG
+ * with icpwp = 0, which the reference's branch does not reach. */
+int gfsph_batch_scale_pwp(gfsph_ctx *ctx, const double *factor);
+
 /* Sets time_sph of every realisation, e.g. to time_sph_restart
  * ("6 SPH_SW_2019.f90":3302) when a run continues from restart.dat. */
 int gfsph_set_time(gfsph_ctx *ctx, double time);
@@ -372,21 +379,35 @@ int gfsph_inputs_particles(gfsph_inputs *in, double *x, double *v, double *h, do
 /* gfsph_create on the inputs (init_input = 1), then ic_adapt_dt_sph of
  * MASTER.dat and time_sph_restart. */
 int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out);
-/* The case table of MASTER.dat (ic_cases_mat 1: one variable at a time, 2:
- * every combination; Driver_Input, "8 SPH_FEM_Driver_2019.f90":475-639):
- * table is OC_mat, [nproblems][3 + ncases + 2] (case, value index, variable
- * index, the value of every case variable, the y shift of variable 521);
- * stop3 = vstop_cases, timestop_cases, dtimestop_cases.  Any pointer may be
- * NULL (query the sizes first). */
+/* The case table of MASTER.dat (Driver_Input, "8 SPH_FEM_Driver_2019.f90":
+ * 475-1236): ic_cases_mat 1 one variable at a time, 2 every combination, 3
+ * n_MCarlo_problems rows drawn from the combinations, 4 / 31 / 32 values
+ * drawn from each variable's law (constant, triangular, normal, log-normal,
+ * histogram: GetMCarlo_Cont_values / GetMCarlo_Dscrt_values, 8:1282-1439),
+ * 20 / 21 the 2 ncases + 1 FOSM sets (means, then every variable at mean +-
+ * sigma/10 as the reference fills them, 8:974-1051).  table is OC_mat,
+ * [nproblems][3 + ncases + 2] (case, value index, variable index, the value
+ * of every case variable, the y shift of variable 521); stop3 =
+ * vstop_cases, timestop_cases, dtimestop_cases.  Any pointer may be NULL
+ * (query the sizes first).  The reference seeds its draws from the system
+ * clock (8:1306-1312); here they come from a seed (1 after reading, see
+ * gfsph_inputs_resample): variable k's uniforms are SplitMix64 from state
+ * seed + 0x9E3779B97F4A7C15 (k + 1), output u giving ((u >> 11) + 0.5) 2^-53;
+ * mode 3 draws its rows from stream k = 0.  Modes 5 and 6 and n_GZones > 1
+ * (draws per geological zone of the terrain) are rejected. */
 int gfsph_inputs_cases(gfsph_inputs *in, int32_t *mode, int32_t *ncases, int32_t *nproblems, double *table,
                        double *stop3);
+/* Draws a sampled set's problems again from another seed (ic_cases_mat 3, 4,
+ * 31, 32; GFSPH_ERR_ARG for the deterministic tables). */
+int gfsph_inputs_resample(gfsph_inputs *in, uint64_t seed);
 /* All problems of the case table as one batch (replaces the problem loop of
  * the reference's main program, 8:60-130): realisation p is the release and
  * const(:) as Input_SW changes them for problem p (6:1040-1232: variables 501
- * depth factor, 511 expansion, 521 shift, const indices; 401 is rejected),
- * with ic_adapt_dt_sph and the case control (gfsph_batch_cases) of the loop
- * the reference runs for each problem: Driver_Flow for mode 1, Driver_Flow_23
- * (steps cut at every plot_step*dt; pixel statistics if pixels) for mode 2. */
+ * depth factor, 511 expansion, 521 shift, const indices; 401 the initial
+ * pore pressure's factor, 6:1453-1479), with ic_adapt_dt_sph and the case
+ * control (gfsph_batch_cases) of the loop the reference runs for each
+ * problem: Driver_Flow for mode 1, Driver_Flow_23 (steps cut at every
+ * plot_step*dt; pixel statistics if pixels) for the others. */
 int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out);
 const char *gfsph_inputs_error(gfsph_inputs *in);
 void gfsph_inputs_destroy(gfsph_inputs *in);

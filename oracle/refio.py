@@ -136,12 +136,27 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
         f.write("\n".join(rows) + "\n")
 
 
+def _law_lines(law) -> list:
+    """One sampled variable: (law, params) for a continuous law (8:737-745) or
+    (100, nodes, counts) for a histogram (8:1391-1406)."""
+    if law[0] == 100:
+        _, xh, ph = law
+        return ["probab_law", "100", "histogram np_h", str(len(xh)), "nodes", " ".join(_r(v) for v in xh),
+                "counts", " ".join(_r(v) for v in ph)]
+    prm = list(law[1]) + [0.0] * (10 - len(law[1]))
+    return ["probab_law", str(int(law[0])), "params", " ".join(_r(v) for v in prm)]
+
+
 def write_master(d: str, name: str, dt: float, adapt: int = 0, cases: int = 0, case_rows=(), stop=(0.0, 1e9, 0.0),
-                 lists=(), dt_driver: float | None = None, time_end: float = 1.e9, plot_step: int = 1000000) -> None:
+                 lists=(), dt_driver: float | None = None, time_end: float = 1.e9, plot_step: int = 1000000,
+                 nmc: int = 0, laws=(), fosm=None, records=()) -> None:
     """<name>.MASTER.dat in Driver_Input's layout ("8 SPH_FEM_Driver_2019.f90":
-    229-357, 475-639): SPH only, RK4, dt_sph and ic_adapt_dt_sph, optionally a
-    case table (ic_cases_mat 1 or 2: rows of Ccases_mat, statistic lists as
-    (values, probabilities) pairs, the stop thresholds)."""
+    229-357, 475-1236): SPH only, RK4, dt_sph and ic_adapt_dt_sph, optionally a
+    case table: ic_cases_mat 1 or 2 (rows of Ccases_mat, statistic lists as
+    (values, probabilities) pairs, the stop thresholds), 3 (as 2, then nmc
+    rows drawn), 4 / 31 (rows, stop, nmc, one law per variable), 20 / 21
+    (rows, stop, fosm = (means, sigmas)), 32 (stop, ncases, records (variable,
+    law, params) of which the first with a law > 0 are the variables, nmc)."""
     dtd = dt if dt_driver is None else dt_driver
     L = ["1", "run control written for the reference's input path",
          "if_sph if_gfl ic_cases_mat", f"1 0 {int(cases)}",
@@ -151,7 +166,16 @@ def write_master(d: str, name: str, dt: float, adapt: int = 0, cases: int = 0, c
          "print_step save_step plot_step", f"1000000 1000000 {int(plot_step)}",
          "dt_sph ic_adapt", f"{_r(dt)} {int(adapt)}",
          "Ntime curves max pts", "0 0"]
-    if cases:
+    if cases == 32:
+        nvar = sum(1 for r in records if r[1] > 0)
+        L += ["vstop timestop dtimestop", " ".join(_r(v) for v in stop)]
+        L += ["ncases_mat mx_cases ninfo_mat", f"{nvar} {len(records)} 12"]
+        L += ["n_MCarlo_problems n_GZones", f"{int(nmc)} 1"]
+        L += ["var law params"]
+        for var, law, prm in records:
+            prm = list(prm) + [0.0] * (10 - len(prm))
+            L += [f"{int(var)} {int(law)} " + " ".join(_r(v) for v in prm)]
+    elif cases:
         ninfo = max(len(r) for r in case_rows)
         L += ["ncases_mat ninfo_mat", f"{len(case_rows)} {ninfo}", "index val0 nval min max law ..."]
         L += [" ".join(_r(v) for v in list(r) + [0.0] * (ninfo - len(r))) for r in case_rows]
@@ -160,6 +184,16 @@ def write_master(d: str, name: str, dt: float, adapt: int = 0, cases: int = 0, c
             for vals, probs in lists:
                 L += [" ".join(_r(v) for v in vals), " ".join(_r(v) for v in probs)]
         L += ["vstop timestop dtimestop", " ".join(_r(v) for v in stop)]
+        if cases == 3:
+            L += ["n_MCarlo_problems", str(int(nmc))]
+        elif cases in (4, 31):
+            L += ["n_MCarlo_problems" + (" n_GZones" if cases == 31 else ""),
+                  f"{int(nmc)}" + (" 1" if cases == 31 else "")]
+            for law in laws:
+                L += _law_lines(law)
+        elif cases in (20, 21):
+            mu, sg = fosm
+            L += ["means sigmas", " ".join(_r(v) for v in mu), " ".join(_r(v) for v in sg)]
     L += ["win", "0"]
     with open(os.path.join(d, name + ".MASTER.dat"), "w") as f:
         f.write("\n".join(L) + "\n")

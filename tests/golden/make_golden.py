@@ -408,12 +408,22 @@ def save_mc_hazard(out_dir=HERE, nsteps=100, plot_every=10):
 # plot times 0.5, 1.0, ..., where time_sph equals a multiple of 0.5 only to
 # within rounding, so a stop time on a plot time would decide the stop by
 # the last bit of time_sph.
+# t20: the FOSM sets (ic_cases_mat 20, 8:974-1051) of mu and xi; t401: the
+# initial pore pressure's factor (case variable 401, 6:1453-1479) with
+# tan(phi) over the frictional pore-pressure fixture.
 CASE_TABLES = {
     "t2": ([[9, 0.2, 3, 0.12, 0.28, 1], [501, 1.0, 2, 0.8, 1.2, 1]],
            dict(adapt=2, cases=2, stop=(0.05, 2.53, 0.0), dt_driver=0.5, plot_step=1, time_end=1e9)),
     "t1": ([[9, 0.2, 2, 0.15, 0.3, 1, 0, 0], [3, 500.0, 2, 300.0, 900.0, 1, 0, 0]],
            dict(adapt=1, cases=1, stop=(0.1, 1.7, 0.0), dt_driver=0.05, plot_step=1000000, time_end=1e9)),
+    "t20": ([[9, 0.2, 1, 0.0, 0.0, 1], [3, 500.0, 1, 0.0, 0.0, 1]],
+            dict(adapt=2, cases=20, stop=(0.05, 2.53, 0.0), dt_driver=0.5, plot_step=1, time_end=1e9,
+                 fosm=((0.2, 500.0), (0.5, 2000.0)))),
+    "t401": ([[401, 1.0, 2, 0.5, 1.0, 1], [9, 0.5, 2, 0.45, 0.6, 1]],
+             dict(adapt=0, cases=2, stop=(0.05, 1.53, 0.0), dt_driver=0.5, plot_step=1, time_end=1e9)),
 }
+# the fixture whose release and terrain a table runs on (g_voellmy_plane if absent)
+CASE_BASE = {"t401": "g_frictional_pwp"}
 
 
 def save_case_driver(out_dir=HERE):
@@ -424,9 +434,9 @@ def save_case_driver(out_dir=HERE):
     stop_case_SPH call (oracle/ref_main_probe.f90)."""
     sys.path.insert(0, os.path.dirname(HERE))
     import goldens
-    case, _ = goldens.case_of("g_voellmy_plane")
     arr = {}
     for tname, (rows, opts) in CASE_TABLES.items():
+        case, _ = goldens.case_of(CASE_BASE.get(tname, "g_voellmy_plane"))
         tr = refio.run_reference_main(case, case_rows=rows, **opts)
         probs = sorted(set(r["problem"] for r in tr))
         arr[f"{tname}_nprob"] = np.array(len(probs))
