@@ -136,7 +136,6 @@ struct NewSet {
   int *flag;
   double *col;  // icpwp = 2: columns after the step [npwp][n]
   double *dh;   // icpwp = 2: the step's depth change Drho_RK, read by the column update
-  double4 *vhm;  // (vx, vy, h, mass) after the step, one record for the XSPH gathers
 };
 
 // The state after a step: positions, depths, smoothing lengths, pore
@@ -1376,7 +1375,6 @@ __device__ __forceinline__ void finish_particle(const DevParams &P, const DevDem
   o.pos[i] = x;
   o.vel[i] = v;
   o.h[i] = hnew;
-  o.vhm[i] = make_double4(v.x, v.y, hnew, mnew);
   o.hsml[i] = hs;
   o.u[i] = u;
   o.uaux[i] = uaux_new;
@@ -2347,33 +2345,7 @@ __global__ XSPH_LB void k_xsph_m(int n, DevParams P, Cells cl,
         hy[hh] = hy[hh] - c * (vi.y - vj[u].y);
       }
     };
-    // equal smoothing lengths, every particle averaged: a neighbour's
-    // position is the first half of its record, its new velocity, depth and
-    // mass one 32-byte record of the step output
-    auto body_v = [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
-      double2 xy[NB_BATCH];
-      double4 r[NB_BATCH];
-      const double2 *nxy = reinterpret_cast<const double2 *>(s.nb);
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        xy[u] = nxy[2 * (long)js[u]];
-        r[u] = o.vhm[js[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < NB_BATCH; u++) {
-        double dx, dy, dr, mh;
-        pair_geom(pi_, hi, make_double4(xy[u].x, xy[u].y, hi, r[u].w), dx, dy, dr, mh);
-        const double w = sph_w_t<SKF, true>(P.kc, P.pi, dr, mh, ihi);
-        const double mj = ok[u] ? r[u].w : 0.0;
-        const double c = (2 * mj) * w * rcp_nr(rhoi + r[u].z);
-        const int hh = TM == 1 ? (u & 1) : 0;
-        hx[hh] = hx[hh] - c * (vi.x - r[u].x);
-        hy[hh] = hy[hh] - c * (vi.y - r[u].y);
-      }
-    };
-    if (!skip0 && g.uniform_h)
-      word_batches(cs, w, i, body_v);
-    else if (g.uniform_h)
+    if (g.uniform_h)
       word_batches(cs, w, i, [&](const int(&js)[NB_BATCH], const bool(&ok)[NB_BATCH]) {
         body(std::true_type{}, js, ok);
       });
@@ -4586,7 +4558,7 @@ static int alloc_sets(gfsph_ctx *c, int n) {
   NewSet &o = c->nw;
   if ((rc = dalloc(c, &o.pos, n)) || (rc = dalloc(c, &o.vel, n)) || (rc = dalloc(c, &o.h, n)) ||
       (rc = dalloc(c, &o.hsml, n)) || (rc = dalloc(c, &o.u, n)) || (rc = dalloc(c, &o.uaux, n)) ||
-      (rc = dalloc(c, &o.flag, n)) || (rc = dalloc(c, &o.vhm, n)))
+      (rc = dalloc(c, &o.flag, n)))
     return rc;
   if (c->P.npwp) {  // pore-pressure columns, node-major, cleared
     const size_t nc = (size_t)c->P.npwp * n;
