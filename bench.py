@@ -56,7 +56,19 @@ B_ALG = {"neighbour_build": 64.0, "k_density": 44.0, "k_pforce": 56.0, "k_rk": 1
 # this workload (200k particles, the steps of the developed-flow window after
 # 1,000 untimed steps), each scaled by what the counter reads for a device
 # copy of known size (fetch x2.0, write x1.0)
-MEASURED_TRAFFIC_200K = {"k_density": 32.69e6, "k_pforce": 33.33e6, "k_xsph": 38.63e6, "k_rk": 50.00e6}
+MEASURED_TRAFFIC_200K = {"k_density": 39.10e6, "k_pforce": 33.11e6, "k_xsph": 38.64e6, "k_rk": 49.97e6}
+
+# What bounds the gathers instead of HBM, per CU and cycle from rocprofv3 --pmc
+# on the 256-realisation MC batch (7.37M particles; profiles/gather_counters_mc256.md):
+# VALU issue (one wave64 instruction per CU per cycle at most), the load-data
+# return unit TD, L1 tag lookups (one per cycle at most).  Static figures of
+# that profile, not measured by this run.
+GATHER_LIMITS_MC = {
+    "source": "profiles/gather_counters_mc256.md", "workload": "MC batch, 256 realisations, 7,366,052 particles",
+    "k_density": {"valu_busy": 0.73, "td_busy": 0.96, "l1_lookups_per_clk_per_cu": 0.77},
+    "k_pforce": {"valu_busy": 0.87, "td_busy": 0.94, "l1_lookups_per_clk_per_cu": 0.86},
+    "k_xsph": {"valu_busy": 0.70, "td_busy": 0.96, "l1_lookups_per_clk_per_cu": 0.94},
+}
 
 
 def cpu_info() -> dict:
@@ -186,7 +198,8 @@ def main() -> int:
                 "algorithmic_bytes_per_launch": B_ALG[dom] * n,
                 "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                 "window": f"steps {develop + args.warmup + args.steps + 1}..{develop + args.warmup + args.steps + prof_steps}",
-                "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+                "kernel_ms": {k: round(v, 4) for k, v in prof.items()},
+                "gather_limits": GATHER_LIMITS_MC}
 
     cpu = None
     if not args.no_cpu_baseline:
