@@ -75,6 +75,19 @@ struct DevParams {
   // (K/tau_y)^(1/m), the run-constant factor of a_hat for m = 1 and 2
   int hb_m, pad_h;
   double hb_A;
+  // terrain zones of geometry 1 read the realisation's current Z at
+  // DevDem::zcur + zoff (topol(1), which erosion lowers)
+  long zoff;
+};
+
+// One terrain zone (basal_type, basal_law, basal_props of 2:516-576): geom =
+// int(props(1) + 0.1) as Which_Topo_Zone takes it, egeom = int(props(1) +
+// 0.01) as Get_Erosion_Rate_SW does; cth, sth: cos and sin of the geometry-5
+// angle, from the host's libm.
+struct DevZone {
+  int type, law, geom, egeom;
+  double p[16];
+  double cth, sth;
 };
 
 struct DevDem {
@@ -82,6 +95,15 @@ struct DevDem {
   double xmin, ymin, dx, dy, xmax, ymax;
   const double2 *zxy;  // [npoig] (Zx, Zy)
   const double *top;   // [npoig][6] top4..top9, or null
+  // terrain zones: zkinds bit 0 = a friction zone exists, bit 1 = an
+  // erosion zone; znodes: node-list bits of geometry-101 zones; zcur, inm:
+  // Z and poig_in_model for geometry 1 (null when no zone needs them)
+  int nzones, zkinds;
+  const DevZone *zones;
+  const int *znodes;
+  const double *zcur;
+  const int *inm;
+  const double *zavail;  // topol(15): available erosion depth (ErosGrid::avail), per realisation at zoff
 };
 
 struct GridInfo {
@@ -368,6 +390,70 @@ __device__ __forceinline__ void get_z_topo(const DevDem &d, double xx, double yy
   }
 }
 
+// Z at (xx, yy) as Get_Z_Topo interpolates topol(1) (2:2395-2470), with
+// the weighting by poig_in_model at the edge of the model (2:2455-2462);
+// zc: the realisation's current Z.
+__device__ __forceinline__ double z_topo(const DevDem &d, const double *zc, double xx, double yy) {
+  int ipx = (int)((xx - d.xmin) / d.dx + 1);
+  int ipy = (int)((yy - d.ymin) / d.dy + 1);
+  if (ipx == d.nx && fabs(d.xmax - xx) <= 1.e-3) ipx = ipx - 1;
+  if (ipy == d.ny && fabs(d.ymax - yy) <= 1.e-3) ipy = ipy - 1;
+  ipx = ipx < 1 ? 1 : (ipx > d.nx - 1 ? d.nx - 1 : ipx);
+  ipy = ipy < 1 ? 1 : (ipy > d.ny - 1 ? d.ny - 1 : ipy);
+  const long n1 = (long)(ipy - 1) * d.nx + (ipx - 1);
+  const long n2 = n1 + 1, n4 = n1 + d.nx, n3 = n4 + 1;
+  const double xref = d.xmin + (ipx - 1) * d.dx;
+  const double yref = d.ymin + (ipy - 1) * d.dy;
+  const double xi = (xx - xref) / d.dx, eta = (yy - yref) / d.dy;
+  const double sh1 = (1. - xi) * (1. - eta), sh2 = xi * (1. - eta), sh3 = xi * eta, sh4 = (1. - xi) * eta;
+  double Zp = sh1 * zc[n1] + sh2 * zc[n2] + sh3 * zc[n3] + sh4 * zc[n4];
+  if (d.inm) {
+    const int i1 = d.inm[n1], i2 = d.inm[n2], i3 = d.inm[n3], i4 = d.inm[n4];
+    if (i1 + i2 + i3 + i4 < 4) {
+      Zp = i1 * sh1 * zc[n1] + i2 * sh2 * zc[n2] + i3 * sh3 * zc[n3] + i4 * sh4 * zc[n4];
+      const double den = i1 * sh1 + i2 * sh2 + i3 * sh3 + i4 * sh4;
+      if (den > 1.e-3) Zp = Zp / den;
+    }
+  }
+  return Zp;
+}
+
+// Which_Topo_Zone (2:2635-2808): the first zone of basal type btype that
+// holds (xx, yy), or -1.  Geometry 101 takes the cell of the point (the
+// corner node's index as Get_Z_Topo forms it, nodes n1, n1+1, n1+1+nx,
+// n1+nx) and is inside when one of the four nodes is in the zone's list.
+__device__ __noinline__ int which_zone(const DevDem &d, const DevParams &P, double xx, double yy, int btype) {
+  for (int z = 0; z < d.nzones; z++) {
+    const DevZone &Z = d.zones[z];
+    if (Z.type != btype) continue;
+    bool in;
+    if (Z.geom <= 3) {
+      const double Zp = Z.geom == 1 ? z_topo(d, d.zcur + P.zoff, xx, yy) : (Z.geom == 2 ? xx : yy);
+      in = (Z.p[2] - Zp) * (Zp - Z.p[1]) >= 0;
+    } else if (Z.geom == 4) {
+      in = (Z.p[2] - xx) * (xx - Z.p[1]) >= 0 && (Z.p[4] - yy) * (yy - Z.p[3]) >= 0;
+    } else if (Z.geom == 5) {
+      const double Rvx = xx - Z.p[1], Rvy = yy - Z.p[2];
+      const double Ra = Z.cth * Rvx + Z.sth * Rvy;
+      const double Rb = (-Z.sth) * Rvx + Z.cth * Rvy;
+      in = Ra >= 0.0 && Ra <= Z.p[3] && Rb >= 0.0 && Rb <= Z.p[4];
+    } else {  // 101
+      int ipx = (int)((xx - d.xmin) / d.dx + 1);
+      int ipy = (int)((yy - d.ymin) / d.dy + 1);
+      if (ipx == d.nx && fabs(d.xmax - xx) <= 1.e-3) ipx = ipx - 1;
+      if (ipy == d.ny && fabs(d.ymax - yy) <= 1.e-3) ipy = ipy - 1;
+      const long np = (long)d.nx * d.ny;
+      const long n1 = (long)(ipy - 1) * d.nx + (ipx - 1);
+      const long nn[4] = {n1, n1 + 1, n1 + 1 + d.nx, n1 + d.nx};
+      in = false;
+      for (int k = 0; k < 4; k++)
+        if (nn[k] >= 0 && nn[k] < np && ((d.znodes[nn[k]] >> z) & 1)) in = true;
+    }
+    if (in) return z;
+  }
+  return -1;
+}
+
 // ---------------------------------------------------------------------------
 // a6 + a8. ExtForces_SW(0) body for one particle (6:8532-8657) with
 // Get_Grav_Ext_SW (6:9241-9349) and basal_friction_Ext_SW (6:9355-9901).
@@ -427,6 +513,58 @@ __device__ __forceinline__ double erosion_rate(const DevParams &P, double h, dou
       const double Tauf = P.eros_c23 + P.dens_bar_eff * P.cgra * h * (1. - P.eros_c25) * P.eros_c24;
       if (Taub > Tauf && vmod > 1.e-3) er = P.keros * (Taub - Tauf) / P.dens_bar_eff;
     }
+  }
+  return er;
+}
+
+// Erosion zones of the terrain (6:10407-10503): inside an erosion zone its
+// own law replaces er -- note the zone laws' own forms: law 2 takes the
+// equilibrium angle as atan2(xden, xnum), law 3 the basal stress dens h
+// |frict| against c + dens g h (1 - beta) tan(phi_b); a zone of geometry 101
+// or 102 erodes only where its cell's nodes have depth available
+// (topol(15), the realisation's current values).
+__device__ __noinline__ double zone_erosion_rate(const DevParams &P, const DevDem &dem, double xx, double yy,
+                                                 double er, double h, double vmod, double vxx, double vyy,
+                                                 double Zpx, double Zpy, double tanfi, double frictx, double fricty) {
+  const int z = which_zone(dem, P, xx, yy, 2);
+  if (z < 0) return er;
+  const DevZone &Z = dem.zones[z];
+  if (Z.law == 0) {
+    er = 0.0;
+  } else if (Z.law == 1) {
+    er = Z.p[6] * h * vmod;
+  } else if (Z.law == 2) {
+    const double denss = P.denss, densw = P.densw_e;
+    const double vol_conc = (P.dens - densw) / (denss - densw);
+    er = 0.0;
+    if (vmod > 1.e-5) {  // (at rest the reference stops: 6:10445-10447)
+      const double cosa = vxx / vmod, sina = vyy / vmod;
+      const double gradZv = Zpx * cosa + Zpy * sina;
+      const double theta = atan2(-gradZv, 1.);
+      double xnum = (denss - densw) * vol_conc;
+      const double xden = xnum + densw;
+      xnum = xnum * tanfi;
+      const double thetae = atan2(xden, xnum);
+      if (theta > thetae) er = Z.p[6] * Z.p[7] * vmod * tan(theta - thetae);
+    }
+  } else if (Z.law == 3) {
+    er = 0.0;
+    if (vmod > 1.e-5) {
+      const double Taub = P.dens * h * sqrt(frictx * frictx + fricty * fricty);
+      const double Tauf = Z.p[7] + P.dens * P.cgra * h * (1. - Z.p[9]) * Z.p[8];
+      if (Taub > Tauf && vmod > 1.e-3) er = Z.p[6] * (Taub - Tauf) / P.dens;
+    }
+  }
+  if (Z.egeom == 101 || Z.egeom == 102) {
+    const int ipx = (int)((xx - dem.xmin) / dem.dx + 1);
+    const int ipy = (int)((yy - dem.ymin) / dem.dy + 1);
+    const long np = (long)dem.nx * dem.ny;
+    const long n1 = (long)(ipy - 1) * dem.nx + (ipx - 1);
+    const long nn[4] = {n1, n1 + 1, n1 + dem.nx, n1 + dem.nx + 1};
+    const double *av = dem.zavail + P.zoff;
+    double auxt = 0.0;
+    for (int k = 0; k < 4; k++) auxt = auxt + ((nn[k] >= 0 && nn[k] < np) ? av[nn[k]] : 0.0);
+    if (auxt < 1.e-6) er = 0.0;
   }
   return er;
 }
@@ -618,6 +756,10 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       if (factpw > P.fact_pwp_max) factpw = P.fact_pwp_max;
       tanfi = tanfi * factpw;
     }
+    if (dem.zkinds & 1) {  // friction zones of the terrain (6:9697-9702, 10515-10545)
+      const int z = which_zone(dem, P, xx, yy, 1);
+      if (z >= 0 && dem.zones[z].p[6] >= 0) tanfi = dem.zones[z].p[6];
+    }
     tanfi_m = tanfi;
     double gtanfi = (gravz + cv2byR) * tanfi * P.fact_dens;
     if (gtanfi < 0.0) gtanfi = 0.0;
@@ -707,7 +849,8 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
   exo = exx - frictx / h;
   eyo = exy - fricty / h;
   if (EROS && P.eros_on) {
-    const double er = erosion_rate(P, h, vmod, vxx, vyy, tp.Zx, tp.Zy, tanfi_m, frictx, fricty);
+    double er = erosion_rate(P, h, vmod, vxx, vyy, tp.Zx, tp.Zy, tanfi_m, frictx, fricty);
+    if (dem.zkinds & 2) er = zone_erosion_rate(P, dem, xx, yy, er, h, vmod, vxx, vyy, tp.Zx, tp.Zy, tanfi_m, frictx, fricty);
     if (!end) {
       exo = exo - er * vx1 * hinv * P.slip_fact;
       eyo = eyo - er * vy1 * hinv * P.slip_fact;
@@ -1273,13 +1416,15 @@ __device__ __forceinline__ void erosion_scatter(const DevDem &d, double *aux, do
 }
 
 // initial erosion state: realisation 0's Z (uploaded) to every realisation
-__global__ void k_eros_init(long total, long np, const double *avail0, ErosGrid g) {
+__global__ void k_eros_init(long total, long np, const double *avail0, const double *zone_avail, const int *lawon,
+                            ErosGrid g) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total) return;
   const long r = k / np;
   if (r > 0) g.z[k] = g.z[k - r * np];
   g.eroded[k] = 0.0;
-  g.avail[k] = avail0[r];
+  // const(48) or 999 with a surface law; else the terrain's erodible depths
+  g.avail[k] = (lawon[r] || !zone_avail) ? avail0[r] : zone_avail[k - r * np];
   g.aux[2 * k] = 0.0;
   g.aux[2 * k + 1] = 0.0;
 }
@@ -4217,6 +4362,7 @@ struct gfsph_ctx {
   int *bad = nullptr;
   double2 *zxy = nullptr;
   double *top = nullptr;
+  int has_zone1 = 0;  // a terrain zone of geometry 1 reads the current Z
   double *raster5 = nullptr, *mc_acc = nullptr;
   int *mc_cnt = nullptr;
   BinghamState bs{};
@@ -5019,6 +5165,52 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
     *out = c;
     return rc;
   }
+  // terrain zones (ic_topo_props = 10): friction zones and erosion zones
+  int zkinds = 0;
+  if (dem->nzones < 0 || dem->nzones > 32 || (dem->nzones > 0 && !dem->zones)) {
+    c->err = "nzones must be 0..32 with a zone array";
+    *out = c;
+    return GFSPH_ERR_ARG;
+  }
+  for (int z = 0; z < dem->nzones; z++) {
+    const gfsph_zone &Z = dem->zones[z];
+    const int geom = (int)(Z.props[0] + 0.1);
+    std::string why;
+    if (Z.type != 1 && Z.type != 2)
+      why = "basal type " + std::to_string(Z.type) +
+            " (pore-pressure boundary, flux and mass zones are outside the implemented path)";
+    else if (!(geom >= 1 && geom <= 5) && geom != 101)
+      why = "geometry " + std::to_string(geom) + " (1-5 and 101 are implemented)";
+    else if (geom == 101 && !dem->zone_nodes)
+      why = "geometry 101 needs zone_nodes";
+    else if (Z.type == 2 && (Z.law < 0 || Z.law > 3))
+      why = "erosion law " + std::to_string(Z.law);
+    if (why.empty() && Z.type == 2 && (Z.law == 2 || Z.law == 3))
+      for (int r = 0; r < R && why.empty(); r++)
+        if (Ph[r].denss <= 1.e-3 || Ph[r].densw_e <= 1.e-3)
+          why = "erosion law " + std::to_string(Z.law) + " needs const(17) and const(18) (the reference stops here)";
+    if (why.empty() && Z.type == 2 && Z.law == 2)
+      for (int r = 0; r < R && why.empty(); r++)
+        if (!Ph[r].if_frictional) why = "erosion law 2 reads the frictional law's tan(phi): it needs a frictional nfric";
+    if (!why.empty()) {
+      c->err = "terrain zone " + std::to_string(z + 1) + ": " + why;
+      *out = c;
+      return why.find("implemented path") != std::string::npos || why.find("are implemented") != std::string::npos
+                 ? GFSPH_ERR_UNSUPPORTED
+                 : GFSPH_ERR_ARG;
+    }
+    zkinds |= Z.type == 1 ? 1 : 2;
+    if (geom == 1) c->has_zone1 = 1;
+  }
+  if (zkinds & 2)
+    for (int r = 0; r < R; r++) {
+      if (Ph[r].nfric == 4) {
+        c->err = "erosion with the Bingham law (nfric = 4) is outside the implemented path";
+        *out = c;
+        return GFSPH_ERR_UNSUPPORTED;
+      }
+      Ph[r].eros_on = 1;  // ic_erosion = 1 with basal erosion zones (6:1288-1290)
+    }
   for (int r = 0; r < R; r++) {
     Ph[r].eros_single = (counts[r] == 1);
     c->eros = c->eros || Ph[r].eros_on;
@@ -5096,6 +5288,48 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
   }
   c->dem.zxy = c->zxy;
   c->dem.top = c->top;
+  if (zkinds) {
+    std::vector<DevZone> zh(dem->nzones);
+    for (int z = 0; z < dem->nzones; z++) {
+      const gfsph_zone &Z = dem->zones[z];
+      DevZone &d = zh[z];
+      d.type = Z.type;
+      d.law = Z.law;
+      d.geom = (int)(Z.props[0] + 0.1);
+      d.egeom = (int)(Z.props[0] + 0.01);
+      for (int k = 0; k < 16; k++) d.p[k] = Z.props[k];
+      // theta = thetadeg pi/180 with pi = 4 atan(1) (2:2682, 2:2716)
+      const double theta = Z.props[5] * (4. * atan(1.0)) / 180.;
+      d.cth = cos(theta);
+      d.sth = sin(theta);
+    }
+    DevZone *zd;
+    if ((rc = dalloc(c, &zd, dem->nzones))) return rc;
+    HIPCHK(c, hipMemcpyAsync(zd, zh.data(), sizeof(DevZone) * dem->nzones, hipMemcpyHostToDevice, st));
+    c->dem.zones = zd;
+    c->dem.nzones = dem->nzones;
+    c->dem.zkinds = zkinds;
+    if (dem->zone_nodes) {
+      int *zn;
+      if ((rc = dalloc(c, &zn, np))) return rc;
+      HIPCHK(c, hipMemcpyAsync(zn, dem->zone_nodes, sizeof(int) * np, hipMemcpyHostToDevice, st));
+      c->dem.znodes = zn;
+    }
+    if (c->has_zone1) {
+      if (dem->in_model) {
+        int *im;
+        if ((rc = dalloc(c, &im, np))) return rc;
+        HIPCHK(c, hipMemcpyAsync(im, dem->in_model, sizeof(int) * np, hipMemcpyHostToDevice, st));
+        c->dem.inm = im;
+      }
+      if (!c->eros) {  // the terrain does not change: Z as given
+        double *z0;
+        if ((rc = dalloc(c, &z0, np))) return rc;
+        HIPCHK(c, hipMemcpyAsync(z0, dem->Z, sizeof(double) * np, hipMemcpyHostToDevice, st));
+        c->dem.zcur = z0;
+      }
+    }
+  }
   if (c->eros) {
     // topol(1) = Z, topol(14) = 0, topol(15) = the available depth of each
     // realisation (6:1294-1301), sums cleared
@@ -5111,8 +5345,29 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
     double *avd;
     if ((rc = dalloc(c, &avd, R))) return rc;
     HIPCHK(c, hipMemcpyAsync(avd, av.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
-    k_eros_init<<<(int)((tot + GF_BLOCK - 1) / GF_BLOCK), GF_BLOCK, 0, st>>>((long)tot, np, avd, g);
+    // without a surface erosion law (const(21) = 0) the available depth is
+    // the terrain's topol(15,:) with values below 1e-6 set to 0 (6:1302-1307)
+    const double *ea = nullptr;
+    std::vector<double> eah;
+    if (zkinds & 2) {
+      eah.assign(np, 0.0);
+      if (dem->eros_avail)
+        for (long k = 0; k < np; k++) eah[k] = dem->eros_avail[k] < 1.e-6 ? 0.0 : dem->eros_avail[k];
+      double *ead;
+      if ((rc = dalloc(c, &ead, np))) return rc;
+      HIPCHK(c, hipMemcpyAsync(ead, eah.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
+      ea = ead;
+    }
+    std::vector<int> lawon(R);
+    for (int r = 0; r < R; r++) lawon[r] = Ph[r].eros_law > 0;
+    int *lawd;
+    if ((rc = dalloc(c, &lawd, R))) return rc;
+    HIPCHK(c, hipMemcpyAsync(lawd, lawon.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+    k_eros_init<<<(int)((tot + GF_BLOCK - 1) / GF_BLOCK), GF_BLOCK, 0, st>>>((long)tot, np, avd, ea, lawd, g);
+    c->dem.zavail = g.avail;
+    if (c->has_zone1) c->dem.zcur = g.z;
   }
+  for (int r = 0; r < R; r++) Ph[r].zoff = c->eros ? (long)r * np : 0;
 
   // particles -> set[0] in caller order; the first neighbour build sorts them
   SortedSet &s0 = c->set[0];
@@ -5870,6 +6125,7 @@ static int adjoint_supported(gfsph_ctx *c) {
   if (c->cases) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without case control");
   if (c->eros) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without erosion");
   if (c->staged) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without ic_K0 and ic_cosTheta");
+  if (c->dem.nzones) return fail(c, GFSPH_ERR_UNSUPPORTED, "the adjoint runs without terrain zones");
   for (const DevParams &P : c->Ph) {
     if (P.if_V2byR || P.icpwp != 0 || P.sle != 0 || P.skf != 1 || P.pa_sph != 2 || P.nfric != c->Ph[0].nfric)
       return fail(c, GFSPH_ERR_UNSUPPORTED,

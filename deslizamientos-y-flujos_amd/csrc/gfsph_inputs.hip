@@ -521,6 +521,11 @@ __global__ void k_src_area(int nx, int ny, double *aux) {
 // ---------------------------------------------------------------------------
 struct TopoRaw {
   int ictop = 0, fdiff = 0;
+  int topo_props = 0;               // ic_topo_props
+  std::vector<gfsph_zone> zones;    // ic_topo_props = 10
+  std::vector<double> epts;         // <topo>.eros.pts: [n][3] x, y, depth
+  int eros_type = 0;                // type_Eros
+  double eros_dxs = 0;              // dx_s
   std::vector<double> px, py, pz;  // DTM points (ictop 10-16) or grid nodes (ictop 0)
   double deltx = 0;
   double xw0 = 0, xw1 = 0, yw0 = 0, yw1 = 0;
@@ -654,6 +659,8 @@ struct gfsph_inputs {
   bool built = false;
   std::vector<double> Z, topol;  // [npoig], [npoig][9] (empty when slopes are finite differences)
   std::vector<int32_t> in_model;
+  std::vector<int32_t> zone_nodes;  // node-list bits of geometry-101 zones (empty: none)
+  std::vector<double> eros_avail;   // topol(15,:) after the terrain zones (empty: none)
   int n = 0;
   std::vector<double> x, v, h, mass, hsml, u;
   std::vector<int32_t> itype;
@@ -1160,9 +1167,68 @@ int read_top(gfsph_inputs *in, const std::string &path, TopoRaw &T) {
     return ifail(in, GFSPH_ERR_UNSUPPORTED,
                  path + ": ictop " + std::to_string(T.ictop) + " (0, 10, 11, 15, 16 and +100 are read)");
   }
+  // terrain properties (Topo_main 2:453-576): 1-4 the obsolete topo_props
+  // table, read and unused by the propagation step; 10 the terrain zones
   RD(R.text());
   RD(R.ints(iv, 1));
-  if (iv[0] != 0) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": terrain properties (ic_topo_props != 0)");
+  T.topo_props = (int)iv[0];
+  const int itp = T.topo_props;
+  if (itp == 11 || itp == 12)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": check dams (ic_topo_props " + std::to_string(itp) + ")");
+  if (itp == 100 || itp == 101 || itp == -101)
+    return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": geological zones (ic_topo_props " + std::to_string(itp) + ")");
+  if (itp >= 1 && itp <= 4) {
+    const int nprops = itp == 1 ? 4 : (itp == 4 ? 7 : 6);
+    RD(R.text());
+    RD(R.ints(iv, 1));
+    const long nz = iv[0];
+    RD(R.text());
+    for (long z = 0; z < nz; z++) RD(R.reals(rv, nprops));
+  } else if (itp == 10) {
+    RD(R.text());
+    RD(R.ints(iv, 1));
+    const long nz = iv[0];
+    if (nz < 0 || nz > 32) return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": more than 32 terrain zones");
+    bool eros_pts = false;
+    for (long z = 0; z < nz; z++) {
+      gfsph_zone Z{};
+      RD(R.text());
+      RD(R.ints(iv, 1));
+      Z.type = (int32_t)iv[0];
+      RD(R.text());
+      RD(R.ints(iv, 1));
+      Z.law = (int32_t)iv[0];
+      RD(R.text());
+      RD(R.reals(rv, 6));
+      for (int k = 0; k < 6; k++) Z.props[k] = rv[k];
+      RD(R.text());
+      RD(R.reals(rv, 10));
+      for (int k = 0; k < 10; k++) Z.props[6 + k] = rv[k];
+      const int g = (int)(Z.props[0] + 0.01);
+      if (g == 102 || g == 200)
+        return ifail(in, GFSPH_ERR_UNSUPPORTED,
+                     path + ": terrain zone of geometry " + std::to_string(g) + " (structures, obstacles)");
+      if (g == 101) eros_pts = true;
+      T.zones.push_back(Z);
+    }
+    if (eros_pts) {  // Get_Eros_Pts reads <terrain>.eros.pts (2:943-978)
+      std::string ep = path.substr(0, path.size() - 4) + ".eros.pts";
+      FReader E;
+      if (!E.open(ep)) return ifail(in, GFSPH_ERR_ARG, "cannot open " + ep);
+      auto efail = [&]() { return ifail(in, GFSPH_ERR_ARG, ep + ": " + E.error()); };
+      if (!E.text() || !E.reals(rv, 3)) return efail();
+      const long npts = (long)rv[0];
+      T.eros_type = (int)rv[1];
+      T.eros_dxs = rv[2];
+      if (npts < 0) return ifail(in, GFSPH_ERR_ARG, ep + ": npts_Eros");
+      if (!E.text()) return efail();
+      T.epts.resize(3 * npts);
+      for (long k = 0; k < npts; k++) {
+        if (!E.reals(rv, 3)) return efail();
+        for (int j = 0; j < 3; j++) T.epts[3 * k + j] = rv[j];
+      }
+    }
+  }
   return GFSPH_OK;
 }
 
@@ -1523,9 +1589,39 @@ int parse_all(gfsph_inputs *in, const std::string &dir, const std::string &name,
   return read_restart(in, base + "restart.dat");
 }
 
+// Terrain zones after the grid (Topo_main reads them after Get_DTM and
+// before the slopes, 2:453-617): each geometry-101 zone runs Get_Eros_Pts,
+// which sets topol(15,:) afresh and may change Z (type_Eros -1, -2, -3).
+int zones_build(gfsph_inputs *in) {
+  TopoRaw &T = in->topo;
+  if (T.zones.empty()) return GFSPH_OK;
+  const gfsph_grid &G = in->grid;
+  const long np = (long)G.nx * G.ny;
+  in->eros_avail.assign(np, 0.0);
+  bool any101 = false;
+  std::vector<int32_t> listed(np);
+  for (size_t z = 0; z < T.zones.size(); z++) {
+    gfsph_zone &Z = T.zones[z];
+    if ((int)(Z.props[0] + 0.01) != 101) continue;
+    if (!any101) in->zone_nodes.assign(np, 0);
+    any101 = true;
+    const bool expl = T.ictop == 0;  // explicit grid: coorg are the coordinates read
+    double box[4];
+    const int rc = gfsph_terrain_eros_pts(G.nx, G.ny, G.xmin, G.ymin, G.dx, G.dy, expl ? T.px.data() : nullptr,
+                                          expl ? T.py.data() : nullptr, (int32_t)(T.epts.size() / 3),
+                                          T.epts.data(), T.eros_type, T.eros_dxs, in->Z.data(),
+                                          in->eros_avail.data(), listed.data(), box);
+    if (rc) return ifail(in, rc, "terrain zone " + std::to_string(z + 1) + ": erodible points");
+    for (int k = 0; k < 4; k++) Z.props[1 + k] = box[k];  // basal_props(2:5) (2:989-992)
+    for (long k = 0; k < np; k++)
+      if (listed[k]) in->zone_nodes[k] |= (int32_t)(1u << z);
+  }
+  return GFSPH_OK;
+}
+
 int build_all(gfsph_inputs *in) {
   int rc;
-  if ((rc = grid_build(in)) || (rc = topol_build(in))) return rc;
+  if ((rc = grid_build(in)) || (rc = zones_build(in)) || (rc = topol_build(in))) return rc;
   if (in->icunk == 7 && (rc = release_build(in))) return rc;
   in->built = true;
   return GFSPH_OK;
@@ -1592,7 +1688,7 @@ int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out) {
   if (!in->built || in->n <= 0) return ifail(in, GFSPH_ERR_ARG, "inputs not built (gfsph_inputs_read)");
   *out = nullptr;
   if (!(in->P.dt > 0)) return ifail(in, GFSPH_ERR_ARG, "dt_sph is not set (no MASTER.dat): set it in the params");
-  gfsph_dem d;
+  gfsph_dem d{};
   d.nx = in->grid.nx;
   d.ny = in->grid.ny;
   d.xmin = in->grid.xmin;
@@ -1604,6 +1700,10 @@ int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out) {
   d.Z = in->Z.data();
   d.in_model = in->in_model.data();
   d.topol = in->topol.empty() ? nullptr : in->topol.data();
+  d.nzones = (int32_t)in->topo.zones.size();
+  d.zones = in->topo.zones.empty() ? nullptr : in->topo.zones.data();
+  d.zone_nodes = in->zone_nodes.empty() ? nullptr : in->zone_nodes.data();
+  d.eros_avail = in->eros_avail.empty() ? nullptr : in->eros_avail.data();
   int rc = gfsph_create(&in->P, &d, in->n, in->x.data(), in->v.data(), in->h.data(), in->mass.data(),
                         in->hsml.data(), in->u.data(), in->itype.data(), out);
   if (rc) {
@@ -1716,7 +1816,7 @@ int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out)
       }
     }
   }
-  gfsph_dem d;
+  gfsph_dem d{};
   d.nx = in->grid.nx;
   d.ny = in->grid.ny;
   d.xmin = in->grid.xmin;
@@ -1728,6 +1828,10 @@ int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out)
   d.Z = in->Z.data();
   d.in_model = in->in_model.data();
   d.topol = in->topol.empty() ? nullptr : in->topol.data();
+  d.nzones = (int32_t)in->topo.zones.size();
+  d.zones = in->topo.zones.empty() ? nullptr : in->topo.zones.data();
+  d.zone_nodes = in->zone_nodes.empty() ? nullptr : in->zone_nodes.data();
+  d.eros_avail = in->eros_avail.empty() ? nullptr : in->eros_avail.data();
   int rc = gfsph_batch_create(&in->P, &d, R, counts.data(), cst.data(), x.data(), v.data(), h.data(), mass.data(),
                               hsml.data(), u.data(), out);
   auto bail = [&](const char *what) {
@@ -1759,6 +1863,89 @@ int gfsph_inputs_batch_create(gfsph_inputs *in, int32_t pixels, gfsph_ctx **out)
   ctl.pixels = pixels && T.mode >= 2 ? 1 : 0;
   ctl.pad_ = 0;
   if ((rc = gfsph_batch_cases(*out, &ctl))) return bail("gfsph_batch_cases");
+  return GFSPH_OK;
+}
+
+int gfsph_terrain_eros_pts(int32_t nx, int32_t ny, double xmin, double ymin, double dx, double dy, const double *cx,
+                           const double *cy, int32_t npts, const double *pts, int32_t type_eros, double dx_s,
+                           double *Z, double *avail, int32_t *listed, double *box) {
+  if (nx < 2 || ny < 2 || npts < 0 || (npts > 0 && !pts) || !Z || !avail || !listed || !(dx > 0) || !(dy > 0))
+    return GFSPH_ERR_ARG;
+  const long np = (long)nx * ny;
+  auto coorg = [&](long ip, int d) {  // node ip (0-based)
+    if (d == 0) return cx ? cx[ip] : xmin + (double)(ip % nx) * dx;
+    return cy ? cy[ip] : ymin + (double)(ip / nx) * dy;
+  };
+  const double dist2D = sqrt(dx * dx + dy * dy);
+  std::vector<double> a2(np, 0.0), a3(np, 0.0);
+  for (long k = 0; k < np; k++) avail[k] = 0.0;  // topol(15,:) = 0
+  double xemin = 1.e10, xemax = -1.e10, yemin = 1.e10, yemax = -1.e10;
+  for (int32_t e = 0; e < npts; e++) {
+    const double xe = pts[3 * e], ye = pts[3 * e + 1];
+    if (xe >= xemax) xemax = xe;
+    if (ye >= yemax) yemax = ye;
+    if (xe <= xemin) xemin = xe;
+    if (ye <= yemin) yemin = ye;
+  }
+  if (box) {
+    box[0] = xemin;
+    box[1] = xemax;
+    box[2] = yemin;
+    box[3] = yemax;
+  }
+  const int ixmin = (int)((xemin - xmin) / dx + 1), ixmax = (int)((xemax - xmin) / dx + 1);
+  const int iymin = (int)((yemin - ymin) / dy + 1), iymax = (int)((yemax - ymin) / dy + 1);
+  const int ndx = (int)std::max(2., dx_s / dx), ndy = (int)std::max(2., dx_s / dy);
+  for (int32_t e = 0; e < npts; e++) {
+    const double xx = pts[3 * e], yy = pts[3 * e + 1], dep = pts[3 * e + 2];
+    const int ipx = (int)((xx - xmin) / dx + 1), ipy = (int)((yy - ymin) / dy + 1);
+    const int inx0 = std::max(std::max(1, ipx - ndx), ixmin), inx1 = std::min(std::min(nx, ipx + ndx + 1), ixmax);
+    const int iny0 = std::max(std::max(1, ipy - ndy), iymin), iny1 = std::min(std::min(ny, ipy + ndy + 1), iymax);
+    for (int inx = inx0; inx <= inx1; inx++)
+      for (int iny = iny0; iny <= iny1; iny++) {
+        const long ip = (long)(iny - 1) * nx + (inx - 1);
+        double dx2 = xx - coorg(ip, 0);
+        dx2 = dx2 * dx2;
+        double dy2 = yy - coorg(ip, 1);
+        dy2 = dy2 * dy2;
+        double dist = std::max(1.e-6, sqrt(dx2 + dy2));
+        dist = dist / dist2D;
+        a2[ip] = a2[ip] + 1.0 / dist;
+        a3[ip] = a3[ip] + dep / dist;
+      }
+  }
+  for (long ip = 0; ip < np; ip++) {
+    listed[ip] = a2[ip] >= 1.e-6;
+    if (listed[ip]) avail[ip] = a3[ip] / a2[ip];
+  }
+  if (type_eros == -1) {
+    for (long ip = 0; ip < np; ip++) Z[ip] = Z[ip] + avail[ip];
+  } else if (type_eros == -2) {
+    for (long ip = 0; ip < np; ip++) {
+      const double z0 = Z[ip];
+      Z[ip] = std::max(Z[ip], avail[ip]);
+      avail[ip] = Z[ip] - z0;
+    }
+  } else if (type_eros == -3) {
+    for (long ip = 0; ip < np; ip++) Z[ip] = Z[ip] - avail[ip];
+  }
+  return GFSPH_OK;
+}
+
+int gfsph_inputs_zones(gfsph_inputs *in, int32_t *nzones, gfsph_zone *zones, int32_t *zone_nodes,
+                       double *eros_avail) {
+  if (!in) return GFSPH_ERR_ARG;
+  if (!in->built) return ifail(in, GFSPH_ERR_ARG, "inputs not built (gfsph_inputs_read)");
+  const long np = (long)in->grid.nx * in->grid.ny;
+  if (nzones) *nzones = (int32_t)in->topo.zones.size();
+  if (zones && !in->topo.zones.empty())
+    memcpy(zones, in->topo.zones.data(), sizeof(gfsph_zone) * in->topo.zones.size());
+  if (zone_nodes) {
+    for (long k = 0; k < np; k++) zone_nodes[k] = in->zone_nodes.empty() ? 0 : in->zone_nodes[k];
+  }
+  if (eros_avail) {
+    for (long k = 0; k < np; k++) eros_avail[k] = in->eros_avail.empty() ? 0.0 : in->eros_avail[k];
+  }
   return GFSPH_OK;
 }
 

@@ -41,12 +41,21 @@ module gfsph_c
      real(c_double)     :: k0_pasive         = 1.0_c_double
   end type gfsph_params_t
 
+  type, bind(C), public :: gfsph_zone_t                ! basal_type, basal_law, basal_props(1:16)
+     integer(c_int32_t) :: type = 0, law = 0
+     real(c_double)     :: props(16) = 0.0_c_double
+  end type gfsph_zone_t
+
   type, bind(C), public :: gfsph_dem_t
      integer(c_int32_t) :: nx = 0, ny = 0
      real(c_double)     :: xmin = 0, ymin = 0, dx = 0, dy = 0, xmax = 0, ymax = 0
      type(c_ptr)        :: Z        = c_null_ptr
      type(c_ptr)        :: in_model = c_null_ptr
      type(c_ptr)        :: topol    = c_null_ptr
+     integer(c_int32_t) :: nzones = 0, pad_z = 0          ! terrain zones (ic_topo_props = 10)
+     type(c_ptr)        :: zones      = c_null_ptr       ! gfsph_zone_t(nzones)
+     type(c_ptr)        :: zone_nodes = c_null_ptr       ! node-list bits of geometry-101 zones
+     type(c_ptr)        :: eros_avail = c_null_ptr       ! topol(15,:) of the terrain
   end type gfsph_dem_t
 
   public :: gfsph_create, gfsph_step, gfsph_sync, gfsph_get_state, gfsph_destroy, gfsph_last_error_f

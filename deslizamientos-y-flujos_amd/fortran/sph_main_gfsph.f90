@@ -114,8 +114,9 @@ subroutine create_context
    type(gfsph_dem_t)    :: d
    real(c_double),  allocatable, target :: t9(:,:), zz(:)
    integer(c_int32_t), allocatable, target :: inm(:), ity(:)
+   type(gfsph_zone_t), allocatable, target :: zn(:)
    integer(c_int) rc
-   integer k
+   integer k, np16
 
    if (ntotal .ne. npoin) then
       write(*,*) ' gfsph: virtual, absorbing or vn0 particles are not supported'
@@ -161,6 +162,21 @@ subroutine create_context
    d%Z        = c_loc(zz)
    d%in_model = c_loc(inm)
    d%topol    = c_loc(t9)
+   if (ic_topo_props .eq. 10 .and. ntopo_zones .gt. 0) then   ! terrain zones (2:516-576)
+      allocate (zn(ntopo_zones))
+      np16 = min(16, ntopo_props)
+      do k = 1, ntopo_zones
+         zn(k)%type = basal_type(k)
+         zn(k)%law  = basal_law(k)
+         zn(k)%props(1:np16) = real(basal_props(1:np16, k), c_double)
+         if (int(basal_props(1, k) + 0.1) .eq. 101) then
+            write(*,*) ' gfsph: terrain zones of geometry 101 (node lists of the terrain module) are not passed'
+            stop
+         endif
+      enddo
+      d%nzones = int(ntopo_zones, c_int32_t)
+      d%zones  = c_loc(zn)
+   endif
 
    rc = gfsph_create(p, d, int(npoin, c_int32_t), x, vx, rho, mass, hsml, u, ity, ctx)
    call check(rc, 'gfsph_create')
@@ -178,6 +194,7 @@ subroutine create_context
       call check(rc, 'gfsph_set_pwp_column')
    endif
    deallocate (t9, zz, inm, ity)
+   if (allocated(zn)) deallocate (zn)
 
 end subroutine create_context
 

@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "gfsph_batch_cases", "gfsph_batch_status", "gfsph_inputs_cases", "gfsph_inputs_batch_create",
     "gfsph_get_mass", "gfsph_erosion_read", "gfsph_erosion_set", "gfsph_get_pwp_column", "gfsph_set_pwp_column",
     "gfsph_neighbour_pairs_noalloc", "gfsph_set_mass0", "gfsph_batch_scale_pwp", "gfsph_inputs_resample",
+    "gfsph_terrain_eros_pts", "gfsph_inputs_zones",
 )
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "unsupported option", -3: "HIP runtime failure",
@@ -53,11 +54,17 @@ class CParams(C.Structure):
                 ("ic_k0", C.c_int32), ("pad_k0", C.c_int32), ("k0_active", C.c_double), ("k0_pasive", C.c_double)]
 
 
+class CZone(C.Structure):
+    _fields_ = [("type", C.c_int32), ("law", C.c_int32), ("props", C.c_double * 16)]
+
+
 class CDem(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("xmin", C.c_double), ("ymin", C.c_double),
                 ("dx", C.c_double), ("dy", C.c_double), ("xmax", C.c_double), ("ymax", C.c_double),
                 ("Z", C.POINTER(C.c_double)), ("in_model", C.POINTER(C.c_int32)),
-                ("topol", C.POINTER(C.c_double))]
+                ("topol", C.POINTER(C.c_double)), ("nzones", C.c_int32), ("pad_z", C.c_int32),
+                ("zones", C.POINTER(CZone)), ("zone_nodes", C.POINTER(C.c_int32)),
+                ("eros_avail", C.POINTER(C.c_double))]
 
 
 class CRun(C.Structure):
@@ -129,6 +136,9 @@ def load_library(path: str | None = None):
     L.gfsph_set_mass0.argtypes = [vp, dp]
     L.gfsph_batch_scale_pwp.argtypes = [vp, dp]
     L.gfsph_inputs_resample.argtypes = [vp, C.c_uint64]
+    L.gfsph_terrain_eros_pts.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp,
+                                         C.c_int32, dp, C.c_int32, C.c_double, dp, dp, ip, dp]
+    L.gfsph_inputs_zones.argtypes = [vp, ip, C.POINTER(CZone), ip, dp]
     L.gfsph_get_pwp_column.argtypes = [vp, dp]
     L.gfsph_set_pwp_column.argtypes = [vp, dp]
     L.gfsph_erosion_read.argtypes = [vp, C.c_int32, dp, dp, dp]
@@ -214,7 +224,42 @@ class _DemArgs:
         t9 = getattr(dem, "topol9", None)
         self._t9 = None if t9 is None else np.ascontiguousarray(t9, dtype=np.float64)
         d.topol = _dp(self._t9) if self._t9 is not None else C.POINTER(C.c_double)()
+        # terrain zones (gfsph.dem.TerrainZones): friction and erosion zones
+        tz = getattr(dem, "zones", None)
+        self._zones = self._zn = self._ea = None
+        if tz is not None and len(tz.zones):
+            self._zones = (CZone * len(tz.zones))()
+            for k, z in enumerate(tz.zones):
+                self._zones[k].type = int(z.type)
+                self._zones[k].law = int(z.law)
+                for j, v in enumerate(z.props):
+                    self._zones[k].props[j] = float(v)
+            d.nzones = len(tz.zones)
+            d.zones = C.cast(self._zones, C.POINTER(CZone))
+            if tz.nodes is not None:
+                self._zn = np.ascontiguousarray(tz.nodes, dtype=np.int32)
+                d.zone_nodes = _ip(self._zn)
+            if tz.avail is not None:
+                self._ea = np.ascontiguousarray(tz.avail, dtype=np.float64)
+                d.eros_avail = _dp(self._ea)
         self.c = d
+
+
+def terrain_eros_pts(dem, pts, type_eros: int, dx_s: float):
+    """Get_Eros_Pts ("2 SPH_FEM_topo_2019.f90":909-1110) on ``dem``'s grid
+    (host code of the library): returns (Z after type_eros, topol(15), node
+    list as a bool array, basal_props(2:5)).  pts [n, 3]: x, y, depth."""
+    L = load_library()
+    Z = np.ascontiguousarray(dem.Z, dtype=np.float64).ravel().copy()
+    avail = np.zeros_like(Z)
+    listed = np.zeros(Z.shape[0], np.int32)
+    box = np.zeros(4)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    rc = L.gfsph_terrain_eros_pts(dem.nx, dem.ny, dem.xmin, dem.ymin, dem.dx, dem.dy, None, None, pts.shape[0],
+                                  _dp(pts), int(type_eros), float(dx_s), _dp(Z), _dp(avail), _ip(listed), _dp(box))
+    if rc != 0:
+        raise GfsphError(rc, "gfsph_terrain_eros_pts")
+    return Z.reshape(dem.Z.shape), avail, listed.astype(bool), box
 
 
 class SPHModel:

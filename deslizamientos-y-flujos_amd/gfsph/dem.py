@@ -18,7 +18,7 @@ import math
 
 import numpy as np
 
-__all__ = ["DEM", "fdiff_derivs", "plane_runout_dem", "fractal_dem"]
+__all__ = ["DEM", "Zone", "TerrainZones", "fdiff_derivs", "plane_runout_dem", "fractal_dem"]
 
 
 def fdiff_derivs(Z: np.ndarray, dx: float, dy: float, in_model: np.ndarray | None = None) -> np.ndarray:
@@ -53,6 +53,39 @@ def fdiff_derivs(Z: np.ndarray, dx: float, dy: float, in_model: np.ndarray | Non
 
 
 @dataclasses.dataclass
+class Zone:
+    """One terrain zone of a terrain file's ic_topo_props = 10 block
+    ("2 SPH_FEM_topo_2019.f90":516-576): basal_type (1 friction, 2
+    erosion), basal_law (the zone's erosion law 0..3) and basal_props(1:16)
+    (1: the geometry -- 1, 2, 3 a range of Z, x, y; 4 a rectangle; 5 a turned
+    rectangle; 101 erodible points; 2-6 its bounds; 7.. the law's
+    constants)."""
+
+    type: int
+    law: int
+    props: tuple
+
+    def __post_init__(self):
+        p = [float(v) for v in self.props] + [0.0] * 16
+        self.props = tuple(p[:16])
+
+
+@dataclasses.dataclass
+class TerrainZones:
+    """The terrain zones of a DEM.  ``nodes`` [ny*nx] int32: bit z set when
+    the node is in zone z's node list (geometry 101); ``avail`` [ny*nx] the
+    terrain's topol(15) (erodible depths of geometry-101 zones, 0
+    elsewhere); ``eros_pts``: the points file those come from
+    ({"pts": [n, 3] x, y, depth, "type": type_Eros, "dx_s": spacing}), as the
+    reference reads it from <terrain>.eros.pts."""
+
+    zones: list
+    nodes: np.ndarray | None = None
+    avail: np.ndarray | None = None
+    eros_pts: dict | None = None
+
+
+@dataclasses.dataclass
 class DEM:
     """Regular DEM grid. ``Z`` has shape (ny, nx); coordinates are exact
     multiples of the spacing so that the reference's Xmaxg-Xming spacing
@@ -64,6 +97,7 @@ class DEM:
     dx: float
     dy: float
     in_model: np.ndarray | None = None
+    zones: TerrainZones | None = None
 
     @property
     def nx(self) -> int:

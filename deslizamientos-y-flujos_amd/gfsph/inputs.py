@@ -16,7 +16,7 @@ import numpy as np
 
 from . import cases as Cs
 from .api import BatchModel, CGrid, CParams, CRun, GfsphError, SPHModel, _dp, _ip, load_library
-from .dem import DEM
+from .dem import DEM, TerrainZones, Zone
 
 __all__ = ["Inputs"]
 
@@ -73,6 +73,22 @@ class Inputs:
         self._check(self.lib.gfsph_inputs_dem(self.h, _dp(Z), _ip(im), _dp(top) if top is not None else None))
         return {"Z": Z.reshape(g.ny, g.nx), "in_model": im.reshape(g.ny, g.nx), "topol": top}
 
+    def zones(self):
+        """The terrain zones read (ic_topo_props = 10) as TerrainZones, or
+        None: the zones, their node-list bits and topol(15) after
+        Get_Eros_Pts."""
+        from .api import CZone
+        nz = C.c_int32()
+        self._check(self.lib.gfsph_inputs_zones(self.h, C.byref(nz), None, None, None))
+        if nz.value == 0:
+            return None
+        g = self.grid
+        zs = (CZone * nz.value)()
+        nodes = np.zeros(g.nx * g.ny, np.int32)
+        avail = np.zeros(g.nx * g.ny)
+        self._check(self.lib.gfsph_inputs_zones(self.h, None, C.cast(zs, C.POINTER(CZone)), _ip(nodes), _dp(avail)))
+        return TerrainZones([Zone(z.type, z.law, tuple(z.props)) for z in zs], nodes=nodes, avail=avail)
+
     def particles(self) -> dict:
         n = self.n
         out = {"x": np.zeros((n, 2)), "v": np.zeros((n, 2)), "h": np.zeros(n), "mass": np.zeros(n),
@@ -95,6 +111,7 @@ class Inputs:
         d = self.dem()
         g = self.grid
         dem = FileDEM(d["Z"], g.xmin, g.ymin, g.dx, g.dy, d["in_model"], g.xmax, g.ymax, d["topol"])
+        dem.zones = self.zones()
         q = self.particles()
         parts = Cs.Particles(x=q["x"], v=q["v"], h=q["h"], mass=q["mass"], hsml=q["hsml"], u=q["u"],
                              itype=q["itype"])

@@ -73,11 +73,39 @@ typedef struct gfsph_params {
   double k0_active, k0_pasive;
 } gfsph_params;
 
+/* One terrain zone of the terrain file's ic_topo_props = 10 block
+ * ("2 SPH_FEM_topo_2019.f90":516-576: basal_type, basal_law, basal_props).
+ * type 1: basal friction -- a particle of a frictional law inside the zone
+ * takes tan(phi) = props[6] when props[6] >= 0 (Get_Topo_Basal_Friction_SW,
+ * "6 SPH_SW_2019.f90":10515-10545, applied at 6:9697-9702).  type 2: erosion
+ * -- inside the zone the zone's own law replaces the erosion rate
+ * (Get_Erosion_Rate_SW 6:10407-10505): law 0 none, 1 props[6] h |v|, 2 the
+ * slope law with K = props[6], C* = props[7], 3 the basal-stress law with
+ * K, c, tan(phi_b), beta = props[6..9].  props[0] is the geometry
+ * (Which_Topo_Zone 2:2635-2808): 1, 2, 3 a range props[1]..props[2] of the
+ * terrain Z, of x, of y; 4 the rectangle x props[1]..props[2], y
+ * props[3]..props[4]; 5 the rectangle of corner (props[1], props[2]), sides
+ * props[3], props[4] turned props[5] degrees; 101 the cells touching a node
+ * of the zone's node list (gfsph_dem.zone_nodes).  The first zone of the
+ * type that holds the point is taken. */
+typedef struct gfsph_zone {
+  int32_t type;      /* basal_type: 1 friction, 2 erosion                     */
+  int32_t law;       /* basal_law: the erosion law of a type-2 zone (0..3)    */
+  double props[16];  /* basal_props(1:16)                                     */
+} gfsph_zone;
+
 /* Regular DEM grid ("2 SPH_FEM_topo_2019.f90":94-110, 223-252).  Z is
  * [ny][nx] (node ipoig = iy*nx+ix).  If topol is NULL the nodal slopes and
  * curvature terms are derived on the device by central differences
  * (Fdiff_derivs, 2:2812-2877); otherwise topol is [ny*nx][9] holding
- * Z, Zx, Zy, 1+Zx^2, ZxZy, 1+Zy^2, Zxx/rt, Zxy/rt, Zyy/rt. */
+ * Z, Zx, Zy, 1+Zx^2, ZxZy, 1+Zy^2, Zxx/rt, Zxy/rt, Zyy/rt.
+ * Terrain zones (all zero: none): nzones <= 32 zones of types 1 and 2
+ * (gfsph_zone); zone_nodes[ipoig] has bit z set when the node is in the node
+ * list of zone z (geometry 101: the nodes Get_Eros_Pts, 2:909-1110, gave an
+ * erodible depth); eros_avail is topol(15,:) as the terrain reading leaves
+ * it (those depths, 0 elsewhere), the available depth of zone erosion when
+ * const(21) = 0 (Input_SW 6:1288-1308; with const(21) > 0 the reference
+ * sets every node to const(48) or 999). */
 typedef struct gfsph_dem {
   int32_t nx, ny;
   double xmin, ymin, dx, dy; /* Xming, Yming, deltxg, deltyg                   */
@@ -85,6 +113,11 @@ typedef struct gfsph_dem {
   const double *Z;           /* [ny*nx]                                        */
   const int32_t *in_model;   /* [ny*nx] 0/1 or NULL (all in model)             */
   const double *topol;       /* [ny*nx][9] or NULL                             */
+  int32_t nzones;            /* ntopo_zones (ic_topo_props = 10), 0: none      */
+  int32_t pad_z;
+  const gfsph_zone *zones;   /* [nzones]                                       */
+  const int32_t *zone_nodes; /* [ny*nx] node-list bits, or NULL                */
+  const double *eros_avail;  /* [ny*nx] topol(15,:) of the terrain, or NULL    */
 } gfsph_dem;
 
 /* Uploads the DEM and the particle set and, if p->init_input, performs the
@@ -400,6 +433,25 @@ int gfsph_inputs_cases(gfsph_inputs *in, int32_t *mode, int32_t *ncases, int32_t
 /* Draws a sampled set's problems again from another seed (ic_cases_mat 3, 4,
  * 31, 32; GFSPH_ERR_ARG for the deterministic tables). */
 int gfsph_inputs_resample(gfsph_inputs *in, uint64_t seed);
+/* Erodible points of a geometry-101 terrain zone: Get_Eros_Pts
+ * ("2 SPH_FEM_topo_2019.f90":909-1110) on the grid (nx, ny, xmin, ymin,
+ * dx, dy; node coordinates cx, cy [ny*nx], or NULL for xmin + ix dx,
+ * ymin + iy dy).  pts [npts][3] are the file's x, y, erodible depth; every
+ * node within max(2, dx_s/dx) nodes of a point (and inside the points'
+ * bounding box) takes its inverse-distance weight.  Out: avail = topol(15,:)
+ * (the weighted depth, 0 where no point reached), listed = the zone's node
+ * list (nodes some point reached), box = basal_props(2:5) (the points'
+ * x min, x max, y min, y max); Z is changed by type_eros -1 (Z + depth),
+ * -2 (max(Z, depth), depth = the rise), -3 (Z - depth). */
+/* The terrain zones read (ic_topo_props = 10): their number, the zones as
+ * read (geometry-101 zones with basal_props(2:5) = their points' box), the
+ * node-list bits and topol(15,:) [ny*nx] (zeros without such zones).  Any
+ * pointer may be NULL. */
+int gfsph_inputs_zones(gfsph_inputs *in, int32_t *nzones, gfsph_zone *zones, int32_t *zone_nodes,
+                       double *eros_avail);
+int gfsph_terrain_eros_pts(int32_t nx, int32_t ny, double xmin, double ymin, double dx, double dy, const double *cx,
+                           const double *cy, int32_t npts, const double *pts, int32_t type_eros, double dx_s,
+                           double *Z, double *avail, int32_t *listed, double *box);
 /* All problems of the case table as one batch (replaces the problem loop of
  * the reference's main program, 8:60-130): realisation p is the release and
  * const(:) as Input_SW changes them for problem p (6:1040-1232: variables 501

@@ -60,6 +60,9 @@ typedef struct {
   int eros_law;  /* const(21) + 0.1: 1 Hungr, 2 slope-angle, 3 basal stress */
   double keros, eros_c23, eros_c24, eros_c25, denss, densw_e, dens_bar_eff, slip_fact, avail0;
   double Cv; /* consolidation coefficient 4 Bfact / pi^2 (6:10926) */
+  /* terrain zones: the current erosion state [npoig][3] (topol(1), (14),
+     (15)) while a run erodes, else NULL */
+  const double *eros;
 } derived_t;
 
 static void derive(const gfo_params *p, derived_t *d) {
@@ -389,6 +392,90 @@ static void get_z_topo(const gfo_dem *d, double xx, double yy, int want_top, top
 #undef TV
 }
 
+/* Get_Z_Topo's Z (2:2437-2462) from the current Z (zc[k * zs], node k) */
+static double z_topo_cur(const gfo_dem *d, const double *zc, int zs, double xx, double yy) {
+  int ipx = (int)((xx - d->xmin) / d->dx + 1);
+  int ipy = (int)((yy - d->ymin) / d->dy + 1);
+  double xmax = d->xmin + (d->nx - 1) * d->dx, ymax = d->ymin + (d->ny - 1) * d->dy;
+  if (d->xmax_set) {
+    xmax = d->xmax;
+    ymax = d->ymax;
+  }
+  if (ipx == d->nx && fabs(xmax - xx) <= 1.e-3) ipx = ipx - 1;
+  if (ipy == d->ny && fabs(ymax - yy) <= 1.e-3) ipy = ipy - 1;
+  if (ipx < 1) ipx = 1;
+  if (ipy < 1) ipy = 1;
+  if (ipx > d->nx - 1) ipx = d->nx - 1;
+  if (ipy > d->ny - 1) ipy = d->ny - 1;
+  int64_t n1 = (int64_t)(ipy - 1) * d->nx + ipx - 1, n2 = n1 + 1, n3 = n1 + 1 + d->nx, n4 = n3 - 1;
+  double xi = (xx - (d->xmin + (ipx - 1) * d->dx)) / d->dx, eta = (yy - (d->ymin + (ipy - 1) * d->dy)) / d->dy;
+  double sh1 = (1. - xi) * (1. - eta), sh2 = xi * (1. - eta), sh3 = xi * eta, sh4 = (1. - xi) * eta;
+  const double z1 = zc[n1 * zs], z2 = zc[n2 * zs], z3 = zc[n3 * zs], z4 = zc[n4 * zs];
+  double Zp = sh1 * z1 + sh2 * z2 + sh3 * z3 + sh4 * z4;
+  if (d->in_model) {
+    const int32_t *m = d->in_model;
+    if (m[n1] + m[n2] + m[n3] + m[n4] < 4) {
+      Zp = m[n1] * sh1 * z1 + m[n2] * sh2 * z2 + m[n3] * sh3 * z3 + m[n4] * sh4 * z4;
+      double den = m[n1] * sh1 + m[n2] * sh2 + m[n3] * sh3 + m[n4] * sh4;
+      if (den > 1.e-3) Zp = Zp / den;
+    }
+  }
+  return Zp;
+}
+
+/* Which_Topo_Zone (2:2635-2808): the first zone of basal type btype whose
+   region holds (xx, yy), or -1.  typ_geom = int(props(1) + 0.1); geometry
+   1-3 a range of Z, x, y (ZZdiscr = (max - v)(v - min) >= 0), 4 a
+   rectangle, 5 a rectangle turned props(6) degrees with pi = 4 atan(1),
+   101 the nodes n1, n1+1, n1+1+nx, n1+nx of the point's cell (the cell
+   index as Get_Z_Topo forms it) against the zone's node list.  zc, zs: the
+   current Z of node k at zc[k * zs], for geometry 1. */
+static int which_zone(const gfo_dem *d, const double *zc, int zs, double xx, double yy, int btype) {
+  for (int z = 0; z < d->nzones; z++) {
+    const gfo_zone *Z = &d->zones[z];
+    if (Z->type != btype) continue;
+    const int geom = (int)(Z->props[0] + 0.1);
+    const double *P = Z->props;
+    double discr = -999;
+    if (geom <= 3) {
+      double Zp = geom == 1 ? z_topo_cur(d, zc, zs, xx, yy) : (geom == 2 ? xx : yy);
+      discr = (P[2] - Zp) * (Zp - P[1]);
+    } else if (geom == 4) {
+      double XX = (P[2] - xx) * (xx - P[1]), YY = (P[4] - yy) * (yy - P[3]);
+      if (XX >= 0 && YY >= 0) discr = 999;
+    } else if (geom == 5) {
+      double pi = 4. * atan(1.0);
+      double theta = P[5] * pi / 180.;
+      double costh = cos(theta), sinth = sin(theta);
+      double Rvx = xx - P[1], Rvy = yy - P[2];
+      double avx = costh, avy = sinth, bvx = -sinth, bvy = costh;
+      double Ra = avx * Rvx + avy * Rvy, Rb = bvx * Rvx + bvy * Rvy;
+      if (Ra >= 0.0 && Ra <= P[3] && Rb >= 0.0 && Rb <= P[4]) discr = 999;
+    } else if (geom == 101 || geom == 102) {
+      int ipx = (int)((xx - d->xmin) / d->dx + 1), ipy = (int)((yy - d->ymin) / d->dy + 1);
+      double xmax = d->xmin + (d->nx - 1) * d->dx, ymax = d->ymin + (d->ny - 1) * d->dy;
+      if (d->xmax_set) {
+        xmax = d->xmax;
+        ymax = d->ymax;
+      }
+      if (ipx == d->nx && fabs(xmax - xx) <= 1.e-3) ipx = ipx - 1;
+      if (ipy == d->ny && fabs(ymax - yy) <= 1.e-3) ipy = ipy - 1;
+      int64_t npg = (int64_t)d->nx * d->ny, n1 = (int64_t)(ipy - 1) * d->nx + ipx - 1;
+      int64_t nn[4] = {n1, n1 + 1, n1 + 1 + d->nx, n1 + d->nx};
+      for (int k = 0; k < 4; k++)
+        if (nn[k] >= 0 && nn[k] < npg && d->zone_nodes && ((d->zone_nodes[nn[k]] >> z) & 1)) discr = 999;
+    }
+    if (discr >= 0) return z;
+  }
+  return -1;
+}
+
+void gfo_which_zone(const gfo_dem *d, const double *zc, int n, const double *xy, int btype, int32_t *out) {
+  for (int i = 0; i < n; i++)
+    out[i] = zc ? which_zone(d, zc, 1, xy[2 * i], xy[2 * i + 1], btype)
+                : which_zone(d, d->topol, 9, xy[2 * i], xy[2 * i + 1], btype);
+}
+
 void gfo_topo_lookup(const gfo_dem *d, int n, const double *xy, double *out9) {
   for (int i = 0; i < n; i++) {
     topo_pt t;
@@ -515,6 +602,59 @@ static double erosion_rate(const derived_t *D, double h, double vmod, double vxx
       double Tauf = C_basal + D->dens_bar_eff * D->cgra * h * (1. - beta_basal) * tanfi_basal;
       if (Taub > Tauf && vmod > 1.e-3) er = D->keros * (Taub - Tauf) / D->dens_bar_eff;
     }
+  }
+  return er;
+}
+
+/* Get_Erosion_Rate_SW's erosion zones (6:10407-10503, single phase): in an
+   erosion zone its own law replaces er (law 2 with thetae = atan2(xden,
+   xnum); law 3 with Taub = dens h |frict|, Tauf = c + dens g h (1 - beta)
+   tan(phi_b)); geometry 101 / 102 erodes only where the cell's nodes hold
+   available depth (current topol(15)).  zstate: the current [npoig][3]. */
+static double zone_erosion_rate(const derived_t *D, const gfo_dem *dem, const double *zstate, double xx, double yy,
+                                double er, double h, double vmod, double vxx, double vyy, double Zpx, double Zpy,
+                                double tanfi, double frictx, double fricty) {
+  const int64_t npg = (int64_t)dem->nx * dem->ny;
+  const int z = which_zone(dem, zstate, 3, xx, yy, 2);
+  if (z < 0) return er;
+  const gfo_zone *Z = &dem->zones[z];
+  const double *P = Z->props;
+  const int eros_geom = (int)(P[0] + 0.01);
+  if (Z->law == 0) {
+    er = 0;
+  } else if (Z->law == 1) {
+    double Keros = P[6];
+    er = Keros * h * vmod;
+  } else if (Z->law == 2) {
+    double Keros = P[6], Cstar = P[7], denss = D->denss, densw = D->densw_e;
+    double vol_conc = (D->dens - densw) / (denss - densw);
+    er = 0.0;
+    if (vmod > 1.e-5) {
+      double cosa = vxx / vmod, sina = vyy / vmod;
+      double gradZv = Zpx * cosa + Zpy * sina;
+      double theta = atan2(-gradZv, 1.);
+      double xnum = (denss - densw) * vol_conc;
+      double xden = xnum + densw;
+      xnum = xnum * tanfi;
+      double thetae = atan2(xden, xnum);
+      if (theta > thetae) er = Keros * Cstar * vmod * tan(theta - thetae);
+    }
+  } else if (Z->law == 3) {
+    double Keros = P[6], C_basal = P[7], tanfi_basal = P[8], beta_basal = P[9];
+    er = 0.0;
+    if (vmod > 1.e-5) {
+      double Taub = D->dens * h * sqrt(frictx * frictx + fricty * fricty);
+      double Tauf = C_basal + D->dens * D->cgra * h * (1. - beta_basal) * tanfi_basal;
+      if (Taub > Tauf && vmod > 1.e-3) er = Keros * (Taub - Tauf) / D->dens;
+    }
+  }
+  if (eros_geom == 101 || eros_geom == 102) {
+    int ipx = (int)((xx - dem->xmin) / dem->dx + 1), ipy = (int)((yy - dem->ymin) / dem->dy + 1);
+    int64_t n1 = (int64_t)(ipy - 1) * dem->nx + ipx - 1;
+    int64_t nn[4] = {n1, n1 + 1, n1 + dem->nx, n1 + dem->nx + 1};
+    double auxt = 0.0;
+    for (int k = 0; k < 4; k++) auxt = auxt + ((nn[k] >= 0 && nn[k] < npg) ? zstate[3 * nn[k] + 2] : 0.0);
+    if (auxt < 1.e-6) er = 0.0;
   }
   return er;
 }
@@ -720,6 +860,12 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
       if (factpw > D->fact_pwp_max) factpw = D->fact_pwp_max;
       tanfi = tanfi * factpw;
     }
+    /* friction zones of the terrain: Get_Topo_Basal_Friction_SW (6:9697-9702,
+       6:10515-10545) -- Tandelta = basal_props(7) of the zone, if >= 0 */
+    if (dem->nzones) {
+      const int z = D->eros ? which_zone(dem, D->eros, 3, xx, yy, 1) : which_zone(dem, dem->topol, 9, xx, yy, 1);
+      if (z >= 0 && dem->zones[z].props[6] >= 0) tanfi = dem->zones[z].props[6];
+    }
     tanfi_m = tanfi;
     double gtanfi = (gravz + cv2byR) * tanfi * D->fact_dens;
     if (gtanfi < 0.0) gtanfi = 0.0;
@@ -816,6 +962,9 @@ static void ext_force_one(const gfo_params *p, const derived_t *D, const gfo_dem
   o->er = 0.0;
   if (D->eros_on) { /* 6:8664-8693; the momentum term only inside the stages */
     o->er = erosion_rate(D, h, vmod, vxx, vyy, tp.Zx, tp.Zy, tanfi_m, frictx, fricty);
+    if (dem->nzones)
+      o->er = zone_erosion_rate(D, dem, D->eros, xx, yy, o->er, h, vmod, vxx, vyy, tp.Zx, tp.Zy, tanfi_m, frictx,
+                                fricty);
     if (!acc0) {
       exx = exx - o->er * vx1 * hinv * D->slip_fact;
       exy = exy - o->er * vy1 * hinv * D->slip_fact;
@@ -1077,6 +1226,9 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
             int nsteps, double *tau0_io, double *dt_time_io, double *eros) {
   derived_t D;
   derive(p_in, &D);
+  for (int z = 0; z < dem->nzones; z++)
+    if (dem->zones[z].type == 2) D.eros_on = 1; /* ic_basal_erosion (6:1288-1290) */
+  D.eros = D.eros_on ? eros : NULL;
   if (D.eros_on && !eros) return -2;
   if (D.nfric == 3 && !p_in->mass0) return -3;
   const int64_t npg = (int64_t)dem->nx * dem->ny;

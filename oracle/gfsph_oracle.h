@@ -36,6 +36,12 @@ typedef struct {
   const double *mass0;
 } gfo_params;
 
+/* one terrain zone: basal_type, basal_law, basal_props(1:16) (2:516-576) */
+typedef struct {
+  int32_t type, law;
+  double props[16];
+} gfo_zone;
+
 typedef struct {
   int32_t nx, ny;             /* npoigx, npoigy */
   int32_t xmax_set, pad_;     /* 1: use xmax/ymax below (Xmaxg/Ymaxg as read) */
@@ -44,6 +50,12 @@ typedef struct {
   const double *topol;        /* [nx*ny][9]: Z Zx Zy top4..top9 */
   const int32_t *in_model;    /* [nx*ny] or NULL (all 1) */
   const double *coorg;        /* [nx*ny][2] or NULL (regular grid) */
+  /* terrain zones (ic_topo_props = 10): friction (type 1) and erosion
+   * (type 2) zones; zone_nodes [nx*ny] bit z: node in zone z's node list
+   * (geometry 101), or NULL */
+  int32_t nzones, pad2_;
+  const gfo_zone *zones;
+  const int32_t *zone_nodes;
 } gfo_dem;
 
 void gfo_kernel(int skf, double r, double dx, double dy, double hsml, double *w, double *dwx,
@@ -51,6 +63,9 @@ void gfo_kernel(int skf, double r, double dx, double dy, double hsml, double *w,
 int64_t gfo_pairs(int n, int skf, const double *x, const double *hsml, const int32_t *flag,
                   int brute, int64_t cap, int32_t *ij, double *w, double *dwdx);
 void gfo_topo_lookup(const gfo_dem *d, int n, const double *xy, double *out9);
+/* Which_Topo_Zone (2:2635-2808) at n points for basal type btype: zone
+ * index (0-based) or -1; zc: current Z [nx*ny] or NULL (topol's Z) */
+void gfo_which_zone(const gfo_dem *d, const double *zc, int n, const double *xy, int btype, int32_t *out);
 void gfo_fdiff_derivs(int nx, int ny, double dxg, double dyg, const int32_t *in_model,
                       double *topol);
 int gfo_init(const gfo_params *p, const gfo_dem *dem, int n, const double *x, double *h,

@@ -27,12 +27,17 @@ class GfoParams(C.Structure):
                 ("mass0", C.POINTER(C.c_double))]
 
 
+class GfoZone(C.Structure):
+    _fields_ = [("type", C.c_int32), ("law", C.c_int32), ("props", C.c_double * 16)]
+
+
 class GfoDem(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("xmax_set", C.c_int32), ("pad_", C.c_int32),
                 ("xmin", C.c_double), ("ymin", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
                 ("xmax", C.c_double), ("ymax", C.c_double),
                 ("topol", C.POINTER(C.c_double)), ("in_model", C.POINTER(C.c_int32)),
-                ("coorg", C.POINTER(C.c_double))]
+                ("coorg", C.POINTER(C.c_double)), ("nzones", C.c_int32), ("pad2_", C.c_int32),
+                ("zones", C.POINTER(GfoZone)), ("zone_nodes", C.POINTER(C.c_int32))]
 
 
 _lib = None
@@ -53,6 +58,7 @@ def lib():
         L.gfo_pairs.argtypes = [C.c_int, C.c_int, dp, dp, ip, C.c_int, C.c_int64, ip, dp, dp]
         L.gfo_pairs.restype = C.c_int64
         L.gfo_topo_lookup.argtypes = [C.POINTER(GfoDem), C.c_int, dp, dp]
+        L.gfo_which_zone.argtypes = [C.POINTER(GfoDem), dp, C.c_int, dp, C.c_int, ip]
         L.gfo_fdiff_derivs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp]
         L.gfo_init.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, ip]
         L.gfo_run.argtypes = [C.POINTER(GfoParams), C.POINTER(GfoDem), C.c_int, dp, dp, dp, dp, dp, dp, dp, ip,
@@ -114,6 +120,19 @@ class DemHandle:
         d.topol = _dp(self.topol)
         d.in_model = _ip(self.in_model) if self.in_model is not None else C.POINTER(C.c_int32)()
         d.coorg = C.POINTER(C.c_double)()
+        tz = getattr(dem, "zones", None)
+        self._zones = self._zn = None
+        if tz is not None and len(tz.zones):
+            self._zones = (GfoZone * len(tz.zones))()
+            for k, z in enumerate(tz.zones):
+                self._zones[k].type, self._zones[k].law = int(z.type), int(z.law)
+                for j, v in enumerate(z.props):
+                    self._zones[k].props[j] = float(v)
+            d.nzones = len(tz.zones)
+            d.zones = C.cast(self._zones, C.POINTER(GfoZone))
+            if tz.nodes is not None:
+                self._zn = np.ascontiguousarray(tz.nodes, dtype=np.int32)
+                d.zone_nodes = _ip(self._zn)
         self.s = d
 
 
@@ -168,22 +187,41 @@ class State:
         self.eros = None         # [npoig, 3] topol(1), topol(14), topol(15) when erosion is on
 
 
-def erosion_on(prm) -> bool:
+def erosion_on(prm, dem=None) -> bool:
     """ic_erosion (6:1180-1184, 6:1288-1290): const(21) > 1e-9, or the older
-    form const(4) > 1e-4."""
+    form const(4) > 1e-4, or an erosion zone of the terrain."""
     c = prm.cst()
-    return bool(c[21] > 1e-9 or c[4] > 1e-4)
+    tz = getattr(dem, "zones", None)
+    zones = tz is not None and any(int(z.type) == 2 for z in tz.zones)
+    return bool(c[21] > 1e-9 or c[4] > 1e-4 or zones)
 
 
 def erosion_grid(case):
     """Initial erosion state of the terrain: topol(1) = Z, topol(14) = 0 and
-    the available depth topol(15) = const(48) if given, else 999
-    (6:1294-1301)."""
+    the available depth topol(15) = const(48) if given, else 999 with a
+    surface law (6:1294-1301); with erosion zones alone the terrain's
+    topol(15) with values below 1e-6 set to 0 (6:1302-1307)."""
     c = case.params.cst()
     g = np.zeros((case.dem.nx * case.dem.ny, 3))
     g[:, 0] = np.asarray(case.dem.Z, dtype=np.float64).ravel()
-    g[:, 2] = c[48] if c[48] > 1e-6 else 999.0
+    c21 = 1.0 if c[4] > 1e-4 else c[21]
+    tz = getattr(case.dem, "zones", None)
+    if c21 > 1e-9 or tz is None:
+        g[:, 2] = c[48] if c[48] > 1e-6 else 999.0
+    elif tz.avail is not None:
+        a = np.asarray(tz.avail, dtype=np.float64).ravel()
+        g[:, 2] = np.where(a < 1e-6, 0.0, a)
     return g
+
+
+def which_zone(dem, xy, btype, zc=None):
+    """Which_Topo_Zone (2:2635-2808) at the points xy [n, 2]: zone index or -1."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    out = np.zeros(xy.shape[0], np.int32)
+    dh = DemHandle(dem)
+    lib().gfo_which_zone(C.byref(dh.s), None if zc is None else _dp(np.ascontiguousarray(zc, dtype=np.float64)),
+                         xy.shape[0], _dp(xy), int(btype), _ip(out))
+    return out
 
 
 def init(case, state=None):
@@ -195,7 +233,7 @@ def init(case, state=None):
     dh = DemHandle(case.dem)
     lib().gfo_init(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.h), _dp(st.mass), _dp(st.hsml),
                    _dp(st.u), _dp(st.uaux), _ip(st.flag))
-    if erosion_on(case.params) and st.eros is None:
+    if erosion_on(case.params, case.dem) and st.eros is None:
         st.eros = erosion_grid(case)
     return st
 
@@ -210,7 +248,7 @@ def run(case, st, nsteps, rev_order=False):
     if st.dt_time is None:
         st.dt_time = np.array([case.params.dt, 0.0])
     dh = DemHandle(case.dem)
-    if erosion_on(case.params) and st.eros is None:
+    if erosion_on(case.params, case.dem) and st.eros is None:
         st.eros = erosion_grid(case)
     rc = lib().gfo_run(C.byref(p), C.byref(dh.s), st.h.shape[0], _dp(st.x), _dp(st.v), _dp(st.h), _dp(st.mass),
                        _dp(st.hsml), _dp(st.u), _dp(st.uaux), _ip(st.flag), nsteps, _dp(st.tau0),

@@ -40,6 +40,14 @@
 !     real64 (topol(1,ip), topol(14,ip), topol(15,ip), ip = 1, npoig)
 !  If gid_final.ctl exists in the run directory, the reference's GiD result
 !  writer (OutputRes_SW) also runs once after the last step.
+!  If zones.ctl exists (line 1: npts, then npts lines x y), the harness calls
+!  the reference's Topo_main itself before Init_sph (whose own call then
+!  returns at once) and writes zones.bin -- the terrain zones as Topo_main
+!  leaves them, before Input_SW and restart.top change topol:
+!     int32  magic npoig npts
+!     real64 (topol(1,ip), topol(15,ip), ip = 1, npoig)
+!     int32  (Which_Topo_Zone(x,y,1), Which_Topo_Zone(x,y,2), point = 1, npts)
+!  (-999: in no zone of that type).
 !  It is built only by oracle/build_ref.sh and never ships with the product.
 !  Built with -DGFSPH_GPU it calls time_integration_sph_gfsph (module
 !  SPH_MAIN_GFSPH of the product's Fortran layer) instead, i.e. the
@@ -55,7 +63,7 @@ use SPH_Main_Vars_2019
 use SPH_SW_Vars_2019
 use SPH_SW_2019, only: OutputRes_SW, Get_MC_pixel_Vars_SW, Norm_MC_pixel_Vars_SW
 use SPH_FEM_topo_2019, only: npoigx, npoigy, Xming, Xmaxg, Yming, Ymaxg, deltxg, deltyg, &
-                             xshiftg, yshiftg, topol, poig_in_model
+                             xshiftg, yshiftg, topol, poig_in_model, Topo_main, Which_Topo_Zone
 #ifdef GFSPH_GPU
 use SPH_MAIN_GFSPH, only: time_integration_sph_gfsph, gfsph_release, gfsph_sync_state, gfsph_pulls, &
                           gfsph_pull_every_step
@@ -72,6 +80,7 @@ integer        nprob, plot_every, iprob, ios
 character(16), allocatable :: pnames(:)
 integer(8)     clk0, clk1, clkrate, cd0, cd1, clkdump
 character(8)   envv
+logical        zones_ctl
 
 open (newunit=ictl, file='harness.ctl', status='old')
 read (ictl,'(a16)') name
@@ -128,6 +137,8 @@ if (nprob.gt.0) then
    dt = dtin; dt_sph = dtin
    t_print_reset = 0.; t_save_reset = 0.; t_plot_reset = 0.
 endif
+inquire (file='zones.ctl', exist=zones_ctl)
+if (iprob.eq.1.and.zones_ctl) call write_zones
 call Init_sph
 
 !      ------  Driver_Flow (8:1565-1620) for one SPH substep per step
@@ -206,6 +217,32 @@ subroutine write_mc_pixel
    enddo
    close (iu)
 end subroutine write_mc_pixel
+
+subroutine write_zones
+   integer iu, ip, np9, npts, izc, z1, z2
+   real(8), allocatable :: px(:), py(:)
+   real(8) xx, yy
+   call Topo_main
+   open (newunit=izc, file='zones.ctl', status='old')
+   read (izc,*) npts
+   allocate (px(max(npts,1)), py(max(npts,1)))
+   do ip = 1, npts
+      read (izc,*) px(ip), py(ip)
+   enddo
+   close (izc)
+   np9 = npoigx*npoigy
+   open (newunit=iu, file='zones.bin', access='stream', form='unformatted', status='replace')
+   write (iu) int(1195725648,4), int(np9,4), int(npts,4)
+   write (iu) (real(topol(1,ip),8), real(topol(15,ip),8), ip=1,np9)
+   do ip = 1, npts
+      xx = px(ip); yy = py(ip)
+      call Which_Topo_Zone (xx, yy, 1, z1)
+      call Which_Topo_Zone (xx, yy, 2, z2)
+      write (iu) int(z1,4), int(z2,4)
+   enddo
+   close (iu)
+   deallocate (px, py)
+end subroutine write_zones
 
 ! topo.bin: the terrain Init_sph built (Topo_main's module state)
 !    int32  magic(=1195725648) version(=2) npoigx npoigy

@@ -107,21 +107,40 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
                 rows.append(f"{k} {_r(xs[ix])} {_r(ys[iy])} {_r(Z[iy, ix])}")
                 k += 1
         f.write("\n".join(rows) + "\n")
-        f.write("topo props (zones) 0 = none\n0\n")
+        tz = getattr(dem, "zones", None)
+        if tz is None or not tz.zones:
+            f.write("topo props (zones) 0 = none\n0\n")
+        else:  # ic_topo_props = 10 (Topo_main 2:516-576)
+            f.write(f"topo props: basal zones\n10\nntopo_zones\n{len(tz.zones)}\n")
+            for z in tz.zones:
+                f.write(f"basal type\n{int(z.type)}\nbasal law\n{int(z.law)}\n")
+                f.write("props 1..6\n" + " ".join(_r(v) for v in z.props[:6]) + "\n")
+                f.write("props 7..16\n" + " ".join(_r(v) for v in z.props[6:16]) + "\n")
+    tz = getattr(dem, "zones", None)
+    if tz is not None and tz.eros_pts is not None:  # Get_Eros_Pts (2:943-978)
+        ep = tz.eros_pts
+        pts = np.asarray(ep["pts"], dtype=np.float64)
+        with open(os.path.join(d, topo + ".eros.pts"), "w") as f:
+            f.write("erodible points written for the reference oracle\n")
+            f.write(f"{pts.shape[0]} {int(ep['type'])} {_r(ep['dx_s'])}\nx y depth\n")
+            f.write("\n".join(f"{_r(a)} {_r(b)} {_r(cc)}" for a, b, cc in pts) + "\n")
+    eros_zones = tz is not None and any(int(z.type) == 2 for z in tz.zones)
 
-    if c[21] > 1e-9 or c[4] > 1e-4:
+    if c[21] > 1e-9 or c[4] > 1e-4 or eros_zones:
         # erosion with restart.dat input reads the terrain's erosion state from
         # restart.top (Get_Topo_restart, "2 SPH_FEM_topo_2019.f90":2880-2921):
         # Z, eroded depth 0, available depth (reset by Input_SW right after)
         with open(os.path.join(d, "restart.top"), "w") as f:
             f.write("terrain erosion state written for the reference oracle\n")
             f.write(f"{nx * ny} 0.0\nipoig topol(1) topol(14) topol(15)\n")
-            avail = c[48] if c[48] > 1e-6 else 999.0
+            avail = np.full(nx * ny, c[48] if c[48] > 1e-6 else 999.0)
+            if not (c[21] > 1e-9 or c[4] > 1e-4):  # zones alone: the terrain's topol(15)
+                avail = np.zeros(nx * ny) if tz.avail is None else np.asarray(tz.avail, dtype=np.float64).ravel()
             rows = []
             k = 1
             for iy in range(ny):
                 for ix in range(nx):
-                    rows.append(f"{k} {_r(Z[iy, ix])} 0.0 {_r(avail)}")
+                    rows.append(f"{k} {_r(Z[iy, ix])} 0.0 {_r(avail[k - 1])}")
                     k += 1
             f.write("\n".join(rows) + "\n")
 
@@ -279,6 +298,41 @@ def run_reference_files(src_dir: str, name: str, files, nsteps: int, dt: float, 
             raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
         out = {s: read_dump(os.path.join(tmp, f"dump_{s}.bin")) for s in steps}
         return out, read_topo(os.path.join(tmp, "topo.bin"))
+    finally:
+        subprocess.run(["rm", "-rf", tmp])
+
+
+def read_zones(path: str, npts: int) -> dict:
+    """zones.bin of the harness: topol(1) and topol(15) as Topo_main left
+    them, and Which_Topo_Zone's answer (1-based zone, -999 none) for basal
+    types 1 and 2 at each sample point."""
+    raw = open(path, "rb").read()
+    magic, npg, n = (int(v) for v in np.frombuffer(raw, dtype=np.int32, count=3))
+    assert magic == 1195725648 and n == npts
+    t = np.frombuffer(raw, dtype=np.float64, count=2 * npg, offset=12).reshape(npg, 2).copy()
+    z = np.frombuffer(raw, dtype=np.int32, count=2 * n, offset=12 + 16 * npg).reshape(n, 2).copy()
+    return {"z": t[:, 0], "avail": t[:, 1], "zone1": z[:, 0], "zone2": z[:, 1]}
+
+
+def run_reference_zones(case, points, timeout: float = 600.0) -> dict:
+    """The reference's Topo_main on ``case``'s terrain (its zones and
+    erodible points), then Which_Topo_Zone at ``points`` [n, 2] (harness
+    zones.ctl mode; the run goes on through Init_sph, 0 steps)."""
+    if not os.path.exists(REF_EXE):
+        raise FileNotFoundError(REF_EXE)
+    tmp = tempfile.mkdtemp(prefix="gfsph_ref_")
+    try:
+        write_inputs(tmp, case)
+        pts = np.asarray(points, dtype=np.float64)
+        with open(os.path.join(tmp, "zones.ctl"), "w") as f:
+            f.write(f"{pts.shape[0]}\n" + "\n".join(f"{_r(a)} {_r(b)}" for a, b in pts) + "\n")
+        with open(os.path.join(tmp, "harness.ctl"), "w") as f:
+            f.write(f"case\n0 {_r(case.params.dt)} 0\n0\n")
+        r = subprocess.run([REF_EXE], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
+                           timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        return read_zones(os.path.join(tmp, "zones.bin"), pts.shape[0])
     finally:
         subprocess.run(["rm", "-rf", tmp])
 

@@ -28,7 +28,7 @@ sys.path.insert(0, REPO)
 sys.path.insert(0, os.path.join(REPO, "deslizamientos-y-flujos_amd"))
 
 from gfsph import cases as C  # noqa: E402
-from gfsph.dem import DEM, fractal_dem, plane_runout_dem  # noqa: E402
+from gfsph.dem import DEM, TerrainZones, Zone, fractal_dem, plane_runout_dem  # noqa: E402
 from oracle import refio  # noqa: E402
 
 
@@ -207,6 +207,65 @@ def case_defs():
         case=C.Case("g_k0_pwp", C.Params(integrator=4, dt=0.02, h_inf=1e-3, icpwp=3, pwp0rel=0.6, ic_k0=1,
                                          k0_active=0.5, k0_pasive=2.0, const=cp), dem, scar_on(dem, 300, 53), 100),
         dumps=[1, 10, 100])
+    # terrain zones (ic_topo_props = 10, Topo_main 2:516-576, Which_Topo_Zone
+    # 2:2635-2808): friction zones of geometries 4 (a faster rectangle), 5 (a
+    # turned rectangle), 1 (a range of Z) and 2 with a negative tan(phi) (no
+    # change) on a frictional + Voellmy flow with pore pressure (the zone's
+    # tan(phi) replaces the pore-pressure-reduced one, 6:9697-9702)
+    dz = small_plane()
+    dz.zones = TerrainZones([Zone(1, 0, (2, 60.0, 68.0, 0, 0, 0, -1.0)),
+                             Zone(1, 0, (5, 68.0, 75.0, 20.0, 15.0, 10.0, 0.35)),
+                             Zone(1, 0, (4, 95.0, 130.0, 70.0, 110.0, 0, 0.05)),
+                             Zone(1, 0, (1, 100.0, 115.0, 0, 0, 0, 0.4))])
+    cz = C.voellmy(0.2, 500.0)
+    cz.update({13: math.pi ** 2 * 0.01 / 4.0, 14: 1.0})  # consolidation coefficient cv = 0.01
+    defs["g_zones_friction"] = dict(
+        case=C.Case("g_zones_friction", C.Params(integrator=4, dt=0.05, h_inf=1e-3, icpwp=3, pwp0rel=0.5,
+                                                 const=cz), dz,
+                    C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
+        dumps=[1, 10, 100])
+    # erosion zones alone (const(21) = 0; Get_Erosion_Rate_SW 6:10407-10503):
+    # a geometry-101 zone of erodible points (law 1, erodes only where its
+    # cells hold depth: the points' IDW depths are the terrain's topol(15),
+    # 0 elsewhere), a turned rectangle with law 0 over part of the release,
+    # law 1 in a rectangle, law 2 (slope angle) in an x range, law 3 (basal
+    # stress) in a y range covering the rest
+    # (a 70-degree slope, so that the zone form of law 2 -- equilibrium angle
+    # atan2(xden, xnum) = 67 degrees for phi = 40 -- erodes; the release moves
+    # from the start, since at rest a law-2 zone stops the reference)
+    de = plane_runout_dem(nx=81, ny=41, d=5.0, slope_deg=70.0, x_break=250.0, blend=50.0)
+    rng = np.random.Generator(np.random.Philox(71))
+    epts = np.stack([rng.uniform(100, 140, 60), rng.uniform(70, 110, 60), rng.uniform(0.1, 0.6, 60)], axis=1)
+    de.zones = TerrainZones([Zone(2, 1, (101, 0, 0, 0, 0, 0, 0.01)),
+                             Zone(2, 0, (5, 58.0, 78.0, 16.0, 11.0, 0.0)),
+                             Zone(2, 2, (2, 80.0, 100.0, 0, 0, 0, 0.05, 0.6)),
+                             Zone(2, 1, (4, 50.0, 80.0, 60.0, 90.0, 0, 0.004)),
+                             Zone(2, 3, (3, 0.0, 200.0, 0, 0, 0, 0.002, 50.0, 0.3, 0.2))],
+                            eros_pts={"pts": epts, "type": 1, "dx_s": 6.0})
+    _zone_nodes(de)
+    cze = C.frictional(40.0)
+    cze.update({11: -1.0, 17: 2650.0, 18: 1000.0})
+    pze = C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5)
+    pze.v[:] = [3.0, 0.5]
+    defs["g_zones_erosion"] = dict(
+        case=C.Case("g_zones_erosion", C.Params(integrator=4, dt=0.02, h_inf=1e-3, const=cze), de, pze, 100),
+        dumps=[1, 10, 100])
+    # erosion zones with a surface law (const(21) = 1): every node's available
+    # depth is const(48) (6:1294-1301, the points' depths are overwritten);
+    # outside the zones the surface law erodes; a friction zone over a range
+    # of the terrain's Z, which the erosion lowers
+    dm = small_plane()
+    dm.zones = TerrainZones([Zone(2, 1, (101, 0, 0, 0, 0, 0, 0.02)),
+                             Zone(1, 0, (1, 95.0, 103.0, 0, 0, 0, 0.1)),
+                             Zone(2, 0, (4, 60.0, 75.0, 80.0, 90.0))],
+                            eros_pts={"pts": epts, "type": 1, "dx_s": 6.0})
+    _zone_nodes(dm)
+    czm = C.voellmy(0.2, 500.0)
+    czm.update({11: -1.0, 21: 1, 22: 0.002, 30: -1.0, 45: -1.0, 48: 0.5})
+    defs["g_zones_erosion_main"] = dict(
+        case=C.Case("g_zones_erosion_main", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=czm), dm,
+                    C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 100),
+        dumps=[1, 10, 100])
     # random cloud with varied smoothing lengths: neighbour set only
     rng = np.random.Generator(np.random.Philox(99))
     n = 600
@@ -219,6 +278,57 @@ def case_defs():
                                                     const=C.voellmy(0.6, 500.0)),
                     small_fractal(3), pr, 1), dumps=[1])
     return defs
+
+
+def _zone_nodes(dem):
+    """Node lists and topol(15) of the geometry-101 zones from the erodible
+    points (the library's Get_Eros_Pts; g_zones_map pins it to the
+    reference's)."""
+    from gfsph import api
+    tz = dem.zones
+    nodes = np.zeros(dem.nx * dem.ny, np.int32)
+    avail = np.zeros(dem.nx * dem.ny)
+    for k, z in enumerate(tz.zones):
+        if int(z.props[0] + 0.01) == 101:
+            ep = tz.eros_pts
+            Z, avail, listed, box = api.terrain_eros_pts(dem, ep["pts"], ep["type"], ep["dx_s"])
+            nodes |= np.where(listed, 1 << k, 0).astype(np.int32)
+    tz.nodes, tz.avail = nodes, avail
+
+
+def save_zones_map(out_dir=HERE):
+    """The reference's terrain zones at 20,909 points of the plane (Topo_main
+    then Which_Topo_Zone for basal types 1 and 2) and topol(1), topol(15) as
+    Topo_main leaves them, for erodible-point files of type_Eros 1, -1, -2,
+    -3 (Get_Eros_Pts 2:909-1110)."""
+    rng = np.random.Generator(np.random.Philox(72))
+    epts = np.stack([rng.uniform(280, 330, 40), rng.uniform(60, 120, 40), rng.uniform(0.2, 1.0, 40)], axis=1)
+    epts2 = np.stack([rng.uniform(300, 360, 30), rng.uniform(150, 190, 30), rng.uniform(0.0, 2.0, 30)], axis=1)
+    zones = [Zone(1, 0, (4, 100.0, 180.0, 60.0, 140.0, 0.0, 0.05)),
+             Zone(1, 0, (5, 200.0, 50.0, 80.0, 60.0, 30.0, 0.3)),
+             Zone(1, 0, (1, 10.0, 60.0, 0, 0, 0, 0.25)),
+             Zone(2, 1, (2, 120.0, 260.0, 0, 0, 0, 0.003)),
+             Zone(2, 0, (3, 0.0, 90.0)),
+             Zone(2, 1, (101, 0, 0, 0, 0, 0, 0.01)),
+             Zone(1, 0, (5, 20.0, 140.0, 50.0, 30.0, -35.0, 0.2))]
+    xs = np.linspace(-2.0, 402.0, 203)
+    ys = np.linspace(-2.0, 202.0, 103)
+    P = np.array([(x, y) for y in ys for x in xs])
+    arr = {"points": P, "zone_type": np.array([z.type for z in zones], np.int32),
+           "zone_law": np.array([z.law for z in zones], np.int32), "zone_props": np.array([z.props for z in zones])}
+    for tag, pts, typ in (("t1", epts, 1), ("tm1", epts2, -1), ("tm2", epts2, -2), ("tm3", epts2, -3)):
+        dem = small_plane()
+        dem.zones = TerrainZones(zones, eros_pts={"pts": pts, "type": typ, "dx_s": 6.0})
+        case = C.Case("zones", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=C.voellmy(0.2, 500.0)), dem,
+                      C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 0)
+        r = refio.run_reference_zones(case, P)
+        arr[f"{tag}_pts"] = pts
+        arr[f"{tag}_z"] = r["z"]
+        arr[f"{tag}_avail"] = r["avail"]
+        arr[f"{tag}_zone1"] = r["zone1"]
+        arr[f"{tag}_zone2"] = r["zone2"]
+    arr["dem_Z"] = small_plane().Z
+    np.savez_compressed(os.path.join(out_dir, "g_zones_map.npz"), **arr)
 
 
 def cfg1_full_def(nsteps=100):
@@ -242,6 +352,17 @@ def save_case(name, d, out_dir=HERE):
         "p_x": P.x, "p_v": P.v, "p_h": P.h, "p_mass": P.mass, "p_hsml": P.hsml, "p_u": P.u,
         "nsteps": np.array(case.nsteps), "dumps": np.array(sorted(set([0] + list(dumps)))),
     }
+    tz = getattr(case.dem, "zones", None)
+    if tz is not None:  # terrain zones (inputs): the zones, node lists, topol(15), erodible points
+        arr["zone_type"] = np.array([z.type for z in tz.zones], np.int32)
+        arr["zone_law"] = np.array([z.law for z in tz.zones], np.int32)
+        arr["zone_props"] = np.array([z.props for z in tz.zones])
+        if tz.nodes is not None:
+            arr["zone_nodes"] = tz.nodes
+            arr["zone_avail"] = tz.avail
+        if tz.eros_pts is not None:
+            arr["eros_pts"] = np.asarray(tz.eros_pts["pts"])
+            arr["eros_type_dxs"] = np.array([tz.eros_pts["type"], tz.eros_pts["dx_s"]])
     for s, dd in ref.items():
         arr[f"s{s}_x"] = dd["x"]
         arr[f"s{s}_v"] = dd["v"]
@@ -451,7 +572,7 @@ def save_case_driver(out_dir=HERE):
     np.savez_compressed(os.path.join(out_dir, "g_case_driver_ref.npz"), **arr)
 
 
-SPECIAL = {"g_case_driver_ref": save_case_driver, "g_mc_hazard_ref": save_mc_hazard, "g_cfg1_full": save_cfg1, "g_cfg1_growth": save_cfg1_growth, "g_cfg2_full": save_cfg2}
+SPECIAL = {"g_zones_map": save_zones_map, "g_case_driver_ref": save_case_driver, "g_mc_hazard_ref": save_mc_hazard, "g_cfg1_full": save_cfg1, "g_cfg1_growth": save_cfg1_growth, "g_cfg2_full": save_cfg2}
 
 
 def main(argv):

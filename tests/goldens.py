@@ -8,7 +8,7 @@ import os
 import numpy as np
 
 from gfsph import cases as C
-from gfsph.dem import DEM
+from gfsph.dem import DEM, TerrainZones, Zone
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -17,10 +17,13 @@ STATE_CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_curvature7
                "g_kernel_skf4", "g_domain_exit", "g_adaptive_dt2", "g_adaptive_dt7", "g_adaptive_dt10", "g_erosion_hungr",
                "g_erosion_slope", "g_erosion_stress", "g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step",
                "g_k0", "g_costheta_k0", "g_k0_pwp", "g_nfric22", "g_hb23_m1", "g_hb23_m2", "g_hb23_m3",
-               "g_nfric3_erosion", "g_sle2", "g_sle3"]
+               "g_nfric3_erosion", "g_sle2", "g_sle3", "g_zones_friction", "g_zones_erosion",
+               "g_zones_erosion_main"]
 STAGED_CASES = ["g_k0", "g_costheta_k0", "g_k0_pwp"]
 PWP_COLUMN_CASES = ["g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step"]
-EROSION_CASES = ["g_erosion_hungr", "g_erosion_slope", "g_erosion_stress", "g_nfric3_erosion"]
+EROSION_CASES = ["g_erosion_hungr", "g_erosion_slope", "g_erosion_stress", "g_nfric3_erosion", "g_zones_erosion",
+                 "g_zones_erosion_main"]
+ZONE_CASES = ["g_zones_friction", "g_zones_erosion", "g_zones_erosion_main"]
 ALL_CASES = STATE_CASES + ["g_random_neighbours"]
 
 
@@ -40,6 +43,12 @@ def case_of(name: str):
     prm = C.Params(**pj, const=const)
     xmin, ymin, dx, dy = (float(v) for v in f["dem_geom"])
     dem = DEM(Z=f["dem_Z"], xmin=xmin, ymin=ymin, dx=dx, dy=dy)
+    if "zone_type" in f:
+        zs = [Zone(int(t), int(l), tuple(p)) for t, l, p in zip(f["zone_type"], f["zone_law"], f["zone_props"])]
+        ep = None
+        if "eros_pts" in f:
+            ep = {"pts": f["eros_pts"], "type": int(f["eros_type_dxs"][0]), "dx_s": float(f["eros_type_dxs"][1])}
+        dem.zones = TerrainZones(zs, nodes=f.get("zone_nodes"), avail=f.get("zone_avail"), eros_pts=ep)
     n = f["p_x"].shape[0]
     P = C.Particles(x=f["p_x"], v=f["p_v"], h=f["p_h"], mass=f["p_mass"], hsml=f["p_hsml"], u=f["p_u"],
                     itype=np.ones(n, np.int32))
