@@ -756,7 +756,7 @@ __device__ __forceinline__ void ext_force(const DevParams &P, const DevDem &dem,
       if (factpw > P.fact_pwp_max) factpw = P.fact_pwp_max;
       tanfi = tanfi * factpw;
     }
-    if (dem.zkinds & 1) {  // friction zones of the terrain (6:9697-9702, 10515-10545)
+    if (!F5 && (dem.zkinds & 1)) {  // friction zones of the terrain (6:9697-9702, 10515-10545)
       const int z = which_zone(dem, P, xx, yy, 1);
       if (z >= 0 && dem.zones[z].p[6] >= 0) tanfi = dem.zones[z].p[6];
     }
@@ -5216,7 +5216,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
     c->eros = c->eros || Ph[r].eros_on;
     c->staged = c->staged || Ph[r].ic_k0 || Ph[r].costheta;
     if (r == 0) c->law = LAW_FRIC5;
-    if (Ph[r].nfric != 5 || Ph[r].costheta || Ph[r].if_V2byR) c->law = LAW_ANY;
+    if (Ph[r].nfric != 5 || Ph[r].costheta || Ph[r].if_V2byR || zkinds) c->law = LAW_ANY;
   }
   c->P = Ph[0];
   c->R = R;
