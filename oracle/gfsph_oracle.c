@@ -29,6 +29,10 @@
  *   adaptive dt          - reference dumps of modes 2 and 10 (states and
  *                          time_sph after 1, 10, 100 steps); modes 1 and 4
  *                          share their terms (pinned through mode 10)
+ *   terrain zones        - Which_Topo_Zone at 20,909 points (reference
+ *                          dump), reference runs with friction and erosion
+ *                          zones (states, masses, terrain), each zone shown
+ *                          to change the reference's state
  *   rasters              - single-particle closed forms, constant-field
  *                          reproduction (no reference dump: the reference
  *                          writes them only as GiD text)
