@@ -12,11 +12,17 @@ the reference's writers:
   the model, and one point element per particle.
 * results: ``OutputRes_WIR`` ("6 SPH_SW_2019.f90":14500-14917), blocks of
   the GiD mask (``Gid_Mask_SW``, 6:1536-1559): 1 height soil, 2
-  displacement, 3 velocity, 4 pore pressure (icpwp 1-3), 6 Z topo (first
-  output only: the two-phase ``hs_plus_Z_topo`` form is not built), 7 with
-  the value 999 (the mass-based height "hs", 6:14661-14713) and 10 joint
-  height.  A mask entry of 1 writes the block; 9 (skip empty nodes) and the
-  blocks 5, 7 (other than 999), 8, 9, 11, 12 raise ``ValueError``.
+  displacement, 3 velocity, 4 pore pressure (icpwp 1-3), 5 eroded depth,
+  available depth and the terrain under and over the flow (erosion), 6 Z
+  topo (first output only: the two-phase ``hs_plus_Z_topo`` form is not
+  built), 7 with the value 999 (the mass-based height "hs",
+  6:14661-14713), 8 water height (0 without water particles), 9 the rise
+  of the flow surface since the run's first output (eta), 10 joint height,
+  11 the IDW sum of the smoothing lengths (1) or the Froude number (-1);
+  12 writes nothing in two dimensions, as there.  A mask entry of 1 writes
+  the block, 9 (-9 for 11) writes it without the nodes no particle reaches
+  (6:14279-14281); block 4 = -1 (total pressure), 7 = 1 (Peclet / law 202)
+  and other values raise ``ValueError``.
 
 Numbers are written as shortest round-trip decimals; GiD reads the files
 as whitespace-separated tokens, so the values (not the spacing of the
@@ -27,7 +33,7 @@ from __future__ import annotations
 
 import numpy as np
 
-BUILT_BLOCKS = (1, 2, 3, 4, 6, 10)
+BUILT_BLOCKS = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12)
 
 
 def _f(v: float) -> str:
@@ -142,19 +148,59 @@ def _block(title: str, time: float, rows) -> list[str]:
     return [f' Result {title}  {_f(time)}  Vector OnNodes "where"', "  Values", *rows, "  End Values"]
 
 
+def hsml_sum(dem, x: np.ndarray, hsml: np.ndarray, live: np.ndarray) -> np.ndarray:
+    """The smoothing-length sum of the raster (auxv(13), 6:14381-14383): every
+    particle in the domain adds hsml/d to the nodes of its 4 x 4 stencil, d
+    its distance to the node over the cell diagonal (at least 1e-6); the sum
+    is written as it is (the reference divides it by the weights only for
+    hrelSat, 6:14444-14446)."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 2)
+    hs = np.asarray(hsml, dtype=np.float64)
+    nx, ny = dem.nx, dem.ny
+    distg = (dem.dx * dem.dx + dem.dy * dem.dy) ** 0.5
+    out = np.zeros(nx * ny)
+    for i in np.nonzero(live)[0]:
+        xx, yy = x[i]
+        ipx = int((xx - dem.xmin) / dem.dx + 1)
+        ipy = int((yy - dem.ymin) / dem.dy + 1)
+        for inx in range(max(1, ipx - 1), min(nx, ipx + 2) + 1):
+            for iny in range(max(1, ipy - 1), min(ny, ipy + 2) + 1):
+                gx = dem.xmin + (inx - 1) * dem.dx
+                gy = dem.ymin + (iny - 1) * dem.dy
+                a = (xx - gx) * (xx - gx)
+                b = (yy - gy) * (yy - gy)
+                d = max(1.0e-6, (a + b) ** 0.5) / distg
+                out[(iny - 1) * nx + inx - 1] += hs[i] / d
+    return out
+
+
 def post_res_blocks(dem, time: float, raster: np.ndarray, x: np.ndarray, v: np.ndarray, x00: np.ndarray,
                     z00: np.ndarray, out_domain: np.ndarray | None = None, mask=(1, 1, 1, 1, 0, 1, 0, 0, 0, 1, 0, 0),
-                    first: bool = True, icpwp: int = 0, mass: np.ndarray | None = None) -> list[str]:
+                    first: bool = True, icpwp: int = 0, mass: np.ndarray | None = None,
+                    wsum: np.ndarray | None = None, hsml: np.ndarray | None = None, erosion: dict | None = None,
+                    basal_erosion: bool = False, history: dict | None = None, cst=None) -> list[str]:
     """Lines of one output time: raster [npoig][4] = (h, vx, vy, pwp) of the
     IDW raster, x/v [n][2] the particles, x00/z00 where they started
     (write_post_msh's return), out_domain [n] nonzero for particles out of
     the domain, mask the 12 GiD mask entries, first for the first output of
     the run (the Z topo block), icpwp the pore-pressure model, mass [n] the
-    particle masses (needed by the "hs" block, mask entry 7 = 999)."""
+    particle masses (needed by the "hs" block, mask entry 7 = 999).  Further
+    blocks need: wsum [npoig] the raster's weight sums (mask value 9),
+    hsml [n] (block 11 = 1), erosion = {"z", "eroded", "avail"} the
+    terrain's topol(1), (14), (15) (block 5; block 9 reads the current Z
+    from it when given), basal_erosion for terrain erosion zones (block 5),
+    history a dict kept across the run's outputs (block 9 stores the first
+    output's surface there), cst const(:) (block 11 = -1)."""
     mask = [int(m) for m in mask] + [0] * (12 - len(mask))
+    skip = [False] * 12
     for k, m in enumerate(mask, 1):
-        if m not in (0, 1) and not (k == 7 and m == 999):
-            raise ValueError(f"GiD mask entry {k} = {m}: only 0 and 1 are built")
+        if m in (9, -9):  # the block, without the nodes no particle reaches (6:14279-14281)
+            skip[k - 1] = True
+            mask[k - 1] = 1 if m == 9 else -1
+            m = mask[k - 1]
+        ok = m in (0, 1) or (k == 7 and m == 999) or (k == 11 and m == -1)
+        if not ok:
+            raise ValueError(f"GiD mask entry {k} = {m}: not built")
         if m == 1 and k not in BUILT_BLOCKS:
             raise ValueError(f"GiD result block {k} is not built (blocks {BUILT_BLOCKS})")
     raster = np.asarray(raster, dtype=np.float64).reshape(-1, 4)
@@ -164,6 +210,13 @@ def post_res_blocks(dem, time: float, raster: np.ndarray, x: np.ndarray, v: np.n
     n = x.shape[0]
     live = np.ones(n, bool) if out_domain is None else (np.asarray(out_domain) == 0)
     _, _, Ztop = dem_nodes(dem)
+    if any(skip) and wsum is None:
+        raise ValueError("GiD mask value 9 needs the raster's weight sums (wsum)")
+    empty = np.zeros(npoig, bool) if wsum is None else (np.asarray(wsum, dtype=np.float64).ravel() <= 1.0e-6)
+
+    def nodes(k):  # node indices block k writes
+        return [j for j in range(npoig) if not (skip[k - 1] and empty[j])]
+
     L: list[str] = []
     if mask[0]:
         L += _block('"height soil" "Height soil"', time,
@@ -175,11 +228,24 @@ def post_res_blocks(dem, time: float, raster: np.ndarray, x: np.ndarray, v: np.n
         L += _block('"dis" "Disp"', time, [f" {i + 2 * npoig + 1} {_f(d[i, 0])} {_f(d[i, 1])} {_f(dz[i])}"
                                             for i in range(n) if live[i]])
     if mask[2]:
-        rows = [f" {k + 1} {_f(raster[k, 1])} {_f(raster[k, 2])} 0.0" for k in range(npoig)]
+        rows = [f" {k + 1} {_f(raster[k, 1])} {_f(raster[k, 2])} 0.0" for k in nodes(3)]
         rows += [f" {i + 2 * npoig + 1} {_f(v[i, 0])} {_f(v[i, 1])} 0.0" for i in range(n) if live[i]]
         L += _block('"vel" "veloc"', time, rows)
     if mask[3] and icpwp in (1, 2, 3):
-        L += _block('"Pwp" "pwpress"', time, [f" {k + 1} 0.0 0.0 {_f(raster[k, 3])}" for k in range(npoig)])
+        L += _block('"Pwp" "pwpress"', time, [f" {k + 1} 0.0 0.0 {_f(raster[k, 3])}" for k in nodes(4)])
+    zcur = Ztop if erosion is None else np.asarray(erosion["z"], dtype=np.float64).ravel()
+    if mask[4]:  # 6:14591-14636
+        if erosion is None:
+            raise ValueError("GiD result block 5 needs the erosion state")
+        er = np.asarray(erosion["eroded"], dtype=np.float64).ravel()
+        av = np.asarray(erosion["avail"], dtype=np.float64).ravel()
+        L += _block('"erodedTopo" "-erodedTopo-"', time, [f" {k + 1} 0. 0. {_f(er[k])}" for k in range(npoig)])
+        L += _block('"AvailErosD" "ErosDepth"', time, [f" {k + 1} 0. 0. {_f(av[k])}" for k in range(npoig)])
+        under = zcur - er if basal_erosion else zcur
+        over = zcur + raster[:, 0] - er if basal_erosion else zcur + raster[:, 0]
+        rows = [f" {k + 1} 0. 0. {_f(under[k])}" for k in range(npoig)]
+        rows += [f" {npoig + k + 1} 0. 0. {_f(over[k])}" for k in range(npoig)]
+        L += _block('"hs over topo" "-hs over topo-"', time, rows)
     if mask[5] and first:
         L += _block('" Z topo " "   Z   "', time, [f" {k + 1} 0. 0. {_f(Ztop[k])}" for k in range(npoig)])
     if mask[6] == 999:
@@ -187,10 +253,42 @@ def post_res_blocks(dem, time: float, raster: np.ndarray, x: np.ndarray, v: np.n
             raise ValueError("GiD mask entry 7 = 999 (the hs block) needs the particle masses")
         hg = hs_nodes(dem, x, mass, live)
         L += _block('"hs" "hs"', time, [f" {k + 1} 0. 0. {_f(hg[k])}" for k in range(npoig) if hg[k] > 0.005])
+    if mask[7]:  # water height: no water particles here, 0 (6:14749-14758)
+        L += _block('"height water" "Height water"', time, [f" {k + 1} 0. 0. 0.0" for k in nodes(8)])
+    if mask[8]:  # the flow surface's rise since the first output (6:14500-14503, 14762-14784)
+        if history is None:
+            raise ValueError("GiD result block 9 needs a history dict kept across the run's outputs")
+        surf = raster[:, 0] + zcur
+        if "etag" not in history:
+            history["etag"] = surf.copy()
+        eta = surf - history["etag"]
+        L += _block('"eta" " sobreelev"', time, [f" {k + 1} 0. 0. {_f(eta[k])}" for k in nodes(9)])
     if mask[9]:
-        rows = [f" {k + 1} 0. 0. {_f(raster[k, 0])}" for k in range(npoig)]
-        rows += [f" {npoig + k + 1} 0. 0. {_f(raster[k, 0])}" for k in range(npoig)]
+        rows = [f" {k + 1} 0. 0. {_f(raster[k, 0])}" for k in nodes(10)]
+        rows += [f" {npoig + k + 1} 0. 0. {_f(raster[k, 0])}" for k in nodes(10)]
         L += _block('"jointh" "JOINTh"', time, rows)
+    if mask[10] == 1:  # 6:14810-14843 (no hrelSat)
+        if hsml is None:
+            raise ValueError("GiD result block 11 needs the smoothing lengths")
+        hg = hsml_sum(dem, x, hsml, live)
+        L += _block('"hsml" "-HSML-"', time, [f" {k + 1} 0. 0. {_f(hg[k])}" for k in nodes(11)])
+    elif mask[10] == -1:  # Froude number (6:14845-14869), g = const(1) (ic_cosTheta = 0)
+        if cst is None:
+            raise ValueError("GiD result block 11 = -1 needs const(:)")
+        c = np.asarray(cst, dtype=np.float64)
+        rows = []
+        for k in nodes(11):
+            vvv = (raster[k, 1] * raster[k, 1] + raster[k, 2] * raster[k, 2]) ** 0.5
+            hhh = raster[k, 0]
+            fr = 0.0
+            if hhh > c[10]:
+                fr = vvv / ((hhh * c[1]) ** 0.5)
+                if c[5] == 8:
+                    if hhh < c[6]:
+                        hhh = c[6]
+                    fr = fr * (c[6] / hhh)
+            rows.append(f" {k + 1} 0. 0. {_f(fr)}")
+        L += _block('"F or I" "-F or I -"', time, rows)
     return L
 
 
@@ -237,5 +335,7 @@ def read_tokens(path: str) -> dict:
                 out[cur] = np.array(rows, dtype=np.float64)
                 cur = None
             elif cur is not None:
+                if '"' in s:  # the rest of a header line the writer wrapped
+                    continue
                 rows.append([float(t) for t in s.split()])
     return out

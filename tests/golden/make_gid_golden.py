@@ -38,6 +38,20 @@ def main():
     out["nsteps"] = np.array(NSTEPS)
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "gid_hs_frictional_pwp.npz"), **out)
     print(sorted(out))
+    # erosion, water height, eta, smoothing lengths / Froude number and the
+    # mask value 9 (nodes no particle reaches left out), on an erosion-zone
+    # run (terrain erosion zones) and a surface-law erosion run
+    for fname, name, mask in (("gid_zones_erosion", "g_zones_erosion", (1, 0, 9, 0, 1, 0, 0, 9, 1, 9, 1, 0)),
+                              ("gid_erosion_hungr", "g_erosion_hungr", (0, 0, 0, 0, 1, 0, 0, 1, 9, 0, -9, 1))):
+        case, _ = goldens.case_of(name)
+        case.gid_mask = mask
+        tmp = tempfile.mkdtemp(prefix="gfsph_gid_")
+        refio.run_reference(case, NSTEPS, [NSTEPS], workdir=tmp, gid_final=True)
+        out = {f"res: {k}": v for k, v in gid.read_tokens(os.path.join(tmp, "case.post.res")).items()}
+        out["nsteps"] = np.array(NSTEPS)
+        out["mask"] = np.array(mask)
+        np.savez_compressed(os.path.join(ROOT, "tests", "golden", fname + ".npz"), **out)
+        print(sorted(out))
 
 
 if __name__ == "__main__":

@@ -68,8 +68,11 @@ def test_gid_writer_rejects_unbuilt_blocks(tmp_path):
     case, _ = goldens.case_of("g_frictional_pwp")
     st = oracle.init(case)
     ras = np.zeros((case.dem.nx * case.dem.ny, 4))
-    for mask in ((1, 0, 0, 0, 1), (9, 0, 0), (0,) * 10 + (1,), (0,) * 6 + (1,), (0,) * 6 + (999,)):
-        with pytest.raises(ValueError):  # the last: 999 (hs) without the masses
+    # block 5 without the erosion state, mask value 9 without the weights,
+    # block 11 without the smoothing lengths, block 7 = 1 (Peclet / law 202),
+    # block 4 = -1 (total pressure), 999 (hs) without the masses
+    for mask in ((1, 0, 0, 0, 1), (9, 0, 0), (0,) * 10 + (1,), (0,) * 6 + (1,), (0, 0, 0, -1), (0,) * 6 + (999,)):
+        with pytest.raises(ValueError):
             gid.post_res_blocks(case.dem, 0.0, ras, st.x, st.v, st.x, np.zeros(st.x.shape[0]), mask=mask)
 
 
@@ -134,3 +137,60 @@ def test_gpu_gid_results_match_reference_writer(tmp_path):
         ras, _ = m.raster_idw()
     t = float(fx["res: time vel #1"][0])
     _check_res(_write(tmp_path, case, s["x"], s["v"], s["flag"], ras, x00, z00, t, "g1"), fx, 1, 1e-9)
+
+
+def _blocks(lines):
+    """Parsed rows of every result block in writer lines."""
+    out, cur, rows = {}, None, []
+    for line in lines:
+        s = line.strip()
+        if s.startswith("Result "):
+            cur, rows = s.split('"')[1], []
+        elif s == "End Values":
+            out[cur] = np.array(rows, dtype=np.float64).reshape(-1, 4)
+        elif s != "Values" and cur is not None:
+            rows.append([float(t) for t in s.split()])
+    return out
+
+
+@pytest.mark.parametrize("fname,name", [("gid_zones_erosion", "g_zones_erosion"),
+                                        ("gid_erosion_hungr", "g_erosion_hungr")])
+def test_gid_erosion_eta_hsml_froude_blocks_match_reference_writer(fname, name):
+    """Blocks 5 (eroded and available depth, the terrain under and over the
+    flow, with terrain erosion zones and with the surface law), 8 (water
+    height), 9 (the surface's rise since the first output), 11 (smoothing
+    length sum; Froude number) and 12 (nothing in two dimensions), with the
+    mask value 9 / -9 leaving out the nodes no particle reaches, against the
+    reference writer's files at the initial state and after 100 RK4 steps."""
+    fx = goldens.load(fname)
+    case, _ = goldens.case_of(name)
+    mask = tuple(int(m) for m in fx["mask"])
+    basal = any(int(z.type) == 2 for z in case.dem.zones.zones) if case.dem.zones is not None else False
+    st = oracle.init(case)
+    hist = {}
+    for k, steps in ((0, 0), (1, int(fx["nsteps"]))):
+        if steps:
+            oracle.run(case, st, steps)
+        ras, ws = oracle.raster_idw(case.dem, st)
+        er = {"z": st.eros[:, 0], "eroded": st.eros[:, 1], "avail": st.eros[:, 2]}
+        t = float(fx[f"res: time erodedTopo #{k}"][0])
+        L = gid.post_res_blocks(case.dem, t, ras, st.x, st.v, st.x, np.zeros(st.x.shape[0]), out_domain=st.flag,
+                                mask=mask, wsum=ws, hsml=st.hsml, erosion=er, basal_erosion=basal, history=hist,
+                                cst=case.params.cst(), first=(k == 0))
+        got = _blocks(L)
+        want = sorted(key.split(" #")[0][len("res: result "):] for key in fx if key.startswith("res: result ")
+                      and key.endswith(f"#{k}"))
+        assert sorted(got) == want
+        for b in want:
+            ref = fx[f"res: result {b} #{k}"]
+            if b == "F or I" and k:
+                # the reference's Froude block divides by a gravz its writer
+                # never sets (6:14845-14869; its file holds values ~1e162):
+                # only the nodes written are compared, the values are the
+                # evident |v| / sqrt(const(1) h) (parity unpinned)
+                np.testing.assert_array_equal(got[b][:, 0], ref[:, 0])
+                continue
+            _close(got[b], ref, 1e-12 if k else 1e-14, f"{b} #{k}")
+        # mask value 9: the skipped blocks hold fewer nodes than the grid
+        if mask[2] == 9:
+            assert got["vel"].shape[0] - st.x.shape[0] < case.dem.nx * case.dem.ny
