@@ -194,3 +194,29 @@ def test_gid_erosion_eta_hsml_froude_blocks_match_reference_writer(fname, name):
         # mask value 9: the skipped blocks hold fewer nodes than the grid
         if mask[2] == 9:
             assert got["vel"].shape[0] - st.x.shape[0] < case.dem.nx * case.dem.ny
+
+
+@pytest.mark.gpu
+def test_gpu_gid_erosion_blocks_match_reference_writer():
+    """The erosion, water-height, eta, smoothing-length and mask-9 blocks
+    from the library's raster, state and terrain (gid_zones_erosion)."""
+    from gfsph import SPHModel
+    fx = goldens.load("gid_zones_erosion")
+    case, _ = goldens.case_of("g_zones_erosion")
+    mask = tuple(int(m) for m in fx["mask"])
+    hist = {}
+    with SPHModel(case) as m:
+        for k, steps in ((0, 0), (1, int(fx["nsteps"]))):
+            if steps:
+                m.time_integration_sph(steps)
+            s = m.state()
+            ras, ws = m.raster_idw()
+            t = float(fx[f"res: time erodedTopo #{k}"][0])
+            L = gid.post_res_blocks(case.dem, t, ras, s["x"], s["v"], s["x"], np.zeros(s["x"].shape[0]),
+                                    out_domain=s["flag"], mask=mask, wsum=ws, hsml=s["hsml"], erosion=m.erosion(),
+                                    basal_erosion=True, history=hist, first=(k == 0))
+            got = _blocks(L)
+            for key in fx:
+                if key.startswith("res: result ") and key.endswith(f"#{k}"):
+                    b = key.split(" #")[0][len("res: result "):]
+                    _close(got[b], fx[key], 1e-9 if k else 1e-12, f"{b} #{k}")
