@@ -96,7 +96,7 @@ struct DevDem {
   const double2 *zxy;  // [npoig] (Zx, Zy)
   const double *top;   // [npoig][6] top4..top9, or null
   // terrain zones: zkinds bit 0 = a friction zone exists, bit 1 = an
-  // erosion zone; znodes: node-list bits of geometry-101 zones; zcur, inm:
+  // erosion zone, bit 2 = a pore-pressure zone; znodes: node-list bits of geometry-101 zones; zcur, inm:
   // Z and poig_in_model for geometry 1 (null when no zone needs them)
   int nzones, zkinds;
   const DevZone *zones;
@@ -3746,7 +3746,7 @@ __global__ void k_init_pwp(int n, DevParams P, double alfs, double pwp0rel, int 
 // ---------------------------------------------------------------------------
 #define PWP_MAX_SUBSTEPS (1 << 16)
 
-__global__ void k_pwp_column(int n, const DevParams *Pr, SortedSet s, NewSet o, const int *stopr) {
+__global__ void k_pwp_column(int n, const DevParams *Pr, DevDem dem, SortedSet s, NewSet o, const int *stopr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int r = s.rid[i];
@@ -3775,6 +3775,19 @@ __global__ void k_pwp_column(int n, const DevParams *Pr, SortedSet s, NewSet o, 
   const double dhhstep = dhh / ntpwp, dpwp_dn_step = 0.0 / ntpwp;
   const double r_ = Cv * dtpwp / dxpwp2;
   const double fmax_ = P.fpwp_max, fmin_ = P.fpwp_min;
+  // the bed condition of a pore-pressure zone of the terrain (basal type 3:
+  // LBc1 = int(basal_props(7)), VBc1 = basal_props(8), 6:10987-11001), at
+  // the step's final position
+  int bt1 = P.bc_t1;
+  double bv1 = P.bc_v1;
+  if (dem.zkinds & 4) {
+    const double2 pp = o.pos[i];
+    const int z = which_zone(dem, P, pp.x, pp.y, 3);
+    if (z >= 0) {
+      bt1 = (int)dem.zones[z].p[6];
+      bv1 = dem.zones[z].p[7];
+    }
+  }
   for (int it = 0; it < ntpwp; it++) {
     double prev = col[0], cur = col[(long)n];
     for (int k = 1; k < N - 1; k++) {
@@ -3789,8 +3802,8 @@ __global__ void k_pwp_column(int n, const DevParams *Pr, SortedSet s, NewSet o, 
       cur = next;
     }
     double b = col[0];
-    if (P.bc_t1 == 1) b = P.bc_v1 * hh * alphaS;
-    else if (P.bc_t1 == 2) b = (4. * col[(long)n] - col[(long)2 * n] - 2. * P.bc_v1 * dxpwp) / 3.;
+    if (bt1 == 1) b = bv1 * hh * alphaS;
+    else if (bt1 == 2) b = (4. * col[(long)n] - col[(long)2 * n] - 2. * bv1 * dxpwp) / 3.;
     if (b > hh) b = hh;
     col[0] = b;  // before the surface condition, which reads node 0 when N = 3
     double t = col[(long)(N - 1) * n];
@@ -4824,7 +4837,8 @@ static void launch_pforce(gfsph_ctx *c, SortedSet &s) {
 // the pore-pressure columns after the stages (icpwp = 2)
 static void launch_column(gfsph_ctx *c, SortedSet &s) {
   if (c->P.npwp)
-    k_pwp_column<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->Pr, s, c->nw, c->cases ? c->stopr : nullptr);
+    k_pwp_column<<<nblk(c->n), GF_BLOCK, 0, c->stream>>>(c->n, c->Pr, c->dem, s, c->nw,
+                                                          c->cases ? c->stopr : nullptr);
 }
 
 static void launch_pforce(gfsph_ctx *c, SortedSet &s);
@@ -5176,9 +5190,10 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
     const gfsph_zone &Z = dem->zones[z];
     const int geom = (int)(Z.props[0] + 0.1);
     std::string why;
-    if (Z.type != 1 && Z.type != 2)
-      why = "basal type " + std::to_string(Z.type) +
-            " (pore-pressure boundary, flux and mass zones are outside the implemented path)";
+    if (Z.type != 1 && Z.type != 2 && Z.type != 3)
+      why = "basal type " + std::to_string(Z.type) + " (flux and mass zones are outside the implemented path)";
+    else if (Z.type == 3 && Z.props[7] < -900)
+      why = "pore-pressure zone with the drained-bed value (< -900) is outside the implemented path";
     else if (!(geom >= 1 && geom <= 5) && geom != 101)
       why = "geometry " + std::to_string(geom) + " (1-5 and 101 are implemented)";
     else if (geom == 101 && !dem->zone_nodes)
@@ -5199,7 +5214,7 @@ static int create_impl(const gfsph_params *p, const gfsph_dem *dem, int32_t R, c
                  ? GFSPH_ERR_UNSUPPORTED
                  : GFSPH_ERR_ARG;
     }
-    zkinds |= Z.type == 1 ? 1 : 2;
+    zkinds |= Z.type == 1 ? 1 : (Z.type == 2 ? 2 : 4);
     if (geom == 1) c->has_zone1 = 1;
   }
   if (zkinds & 2)

@@ -81,7 +81,10 @@ typedef struct gfsph_params {
  * -- inside the zone the zone's own law replaces the erosion rate
  * (Get_Erosion_Rate_SW 6:10407-10505): law 0 none, 1 props[6] h |v|, 2 the
  * slope law with K = props[6], C* = props[7], 3 the basal-stress law with
- * K, c, tan(phi_b), beta = props[6..9].  props[0] is the geometry
+ * K, c, tan(phi_b), beta = props[6..9].  type 3: pore pressure -- with a
+ * column (icpwp = 2) the bed condition of a particle inside the zone at the
+ * step's end is type int(props[6]), value props[7] (Get_Pwp_SW
+ * 6:10987-11001).  props[0] is the geometry
  * (Which_Topo_Zone 2:2635-2808): 1, 2, 3 a range props[1]..props[2] of the
  * terrain Z, of x, of y; 4 the rectangle x props[1]..props[2], y
  * props[3]..props[4]; 5 the rectangle of corner (props[1], props[2]), sides
@@ -89,7 +92,7 @@ typedef struct gfsph_params {
  * of the zone's node list (gfsph_dem.zone_nodes).  The first zone of the
  * type that holds the point is taken. */
 typedef struct gfsph_zone {
-  int32_t type;      /* basal_type: 1 friction, 2 erosion                     */
+  int32_t type;      /* basal_type: 1 friction, 2 erosion, 3 pore pressure   */
   int32_t law;       /* basal_law: the erosion law of a type-2 zone (0..3)    */
   double props[16];  /* basal_props(1:16)                                     */
 } gfsph_zone;

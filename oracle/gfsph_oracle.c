@@ -1102,8 +1102,8 @@ double gfo_adaptive_dt(const gfo_params *p, const gfo_dem *dem, int n, const dou
    PWP_MAX_SUBSTEPS (the reference has no cap; a cap is only reached for a
    vanishing depth). */
 #define PWP_MAX_SUBSTEPS (1 << 16)
-static void pwp_column(const gfo_params *p, const derived_t *D, int n, int N, double dt, const double *rho,
-                       const double *drho, const int32_t *flag, double *uaux) {
+static void pwp_column(const gfo_params *p, const derived_t *D, const gfo_dem *dem, const double *x, int n, int N,
+                       double dt, const double *rho, const double *drho, const int32_t *flag, double *uaux) {
   const double Cv = D->Cv, alphaS = p->cst[14];
   double *pwp0 = (double *)malloc(sizeof(double) * N);
   for (int i = 0; i < n; i++) {
@@ -1112,8 +1112,17 @@ static void pwp_column(const gfo_params *p, const derived_t *D, int n, int N, do
       for (int k = 0; k < N; k++) pwp[k] = 0.0;
       continue;
     }
-    const int LBc1 = p->pwp_bc_type[0], LBc2 = p->pwp_bc_type[1];
+    int LBc1 = p->pwp_bc_type[0];
+    const int LBc2 = p->pwp_bc_type[1];
     double VBc1 = p->pwp_bc_val[0];
+    if (dem->nzones) { /* pore-pressure zones of the terrain (basal type 3, 6:10987-11001) */
+      const int z = D->eros ? which_zone(dem, D->eros, 3, x[2 * i], x[2 * i + 1], 3)
+                            : which_zone(dem, dem->topol, 9, x[2 * i], x[2 * i + 1], 3);
+      if (z >= 0) {
+        LBc1 = (int)dem->zones[z].props[6];
+        VBc1 = dem->zones[z].props[7];
+      }
+    }
     const double VBc2 = p->pwp_bc_val[1];
     double fpwp_min = D->fpwp_min;
     const double fpwp_max = D->fpwp_max;
@@ -1413,7 +1422,7 @@ int gfo_run(const gfo_params *p_in, const gfo_dem *dem, int n, double *x, double
     }
     /* 8. pore pressure on the column, npwp > 1 (Get_Pwp_SW 6:10966-11174) */
     if (ncol > 1) {
-      pwp_column(p, &D, n, ncol, dt, rho, Drho, flag, uaux);
+      pwp_column(p, &D, dem, x, n, ncol, dt, rho, Drho, flag, uaux);
       for (int i = 0; i < n; i++) u[i] = uaux[(int64_t)i * ncol];
     } else if (nuaux >= 1) { /* npwp = 1 (Get_Pwp_SW 6:11176-11310) */
       for (int i = 0; i < n; i++) {

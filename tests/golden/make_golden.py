@@ -188,6 +188,18 @@ def case_defs():
                                      pwp_bc_type=bt, pwp_bc_val=bv, pwp0_type=p0t, xirel=xirel,
                                      const=C.frictional(28.0, cv=cv)),
                         dem, scar_on(dem, 300, seed), 100), dumps=[1, 10, 100])
+    # pore-pressure zones of the terrain (basal type 3): the column's bed
+    # condition of the zone holding the particle at the step's end
+    # (Get_Pwp_SW 6:10987-11001): a prescribed bed value in a rectangle, a bed
+    # gradient over a range of y, the run's own condition elsewhere
+    dpz = small_fractal()
+    dpz.zones = TerrainZones([Zone(3, 0, (4, 30.0, 42.0, 50.0, 66.0, 0, 1, 0.3)),
+                              Zone(3, 0, (3, 66.0, 200.0, 0, 0, 0, 2, 0.2))])
+    defs["g_zones_pwp"] = dict(
+        case=C.Case("g_zones_pwp", C.Params(integrator=4, dt=0.02, h_inf=1e-3, icpwp=2, pwp0rel=0.6, nuaux=11,
+                                            pwp_bc_type=(2, 1), pwp_bc_val=(0.0, 0.0), pwp0_type=1, xirel=0.0,
+                                            const=C.frictional(28.0, cv=0.01)),
+                    dpz, scar_on(dpz, 300, 41), 100), dumps=[1, 10, 100])
     # pressure that changes within the step (IntForces_SW in every stage):
     # K0 active / passive by the sign of the velocity divergence (6:6736-6760,
     # 6:6857-6863), g along the terrain normal (ic_cosTheta = 1, 6:6774-6783,

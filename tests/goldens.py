@@ -18,12 +18,12 @@ STATE_CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_curvature7
                "g_erosion_slope", "g_erosion_stress", "g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step",
                "g_k0", "g_costheta_k0", "g_k0_pwp", "g_nfric22", "g_hb23_m1", "g_hb23_m2", "g_hb23_m3",
                "g_nfric3_erosion", "g_sle2", "g_sle3", "g_zones_friction", "g_zones_erosion",
-               "g_zones_erosion_main"]
+               "g_zones_erosion_main", "g_zones_pwp"]
 STAGED_CASES = ["g_k0", "g_costheta_k0", "g_k0_pwp"]
-PWP_COLUMN_CASES = ["g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step"]
+PWP_COLUMN_CASES = ["g_pwp_column_lin", "g_pwp_column_cos", "g_pwp_column_step", "g_zones_pwp"]
 EROSION_CASES = ["g_erosion_hungr", "g_erosion_slope", "g_erosion_stress", "g_nfric3_erosion", "g_zones_erosion",
                  "g_zones_erosion_main"]
-ZONE_CASES = ["g_zones_friction", "g_zones_erosion", "g_zones_erosion_main"]
+ZONE_CASES = ["g_zones_friction", "g_zones_erosion", "g_zones_erosion_main", "g_zones_pwp"]
 ALL_CASES = STATE_CASES + ["g_random_neighbours"]
 
 

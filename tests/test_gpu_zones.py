@@ -26,6 +26,8 @@ def _compare(m, st, tol, what, eros):
         err = goldens.rel_err(got[k], getattr(st, k))
         assert err <= tol, f"{what} {k}: {err:.3e}"
     assert np.array_equal(got["flag"], st.flag)
+    if np.ndim(st.uaux) == 2:  # the whole pore-pressure column
+        assert goldens.rel_err(m.pwp_column(), st.uaux) <= tol, what
     if eros:
         assert goldens.rel_err(m.mass(), st.mass) <= tol, what
         e = m.erosion()
@@ -35,13 +37,14 @@ def _compare(m, st, tol, what, eros):
 
 
 @pytest.mark.parametrize("name,integrator,adapt", [("g_zones_friction", 2, 0), ("g_zones_friction", 4, 10),
-                                                   ("g_zones_erosion", 2, 2), ("g_zones_erosion_main", 2, 0)])
+                                                   ("g_zones_erosion", 2, 2), ("g_zones_erosion_main", 2, 0),
+                                                   ("g_zones_pwp", 2, 0)])
 def test_zones_match_oracle(name, integrator, adapt):
     case, _ = goldens.case_of(name)
     case.params.integrator = integrator
     case.params.adapt_dt = adapt
     st = oracle.init(case)
-    eros = name != "g_zones_friction"
+    eros = name in goldens.EROSION_CASES
     with SPHModel(case) as m:
         for chunk in (1, 9, 60):
             m.time_integration_sph(chunk)
@@ -49,7 +52,7 @@ def test_zones_match_oracle(name, integrator, adapt):
             _compare(m, st, TOL_EARLY if chunk < 60 else TOL, f"{name} rk{integrator} adapt{adapt}", eros)
 
 
-@pytest.mark.parametrize("name", ["g_zones_friction", "g_zones_erosion", "g_zones_erosion_main"])
+@pytest.mark.parametrize("name", ["g_zones_friction", "g_zones_erosion", "g_zones_erosion_main", "g_zones_pwp"])
 def test_zones_batch_equals_single_runs(name):
     """Three realisations on the zoned terrain with their own tan(phi) and
     releases shifted along y: each equals its run alone bit for bit, its
@@ -64,7 +67,7 @@ def test_zones_batch_equals_single_runs(name):
         c.particles.x = c.particles.x + np.array([0.0, 4.0 * r])
         cases.append(c)
     nsteps = 60
-    eros = name != "g_zones_friction"
+    eros = name in goldens.EROSION_CASES
     with BatchModel(cases) as b:
         b.time_integration_sph(nsteps)
         per = b.split(b.state())
@@ -114,9 +117,11 @@ def test_zone_combinations_refused():
         with AdjointModel(case) as m:
             m.forward(5)
     assert e.value.code == -2
-    # pore-pressure boundary zones (type 3) and structures (geometry 102)
+    # flux zones (type 4), the drained bed in a pore-pressure zone and
+    # structures (geometry 102)
     from gfsph.dem import Zone
-    for z, code in ((Zone(3, 0, (4, 0, 10, 0, 10)), -2), (Zone(1, 0, (102, 0, 0, 0, 0, 0, 0.1)), -2),
+    for z, code in ((Zone(4, 0, (4, 0, 10, 0, 10)), -2), (Zone(3, 0, (4, 0, 10, 0, 10, 0, 1, -999.0)), -2),
+                    (Zone(1, 0, (102, 0, 0, 0, 0, 0, 0.1)), -2),
                     (Zone(2, 5, (4, 0, 10, 0, 10)), -1)):
         c2 = copy.deepcopy(case)
         c2.dem.zones.zones = list(c2.dem.zones.zones) + [z]

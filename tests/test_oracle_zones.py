@@ -10,8 +10,8 @@ topol(1) and topol(15) bit for bit, and with its node lists the oracle's
 zone lookup must give the reference's zone at every point -- geometries 1
 (a range of the terrain Z), 2, 4, 5 (turned by +30 and -35 degrees) and 101.
 
-The state fixtures g_zones_friction, g_zones_erosion and
-g_zones_erosion_main (reference runs, checked against the oracle with the
+The state fixtures g_zones_friction, g_zones_erosion, g_zones_erosion_main
+and g_zones_pwp (reference runs, checked against the oracle with the
 other state fixtures in test_oracle_golden.py) pin the zones' effect on the
 flow; the last test here shows that each of their zones changes the
 reference's state, so a zone the oracle ignored or mislaid would fail.
@@ -74,12 +74,12 @@ def test_zone_lookup_matches_reference(tag):
 
 def _without(case, k):
     """The case with zone k switched off: removed when it only shadows later
-    zones (a friction zone with tan(phi) < 0, an erosion zone with law 0),
-    else its tan(phi) or erosion law changed."""
+    zones (a friction zone with tan(phi) < 0, an erosion zone with law 0) or
+    is a pore-pressure zone, else its tan(phi) or erosion law changed."""
     c2 = copy.deepcopy(case)
     zs = list(c2.dem.zones.zones)
     z = zs[k]
-    if (z.type == 1 and z.props[6] < 0) or (z.type == 2 and z.law == 0):
+    if (z.type == 1 and z.props[6] < 0) or (z.type == 2 and z.law == 0) or z.type == 3:
         del zs[k]
         nodes = c2.dem.zones.nodes
         if nodes is not None:  # node-list bits follow their zones
