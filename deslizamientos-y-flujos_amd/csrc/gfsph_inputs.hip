@@ -526,6 +526,8 @@ struct TopoRaw {
   std::vector<double> epts;         // <topo>.eros.pts: [n][3] x, y, depth
   int eros_type = 0;                // type_Eros
   double eros_dxs = 0;              // dx_s
+  std::vector<double> gen;          // generated terrains (ictop 1-7, -7): their parameters
+  double gx0 = 0, gx1 = 0, gy0 = 0, gy1 = 0;  // Xming, Xmaxg, Yming, Ymaxg read
   std::vector<double> px, py, pz;  // DTM points (ictop 10-16) or grid nodes (ictop 0)
   double deltx = 0;
   double xw0 = 0, xw1 = 0, yw0 = 0, yw1 = 0;
@@ -1139,6 +1141,35 @@ int read_top(gfsph_inputs *in, const std::string &path, TopoRaw &T) {
       T.py[ip] = rv[2];
       T.pz[ip] = rv[3];
     }
+  } else if ((T.ictop >= 1 && T.ictop <= 7) || T.ictop == -7) {
+    // a grid generated from a formula (2:260-444): the extent and node
+    // counts, then the formula's parameters
+    RD(R.text());
+    RD(R.reals(rv, 4));
+    T.gx0 = rv[0];
+    T.gx1 = rv[1];
+    T.gy0 = rv[2];
+    T.gy1 = rv[3];
+    RD(R.text());
+    RD(R.ints(iv, 2));
+    in->grid.nx = (int32_t)iv[0];
+    in->grid.ny = (int32_t)iv[1];
+    if (in->grid.nx < 2 || in->grid.ny < 2) return ifail(in, GFSPH_ERR_ARG, path + ": npoigx, npoigy");
+    RD(R.text());
+    if (T.ictop == 4) {  // multilinear in x: nptsZ, then the x's and the Z's
+      RD(R.ints(iv, 1));
+      const long nz = iv[0];
+      if (nz < 2) return ifail(in, GFSPH_ERR_ARG, path + ": nptsZ");
+      RD(R.text());
+      RD(R.reals(rv, nz));
+      T.gen.assign(rv.begin(), rv.begin() + nz);
+      RD(R.reals(rv, nz));
+      T.gen.insert(T.gen.end(), rv.begin(), rv.begin() + nz);
+    } else {
+      const int nr = T.ictop == 1 ? 1 : (T.ictop == 3 ? 6 : (T.ictop == 7 || T.ictop == -7 ? 2 : 4));
+      RD(R.reals(rv, nr));
+      T.gen.assign(rv.begin(), rv.begin() + nr);
+    }
   } else if (T.ictop == 10 || T.ictop == 11 || T.ictop == 15 || T.ictop == 16) {
     RD(R.text());
     RD(R.reals(rv, 2));
@@ -1165,7 +1196,7 @@ int read_top(gfsph_inputs *in, const std::string &path, TopoRaw &T) {
     }
   } else {
     return ifail(in, GFSPH_ERR_UNSUPPORTED,
-                 path + ": ictop " + std::to_string(T.ictop) + " (0, 10, 11, 15, 16 and +100 are read)");
+                 path + ": ictop " + std::to_string(T.ictop) + " (0-7, -7, 10, 11, 15, 16 and +100 are read)");
   }
   // terrain properties (Topo_main 2:453-576): 1-4 the obsolete topo_props
   // table, read and unused by the propagation step; 10 the terrain zones
@@ -1269,6 +1300,15 @@ int grid_geometry(gfsph_inputs *in) {
   TopoRaw &T = in->topo;
   gfsph_grid &G = in->grid;
   G.fdiff = T.fdiff;
+  if ((T.ictop >= 1 && T.ictop <= 7) || T.ictop == -7) {  // 2:262-290
+    G.xmin = T.gx0;
+    G.xmax = T.gx1;
+    G.ymin = T.gy0;
+    G.ymax = T.gy1;
+    G.dx = (G.xmax - G.xmin) / (G.nx - 1);
+    G.dy = (G.ymax - G.ymin) / (G.ny - 1);
+    return GFSPH_OK;
+  }
   if (T.ictop == 0) {
     double xmin = 1.e9, ymin = 1.e9, xmax = -1.e9, ymax = -1.e9;
     for (size_t k = 0; k < T.px.size(); k++) {
@@ -1359,6 +1399,70 @@ int grid_geometry(gfsph_inputs *in) {
   return GFSPH_OK;
 }
 
+// Z of a generated grid (Topo_main 2:292-444, get_cone 2:1113-1151), node
+// ipoig = iy nx + ix at (xmin + ix dx, ymin + iy dy); host, once.  The
+// slope-with-transition (2) and bilinear (3) forms leave a node outside
+// their segments at the previous node's value -- an input error here.
+int generated_terrain(gfsph_inputs *in) {
+  const TopoRaw &T = in->topo;
+  const gfsph_grid &G = in->grid;
+  const long np = (long)G.nx * G.ny;
+  const std::vector<double> &g = T.gen;
+  in->Z.assign(np, 0.0);
+  in->in_model.assign(np, 1);
+  for (long ip = 0; ip < np; ip++) {
+    const double xx = G.xmin + (double)(ip % G.nx) * G.dx, yy = G.ymin + (double)(ip / G.nx) * G.dy;
+    double zz = 0.0;
+    if (T.ictop == 1) {
+      zz = g[0];
+    } else if (T.ictop == 2) {  // slope T1, circular transition of radius R, runout T3
+      const double thetard = g[2] * 3.1416 / 180;
+      const double x0 = 0.0, x1 = x0 + g[0], x2 = x1 + g[1] * sin(thetard), x3 = x2 + g[3];
+      const double H1 = g[1] * (1. - cos(thetard));
+      if (xx >= x0 && xx < x1) zz = H1 + (x1 - xx) * tan(thetard);
+      else if (xx >= x1 && xx < x2) zz = g[1] * (1 - cos(asin((x2 - xx) / g[1])));
+      else if (xx >= x2 && xx <= x3) zz = 0.0;
+      else return ifail(in, GFSPH_ERR_ARG, "ictop 2: a node lies outside the slope's segments");
+    } else if (T.ictop == 3) {  // bilinear in x through (x0, d0), (x1, d1), (x2, d2)
+      const double x0 = g[0], d0 = g[1], x1 = g[2], d1 = g[3], x2 = g[4], d2 = g[5];
+      if (xx >= x0 && xx <= x1) {
+        const double xn0 = (xx - x1) / (x0 - x1), xn1 = 1. - xn0;
+        zz = xn0 * d0 + xn1 * d1;
+      } else if (xx >= x1 && xx <= x2) {
+        const double xn1 = (xx - x2) / (x1 - x2), xn2 = 1. - xn1;
+        zz = xn1 * d1 + xn2 * d2;
+      } else {
+        return ifail(in, GFSPH_ERR_ARG, "ictop 3: a node lies outside [x0, x2]");
+      }
+    } else if (T.ictop == 4) {  // multilinear in x; 0 outside the points
+      const size_t nz = g.size() / 2;
+      for (size_t k = 0; k + 1 < nz; k++) {
+        const double x0 = g[k], x1 = g[k + 1];
+        if (xx >= x0 && xx <= x1) {
+          const double f0 = (xx - x1) / (x0 - x1), f1 = 1. - f0;
+          zz = f0 * g[nz + k] + f1 * g[nz + k + 1];
+          break;
+        }
+      }
+    } else if (T.ictop == 5) {  // cone: r_top, R_bottom, Z_top, Z_bottom around the origin
+      const double rr = sqrt(xx * xx + yy * yy);
+      if (rr <= g[0]) zz = g[2];
+      else if (rr > g[1]) zz = g[3];
+      else zz = g[3] + (g[2] - g[3]) * (g[1] - rr) / (g[1] - g[0]);
+    } else if (T.ictop == 6) {  // drum of radius R
+      if (fabs(xx) > g[2]) return ifail(in, GFSPH_ERR_ARG, "ictop 6: a node lies outside the drum's radius");
+      zz = g[2] * (1 - cos(asin(xx / g[2])));
+    } else if (T.ictop == 7) {
+      const double a0z = g[0] / (2. * g[1]);
+      zz = a0z * xx * xx;
+    } else {  // -7
+      zz = 0.5 * g[0] * (1. + cos(xx * 3.141592 / g[1]));
+    }
+    in->Z[ip] = zz;
+  }
+  return GFSPH_OK;
+}
+
 // Get_DTM's gridding (2:1360-1404) on the device
 int grid_build(gfsph_inputs *in) {
   TopoRaw &T = in->topo;
@@ -1369,6 +1473,7 @@ int grid_build(gfsph_inputs *in) {
     in->in_model.assign(npoig, 1);
     return GFSPH_OK;
   }
+  if ((T.ictop >= 1 && T.ictop <= 7) || T.ictop == -7) return generated_terrain(in);
   const long np = (long)T.px.size();
   DevBuf<double> dx_, dy_, dz_, dZ;
   DevBuf<int> dstart, dpts, dmod;

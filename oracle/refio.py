@@ -337,6 +337,43 @@ def run_reference_zones(case, points, timeout: float = 600.0) -> dict:
         subprocess.run(["rm", "-rf", tmp])
 
 
+def generated_top(ictop: int, extent, nx: int, ny: int, params) -> str:
+    """A terrain file of a generated grid (Topo_main 2:262-444): ictop (1-7,
+    -7; +100 for finite-difference slopes), Xming Xmaxg Yming Ymaxg, npoigx
+    npoigy and the formula's parameters (ictop 4: the x's, then the Z's)."""
+    base = abs(ictop) % 100 if ictop > 0 else ictop
+    L = ["generated terrain written for the reference oracle", str(int(ictop)), "Xming Xmaxg Yming Ymaxg",
+         " ".join(_r(v) for v in extent), "npoigx npoigy", f"{int(nx)} {int(ny)}", "terrain parameters"]
+    if base == 4:
+        n = len(params) // 2
+        L += [str(n), "x and Z of the points", " ".join(_r(v) for v in params[:n]), " ".join(_r(v) for v in params[n:])]
+    else:
+        L += [" ".join(_r(v) for v in params)]
+    L += ["topo props (zones) 0 = none", "0"]
+    return "\n".join(L) + "\n"
+
+
+def run_reference_topo(case, top_text: str, timeout: float = 600.0) -> dict:
+    """The reference's Init_sph on ``case`` with its terrain file replaced by
+    ``top_text``; returns the terrain it built (topo.bin)."""
+    if not os.path.exists(REF_EXE):
+        raise FileNotFoundError(REF_EXE)
+    tmp = tempfile.mkdtemp(prefix="gfsph_ref_")
+    try:
+        write_inputs(tmp, case)
+        with open(os.path.join(tmp, "topo.top"), "w") as f:
+            f.write(top_text)
+        with open(os.path.join(tmp, "harness.ctl"), "w") as f:
+            f.write(f"case\n0 {_r(case.params.dt)} 0\n0\n")
+        r = subprocess.run([REF_EXE], input="topo\n" + "\n" * 8, cwd=tmp, capture_output=True, text=True,
+                           timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        return read_topo(os.path.join(tmp, "topo.bin"))
+    finally:
+        subprocess.run(["rm", "-rf", tmp])
+
+
 def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, timeout: float = 3600.0,
                   keep: bool = False, exe: str = REF_EXE, gid_final: bool = False, env: dict | None = None) -> dict:
     """Run the reference on ``case`` for ``nsteps`` RK4 steps; return

@@ -343,6 +343,40 @@ def save_zones_map(out_dir=HERE):
     np.savez_compressed(os.path.join(out_dir, "g_zones_map.npz"), **arr)
 
 
+GEN_TERRAINS = {  # ictop: (extent, nx, ny, parameters) -- generated grids (Topo_main 2:262-444)
+    1: ((0.0, 400.0, 0.0, 200.0), 81, 41, (12.5,)),
+    2: ((0.0, 400.0, 0.0, 200.0), 81, 41, (200.0, 100.0, 30.0, 150.0)),
+    3: ((0.0, 400.0, 0.0, 200.0), 81, 41, (0.0, 100.0, 200.0, 20.0, 400.0, 0.0)),
+    4: ((0.0, 400.0, 0.0, 200.0), 81, 41, (0.0, 150.0, 300.0, 400.0, 120.0, 60.0, 10.0, 0.0)),
+    5: ((-200.0, 200.0, -100.0, 100.0), 81, 41, (20.0, 150.0, 100.0, 0.0)),
+    6: ((-100.0, 100.0, -50.0, 50.0), 81, 41, (-100.0, 100.0, 120.0, 2.5)),
+    7: ((0.0, 400.0, 0.0, 200.0), 81, 41, (50.0, 400.0)),
+    -7: ((0.0, 400.0, 0.0, 200.0), 81, 41, (40.0, 400.0)),
+}
+
+
+def save_terrain_gen(out_dir=HERE):
+    """The terrain the reference builds from each generated-grid form
+    (ictop 1-7, -7): topol(1:9) with the nodal slopes of Get_Topol, and
+    with finite-difference slopes (ictop + 100) for the positive forms."""
+    release = C.lattice_block(10.0, 10.0, 20.0, 20.0, 10, 10, 2.0, 1.5)
+    case = C.Case("gen", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=C.voellmy(0.2, 500.0)), small_plane(),
+                  release, 0)
+    arr = {}
+    for ictop, (ext, nx, ny, prm) in GEN_TERRAINS.items():
+        for fd in ((0, 100) if ictop > 0 else (0,)):
+            t = refio.run_reference_topo(case, refio.generated_top(ictop + fd, ext, nx, ny, prm))
+            key = f"t{ictop + fd}".replace("-", "m")
+            arr[key + "_topol"] = t["topol"]
+            arr[key + "_grid"] = np.array([t["xmin"], t["xmax"], t["ymin"], t["ymax"], t["dx"], t["dy"]])
+            arr[key + "_in_model"] = t["in_model"]
+        k0 = f"t{ictop}".replace("-", "m")
+        arr[k0 + "_extent"] = np.array(ext)
+        arr[k0 + "_nxny"] = np.array([nx, ny])
+        arr[k0 + "_params"] = np.array(prm)
+    np.savez_compressed(os.path.join(out_dir, "g_terrain_gen.npz"), **arr)
+
+
 def cfg1_full_def(nsteps=100):
     case = C.cfg_plane_voellmy(nsteps=nsteps)
     return case
@@ -584,7 +618,7 @@ def save_case_driver(out_dir=HERE):
     np.savez_compressed(os.path.join(out_dir, "g_case_driver_ref.npz"), **arr)
 
 
-SPECIAL = {"g_zones_map": save_zones_map, "g_case_driver_ref": save_case_driver, "g_mc_hazard_ref": save_mc_hazard, "g_cfg1_full": save_cfg1, "g_cfg1_growth": save_cfg1_growth, "g_cfg2_full": save_cfg2}
+SPECIAL = {"g_terrain_gen": save_terrain_gen, "g_zones_map": save_zones_map, "g_case_driver_ref": save_case_driver, "g_mc_hazard_ref": save_mc_hazard, "g_cfg1_full": save_cfg1, "g_cfg1_growth": save_cfg1_growth, "g_cfg2_full": save_cfg2}
 
 
 def main(argv):

@@ -115,7 +115,7 @@ def test_list_directed_reading(tmp_path):
 
 
 @pytest.mark.parametrize("edit,what", [
-    (("100", "1"), "ictop"),
+    (("100", "20"), "ictop"),  # Get_DTM_FEM (ictop 20, 21) is not read
     (("200 ", "1 "), "icunk"),
 ])
 def test_unsupported_options_are_rejected(edit, what, tmp_path):
