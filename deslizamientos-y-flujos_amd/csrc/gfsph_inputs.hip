@@ -983,6 +983,8 @@ int read_master(gfsph_inputs *in, const std::string &path) {
 }
 
 // Input_SW (6:812-1570) for a single-phase soil release
+int generated_release(gfsph_inputs *in, FReader &R, const std::string &path);
+
 int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
   FReader R;
   if (!R.open(path)) return ifail(in, GFSPH_ERR_ARG, "cannot open " + path);
@@ -1012,13 +1014,19 @@ int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
   if (t.size() < 2) return ifail(in, GFSPH_ERR_ARG, path + ": icunk_s, h_inf_SW expected");
   in->icunk = atoi(t[0].c_str());
   if (!FReader::real_of(t[1], P.h_inf)) return ifail(in, GFSPH_ERR_ARG, path + ": h_inf_SW");
-  if (in->icunk != 7 && in->icunk != 200)
+  if (in->icunk != 7 && in->icunk != 200 && in->icunk != 11 && in->icunk != 12)
     return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": release type icunk_s " + std::to_string(in->icunk) +
-                                                " (7: gridded .pts and 200: restart.dat are read)");
-  RD(R.text());
-  std::string nm;
-  RD(R.text(&nm));
-  src_name = trim(nm);
+                                                " (7: gridded .pts, 11: rectangle, 12: ellipse and 200: restart.dat"
+                                                " are read)");
+  if (in->icunk == 11 || in->icunk == 12) {
+    int rc = generated_release(in, R, path);
+    if (rc) return rc;
+  } else {
+    RD(R.text());
+    std::string nm;
+    RD(R.text(&nm));
+    src_name = trim(nm);
+  }
   // switches (6:1101-1110)
   RD(R.text());
   RD(R.ints(iv, 4));
@@ -1106,6 +1114,80 @@ int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
   }
   P.integrator = 4;
   P.init_input = 1;
+  return GFSPH_OK;
+}
+
+// A release the reference generates from a few numbers of the .dat file:
+// Read_rect_SW (icunk_s 11, "6 SPH_SW_2019.f90":4426-4497) -- npoinx x
+// npoiny points over the rectangle of axis (x0, y0)-(x1, y1) and half-width
+// b, depth d0, equal masses 4ab d0 / n, hsml sqrt(4ab/n) facthsml -- and
+// Read_ellipsoidal_SW (icunk_s 12, 6:3818-3899) -- the points of an ellipse
+// of half-axes a, b turned theta0 about (xp0, yp0), rows of spacing drr
+// across the major axis, depth hp0 ((1 - r) + 0.1 r) for r the elliptic
+// radius squared, mass the depth times the row's cell.  At rest, u = 0.
+int generated_release(gfsph_inputs *in, FReader &R, const std::string &path) {
+  std::vector<long> iv;
+  std::vector<double> rv;
+  auto add = [&](double x, double y, double h, double m, double hs) {
+    in->x.push_back(x);
+    in->x.push_back(y);
+    in->v.push_back(0.0);
+    in->v.push_back(0.0);
+    in->h.push_back(h);
+    in->mass.push_back(m);
+    in->hsml.push_back(hs);
+    in->u.push_back(0.0);
+    in->itype.push_back(1);
+  };
+  if (in->icunk == 11) {
+    RD(R.text());
+    RD(R.ints(iv, 2));
+    const long npx = iv[0], npy = iv[1];
+    if (npx < 2 || npy < 2) return ifail(in, GFSPH_ERR_ARG, path + ": npoinx, npoiny must be >= 2");
+    RD(R.text());
+    RD(R.reals(rv, 7));
+    const double x0 = rv[0], y0 = rv[1], x1 = rv[2], y1 = rv[3], b = rv[4], d0 = rv[5], fh = rv[6];
+    const double xc = (x0 + x1) / 2., yc = (y0 + y1) / 2.;
+    const double a = sqrt((x1 - xc) * (x1 - xc) + (y1 - yc) * (y1 - yc));
+    const double vnxi = (x1 - xc) / a, vneta = (y1 - yc) / a, vvxi = -vneta, vveta = vnxi;
+    const double xs = x0 + b * vneta, ys = y0 - b * vnxi;
+    const long np = npx * npy;
+    const double area = 4 * a * b / (double)np, mss = area * d0, hs = sqrt(area) * fh;
+    for (long jy = 1; jy <= npy; jy++)
+      for (long jx = 1; jx <= npx; jx++)
+        add(xs + 2. * a * vnxi * (double)(jx - 1) / (double)(npx - 1) + 2. * b * vvxi * (double)(jy - 1) / (double)(npy - 1),
+            ys + 2. * a * vneta * (double)(jx - 1) / (double)(npx - 1) + 2. * b * vveta * (double)(jy - 1) / (double)(npy - 1),
+            d0, mss, hs);
+  } else {
+    RD(R.text());
+    RD(R.reals(rv, 6));
+    const double xp0 = rv[0], yp0 = rv[1], a = rv[2], b = rv[3], th = rv[4], hp0 = rv[5];
+    RD(R.text());
+    RD(R.reals(rv, 2));
+    const double drr = rv[0], fh = rv[1];
+    if (!(drr > 0) || !(a > 0) || !(b > 0)) return ifail(in, GFSPH_ERR_ARG, path + ": a, b, drr must be > 0");
+    const int nxi = (int)(2 * a / drr + 1);
+    if (nxi < 2) return ifail(in, GFSPH_ERR_ARG, path + ": the ellipse is shorter than two rows");
+    const double dxi = 2 * a / (nxi - 1);
+    for (int ixi = 1; ixi <= nxi; ixi++) {
+      const double xi = -a + (ixi - 1) * dxi;
+      double bmin = (1 - xi / a) * (1 + xi / a);
+      if (bmin > 0.0) bmin = b * sqrt(bmin);
+      const int neta = (int)(2 * bmin / drr + 1);
+      const double deta = neta > 1 ? 2 * bmin / (neta - 1) : drr;
+      const double dArea = dxi * deta;
+      for (int ieta = 1; ieta <= neta; ieta++) {
+        const double eta = -bmin + (ieta - 1) * deta;
+        const double rpp = (xi / a) * (xi / a) + (eta / b) * (eta / b);
+        if (rpp <= 1) {
+          const double hfact = (1. - rpp) + 0.1 * rpp;
+          const double h = hp0 * hfact;
+          add(xp0 + xi * cos(th) - eta * sin(th), yp0 + xi * sin(th) + eta * cos(th), h, h * dArea, drr * fh);
+        }
+      }
+    }
+  }
+  in->n = (int)in->h.size();
   return GFSPH_OK;
 }
 
@@ -1691,6 +1773,7 @@ int parse_all(gfsph_inputs *in, const std::string &dir, const std::string &name,
   if ((rc = grid_geometry(in))) return rc;
   // the release file is <problem name>.pts (6:5053-5055); a restart is restart.dat
   if (in->icunk == 7) return read_pts7(in, base + name + ".pts");
+  if (in->icunk == 11 || in->icunk == 12) return GFSPH_OK;  // generated while the .dat was read
   return read_restart(in, base + "restart.dat");
 }
 

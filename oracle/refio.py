@@ -36,7 +36,13 @@ def _r(v: float) -> str:
     return repr(float(v))
 
 
-def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
+def write_inputs(d: str, case, name: str = "case", topo: str = "topo", release=None) -> None:
+    """The reference's input files for ``case``: <name>.dat, <topo>.top and
+    restart.dat (the release as a restart, icunk_s 200), or with ``release``
+    = ("rect", (npoinx, npoiny), (x0, y0, x1, y1, b, d0, facthsml)) or
+    ("ellipse", (xp0, yp0, a, b, theta0, hp0), (drr, facthsml)) the
+    release generated by the reference itself (icunk_s 11, 12: Read_rect_SW
+    6:4426-4497, Read_ellipsoidal_SW 6:3818-3899)."""
     prm, dem, P = case.params, case.dem, case.particles
     c = prm.cst()
     # const(16:30), (31:45), (46:60) follow on lines of their own when const(11),
@@ -56,8 +62,19 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo") -> None:
     L += ["nhist", "0"]
     L += ["ndimn", "2"]
     L += ["ic_soil ic_water ic_vps ic_abs", "1 0 0 0"]
-    L += ["Soil unkno  h_inf_SW", f"200 {_r(prm.h_inf)}"]
-    L += ["restart name", name]
+    if release is None:
+        L += ["Soil unkno  h_inf_SW", f"200 {_r(prm.h_inf)}"]
+        L += ["restart name", name]
+    elif release[0] == "rect":
+        L += ["Soil unkno  h_inf_SW", f"11 {_r(prm.h_inf)}"]
+        L += ["npoinx npoiny", " ".join(str(int(v)) for v in release[1])]
+        L += ["x0 y0 x1 y1 b d0 facthsml", " ".join(_r(v) for v in release[2])]
+    elif release[0] == "ellipse":
+        L += ["Soil unkno  h_inf_SW", f"12 {_r(prm.h_inf)}"]
+        L += ["xp0 yp0 a b theta0 hp0", " ".join(_r(v) for v in release[1])]
+        L += ["drr facthsml", " ".join(_r(v) for v in release[2])]
+    else:
+        raise ValueError(f"release form {release[0]}")
     L += ["pa_sph nnps sle skf", f"{prm.pa_sph} {prm.nnps} {prm.sle} {prm.skf}"]
     tf = lambda b: "T" if b else "F"
     L += ["sum_den av_vel virt_part nor_dens",
@@ -375,7 +392,8 @@ def run_reference_topo(case, top_text: str, timeout: float = 600.0) -> dict:
 
 
 def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, timeout: float = 3600.0,
-                  keep: bool = False, exe: str = REF_EXE, gid_final: bool = False, env: dict | None = None) -> dict:
+                  keep: bool = False, exe: str = REF_EXE, gid_final: bool = False, env: dict | None = None,
+                  release=None) -> dict:
     """Run the reference on ``case`` for ``nsteps`` RK4 steps; return
     {step: dump}.  Raises if the reference build is absent."""
     if not os.path.exists(exe):
@@ -383,7 +401,7 @@ def run_reference(case, nsteps: int, dump_steps, workdir: str | None = None, tim
     if case.params.integrator != 4:
         raise ValueError("the reference integrates with RK4 only")
     tmp = workdir or tempfile.mkdtemp(prefix="gfsph_ref_")
-    write_inputs(tmp, case)
+    write_inputs(tmp, case, release=release)
     steps = sorted(set(int(s) for s in dump_steps))
     with open(os.path.join(tmp, "harness.ctl"), "w") as f:
         f.write("case\n")

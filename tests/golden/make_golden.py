@@ -377,6 +377,27 @@ def save_terrain_gen(out_dir=HERE):
     np.savez_compressed(os.path.join(out_dir, "g_terrain_gen.npz"), **arr)
 
 
+GEN_RELEASES = {  # releases the reference generates (Read_rect_SW, Read_ellipsoidal_SW)
+    "rect": ("rect", (12, 8), (60.0, 80.0, 100.0, 84.0, 12.0, 3.0, 1.3)),
+    "ellipse": ("ellipse", (80.0, 100.0, 25.0, 15.0, 0.3, 4.0), (2.5, 1.3)),
+}
+
+
+def save_release_gen(out_dir=HERE):
+    """The reference's run from a generated release (icunk_s 11, 12): its
+    particles after Init_sph and after 1 and 10 RK4 steps."""
+    arr = {}
+    for tag, rel in GEN_RELEASES.items():
+        case = C.Case("rel", C.Params(integrator=4, dt=0.05, h_inf=1e-3, const=C.voellmy(0.2, 500.0)),
+                      small_plane(), C.lattice_block(60.0, 80.0, 30.0, 20.0, 15, 10, 5.0, 1.5), 10)
+        ref = refio.run_reference(case, 10, [0, 1, 10], release=rel)
+        for s_, d in ref.items():
+            for k in ("x", "v", "h", "mass", "hsml", "u"):
+                arr[f"{tag}_s{s_}_{k}"] = d[k]
+            arr[f"{tag}_s{s_}_flag"] = d["flag"]
+    np.savez_compressed(os.path.join(out_dir, "g_release_gen.npz"), **arr)
+
+
 def cfg1_full_def(nsteps=100):
     case = C.cfg_plane_voellmy(nsteps=nsteps)
     return case
@@ -618,7 +639,7 @@ def save_case_driver(out_dir=HERE):
     np.savez_compressed(os.path.join(out_dir, "g_case_driver_ref.npz"), **arr)
 
 
-SPECIAL = {"g_terrain_gen": save_terrain_gen, "g_zones_map": save_zones_map, "g_case_driver_ref": save_case_driver, "g_mc_hazard_ref": save_mc_hazard, "g_cfg1_full": save_cfg1, "g_cfg1_growth": save_cfg1_growth, "g_cfg2_full": save_cfg2}
+SPECIAL = {"g_release_gen": save_release_gen, "g_terrain_gen": save_terrain_gen, "g_zones_map": save_zones_map, "g_case_driver_ref": save_case_driver, "g_mc_hazard_ref": save_mc_hazard, "g_cfg1_full": save_cfg1, "g_cfg1_growth": save_cfg1_growth, "g_cfg2_full": save_cfg2}
 
 
 def main(argv):
