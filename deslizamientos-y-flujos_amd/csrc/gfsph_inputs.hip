@@ -1014,11 +1014,12 @@ int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
   if (t.size() < 2) return ifail(in, GFSPH_ERR_ARG, path + ": icunk_s, h_inf_SW expected");
   in->icunk = atoi(t[0].c_str());
   if (!FReader::real_of(t[1], P.h_inf)) return ifail(in, GFSPH_ERR_ARG, path + ": h_inf_SW");
-  if (in->icunk != 7 && in->icunk != 200 && in->icunk != 11 && in->icunk != 12)
+  const bool gen = in->icunk == 1 || in->icunk == 11 || in->icunk == 12 || in->icunk == 13;
+  if (in->icunk != 7 && in->icunk != 200 && !gen)
     return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": release type icunk_s " + std::to_string(in->icunk) +
-                                                " (7: gridded .pts, 11: rectangle, 12: ellipse and 200: restart.dat"
-                                                " are read)");
-  if (in->icunk == 11 || in->icunk == 12) {
+                                                " (1: cylinder, 7: gridded .pts, 11: rectangle, 12: ellipse, 13:"
+                                                " rectangle with a paraboloid depth and 200: restart.dat are read)");
+  if (gen) {
     int rc = generated_release(in, R, path);
     if (rc) return rc;
   } else {
@@ -1124,7 +1125,12 @@ int read_dat(gfsph_inputs *in, const std::string &path, std::string &src_name) {
 // Read_ellipsoidal_SW (icunk_s 12, 6:3818-3899) -- the points of an ellipse
 // of half-axes a, b turned theta0 about (xp0, yp0), rows of spacing drr
 // across the major axis, depth hp0 ((1 - r) + 0.1 r) for r the elliptic
-// radius squared, mass the depth times the row's cell.  At rest, u = 0.
+// radius squared, mass the depth times the row's cell -- and
+// Read_rect_ellipsoid (icunk_s 13, 6:3903-3979: the rectangle's points with
+// the depth d0 (1 - xi^2/a^2)(1 - eta^2/b^2) + sqrt(d0^2/n)) and
+// Read_Cylinder_SW (icunk_s 1, 6:3983-4089: a centre point and rings of
+// 6k points at radius k rp0/nlinep, depth constant, a hemisphere-like
+// sqrt(1 - r^2/rp0^2) or a second depth inside rp1).  At rest, u = 0.
 int generated_release(gfsph_inputs *in, FReader &R, const std::string &path) {
   std::vector<long> iv;
   std::vector<double> rv;
@@ -1139,7 +1145,49 @@ int generated_release(gfsph_inputs *in, FReader &R, const std::string &path) {
     in->u.push_back(0.0);
     in->itype.push_back(1);
   };
-  if (in->icunk == 11) {
+  if (in->icunk == 1) {
+    RD(R.text());
+    std::vector<std::string> t;
+    RD(R.items(5, t));
+    double c5[4];
+    for (int k = 0; k < 4; k++)
+      if (!FReader::real_of(t[k], c5[k])) return ifail(in, GFSPH_ERR_ARG, path + ": xp0, yp0, rp0, hp0");
+    const double xp0 = c5[0], yp0 = c5[1], rp0 = c5[2], hp0 = c5[3];
+    const int icdens = atoi(t[4].c_str());
+    if (icdens < 0 || icdens > 2) return ifail(in, GFSPH_ERR_ARG, path + ": icdens must be 0, 1 or 2");
+    double rp1 = 0, hp1 = 0, fp0 = 1.0;
+    if (icdens == 2) {
+      RD(R.text());
+      RD(R.reals(rv, 2));
+      rp1 = rv[0];
+      hp1 = rv[1];
+      fp0 = hp1 / hp0;
+    }
+    RD(R.text());
+    RD(R.items(2, t));
+    const long nlinep = atol(t[0].c_str());
+    double fh;
+    if (nlinep < 0 || !FReader::real_of(t[1], fh)) return ifail(in, GFSPH_ERR_ARG, path + ": nlinep, facthsml");
+    const double pi = 4. * atan(1.0);
+    const long np = 1 + 3 * nlinep * (nlinep + 1);
+    const double mss = pi * rp0 * rp0 * hp0 / (double)np;
+    const double drr = nlinep >= 1 ? rp0 / (double)nlinep : rp0;
+    add(xp0, yp0, hp0 * fp0, mss * fp0, 2. * drr);
+    for (long il = 1; il <= nlinep; il++) {
+      const double delth = 2. * pi / (6. * (double)il);
+      double rpp = (double)il * drr;
+      if ((rpp / rp0) > 1.00) rpp = 0.9999999 * rp0;
+      const double f1 = (icdens == 2 && rpp <= rp1) ? hp1 / hp0 : 1.0;
+      double hfact = sqrt(1.0 - (rpp / rp0) * (rpp / rp0));
+      if (hfact < 0.1) hfact = 0.1;
+      for (long ith = 1; ith <= 6 * il; ith++) {
+        const double theta = (double)ith * delth;
+        const double h = icdens == 0 ? hp0 : (icdens == 1 ? hp0 * hfact : hp0 * f1);
+        const double m = icdens == 0 ? mss : (icdens == 1 ? mss * hfact : mss * f1);
+        add(xp0 + rpp * cos(theta), yp0 + rpp * sin(theta), h, m, 2. * drr * fh);
+      }
+    }
+  } else if (in->icunk == 11 || in->icunk == 13) {
     RD(R.text());
     RD(R.ints(iv, 2));
     const long npx = iv[0], npy = iv[1];
@@ -1153,11 +1201,22 @@ int generated_release(gfsph_inputs *in, FReader &R, const std::string &path) {
     const double xs = x0 + b * vneta, ys = y0 - b * vnxi;
     const long np = npx * npy;
     const double area = 4 * a * b / (double)np, mss = area * d0, hs = sqrt(area) * fh;
+    const double hmin = sqrt(d0 * d0 / (double)np);  // icunk_s 13 outside the rectangle's ellipsoid
     for (long jy = 1; jy <= npy; jy++)
-      for (long jx = 1; jx <= npx; jx++)
-        add(xs + 2. * a * vnxi * (double)(jx - 1) / (double)(npx - 1) + 2. * b * vvxi * (double)(jy - 1) / (double)(npy - 1),
-            ys + 2. * a * vneta * (double)(jx - 1) / (double)(npx - 1) + 2. * b * vveta * (double)(jy - 1) / (double)(npy - 1),
-            d0, mss, hs);
+      for (long jx = 1; jx <= npx; jx++) {
+        const double x = xs + 2. * a * vnxi * (double)(jx - 1) / (double)(npx - 1) +
+                         2. * b * vvxi * (double)(jy - 1) / (double)(npy - 1);
+        const double y = ys + 2. * a * vneta * (double)(jx - 1) / (double)(npx - 1) +
+                         2. * b * vveta * (double)(jy - 1) / (double)(npy - 1);
+        if (in->icunk == 11) {
+          add(x, y, d0, mss, hs);
+        } else {
+          const double xi = (x - xc) * vnxi + (y - yc) * vneta, eta = (x - xc) * vvxi + (y - yc) * vveta;
+          double h = hmin;
+          if (fabs(xi) <= a && fabs(eta) <= b) h = d0 * (1 - xi / a) * (1 + xi / a) * (1 - eta / b) * (1 + eta / b) + hmin;
+          add(x, y, h, h * area, hs);
+        }
+      }
   } else {
     RD(R.text());
     RD(R.reals(rv, 6));
@@ -1773,7 +1832,7 @@ int parse_all(gfsph_inputs *in, const std::string &dir, const std::string &name,
   if ((rc = grid_geometry(in))) return rc;
   // the release file is <problem name>.pts (6:5053-5055); a restart is restart.dat
   if (in->icunk == 7) return read_pts7(in, base + name + ".pts");
-  if (in->icunk == 11 || in->icunk == 12) return GFSPH_OK;  // generated while the .dat was read
+  if (in->icunk == 1 || in->icunk == 11 || in->icunk == 12 || in->icunk == 13) return GFSPH_OK;  // generated
   return read_restart(in, base + "restart.dat");
 }
 

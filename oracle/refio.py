@@ -69,6 +69,17 @@ def write_inputs(d: str, case, name: str = "case", topo: str = "topo", release=N
         L += ["Soil unkno  h_inf_SW", f"11 {_r(prm.h_inf)}"]
         L += ["npoinx npoiny", " ".join(str(int(v)) for v in release[1])]
         L += ["x0 y0 x1 y1 b d0 facthsml", " ".join(_r(v) for v in release[2])]
+    elif release[0] == "rect_ellipsoid":
+        L += ["Soil unkno  h_inf_SW", f"13 {_r(prm.h_inf)}"]
+        L += ["npoinx npoiny", " ".join(str(int(v)) for v in release[1])]
+        L += ["x0 y0 x1 y1 b d0 facthsml", " ".join(_r(v) for v in release[2])]
+    elif release[0] == "cylinder":  # (xp0, yp0, rp0, hp0, icdens), (rp1, hp1) or None, (nlinep, facthsml)
+        c5 = release[1]
+        L += ["Soil unkno  h_inf_SW", f"1 {_r(prm.h_inf)}"]
+        L += ["xp0 yp0 rp0 hp0 icdens", " ".join(_r(v) for v in c5[:4]) + f" {int(c5[4])}"]
+        if int(c5[4]) == 2:
+            L += ["rp1 hp1", " ".join(_r(v) for v in release[2])]
+        L += ["nlinep facthsml", f"{int(release[3][0])} {_r(release[3][1])}"]
     elif release[0] == "ellipse":
         L += ["Soil unkno  h_inf_SW", f"12 {_r(prm.h_inf)}"]
         L += ["xp0 yp0 a b theta0 hp0", " ".join(_r(v) for v in release[1])]

@@ -380,6 +380,9 @@ def save_terrain_gen(out_dir=HERE):
 GEN_RELEASES = {  # releases the reference generates (Read_rect_SW, Read_ellipsoidal_SW)
     "rect": ("rect", (12, 8), (60.0, 80.0, 100.0, 84.0, 12.0, 3.0, 1.3)),
     "ellipse": ("ellipse", (80.0, 100.0, 25.0, 15.0, 0.3, 4.0), (2.5, 1.3)),
+    "rect_ellipsoid": ("rect_ellipsoid", (14, 9), (60.0, 84.0, 100.0, 80.0, 12.0, 3.0, 1.3)),
+    "cylinder1": ("cylinder", (80.0, 100.0, 15.0, 4.0, 1), None, (6, 1.2)),
+    "cylinder2": ("cylinder", (80.0, 100.0, 15.0, 4.0, 2), (7.0, 6.0), (6, 1.2)),
 }
 
 

@@ -116,7 +116,7 @@ def test_list_directed_reading(tmp_path):
 
 @pytest.mark.parametrize("edit,what", [
     (("100", "20"), "ictop"),  # Get_DTM_FEM (ictop 20, 21) is not read
-    (("200 ", "1 "), "icunk"),
+    (("200 ", "4 "), "icunk"),  # a 3-D point cloud (Read_3DCloud) is not read
 ])
 def test_unsupported_options_are_rejected(edit, what, tmp_path):
     case, _ = goldens.case_of("g_voellmy_plane")

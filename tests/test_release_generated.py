@@ -1,6 +1,9 @@
 """Releases the reference generates from the .dat file (icunk_s 11, a
 rectangle: Read_rect_SW "6 SPH_SW_2019.f90":4426-4497; icunk_s 12, an
-ellipse: Read_ellipsoidal_SW 6:3818-3899) through the library's reader,
+ellipse: Read_ellipsoidal_SW 6:3818-3899; icunk_s 13, a rectangle with a
+paraboloid depth: Read_rect_ellipsoid 6:3903-3979; icunk_s 1, a cylinder
+with a constant, hemispherical or two-level depth: Read_Cylinder_SW
+6:3983-4089) through the library's reader,
 against the reference's own run from the same files
 (tests/golden/g_release_gen.npz): the particles as generated, bit for bit,
 and on the GPU the state after Init_sph and after 1 and 10 RK4 steps."""
@@ -15,7 +18,10 @@ from gfsph.dem import plane_runout_dem
 from oracle import refio
 
 RELEASES = {"rect": ("rect", (12, 8), (60.0, 80.0, 100.0, 84.0, 12.0, 3.0, 1.3)),
-            "ellipse": ("ellipse", (80.0, 100.0, 25.0, 15.0, 0.3, 4.0), (2.5, 1.3))}
+            "ellipse": ("ellipse", (80.0, 100.0, 25.0, 15.0, 0.3, 4.0), (2.5, 1.3)),
+            "rect_ellipsoid": ("rect_ellipsoid", (14, 9), (60.0, 84.0, 100.0, 80.0, 12.0, 3.0, 1.3)),
+            "cylinder1": ("cylinder", (80.0, 100.0, 15.0, 4.0, 1), None, (6, 1.2)),
+            "cylinder2": ("cylinder", (80.0, 100.0, 15.0, 4.0, 2), (7.0, 6.0), (6, 1.2))}
 
 
 def _files(tmp_path, tag):
