@@ -819,10 +819,10 @@ int read_case_table(gfsph_inputs *in, FReader &R, const std::string &path) {
   T.dtimestop = rv[2];
   const int ncol = 3 + T.ncases + 2;
   T.OC.clear();
-  if (T.mode == 4 || T.mode == 31) {  // sampled values (8:693-772, 8:1053-1136)
+  if (T.mode == 4 || T.mode == 5 || T.mode == 31) {  // sampled values (8:693-857, 8:1053-1136)
     RD(R.text());
     long nz = 1;
-    if (T.mode == 31) {
+    if (T.mode == 31 || T.mode == 5) {
       RD(R.ints(iv, 2));
       nz = iv[1];
     } else {
@@ -933,11 +933,11 @@ int read_master(gfsph_inputs *in, const std::string &path) {
   const long if_sph = iv[0], if_gfl = iv[1], cases = iv[2];
   if (if_sph != 1 || if_gfl != 0)
     return ifail(in, GFSPH_ERR_UNSUPPORTED, path + ": only if_sph = 1, if_gfl = 0 runs are read");
-  if (!(cases == 0 || cases == 1 || cases == 2 || cases == 3 || cases == 4 || cases == 20 || cases == 21 ||
-        cases == 31 || cases == 32))
+  if (!(cases == 0 || cases == 1 || cases == 2 || cases == 3 || cases == 4 || cases == 5 || cases == 20 ||
+        cases == 21 || cases == 31 || cases == 32))
     return ifail(in, GFSPH_ERR_UNSUPPORTED,
                  path + ": ic_cases_mat " + std::to_string(cases) +
-                     " (0-4, 20, 21, 31, 32 are read; 5 and 6 draw per geological zone of the terrain)");
+                     " (0-5, 20, 21, 31, 32 are read; 6 is the trigger module's FOSM with return periods)");
   in->cases.mode = (int)cases;
   RD(R.text());
   RD(R.ints(iv, 2));
@@ -2199,7 +2199,7 @@ int gfsph_inputs_zones(gfsph_inputs *in, int32_t *nzones, gfsph_zone *zones, int
 int gfsph_inputs_resample(gfsph_inputs *in, uint64_t seed) {
   if (!in) return GFSPH_ERR_ARG;
   CaseTable &T = in->cases;
-  if (!(T.mode == 3 || T.mode == 4 || T.mode == 31 || T.mode == 32))
+  if (!(T.mode == 3 || T.mode == 4 || T.mode == 5 || T.mode == 31 || T.mode == 32))
     return ifail(in, GFSPH_ERR_ARG, "MASTER.dat holds no sampled case set (ic_cases_mat 3, 4, 31, 32)");
   T.seed = seed;
   mc_fill(T);

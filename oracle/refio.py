@@ -233,9 +233,9 @@ def write_master(d: str, name: str, dt: float, adapt: int = 0, cases: int = 0, c
         L += ["vstop timestop dtimestop", " ".join(_r(v) for v in stop)]
         if cases == 3:
             L += ["n_MCarlo_problems", str(int(nmc))]
-        elif cases in (4, 31):
-            L += ["n_MCarlo_problems" + (" n_GZones" if cases == 31 else ""),
-                  f"{int(nmc)}" + (" 1" if cases == 31 else "")]
+        elif cases in (4, 5, 31):
+            L += ["n_MCarlo_problems" + (" n_GZones" if cases in (5, 31) else ""),
+                  f"{int(nmc)}" + (" 1" if cases in (5, 31) else "")]
             for law in laws:
                 L += _law_lines(law)
         elif cases in (20, 21):

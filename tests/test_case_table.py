@@ -117,9 +117,10 @@ def _check_draws(T, col, law, y):
         np.testing.assert_allclose(np.interp(x, xh, F), y, rtol=0, atol=1e-12)
 
 
-@pytest.mark.parametrize("mode", [4, 31])
+@pytest.mark.parametrize("mode", [4, 5, 31])
 def test_sampled_values(tmp_path, mode):
-    """ic_cases_mat 4 and 31 (8:693-772, 8:1053-1136): every variable's
+    """ic_cases_mat 4, 5 (one geological zone) and 31 (8:693-857,
+    8:1053-1136): every variable's
     n_MCarlo_problems values inverted through its law's distribution
     function give back the uniforms that drew them; another seed draws
     others, the same seed the same."""
@@ -187,8 +188,8 @@ def test_fosm_sets(tmp_path):
 
 def test_rejected_case_sets(tmp_path):
     with pytest.raises(GfsphError) as e:
-        _inputs(tmp_path, cases=5, case_rows=[[9, 0.2, 3, 0.1, 0.3, 1]])
-    assert e.value.code == -2 and "ic_cases_mat 5" in str(e.value)
+        _inputs(tmp_path, cases=6, case_rows=[[9, 0.2, 3, 0.1, 0.3, 1]])
+    assert e.value.code == -2 and "ic_cases_mat 6" in str(e.value)
     d = tmp_path / "b"
     d.mkdir()
     with pytest.raises(GfsphError) as e:
