@@ -417,7 +417,7 @@ int gfsph_inputs_particles(gfsph_inputs *in, double *x, double *v, double *h, do
 int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out);
 /* The case table of MASTER.dat (Driver_Input, "8 SPH_FEM_Driver_2019.f90":
  * 475-1236): ic_cases_mat 1 one variable at a time, 2 every combination, 3
- * n_MCarlo_problems rows drawn from the combinations, 4 / 31 / 32 values
+ * n_MCarlo_problems rows drawn from the combinations, 4 / 5 / 31 / 32 values
  * drawn from each variable's law (constant, triangular, normal, log-normal,
  * histogram: GetMCarlo_Cont_values / GetMCarlo_Dscrt_values, 8:1282-1439),
  * 20 / 21 the 2 ncases + 1 FOSM sets (means, then every variable at mean +-
@@ -429,7 +429,8 @@ int gfsph_inputs_create(gfsph_inputs *in, gfsph_ctx **out);
  * clock (8:1306-1312); here they come from a seed (1 after reading, see
  * gfsph_inputs_resample): variable k's uniforms are SplitMix64 from state
  * seed + 0x9E3779B97F4A7C15 (k + 1), output u giving ((u >> 11) + 0.5) 2^-53;
- * mode 3 draws its rows from stream k = 0.  Modes 5 and 6 and n_GZones > 1
+ * mode 3 draws its rows from stream k = 0.  Mode 5 is read as mode 4 with
+ * one geological zone; mode 6 (the trigger module's FOSM) and n_GZones > 1
  * (draws per geological zone of the terrain) are rejected. */
 int gfsph_inputs_cases(gfsph_inputs *in, int32_t *mode, int32_t *ncases, int32_t *nproblems, double *table,
                        double *stop3);
