@@ -184,8 +184,10 @@ def test_case_table_batch_matches_reference_main(tname, tmp_path):
         np.testing.assert_array_equal(per[p]["flag"], want["flag"])
 
 
-def test_sampled_set_batch_equals_each_problem(tmp_path):
-    """A sampled case set (ic_cases_mat 4: tan(phi) triangular, the initial
+@pytest.mark.parametrize("mode", [4, 5])
+def test_sampled_set_batch_equals_each_problem(tmp_path, mode):
+    """A sampled case set (ic_cases_mat 4, or 5 with one geological zone:
+    tan(phi) triangular, the initial
     pore pressure's factor -- case variable 401 -- uniform in a histogram)
     over the frictional pore-pressure fixture as one batch, against every
     problem built by hand (const(9) from the table, the pore pressure scaled
@@ -199,7 +201,7 @@ def test_sampled_set_batch_equals_each_problem(tmp_path):
     refio.write_inputs(str(tmp_path), case, name="case", topo="topo")
     rows = [[9, 0.5, 1, 0, 0, 1], [401, 1.0, 1, 0, 0, 1]]
     laws = [(1, [0.42, 0.5, 0.62]), (100, [0.4, 1.0], [1.0])]
-    refio.write_master(str(tmp_path), "case", case.params.dt, adapt=0, cases=4, case_rows=rows, nmc=6, laws=laws,
+    refio.write_master(str(tmp_path), "case", case.params.dt, adapt=0, cases=mode, case_rows=rows, nmc=6, laws=laws,
                        stop=(0.05, 1.53, 0.0), dt_driver=0.5, plot_step=1, time_end=1e9)
     with Inputs(str(tmp_path), "case", "topo") as inp:
         tab = inp.cases()["table"]
