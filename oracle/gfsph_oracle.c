@@ -3,8 +3,9 @@
  * depth-integrated SPH propagation step.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * or the CPU baseline; the HIP product path never calls it.
+ * cpu_baseline leg of bench.py may load this library, and only to check the
+ * GPU's results or to time the CPU baseline; the HIP product path never calls
+ * it.
  *
  * Every routine cites the reference file:line it follows (file prefixes as in
  * "Codes Geoflow_SPH/<n> ...f90").  Parity pins (see tests/test_oracle_*.py):

@@ -1,7 +1,8 @@
 """ctypes binding of the oracle library (liboracle_gfsph.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker / CPU baseline.  The product path
+cpu_baseline leg of bench.py, to check the GPU's results or to time the CPU
+baseline.  The product path
 (the HIP library) never calls it.
 """
 from __future__ import annotations
