@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = ["g_voellmy_plane", "g_frictional_pwp", "g_bingham", "g_icpwp1_sle1", "g_curvature7", "g_adaptive_dt10",
          "g_erosion_hungr", "g_erosion_slope", "g_pwp_column_cos", "g_k0", "g_costheta_k0", "g_nfric3_erosion",
-         "g_hb23_m3", "g_sle2", "g_sle3", "g_adaptive_dt7", "g_zones_friction"]
+         "g_hb23_m3", "g_sle2", "g_sle3", "g_adaptive_dt7", "g_zones_friction", "g_zones_pwp"]
 
 
 @pytest.mark.skipif(not os.path.exists(refio.REF_GPU_EXE),
